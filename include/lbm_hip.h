@@ -1,0 +1,186 @@
+/*
+ * lbm_hip.h - C ABI of the MI355X (gfx950) time step of the D2Q9 MRT
+ * lattice-Boltzmann + implicit immersed-boundary multi-airfoil solver.
+ *
+ * The reference program (Fortran 90 + OpenACC) has no library boundary: its
+ * step loop calls plain subroutines that share module globals.  Each entry
+ * point below replaces one of those call sites; the file:line citations name
+ * the reference code whose behaviour the entry point reproduces.  The Fortran
+ * host binds these symbols with iso_c_binding (bind(C, name=...)), in the
+ * same style as its existing libzmq binding (zmq_comm.f90:23-81).
+ *
+ * Conventions
+ *   - Every function returns 0 on success and a non-zero LBM_E_* code on
+ *     failure; lbm_last_error() returns the message of the last failure on a
+ *     handle.  Host buffers are owned by the caller and only read or written
+ *     during the call.  A handle owns all device memory it allocates.
+ *   - One host thread per handle.  All work of a handle is ordered on its own
+ *     HIP stream(s); every call returns after that work has completed.
+ *   - Lattice arrays are column-major like the reference's mesh arrays
+ *     (main.f90:167): element (i, j) of an Nx x Ny field is at i + Nx*j
+ *     (0-based), population k (0..8) of f at (i + Nx*j) + Nx*Ny*k.
+ *   - Indices given to or returned by the API are 1-based where they mirror a
+ *     reference quantity (step number n, adjoint node i/j), as in Fortran.
+ */
+#ifndef LBM_HIP_H
+#define LBM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LBM_MAX_WINGS 8                 /* main.f90:17 max_wings_supported        */
+#define LBM_MOTION_COLS_PER_WING 9      /* main.f90:19 motion_columns_per_wing    */
+#define LBM_MAX_MOTION_COLS (1 + LBM_MOTION_COLS_PER_WING * LBM_MAX_WINGS)
+#define LBM_MAX_FORCE_ROW (4 + 12 * LBM_MAX_WINGS) /* main.f90:71               */
+
+#define LBM_MOTION_FILE 1               /* main.f90:88 motion_source_file          */
+#define LBM_MOTION_STREAM 2             /* main.f90:89 motion_source_zmq           */
+
+#define LBM_PRECISION_FP64 0            /* populations stored and collided in FP64 */
+#define LBM_PRECISION_FP32_SHIFTED 1    /* FP32 storage of f - w_i                 */
+
+#define LBM_OK 0
+#define LBM_E_INVALID 1                 /* bad argument / configuration            */
+#define LBM_E_STATE 2                   /* call not valid in the current state     */
+#define LBM_E_DEVICE 3                  /* HIP runtime error                       */
+#define LBM_E_RANGE 4                   /* geometry left the lattice / capacity    */
+#define LBM_E_NUMERIC 5                 /* IBM system not positive definite        */
+
+/*
+ * Run-time configuration.  In the reference all of these are compile-time
+ * parameters of module com_date (main.f90:4-159) or derived in ini_mesh
+ * (ini_mesh.f90:15-67).
+ */
+typedef struct lbm_config {
+    int32_t nx, ny;            /* lattice nodes Nx, Ny (ini_mesh.f90:22-23)              */
+    int32_t wing_count;        /* 1..LBM_MAX_WINGS (main.f90:18)                          */
+    int32_t points_per_wing;   /* markers per wing, closed contour (main.f90:21)          */
+    double scale;              /* lattice step in chord units (main.f90:32)              */
+    double ox, oy;             /* coordinates of node (1,1) (ini_mesh.f90:36-37)          */
+    double uu;                 /* lattice inflow velocity UU (main.f90:12)               */
+    double re;                 /* Reynolds number (main.f90:106); nu = UU/scale/Re        */
+    double s[9];               /* MRT rates (ini_mesh.f90:59-67); s[7], s[8] <= 0 means
+                                  1/tau with tau = 3 nu + 0.5 (ini_mesh.f90:30-31)         */
+    double wing_x0;            /* wing_initial_x_origin (main.f90:34)                     */
+    double wing_dx;            /* wing_initial_x_spacing (main.f90:35)                    */
+    double wing_y0;            /* wing_initial_y_origin (main.f90:36)                     */
+    double xc, yc;             /* flow-centre offset (main.f90:107)                       */
+    double xp, yp;             /* rotation-axis offset of the marker files (main.f90:108) */
+    int32_t motion_source;     /* LBM_MOTION_FILE or LBM_MOTION_STREAM (driver.f90:9-31)  */
+    int32_t precision;         /* LBM_PRECISION_*                                         */
+    const double *xb0;         /* wing_count*points_per_wing marker x (chord units), as
+                                  read by read_boundary (airfoil.f90:229)                  */
+    const double *yb0;         /* same for y                                              */
+} lbm_config;
+
+typedef struct lbm_handle lbm_handle;
+
+/* Allocate a solver instance on the current HIP device.  Replaces ini_mesh +
+ * the allocations of init (ini_mesh.f90:1-80, solution.f90:67-69).  The
+ * marker arrays are copied.  The instance starts in the RESET state. */
+int lbm_create(const lbm_config *cfg, lbm_handle **out);
+int lbm_destroy(lbm_handle *h);
+
+/* Episode reset: init + read_boundary + zeroed rigid-body state
+ * (solution.f90:1-75, airfoil.f90:207-242, driver.f90:205-258).  The next
+ * step to run is n = 1.  In stream mode the motion segment becomes the zero
+ * row at time 0 (driver.f90:120-124). */
+int lbm_reset(lbm_handle *h);
+
+/* File-mode motion: the raw motion_params rows (row-major, nrows x ncols with
+ * ncols = 1 + 9*wing_count), validated and resampled on the solver step
+ * dt = scale*UU (mode.f90:104-178, 327-412).  end_time receives
+ * motion_end_time (mode.f90:163). */
+int lbm_set_motion_table(lbm_handle *h, const double *rows, int32_t nrows,
+                         int32_t ncols, double *end_time);
+
+/* Stream-mode motion segment: start row and target row of 1 + 9*wing_count
+ * columns, evaluated per step by lookup_stream_motion_row
+ * (mode.f90:414-449).  Built by the host from a STEP command with
+ * lbm_build_stream_target_row. */
+int lbm_set_stream_segment(lbm_handle *h, const double *start_row,
+                           const double *target_row, int32_t ncols);
+
+/* build_stream_target_row (mode.f90:287-325): target row for a STEP with
+ * per-wing (domega, dxv, dyv) increments in delta_row[1..3*wing_count]
+ * (delta_row[0] is ignored).  Host-only arithmetic, no device work.
+ * Returns LBM_E_INVALID when target_time is before the segment start. */
+int lbm_build_stream_target_row(int32_t wing_count, const double *start_row,
+                                double target_time, const double *delta_row,
+                                double *target_row);
+
+/* Run steps n .. n_last (inclusive), where n is the next step of the
+ * instance: per step erranalysis (n >= 2), evolution and the move_mesh
+ * check (main.f90:374-412, driver.f90:362-408, solution.f90:272-422).
+ *   force_rows   optional, (n_last-n+1) x row_len doubles: one force.dat row
+ *                per step (solution.f90:352-378)
+ *   motion_rows  optional, (n_last-n+1) x (1+3*wing_count): one
+ *                airfoil_motion.dat row per step (solution.f90:390-401)
+ *   row_sum      optional, row_len doubles: sum of the force rows of this call
+ *                (latest_force_sum_row, solution.f90:379-385)
+ *   last_row     optional, row_len doubles: force row of step n_last
+ *   err          optional: erranalysis value at the last step (solution.f90:77-130)
+ * row_len = 4 + 12*wing_count.  n_last < n runs nothing. */
+int lbm_advance(lbm_handle *h, int32_t n_last, double *force_rows,
+                double *motion_rows, double *row_sum, double *last_row,
+                double *err);
+
+/* State counters: next step n, movenum (main.f90:143), flag2 (post-move_mesh
+ * window, main.f90:401-412), force row length, adjoint node count. */
+int lbm_get_state(lbm_handle *h, int32_t *n_next, int32_t *movenum,
+                  int32_t *flag2, int32_t *row_len, int32_t *adjoint_n);
+
+/* Macroscopic fields after the last step, Nx*Ny each (any may be NULL):
+ * rho, u, v, pre = rho/3 (solution.f90:546-549, IBM-corrected u/v at the
+ * adjoint nodes, boundary_condition.f90:1650-1651, far-field values on the
+ * boundary ring, boundary_condition.f90:403-853). */
+int lbm_get_fields(lbm_handle *h, double *rho, double *u, double *v,
+                   double *pre);
+
+/* Populations f (9*Nx*Ny, FP64 values; FP32-shifted instances return
+ * w_k + stored deviation) and the persistent f0 boundary ring of the
+ * streaming buffer (same layout; interior values are unspecified). */
+int lbm_get_populations(lbm_handle *h, double *f, double *f0);
+/* Overwrite f (interior and ring) and the f0 ring; u/v of the previous step
+ * become the moments of f (test and restart hook, solution.f90:1721-1851). */
+int lbm_set_populations(lbm_handle *h, const double *f, const double *f0);
+
+/* Markers after the last step: xb1/yb1 (wing_count*points_per_wing) and
+ * segment midpoints xb2/yb2 (airfoil.f90:261-270); any may be NULL. */
+int lbm_get_markers(lbm_handle *h, double *xb1, double *yb1, double *xb2,
+                    double *yb2);
+
+/* IBM data of the last step: adjoint_p (1-based i, j; 2*capacity int32,
+ * i first), adjoint_wing_id (1-based), per-wing normalised force (2*8) and
+ * moment (8) (boundary_condition.f90:1645-1692, solution.f90:330-333). */
+int lbm_get_ibm(lbm_handle *h, int32_t capacity, int32_t *adjoint_ij,
+                int32_t *wing_id, double *wing_force, double *wing_moment);
+
+/* Rigid-body displacement hook: body_rel_pos(wing) (main.f90:77).  In the
+ * reference only the (disabled) dynamics solver moves it; setting |pos| >= 1
+ * for wing 1 triggers move_mesh after the next step (main.f90:393-411). */
+int lbm_set_body_rel_pos(lbm_handle *h, int32_t wing, double pos);
+
+/* Measurement hook (no counterpart in the reference): when enabled, every
+ * launch of the fused lattice kernel is bracketed by HIP events on the
+ * instance's stream; lbm_get_timing returns the summed kernel time (ms) and
+ * the launch count since the last enable, and resets nothing. */
+int lbm_set_timing(lbm_handle *h, int32_t enable);
+int lbm_get_timing(lbm_handle *h, double *lattice_ms, int32_t *launches);
+
+/* Message of the last failed call on h (or of the last failed lbm_create when
+ * h is NULL).  Always NUL-terminates when n > 0. */
+int lbm_last_error(const lbm_handle *h, char *buf, size_t n);
+
+/* Device name / library build string, for logs. */
+const char *lbm_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LBM_HIP_H */

@@ -1,0 +1,876 @@
+// lbm_api.hip - C ABI of include/lbm_hip.h: handle lifecycle, host-side
+// motion tables, and the per-step launch sequence of the FP64 path.
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "lbm_internal.h"
+
+using namespace lbm;
+
+namespace {
+
+char g_create_error[512] = "";
+
+// nizhen (solution.f90:1474-1529): Gauss-Jordan inverse of the MRT matrix,
+// operation for operation, so the device uses the reference's values.
+void gauss_jordan_inverse(const double AA[9][9], double b1[9][9]) {
+  const int Nm = 9;
+  double a[9][9];
+  std::memcpy(a, AA, sizeof(a));
+  for (int i = 0; i < Nm; i++)
+    for (int j = 0; j < Nm; j++) b1[i][j] = i == j ? 1.0 : 0.0;
+  for (int k = 0; k < Nm - 1; k++) {
+    int ik = k;
+    double temp = std::fabs(a[ik][k]);
+    for (int i = k; i < Nm; i++)
+      if (std::fabs(a[i][k]) > std::fabs(temp)) {
+        temp = std::fabs(a[i][k]);
+        ik = i;
+      }
+    for (int j = 0; j < Nm; j++) {
+      std::swap(a[ik][j], a[k][j]);
+      std::swap(b1[ik][j], b1[k][j]);
+    }
+    for (int i = 0; i < Nm; i++) {
+      const double mik = a[i][k] / a[k][k];
+      temp = a[k][k];
+      for (int j = k; j < Nm; j++)
+        if (i != k) a[i][j] = a[i][j] - mik * a[k][j];
+      for (int j = k; j < Nm; j++) a[k][j] = a[k][j] / temp;
+      for (int j = 0; j < Nm; j++)
+        if (i != k) b1[i][j] = b1[i][j] - mik * b1[k][j];
+      for (int j = 0; j < Nm; j++) b1[k][j] = b1[k][j] / temp;
+    }
+  }
+  const double temp = a[Nm - 1][Nm - 1];
+  for (int i = 0; i < Nm - 1; i++) {
+    const double mik = a[i][Nm - 1] / a[Nm - 1][Nm - 1];
+    a[i][Nm - 1] = 0.0;
+    for (int j = 0; j < Nm; j++) b1[i][j] = b1[i][j] - mik * b1[Nm - 1][j];
+  }
+  for (int j = 0; j < Nm; j++) b1[Nm - 1][j] = b1[Nm - 1][j] / temp;
+}
+
+constexpr double kMatM[9][9] = {  // solution.f90:17-25
+    {1, 1, 1, 1, 1, 1, 1, 1, 1},    {-4, -1, -1, -1, -1, 2, 2, 2, 2}, {4, -2, -2, -2, -2, 1, 1, 1, 1},
+    {0, 1, 0, -1, 0, 1, -1, -1, 1}, {0, -2, 0, 2, 0, 1, -1, -1, 1},   {0, 0, 1, 0, -1, 1, 1, -1, -1},
+    {0, 0, -2, 0, 2, 1, 1, -1, -1}, {0, 1, -1, 1, -1, 0, 0, 0, 0},    {0, 0, 0, 0, 0, 1, -1, 1, -1}};
+
+// interpolate_motion_triplet and the smooth ramps (mode.f90:220-285), host side
+// for build_stream_target_row.
+double h_smooth_r(double x) {
+  const double pai = 3.1415926535897932384626433832795;
+  if (x < 0.0) return 0.0;
+  if (x <= 1.0) return 0.5 * x * x + (std::cos(2.0 * pai * x) - 1.0) / (4.0 * pai * pai);
+  return x - 0.5;
+}
+double h_smooth_h(double x) {
+  const double pai = 3.1415926535897932384626433832795;
+  if (x < 0.0) return 0.0;
+  if (x <= 1.0) return x - std::sin(2.0 * pai * x) / (2.0 * pai);
+  return 1.0;
+}
+double h_smooth_d(double x) {
+  const double pai = 3.1415926535897932384626433832795;
+  if (x >= 0.0 && x <= 1.0) return 1.0 - std::cos(2.0 * pai * x);
+  return 0.0;
+}
+void h_interp_triplet(double s0, double v0, double v1, double dt, double lt, double* s, double* v, double* a) {
+  const double dv = v1 - v0;
+  if (dt <= 1.0e-14) {
+    *s = (v0 + dv) * lt + s0;
+    *v = v0 + dv;
+    *a = 0.0;
+    return;
+  }
+  const double x = lt / dt;
+  *s = dv * dt * h_smooth_r(x) + v0 * lt + s0;
+  *v = dv * h_smooth_h(x) + v0;
+  *a = dv / dt * h_smooth_d(x);
+}
+
+}  // namespace
+
+struct lbm_handle {
+  lbm_config cfg;
+  Grid g;
+  MrtParams mrt;
+  IbmParams ip;
+  IbmBuffers B;
+  RingBuffers R;
+  hipStream_t stream = nullptr;
+  int device = 0;
+  double feq[9];
+  double w[9];
+  // populations
+  double* fbuf[2] = {nullptr, nullptr};
+  int cur = 0;
+  // field slots (rho,u,v of the last steps), tagged with their step number
+  FieldSlot slot[3];
+  FieldSlot spare;  // target of the move_mesh field shift
+  int slot_step[3] = {-1, -1, -1};
+  double* err_partial = nullptr;
+  double* err_out = nullptr;
+  // IBM / motion device memory
+  double* d_marker = nullptr;  // xb0,yb0,xb1,yb1 (4*npts), xb2,yb2,arc,ub,vb (5*nseg)
+  int* d_int = nullptr;
+  double* d_adj = nullptr;
+  double* d_band = nullptr;
+  double* d_table = nullptr;
+  size_t table_cap = 0;
+  double* d_stream = nullptr;
+  DevStep* d_st = nullptr;
+  double* d_rows = nullptr;
+  size_t rows_cap = 0;
+  double* d_mrows = nullptr;
+  double* d_rowsum = nullptr;
+  // host state
+  int W, P, SPW, nseg, npts, row_len;
+  int n = 1;
+  int movenum = 0;
+  int flag2 = 1;
+  double body_rel_pos[MAXW] = {};
+  bool motion_ready = false;
+  double motion_end_time = 0.0;
+  std::vector<double> h_table;   // raw table (host)
+  // lattice-kernel timing (lbm_set_timing)
+  bool timing = false;
+  std::vector<hipEvent_t> ev;     // start/stop pairs of the current call
+  double lattice_ms = 0.0;
+  int lattice_launches = 0;
+  char msg[512] = "";
+};
+
+namespace {
+
+void dfree(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+int fail(lbm_handle* h, int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(h ? h->msg : g_create_error, 512, fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+#define HIPCHECK(h, expr)                                                                         \
+  do {                                                                                            \
+    hipError_t e_ = (expr);                                                                       \
+    if (e_ != hipSuccess) return fail((h), LBM_E_DEVICE, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename T>
+int dalloc(lbm_handle* h, T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+  if (e != hipSuccess)
+    return fail(h, LBM_E_DEVICE, "hipMalloc(%zu bytes): %s", count * sizeof(T), hipGetErrorString(e));
+  return 0;
+}
+
+void update_body_extra(lbm_handle* h) {  // update_body_extra_from_rel (mode.f90:473-495)
+  double ap = 0.0;
+  for (int w = 0; w < MAXW; w++) {
+    h->ip.body_extra_v[w] = 0.0;
+    h->ip.body_extra_s[w] = 0.0;
+    h->ip.body_extra_pos[w] = 0.0;
+  }
+  for (int w = 0; w < h->W; w++) {
+    ap = ap + h->body_rel_pos[w];
+    h->ip.body_extra_pos[w] = ap;
+  }
+}
+
+const char* dev_error_text(int code) {
+  switch (code) {
+    case DERR_BOX_EDGE: return "IBM support box of a segment reaches within 3 nodes of the lattice edge";
+    case DERR_ADJ_OVERFLOW: return "adjoint node list overflow";
+    case DERR_BAND: return "IBM band half-width exceeds the solver limit";
+    case DERR_WINGS_COUPLED: return "support boxes of two wings overlap";
+    case DERR_NOT_SPD: return "IBM matrix lost positive definiteness";
+    case DERR_MOTION: return "motion source not initialised (call lbm_set_motion_table)";
+    default: return "unknown device error";
+  }
+}
+
+void free_all(lbm_handle* h) {
+  dfree(h->fbuf[0]);
+  dfree(h->fbuf[1]);
+  for (auto& s : h->slot) {
+    dfree(s.rho);
+    dfree(s.u);
+    dfree(s.v);
+  }
+  dfree(h->spare.rho);
+  dfree(h->spare.u);
+  dfree(h->spare.v);
+  dfree(h->err_partial);
+  dfree(h->err_out);
+  dfree(h->R.left);
+  dfree(h->R.right);
+  dfree(h->R.mom);
+  dfree(h->d_marker);
+  dfree(h->d_int);
+  dfree(h->d_adj);
+  dfree(h->d_band);
+  dfree(h->d_table);
+  dfree(h->d_stream);
+  dfree(h->d_st);
+  dfree(h->d_rows);
+  dfree(h->d_mrows);
+  dfree(h->d_rowsum);
+  dfree(h->B.owner);
+  dfree(h->B.aidx);
+  for (auto e : h->ev) (void)hipEventDestroy(e);
+  h->ev.clear();
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+}
+
+int ensure_rows(lbm_handle* h, size_t steps) {
+  if (steps <= h->rows_cap) return 0;
+  dfree(h->d_rows);
+  dfree(h->d_mrows);
+  h->d_rows = h->d_mrows = nullptr;
+  size_t cap = std::max<size_t>(steps, 1024);
+  int rc = dalloc(h, &h->d_rows, cap * h->row_len);
+  if (rc) return rc;
+  rc = dalloc(h, &h->d_mrows, cap * (1 + 3 * h->W));
+  if (rc) return rc;
+  h->rows_cap = cap;
+  h->B.rows = h->d_rows;
+  h->B.mrows = h->d_mrows;
+  return 0;
+}
+
+int upload_static(lbm_handle* h, const lbm_config* cfg, double* xb0, double* yb0, double* arc, int* seg_of_pos,
+                  int* pos_of_seg);
+
+}  // namespace
+
+extern "C" {
+
+const char* lbm_version(void) { return "lbm_mutiwing_amd gfx950 FP64 step 0.1"; }
+
+int lbm_last_error(const lbm_handle* h, char* buf, size_t n) {
+  if (!buf || n == 0) return LBM_E_INVALID;
+  snprintf(buf, n, "%s", h ? h->msg : g_create_error);
+  return 0;
+}
+
+int lbm_create(const lbm_config* cfg, lbm_handle** out) {
+  if (!cfg || !out) return fail(nullptr, LBM_E_INVALID, "null argument");
+  *out = nullptr;
+  if (cfg->nx < 8 || cfg->ny < 8) return fail(nullptr, LBM_E_INVALID, "lattice must be at least 8x8");
+  if (cfg->wing_count < 0 || cfg->wing_count > MAXW) return fail(nullptr, LBM_E_INVALID, "wing_count out of range");
+  if (cfg->wing_count > 0 && (cfg->points_per_wing < 3 || !cfg->xb0 || !cfg->yb0))
+    return fail(nullptr, LBM_E_INVALID, "marker arrays missing");
+  if (!(cfg->scale > 0.0) || !(cfg->uu > 0.0) || !(cfg->re > 0.0))
+    return fail(nullptr, LBM_E_INVALID, "scale, uu and re must be positive");
+  if (cfg->motion_source != LBM_MOTION_FILE && cfg->motion_source != LBM_MOTION_STREAM)
+    return fail(nullptr, LBM_E_INVALID, "motion_source must be 1 (file) or 2 (stream)");
+  if (cfg->precision != LBM_PRECISION_FP64)
+    return fail(nullptr, LBM_E_INVALID, "only LBM_PRECISION_FP64 is implemented by this build");
+  lbm_handle* h = new lbm_handle();
+  h->cfg = *cfg;
+  h->cfg.xb0 = h->cfg.yb0 = nullptr;
+  h->W = cfg->wing_count;
+  h->P = h->W > 0 ? cfg->points_per_wing : 0;
+  h->SPW = h->P > 0 ? h->P - 1 : 0;
+  h->nseg = h->W * h->SPW;
+  h->npts = h->W * h->P;
+  h->row_len = 4 + 12 * h->W;
+  (void)hipGetDevice(&h->device);
+  int lds_max = 0;
+  (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device);
+  if (h->W > 0 && ibm_solve_lds_bytes(h->SPW) > (size_t)lds_max) {
+    int rc = fail(nullptr, LBM_E_INVALID, "%d segments per wing need %zu B of LDS, device allows %d", h->SPW,
+                  ibm_solve_lds_bytes(h->SPW), lds_max);
+    delete h;
+    return rc;
+  }
+  // model constants (D2Q9model.f90, ini_mesh.f90:30-31, 59-67, boundary_condition.f90:304)
+  const double niu = cfg->uu * (1.0 / cfg->scale) / cfg->re;
+  const double tau = 3.0 * niu + 0.5;
+  for (int k = 0; k < 9; k++) h->mrt.s[k] = cfg->s[k];
+  if (!(h->mrt.s[7] > 0.0)) h->mrt.s[7] = 1.0 / tau;
+  if (!(h->mrt.s[8] > 0.0)) h->mrt.s[8] = 1.0 / tau;
+  double mi[9][9];
+  gauss_jordan_inverse(kMatM, mi);
+  for (int r = 0; r < 9; r++)
+    for (int c = 0; c < 9; c++) h->mrt.minv[9 * r + c] = mi[r][c];
+  h->mrt.cs = (double)sqrtf(3.0f) / 3.0;
+  h->w[0] = (double)(4.0f / 9.0f);
+  for (int k = 1; k < 5; k++) h->w[k] = (double)(1.0f / 9.0f);
+  for (int k = 5; k < 9; k++) h->w[k] = (double)(1.0f / 36.0f);
+  for (int k = 0; k < 9; k++) {  // D2Q9feq(1, UU, 0) (D2Q9model.f90:35-46)
+    const double u1 = cfg->uu, v1 = 0.0;
+    const double eu = (double)EX(k) * u1 + (double)EY(k) * v1;
+    const double utwo = u1 * u1 + v1 * v1;
+    h->feq[k] = h->w[k] * 1.0 * (1.0 + 3.0 * eu + 4.5 * eu * eu - 1.5 * utwo);
+  }
+  h->g.nx = cfg->nx;
+  h->g.ny = cfg->ny;
+  const long long np = (long long)cfg->nx * cfg->ny;
+  h->g.ps = np;
+  // IBM params
+  IbmParams& ip = h->ip;
+  ip.W = h->W; ip.P = h->P; ip.SPW = h->SPW; ip.nseg = h->nseg; ip.npts = h->npts;
+  ip.cap = BOXW * BOXW * h->nseg;
+  ip.nx = cfg->nx; ip.ny = cfg->ny;
+  ip.scale = cfg->scale; ip.ox = cfg->ox; ip.oy = cfg->oy; ip.uu = cfg->uu;
+  ip.dt = cfg->scale * cfg->uu;
+  ip.wing_x0 = cfg->wing_x0; ip.wing_dx = cfg->wing_dx; ip.wing_y0 = cfg->wing_y0;
+  ip.xc = cfg->xc; ip.yc = cfg->yc;
+  ip.motion_source = cfg->motion_source;
+  ip.movenum = 0; ip.step_count = 0; ip.ncols = 1 + LBM_MOTION_COLS_PER_WING * h->W;
+  ip.motion_dt = ip.dt;
+  update_body_extra(h);
+
+  int rc = 0;
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete h;
+    return fail(nullptr, LBM_E_DEVICE, "hipStreamCreate failed");
+  }
+#define ALLOC(ptr, n)                   \
+  if ((rc = dalloc(h, &(ptr), (n)))) { \
+    snprintf(g_create_error, 512, "%s", h->msg); \
+    free_all(h);                        \
+    delete h;                           \
+    return rc;                          \
+  }
+  ALLOC(h->fbuf[0], 9 * np);
+  ALLOC(h->fbuf[1], 9 * np);
+  for (auto& s : h->slot) {
+    ALLOC(s.rho, np);
+    ALLOC(s.u, np);
+    ALLOC(s.v, np);
+  }
+  ALLOC(h->spare.rho, np);
+  ALLOC(h->spare.u, np);
+  ALLOC(h->spare.v, np);
+  ALLOC(h->err_partial, 2 * (size_t)err_blocks(h->g));
+  ALLOC(h->err_out, 1);
+  ALLOC(h->R.left, 9 * (size_t)cfg->ny);
+  ALLOC(h->R.right, 9 * (size_t)cfg->ny);
+  ALLOC(h->R.mom, 6 * (size_t)cfg->ny);
+  const int npts = std::max(h->npts, 1), nseg = std::max(h->nseg, 1), cap = std::max(ip.cap, 1);
+  ALLOC(h->d_marker, 4 * (size_t)npts + 5 * (size_t)nseg + 2 * (size_t)BOXW * nseg);
+  ALLOC(h->d_int, 4 * (size_t)nseg + 2 * ((size_t)nseg + 1) + 2 * (size_t)nseg + 3 * (size_t)cap);
+  ALLOC(h->d_adj, 10 * (size_t)cap);
+  ALLOC(h->d_band, (size_t)nseg * (BMAX + 1) + 2 * (size_t)nseg);
+  ALLOC(h->d_stream, 2 * LBM_MAX_MOTION_COLS);
+  ALLOC(h->d_st, 1);
+  ALLOC(h->d_rowsum, h->row_len);
+  ALLOC(h->B.owner, np);
+  ALLOC(h->B.aidx, np);
+#undef ALLOC
+  // carve the IBM buffers
+  IbmBuffers& B = h->B;
+  double* dm = h->d_marker;
+  double* xb0 = dm; dm += npts;
+  double* yb0 = dm; dm += npts;
+  B.xb1 = dm; dm += npts;
+  B.yb1 = dm; dm += npts;
+  B.xb2 = dm; dm += nseg;
+  B.yb2 = dm; dm += nseg;
+  double* arc = dm; dm += nseg;
+  B.ub = dm; dm += nseg;
+  B.vb = dm; dm += nseg;
+  B.phx = dm; dm += BOXW * nseg;
+  B.phy = dm; dm += BOXW * nseg;
+  B.xb0 = xb0; B.yb0 = yb0; B.arc = arc;
+  int* di = h->d_int;
+  B.box = di; di += 4 * nseg;
+  B.seg_cnt = di; di += nseg + 1;
+  B.seg_off = di; di += nseg + 1;
+  int* seg_of_pos = di; di += nseg;
+  int* pos_of_seg = di; di += nseg;
+  B.adj_ij = di; di += 2 * cap;
+  B.adj_seg = di; di += cap;
+  B.seg_of_pos = seg_of_pos; B.pos_of_seg = pos_of_seg;
+  double* da = h->d_adj;
+  B.adj_rho = da; da += cap;
+  B.adj_us = da; da += cap;
+  B.adj_vs = da; da += cap;
+  B.adj_u = da; da += cap;
+  B.adj_v = da; da += cap;
+  B.adj_gx = da; da += cap;
+  B.adj_gy = da; da += cap;
+  B.adj_mom = da; da += cap;
+  B.band = h->d_band;
+  B.rhs = h->d_band + (size_t)nseg * (BMAX + 1);
+  B.stream_rows = h->d_stream;
+  B.step_table = nullptr;
+  B.st = h->d_st;
+  B.row_sum = h->d_rowsum;
+  B.prev_ij = nullptr; B.prev_n = nullptr;
+  rc = upload_static(h, cfg, xb0, yb0, arc, seg_of_pos, pos_of_seg);
+  if (rc == 0) rc = lbm_reset(h);
+  if (rc) {
+    snprintf(g_create_error, 512, "%s", h->msg);
+    free_all(h);
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return 0;
+}
+
+}  // extern "C"
+
+namespace {
+
+int upload_static(lbm_handle* h, const lbm_config* cfg, double* xb0, double* yb0, double* arc, int* seg_of_pos,
+                  int* pos_of_seg) {
+  const long long np = (long long)cfg->nx * cfg->ny;
+  // markers: read_boundary (airfoil.f90:229-241) arc lengths and offset
+  if (h->W > 0) {
+    std::vector<double> hx(h->npts), hy(h->npts), ha(h->nseg);
+    for (int q = 0; q < h->npts; q++) {
+      hx[q] = cfg->xb0[q];
+      hy[q] = cfg->yb0[q];
+    }
+    for (int w = 0; w < h->W; w++)
+      for (int i0 = 0; i0 < h->SPW; i0++) {
+        const int a = w * h->P + i0;
+        const double dx = hx[a + 1] - hx[a], dy = hy[a + 1] - hy[a];
+        ha[w * h->SPW + i0] = std::sqrt(dx * dx + dy * dy) / cfg->scale;
+      }
+    for (int q = 0; q < h->npts; q++) {
+      hx[q] = hx[q] + (cfg->xc - cfg->xp);
+      hy[q] = hy[q] + (cfg->yc - cfg->yp);
+    }
+    // zig-zag band order per wing: positions 0,1,2,3,... hold segments 0, n-1, 1, n-2, ...
+    std::vector<int> sop(h->nseg), pos(h->nseg);
+    for (int w = 0; w < h->W; w++) {
+      const int n = h->SPW;
+      int p = 0;
+      for (int t = 0; p < n; t++) {
+        sop[w * n + p] = w * n + t;
+        pos[w * n + t] = p;
+        p++;
+        if (p < n && n - 1 - t != t) {
+          sop[w * n + p] = w * n + (n - 1 - t);
+          pos[w * n + (n - 1 - t)] = p;
+          p++;
+        }
+      }
+    }
+    HIPCHECK(h, hipMemcpy(xb0, hx.data(), sizeof(double) * h->npts, hipMemcpyHostToDevice));
+    HIPCHECK(h, hipMemcpy(yb0, hy.data(), sizeof(double) * h->npts, hipMemcpyHostToDevice));
+    HIPCHECK(h, hipMemcpy(arc, ha.data(), sizeof(double) * h->nseg, hipMemcpyHostToDevice));
+    HIPCHECK(h, hipMemcpy(seg_of_pos, sop.data(), sizeof(int) * h->nseg, hipMemcpyHostToDevice));
+    HIPCHECK(h, hipMemcpy(pos_of_seg, pos.data(), sizeof(int) * h->nseg, hipMemcpyHostToDevice));
+  }
+  // owner map starts empty everywhere
+  {
+    std::vector<int> big(np, 0x7fffffff), neg(np, -1);
+    HIPCHECK(h, hipMemcpy(h->B.owner, big.data(), sizeof(int) * np, hipMemcpyHostToDevice));
+    HIPCHECK(h, hipMemcpy(h->B.aidx, neg.data(), sizeof(int) * np, hipMemcpyHostToDevice));
+  }
+  HIPCHECK(h, hipMemset(h->d_st, 0, sizeof(DevStep)));
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lbm_destroy(lbm_handle* h) {
+  if (!h) return 0;
+  (void)hipStreamSynchronize(h->stream);
+  free_all(h);
+  delete h;
+  return 0;
+}
+
+int lbm_reset(lbm_handle* h) {
+  if (!h) return LBM_E_INVALID;
+  h->cur = 0;
+  launch_init_fields(h->stream, h->g, h->cfg.uu, h->feq, h->fbuf[0], h->R, h->slot[0]);
+  launch_ring_moments(h->stream, h->g, h->R);
+  for (int s = 0; s < 3; s++) h->slot_step[s] = -1;
+  h->slot_step[0] = 0;
+  // clear the owner maps at the previous episode's adjoint list, then the step state
+  if (h->W > 0) launch_clear_adjoint(h->stream, h->ip, h->B);
+  HIPCHECK(h, hipStreamSynchronize(h->stream));
+  HIPCHECK(h, hipMemset(h->d_st, 0, sizeof(DevStep)));
+  if (h->cfg.motion_source == LBM_MOTION_STREAM) {  // driver.f90:120-124
+    std::vector<double> z(2 * LBM_MAX_MOTION_COLS, 0.0);
+    HIPCHECK(h, hipMemcpy(h->d_stream, z.data(), sizeof(double) * z.size(), hipMemcpyHostToDevice));
+    h->motion_ready = true;
+  }
+  for (int w = 0; w < MAXW; w++) h->body_rel_pos[w] = 0.0;
+  update_body_extra(h);
+  h->n = 1;
+  h->movenum = 0;
+  h->flag2 = 1;
+  h->ip.movenum = 0;
+  HIPCHECK(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+// load_motion_table checks + build_motion_step_table (mode.f90:104-178, 327-412).
+int lbm_set_motion_table(lbm_handle* h, const double* rows, int32_t nrows, int32_t ncols, double* end_time) {
+  if (!h || !rows) return fail(h, LBM_E_INVALID, "null argument");
+  const int nc = h->ip.ncols;
+  if (ncols != nc) return fail(h, LBM_E_INVALID, "motion table must have %d columns, got %d", nc, ncols);
+  if (nrows < 1) return fail(h, LBM_E_INVALID, "motion table is empty");
+  for (int i = 1; i < nrows; i++)
+    if (rows[(size_t)i * nc] < rows[(size_t)(i - 1) * nc])
+      return fail(h, LBM_E_INVALID, "motion table time column must be non-decreasing");
+  const double end = rows[(size_t)(nrows - 1) * nc];
+  const double dt = h->ip.dt;
+  const long long count = (long long)((end + 1.0e-12) / dt) + 1;
+  std::vector<double> tab((size_t)count * nc);
+  auto T = [&](int r, int c) { return rows[(size_t)(r - 1) * nc + c]; };
+  int m = 1;
+  for (long long s = 1; s <= count; s++) {
+    const double tt = (double)(s - 1) * dt;
+    double* row = &tab[(size_t)(s - 1) * nc];
+    if (tt <= T(1, 0) + 1.0e-12) {
+      m = 1;
+      for (int c = 0; c < nc; c++) row[c] = T(1, c);
+    } else if (tt >= T(nrows, 0) - 1.0e-12) {
+      m = nrows;
+      for (int c = 0; c < nc; c++) row[c] = T(nrows, c);
+    } else {
+      m = std::max(1, std::min(m, nrows - 1));
+      while (m > 1 && tt <= T(m, 0) + 1.0e-12) m--;
+      while (m < nrows - 1 && tt > T(m + 1, 0) + 1.0e-12) m++;
+      const double t1 = T(m, 0), t2 = T(m + 1, 0), d = t2 - t1;
+      if (std::fabs(d) <= 1.0e-14) {
+        for (int c = 0; c < nc; c++) row[c] = T(m + 1, c);
+      } else {
+        double ratio = (tt - t1) / d;
+        ratio = std::min(std::max(ratio, 0.0), 1.0);
+        for (int c = 0; c < nc; c++) row[c] = T(m, c) + ratio * (T(m + 1, c) - T(m, c));
+      }
+    }
+    row[0] = tt;
+  }
+  if (tab.size() > h->table_cap) {
+    dfree(h->d_table);
+    h->d_table = nullptr;
+    int rc = dalloc(h, &h->d_table, tab.size());
+    if (rc) return rc;
+    h->table_cap = tab.size();
+  }
+  HIPCHECK(h, hipMemcpy(h->d_table, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice));
+  h->B.step_table = h->d_table;
+  h->ip.step_count = (int)count;
+  h->motion_end_time = end;
+  h->motion_ready = true;
+  if (end_time) *end_time = end;
+  return 0;
+}
+
+int lbm_set_stream_segment(lbm_handle* h, const double* start_row, const double* target_row, int32_t ncols) {
+  if (!h || !start_row || !target_row) return fail(h, LBM_E_INVALID, "null argument");
+  if (ncols != h->ip.ncols) return fail(h, LBM_E_INVALID, "stream rows must have %d columns", h->ip.ncols);
+  std::vector<double> z(2 * LBM_MAX_MOTION_COLS, 0.0);
+  for (int c = 0; c < ncols; c++) {
+    z[c] = start_row[c];
+    z[LBM_MAX_MOTION_COLS + c] = target_row[c];
+  }
+  HIPCHECK(h, hipMemcpy(h->d_stream, z.data(), sizeof(double) * z.size(), hipMemcpyHostToDevice));
+  h->motion_ready = true;
+  return 0;
+}
+
+int lbm_build_stream_target_row(int32_t W, const double* start, double target_time, const double* delta,
+                                double* target) {
+  if (W < 0 || W > MAXW || !start || !delta || !target) return LBM_E_INVALID;
+  const int nc = 1 + LBM_MOTION_COLS_PER_WING * W;
+  const double dt = target_time - start[0];
+  if (dt < -1.0e-12) return LBM_E_INVALID;
+  for (int c = 0; c < nc; c++) target[c] = start[c];
+  target[0] = target_time;
+  const double lt = std::max(dt, 0.0);
+  for (int w = 0; w < W; w++) {
+    const int col = 1 + w * 9, dc = 1 + w * 3;
+    h_interp_triplet(start[col], start[col + 1], start[col + 1] + delta[dc], dt, lt, &target[col],
+                     &target[col + 1], &target[col + 2]);
+    h_interp_triplet(start[col + 3], start[col + 4], start[col + 4] + delta[dc + 1], dt, lt, &target[col + 3],
+                     &target[col + 4], &target[col + 5]);
+    h_interp_triplet(start[col + 6], start[col + 7], start[col + 7] + delta[dc + 2], dt, lt, &target[col + 6],
+                     &target[col + 7], &target[col + 8]);
+  }
+  return 0;
+}
+
+int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motion_rows, double* row_sum,
+                double* last_row, double* err) {
+  if (!h) return LBM_E_INVALID;
+  if (h->cfg.motion_source == LBM_MOTION_FILE && !h->motion_ready && h->W > 0)
+    return fail(h, LBM_E_STATE, "motion table not loaded");
+  const int n0 = h->n;
+  const int nsteps = n_last >= n0 ? n_last - n0 + 1 : 0;
+  int rc = ensure_rows(h, (size_t)std::max(nsteps, 1));
+  if (rc) return rc;
+  if (h->timing)
+    while (h->ev.size() < 2 * (size_t)nsteps) {
+      hipEvent_t e;
+      HIPCHECK(h, hipEventCreate(&e));
+      h->ev.push_back(e);
+    }
+  HIPCHECK(h, hipMemsetAsync(h->d_rowsum, 0, sizeof(double) * h->row_len, h->stream));
+  HIPCHECK(h, hipMemsetAsync(&h->d_st->error, 0, sizeof(int), h->stream));
+  // moves of the frame are fully determined by body_rel_pos(1), which only
+  // this host changes (main.f90:393-411)
+  int last_move_step = -1;
+  for (int m = n0; m <= n_last; m++) {
+    const int s = m % 3;
+    double brp = h->W > 0 ? h->body_rel_pos[0] : 0.0;
+    const int dr = brp >= 1.0 ? 1 : (brp <= -1.0 ? -1 : 0);
+    const bool want_fields = (m >= n_last - 2) || dr != 0;
+    FieldSlot* fs = want_fields ? &h->slot[s] : nullptr;
+    const double* src = h->fbuf[h->cur];
+    double* dst = h->fbuf[1 - h->cur];
+    if (h->W > 0) {
+      launch_motion_markers(h->stream, h->ip, h->B, m);
+      launch_adjoint_build(h->stream, h->ip, h->B);
+      launch_ibm_gather(h->stream, h->ip, h->B, h->g, src);
+      launch_ibm_assemble(h->stream, h->ip, h->B);
+      launch_ibm_solve(h->stream, h->ip, h->B);
+      launch_ibm_spread(h->stream, h->ip, h->B);
+    } else {
+      launch_motion_markers(h->stream, h->ip, h->B, m);
+    }
+    launch_forces(h->stream, h->ip, h->B, m - n0, h->row_len);
+    const int step_i = m - n0;
+    if (h->timing) HIPCHECK(h, hipEventRecord(h->ev[2 * step_i], h->stream));
+    launch_lattice(h->stream, h->g, h->mrt, src, dst, fs);
+    if (h->timing) HIPCHECK(h, hipEventRecord(h->ev[2 * step_i + 1], h->stream));
+    if (h->W > 0) launch_adj_collide(h->stream, h->g, h->mrt, h->B, h->ip.cap, src, dst, fs);
+    launch_bc_ring(h->stream, h->g, h->mrt, h->R, src, dst, fs);
+    if (want_fields) h->slot_step[s] = m;
+    h->cur = 1 - h->cur;
+    if (dr != 0) {  // move_mesh (move_mesh.f90:17-58, 171-214)
+      double* a = h->fbuf[h->cur];
+      double* b = h->fbuf[1 - h->cur];
+      launch_shift_x(h->stream, h->g, dr, a, b, 9);
+      h->cur = 1 - h->cur;
+      launch_copy_ring_column(h->stream, h->g, h->fbuf[h->cur], dr == 1 ? 0 : h->g.nx - 1,
+                              dr == 1 ? h->R.left : h->R.right);
+      launch_ring_moments(h->stream, h->g, h->R);
+      FieldSlot& cs = h->slot[s];
+      launch_shift_x(h->stream, h->g, dr, cs.rho, h->spare.rho, 1);
+      launch_shift_x(h->stream, h->g, dr, cs.u, h->spare.u, 1);
+      launch_shift_x(h->stream, h->g, dr, cs.v, h->spare.v, 1);
+      std::swap(h->slot[s], h->spare);
+      h->body_rel_pos[0] = h->body_rel_pos[0] - (double)dr;
+      update_body_extra(h);
+      h->movenum += dr;
+      h->ip.movenum = h->movenum;
+      h->flag2 = 3;
+      last_move_step = m;
+    }
+    if (h->flag2 > 0) h->flag2 = h->flag2 - 1;
+  }
+  h->n = std::max(n0, n_last + 1);
+  // erranalysis of the last step (solution.f90:77-130): u of step n-1 against step n-2
+  double errv = 0.0;
+  const int ml = n_last;
+  bool have_err = false;
+  if (nsteps > 0 && ml >= 2) {
+    if (last_move_step == ml - 1) {
+      have_err = true;  // move_mesh copies u into u0: the interior difference is zero
+    } else {
+      const int s1 = (ml - 1) % 3, s2 = (ml - 2) % 3;
+      if (h->slot_step[s1] == ml - 1 && h->slot_step[s2] == ml - 2) {
+        const int nb = err_blocks(h->g);
+        launch_err_partial(h->stream, h->g, h->slot[s1], h->slot[s2], h->err_partial, nb);
+        launch_err_final(h->stream, h->err_partial, nb, h->err_out);
+        have_err = true;
+      }
+    }
+  }
+  HIPCHECK(h, hipGetLastError());
+  HIPCHECK(h, hipStreamSynchronize(h->stream));
+  DevStep st_host;
+  HIPCHECK(h, hipMemcpy(&st_host, h->d_st, sizeof(DevStep), hipMemcpyDeviceToHost));
+  if (st_host.error != 0)
+    return fail(h, st_host.error == DERR_MOTION ? LBM_E_STATE : (st_host.error == DERR_NOT_SPD ? LBM_E_NUMERIC : LBM_E_RANGE),
+                "step failed: %s (info %d)", dev_error_text(st_host.error), st_host.error_info);
+  if (h->timing)
+    for (int i = 0; i < nsteps; i++) {
+      float ms = 0.f;
+      HIPCHECK(h, hipEventElapsedTime(&ms, h->ev[2 * i], h->ev[2 * i + 1]));
+      h->lattice_ms += ms;
+      h->lattice_launches++;
+    }
+  if (have_err && last_move_step != ml - 1) HIPCHECK(h, hipMemcpy(&errv, h->err_out, sizeof(double), hipMemcpyDeviceToHost));
+  if (err) *err = errv;
+  if (nsteps > 0) {
+    if (force_rows)
+      HIPCHECK(h, hipMemcpy(force_rows, h->d_rows, sizeof(double) * h->row_len * nsteps, hipMemcpyDeviceToHost));
+    if (motion_rows)
+      HIPCHECK(h, hipMemcpy(motion_rows, h->d_mrows, sizeof(double) * (1 + 3 * h->W) * nsteps, hipMemcpyDeviceToHost));
+    if (last_row)
+      HIPCHECK(h, hipMemcpy(last_row, h->d_rows + (size_t)(nsteps - 1) * h->row_len, sizeof(double) * h->row_len,
+                            hipMemcpyDeviceToHost));
+  }
+  if (row_sum) {
+    if (nsteps > 0)
+      HIPCHECK(h, hipMemcpy(row_sum, h->d_rowsum, sizeof(double) * h->row_len, hipMemcpyDeviceToHost));
+    else
+      for (int c = 0; c < h->row_len; c++) row_sum[c] = 0.0;
+  }
+  return 0;
+}
+
+int lbm_get_state(lbm_handle* h, int32_t* n_next, int32_t* movenum, int32_t* flag2, int32_t* row_len,
+                  int32_t* adjoint_n) {
+  if (!h) return LBM_E_INVALID;
+  if (n_next) *n_next = h->n;
+  if (movenum) *movenum = h->movenum;
+  if (flag2) *flag2 = h->flag2;
+  if (row_len) *row_len = h->row_len;
+  if (adjoint_n) {
+    DevStep st;
+    HIPCHECK(h, hipMemcpy(&st, h->d_st, sizeof(DevStep), hipMemcpyDeviceToHost));
+    *adjoint_n = st.adj_n;
+  }
+  return 0;
+}
+
+int lbm_get_fields(lbm_handle* h, double* rho, double* u, double* v, double* pre) {
+  if (!h) return LBM_E_INVALID;
+  const int m = h->n - 1;
+  const int s = ((m % 3) + 3) % 3;
+  if (h->slot_step[s] != m) return fail(h, LBM_E_STATE, "fields of step %d are not resident", m);
+  const size_t np = (size_t)h->g.nx * h->g.ny;
+  HIPCHECK(h, hipStreamSynchronize(h->stream));
+  if (rho) HIPCHECK(h, hipMemcpy(rho, h->slot[s].rho, np * sizeof(double), hipMemcpyDeviceToHost));
+  if (u) HIPCHECK(h, hipMemcpy(u, h->slot[s].u, np * sizeof(double), hipMemcpyDeviceToHost));
+  if (v) HIPCHECK(h, hipMemcpy(v, h->slot[s].v, np * sizeof(double), hipMemcpyDeviceToHost));
+  if (pre) {
+    std::vector<double> r(np);
+    HIPCHECK(h, hipMemcpy(r.data(), h->slot[s].rho, np * sizeof(double), hipMemcpyDeviceToHost));
+    for (size_t q = 0; q < np; q++) pre[q] = r[q] / 3.0;
+  }
+  return 0;
+}
+
+int lbm_get_populations(lbm_handle* h, double* f, double* f0) {
+  if (!h) return LBM_E_INVALID;
+  const size_t np = (size_t)h->g.nx * h->g.ny;
+  HIPCHECK(h, hipStreamSynchronize(h->stream));
+  if (f)
+    for (int k = 0; k < 9; k++)
+      HIPCHECK(h, hipMemcpy(f + k * np, h->fbuf[h->cur] + k * h->g.ps, np * sizeof(double), hipMemcpyDeviceToHost));
+  if (f0) {
+    std::memset(f0, 0, 9 * np * sizeof(double));
+    std::vector<double> L(9 * (size_t)h->g.ny), Rr(9 * (size_t)h->g.ny);
+    HIPCHECK(h, hipMemcpy(L.data(), h->R.left, L.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHECK(h, hipMemcpy(Rr.data(), h->R.right, Rr.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int k = 0; k < 9; k++)
+      for (int j = 0; j < h->g.ny; j++) {
+        f0[k * np + (size_t)j * h->g.nx] = L[(size_t)k * h->g.ny + j];
+        f0[k * np + (size_t)j * h->g.nx + h->g.nx - 1] = Rr[(size_t)k * h->g.ny + j];
+      }
+  }
+  return 0;
+}
+
+int lbm_set_populations(lbm_handle* h, const double* f, const double* f0) {
+  if (!h) return LBM_E_INVALID;
+  const size_t np = (size_t)h->g.nx * h->g.ny;
+  HIPCHECK(h, hipStreamSynchronize(h->stream));
+  if (f) {
+    for (int k = 0; k < 9; k++)
+      HIPCHECK(h, hipMemcpy(h->fbuf[h->cur] + k * h->g.ps, f + k * np, np * sizeof(double), hipMemcpyHostToDevice));
+    const int m = h->n - 1;
+    const int s = ((m % 3) + 3) % 3;
+    launch_moments_all(h->stream, h->g, h->fbuf[h->cur], h->slot[s]);
+    h->slot_step[s] = m;
+  }
+  if (f0) {
+    std::vector<double> L(9 * (size_t)h->g.ny), Rr(9 * (size_t)h->g.ny);
+    for (int k = 0; k < 9; k++)
+      for (int j = 0; j < h->g.ny; j++) {
+        L[(size_t)k * h->g.ny + j] = f0[k * np + (size_t)j * h->g.nx];
+        Rr[(size_t)k * h->g.ny + j] = f0[k * np + (size_t)j * h->g.nx + h->g.nx - 1];
+      }
+    HIPCHECK(h, hipMemcpy(h->R.left, L.data(), L.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHECK(h, hipMemcpy(h->R.right, Rr.data(), Rr.size() * sizeof(double), hipMemcpyHostToDevice));
+    launch_ring_moments(h->stream, h->g, h->R);
+  }
+  HIPCHECK(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int lbm_get_markers(lbm_handle* h, double* xb1, double* yb1, double* xb2, double* yb2) {
+  if (!h) return LBM_E_INVALID;
+  HIPCHECK(h, hipStreamSynchronize(h->stream));
+  if (h->npts > 0) {
+    if (xb1) HIPCHECK(h, hipMemcpy(xb1, h->B.xb1, h->npts * sizeof(double), hipMemcpyDeviceToHost));
+    if (yb1) HIPCHECK(h, hipMemcpy(yb1, h->B.yb1, h->npts * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  if (h->nseg > 0) {
+    if (xb2) HIPCHECK(h, hipMemcpy(xb2, h->B.xb2, h->nseg * sizeof(double), hipMemcpyDeviceToHost));
+    if (yb2) HIPCHECK(h, hipMemcpy(yb2, h->B.yb2, h->nseg * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  return 0;
+}
+
+int lbm_get_ibm(lbm_handle* h, int32_t capacity, int32_t* ij, int32_t* wing_id, double* wing_force,
+                double* wing_moment) {
+  if (!h) return LBM_E_INVALID;
+  HIPCHECK(h, hipStreamSynchronize(h->stream));
+  DevStep st;
+  HIPCHECK(h, hipMemcpy(&st, h->d_st, sizeof(DevStep), hipMemcpyDeviceToHost));
+  const int n = std::min(st.adj_n, capacity);
+  if (n > 0 && (ij || wing_id)) {
+    std::vector<int> hij(2 * (size_t)n), hseg(n);
+    HIPCHECK(h, hipMemcpy(hij.data(), h->B.adj_ij, sizeof(int) * 2 * n, hipMemcpyDeviceToHost));
+    HIPCHECK(h, hipMemcpy(hseg.data(), h->B.adj_seg, sizeof(int) * n, hipMemcpyDeviceToHost));
+    for (int a = 0; a < n; a++) {
+      if (ij) {
+        ij[2 * a] = hij[2 * a] + 1;
+        ij[2 * a + 1] = hij[2 * a + 1] + 1;
+      }
+      if (wing_id) wing_id[a] = hseg[a] / h->SPW + 1;
+    }
+  }
+  for (int w = 0; w < MAXW; w++) {
+    if (wing_force) {
+      wing_force[2 * w] = w < h->W ? st.force[w][0] : 0.0;
+      wing_force[2 * w + 1] = w < h->W ? st.force[w][1] : 0.0;
+    }
+    if (wing_moment) wing_moment[w] = w < h->W ? st.moment[w] : 0.0;
+  }
+  return 0;
+}
+
+int lbm_set_timing(lbm_handle* h, int32_t enable) {
+  if (!h) return LBM_E_INVALID;
+  h->timing = enable != 0;
+  h->lattice_ms = 0.0;
+  h->lattice_launches = 0;
+  return 0;
+}
+
+int lbm_get_timing(lbm_handle* h, double* lattice_ms, int32_t* launches) {
+  if (!h) return LBM_E_INVALID;
+  if (lattice_ms) *lattice_ms = h->lattice_ms;
+  if (launches) *launches = h->lattice_launches;
+  return 0;
+}
+
+int lbm_set_body_rel_pos(lbm_handle* h, int32_t wing, double pos) {
+  if (!h || wing < 1 || wing > h->W) return fail(h, LBM_E_INVALID, "wing out of range");
+  h->body_rel_pos[wing - 1] = pos;
+  update_body_extra(h);
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,228 @@
+"""Host-side handle of one solver instance behind the C ABI (include/lbm_hip.h).
+
+The reference's host program calls evolution/erranalysis/move_mesh/init
+directly (main.f90:315-411, driver.f90:205-408).  :class:`Solver` exposes the
+same operations over the C ABI: ``reset`` (init + read_boundary),
+``set_motion_table`` / ``set_stream_segment`` (mode.f90 motion sources),
+``advance`` (the step loop up to a target step) and field / marker / IBM
+readback for SAVE and tests.
+
+The class is bound to a function table ``lib`` with a symbol prefix; the
+product binds it to liblbm_hip.so (prefix ``lbm_``).
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+from .config import SolverConfig
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int32)
+
+_PROTOS = {
+    "create": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
+    "destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "reset": (ctypes.c_int, [ctypes.c_void_p]),
+    "set_motion_table": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.c_int32, ctypes.c_int32, _dp]),
+    "set_stream_segment": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, ctypes.c_int32]),
+    "build_stream_target_row": (ctypes.c_int, [ctypes.c_int32, _dp, ctypes.c_double, _dp, _dp]),
+    "advance": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _dp, _dp, _dp, _dp, _dp]),
+    "get_state": (ctypes.c_int, [ctypes.c_void_p, _ip, _ip, _ip, _ip, _ip]),
+    "get_fields": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _dp]),
+    "get_populations": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
+    "set_populations": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
+    "get_markers": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _dp]),
+    "get_ibm": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _ip, _ip, _dp, _dp]),
+    "set_body_rel_pos": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_double]),
+    "last_error": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]),
+}
+
+
+def bind(lib: ctypes.CDLL, prefix: str) -> None:
+    """Attach argument/result types to every C-ABI entry point of ``lib``."""
+    for name, (res, args) in _PROTOS.items():
+        fn = getattr(lib, prefix + name)
+        fn.restype = res
+        fn.argtypes = args
+
+
+def _p(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(_dp)
+
+
+class LbmError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+@dataclass
+class AdvanceResult:
+    force_rows: Optional[np.ndarray]
+    motion_rows: Optional[np.ndarray]
+    row_sum: np.ndarray
+    last_row: np.ndarray
+    err: float
+    n_steps: int
+
+
+class Solver:
+    """One solver instance (one episode's lattice, IBM and motion state)."""
+
+    prefix = "lbm_"
+
+    def __init__(self, cfg: SolverConfig, lib: Optional[ctypes.CDLL] = None):
+        if lib is None:
+            from ._native import load_library
+            lib = load_library()
+        self.lib = lib
+        self.cfg = cfg
+        self._h = ctypes.c_void_p()
+        c = cfg.to_c()
+        rc = self._fn("create")(ctypes.byref(c), ctypes.byref(self._h))
+        if rc != 0:
+            raise LbmError(rc, self._msg(None))
+        self.nx, self.ny = cfg.nx, cfg.ny
+        self.row_len = cfg.row_len
+
+    # ------------------------------------------------------------ plumbing
+    def _fn(self, name):
+        return getattr(self.lib, self.prefix + name)
+
+    def _msg(self, h) -> str:
+        buf = ctypes.create_string_buffer(512)
+        self._fn("last_error")(h, buf, 512)
+        return buf.value.decode(errors="replace")
+
+    def _check(self, rc: int) -> None:
+        if rc != 0:
+            raise LbmError(rc, self._msg(self._h))
+
+    def close(self) -> None:
+        if self._h:
+            self._fn("destroy")(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ----------------------------------------------------------- operations
+    def reset(self) -> None:
+        self._check(self._fn("reset")(self._h))
+
+    def set_motion_table(self, rows: np.ndarray) -> float:
+        rows = np.ascontiguousarray(rows, dtype=np.float64)
+        end = ctypes.c_double()
+        self._check(self._fn("set_motion_table")(self._h, _p(rows), rows.shape[0], rows.shape[1],
+                                                 ctypes.byref(end)))
+        return end.value
+
+    def set_stream_segment(self, start_row: np.ndarray, target_row: np.ndarray) -> None:
+        s = np.ascontiguousarray(start_row, dtype=np.float64)
+        t = np.ascontiguousarray(target_row, dtype=np.float64)
+        self._check(self._fn("set_stream_segment")(self._h, _p(s), _p(t), s.shape[0]))
+
+    def build_stream_target_row(self, start_row: np.ndarray, target_time: float,
+                                delta_row: np.ndarray) -> np.ndarray:
+        s = np.ascontiguousarray(start_row, dtype=np.float64)
+        d = np.zeros(1 + 3 * self.cfg.wing_count, dtype=np.float64)
+        dd = np.asarray(delta_row, dtype=np.float64)
+        d[: min(d.size, dd.size)] = dd[: d.size]
+        out = np.zeros_like(s)
+        rc = self._fn("build_stream_target_row")(self.cfg.wing_count, _p(s), float(target_time),
+                                                 _p(d), _p(out))
+        if rc != 0:
+            raise LbmError(rc, "STEP time is before the segment start")
+        return out
+
+    def state(self):
+        vals = [ctypes.c_int32() for _ in range(5)]
+        self._check(self._fn("get_state")(self._h, *[ctypes.byref(v) for v in vals]))
+        keys = ("n_next", "movenum", "flag2", "row_len", "adjoint_n")
+        return dict(zip(keys, (v.value for v in vals)))
+
+    def advance(self, n_last: int, want_rows: bool = False, want_motion: bool = False) -> AdvanceResult:
+        st = self.state()
+        nsteps = max(0, n_last - st["n_next"] + 1)
+        rows = np.zeros((max(nsteps, 1), self.row_len)) if want_rows else None
+        mrows = np.zeros((max(nsteps, 1), 1 + 3 * self.cfg.wing_count)) if want_motion else None
+        rsum = np.zeros(self.row_len)
+        last = np.zeros(self.row_len)
+        err = ctypes.c_double()
+        self._check(self._fn("advance")(self._h, int(n_last), _p(rows), _p(mrows), _p(rsum),
+                                        _p(last), ctypes.byref(err)))
+        if rows is not None:
+            rows = rows[:nsteps]
+        if mrows is not None:
+            mrows = mrows[:nsteps]
+        return AdvanceResult(rows, mrows, rsum, last, err.value, nsteps)
+
+    def fields(self):
+        n = self.nx * self.ny
+        rho, u, v, pre = (np.zeros(n) for _ in range(4))
+        self._check(self._fn("get_fields")(self._h, _p(rho), _p(u), _p(v), _p(pre)))
+        shp = (self.ny, self.nx)  # [j, i]: i fastest, as the C layout
+        return rho.reshape(shp), u.reshape(shp), v.reshape(shp), pre.reshape(shp)
+
+    def populations(self):
+        n = self.nx * self.ny
+        f = np.zeros(9 * n)
+        f0 = np.zeros(9 * n)
+        self._check(self._fn("get_populations")(self._h, _p(f), _p(f0)))
+        return f.reshape(9, self.ny, self.nx), f0.reshape(9, self.ny, self.nx)
+
+    def set_populations(self, f: np.ndarray, f0: Optional[np.ndarray] = None) -> None:
+        fa = np.ascontiguousarray(f, dtype=np.float64).reshape(-1)
+        f0a = None if f0 is None else np.ascontiguousarray(f0, dtype=np.float64).reshape(-1)
+        self._check(self._fn("set_populations")(self._h, _p(fa), _p(f0a)))
+
+    def markers(self):
+        npts = self.cfg.wing_count * self.cfg.points_per_wing
+        nseg = self.cfg.wing_count * max(self.cfg.points_per_wing - 1, 0)
+        xb1, yb1 = np.zeros(max(npts, 1)), np.zeros(max(npts, 1))
+        xb2, yb2 = np.zeros(max(nseg, 1)), np.zeros(max(nseg, 1))
+        self._check(self._fn("get_markers")(self._h, _p(xb1), _p(yb1), _p(xb2), _p(yb2)))
+        return xb1[:npts], yb1[:npts], xb2[:nseg], yb2[:nseg]
+
+    def ibm(self):
+        n = self.state()["adjoint_n"]
+        ij = np.zeros(2 * max(n, 1), dtype=np.int32)
+        wid = np.zeros(max(n, 1), dtype=np.int32)
+        wf = np.zeros(16)
+        wm = np.zeros(8)
+        self._check(self._fn("get_ibm")(self._h, n, ij.ctypes.data_as(_ip), wid.ctypes.data_as(_ip),
+                                        _p(wf), _p(wm)))
+        return ij[: 2 * n].reshape(n, 2), wid[:n], wf.reshape(8, 2), wm
+
+    def set_body_rel_pos(self, wing: int, pos: float) -> None:
+        self._check(self._fn("set_body_rel_pos")(self._h, int(wing), float(pos)))
+
+
+class TimedSolver(Solver):
+    """Solver with the lattice-kernel event timing of the HIP library."""
+
+    def set_timing(self, enable: bool) -> None:
+        fn = self.lib.lbm_set_timing
+        fn.restype, fn.argtypes = ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]
+        self._check(fn(self._h, 1 if enable else 0))
+
+    def timing(self):
+        fn = self.lib.lbm_get_timing
+        fn.restype = ctypes.c_int
+        fn.argtypes = [ctypes.c_void_p, _dp, _ip]
+        ms, n = ctypes.c_double(), ctypes.c_int32()
+        self._check(fn(self._h, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value

@@ -1,0 +1,289 @@
+"""CPU tests that pin the oracle (oracle/lbm_oracle.c) routine by routine with
+closed forms, textbook identities and brute force on tiny inputs."""
+import math
+
+import numpy as np
+import pytest
+
+from lbm_mutiwing_amd import SolverConfig, naca0012_markers, preset, sinusoidal_motion_table, WingMotion
+
+E = np.array([(0, 0), (1, 0), (0, 1), (-1, 0), (0, -1), (1, 1), (-1, 1), (-1, -1), (1, -1)])
+W_EXACT = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
+
+
+def _feq_exact(rho, u, v):
+    eu = E[:, 0] * u + E[:, 1] * v
+    return W_EXACT * rho * (1 + 3 * eu + 4.5 * eu ** 2 - 1.5 * (u * u + v * v))
+
+
+def _mats(oracle_mod):
+    import ctypes
+    lib = oracle_mod.load()
+    m = np.zeros(81)
+    mi = np.zeros(81)
+    dp = ctypes.POINTER(ctypes.c_double)
+    lib.orc_mrt_matrices(m.ctypes.data_as(dp), mi.ctypes.data_as(dp))
+    return m.reshape(9, 9), mi.reshape(9, 9)
+
+
+def test_weights_are_single_precision_quotients(oracle_mod):
+    import ctypes
+    w = np.zeros(9)
+    oracle_mod.load().orc_weights(w.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    # D2Q9model.f90:24-32 divides single-precision literals
+    assert w[0] == np.float64(np.float32(4.0) / np.float32(9.0)) == 0.44444444775581360
+    assert np.all(w[1:5] == 0.1111111119389534)
+    assert np.all(w[5:] == 0.027777777984738350)
+    assert abs(w.sum() - 1.0) < 1e-8 and w.sum() != 1.0
+
+
+def test_mrt_matrix_and_gauss_jordan_inverse(oracle_mod):
+    m, mi = _mats(oracle_mod)
+    # M is orthogonal with row norms (9,36,36,6,12,6,12,4,4): M^-1 = M^T diag(1/norm)
+    assert np.allclose(m @ m.T, np.diag([9, 36, 36, 6, 12, 6, 12, 4, 4]), atol=0)
+    closed = m.T @ np.diag(1.0 / np.array([9, 36, 36, 6, 12, 6, 12, 4, 4]))
+    assert np.allclose(mi, closed, rtol=0, atol=1e-16)
+    assert np.array_equal(mi == 0, closed == 0)
+    assert np.allclose(m @ mi, np.eye(9), atol=1e-15)
+    # rows of M are the Lallemand-Luo moments rho, e, eps, jx, qx, jy, qy, pxx, pxy of E
+    ex, ey = E[:, 0], E[:, 1]
+    e2 = ex ** 2 + ey ** 2
+    assert np.array_equal(m[0], np.ones(9))
+    assert np.array_equal(m[1], -4 + 3 * e2)
+    assert np.array_equal(m[2], 4 - 21 * e2 / 2 + 9 * e2 ** 2 / 2)
+    assert np.array_equal(m[3], ex) and np.array_equal(m[5], ey)
+    assert np.array_equal(m[4], (-5 + 3 * e2) * ex) and np.array_equal(m[6], (-5 + 3 * e2) * ey)
+    assert np.array_equal(m[7], ex ** 2 - ey ** 2) and np.array_equal(m[8], ex * ey)
+
+
+def test_delta_kernel_moments(oracle_mod):
+    phi = oracle_mod.load().orc_delta_kernel
+    assert phi(2.5) == pytest.approx(0.0, abs=1e-15)
+    assert phi(3.0) == 0.0
+    # continuity at the piece boundaries
+    for r in (0.5, 1.5):
+        assert phi(r - 1e-12) == pytest.approx(phi(r + 1e-12), abs=1e-9)
+    rng = np.random.default_rng(0)
+    for x in rng.uniform(0, 1, 50):
+        d = np.arange(-4, 5) - x
+        vals = np.array([phi(abs(t)) for t in d])
+        assert vals.sum() == pytest.approx(1.0, abs=1e-14)          # partition of unity
+        assert (d * vals).sum() == pytest.approx(0.0, abs=1e-14)    # zero first moment
+        assert np.all(vals >= -1e-16)
+
+
+def _cfg_nowings(**kw):
+    c = SolverConfig(nx=24, ny=16, scale=1 / 8, uu=0.05, re=50.0, wing_count=0,
+                     s=(0.0, 1.4, 1.4, 0.0, 1.2, 0.0, 1.2, 0.0, 0.0))
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
+
+
+def test_collision_equilibrium_fixed_point_and_conservation(oracle_mod):
+    o = oracle_mod.OracleSolver(_cfg_nowings())
+    rng = np.random.default_rng(1)
+    for _ in range(20):
+        u, v = rng.uniform(-0.1, 0.1, 2)
+        f = _feq_exact(1.0, u, v)
+        out = o.collide(f, 1.0, u, v)
+        assert np.allclose(out, f, rtol=0, atol=4e-16)            # equilibrium is a fixed point
+        g = rng.normal(0, 1e-3, 9) * W_EXACT
+        f2 = f + g
+        rho = f2.sum()
+        jx, jy = (E[:, 0] * f2).sum(), (E[:, 1] * f2).sum()
+        gx, gy = rng.normal(0, 1e-4, 2)
+        out = o.collide(f2, rho, jx / rho, jy / rho, gx, gy)
+        assert out.sum() == pytest.approx(rho, abs=1e-15)         # mass
+        assert (E[:, 0] * out).sum() == pytest.approx(jx + gx, abs=1e-15)  # momentum + body force
+        assert (E[:, 1] * out).sum() == pytest.approx(jy + gy, abs=1e-15)
+
+
+def test_collision_relaxes_each_moment_with_its_rate(oracle_mod):
+    o = oracle_mod.OracleSolver(_cfg_nowings())
+    m, mi = _mats(oracle_mod)
+    s = o.rates()
+    u, v = 0.03, -0.02
+    feq = _feq_exact(1.0, u, v)
+    for k in (1, 2, 4, 6, 7, 8):
+        d = 1e-4 * mi[:, k]            # perturb moment k only
+        out = o.collide(feq + d, 1.0, u, v)
+        assert np.allclose(out - feq, (1 - s[k]) * d, rtol=0, atol=4e-16)
+
+
+def test_collision_forcing_is_projected_guo_term(oracle_mod):
+    """Moment-space forcing equals (I - S/2) M F with Guo's velocity-space term
+    F_i = w_i [3 (e_i - u).g + 9 (e_i.u)(e_i.g)] (Guo, Zheng & Shi 2002)."""
+    o = oracle_mod.OracleSolver(_cfg_nowings())
+    m, _ = _mats(oracle_mod)
+    s = o.rates()
+    u, v, gx, gy = 0.04, -0.01, 2e-4, -3e-4
+    f = _feq_exact(1.0, u, v)
+    d_out = o.collide(f, 1.0, u, v, gx, gy) - o.collide(f, 1.0, u, v)
+    eu = E[:, 0] * u + E[:, 1] * v
+    eg = E[:, 0] * gx + E[:, 1] * gy
+    F = W_EXACT * (3 * (eg - (u * gx + v * gy)) + 9 * eu * eg)
+    assert np.allclose(m @ d_out, (1 - s / 2) * (m @ F), rtol=0, atol=2e-15)
+
+
+def test_boundary_extrapolation_keeps_uniform_stream_and_nonequilibrium(oracle_mod):
+    o = oracle_mod.OracleSolver(_cfg_nowings())
+    _, mi = _mats(oracle_mod)
+    s = o.rates()
+    u, v = 0.05, 0.0
+    feq = _feq_exact(1.0, u, v)
+    assert np.allclose(o.bc_node(1.0, u, v, 1.0, u, v, feq), feq, rtol=0, atol=4e-16)
+    for k in (1, 2, 4, 6, 7, 8):
+        d = 1e-4 * mi[:, k]
+        out = o.bc_node(1.0, u, v, 1.0, u, v, feq + d)
+        assert np.allclose(out - feq, (1 - s[k]) * d, rtol=0, atol=4e-16)
+
+
+def test_streaming_is_a_pull_permutation(oracle_mod):
+    # rates ~0: collision is the identity up to rounding, so one step moves
+    # each population one node along e_k
+    cfg = _cfg_nowings(s=(0.0,) * 7 + (1e-300, 1e-300))
+    o = oracle_mod.OracleSolver(cfg)
+    rng = np.random.default_rng(2)
+    f = (W_EXACT[:, None, None] * (1 + 0.01 * rng.normal(size=(9, cfg.ny, cfg.nx))))
+    o.set_populations(f)
+    o.advance(1)
+    f1, _ = o.populations()
+    for k in range(9):
+        exp = np.roll(np.roll(f[k], E[k, 1], axis=0), E[k, 0], axis=1)
+        assert np.allclose(f1[k, 1:-1, 1:-1], exp[1:-1, 1:-1], rtol=0, atol=6e-16)
+    rho, u, v, _ = o.fields()
+    fs = np.stack([np.roll(np.roll(f[k], E[k, 1], 0), E[k, 0], 1) for k in range(9)])
+    r = fs.sum(0)
+    assert np.allclose(rho[1:-1, 1:-1], r[1:-1, 1:-1], atol=1e-15)
+    assert np.allclose(u[1:-1, 1:-1], ((E[:, 0, None, None] * fs).sum(0) / r)[1:-1, 1:-1], atol=1e-15)
+
+
+def test_walls_and_uniform_far_field(oracle_mod):
+    o = oracle_mod.OracleSolver(_cfg_nowings())
+    o.advance(3)
+    rho, u, v, _ = o.fields()
+    assert np.all(u[0] == 0) and np.all(v[0] == 0) and np.all(u[-1] == 0) and np.all(v[-1] == 0)
+    assert np.array_equal(rho[0], rho[1]) and np.array_equal(rho[-1], rho[-2])
+    # far from the walls the freestream is preserved to the weight rounding
+    assert np.allclose(u[4:-4, :], 0.05, atol=1e-6)
+
+
+def test_err_is_relative_velocity_change(oracle_mod):
+    cfg = preset("small")
+    rows = sinusoidal_motion_table(cfg, [WingMotion(0.1, 0.1), WingMotion(0.0, 0.2)], 50)
+    o = oracle_mod.OracleSolver(cfg)
+    o.set_motion_table(rows)
+    o.advance(8)
+    _, u1, v1, _ = o.fields()
+    o.advance(9)
+    _, u2, v2, _ = o.fields()
+    r = o.advance(10)
+    du = (u2 - u1)[1:-1, 1:-1]
+    dv = (v2 - v1)[1:-1, 1:-1]
+    exp = math.sqrt((du ** 2 + dv ** 2).sum()) / (math.sqrt((u2[1:-1, 1:-1] ** 2 + v2[1:-1, 1:-1] ** 2).sum()) + 1e-30)
+    assert r.err == pytest.approx(exp, rel=1e-12)
+
+
+def _static_translation_cfg():
+    cfg = preset("small")
+    return cfg
+
+
+def test_ibm_enforces_no_slip_at_every_segment(oracle_mod):
+    """After the correction, the velocity interpolated to each segment
+    midpoint equals the rigid-body velocity (here a pure translation)."""
+    cfg = preset("small")
+    n = 30
+    rows = sinusoidal_motion_table(cfg, [WingMotion(), WingMotion()], n)
+    rows[:, 1 + 4] = 0.3      # wing 1 x-velocity column
+    rows[:, 10 + 7] = -0.2    # wing 2 y-velocity column
+    o = oracle_mod.OracleSolver(cfg)
+    o.set_motion_table(rows)
+    o.advance(n)
+    us, vs, u, v, rho = o.ibm_nodes()
+    ij, wid, wf, wm = o.ibm()
+    _, _, xb2, yb2 = o.markers()
+    ox, oy = cfg.origin()
+    phi = np.vectorize(oracle_mod.load().orc_delta_kernel)
+    x = ox + cfg.scale * (ij[:, 0] - 1)
+    y = oy + cfg.scale * (ij[:, 1] - 1)
+    D = phi(np.abs((x[None, :] - xb2[:, None]) / cfg.scale)) * phi(np.abs((y[None, :] - yb2[:, None]) / cfg.scale))
+    nseg_w = cfg.points_per_wing - 1
+    ub = np.where(np.arange(xb2.size) < nseg_w, cfg.uu * 0.3, 0.0)
+    vb = np.where(np.arange(xb2.size) < nseg_w, 0.0, cfg.uu * -0.2)
+    assert np.abs(D @ u - ub).max() < 1e-12
+    assert np.abs(D @ v - vb).max() < 1e-12
+    assert np.abs(D @ us - ub).max() > 1e-4      # the correction did something
+    # per-wing force: minus the momentum the correction injects, normalised
+    s = cfg.scale
+    for w in (1, 2):
+        sel = wid == w
+        fx = -(2 * rho[sel] * (u[sel] - us[sel])).sum() * 2 / (cfg.uu ** 2 / s)
+        fy = -(2 * rho[sel] * (v[sel] - vs[sel])).sum() * 2 / (cfg.uu ** 2 / s)
+        assert wf[w - 1, 0] == pytest.approx(fx, rel=1e-9, abs=1e-12)
+        assert wf[w - 1, 1] == pytest.approx(fy, rel=1e-9, abs=1e-12)
+
+
+def test_adjoint_list_is_first_claim_union_of_boxes(oracle_mod):
+    cfg = preset("small")
+    rows = sinusoidal_motion_table(cfg, [WingMotion(0.2, 0.1, phase_p=0.3), WingMotion(-0.1, 0.2)], 20)
+    o = oracle_mod.OracleSolver(cfg)
+    o.set_motion_table(rows)
+    o.advance(20)
+    ij, wid, _, _ = o.ibm()
+    _, _, xb2, yb2 = o.markers()
+    ox, oy = cfg.origin()
+    h = cfg.scale
+    first = {}
+    for k in range(xb2.size):
+        # nodes within 2.5 lattice units of the midpoint, 1-based indices
+        ii = [i for i in range(1, cfg.nx + 1) if abs((ox + (i - 1) * h - xb2[k]) / h) <= 2.5]
+        jj = [j for j in range(1, cfg.ny + 1) if abs((oy + (j - 1) * h - yb2[k]) / h) <= 2.5]
+        for i in ii:
+            for j in jj:
+                first.setdefault((i, j), k)
+    expect = sorted(first, key=lambda n: (first[n], n[0], n[1]))
+    assert [tuple(t) for t in ij] == expect
+    spw = cfg.points_per_wing - 1
+    assert np.array_equal(wid, np.array([first[n] // spw + 1 for n in expect]))
+
+
+def test_stream_target_row_ramps(oracle_mod):
+    from lbm_mutiwing_amd import Solver
+    o = oracle_mod.OracleSolver(SolverConfig(nx=16, ny=16, scale=1 / 8, wing_count=1,
+                                             markers=naca0012_markers(4), motion_source=2))
+    start = np.zeros(10)
+    start[0] = 0.5
+    start[1:10] = [0.1, 0.2, 0.0, 0.3, 0.4, 0.0, -0.1, 0.05, 0.0]
+    T = 0.1
+    tgt = o.build_stream_target_row(start, 0.5 + T, [0.6, 0.3, -0.2, 0.1])
+    for c, dv in ((1, 0.3), (4, -0.2), (7, 0.1)):
+        v0 = start[c + 1]
+        assert tgt[c + 1] == pytest.approx(v0 + dv, abs=1e-15)          # v(T) = v0 + dv
+        assert tgt[c + 2] == pytest.approx(0.0, abs=1e-12)              # a(T) = 0 (d(1) = 0)
+        assert tgt[c] == pytest.approx(start[c] + v0 * T + dv * T / 2, abs=1e-14)  # s = s0 + mean v * T
+    with pytest.raises(Exception):
+        o.build_stream_target_row(start, 0.4, [0, 0, 0, 0])
+
+
+def test_file_table_resampling_is_linear_and_clamped(oracle_mod):
+    cfg = preset("small")
+    rows = np.zeros((3, cfg.motion_cols))
+    rows[:, 0] = [0.0, 0.1, 0.3]
+    rows[:, 1] = [0.0, 0.2, -0.2]              # wing 1 angle, piecewise linear
+    rows[:, 4] = cfg.wing_x0
+    rows[:, 7] = cfg.wing_y0
+    rows[:, 13] = cfg.wing_x0 + cfg.wing_dx
+    rows[:, 16] = cfg.wing_y0
+    o = oracle_mod.OracleSolver(cfg)
+    end = o.set_motion_table(rows)
+    assert end == 0.3
+    n_last = cfg.n_steps_for(end)
+    r = o.advance(n_last, want_rows=True)
+    t = r.force_rows[:, 0]
+    ang = r.force_rows[:, 4 + 24 - 18]  # first motion column block starts at 1+3W+3 = 10; angle is +6
+    ang = r.force_rows[:, 10 + 6]
+    exp = np.interp(t, rows[:, 0], rows[:, 1])
+    assert np.allclose(ang, exp, atol=1e-12)
+    assert np.allclose(t, np.arange(n_last) * cfg.dt, atol=1e-15)
