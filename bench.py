@@ -38,6 +38,9 @@ def parse():
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="after the timed steps, copy a device buffer of this size 3 times "
+                         "(calibration of rocprofv3 FETCH_SIZE/WRITE_SIZE against known bytes)")
     return ap.parse_args()
 
 
@@ -131,6 +134,14 @@ def main():
         "last_force_row": [round(x, 8) for x in r.last_row[:10]],
     }
     s.close()
+    if args.copy_mb > 0:
+        n = args.copy_mb * (1 << 20) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        out["copy_calibration_bytes"] = 2 * n * 8
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.config, args.seed, args.cpu_seconds)
     print(json.dumps(out), flush=True)

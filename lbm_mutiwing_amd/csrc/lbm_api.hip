@@ -102,7 +102,9 @@ struct lbm_handle {
   IbmParams ip;
   IbmBuffers B;
   RingBuffers R;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;      // lattice kernels, copies
+  hipStream_t stream_ibm = nullptr;  // IBM chain, overlapped with the lattice kernel
+  hipEvent_t ev_ibm = nullptr, ev_step = nullptr;
   int device = 0;
   double feq[9];
   double w[9];
@@ -120,6 +122,7 @@ struct lbm_handle {
   int* d_int = nullptr;
   double* d_adj = nullptr;
   double* d_band = nullptr;
+  double* d_bcr = nullptr;
   double* d_table = nullptr;
   size_t table_cap = 0;
   double* d_stream = nullptr;
@@ -133,6 +136,7 @@ struct lbm_handle {
   int n = 1;
   int movenum = 0;
   int flag2 = 1;
+  int last_move_step = -1;           // step after which move_mesh last ran
   double body_rel_pos[MAXW] = {};
   bool motion_ready = false;
   double motion_end_time = 0.0;
@@ -220,6 +224,7 @@ void free_all(lbm_handle* h) {
   dfree(h->d_int);
   dfree(h->d_adj);
   dfree(h->d_band);
+  dfree(h->d_bcr);
   dfree(h->d_table);
   dfree(h->d_stream);
   dfree(h->d_st);
@@ -228,6 +233,11 @@ void free_all(lbm_handle* h) {
   dfree(h->d_rowsum);
   dfree(h->B.owner);
   dfree(h->B.aidx);
+  dfree(h->B.kmin);
+  dfree(h->B.kmax);
+  if (h->ev_ibm) (void)hipEventDestroy(h->ev_ibm);
+  if (h->ev_step) (void)hipEventDestroy(h->ev_step);
+  if (h->stream_ibm) (void)hipStreamDestroy(h->stream_ibm);
   for (auto e : h->ev) (void)hipEventDestroy(e);
   h->ev.clear();
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -334,9 +344,13 @@ int lbm_create(const lbm_config* cfg, lbm_handle** out) {
   update_body_extra(h);
 
   int rc = 0;
-  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&h->stream_ibm, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&h->ev_ibm, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&h->ev_step, hipEventDisableTiming) != hipSuccess) {
+    free_all(h);
     delete h;
-    return fail(nullptr, LBM_E_DEVICE, "hipStreamCreate failed");
+    return fail(nullptr, LBM_E_DEVICE, "stream/event creation failed");
   }
 #define ALLOC(ptr, n)                   \
   if ((rc = dalloc(h, &(ptr), (n)))) { \
@@ -365,11 +379,14 @@ int lbm_create(const lbm_config* cfg, lbm_handle** out) {
   ALLOC(h->d_int, 4 * (size_t)nseg + 2 * ((size_t)nseg + 1) + 2 * (size_t)nseg + 3 * (size_t)cap);
   ALLOC(h->d_adj, 10 * (size_t)cap);
   ALLOC(h->d_band, (size_t)nseg * (BMAX + 1) + 2 * (size_t)nseg);
+  ALLOC(h->d_bcr, ibm_bcr_ws_doubles(std::max(h->W, 1)));
   ALLOC(h->d_stream, 2 * LBM_MAX_MOTION_COLS);
   ALLOC(h->d_st, 1);
   ALLOC(h->d_rowsum, h->row_len);
   ALLOC(h->B.owner, np);
   ALLOC(h->B.aidx, np);
+  ALLOC(h->B.kmin, np);
+  ALLOC(h->B.kmax, np);
 #undef ALLOC
   // carve the IBM buffers
   IbmBuffers& B = h->B;
@@ -405,6 +422,7 @@ int lbm_create(const lbm_config* cfg, lbm_handle** out) {
   B.adj_gy = da; da += cap;
   B.adj_mom = da; da += cap;
   B.band = h->d_band;
+  B.bcr_ws = h->d_bcr;
   B.rhs = h->d_band + (size_t)nseg * (BMAX + 1);
   B.stream_rows = h->d_stream;
   B.step_table = nullptr;
@@ -471,9 +489,11 @@ int upload_static(lbm_handle* h, const lbm_config* cfg, double* xb0, double* yb0
   }
   // owner map starts empty everywhere
   {
-    std::vector<int> big(np, 0x7fffffff), neg(np, -1);
+    std::vector<int> big(np, 0x7fffffff), neg(np, -1), small(np, (int)0x80000000);
     HIPCHECK(h, hipMemcpy(h->B.owner, big.data(), sizeof(int) * np, hipMemcpyHostToDevice));
     HIPCHECK(h, hipMemcpy(h->B.aidx, neg.data(), sizeof(int) * np, hipMemcpyHostToDevice));
+    HIPCHECK(h, hipMemcpy(h->B.kmin, big.data(), sizeof(int) * np, hipMemcpyHostToDevice));
+    HIPCHECK(h, hipMemcpy(h->B.kmax, small.data(), sizeof(int) * np, hipMemcpyHostToDevice));
   }
   HIPCHECK(h, hipMemset(h->d_st, 0, sizeof(DevStep)));
   return 0;
@@ -512,6 +532,7 @@ int lbm_reset(lbm_handle* h) {
   h->n = 1;
   h->movenum = 0;
   h->flag2 = 1;
+  h->last_move_step = -1;
   h->ip.movenum = 0;
   HIPCHECK(h, hipStreamSynchronize(h->stream));
   return 0;
@@ -623,9 +644,9 @@ int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motio
     }
   HIPCHECK(h, hipMemsetAsync(h->d_rowsum, 0, sizeof(double) * h->row_len, h->stream));
   HIPCHECK(h, hipMemsetAsync(&h->d_st->error, 0, sizeof(int), h->stream));
+  HIPCHECK(h, hipStreamSynchronize(h->stream_ibm));
   // moves of the frame are fully determined by body_rel_pos(1), which only
   // this host changes (main.f90:393-411)
-  int last_move_step = -1;
   for (int m = n0; m <= n_last; m++) {
     const int s = m % 3;
     double brp = h->W > 0 ? h->body_rel_pos[0] : 0.0;
@@ -634,23 +655,26 @@ int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motio
     FieldSlot* fs = want_fields ? &h->slot[s] : nullptr;
     const double* src = h->fbuf[h->cur];
     double* dst = h->fbuf[1 - h->cur];
+    // IBM chain on the second stream, concurrent with the full-lattice kernel;
+    // it starts once the previous step has finished on the lattice stream.
+    HIPCHECK(h, hipEventRecord(h->ev_step, h->stream));
+    HIPCHECK(h, hipStreamWaitEvent(h->stream_ibm, h->ev_step, 0));
+    launch_motion_markers(h->stream_ibm, h->ip, h->B, m);
     if (h->W > 0) {
-      launch_motion_markers(h->stream, h->ip, h->B, m);
-      launch_adjoint_build(h->stream, h->ip, h->B);
-      launch_ibm_gather(h->stream, h->ip, h->B, h->g, src);
-      launch_ibm_assemble(h->stream, h->ip, h->B);
-      launch_ibm_solve(h->stream, h->ip, h->B);
-      launch_ibm_spread(h->stream, h->ip, h->B);
-    } else {
-      launch_motion_markers(h->stream, h->ip, h->B, m);
+      launch_adjoint_build(h->stream_ibm, h->ip, h->B);
+      launch_ibm_gather(h->stream_ibm, h->ip, h->B, h->g, src);
+      launch_ibm_assemble(h->stream_ibm, h->ip, h->B);
+      launch_ibm_wing(h->stream_ibm, h->ip, h->B);
     }
-    launch_forces(h->stream, h->ip, h->B, m - n0, h->row_len);
+    launch_forces(h->stream_ibm, h->ip, h->B, m - n0, h->row_len);
+    HIPCHECK(h, hipEventRecord(h->ev_ibm, h->stream_ibm));
     const int step_i = m - n0;
     if (h->timing) HIPCHECK(h, hipEventRecord(h->ev[2 * step_i], h->stream));
     launch_lattice(h->stream, h->g, h->mrt, src, dst, fs);
     if (h->timing) HIPCHECK(h, hipEventRecord(h->ev[2 * step_i + 1], h->stream));
-    if (h->W > 0) launch_adj_collide(h->stream, h->g, h->mrt, h->B, h->ip.cap, src, dst, fs);
     launch_bc_ring(h->stream, h->g, h->mrt, h->R, src, dst, fs);
+    HIPCHECK(h, hipStreamWaitEvent(h->stream, h->ev_ibm, 0));
+    if (h->W > 0) launch_adj_collide(h->stream, h->g, h->mrt, h->B, h->ip.cap, src, dst, fs);
     if (want_fields) h->slot_step[s] = m;
     h->cur = 1 - h->cur;
     if (dr != 0) {  // move_mesh (move_mesh.f90:17-58, 171-214)
@@ -671,7 +695,7 @@ int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motio
       h->movenum += dr;
       h->ip.movenum = h->movenum;
       h->flag2 = 3;
-      last_move_step = m;
+      h->last_move_step = m;
     }
     if (h->flag2 > 0) h->flag2 = h->flag2 - 1;
   }
@@ -681,7 +705,7 @@ int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motio
   const int ml = n_last;
   bool have_err = false;
   if (nsteps > 0 && ml >= 2) {
-    if (last_move_step == ml - 1) {
+    if (h->last_move_step == ml - 1) {
       have_err = true;  // move_mesh copies u into u0: the interior difference is zero
     } else {
       const int s1 = (ml - 1) % 3, s2 = (ml - 2) % 3;
@@ -707,7 +731,7 @@ int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motio
       h->lattice_ms += ms;
       h->lattice_launches++;
     }
-  if (have_err && last_move_step != ml - 1) HIPCHECK(h, hipMemcpy(&errv, h->err_out, sizeof(double), hipMemcpyDeviceToHost));
+  if (have_err && h->last_move_step != ml - 1) HIPCHECK(h, hipMemcpy(&errv, h->err_out, sizeof(double), hipMemcpyDeviceToHost));
   if (err) *err = errv;
   if (nsteps > 0) {
     if (force_rows)

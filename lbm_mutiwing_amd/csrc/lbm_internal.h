@@ -10,7 +10,7 @@
 namespace lbm {
 
 constexpr int MAXW = LBM_MAX_WINGS;
-constexpr int BMAX = 24;          // largest IBM band half-width the LDS solver accepts
+constexpr int BMAX = 24;          // largest IBM band half-width the solvers accept
 constexpr int BOXW = 6;           // nodes per axis of a delta support box (|r| <= 2.5)
 
 // Error codes raised by kernels into DevStep::error (first one wins).
@@ -40,6 +40,7 @@ struct DevStep {
   int pad;
   double force[MAXW][2];   // normalised per-wing force of the last step
   double moment[MAXW];
+  double raw[MAXW][3];     // un-normalised Fx, Fy, M of the last step
 };
 
 // Host-known constants of the IBM / motion kernels.
@@ -68,6 +69,8 @@ struct IbmBuffers {
   int* seg_cnt; int* seg_off;             // adjoint nodes claimed per segment / offsets (nseg+1)
   int* owner;                             // lattice map: claiming segment (nx*ny, INT_MAX = none)
   int* aidx;                              // lattice map: adjoint index (nx*ny, -1 = none)
+  int* kmin; int* kmax;                   // lattice maps: lowest / highest band key of the
+                                          // segments whose support holds the node
   int* adj_ij;                            // (i, j) 0-based per adjoint node (cap*2)
   int* adj_seg;                           // claiming segment per adjoint node (cap)
   double* adj_rho; double* adj_us; double* adj_vs;   // moments at adjoint nodes (cap)
@@ -78,6 +81,7 @@ struct IbmBuffers {
   const int* seg_of_pos;                  // zig-zag order: segment at band position (nseg)
   const int* pos_of_seg;                  // band position of each segment (nseg)
   double* band;                           // band matrix, W * SPW * (BMAX+1)
+  double* bcr_ws;                         // block cyclic reduction workspace per wing
   double* rhs;                            // 2 * nseg right-hand side, solution in place
   const double* step_table;               // file mode: step_count * ncols
   const double* stream_rows;              // stream mode: start row, target row (2*LBM_MAX_MOTION_COLS)
@@ -122,9 +126,9 @@ void launch_clear_adjoint(hipStream_t s, const IbmParams& p, const IbmBuffers& B
 void launch_ibm_gather(hipStream_t s, const IbmParams& p, const IbmBuffers& B, const Grid& g,
                        const double* src);
 void launch_ibm_assemble(hipStream_t s, const IbmParams& p, const IbmBuffers& B);
-void launch_ibm_solve(hipStream_t s, const IbmParams& p, const IbmBuffers& B);
-void launch_ibm_spread(hipStream_t s, const IbmParams& p, const IbmBuffers& B);
+void launch_ibm_wing(hipStream_t s, const IbmParams& p, const IbmBuffers& B);
 void launch_forces(hipStream_t s, const IbmParams& p, const IbmBuffers& B, int slot, int row_len);
 size_t ibm_solve_lds_bytes(int spw);
+size_t ibm_bcr_ws_doubles(int W);
 
 }  // namespace lbm

@@ -13,21 +13,18 @@ namespace lbm {
 
 constexpr int LAT_BLOCK = 256;
 
-// Workgroup b is dispatched to XCD (b mod 8); give each XCD a contiguous run
-// of row tiles so that the halo lines shared by neighbouring tiles stay in
-// one L2.
-__device__ __forceinline__ int xcd_tile(int b, int per) { return (b & 7) * per + (b >> 3); }
-
+// One thread per node of a row tile; tiles start at i = 0 so that every
+// wavefront stores whole, aligned 512-byte spans of each population plane
+// (the ring columns i = 0 and i = nx-1 are skipped and left to k_bc_ring).
+// Workgroups follow the hardware's round-robin placement over the XCDs,
+// which measured faster than remapping tiles to XCD-contiguous runs.
 template <bool FIELDS>
 __global__ __launch_bounds__(LAT_BLOCK) void k_lattice(Grid g, MrtParams P, const double* __restrict__ src,
-                                                       double* __restrict__ dst, FieldSlot fs, int tiles_x,
-                                                       int ntiles, int per) {
-  const int t = xcd_tile(blockIdx.x, per);
-  if (t >= ntiles) return;
-  const int ty = t / tiles_x, tx = t - ty * tiles_x;
-  const int i = 1 + tx * LAT_BLOCK + (int)threadIdx.x;
+                                                       double* __restrict__ dst, FieldSlot fs, int tiles_x) {
+  const int ty = blockIdx.x / tiles_x, tx = blockIdx.x - ty * tiles_x;
+  const int i = tx * LAT_BLOCK + (int)threadIdx.x;
   const int j = 1 + ty;
-  if (i > g.nx - 2) return;
+  if (i < 1 || i > g.nx - 2) return;
   double f[9];
   pull_node(src, g, i, j, f);
   double rho, u, v;
@@ -36,7 +33,7 @@ __global__ __launch_bounds__(LAT_BLOCK) void k_lattice(Grid g, MrtParams P, cons
   collide_node(P, f, rho, u, v, 0.0, 0.0, out);
   const long long o = (long long)j * g.nx + i;
 #pragma unroll
-  for (int k = 0; k < 9; k++) __builtin_nontemporal_store(out[k], dst + (long long)k * g.ps + o);
+  for (int k = 0; k < 9; k++) dst[(long long)k * g.ps + o] = out[k];
   if (FIELDS) {
     fs.rho[o] = rho;
     fs.u[o] = u;
@@ -46,15 +43,12 @@ __global__ __launch_bounds__(LAT_BLOCK) void k_lattice(Grid g, MrtParams P, cons
 
 void launch_lattice(hipStream_t s, const Grid& g, const MrtParams& P, const double* src, double* dst,
                     FieldSlot* fields) {
-  const int tiles_x = (g.nx - 2 + LAT_BLOCK - 1) / LAT_BLOCK;
-  const int ntiles = tiles_x * (g.ny - 2);
-  const int per = (ntiles + 7) / 8;
-  dim3 grid(8 * per), block(LAT_BLOCK);
+  const int tiles_x = (g.nx + LAT_BLOCK - 1) / LAT_BLOCK;
+  dim3 grid(tiles_x * (g.ny - 2)), block(LAT_BLOCK);
   if (fields)
-    hipLaunchKernelGGL(k_lattice<true>, grid, block, 0, s, g, P, src, dst, *fields, tiles_x, ntiles, per);
+    hipLaunchKernelGGL(k_lattice<true>, grid, block, 0, s, g, P, src, dst, *fields, tiles_x);
   else
-    hipLaunchKernelGGL(k_lattice<false>, grid, block, 0, s, g, P, src, dst, FieldSlot{}, tiles_x, ntiles,
-                       per);
+    hipLaunchKernelGGL(k_lattice<false>, grid, block, 0, s, g, P, src, dst, FieldSlot{}, tiles_x);
 }
 
 // Re-collide the adjoint nodes with the IBM-corrected velocity and the body
