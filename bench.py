@@ -26,7 +26,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_NODE_FP64 = 144.0   # 9 population reads + 9 writes of 8 B
+BYTES_PER_NODE = {"fp64": 144.0,   # 9 population reads + 9 writes of 8 B
+                  "fp32": 72.0}    # the same in FP32 deviations
 
 
 def parse():
@@ -36,6 +37,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--config", default="c2", choices=["c1", "c2", "c5", "small"])
     ap.add_argument("--seed", type=int, default=2026)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"],
+                    help="population storage: FP64, or FP32 deviations from the weights")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--copy-mb", type=int, default=0,
@@ -44,9 +47,9 @@ def parse():
     return ap.parse_args()
 
 
-def workload(name, seed):
-    from lbm_mutiwing_amd import preset, seeded_motions, sinusoidal_motion_table
-    cfg = preset(name)
+def workload(name, seed, precision="fp64"):
+    from lbm_mutiwing_amd import PRECISION_FP32_SHIFTED, PRECISION_FP64, preset, seeded_motions
+    cfg = preset(name, precision=PRECISION_FP32_SHIFTED if precision == "fp32" else PRECISION_FP64)
     if name == "c2":
         mot = seeded_motions(cfg, seed, pitch_deg=(5.0, 15.0), heave=(0.1, 0.3))
     else:
@@ -80,13 +83,14 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                          "(episodes shard by instance, see DESIGN.md)"}))
+                          "(episodes are independent and shard one instance per GPU)"}))
         sys.exit(2)
     import torch  # noqa: F401  (device visibility check only)
     from lbm_mutiwing_amd import sinusoidal_motion_table
     from lbm_mutiwing_amd.solver import TimedSolver
 
-    cfg, mot = workload(args.config, args.seed)
+    cfg, mot = workload(args.config, args.seed, args.precision)
+    bpn = BYTES_PER_NODE[args.precision]
     n_total = args.warmup + args.steps + 1
     s = TimedSolver(cfg)
     s.set_motion_table(sinusoidal_motion_table(cfg, mot, n_total + 1))
@@ -102,14 +106,14 @@ def main():
     ms_step = dt * 1e3 / args.steps
     lat_avg_ms = lat_ms / max(launches, 1)
     interior = (cfg.nx - 2) * (cfg.ny - 2)
-    achieved = BYTES_PER_NODE_FP64 * interior / (lat_avg_ms * 1e-3) / 1e9
+    achieved = bpn * interior / (lat_avg_ms * 1e-3) / 1e9
     traffic = None
-    tp = os.path.join(ROOT, "profiles", f"traffic_{args.config}_fp64.json")
+    tp = os.path.join(ROOT, "profiles", f"traffic_{args.config}_{args.precision}.json")
     if os.path.exists(tp):
         with open(tp) as fh:
             traffic = json.load(fh).get("hbm_bytes_per_launch")
     out = {
-        "metric": "MLUPS (whole time step incl. IBM), FP64 single instance",
+        "metric": f"MLUPS (whole time step incl. IBM), {args.precision.upper()} single instance",
         "value": round(mlups, 1),
         "unit": "MLUPS",
         "n_gpus": 1,
@@ -119,18 +123,18 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == "fp64" else "f32",
         "data": "synthetic (seeded sinusoidal pitch/heave motion table, generated NACA0012 markers)",
         "config": {"workload": f"{args.config}: {cfg.nx}x{cfg.ny} D2Q9 MRT + IBM, chord {1 / cfg.scale:.0f} nodes, "
                                f"Re {cfg.re:g}, {cfg.wing_count} wings x {cfg.points_per_wing} markers",
                    "nx": cfg.nx, "ny": cfg.ny, "wings": cfg.wing_count, "seed": args.seed,
                    "adjoint_nodes": st["adjoint_n"]},
-        "hbm_roofline_frac_whole_step": round(mlups * 1e6 * BYTES_PER_NODE_FP64 / (HBM_PEAK_GBS * 1e9), 4),
+        "hbm_roofline_frac_whole_step": round(mlups * 1e6 * bpn / (HBM_PEAK_GBS * 1e9), 4),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_lattice (fused pull-stream + moments + MRT collide)",
                      "kernel_avg_us": round(lat_avg_ms * 1e3, 2),
-                     "algorithmic_bytes_per_launch": BYTES_PER_NODE_FP64 * interior},
+                     "algorithmic_bytes_per_launch": bpn * interior},
         "last_force_row": [round(x, 8) for x in r.last_row[:10]],
     }
     s.close()

@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -100,16 +101,22 @@ struct lbm_handle {
   Grid g;
   MrtParams mrt;
   IbmParams ip;
-  IbmBuffers B;
+  GeoBuffers G;
+  FlowBuffers F;
   RingBuffers R;
   hipStream_t stream = nullptr;      // lattice kernels, copies
-  hipStream_t stream_ibm = nullptr;  // IBM chain, overlapped with the lattice kernel
+  hipStream_t stream_ibm = nullptr;  // flow part of the IBM, overlapped with the lattice kernel
+  hipStream_t stream_geo = nullptr;  // IBM geometry of the coming steps
   hipEvent_t ev_ibm = nullptr, ev_step = nullptr;
+  hipEvent_t ev_geo[2] = {nullptr, nullptr};  // geometry of a slot set is ready
+  hipEvent_t ev_rel[2] = {nullptr, nullptr};  // last step using a slot set has finished
   int device = 0;
   double feq[9];
   double w[9];
   // populations
-  double* fbuf[2] = {nullptr, nullptr};
+  void* fbuf[2] = {nullptr, nullptr};   // double, or float deviations when f32
+  bool f32 = false;
+  size_t fsize = sizeof(double);        // bytes per stored population value
   int cur = 0;
   // field slots (rho,u,v of the last steps), tagged with their step number
   FieldSlot slot[3];
@@ -118,11 +125,9 @@ struct lbm_handle {
   double* err_partial = nullptr;
   double* err_out = nullptr;
   // IBM / motion device memory
-  double* d_marker = nullptr;  // xb0,yb0,xb1,yb1 (4*npts), xb2,yb2,arc,ub,vb (5*nseg)
-  int* d_int = nullptr;
-  double* d_adj = nullptr;
-  double* d_band = nullptr;
-  double* d_bcr = nullptr;
+  std::vector<void*> ibm_allocs;     // geometry slots, flow buffers, markers
+  Frame* d_frame = nullptr;
+  size_t frame_cap = 0;
   double* d_table = nullptr;
   size_t table_cap = 0;
   double* d_stream = nullptr;
@@ -131,6 +136,9 @@ struct lbm_handle {
   size_t rows_cap = 0;
   double* d_mrows = nullptr;
   double* d_rowsum = nullptr;
+  int nslot = 1;                     // steps per geometry batch (slots per set)
+  unsigned batch_count = 0;          // geometry batches issued (selects the slot set)
+  int last_slot = -1;                // slot of the last step run (-1: none since reset)
   // host state
   int W, P, SPW, nseg, npts, row_len;
   int n = 1;
@@ -138,12 +146,13 @@ struct lbm_handle {
   int flag2 = 1;
   int last_move_step = -1;           // step after which move_mesh last ran
   double body_rel_pos[MAXW] = {};
+  Frame fr{};                        // movenum and body_extra_* of the next step
   bool motion_ready = false;
   double motion_end_time = 0.0;
-  std::vector<double> h_table;   // raw table (host)
+  std::vector<Frame> frames;         // per step of the current call
   // lattice-kernel timing (lbm_set_timing)
   bool timing = false;
-  std::vector<hipEvent_t> ev;     // start/stop pairs of the current call
+  std::vector<hipEvent_t> ev;        // start/stop pairs of the current call
   double lattice_ms = 0.0;
   int lattice_launches = 0;
   char msg[512] = "";
@@ -171,7 +180,6 @@ int fail(lbm_handle* h, int code, const char* fmt, ...) {
 
 template <typename T>
 int dalloc(lbm_handle* h, T** p, size_t count) {
-  *p = nullptr;
   if (count == 0) count = 1;
   hipError_t e = hipMalloc((void**)p, count * sizeof(T));
   if (e != hipSuccess)
@@ -179,25 +187,34 @@ int dalloc(lbm_handle* h, T** p, size_t count) {
   return 0;
 }
 
+// IBM allocation owned by h->ibm_allocs.
+template <typename T>
+int ialloc(lbm_handle* h, T** p, size_t count) {
+  int rc = dalloc(h, p, count);
+  if (rc == 0) h->ibm_allocs.push_back((void*)*p);
+  return rc;
+}
+
 void update_body_extra(lbm_handle* h) {  // update_body_extra_from_rel (mode.f90:473-495)
   double ap = 0.0;
+  h->fr.movenum = h->movenum;
   for (int w = 0; w < MAXW; w++) {
-    h->ip.body_extra_v[w] = 0.0;
-    h->ip.body_extra_s[w] = 0.0;
-    h->ip.body_extra_pos[w] = 0.0;
+    h->fr.body_extra_v[w] = 0.0;
+    h->fr.body_extra_s[w] = 0.0;
+    h->fr.body_extra_pos[w] = 0.0;
   }
   for (int w = 0; w < h->W; w++) {
     ap = ap + h->body_rel_pos[w];
-    h->ip.body_extra_pos[w] = ap;
+    h->fr.body_extra_pos[w] = ap;
   }
 }
 
 const char* dev_error_text(int code) {
   switch (code) {
     case DERR_BOX_EDGE: return "IBM support box of a segment reaches within 3 nodes of the lattice edge";
-    case DERR_ADJ_OVERFLOW: return "adjoint node list overflow";
+    case DERR_MAP: return "support rectangle of a wing exceeds its claim map";
     case DERR_BAND: return "IBM band half-width exceeds the solver limit";
-    case DERR_WINGS_COUPLED: return "support boxes of two wings overlap";
+    case DERR_WINGS_COUPLED: return "support boxes of two wings share a lattice node";
     case DERR_NOT_SPD: return "IBM matrix lost positive definiteness";
     case DERR_MOTION: return "motion source not initialised (call lbm_set_motion_table)";
     default: return "unknown device error";
@@ -220,24 +237,23 @@ void free_all(lbm_handle* h) {
   dfree(h->R.left);
   dfree(h->R.right);
   dfree(h->R.mom);
-  dfree(h->d_marker);
-  dfree(h->d_int);
-  dfree(h->d_adj);
-  dfree(h->d_band);
-  dfree(h->d_bcr);
+  for (void* p : h->ibm_allocs) dfree(p);
+  h->ibm_allocs.clear();
+  dfree(h->d_frame);
   dfree(h->d_table);
   dfree(h->d_stream);
   dfree(h->d_st);
   dfree(h->d_rows);
   dfree(h->d_mrows);
   dfree(h->d_rowsum);
-  dfree(h->B.owner);
-  dfree(h->B.aidx);
-  dfree(h->B.kmin);
-  dfree(h->B.kmax);
   if (h->ev_ibm) (void)hipEventDestroy(h->ev_ibm);
   if (h->ev_step) (void)hipEventDestroy(h->ev_step);
+  for (int s = 0; s < 2; s++) {
+    if (h->ev_geo[s]) (void)hipEventDestroy(h->ev_geo[s]);
+    if (h->ev_rel[s]) (void)hipEventDestroy(h->ev_rel[s]);
+  }
   if (h->stream_ibm) (void)hipStreamDestroy(h->stream_ibm);
+  if (h->stream_geo) (void)hipStreamDestroy(h->stream_geo);
   for (auto e : h->ev) (void)hipEventDestroy(e);
   h->ev.clear();
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -254,19 +270,76 @@ int ensure_rows(lbm_handle* h, size_t steps) {
   rc = dalloc(h, &h->d_mrows, cap * (1 + 3 * h->W));
   if (rc) return rc;
   h->rows_cap = cap;
-  h->B.rows = h->d_rows;
-  h->B.mrows = h->d_mrows;
+  h->F.rows = h->d_rows;
+  h->F.mrows = h->d_mrows;
   return 0;
 }
 
-int upload_static(lbm_handle* h, const lbm_config* cfg, double* xb0, double* yb0, double* arc, int* seg_of_pos,
-                  int* pos_of_seg);
+int ensure_frames(lbm_handle* h, size_t steps) {
+  if (steps <= h->frame_cap) return 0;
+  dfree(h->d_frame);
+  h->d_frame = nullptr;
+  size_t cap = std::max<size_t>(steps, 1024);
+  int rc = dalloc(h, &h->d_frame, cap);
+  if (rc) return rc;
+  h->frame_cap = cap;
+  h->G.frame = h->d_frame;
+  return 0;
+}
+
+// Per-slot geometry arrays and flow buffers of the IBM.  The number of steps
+// per geometry batch follows from the bytes per step (dominated by the
+// inverse matrices) with a budget of 256 MB per slot set.
+int alloc_ibm(lbm_handle* h, const std::vector<double>& hx, const std::vector<double>& hy) {
+  IbmParams& p = h->ip;
+  GeoBuffers& G = h->G;
+  FlowBuffers& F = h->F;
+  const size_t npts = std::max(h->npts, 1), nseg = std::max(h->nseg, 1), cap = std::max(p.cap, 1);
+  const size_t W = std::max(h->W, 1), spw = std::max(h->SPW, 1);
+  // claim map: square of the largest marker radius about the rotation origin
+  double r2 = 0.0;
+  for (int q = 0; q < h->npts; q++) r2 = std::max(r2, hx[q] * hx[q] + hy[q] * hy[q]);
+  const int side = (int)std::ceil(2.0 * std::sqrt(r2) / h->cfg.scale) + 12;
+  p.map_cap = h->W > 0 ? side * side : 1;
+  const size_t per_slot = sizeof(GeoStep) + 2 * npts * 8 + 4 * nseg * 8 + 4 * nseg * 4 + 2 * BOXW * nseg * 8 +
+                          W * (spw + 1) * 4 + 3 * cap * 4 + W * spw * spw * 8 + W * 2 * (size_t)p.map_cap * 4;
+  h->nslot = h->W > 0 ? (int)std::max<size_t>(1, std::min<size_t>(16, (256u << 20) / per_slot)) : 1;
+  const size_t ns = 2 * (size_t)h->nslot;
+  int rc = 0;
+  double* m0;
+  if ((rc = ialloc(h, &m0, 2 * npts))) return rc;
+  G.xb0 = m0;
+  G.yb0 = m0 + npts;
+  if ((rc = ialloc(h, &G.step, ns)) || (rc = ialloc(h, &G.xb1, ns * npts)) || (rc = ialloc(h, &G.yb1, ns * npts)) ||
+      (rc = ialloc(h, &G.xb2, ns * nseg)) || (rc = ialloc(h, &G.yb2, ns * nseg)) ||
+      (rc = ialloc(h, &G.ub, ns * nseg)) || (rc = ialloc(h, &G.vb, ns * nseg)) ||
+      (rc = ialloc(h, &G.box, ns * nseg * 4)) || (rc = ialloc(h, &G.phx, ns * nseg * BOXW)) ||
+      (rc = ialloc(h, &G.phy, ns * nseg * BOXW)) || (rc = ialloc(h, &G.seg_off, ns * W * (spw + 1))) ||
+      (rc = ialloc(h, &G.adj_ij, ns * cap * 2)) || (rc = ialloc(h, &G.adj_seg, ns * cap)) ||
+      (rc = ialloc(h, &G.sinv, h->W > 0 ? ns * W * spw * spw : 1)) ||
+      (rc = ialloc(h, &G.map, h->W > 0 ? ns * W * 2 * (size_t)p.map_cap : 1)))
+    return rc;
+  const int bpw = adj_collide_blocks_per_wing(p);
+  if ((rc = ialloc(h, &F.cvec, 2 * nseg)) || (rc = ialloc(h, &F.yvec, 2 * nseg)) ||
+      (rc = ialloc(h, &F.adj_g, 2 * cap)) || (rc = ialloc(h, &F.partial, 3 * W * (size_t)bpw)))
+    return rc;
+  G.step_table = nullptr;
+  G.stream_rows = h->d_stream;
+  G.frame = nullptr;
+  F.row_sum = h->d_rowsum;
+  HIPCHECK(h, hipMemset(G.step, 0, ns * sizeof(GeoStep)));
+  if (h->npts > 0) {
+    HIPCHECK(h, hipMemcpy(m0, hx.data(), sizeof(double) * h->npts, hipMemcpyHostToDevice));
+    HIPCHECK(h, hipMemcpy(m0 + npts, hy.data(), sizeof(double) * h->npts, hipMemcpyHostToDevice));
+  }
+  return 0;
+}
 
 }  // namespace
 
 extern "C" {
 
-const char* lbm_version(void) { return "lbm_mutiwing_amd gfx950 FP64 step 0.1"; }
+const char* lbm_version(void) { return "lbm_mutiwing_amd gfx950 step 0.3 (FP64, FP32-shifted)"; }
 
 int lbm_last_error(const lbm_handle* h, char* buf, size_t n) {
   if (!buf || n == 0) return LBM_E_INVALID;
@@ -285,11 +358,13 @@ int lbm_create(const lbm_config* cfg, lbm_handle** out) {
     return fail(nullptr, LBM_E_INVALID, "scale, uu and re must be positive");
   if (cfg->motion_source != LBM_MOTION_FILE && cfg->motion_source != LBM_MOTION_STREAM)
     return fail(nullptr, LBM_E_INVALID, "motion_source must be 1 (file) or 2 (stream)");
-  if (cfg->precision != LBM_PRECISION_FP64)
-    return fail(nullptr, LBM_E_INVALID, "only LBM_PRECISION_FP64 is implemented by this build");
+  if (cfg->precision != LBM_PRECISION_FP64 && cfg->precision != LBM_PRECISION_FP32_SHIFTED)
+    return fail(nullptr, LBM_E_INVALID, "precision must be LBM_PRECISION_FP64 or LBM_PRECISION_FP32_SHIFTED");
   lbm_handle* h = new lbm_handle();
   h->cfg = *cfg;
   h->cfg.xb0 = h->cfg.yb0 = nullptr;
+  h->f32 = cfg->precision == LBM_PRECISION_FP32_SHIFTED;
+  h->fsize = h->f32 ? sizeof(float) : sizeof(double);
   h->W = cfg->wing_count;
   h->P = h->W > 0 ? cfg->points_per_wing : 0;
   h->SPW = h->P > 0 ? h->P - 1 : 0;
@@ -297,14 +372,6 @@ int lbm_create(const lbm_config* cfg, lbm_handle** out) {
   h->npts = h->W * h->P;
   h->row_len = 4 + 12 * h->W;
   (void)hipGetDevice(&h->device);
-  int lds_max = 0;
-  (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device);
-  if (h->W > 0 && ibm_solve_lds_bytes(h->SPW) > (size_t)lds_max) {
-    int rc = fail(nullptr, LBM_E_INVALID, "%d segments per wing need %zu B of LDS, device allows %d", h->SPW,
-                  ibm_solve_lds_bytes(h->SPW), lds_max);
-    delete h;
-    return rc;
-  }
   // model constants (D2Q9model.f90, ini_mesh.f90:30-31, 59-67, boundary_condition.f90:304)
   const double niu = cfg->uu * (1.0 / cfg->scale) / cfg->re;
   const double tau = 3.0 * niu + 0.5;
@@ -325,6 +392,22 @@ int lbm_create(const lbm_config* cfg, lbm_handle** out) {
     const double utwo = u1 * u1 + v1 * v1;
     h->feq[k] = h->w[k] * 1.0 * (1.0 + 3.0 * eu + 4.5 * eu * eu - 1.5 * utwo);
   }
+  // constants of the FP32-shifted storage: weights, rounded rates and inverse,
+  // and the equilibrium moments of the rest state M w
+  for (int k = 0; k < 9; k++) {
+    h->mrt.w[k] = h->w[k];
+    h->mrt.s32[k] = (float)h->mrt.s[k];
+  }
+  for (int q = 0; q < 81; q++) h->mrt.minv32[q] = (float)h->mrt.minv[q];
+  {
+    double mw[9];
+    for (int r = 0; r < 9; r++) {
+      mw[r] = 0.0;
+      for (int c = 0; c < 9; c++) mw[r] += kMatM[r][c] * h->w[c];
+    }
+    h->mrt.eq1 = (float)(-2.0 - mw[1]);
+    h->mrt.eq2 = (float)(1.0 - mw[2]);
+  }
   h->g.nx = cfg->nx;
   h->g.ny = cfg->ny;
   const long long np = (long long)cfg->nx * cfg->ny;
@@ -332,22 +415,36 @@ int lbm_create(const lbm_config* cfg, lbm_handle** out) {
   // IBM params
   IbmParams& ip = h->ip;
   ip.W = h->W; ip.P = h->P; ip.SPW = h->SPW; ip.nseg = h->nseg; ip.npts = h->npts;
-  ip.cap = BOXW * BOXW * h->nseg;
+  ip.capw = BOXC * h->SPW;
+  ip.cap = h->W * ip.capw;
   ip.nx = cfg->nx; ip.ny = cfg->ny;
   ip.scale = cfg->scale; ip.ox = cfg->ox; ip.oy = cfg->oy; ip.uu = cfg->uu;
   ip.dt = cfg->scale * cfg->uu;
   ip.wing_x0 = cfg->wing_x0; ip.wing_dx = cfg->wing_dx; ip.wing_y0 = cfg->wing_y0;
   ip.xc = cfg->xc; ip.yc = cfg->yc;
   ip.motion_source = cfg->motion_source;
-  ip.movenum = 0; ip.step_count = 0; ip.ncols = 1 + LBM_MOTION_COLS_PER_WING * h->W;
+  ip.step_count = 0; ip.ncols = 1 + LBM_MOTION_COLS_PER_WING * h->W;
   ip.motion_dt = ip.dt;
+  int lds_max = 0;
+  (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device);
+  if (h->W > 0 && geo_solve_lds_bytes(ip) > (size_t)lds_max) {
+    int rc = fail(nullptr, LBM_E_INVALID, "%d segments per wing need %zu B of LDS, device allows %d", h->SPW,
+                  geo_solve_lds_bytes(ip), lds_max);
+    delete h;
+    return rc;
+  }
   update_body_extra(h);
 
   int rc = 0;
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&h->stream_ibm, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&h->stream_geo, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&h->ev_ibm, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&h->ev_step, hipEventDisableTiming) != hipSuccess) {
+      hipEventCreateWithFlags(&h->ev_step, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&h->ev_geo[0], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&h->ev_geo[1], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&h->ev_rel[0], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&h->ev_rel[1], hipEventDisableTiming) != hipSuccess) {
     free_all(h);
     delete h;
     return fail(nullptr, LBM_E_DEVICE, "stream/event creation failed");
@@ -359,8 +456,19 @@ int lbm_create(const lbm_config* cfg, lbm_handle** out) {
     delete h;                           \
     return rc;                          \
   }
-  ALLOC(h->fbuf[0], 9 * np);
-  ALLOC(h->fbuf[1], 9 * np);
+  if (h->f32) {
+    float *a = nullptr, *b = nullptr;
+    ALLOC(a, 9 * np);
+    h->fbuf[0] = a;
+    ALLOC(b, 9 * np);
+    h->fbuf[1] = b;
+  } else {
+    double *a = nullptr, *b = nullptr;
+    ALLOC(a, 9 * np);
+    h->fbuf[0] = a;
+    ALLOC(b, 9 * np);
+    h->fbuf[1] = b;
+  }
   for (auto& s : h->slot) {
     ALLOC(s.rho, np);
     ALLOC(s.u, np);
@@ -374,62 +482,18 @@ int lbm_create(const lbm_config* cfg, lbm_handle** out) {
   ALLOC(h->R.left, 9 * (size_t)cfg->ny);
   ALLOC(h->R.right, 9 * (size_t)cfg->ny);
   ALLOC(h->R.mom, 6 * (size_t)cfg->ny);
-  const int npts = std::max(h->npts, 1), nseg = std::max(h->nseg, 1), cap = std::max(ip.cap, 1);
-  ALLOC(h->d_marker, 4 * (size_t)npts + 5 * (size_t)nseg + 2 * (size_t)BOXW * nseg);
-  ALLOC(h->d_int, 4 * (size_t)nseg + 2 * ((size_t)nseg + 1) + 2 * (size_t)nseg + 3 * (size_t)cap);
-  ALLOC(h->d_adj, 10 * (size_t)cap);
-  ALLOC(h->d_band, (size_t)nseg * (BMAX + 1) + 2 * (size_t)nseg);
-  ALLOC(h->d_bcr, ibm_bcr_ws_doubles(std::max(h->W, 1)));
   ALLOC(h->d_stream, 2 * LBM_MAX_MOTION_COLS);
   ALLOC(h->d_st, 1);
   ALLOC(h->d_rowsum, h->row_len);
-  ALLOC(h->B.owner, np);
-  ALLOC(h->B.aidx, np);
-  ALLOC(h->B.kmin, np);
-  ALLOC(h->B.kmax, np);
 #undef ALLOC
-  // carve the IBM buffers
-  IbmBuffers& B = h->B;
-  double* dm = h->d_marker;
-  double* xb0 = dm; dm += npts;
-  double* yb0 = dm; dm += npts;
-  B.xb1 = dm; dm += npts;
-  B.yb1 = dm; dm += npts;
-  B.xb2 = dm; dm += nseg;
-  B.yb2 = dm; dm += nseg;
-  double* arc = dm; dm += nseg;
-  B.ub = dm; dm += nseg;
-  B.vb = dm; dm += nseg;
-  B.phx = dm; dm += BOXW * nseg;
-  B.phy = dm; dm += BOXW * nseg;
-  B.xb0 = xb0; B.yb0 = yb0; B.arc = arc;
-  int* di = h->d_int;
-  B.box = di; di += 4 * nseg;
-  B.seg_cnt = di; di += nseg + 1;
-  B.seg_off = di; di += nseg + 1;
-  int* seg_of_pos = di; di += nseg;
-  int* pos_of_seg = di; di += nseg;
-  B.adj_ij = di; di += 2 * cap;
-  B.adj_seg = di; di += cap;
-  B.seg_of_pos = seg_of_pos; B.pos_of_seg = pos_of_seg;
-  double* da = h->d_adj;
-  B.adj_rho = da; da += cap;
-  B.adj_us = da; da += cap;
-  B.adj_vs = da; da += cap;
-  B.adj_u = da; da += cap;
-  B.adj_v = da; da += cap;
-  B.adj_gx = da; da += cap;
-  B.adj_gy = da; da += cap;
-  B.adj_mom = da; da += cap;
-  B.band = h->d_band;
-  B.bcr_ws = h->d_bcr;
-  B.rhs = h->d_band + (size_t)nseg * (BMAX + 1);
-  B.stream_rows = h->d_stream;
-  B.step_table = nullptr;
-  B.st = h->d_st;
-  B.row_sum = h->d_rowsum;
-  B.prev_ij = nullptr; B.prev_n = nullptr;
-  rc = upload_static(h, cfg, xb0, yb0, arc, seg_of_pos, pos_of_seg);
+  // markers: read_boundary (airfoil.f90:229-241) plus the flow-centre offset
+  std::vector<double> hx(h->npts), hy(h->npts);
+  for (int q = 0; q < h->npts; q++) {
+    hx[q] = cfg->xb0[q] + (cfg->xc - cfg->xp);
+    hy[q] = cfg->yb0[q] + (cfg->yc - cfg->yp);
+  }
+  rc = alloc_ibm(h, hx, hy);
+  if (rc == 0) rc = ensure_frames(h, 1024);
   if (rc == 0) rc = lbm_reset(h);
   if (rc) {
     snprintf(g_create_error, 512, "%s", h->msg);
@@ -441,71 +505,10 @@ int lbm_create(const lbm_config* cfg, lbm_handle** out) {
   return 0;
 }
 
-}  // extern "C"
-
-namespace {
-
-int upload_static(lbm_handle* h, const lbm_config* cfg, double* xb0, double* yb0, double* arc, int* seg_of_pos,
-                  int* pos_of_seg) {
-  const long long np = (long long)cfg->nx * cfg->ny;
-  // markers: read_boundary (airfoil.f90:229-241) arc lengths and offset
-  if (h->W > 0) {
-    std::vector<double> hx(h->npts), hy(h->npts), ha(h->nseg);
-    for (int q = 0; q < h->npts; q++) {
-      hx[q] = cfg->xb0[q];
-      hy[q] = cfg->yb0[q];
-    }
-    for (int w = 0; w < h->W; w++)
-      for (int i0 = 0; i0 < h->SPW; i0++) {
-        const int a = w * h->P + i0;
-        const double dx = hx[a + 1] - hx[a], dy = hy[a + 1] - hy[a];
-        ha[w * h->SPW + i0] = std::sqrt(dx * dx + dy * dy) / cfg->scale;
-      }
-    for (int q = 0; q < h->npts; q++) {
-      hx[q] = hx[q] + (cfg->xc - cfg->xp);
-      hy[q] = hy[q] + (cfg->yc - cfg->yp);
-    }
-    // zig-zag band order per wing: positions 0,1,2,3,... hold segments 0, n-1, 1, n-2, ...
-    std::vector<int> sop(h->nseg), pos(h->nseg);
-    for (int w = 0; w < h->W; w++) {
-      const int n = h->SPW;
-      int p = 0;
-      for (int t = 0; p < n; t++) {
-        sop[w * n + p] = w * n + t;
-        pos[w * n + t] = p;
-        p++;
-        if (p < n && n - 1 - t != t) {
-          sop[w * n + p] = w * n + (n - 1 - t);
-          pos[w * n + (n - 1 - t)] = p;
-          p++;
-        }
-      }
-    }
-    HIPCHECK(h, hipMemcpy(xb0, hx.data(), sizeof(double) * h->npts, hipMemcpyHostToDevice));
-    HIPCHECK(h, hipMemcpy(yb0, hy.data(), sizeof(double) * h->npts, hipMemcpyHostToDevice));
-    HIPCHECK(h, hipMemcpy(arc, ha.data(), sizeof(double) * h->nseg, hipMemcpyHostToDevice));
-    HIPCHECK(h, hipMemcpy(seg_of_pos, sop.data(), sizeof(int) * h->nseg, hipMemcpyHostToDevice));
-    HIPCHECK(h, hipMemcpy(pos_of_seg, pos.data(), sizeof(int) * h->nseg, hipMemcpyHostToDevice));
-  }
-  // owner map starts empty everywhere
-  {
-    std::vector<int> big(np, 0x7fffffff), neg(np, -1), small(np, (int)0x80000000);
-    HIPCHECK(h, hipMemcpy(h->B.owner, big.data(), sizeof(int) * np, hipMemcpyHostToDevice));
-    HIPCHECK(h, hipMemcpy(h->B.aidx, neg.data(), sizeof(int) * np, hipMemcpyHostToDevice));
-    HIPCHECK(h, hipMemcpy(h->B.kmin, big.data(), sizeof(int) * np, hipMemcpyHostToDevice));
-    HIPCHECK(h, hipMemcpy(h->B.kmax, small.data(), sizeof(int) * np, hipMemcpyHostToDevice));
-  }
-  HIPCHECK(h, hipMemset(h->d_st, 0, sizeof(DevStep)));
-  return 0;
-}
-
-}  // namespace
-
-extern "C" {
-
 int lbm_destroy(lbm_handle* h) {
   if (!h) return 0;
   (void)hipStreamSynchronize(h->stream);
+  (void)hipStreamSynchronize(h->stream_geo);
   free_all(h);
   delete h;
   return 0;
@@ -513,13 +516,13 @@ int lbm_destroy(lbm_handle* h) {
 
 int lbm_reset(lbm_handle* h) {
   if (!h) return LBM_E_INVALID;
+  HIPCHECK(h, hipStreamSynchronize(h->stream_geo));
+  HIPCHECK(h, hipStreamSynchronize(h->stream_ibm));
   h->cur = 0;
-  launch_init_fields(h->stream, h->g, h->cfg.uu, h->feq, h->fbuf[0], h->R, h->slot[0]);
+  launch_init_fields(h->stream, h->g, h->mrt, h->cfg.uu, h->feq, h->fbuf[0], h->R, h->slot[0], h->f32);
   launch_ring_moments(h->stream, h->g, h->R);
   for (int s = 0; s < 3; s++) h->slot_step[s] = -1;
   h->slot_step[0] = 0;
-  // clear the owner maps at the previous episode's adjoint list, then the step state
-  if (h->W > 0) launch_clear_adjoint(h->stream, h->ip, h->B);
   HIPCHECK(h, hipStreamSynchronize(h->stream));
   HIPCHECK(h, hipMemset(h->d_st, 0, sizeof(DevStep)));
   if (h->cfg.motion_source == LBM_MOTION_STREAM) {  // driver.f90:120-124
@@ -528,13 +531,12 @@ int lbm_reset(lbm_handle* h) {
     h->motion_ready = true;
   }
   for (int w = 0; w < MAXW; w++) h->body_rel_pos[w] = 0.0;
-  update_body_extra(h);
   h->n = 1;
   h->movenum = 0;
   h->flag2 = 1;
   h->last_move_step = -1;
-  h->ip.movenum = 0;
-  HIPCHECK(h, hipStreamSynchronize(h->stream));
+  h->last_slot = -1;
+  update_body_extra(h);
   return 0;
 }
 
@@ -577,6 +579,7 @@ int lbm_set_motion_table(lbm_handle* h, const double* rows, int32_t nrows, int32
     }
     row[0] = tt;
   }
+  HIPCHECK(h, hipStreamSynchronize(h->stream_geo));
   if (tab.size() > h->table_cap) {
     dfree(h->d_table);
     h->d_table = nullptr;
@@ -585,7 +588,7 @@ int lbm_set_motion_table(lbm_handle* h, const double* rows, int32_t nrows, int32
     h->table_cap = tab.size();
   }
   HIPCHECK(h, hipMemcpy(h->d_table, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice));
-  h->B.step_table = h->d_table;
+  h->G.step_table = h->d_table;
   h->ip.step_count = (int)count;
   h->motion_end_time = end;
   h->motion_ready = true;
@@ -601,6 +604,7 @@ int lbm_set_stream_segment(lbm_handle* h, const double* start_row, const double*
     z[c] = start_row[c];
     z[LBM_MAX_MOTION_COLS + c] = target_row[c];
   }
+  HIPCHECK(h, hipStreamSynchronize(h->stream_geo));
   HIPCHECK(h, hipMemcpy(h->d_stream, z.data(), sizeof(double) * z.size(), hipMemcpyHostToDevice));
   h->motion_ready = true;
   return 0;
@@ -636,69 +640,98 @@ int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motio
   const int nsteps = n_last >= n0 ? n_last - n0 + 1 : 0;
   int rc = ensure_rows(h, (size_t)std::max(nsteps, 1));
   if (rc) return rc;
+  rc = ensure_frames(h, (size_t)std::max(nsteps, 1));
+  if (rc) return rc;
   if (h->timing)
     while (h->ev.size() < 2 * (size_t)nsteps) {
       hipEvent_t e;
       HIPCHECK(h, hipEventCreate(&e));
       h->ev.push_back(e);
     }
+  // The frame of every step is known now: moves are fully determined by
+  // body_rel_pos(1), which only this host changes (main.f90:393-411).
+  if (nsteps > 0) {
+    h->frames.resize(nsteps);
+    double brp0 = h->W > 0 ? h->body_rel_pos[0] : 0.0;
+    Frame f = h->fr;
+    for (int k = 0; k < nsteps; k++) {
+      h->frames[k] = f;
+      const int dr = brp0 >= 1.0 ? 1 : (brp0 <= -1.0 ? -1 : 0);
+      if (dr != 0) {
+        brp0 = brp0 - (double)dr;
+        f.movenum += dr;
+        double ap = brp0;
+        f.body_extra_pos[0] = ap;
+        for (int w = 1; w < h->W; w++) {
+          ap = ap + h->body_rel_pos[w];
+          f.body_extra_pos[w] = ap;
+        }
+      }
+    }
+    HIPCHECK(h, hipStreamSynchronize(h->stream_geo));
+    HIPCHECK(h, hipMemcpy(h->d_frame, h->frames.data(), sizeof(Frame) * nsteps, hipMemcpyHostToDevice));
+  }
   HIPCHECK(h, hipMemsetAsync(h->d_rowsum, 0, sizeof(double) * h->row_len, h->stream));
   HIPCHECK(h, hipMemsetAsync(&h->d_st->error, 0, sizeof(int), h->stream));
-  HIPCHECK(h, hipStreamSynchronize(h->stream_ibm));
-  // moves of the frame are fully determined by body_rel_pos(1), which only
-  // this host changes (main.f90:393-411)
+  HIPCHECK(h, hipStreamSynchronize(h->stream));
+  const int ns = h->nslot;
+  int set = 0, slot = 0;
   for (int m = n0; m <= n_last; m++) {
+    const int k = m - n0;
+    if (h->W > 0 && k % ns == 0) {  // geometry of the next batch of steps, ahead of the flow
+      const int cnt = std::min(ns, n_last - m + 1);
+      set = (int)(h->batch_count++ & 1u);
+      HIPCHECK(h, hipStreamWaitEvent(h->stream_geo, h->ev_rel[set], 0));
+      launch_geometry(h->stream_geo, h->ip, h->G, h->d_st, set * ns, m, k, cnt, true);
+      HIPCHECK(h, hipEventRecord(h->ev_geo[set], h->stream_geo));
+      HIPCHECK(h, hipStreamWaitEvent(h->stream_ibm, h->ev_geo[set], 0));
+    }
+    slot = set * ns + k % ns;
     const int s = m % 3;
     double brp = h->W > 0 ? h->body_rel_pos[0] : 0.0;
     const int dr = brp >= 1.0 ? 1 : (brp <= -1.0 ? -1 : 0);
     const bool want_fields = (m >= n_last - 2) || dr != 0;
     FieldSlot* fs = want_fields ? &h->slot[s] : nullptr;
-    const double* src = h->fbuf[h->cur];
-    double* dst = h->fbuf[1 - h->cur];
-    // IBM chain on the second stream, concurrent with the full-lattice kernel;
-    // it starts once the previous step has finished on the lattice stream.
+    const void* src = h->fbuf[h->cur];
+    void* dst = h->fbuf[1 - h->cur];
+    // IBM flow kernels on the second stream, concurrent with the full-lattice
+    // kernel; they start once the previous step has finished.
+    // The boundary ring reads only src and writes only ring nodes of dst, so
+    // it runs there as well.
     HIPCHECK(h, hipEventRecord(h->ev_step, h->stream));
     HIPCHECK(h, hipStreamWaitEvent(h->stream_ibm, h->ev_step, 0));
-    launch_motion_markers(h->stream_ibm, h->ip, h->B, m);
-    if (h->W > 0) {
-      launch_adjoint_build(h->stream_ibm, h->ip, h->B);
-      launch_ibm_gather(h->stream_ibm, h->ip, h->B, h->g, src);
-      launch_ibm_assemble(h->stream_ibm, h->ip, h->B);
-      launch_ibm_wing(h->stream_ibm, h->ip, h->B);
-    }
-    launch_forces(h->stream_ibm, h->ip, h->B, m - n0, h->row_len);
+    launch_ibm_flow(h->stream_ibm, h->ip, h->mrt, h->G, slot, h->F, h->g, src, h->f32);
+    launch_bc_ring(h->stream_ibm, h->g, h->mrt, h->R, src, dst, fs, h->f32);
     HIPCHECK(h, hipEventRecord(h->ev_ibm, h->stream_ibm));
-    const int step_i = m - n0;
-    if (h->timing) HIPCHECK(h, hipEventRecord(h->ev[2 * step_i], h->stream));
-    launch_lattice(h->stream, h->g, h->mrt, src, dst, fs);
-    if (h->timing) HIPCHECK(h, hipEventRecord(h->ev[2 * step_i + 1], h->stream));
-    launch_bc_ring(h->stream, h->g, h->mrt, h->R, src, dst, fs);
+    if (h->timing) HIPCHECK(h, hipEventRecord(h->ev[2 * k], h->stream));
+    launch_lattice(h->stream, h->g, h->mrt, src, dst, fs, h->f32);
+    if (h->timing) HIPCHECK(h, hipEventRecord(h->ev[2 * k + 1], h->stream));
     HIPCHECK(h, hipStreamWaitEvent(h->stream, h->ev_ibm, 0));
-    if (h->W > 0) launch_adj_collide(h->stream, h->g, h->mrt, h->B, h->ip.cap, src, dst, fs);
+    launch_adj_collide(h->stream, h->g, h->mrt, h->ip, h->G, slot, h->F, h->d_st, m, k, h->row_len, src, dst, fs,
+                       h->f32);
+    if (h->W > 0 && ((k + 1) % ns == 0 || m == n_last)) HIPCHECK(h, hipEventRecord(h->ev_rel[set], h->stream));
     if (want_fields) h->slot_step[s] = m;
     h->cur = 1 - h->cur;
     if (dr != 0) {  // move_mesh (move_mesh.f90:17-58, 171-214)
-      double* a = h->fbuf[h->cur];
-      double* b = h->fbuf[1 - h->cur];
-      launch_shift_x(h->stream, h->g, dr, a, b, 9);
+      launch_shift_x(h->stream, h->g, dr, h->fbuf[h->cur], h->fbuf[1 - h->cur], 9, h->f32);
       h->cur = 1 - h->cur;
-      launch_copy_ring_column(h->stream, h->g, h->fbuf[h->cur], dr == 1 ? 0 : h->g.nx - 1,
-                              dr == 1 ? h->R.left : h->R.right);
+      launch_copy_ring_column(h->stream, h->g, h->mrt, h->fbuf[h->cur], dr == 1 ? 0 : h->g.nx - 1,
+                              dr == 1 ? h->R.left : h->R.right, h->f32);
       launch_ring_moments(h->stream, h->g, h->R);
       FieldSlot& cs = h->slot[s];
-      launch_shift_x(h->stream, h->g, dr, cs.rho, h->spare.rho, 1);
-      launch_shift_x(h->stream, h->g, dr, cs.u, h->spare.u, 1);
-      launch_shift_x(h->stream, h->g, dr, cs.v, h->spare.v, 1);
+      launch_shift_x(h->stream, h->g, dr, cs.rho, h->spare.rho, 1, false);
+      launch_shift_x(h->stream, h->g, dr, cs.u, h->spare.u, 1, false);
+      launch_shift_x(h->stream, h->g, dr, cs.v, h->spare.v, 1, false);
       std::swap(h->slot[s], h->spare);
       h->body_rel_pos[0] = h->body_rel_pos[0] - (double)dr;
-      update_body_extra(h);
       h->movenum += dr;
-      h->ip.movenum = h->movenum;
+      update_body_extra(h);
       h->flag2 = 3;
       h->last_move_step = m;
     }
     if (h->flag2 > 0) h->flag2 = h->flag2 - 1;
   }
+  if (nsteps > 0 && h->W > 0) h->last_slot = slot;
   h->n = std::max(n0, n_last + 1);
   // erranalysis of the last step (solution.f90:77-130): u of step n-1 against step n-2
   double errv = 0.0;
@@ -759,9 +792,12 @@ int lbm_get_state(lbm_handle* h, int32_t* n_next, int32_t* movenum, int32_t* fla
   if (flag2) *flag2 = h->flag2;
   if (row_len) *row_len = h->row_len;
   if (adjoint_n) {
-    DevStep st;
-    HIPCHECK(h, hipMemcpy(&st, h->d_st, sizeof(DevStep), hipMemcpyDeviceToHost));
-    *adjoint_n = st.adj_n;
+    *adjoint_n = 0;
+    if (h->last_slot >= 0) {
+      GeoStep gs;
+      HIPCHECK(h, hipMemcpy(&gs, h->G.step + h->last_slot, sizeof(GeoStep), hipMemcpyDeviceToHost));
+      *adjoint_n = gs.adj_n;
+    }
   }
   return 0;
 }
@@ -788,9 +824,18 @@ int lbm_get_populations(lbm_handle* h, double* f, double* f0) {
   if (!h) return LBM_E_INVALID;
   const size_t np = (size_t)h->g.nx * h->g.ny;
   HIPCHECK(h, hipStreamSynchronize(h->stream));
-  if (f)
-    for (int k = 0; k < 9; k++)
-      HIPCHECK(h, hipMemcpy(f + k * np, h->fbuf[h->cur] + k * h->g.ps, np * sizeof(double), hipMemcpyDeviceToHost));
+  if (f) {
+    std::vector<float> tmp(h->f32 ? np : 0);
+    for (int k = 0; k < 9; k++) {
+      const char* plane = (const char*)h->fbuf[h->cur] + (size_t)k * h->g.ps * h->fsize;
+      if (h->f32) {
+        HIPCHECK(h, hipMemcpy(tmp.data(), plane, np * sizeof(float), hipMemcpyDeviceToHost));
+        for (size_t q = 0; q < np; q++) f[k * np + q] = h->w[k] + (double)tmp[q];
+      } else {
+        HIPCHECK(h, hipMemcpy(f + k * np, plane, np * sizeof(double), hipMemcpyDeviceToHost));
+      }
+    }
+  }
   if (f0) {
     std::memset(f0, 0, 9 * np * sizeof(double));
     std::vector<double> L(9 * (size_t)h->g.ny), Rr(9 * (size_t)h->g.ny);
@@ -810,11 +855,19 @@ int lbm_set_populations(lbm_handle* h, const double* f, const double* f0) {
   const size_t np = (size_t)h->g.nx * h->g.ny;
   HIPCHECK(h, hipStreamSynchronize(h->stream));
   if (f) {
-    for (int k = 0; k < 9; k++)
-      HIPCHECK(h, hipMemcpy(h->fbuf[h->cur] + k * h->g.ps, f + k * np, np * sizeof(double), hipMemcpyHostToDevice));
+    std::vector<float> tmp(h->f32 ? np : 0);
+    for (int k = 0; k < 9; k++) {
+      char* plane = (char*)h->fbuf[h->cur] + (size_t)k * h->g.ps * h->fsize;
+      if (h->f32) {
+        for (size_t q = 0; q < np; q++) tmp[q] = (float)(f[k * np + q] - h->w[k]);
+        HIPCHECK(h, hipMemcpy(plane, tmp.data(), np * sizeof(float), hipMemcpyHostToDevice));
+      } else {
+        HIPCHECK(h, hipMemcpy(plane, f + k * np, np * sizeof(double), hipMemcpyHostToDevice));
+      }
+    }
     const int m = h->n - 1;
     const int s = ((m % 3) + 3) % 3;
-    launch_moments_all(h->stream, h->g, h->fbuf[h->cur], h->slot[s]);
+    launch_moments_all(h->stream, h->g, h->mrt, h->fbuf[h->cur], h->slot[s], h->f32);
     h->slot_step[s] = m;
   }
   if (f0) {
@@ -835,15 +888,21 @@ int lbm_set_populations(lbm_handle* h, const double* f, const double* f0) {
 int lbm_get_markers(lbm_handle* h, double* xb1, double* yb1, double* xb2, double* yb2) {
   if (!h) return LBM_E_INVALID;
   HIPCHECK(h, hipStreamSynchronize(h->stream));
-  if (h->npts > 0) {
-    if (xb1) HIPCHECK(h, hipMemcpy(xb1, h->B.xb1, h->npts * sizeof(double), hipMemcpyDeviceToHost));
-    if (yb1) HIPCHECK(h, hipMemcpy(yb1, h->B.yb1, h->npts * sizeof(double), hipMemcpyDeviceToHost));
-  }
-  if (h->nseg > 0) {
-    if (xb2) HIPCHECK(h, hipMemcpy(xb2, h->B.xb2, h->nseg * sizeof(double), hipMemcpyDeviceToHost));
-    if (yb2) HIPCHECK(h, hipMemcpy(yb2, h->B.yb2, h->nseg * sizeof(double), hipMemcpyDeviceToHost));
-  }
-  return 0;
+  const int s = h->last_slot;
+  auto get = [&](double* out, const double* base, int count) -> int {
+    if (!out || count <= 0) return 0;
+    if (s < 0) {
+      std::memset(out, 0, sizeof(double) * count);
+      return 0;
+    }
+    HIPCHECK(h, hipMemcpy(out, base + (size_t)s * count, sizeof(double) * count, hipMemcpyDeviceToHost));
+    return 0;
+  };
+  int rc = get(xb1, h->G.xb1, h->npts);
+  if (!rc) rc = get(yb1, h->G.yb1, h->npts);
+  if (!rc) rc = get(xb2, h->G.xb2, h->nseg);
+  if (!rc) rc = get(yb2, h->G.yb2, h->nseg);
+  return rc;
 }
 
 int lbm_get_ibm(lbm_handle* h, int32_t capacity, int32_t* ij, int32_t* wing_id, double* wing_force,
@@ -852,17 +911,23 @@ int lbm_get_ibm(lbm_handle* h, int32_t capacity, int32_t* ij, int32_t* wing_id, 
   HIPCHECK(h, hipStreamSynchronize(h->stream));
   DevStep st;
   HIPCHECK(h, hipMemcpy(&st, h->d_st, sizeof(DevStep), hipMemcpyDeviceToHost));
-  const int n = std::min(st.adj_n, capacity);
-  if (n > 0 && (ij || wing_id)) {
-    std::vector<int> hij(2 * (size_t)n), hseg(n);
-    HIPCHECK(h, hipMemcpy(hij.data(), h->B.adj_ij, sizeof(int) * 2 * n, hipMemcpyDeviceToHost));
-    HIPCHECK(h, hipMemcpy(hseg.data(), h->B.adj_seg, sizeof(int) * n, hipMemcpyDeviceToHost));
-    for (int a = 0; a < n; a++) {
-      if (ij) {
-        ij[2 * a] = hij[2 * a] + 1;
-        ij[2 * a + 1] = hij[2 * a + 1] + 1;
+  if (h->last_slot >= 0 && (ij || wing_id)) {
+    GeoStep gs;
+    HIPCHECK(h, hipMemcpy(&gs, h->G.step + h->last_slot, sizeof(GeoStep), hipMemcpyDeviceToHost));
+    int a = 0;
+    for (int w = 0; w < h->W && a < capacity; w++) {
+      const int n = std::min(gs.cnt[w], capacity - a);
+      if (n <= 0) continue;
+      std::vector<int> hij(2 * (size_t)n);
+      const size_t off = (size_t)h->last_slot * h->ip.cap + (size_t)w * h->ip.capw;
+      HIPCHECK(h, hipMemcpy(hij.data(), h->G.adj_ij + 2 * off, sizeof(int) * 2 * n, hipMemcpyDeviceToHost));
+      for (int q = 0; q < n; q++, a++) {
+        if (ij) {
+          ij[2 * a] = hij[2 * q] + 1;
+          ij[2 * a + 1] = hij[2 * q + 1] + 1;
+        }
+        if (wing_id) wing_id[a] = w + 1;
       }
-      if (wing_id) wing_id[a] = hseg[a] / h->SPW + 1;
     }
   }
   for (int w = 0; w < MAXW; w++) {

@@ -24,6 +24,12 @@ struct MrtParams {
   double s[9];      // relaxation rates (ini_mesh.f90:59-67)
   double minv[81];  // inverse of M as computed by nizhen (solution.f90:1474-1529)
   double cs;        // sqrt(3.0)/3 with a single-precision sqrt (boundary_condition.f90:304)
+  double w[9];      // lattice weights (single-precision literals, D2Q9model.f90:21-29)
+  // FP32-shifted storage: populations are kept as d = f - w in FP32 and the
+  // bulk collision runs on d in FP32 with these rounded constants.
+  float s32[9];
+  float minv32[81];
+  float eq1, eq2;   // -2 - (M w)_1 and 1 - (M w)_2: equilibrium moments minus those of w
 };
 
 // Lattice geometry shared by the kernels.
@@ -136,6 +142,33 @@ __device__ __forceinline__ void pull_node(const double* __restrict__ f, const Gr
 #pragma unroll
   for (int k = 0; k < 9; k++)
     fo[k] = f[(long long)k * g.ps + (long long)(j - EY(k)) * g.nx + (i - EX(k))];
+}
+
+// Storage of the populations: FP64 values, or FP32 deviations d = f - w_k
+// from the rest equilibrium.  Loads return the FP64 value of f.
+template <typename S>
+struct Store;
+template <>
+struct Store<double> {
+  static __device__ __forceinline__ double ld(const double* __restrict__ f, long long q, double) { return f[q]; }
+  static __device__ __forceinline__ void st(double* __restrict__ f, long long q, double v, double) { f[q] = v; }
+};
+template <>
+struct Store<float> {
+  static __device__ __forceinline__ double ld(const float* __restrict__ f, long long q, double wk) {
+    return wk + (double)f[q];
+  }
+  static __device__ __forceinline__ void st(float* __restrict__ f, long long q, double v, double wk) {
+    f[q] = (float)(v - wk);
+  }
+};
+
+template <typename S>
+__device__ __forceinline__ void pull_node_s(const S* __restrict__ f, const Grid& g, int i, int j, const double* w,
+                                            double fo[9]) {
+#pragma unroll
+  for (int k = 0; k < 9; k++)
+    fo[k] = Store<S>::ld(f, (long long)k * g.ps + (long long)(j - EY(k)) * g.nx + (i - EX(k)), w[k]);
 }
 
 __device__ __forceinline__ void node_moments(const double fo[9], double& rho, double& u, double& v) {

@@ -10,17 +10,18 @@
 namespace lbm {
 
 constexpr int MAXW = LBM_MAX_WINGS;
-constexpr int BMAX = 24;          // largest IBM band half-width the solvers accept
+constexpr int BMAX = 24;          // largest IBM band half-width the solver accepts
 constexpr int BOXW = 6;           // nodes per axis of a delta support box (|r| <= 2.5)
+constexpr int BOXC = BOXW * BOXW; // cells of a support box
 
 // Error codes raised by kernels into DevStep::error (first one wins).
 enum DevError : int {
   DERR_NONE = 0,
   DERR_BOX_EDGE = 1,       // support box within 3 nodes of the lattice edge
-  DERR_ADJ_OVERFLOW = 2,   // adjoint list capacity exceeded
+  DERR_MAP = 2,            // a wing's support rectangle exceeds its claim map
   DERR_BAND = 3,           // IBM band wider than BMAX
-  DERR_WINGS_COUPLED = 4,  // supports of two wings overlap
-  DERR_NOT_SPD = 5,        // non-positive pivot in the banded Cholesky
+  DERR_WINGS_COUPLED = 4,  // supports of two wings share a node
+  DERR_NOT_SPD = 5,        // non-positive pivot in the band factorisation
   DERR_MOTION = 6,         // motion source not ready
 };
 
@@ -28,67 +29,82 @@ struct WingMotionState {
   double angle, omega, omgac, sa, sau, saac, sb, sbv, sbac;
 };
 
-// Per-step state written by the device (one instance per solver).
+// Error state and last-step forces (one instance per solver).
 struct DevStep {
-  WingMotionState wing[MAXW];
-  double time;
-  int n;
-  int adj_n;
-  int bw;
   int error;
   int error_info;
+  int ticket;              // completion counter of the adjoint-collide blocks
   int pad;
   double force[MAXW][2];   // normalised per-wing force of the last step
   double moment[MAXW];
-  double raw[MAXW][3];     // un-normalised Fx, Fy, M of the last step
+};
+
+// Rigid-frame state of one step, known on the host before the step runs
+// (movenum and update_body_extra_from_rel, main.f90:393-411, mode.f90:473-495).
+struct Frame {
+  int movenum;
+  int pad;
+  double body_extra_v[MAXW], body_extra_s[MAXW], body_extra_pos[MAXW];
+};
+
+// Geometry of one step (motion, markers, adjoint list, inverse IBM matrix),
+// computed ahead of the flow by the geometry kernels.
+struct GeoStep {
+  WingMotionState wing[MAXW];
+  double time;
+  int n;
+  int adj_n;               // adjoint nodes over all wings
+  int cnt[MAXW];           // adjoint nodes per wing
+  int bw[MAXW];            // band half-width of S per wing (zig-zag order)
+  int rect[MAXW][4];       // i0, i1, j0, j1 (0-based) of the union of the wing's boxes
+  Frame frame;
 };
 
 // Host-known constants of the IBM / motion kernels.
 struct IbmParams {
-  int W, P, SPW, nseg, npts, cap;
+  int W, P, SPW, nseg, npts;
+  int capw, cap;           // adjoint capacity per wing (36*SPW) and in total
+  int map_cap;             // cells of one claim map
   int nx, ny;
   double scale, ox, oy, uu, dt;
   double wing_x0, wing_dx, wing_y0, xc, yc;
   int motion_source;       // LBM_MOTION_FILE / LBM_MOTION_STREAM
-  int movenum;
   int step_count;          // rows of the file-mode step table
   int ncols;
+  int pad;
   double motion_dt;
-  double body_extra_v[MAXW], body_extra_s[MAXW], body_extra_pos[MAXW];
 };
 
-// Device buffers of the IBM path.
-struct IbmBuffers {
-  const double* xb0; const double* yb0;   // body-frame markers (npts)
-  double* xb1; double* yb1;               // moved markers (npts)
-  double* xb2; double* yb2;               // segment midpoints (nseg)
-  const double* arc;                      // segment lengths / scale (nseg)
-  double* ub; double* vb;                 // rigid-body velocity at midpoints (nseg)
-  int* box;                               // i1,i2,j1,j2 per segment (0-based, nseg*4)
-  double* phx; double* phy;               // delta weights per box column/row (nseg*BOXW)
-  int* seg_cnt; int* seg_off;             // adjoint nodes claimed per segment / offsets (nseg+1)
-  int* owner;                             // lattice map: claiming segment (nx*ny, INT_MAX = none)
-  int* aidx;                              // lattice map: adjoint index (nx*ny, -1 = none)
-  int* kmin; int* kmax;                   // lattice maps: lowest / highest band key of the
-                                          // segments whose support holds the node
-  int* adj_ij;                            // (i, j) 0-based per adjoint node (cap*2)
-  int* adj_seg;                           // claiming segment per adjoint node (cap)
-  double* adj_rho; double* adj_us; double* adj_vs;   // moments at adjoint nodes (cap)
-  double* adj_u; double* adj_v;           // corrected velocity (cap)
-  double* adj_gx; double* adj_gy;         // body force density 2 rho g (cap)
-  double* adj_mom;                        // moment contribution -x0 t2 + y0 t1 (cap)
-  int* prev_ij; int* prev_n;              // adjoint list of the previous step (cleanup)
-  const int* seg_of_pos;                  // zig-zag order: segment at band position (nseg)
-  const int* pos_of_seg;                  // band position of each segment (nseg)
-  double* band;                           // band matrix, W * SPW * (BMAX+1)
-  double* bcr_ws;                         // block cyclic reduction workspace per wing
-  double* rhs;                            // 2 * nseg right-hand side, solution in place
-  const double* step_table;               // file mode: step_count * ncols
-  const double* stream_rows;              // stream mode: start row, target row (2*LBM_MAX_MOTION_COLS)
-  DevStep* st;
-  double* rows;                           // force rows of the current call
-  double* mrows;                          // airfoil_motion rows of the current call
-  double* row_sum;                        // sum of force rows
+// Per-slot geometry arrays; slot s of an array X of per-slot length L starts
+// at X + s*L.  Adjoint entries of wing w start at w*capw inside a slot.
+struct GeoBuffers {
+  GeoStep* step;           // [slots]
+  const double* xb0;       // body-frame markers (npts), static
+  const double* yb0;
+  double* xb1; double* yb1;     // [slots][npts] moved markers
+  double* xb2; double* yb2;     // [slots][nseg] segment midpoints
+  double* ub; double* vb;       // [slots][nseg] rigid-body velocity at the midpoints
+  int* box;                     // [slots][nseg][4] i1,i2,j1,j2 (0-based)
+  double* phx; double* phy;     // [slots][nseg][BOXW] 1-D delta weights per box column / row
+  int* seg_off;                 // [slots][W][SPW+1] wing-local adjoint offsets per segment
+  int* adj_ij;                  // [slots][cap][2] (i, j) 0-based
+  int* adj_seg;                 // [slots][cap] claiming segment (wing-local)
+  double* sinv;                 // [slots][W][SPW][SPW] inverse of S = D D^T, zig-zag order
+  int* map;                     // [slots][W][2][map_cap] claim maps (owner, lowest position)
+  const Frame* frame;           // per step of the current call
+  const double* step_table;     // file mode: step_count * ncols
+  const double* stream_rows;    // stream mode: start row, target row (2*LBM_MAX_MOTION_COLS)
+};
+
+// Flow-dependent IBM buffers of the step being run.
+struct FlowBuffers {
+  double* cvec;            // [nseg][2] right-hand side U_b - D u* by global band position
+  double* yvec;            // [nseg][2] S^-1 c
+  double* adj_g;           // [cap][2] velocity correction g at the adjoint nodes
+  double* partial;         // [W][blocks per wing][3] force partial sums
+  double* rows;            // force rows of the current call
+  double* mrows;           // airfoil_motion rows of the current call
+  double* row_sum;         // sum of force rows
 };
 
 struct RingBuffers {
@@ -101,34 +117,43 @@ struct FieldSlot {
   double* rho; double* u; double* v;
 };
 
-// lattice kernels (lbm_lattice.hip)
-void launch_lattice(hipStream_t s, const Grid& g, const MrtParams& P, const double* src, double* dst,
-                    FieldSlot* fields);
-void launch_adj_collide(hipStream_t s, const Grid& g, const MrtParams& P, const IbmBuffers& B, int cap,
-                        const double* src, double* dst, FieldSlot* fields);
-void launch_bc_ring(hipStream_t s, const Grid& g, const MrtParams& P, const RingBuffers& R,
-                    const double* src, double* dst, FieldSlot* fields);
+// Band position <-> segment in a wing of n segments: positions 0,1,2,3,...
+// hold segments 0, n-1, 1, n-2, ... so that segments facing each other on
+// the upper and lower surface are neighbours and S is a narrow band.
+__host__ __device__ __forceinline__ int seg_of_pos(int q, int n) { return (q & 1) ? n - 1 - (q >> 1) : (q >> 1); }
+__host__ __device__ __forceinline__ int pos_of_seg(int k, int n) {
+  return k < (n + 1) / 2 ? 2 * k : 2 * (n - 1 - k) + 1;
+}
+
+// lattice kernels (lbm_lattice.hip); populations are double (f32 = false) or
+// FP32 deviations from the weights (f32 = true)
+void launch_lattice(hipStream_t s, const Grid& g, const MrtParams& P, const void* src, void* dst, FieldSlot* fields,
+                    bool f32);
+void launch_adj_collide(hipStream_t s, const Grid& g, const MrtParams& P, const IbmParams& p, const GeoBuffers& G,
+                        int slot, const FlowBuffers& F, DevStep* st, int n, int row, int row_len, const void* src,
+                        void* dst, FieldSlot* fields, bool f32);
+void launch_bc_ring(hipStream_t s, const Grid& g, const MrtParams& P, const RingBuffers& R, const void* src,
+                    void* dst, FieldSlot* fields, bool f32);
 void launch_ring_moments(hipStream_t s, const Grid& g, const RingBuffers& R);
-void launch_init_fields(hipStream_t s, const Grid& g, double uu, const double feq[9], double* f,
-                        const RingBuffers& R, FieldSlot slot);
-void launch_moments_all(hipStream_t s, const Grid& g, const double* f, FieldSlot slot);
+void launch_init_fields(hipStream_t s, const Grid& g, const MrtParams& P, double uu, const double feq[9], void* f,
+                        const RingBuffers& R, FieldSlot slot, bool f32);
+void launch_moments_all(hipStream_t s, const Grid& g, const MrtParams& P, const void* f, FieldSlot slot, bool f32);
 void launch_err_partial(hipStream_t s, const Grid& g, FieldSlot cur, FieldSlot prev, double* partial,
                         int nblocks);
 void launch_err_final(hipStream_t s, const double* partial, int nblocks, double* out);
 int err_blocks(const Grid& g);
-void launch_shift_x(hipStream_t s, const Grid& g, int dr, const double* f, double* fshift, int planes);
-void launch_copy_ring_column(hipStream_t s, const Grid& g, const double* f, int col, double* ring);
+void launch_shift_x(hipStream_t s, const Grid& g, int dr, const void* f, void* fshift, int planes, bool f32);
+void launch_copy_ring_column(hipStream_t s, const Grid& g, const MrtParams& P, const void* f, int col, double* ring,
+                             bool f32);
+int adj_collide_blocks_per_wing(const IbmParams& p);
 
-// IBM kernels (lbm_ibm.hip)
-void launch_motion_markers(hipStream_t s, const IbmParams& p, const IbmBuffers& B, int n);
-void launch_adjoint_build(hipStream_t s, const IbmParams& p, const IbmBuffers& B);
-void launch_clear_adjoint(hipStream_t s, const IbmParams& p, const IbmBuffers& B);
-void launch_ibm_gather(hipStream_t s, const IbmParams& p, const IbmBuffers& B, const Grid& g,
-                       const double* src);
-void launch_ibm_assemble(hipStream_t s, const IbmParams& p, const IbmBuffers& B);
-void launch_ibm_wing(hipStream_t s, const IbmParams& p, const IbmBuffers& B);
-void launch_forces(hipStream_t s, const IbmParams& p, const IbmBuffers& B, int slot, int row_len);
-size_t ibm_solve_lds_bytes(int spw);
-size_t ibm_bcr_ws_doubles(int W);
+// IBM geometry kernels (lbm_ibm.hip), one launch covers `count` consecutive
+// steps n0.. into slots slot0..; frames are indexed from fidx0.
+void launch_geometry(hipStream_t s, const IbmParams& p, const GeoBuffers& G, DevStep* st, int slot0, int n0,
+                     int fidx0, int count, bool with_inverse);
+size_t geo_solve_lds_bytes(const IbmParams& p);
+// IBM flow kernels of one step.
+void launch_ibm_flow(hipStream_t s, const IbmParams& p, const MrtParams& P, const GeoBuffers& G, int slot,
+                     const FlowBuffers& F, const Grid& g, const void* src, bool f32);
 
 }  // namespace lbm

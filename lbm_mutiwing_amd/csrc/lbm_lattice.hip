@@ -1,12 +1,20 @@
-// lbm_lattice.hip - full-lattice kernels of the FP64 time step on gfx950.
+// lbm_lattice.hip - full-lattice kernels of the time step on gfx950.
 //
 // k_lattice fuses streaming (solution.f90:440-466), physical_variables
 // (solution.f90:483-554) and colission (solution.f90:585-655) for every
 // interior node: 9 pulled loads and 9 stores of one SoA plane each, i.e. the
-// 144 B/node floor of a two-buffer FP64 D2Q9 step.  Nodes that the immersed
-// boundary touches this step are collided here with zero body force and
-// collided again by k_adj_collide once the IBM correction is known, so the
-// full-lattice pass never waits for the IBM chain.
+// 144 B/node floor of a two-buffer FP64 D2Q9 step (72 B/node with FP32
+// deviation storage).  Nodes that the immersed boundary touches this step are
+// collided here with zero body force and collided again by k_adj_collide once
+// the IBM correction is known, so the full-lattice pass never waits for the
+// IBM chain.
+//
+// Every kernel that touches populations is a template on the storage type S:
+// double (FP64 values) or float (FP32 deviations d = f - w_k).  Only the bulk
+// collision of the FP32 storage computes in FP32; the boundary ring, the
+// adjoint nodes and all reductions compute in FP64 on loaded f = w_k + d.
+#include <algorithm>
+
 #include "lbm_internal.h"
 
 namespace lbm {
@@ -14,10 +22,10 @@ namespace lbm {
 constexpr int LAT_BLOCK = 256;
 
 // One thread per node of a row tile; tiles start at i = 0 so that every
-// wavefront stores whole, aligned 512-byte spans of each population plane
-// (the ring columns i = 0 and i = nx-1 are skipped and left to k_bc_ring).
-// Workgroups follow the hardware's round-robin placement over the XCDs,
-// which measured faster than remapping tiles to XCD-contiguous runs.
+// wavefront stores whole, aligned spans of each population plane (the ring
+// columns i = 0 and i = nx-1 are skipped and left to k_bc_ring).  Workgroups
+// follow the hardware's round-robin placement over the XCDs, which measured
+// faster than remapping tiles to XCD-contiguous runs.
 template <bool FIELDS>
 __global__ __launch_bounds__(LAT_BLOCK) void k_lattice(Grid g, MrtParams P, const double* __restrict__ src,
                                                        double* __restrict__ dst, FieldSlot fs, int tiles_x) {
@@ -41,48 +49,265 @@ __global__ __launch_bounds__(LAT_BLOCK) void k_lattice(Grid g, MrtParams P, cons
   }
 }
 
-void launch_lattice(hipStream_t s, const Grid& g, const MrtParams& P, const double* src, double* dst,
-                    FieldSlot* fields) {
-  const int tiles_x = (g.nx + LAT_BLOCK - 1) / LAT_BLOCK;
-  dim3 grid(tiles_x * (g.ny - 2)), block(LAT_BLOCK);
-  if (fields)
-    hipLaunchKernelGGL(k_lattice<true>, grid, block, 0, s, g, P, src, dst, *fields, tiles_x);
-  else
-    hipLaunchKernelGGL(k_lattice<false>, grid, block, 0, s, g, P, src, dst, FieldSlot{}, tiles_x);
-}
-
-// Re-collide the adjoint nodes with the IBM-corrected velocity and the body
-// force 2 rho g (boundary_condition.f90:1645-1659 then solution.f90:599-650).
+// FP32 bulk step on deviations d = f - w: rho = 1 + sum d, j = sum e d (the
+// weights carry no momentum), m' = M d, and the equilibrium moments shifted
+// by M w.  Relaxation and the inverse transform are linear, so d* = M^-1 m*'
+// is the deviation of the post-collision populations.
 template <bool FIELDS>
-__global__ __launch_bounds__(256) void k_adj_collide(Grid g, MrtParams P, IbmBuffers B, const double* __restrict__ src,
-                                                     double* __restrict__ dst, FieldSlot fs) {
-  const int a = blockIdx.x * blockDim.x + threadIdx.x;
-  if (B.st->error != 0 || a >= B.st->adj_n) return;
-  const int i = B.adj_ij[2 * a], j = B.adj_ij[2 * a + 1];
-  double f[9];
-  pull_node(src, g, i, j, f);
-  double rho, u0, v0;
-  node_moments(f, rho, u0, v0);
-  const double u = B.adj_u[a], v = B.adj_v[a];
-  double out[9];
-  collide_node(P, f, rho, u, v, B.adj_gx[a], B.adj_gy[a], out);
+__global__ __launch_bounds__(LAT_BLOCK) void k_lattice_f32(Grid g, MrtParams P, const float* __restrict__ src,
+                                                           float* __restrict__ dst, FieldSlot fs, int tiles_x) {
+  const int ty = blockIdx.x / tiles_x, tx = blockIdx.x - ty * tiles_x;
+  const int i = tx * LAT_BLOCK + (int)threadIdx.x;
+  const int j = 1 + ty;
+  if (i < 1 || i > g.nx - 2) return;
+  float d[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) d[k] = src[(long long)k * g.ps + (long long)(j - EY(k)) * g.nx + (i - EX(k))];
+  const float rp = d[0] + d[1] + d[2] + d[3] + d[4] + d[5] + d[6] + d[7] + d[8];
+  const float jx = d[1] - d[3] + d[5] - d[6] - d[7] + d[8];
+  const float jy = d[2] - d[4] + d[5] + d[6] - d[7] - d[8];
+  const float rho = 1.0f + rp;
+  const float u = jx / rho, v = jy / rho;
+  float m[9], t[9];
+  m[0] = rp;
+  m[1] = -4.0f * d[0] - d[1] - d[2] - d[3] - d[4] + 2.0f * (d[5] + d[6] + d[7] + d[8]);
+  m[2] = 4.0f * d[0] - 2.0f * (d[1] + d[2] + d[3] + d[4]) + d[5] + d[6] + d[7] + d[8];
+  m[3] = jx;
+  m[4] = -2.0f * d[1] + 2.0f * d[3] + d[5] - d[6] - d[7] + d[8];
+  m[5] = jy;
+  m[6] = -2.0f * d[2] + 2.0f * d[4] + d[5] + d[6] - d[7] - d[8];
+  m[7] = d[1] - d[2] + d[3] - d[4];
+  m[8] = d[5] - d[6] + d[7] - d[8];
+  const float ru = rho * u, rv = rho * v;
+  const float e2 = ru * ru + rv * rv;
+  t[0] = m[0];
+  t[1] = -2.0f * rp + 3.0f * e2 + P.eq1;
+  t[2] = rp - 3.0f * e2 + P.eq2;
+  t[3] = m[3];
+  t[4] = -ru;
+  t[5] = m[5];
+  t[6] = -rv;
+  t[7] = ru * ru - rv * rv;
+  t[8] = ru * rv;
+#pragma unroll
+  for (int k = 0; k < 9; k++) t[k] = m[k] + (t[k] - m[k]) * P.s32[k];
+  const float* mi = P.minv32;
+  float out[9];
+  out[0] = mi[0] * t[0] + mi[1] * t[1] + mi[2] * t[2];
+#pragma unroll
+  for (int r = 1; r <= 3; r += 2) {
+    const float* a = mi + 9 * r;
+    out[r] = a[0] * t[0] + a[1] * t[1] + a[2] * t[2] + a[3] * t[3] + a[4] * t[4] + a[7] * t[7];
+  }
+#pragma unroll
+  for (int r = 2; r <= 4; r += 2) {
+    const float* a = mi + 9 * r;
+    out[r] = a[0] * t[0] + a[1] * t[1] + a[2] * t[2] + a[5] * t[5] + a[6] * t[6] + a[7] * t[7];
+  }
+#pragma unroll
+  for (int r = 5; r <= 8; r++) {
+    const float* a = mi + 9 * r;
+    out[r] = a[0] * t[0] + a[1] * t[1] + a[2] * t[2] + a[3] * t[3] + a[4] * t[4] + a[5] * t[5] + a[6] * t[6] +
+             a[8] * t[8];
+  }
   const long long o = (long long)j * g.nx + i;
 #pragma unroll
   for (int k = 0; k < 9; k++) dst[(long long)k * g.ps + o] = out[k];
   if (FIELDS) {
-    fs.u[o] = u;
-    fs.v[o] = v;
+    const double rd = 1.0 + (double)rp;
+    fs.rho[o] = rd;
+    fs.u[o] = (double)jx / rd;
+    fs.v[o] = (double)jy / rd;
   }
 }
 
-void launch_adj_collide(hipStream_t s, const Grid& g, const MrtParams& P, const IbmBuffers& B, int cap,
-                        const double* src, double* dst, FieldSlot* fields) {
-  if (cap <= 0) return;
-  dim3 grid((cap + 255) / 256), block(256);
+void launch_lattice(hipStream_t s, const Grid& g, const MrtParams& P, const void* src, void* dst, FieldSlot* fields,
+                    bool f32) {
+  const int tiles_x = (g.nx + LAT_BLOCK - 1) / LAT_BLOCK;
+  dim3 grid(tiles_x * (g.ny - 2)), block(LAT_BLOCK);
+  const FieldSlot fs = fields ? *fields : FieldSlot{};
+  if (f32) {
+    const float* a = (const float*)src;
+    float* b = (float*)dst;
+    if (fields)
+      hipLaunchKernelGGL(k_lattice_f32<true>, grid, block, 0, s, g, P, a, b, fs, tiles_x);
+    else
+      hipLaunchKernelGGL(k_lattice_f32<false>, grid, block, 0, s, g, P, a, b, fs, tiles_x);
+  } else {
+    const double* a = (const double*)src;
+    double* b = (double*)dst;
+    if (fields)
+      hipLaunchKernelGGL(k_lattice<true>, grid, block, 0, s, g, P, a, b, fs, tiles_x);
+    else
+      hipLaunchKernelGGL(k_lattice<false>, grid, block, 0, s, g, P, a, b, fs, tiles_x);
+  }
+}
+
+// Re-collide the adjoint nodes with the IBM-corrected velocity u = u* + g and
+// the body force 2 rho g (boundary_condition.f90:1645-1659, then
+// solution.f90:599-650), and reduce each wing's force and moment
+// (boundary_condition.f90:1662-1675).  Blocks are (wing, node block); each
+// writes its partial sums and the last block to finish adds them in block
+// order and writes the normalised force row (solution.f90:330-401).
+constexpr int ADJ_THREADS = 256;
+constexpr int ADJ_BLOCKS_PER_WING = 16;   // grid-stride over a wing's adjoint nodes
+
+int adj_collide_blocks_per_wing(const IbmParams& p) {
+  return p.W > 0 ? std::min(ADJ_BLOCKS_PER_WING, (p.capw + ADJ_THREADS - 1) / ADJ_THREADS) : 1;
+}
+
+__device__ __forceinline__ double load_agent_d(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ void force_row(const IbmParams& p, const GeoStep* gs, DevStep* st, const FlowBuffers& F,
+                          const double (*raw)[3], int n, int slot_row, int row_len) {
+  const bool ok = st->error == 0;
+  const double UU = p.uu, scale = p.scale;
+  double wf[MAXW][2], wmom[MAXW];
+  double F0 = 0.0, F1 = 0.0, Mt = 0.0;
+  for (int w = 0; w < p.W; w++) {
+    const double* r = raw[w];
+    wf[w][0] = ok ? 2.0 * r[0] / (UU * UU * (1.0 / scale)) : 0.0;
+    wf[w][1] = ok ? 2.0 * r[1] / (UU * UU * (1.0 / scale)) : 0.0;
+    wmom[w] = ok ? -2.0 * r[2] / (UU * UU * (1.0 / (scale * scale))) : 0.0;
+    st->force[w][0] = wf[w][0];
+    st->force[w][1] = wf[w][1];
+    st->moment[w] = wmom[w];
+  }
+  for (int w = 0; w < p.W; w++) {
+    F0 = F0 + wf[w][0];
+    F1 = F1 + wf[w][1];
+  }
+  for (int w = 0; w < p.W; w++) Mt = Mt + wmom[w];
+  const double time = p.W > 0 ? gs->time : (double)(n - 1) * p.dt;
+  double* row = F.rows + (long long)slot_row * row_len;
+  int col = 0;
+  row[col++] = time;
+  for (int w = 0; w < p.W; w++) {
+    row[col++] = wf[w][0];
+    row[col++] = wf[w][1];
+    row[col++] = wmom[w];
+  }
+  row[col++] = F0;
+  row[col++] = F1;
+  row[col++] = Mt;
+  for (int w = 0; w < p.W; w++) {
+    const WingMotionState& m = gs->wing[w];
+    row[col++] = p.wing_x0 + (double)w * p.wing_dx + m.sa + gs->frame.body_extra_s[w] / (1.0 / scale);
+    row[col++] = m.sau;
+    row[col++] = m.saac;
+    row[col++] = p.wing_y0 + m.sb;
+    row[col++] = m.sbv;
+    row[col++] = m.sbac;
+    row[col++] = m.angle;
+    row[col++] = m.omega;
+    row[col++] = m.omgac;
+  }
+  for (int c = 0; c < row_len; c++) F.row_sum[c] = F.row_sum[c] + row[c];
+  double* mrow = F.mrows + (long long)slot_row * (1 + 3 * p.W);
+  int c = 0;
+  mrow[c++] = time;
+  for (int w = 0; w < p.W; w++) {
+    const WingMotionState& m = gs->wing[w];
+    mrow[c++] = 0.0;
+    mrow[c++] = gs->frame.body_extra_v[w] / UU;
+    mrow[c++] = p.wing_x0 + (double)w * p.wing_dx + m.sa + gs->frame.body_extra_s[w] / (1.0 / scale);
+  }
+}
+
+template <typename S, bool FIELDS>
+__global__ __launch_bounds__(ADJ_THREADS) void k_adj_collide(Grid g, MrtParams P, IbmParams p, GeoBuffers G, int slot,
+                                                             FlowBuffers F, DevStep* st, int n, int slot_row,
+                                                             int row_len, int bpw, const S* __restrict__ src,
+                                                             S* __restrict__ dst, FieldSlot fs) {
+  __shared__ double red[3][ADJ_THREADS];
+  __shared__ double s_raw[MAXW][3];
+  __shared__ int s_last;
+  const int t = threadIdx.x;
+  const int w = blockIdx.x / bpw, b = blockIdx.x - w * bpw;
+  const GeoStep* gs = G.step + slot;
+  double fx = 0.0, fy = 0.0, fm = 0.0;
+  const int cnt = p.W > 0 ? gs->cnt[w] : 0;
+  for (int a = b * ADJ_THREADS + t; a < cnt; a += bpw * ADJ_THREADS) {
+    const size_t la = (size_t)slot * p.cap + (size_t)w * p.capw + a;
+    const int i = G.adj_ij[2 * la], j = G.adj_ij[2 * la + 1];
+    double f[9];
+    pull_node_s(src, g, i, j, P.w, f);
+    double rho, us, vs;
+    node_moments(f, rho, us, vs);
+    const size_t ga = (size_t)w * p.capw + a;
+    const double gx = F.adj_g[2 * ga], gy = F.adj_g[2 * ga + 1];
+    const double u = us + gx, v = vs + gy;
+    const double t1 = 2.0 * rho * gx, t2 = 2.0 * rho * gy;
+    double out[9];
+    collide_node(P, f, rho, u, v, t1, t2, out);
+    const long long o = (long long)j * g.nx + i;
+#pragma unroll
+    for (int k = 0; k < 9; k++) Store<S>::st(dst, (long long)k * g.ps + o, out[k], P.w[k]);
+    if (FIELDS) {
+      fs.u[o] = u;
+      fs.v[o] = v;
+    }
+    const WingMotionState& m = gs->wing[w];
+    const double x0 = (p.ox + p.scale * (double)i - (p.xc + p.wing_x0 + (double)w * p.wing_dx + m.sa)) / p.scale -
+                      gs->frame.body_extra_pos[w];
+    const double y0 = (p.oy + p.scale * (double)j - (p.yc + p.wing_y0 + m.sb)) / p.scale;
+    fx = fx - t1;
+    fy = fy - t2;
+    fm = fm + (-x0 * t2 + y0 * t1);
+  }
+  red[0][t] = fx;
+  red[1][t] = fy;
+  red[2][t] = fm;
+  __syncthreads();
+  for (int h = ADJ_THREADS / 2; h > 0; h >>= 1) {
+    if (t < h)
+      for (int c = 0; c < 3; c++) red[c][t] += red[c][t + h];
+    __syncthreads();
+  }
+  if (t == 0) {
+    for (int c = 0; c < 3; c++) F.partial[((size_t)w * bpw + b) * 3 + c] = red[c][0];
+    __threadfence();
+    s_last = atomicAdd(&st->ticket, 1) == (int)gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
+  // last block: per-wing sums of the block partials in block order
+  if (t < 3 * p.W) {
+    const int ww = t / 3, c = t - 3 * ww;
+    double r = 0.0;
+    for (int q = 0; q < bpw; q++) r = r + load_agent_d(&F.partial[((size_t)ww * bpw + q) * 3 + c]);
+    s_raw[ww][c] = r;
+  }
+  __syncthreads();
+  if (t != 0) return;
+  force_row(p, gs, st, F, s_raw, n, slot_row, row_len);
+  st->ticket = 0;
+}
+
+template <typename S>
+void launch_adj_collide_t(hipStream_t s, const Grid& g, const MrtParams& P, const IbmParams& p, const GeoBuffers& G,
+                          int slot, const FlowBuffers& F, DevStep* st, int n, int row, int row_len, const S* src,
+                          S* dst, FieldSlot* fields) {
+  const int bpw = adj_collide_blocks_per_wing(p);
+  dim3 grid(std::max(p.W, 1) * bpw), block(ADJ_THREADS);
   if (fields)
-    hipLaunchKernelGGL(k_adj_collide<true>, grid, block, 0, s, g, P, B, src, dst, *fields);
+    hipLaunchKernelGGL((k_adj_collide<S, true>), grid, block, 0, s, g, P, p, G, slot, F, st, n, row, row_len, bpw,
+                       src, dst, *fields);
   else
-    hipLaunchKernelGGL(k_adj_collide<false>, grid, block, 0, s, g, P, B, src, dst, FieldSlot{});
+    hipLaunchKernelGGL((k_adj_collide<S, false>), grid, block, 0, s, g, P, p, G, slot, F, st, n, row, row_len, bpw,
+                       src, dst, FieldSlot{});
+}
+
+void launch_adj_collide(hipStream_t s, const Grid& g, const MrtParams& P, const IbmParams& p, const GeoBuffers& G,
+                        int slot, const FlowBuffers& F, DevStep* st, int n, int row, int row_len, const void* src,
+                        void* dst, FieldSlot* fields, bool f32) {
+  if (f32)
+    launch_adj_collide_t(s, g, P, p, G, slot, F, st, n, row, row_len, (const float*)src, (float*)dst, fields);
+  else
+    launch_adj_collide_t(s, g, P, p, G, slot, F, st, n, row, row_len, (const double*)src, (double*)dst, fields);
 }
 
 // ------------------------------------------------------------------ far field
@@ -91,23 +316,25 @@ struct NodeVals {
   double rho, u, v;
 };
 
-__device__ __forceinline__ NodeVals interior_vals(const double* __restrict__ src, const Grid& g, int i, int j,
-                                                  double f[9]) {
-  pull_node(src, g, i, j, f);
+template <typename S>
+__device__ __forceinline__ NodeVals interior_vals(const S* __restrict__ src, const Grid& g, const double* w, int i,
+                                                  int j, double f[9]) {
+  pull_node_s(src, g, i, j, w, f);
   NodeVals r;
   node_moments(f, r.rho, r.u, r.v);
   return r;
 }
 
 // Characteristic update of left-column node (0, j) (boundary_condition.f90:418-439).
-__device__ NodeVals lodi_left(const double* __restrict__ src, const Grid& g, const RingBuffers& R, double cs,
-                              int j) {
+template <typename S>
+__device__ NodeVals lodi_left(const S* __restrict__ src, const Grid& g, const double* w, const RingBuffers& R,
+                              double cs, int j) {
   const int ny = g.ny;
   const double* mr = R.mom;  // rho[0..ny), u[ny..2ny), v[2ny..3ny) for the left column
   const double trj = mr[j], tuj = mr[ny + j], tvj = mr[2 * ny + j];
   double f2[9], f3[9];
-  const NodeVals n2 = interior_vals(src, g, 1, j, f2);
-  const NodeVals n3 = interior_vals(src, g, 2, j, f3);
+  const NodeVals n2 = interior_vals(src, g, w, 1, j, f2);
+  const NodeVals n3 = interior_vals(src, g, w, 2, j, f3);
   const double rhox = (-3.0 * trj + 4.0 * n2.rho - n3.rho) * 0.5;
   const double ux = (-3.0 * tuj + 4.0 * n2.u - n3.u) * 0.5;
   const double vx = (-3.0 * tvj + 4.0 * n2.v - n3.v) * 0.5;
@@ -122,14 +349,15 @@ __device__ NodeVals lodi_left(const double* __restrict__ src, const Grid& g, con
 }
 
 // Characteristic update of right-column node (nx-1, j) (boundary_condition.f90:500-521).
-__device__ NodeVals lodi_right(const double* __restrict__ src, const Grid& g, const RingBuffers& R, double cs,
-                               int j) {
+template <typename S>
+__device__ NodeVals lodi_right(const S* __restrict__ src, const Grid& g, const double* w, const RingBuffers& R,
+                               double cs, int j) {
   const int ny = g.ny, nx = g.nx;
   const double* mr = R.mom + 3 * ny;
   const double trj = mr[j], tuj = mr[ny + j], tvj = mr[2 * ny + j];
   double f2[9], f3[9];
-  const NodeVals n2 = interior_vals(src, g, nx - 2, j, f2);
-  const NodeVals n3 = interior_vals(src, g, nx - 3, j, f3);
+  const NodeVals n2 = interior_vals(src, g, w, nx - 2, j, f2);
+  const NodeVals n3 = interior_vals(src, g, w, nx - 3, j, f3);
   const double rhox = (3.0 * trj - 4.0 * n2.rho + n3.rho) * 0.5;
   const double ux = (3.0 * tuj - 4.0 * n2.u + n3.u) * 0.5;
   const double vx = (3.0 * tvj - 4.0 * n2.v + n3.v) * 0.5;
@@ -146,27 +374,28 @@ __device__ NodeVals lodi_right(const double* __restrict__ src, const Grid& g, co
 // One thread per boundary-ring node: left and right columns (j = 1..ny-2),
 // then the top and bottom walls (i = 0..nx-1), in the order and with the
 // overwrites of boundary_condition1 (boundary_condition.f90:403-853).
-template <bool FIELDS>
-__global__ __launch_bounds__(256) void k_bc_ring(Grid g, MrtParams P, RingBuffers R, const double* __restrict__ src,
-                                                 double* __restrict__ dst, FieldSlot fs) {
+template <typename S, bool FIELDS>
+__global__ __launch_bounds__(256) void k_bc_ring(Grid g, MrtParams P, RingBuffers R, const S* __restrict__ src,
+                                                 S* __restrict__ dst, FieldSlot fs) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int nx = g.nx, ny = g.ny, nside = ny - 2;
   const int total = 2 * nside + 2 * nx;
   if (t >= total) return;
   const double cs = P.cs;
+  const double* w = P.w;
   double out[9], f0n[9];
   int bi, bj;
   NodeVals b, nv;
   if (t < nside) {  // left (boundary_condition.f90:413-483)
     bj = 1 + t;
     bi = 0;
-    b = lodi_left(src, g, R, cs, bj);
-    nv = interior_vals(src, g, 1, bj, f0n);
+    b = lodi_left(src, g, w, R, cs, bj);
+    nv = interior_vals(src, g, w, 1, bj, f0n);
   } else if (t < 2 * nside) {  // right (boundary_condition.f90:495-567)
     bj = 1 + t - nside;
     bi = nx - 1;
-    b = lodi_right(src, g, R, cs, bj);
-    nv = interior_vals(src, g, nx - 2, bj, f0n);
+    b = lodi_right(src, g, w, R, cs, bj);
+    nv = interior_vals(src, g, w, nx - 2, bj, f0n);
   } else {  // top (boundary_condition.f90:657-723) / bottom (784-850) walls
     const int tt = t - 2 * nside;
     const bool top = tt < nx;
@@ -174,15 +403,15 @@ __global__ __launch_bounds__(256) void k_bc_ring(Grid g, MrtParams P, RingBuffer
     bj = top ? ny - 1 : 0;
     const int nj = top ? ny - 2 : 1;  // node inside the wall
     if (bi == 0) {
-      nv = lodi_left(src, g, R, cs, nj);
+      nv = lodi_left(src, g, w, R, cs, nj);
 #pragma unroll
       for (int k = 0; k < 9; k++) f0n[k] = R.left[(long long)k * ny + nj];
     } else if (bi == nx - 1) {
-      nv = lodi_right(src, g, R, cs, nj);
+      nv = lodi_right(src, g, w, R, cs, nj);
 #pragma unroll
       for (int k = 0; k < 9; k++) f0n[k] = R.right[(long long)k * ny + nj];
     } else {
-      nv = interior_vals(src, g, bi, nj, f0n);
+      nv = interior_vals(src, g, w, bi, nj, f0n);
     }
     b.rho = nv.rho;
     b.u = 0.0;
@@ -191,7 +420,7 @@ __global__ __launch_bounds__(256) void k_bc_ring(Grid g, MrtParams P, RingBuffer
   bc_extrapolate(P, b.rho, b.u, b.v, nv.rho, nv.u, nv.v, f0n, out);
   const long long o = (long long)bj * nx + bi;
 #pragma unroll
-  for (int k = 0; k < 9; k++) dst[(long long)k * g.ps + o] = out[k];
+  for (int k = 0; k < 9; k++) Store<S>::st(dst, (long long)k * g.ps + o, out[k], w[k]);
   if (FIELDS) {
     fs.rho[o] = b.rho;
     fs.u[o] = b.u;
@@ -199,14 +428,23 @@ __global__ __launch_bounds__(256) void k_bc_ring(Grid g, MrtParams P, RingBuffer
   }
 }
 
-void launch_bc_ring(hipStream_t s, const Grid& g, const MrtParams& P, const RingBuffers& R, const double* src,
-                    double* dst, FieldSlot* fields) {
+void launch_bc_ring(hipStream_t s, const Grid& g, const MrtParams& P, const RingBuffers& R, const void* src,
+                    void* dst, FieldSlot* fields, bool f32) {
   const int total = 2 * (g.ny - 2) + 2 * g.nx;
   dim3 grid((total + 255) / 256), block(256);
-  if (fields)
-    hipLaunchKernelGGL(k_bc_ring<true>, grid, block, 0, s, g, P, R, src, dst, *fields);
-  else
-    hipLaunchKernelGGL(k_bc_ring<false>, grid, block, 0, s, g, P, R, src, dst, FieldSlot{});
+  const FieldSlot fs = fields ? *fields : FieldSlot{};
+  if (f32) {
+    if (fields)
+      hipLaunchKernelGGL((k_bc_ring<float, true>), grid, block, 0, s, g, P, R, (const float*)src, (float*)dst, fs);
+    else
+      hipLaunchKernelGGL((k_bc_ring<float, false>), grid, block, 0, s, g, P, R, (const float*)src, (float*)dst, fs);
+  } else {
+    if (fields)
+      hipLaunchKernelGGL((k_bc_ring<double, true>), grid, block, 0, s, g, P, R, (const double*)src, (double*)dst, fs);
+    else
+      hipLaunchKernelGGL((k_bc_ring<double, false>), grid, block, 0, s, g, P, R, (const double*)src, (double*)dst,
+                         fs);
+  }
 }
 
 // Moments of the persistent f0 ring columns (physical_variables on the ring,
@@ -237,12 +475,13 @@ struct Feq9 {
   double v[9];
 };
 
-__global__ void k_init_fields(Grid g, double uu, Feq9 feq, double* f, RingBuffers R, FieldSlot slot) {
+template <typename S>
+__global__ void k_init_fields(Grid g, MrtParams P, double uu, Feq9 feq, S* f, RingBuffers R, FieldSlot slot) {
   const long long n = (long long)g.nx * g.ny;
   for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < n;
        q += (long long)gridDim.x * blockDim.x) {
 #pragma unroll
-    for (int k = 0; k < 9; k++) f[(long long)k * g.ps + q] = feq.v[k];
+    for (int k = 0; k < 9; k++) Store<S>::st(f, (long long)k * g.ps + q, feq.v[k], P.w[k]);
     slot.rho[q] = 1.0;
     slot.u[q] = uu;
     slot.v[q] = 0.0;
@@ -256,20 +495,24 @@ __global__ void k_init_fields(Grid g, double uu, Feq9 feq, double* f, RingBuffer
   }
 }
 
-void launch_init_fields(hipStream_t s, const Grid& g, double uu, const double feq[9], double* f,
-                        const RingBuffers& R, FieldSlot slot) {
+void launch_init_fields(hipStream_t s, const Grid& g, const MrtParams& P, double uu, const double feq[9], void* f,
+                        const RingBuffers& R, FieldSlot slot, bool f32) {
   Feq9 fe;
   for (int k = 0; k < 9; k++) fe.v[k] = feq[k];
-  hipLaunchKernelGGL(k_init_fields, dim3(2048), dim3(256), 0, s, g, uu, fe, f, R, slot);
+  if (f32)
+    hipLaunchKernelGGL(k_init_fields<float>, dim3(2048), dim3(256), 0, s, g, P, uu, fe, (float*)f, R, slot);
+  else
+    hipLaunchKernelGGL(k_init_fields<double>, dim3(2048), dim3(256), 0, s, g, P, uu, fe, (double*)f, R, slot);
 }
 
-__global__ void k_moments_all(Grid g, const double* __restrict__ f, FieldSlot slot) {
+template <typename S>
+__global__ void k_moments_all(Grid g, MrtParams P, const S* __restrict__ f, FieldSlot slot) {
   const long long n = (long long)g.nx * g.ny;
   for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < n;
        q += (long long)gridDim.x * blockDim.x) {
     double fo[9];
 #pragma unroll
-    for (int k = 0; k < 9; k++) fo[k] = f[(long long)k * g.ps + q];
+    for (int k = 0; k < 9; k++) fo[k] = Store<S>::ld(f, (long long)k * g.ps + q, P.w[k]);
     double rho, u, v;
     node_moments(fo, rho, u, v);
     slot.rho[q] = rho;
@@ -278,8 +521,11 @@ __global__ void k_moments_all(Grid g, const double* __restrict__ f, FieldSlot sl
   }
 }
 
-void launch_moments_all(hipStream_t s, const Grid& g, const double* f, FieldSlot slot) {
-  hipLaunchKernelGGL(k_moments_all, dim3(2048), dim3(256), 0, s, g, f, slot);
+void launch_moments_all(hipStream_t s, const Grid& g, const MrtParams& P, const void* f, FieldSlot slot, bool f32) {
+  if (f32)
+    hipLaunchKernelGGL(k_moments_all<float>, dim3(2048), dim3(256), 0, s, g, P, (const float*)f, slot);
+  else
+    hipLaunchKernelGGL(k_moments_all<double>, dim3(2048), dim3(256), 0, s, g, P, (const double*)f, slot);
 }
 
 // erranalysis (solution.f90:77-130): two fixed-order reductions over the
@@ -335,8 +581,11 @@ void launch_err_final(hipStream_t s, const double* partial, int nblocks, double*
 }
 
 // move_mesh column shift (move_mesh.f90:29-57 dr=+1, 185-213 dr=-1): out(i) =
-// in(i+dr) where that column exists, else the column is kept.
-__global__ void k_shift_x(Grid g, int dr, const double* __restrict__ in, double* __restrict__ out, int planes) {
+// in(i+dr) where that column exists, else the column is kept.  Populations
+// and fields are shifted as stored (the FP32 deviation of a shifted value is
+// the shifted deviation).
+template <typename T>
+__global__ void k_shift_x(Grid g, int dr, const T* __restrict__ in, T* __restrict__ out, int planes) {
   const long long n = (long long)g.nx * g.ny;
   for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < n * planes;
        q += (long long)gridDim.x * blockDim.x) {
@@ -349,19 +598,29 @@ __global__ void k_shift_x(Grid g, int dr, const double* __restrict__ in, double*
   }
 }
 
-void launch_shift_x(hipStream_t s, const Grid& g, int dr, const double* f, double* fshift, int planes) {
-  hipLaunchKernelGGL(k_shift_x, dim3(4096), dim3(256), 0, s, g, dr, f, fshift, planes);
+void launch_shift_x(hipStream_t s, const Grid& g, int dr, const void* f, void* fshift, int planes, bool f32) {
+  if (f32)
+    hipLaunchKernelGGL(k_shift_x<float>, dim3(4096), dim3(256), 0, s, g, dr, (const float*)f, (float*)fshift, planes);
+  else
+    hipLaunchKernelGGL(k_shift_x<double>, dim3(4096), dim3(256), 0, s, g, dr, (const double*)f, (double*)fshift,
+                       planes);
 }
 
-__global__ void k_copy_ring_column(Grid g, const double* __restrict__ f, int col, double* ring) {
+template <typename S>
+__global__ void k_copy_ring_column(Grid g, MrtParams P, const S* __restrict__ f, int col, double* ring) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= 9 * g.ny) return;
   const int k = t / g.ny, j = t - k * g.ny;
-  ring[(long long)k * g.ny + j] = f[(long long)k * g.ps + (long long)j * g.nx + col];
+  ring[(long long)k * g.ny + j] = Store<S>::ld(f, (long long)k * g.ps + (long long)j * g.nx + col, P.w[k]);
 }
 
-void launch_copy_ring_column(hipStream_t s, const Grid& g, const double* f, int col, double* ring) {
-  hipLaunchKernelGGL(k_copy_ring_column, dim3((9 * g.ny + 255) / 256), dim3(256), 0, s, g, f, col, ring);
+void launch_copy_ring_column(hipStream_t s, const Grid& g, const MrtParams& P, const void* f, int col, double* ring,
+                             bool f32) {
+  dim3 grid((9 * g.ny + 255) / 256), block(256);
+  if (f32)
+    hipLaunchKernelGGL(k_copy_ring_column<float>, grid, block, 0, s, g, P, (const float*)f, col, ring);
+  else
+    hipLaunchKernelGGL(k_copy_ring_column<double>, grid, block, 0, s, g, P, (const double*)f, col, ring);
 }
 
 }  // namespace lbm

@@ -11,7 +11,7 @@
  * file are 1-based like the reference (IDX(i,j) converts), so each formula
  * reads as in the Fortran source.
  *
- * Deliberate differences from the reference, all documented in DESIGN.md:
+ * Deliberate differences from the reference:
  *   - cuBLAS DGEMM/DGEMV and cuSOLVER DGETRF/DGETRS (boundary_condition.f90:
  *     1618-1640) are replaced by plain loops: the products sum in ascending
  *     index order skipping exact-zero terms (identical to a naive dense

@@ -1,0 +1,115 @@
+"""FP32-shifted storage (LBM_PRECISION_FP32_SHIFTED) against the FP64 CPU
+oracle on the same seeded inputs.
+
+Populations are stored and bulk-collided in FP32 as deviations f - w_k; the
+boundary ring, the adjoint nodes, the IBM solve and the force reductions stay
+FP64.  Tolerances (FP64 oracle as the reference):
+  * window-averaged per-wing forces and moments: 1e-3 relative (the bar set
+    for the FP32 path),
+  * rho, u, v after the run: 1e-4 relative L2 with wings, 1e-5 without.
+"""
+import numpy as np
+import pytest
+
+from lbm_mutiwing_amd import PRECISION_FP32_SHIFTED, Solver, preset, seeded_motions, sinusoidal_motion_table
+
+pytestmark = pytest.mark.gpu
+
+FORCE_TOL_FP32 = 1e-3
+FIELD_TOL_FP32 = 1e-4
+FIELD_TOL_FP32_NOWING = 1e-5
+
+
+@pytest.fixture(scope="module")
+def lib():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a visible MI355X"
+    from lbm_mutiwing_amd._native import load_library
+    return load_library()
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300))
+
+
+def test_fp32_no_wings(lib, oracle_mod):
+    cfg = preset("small", precision=PRECISION_FP32_SHIFTED)
+    cfg.wing_count = 0
+    g, o = Solver(cfg), oracle_mod.OracleSolver(cfg)
+    rng = np.random.default_rng(11)
+    w = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
+    f = w[:, None, None] * (1 + 0.02 * rng.normal(size=(9, cfg.ny, cfg.nx)))
+    g.set_populations(f, f)
+    o.set_populations(f, f)
+    g.advance(200)
+    o.advance(200)
+    errs = [rel(a, b) for a, b in zip(g.fields()[:3], o.fields()[:3])]
+    print("fp32 no-wing field errors (rho, u, v):", errs)
+    assert max(errs) < FIELD_TOL_FP32_NOWING
+
+
+def test_fp32_storage_roundtrip(lib):
+    """get/set_populations convert between FP64 values and FP32 deviations."""
+    cfg = preset("small", precision=PRECISION_FP32_SHIFTED)
+    g = Solver(cfg)
+    rng = np.random.default_rng(5)
+    w = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
+    f = w[:, None, None] * (1 + 0.05 * rng.normal(size=(9, cfg.ny, cfg.nx)))
+    g.set_populations(f)
+    back, _ = g.populations()
+    wf = np.array([float(np.float32(x)) for x in w])  # weights as stored (single-precision literals)
+    expect = wf[:, None, None] + (f - wf[:, None, None]).astype(np.float32).astype(np.float64)
+    assert np.array_equal(back, expect)
+
+
+@pytest.mark.parametrize("name,steps", [("small", 600), ("c1", 400)])
+def test_fp32_wings_forces(lib, oracle_mod, name, steps):
+    cfg = preset(name, precision=PRECISION_FP32_SHIFTED)
+    rows = sinusoidal_motion_table(cfg, seeded_motions(cfg, 5), steps + 2)
+    g, o = Solver(cfg), oracle_mod.OracleSolver(cfg)
+    g.set_motion_table(rows)
+    o.set_motion_table(rows)
+    rg, ro = g.advance(steps, want_rows=True), o.advance(steps, want_rows=True)
+    W = cfg.wing_count
+    start = steps // 6
+    avg_g = rg.force_rows[start:, 1:1 + 3 * W + 3].mean(0)
+    avg_o = ro.force_rows[start:, 1:1 + 3 * W + 3].mean(0)
+    scale = np.maximum(np.abs(avg_o), 1e-2 * np.abs(avg_o).max())
+    ferr = np.abs(avg_g - avg_o) / scale
+    errs = [rel(a, b) for a, b in zip(g.fields()[:3], o.fields()[:3])]
+    print(f"fp32 {name}: force rel errors {ferr.max():.3e}, field errors {errs}")
+    assert ferr.max() < FORCE_TOL_FP32
+    assert max(errs) < FIELD_TOL_FP32
+    # motion columns are FP64 on both sides
+    assert np.allclose(rg.last_row[1 + 3 * W + 3:], ro.last_row[1 + 3 * W + 3:], rtol=1e-12, atol=1e-14)
+
+
+def test_fp32_bitwise_deterministic(lib):
+    cfg = preset("small", precision=PRECISION_FP32_SHIFTED)
+    rows = sinusoidal_motion_table(cfg, seeded_motions(cfg, 21), 300)
+    out = []
+    for _ in range(2):
+        s = Solver(cfg)
+        s.set_motion_table(rows)
+        r = s.advance(250, want_rows=True)
+        out.append((s.populations()[0], r.force_rows))
+        s.close()
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+
+
+def test_fp32_move_mesh(lib, oracle_mod):
+    cfg = preset("small", precision=PRECISION_FP32_SHIFTED)
+    rows = sinusoidal_motion_table(cfg, seeded_motions(cfg, 9), 120)
+    g, o = Solver(cfg), oracle_mod.OracleSolver(cfg)
+    g.set_motion_table(rows)
+    o.set_motion_table(rows)
+    g.advance(40)
+    o.advance(40)
+    for s_ in (g, o):
+        s_.set_body_rel_pos(1, -1.0)
+    g.advance(80)
+    o.advance(80)
+    assert g.state()["movenum"] == o.state()["movenum"] == -1
+    errs = [rel(a, b) for a, b in zip(g.fields()[:3], o.fields()[:3])]
+    assert max(errs) < FIELD_TOL_FP32
