@@ -172,6 +172,28 @@ int lbm_set_body_rel_pos(lbm_handle *h, int32_t wing, double pos);
 int lbm_set_timing(lbm_handle *h, int32_t enable);
 int lbm_get_timing(lbm_handle *h, double *lattice_ms, int32_t *launches);
 
+/* Batched advance (no counterpart in the reference, which runs one episode
+ * per process; for hosts that serve many episodes on one GPU).  Advances each
+ * of the count instances in hs by `steps` steps from its own next step, with
+ * one launch per kernel phase over all instances: every launch covers a work
+ * list of (instance, block) items, so many small, uneven lattices fill the GPU
+ * together.  Each instance ends in exactly the state lbm_advance would leave
+ * it in.  All instances must share the device and the precision, and none may
+ * have a pending move_mesh (|body_rel_pos(1)| >= 1).
+ *   row_sums, last_rows  optional, count x LBM_MAX_FORCE_ROW doubles: the
+ *                        row_sum / last_row of lbm_advance for instance e at
+ *                        e*LBM_MAX_FORCE_ROW (its row_len entries, rest zero)
+ *   errs                 optional, count doubles
+ * A step error of one instance fails the call; lbm_last_error on that instance
+ * and lbm_batch_last_error name it. */
+typedef struct lbm_batch lbm_batch;
+int lbm_batch_create(lbm_batch **out);
+int lbm_batch_destroy(lbm_batch *b);
+int lbm_batch_advance(lbm_batch *b, lbm_handle *const *hs, int32_t count,
+                      int32_t steps, double *row_sums, double *last_rows,
+                      double *errs);
+int lbm_batch_last_error(const lbm_batch *b, char *buf, size_t n);
+
 /* Message of the last failed call on h (or of the last failed lbm_create when
  * h is NULL).  Always NUL-terminates when n > 0. */
 int lbm_last_error(const lbm_handle *h, char *buf, size_t n);

@@ -963,3 +963,286 @@ int lbm_set_body_rel_pos(lbm_handle* h, int32_t wing, double pos) {
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------ batched advance
+
+struct lbm_batch {
+  hipStream_t stream = nullptr, stream_ibm = nullptr, stream_geo = nullptr;
+  hipEvent_t ev_ibm = nullptr, ev_step = nullptr;
+  hipEvent_t ev_geo[2] = {nullptr, nullptr}, ev_rel[2] = {nullptr, nullptr};
+  int device = 0;
+  EpDesc* d_desc = nullptr;
+  size_t desc_cap = 0;
+  int2* d_items = nullptr;
+  size_t items_cap = 0;
+  char msg[512] = "";
+};
+
+namespace {
+
+int bfail(lbm_batch* b, int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(b->msg, sizeof(b->msg), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+#define BCHECK(b, expr)                                                                            \
+  do {                                                                                             \
+    hipError_t e_ = (expr);                                                                        \
+    if (e_ != hipSuccess) return bfail((b), LBM_E_DEVICE, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+void batch_free(lbm_batch* b) {
+  if (b->d_desc) (void)hipFree(b->d_desc);
+  if (b->d_items) (void)hipFree(b->d_items);
+  for (hipEvent_t e : {b->ev_ibm, b->ev_step, b->ev_geo[0], b->ev_geo[1], b->ev_rel[0], b->ev_rel[1]})
+    if (e) (void)hipEventDestroy(e);
+  for (hipStream_t s : {b->stream, b->stream_ibm, b->stream_geo})
+    if (s) (void)hipStreamDestroy(s);
+}
+
+// erranalysis of step ml after a call that ran steps up to ml (solution.f90:77-130)
+bool issue_err(lbm_handle* h, int ml, int nsteps, hipStream_t s) {
+  if (nsteps <= 0 || ml < 2) return false;
+  if (h->last_move_step == ml - 1) return true;
+  const int s1 = (ml - 1) % 3, s2 = (ml - 2) % 3;
+  if (h->slot_step[s1] != ml - 1 || h->slot_step[s2] != ml - 2) return false;
+  const int nb = err_blocks(h->g);
+  launch_err_partial(s, h->g, h->slot[s1], h->slot[s2], h->err_partial, nb);
+  launch_err_final(s, h->err_partial, nb, h->err_out);
+  return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lbm_batch_create(lbm_batch** out) {
+  if (!out) return LBM_E_INVALID;
+  lbm_batch* b = new lbm_batch();
+  (void)hipGetDevice(&b->device);
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&b->stream_ibm, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&b->stream_geo, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_ibm, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_step, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_geo[0], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_geo[1], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_rel[0], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_rel[1], hipEventDisableTiming) != hipSuccess) {
+    batch_free(b);
+    delete b;
+    return LBM_E_DEVICE;
+  }
+  *out = b;
+  return 0;
+}
+
+int lbm_batch_destroy(lbm_batch* b) {
+  if (!b) return 0;
+  (void)hipStreamSynchronize(b->stream);
+  (void)hipStreamSynchronize(b->stream_geo);
+  batch_free(b);
+  delete b;
+  return 0;
+}
+
+int lbm_batch_last_error(const lbm_batch* b, char* buf, size_t n) {
+  if (!b || !buf || n == 0) return LBM_E_INVALID;
+  snprintf(buf, n, "%s", b->msg);
+  return 0;
+}
+
+int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_t steps, double* row_sums,
+                      double* last_rows, double* errs) {
+  if (!b) return LBM_E_INVALID;
+  if (!hs || count < 1 || steps < 0) return bfail(b, LBM_E_INVALID, "need at least one instance and steps >= 0");
+  const bool f32 = hs[0] && hs[0]->f32;
+  int ns = 1 << 30;
+  bool wings = false;
+  for (int e = 0; e < count; e++) {
+    lbm_handle* h = hs[e];
+    if (!h) return bfail(b, LBM_E_INVALID, "instance %d is null", e);
+    for (int q = 0; q < e; q++)
+      if (hs[q] == h) return bfail(b, LBM_E_INVALID, "instance %d appears twice", e);
+    if (h->f32 != f32) return bfail(b, LBM_E_INVALID, "instance %d has a different precision", e);
+    if (h->device != b->device) return bfail(b, LBM_E_INVALID, "instance %d lives on another device", e);
+    if (h->cfg.motion_source == LBM_MOTION_FILE && !h->motion_ready && h->W > 0)
+      return bfail(b, LBM_E_STATE, "instance %d: motion table not loaded", e);
+    if (h->W > 0 && (h->body_rel_pos[0] >= 1.0 || h->body_rel_pos[0] <= -1.0))
+      return bfail(b, LBM_E_STATE, "instance %d has a pending move_mesh; advance it alone", e);
+    if (h->W > 0) {
+      wings = true;
+      ns = std::min(ns, h->nslot);
+    }
+  }
+  if (!wings) ns = 1;
+  if (steps == 0) {
+    for (int e = 0; e < count; e++) {
+      if (row_sums) std::fill(row_sums + (size_t)e * LBM_MAX_FORCE_ROW, row_sums + (size_t)(e + 1) * LBM_MAX_FORCE_ROW, 0.0);
+      if (last_rows) std::fill(last_rows + (size_t)e * LBM_MAX_FORCE_ROW, last_rows + (size_t)(e + 1) * LBM_MAX_FORCE_ROW, 0.0);
+      if (errs) errs[e] = 0.0;
+    }
+    return 0;
+  }
+  // per-instance call set-up: rows, frames (constant without moves), counters
+  std::vector<EpDesc> desc(count);
+  std::vector<int2> items;
+  BatchItems it{};
+  size_t lds_wing = 0, lds_solve = 0;
+  for (int e = 0; e < count; e++) {
+    lbm_handle* h = hs[e];
+    int rc = ensure_rows(h, (size_t)steps);
+    if (!rc) rc = ensure_frames(h, (size_t)steps);
+    if (rc) return bfail(b, rc, "instance %d: %s", e, h->msg);
+    BCHECK(b, hipStreamSynchronize(h->stream_geo));
+    h->frames.assign((size_t)steps, h->fr);
+    BCHECK(b, hipMemcpy(h->d_frame, h->frames.data(), sizeof(Frame) * steps, hipMemcpyHostToDevice));
+    BCHECK(b, hipMemset(h->d_rowsum, 0, sizeof(double) * h->row_len));
+    BCHECK(b, hipMemset(&h->d_st->error, 0, sizeof(int)));
+    EpDesc& D = desc[e];
+    D.g = h->g;
+    D.P = h->mrt;
+    D.p = h->ip;
+    D.G = h->G;
+    D.F = h->F;
+    D.R = h->R;
+    D.st = h->d_st;
+    D.fbuf[0] = h->fbuf[0];
+    D.fbuf[1] = h->fbuf[1];
+    for (int q = 0; q < 3; q++) D.fslot[q] = h->slot[q];
+    D.cur0 = h->cur;
+    D.n0 = h->n;
+    D.ns = ns;
+    D.bc0 = (int)(h->batch_count & 1u);
+    D.tiles_x = lattice_tiles_x(h->g);
+    D.bpw = adj_collide_blocks_per_wing(h->ip);
+    D.nblk_adj = std::max(h->W, 1) * D.bpw;
+    D.pad = 0;
+    if (h->W > 0) {
+      lds_wing = std::max(lds_wing, geo_wing_lds_bytes(h->SPW));
+      lds_solve = std::max(lds_solve, geo_solve_lds_bytes(h->ip));
+    }
+  }
+  // work lists: lattice, ring, adjoint, segment, spread, then the geometry
+  // lists of a full and of the last (shorter) geometry batch
+  auto add = [&](int e, int nblk) {
+    for (int q = 0; q < nblk; q++) items.push_back(make_int2(e, q));
+  };
+  size_t off[8];
+  off[0] = items.size();
+  for (int e = 0; e < count; e++) add(e, desc[e].tiles_x * (desc[e].g.ny - 2));
+  off[1] = items.size();
+  for (int e = 0; e < count; e++) add(e, bc_ring_blocks(desc[e].g));
+  off[2] = items.size();
+  for (int e = 0; e < count; e++) add(e, desc[e].nblk_adj);
+  off[3] = items.size();
+  for (int e = 0; e < count; e++)
+    if (desc[e].p.W > 0) add(e, (desc[e].p.nseg + 3) / 4);
+  off[4] = items.size();
+  for (int e = 0; e < count; e++)
+    if (desc[e].p.W > 0) add(e, spread_blocks(desc[e].p));
+  off[5] = items.size();
+  const int tail = steps % ns == 0 ? ns : steps % ns;
+  for (int e = 0; e < count; e++)
+    if (desc[e].p.W > 0) add(e, std::min(ns, steps) * desc[e].p.W);
+  off[6] = items.size();
+  for (int e = 0; e < count; e++)
+    if (desc[e].p.W > 0) add(e, tail * desc[e].p.W);
+  off[7] = items.size();
+  if ((size_t)count > b->desc_cap) {
+    if (b->d_desc) (void)hipFree(b->d_desc);
+    b->d_desc = nullptr;
+    BCHECK(b, hipMalloc((void**)&b->d_desc, sizeof(EpDesc) * count));
+    b->desc_cap = count;
+  }
+  if (items.size() > b->items_cap) {
+    if (b->d_items) (void)hipFree(b->d_items);
+    b->d_items = nullptr;
+    BCHECK(b, hipMalloc((void**)&b->d_items, sizeof(int2) * items.size()));
+    b->items_cap = items.size();
+  }
+  BCHECK(b, hipStreamSynchronize(b->stream));
+  BCHECK(b, hipMemcpy(b->d_desc, desc.data(), sizeof(EpDesc) * count, hipMemcpyHostToDevice));
+  BCHECK(b, hipMemcpy(b->d_items, items.data(), sizeof(int2) * items.size(), hipMemcpyHostToDevice));
+  const int2* di = b->d_items;
+  it.lattice = di + off[0]; it.n_lattice = (int)(off[1] - off[0]);
+  it.ring = di + off[1]; it.n_ring = (int)(off[2] - off[1]);
+  it.adj = di + off[2]; it.n_adj = (int)(off[3] - off[2]);
+  it.seg = di + off[3]; it.n_seg = (int)(off[4] - off[3]);
+  it.spread = di + off[4]; it.n_spread = (int)(off[5] - off[4]);
+  for (int k = 0; k < steps; k++) {
+    if (wings && k % ns == 0) {
+      const int set = (k / ns) & 1;
+      const bool last = k + ns >= steps;
+      BCHECK(b, hipStreamWaitEvent(b->stream_geo, b->ev_rel[set], 0));
+      if (last)
+        launch_batch_geometry(b->stream_geo, b->d_desc, di + off[6], (int)(off[7] - off[6]), k, lds_wing, lds_solve);
+      else
+        launch_batch_geometry(b->stream_geo, b->d_desc, di + off[5], (int)(off[6] - off[5]), k, lds_wing, lds_solve);
+      BCHECK(b, hipEventRecord(b->ev_geo[set], b->stream_geo));
+      BCHECK(b, hipStreamWaitEvent(b->stream_ibm, b->ev_geo[set], 0));
+    }
+    const bool fields = k >= steps - 3;
+    BCHECK(b, hipEventRecord(b->ev_step, b->stream));
+    BCHECK(b, hipStreamWaitEvent(b->stream_ibm, b->ev_step, 0));
+    launch_batch_ibm_flow(b->stream_ibm, b->d_desc, it, k, f32);
+    launch_batch_bc_ring(b->stream_ibm, b->d_desc, it, k, f32, fields);
+    BCHECK(b, hipEventRecord(b->ev_ibm, b->stream_ibm));
+    launch_batch_lattice(b->stream, b->d_desc, it, k, f32, fields);
+    BCHECK(b, hipStreamWaitEvent(b->stream, b->ev_ibm, 0));
+    launch_batch_adj_collide(b->stream, b->d_desc, it, k, f32, fields);
+    if (wings && ((k + 1) % ns == 0 || k == steps - 1)) BCHECK(b, hipEventRecord(b->ev_rel[(k / ns) & 1], b->stream));
+  }
+  // host state of every instance as if it had run the steps alone
+  std::vector<char> have_err(count, 0);
+  for (int e = 0; e < count; e++) {
+    lbm_handle* h = hs[e];
+    const int n0 = h->n, n_last = n0 + steps - 1;
+    for (int m = std::max(n0, n_last - 2); m <= n_last; m++) h->slot_step[m % 3] = m;
+    h->cur = (h->cur + steps) & 1;
+    if (h->W > 0) {
+      h->last_slot = ep_slot(desc[e], steps - 1);
+      h->batch_count += (unsigned)((steps + ns - 1) / ns);
+    }
+    h->n = n_last + 1;
+    h->flag2 = std::max(0, h->flag2 - steps);
+    have_err[e] = issue_err(h, n_last, steps, b->stream);
+  }
+  BCHECK(b, hipGetLastError());
+  BCHECK(b, hipStreamSynchronize(b->stream));
+  int first_bad = -1;
+  for (int e = 0; e < count; e++) {
+    lbm_handle* h = hs[e];
+    DevStep st_host;
+    BCHECK(b, hipMemcpy(&st_host, h->d_st, sizeof(DevStep), hipMemcpyDeviceToHost));
+    if (st_host.error != 0) {
+      fail(h, st_host.error == DERR_MOTION ? LBM_E_STATE : (st_host.error == DERR_NOT_SPD ? LBM_E_NUMERIC : LBM_E_RANGE),
+           "step failed: %s (info %d)", dev_error_text(st_host.error), st_host.error_info);
+      if (first_bad < 0) first_bad = e;
+      continue;
+    }
+    double errv = 0.0;
+    if (have_err[e] && h->last_move_step != h->n - 2)
+      BCHECK(b, hipMemcpy(&errv, h->err_out, sizeof(double), hipMemcpyDeviceToHost));
+    if (errs) errs[e] = errv;
+    if (row_sums) {
+      double* r = row_sums + (size_t)e * LBM_MAX_FORCE_ROW;
+      std::fill(r, r + LBM_MAX_FORCE_ROW, 0.0);
+      BCHECK(b, hipMemcpy(r, h->d_rowsum, sizeof(double) * h->row_len, hipMemcpyDeviceToHost));
+    }
+    if (last_rows) {
+      double* r = last_rows + (size_t)e * LBM_MAX_FORCE_ROW;
+      std::fill(r, r + LBM_MAX_FORCE_ROW, 0.0);
+      BCHECK(b, hipMemcpy(r, h->d_rows + (size_t)(steps - 1) * h->row_len, sizeof(double) * h->row_len,
+                          hipMemcpyDeviceToHost));
+    }
+  }
+  if (first_bad >= 0)
+    return bfail(b, LBM_E_RANGE, "instance %d: %s", first_bad, hs[first_bad]->msg);
+  return 0;
+}
+
+}  // extern "C"

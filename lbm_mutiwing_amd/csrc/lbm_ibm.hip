@@ -139,14 +139,14 @@ size_t geo_wing_lds_bytes(int spw) {
 // (atomicMin: lowest segment wins, which is the reference's first claimant)
 // and its band position (atomicMin: the band half-width of S is the largest
 // position spread of any node's segments).
-__global__ __launch_bounds__(GEO_THREADS) void k_geo_wing(IbmParams p, GeoBuffers G, DevStep* st, int slot0, int n0,
-                                                          int fidx0) {
+__device__ __forceinline__ void geo_wing_body(const IbmParams& p, const GeoBuffers& G, DevStep* st, int slot0,
+                                              int n0, int fidx0, int b, int w) {
   extern __shared__ unsigned long long lds_geo[];
   __shared__ WingMotionState s_m;
   __shared__ int s_rect[4];
   __shared__ int s_bw, s_total;
   __shared__ int s_part[GEO_THREADS];
-  const int b = blockIdx.x, w = blockIdx.y, t = threadIdx.x;
+  const int t = threadIdx.x;
   const int slot = slot0 + b, n = n0 + b, SPW = p.SPW;
   unsigned long long* smask = lds_geo;                       // SPW claim masks (bit = box cell)
   int* sbox = reinterpret_cast<int*>(smask + SPW);           // SPW * 4
@@ -351,6 +351,11 @@ __global__ __launch_bounds__(GEO_THREADS) void k_geo_wing(IbmParams p, GeoBuffer
   }
 }
 
+__global__ __launch_bounds__(GEO_THREADS) void k_geo_wing(IbmParams p, GeoBuffers G, DevStep* st, int slot0, int n0,
+                                                          int fidx0) {
+  geo_wing_body(p, G, st, slot0, n0, fidx0, blockIdx.x, blockIdx.y);
+}
+
 // ---------------------------------------------- geometry: S and its inverse
 
 __device__ __forceinline__ double s_entry(const int* box, const double* phx, const double* phy, int k, int l) {
@@ -441,10 +446,11 @@ __device__ void inverse_column(const double* band, const double* invd, int n, in
 
 // One workgroup per (step, wing): adjoint total, wing-coupling check, band
 // assembly of S, L D L^T and the explicit inverse (one column per thread).
-__global__ __launch_bounds__(GEO_THREADS) void k_geo_solve(IbmParams p, GeoBuffers G, DevStep* st, int slot0) {
+__device__ __forceinline__ void geo_solve_body(const IbmParams& p, const GeoBuffers& G, DevStep* st, int slot0, int b,
+                                               int w) {
   extern __shared__ double lds_s[];
   __shared__ int s_skip;
-  const int b = blockIdx.x, w = blockIdx.y, t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int slot = slot0 + b, n = p.SPW;
   GeoStep* gs = G.step + slot;
   if (t == 0) {
@@ -509,6 +515,10 @@ __global__ __launch_bounds__(GEO_THREADS) void k_geo_solve(IbmParams p, GeoBuffe
   }
 }
 
+__global__ __launch_bounds__(GEO_THREADS) void k_geo_solve(IbmParams p, GeoBuffers G, DevStep* st, int slot0) {
+  geo_solve_body(p, G, st, slot0, blockIdx.x, blockIdx.y);
+}
+
 size_t geo_solve_lds_bytes(const IbmParams& p) { return (size_t)p.SPW * (BMAX + 2) * sizeof(double); }
 
 void launch_geometry(hipStream_t s, const IbmParams& p, const GeoBuffers& G, DevStep* st, int slot0, int n0,
@@ -531,9 +541,9 @@ __device__ __forceinline__ double wave_sum(double v) {
 // lane = box cell; u* of the cell is pulled from the previous step's
 // populations with the lattice kernel's arithmetic.
 template <typename S>
-__global__ __launch_bounds__(256) void k_ibm_rhs(IbmParams p, MrtParams P, GeoBuffers G, int slot, FlowBuffers F,
-                                                 Grid g, const S* __restrict__ src) {
-  const int gk = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+__device__ __forceinline__ void ibm_rhs_body(const IbmParams& p, const MrtParams& P, const GeoBuffers& G, int slot,
+                                             const FlowBuffers& F, const Grid& g, const S* __restrict__ src, int blk) {
+  const int gk = blk * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (gk >= p.nseg) return;
   const size_t e = (size_t)slot * p.nseg + gk;
   const int* bx = G.box + e * 4;
@@ -561,9 +571,16 @@ __global__ __launch_bounds__(256) void k_ibm_rhs(IbmParams p, MrtParams P, GeoBu
   }
 }
 
+template <typename S>
+__global__ __launch_bounds__(256) void k_ibm_rhs(IbmParams p, MrtParams P, GeoBuffers G, int slot, FlowBuffers F,
+                                                 Grid g, const S* __restrict__ src) {
+  ibm_rhs_body<S>(p, P, G, slot, F, g, src, blockIdx.x);
+}
+
 // y = S^-1 c, one wave per row of the inverse.
-__global__ __launch_bounds__(256) void k_ibm_matvec(IbmParams p, GeoBuffers G, int slot, FlowBuffers F) {
-  const int gp = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+__device__ __forceinline__ void ibm_matvec_body(const IbmParams& p, const GeoBuffers& G, int slot,
+                                                const FlowBuffers& F, int blk) {
+  const int gp = blk * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (gp >= p.nseg) return;
   const int n = p.SPW, w = gp / n, r = gp - w * n;
   const double* row = G.sinv + (((size_t)slot * p.W + w) * n + r) * (size_t)n;
@@ -583,6 +600,10 @@ __global__ __launch_bounds__(256) void k_ibm_matvec(IbmParams p, GeoBuffers G, i
   }
 }
 
+__global__ __launch_bounds__(256) void k_ibm_matvec(IbmParams p, GeoBuffers G, int slot, FlowBuffers F) {
+  ibm_matvec_body(p, G, slot, F, blockIdx.x);
+}
+
 // g = D^T y at the adjoint nodes (boundary_condition.f90:1639-1651).  A
 // workgroup takes the nodes claimed by SPREAD_SEG consecutive segments; every
 // segment whose box can hold them lies within +-bw band positions of the
@@ -591,13 +612,14 @@ constexpr int SPREAD_SEG = 16;
 constexpr int SPREAD_THREADS = 128;
 constexpr int SPREAD_STAGE = 2 * SPREAD_SEG + 2 * BMAX + 2;
 
-__global__ __launch_bounds__(SPREAD_THREADS) void k_ibm_spread(IbmParams p, GeoBuffers G, int slot, FlowBuffers F) {
+__device__ __forceinline__ void ibm_spread_body(const IbmParams& p, const GeoBuffers& G, int slot,
+                                                const FlowBuffers& F, int blk) {
   __shared__ int s_box[SPREAD_STAGE][4];
   __shared__ double s_ph[SPREAD_STAGE][2 * BOXW];
   __shared__ double s_y[SPREAD_STAGE][2];
   const int n = p.SPW, t = threadIdx.x;
   const int nb = (n + SPREAD_SEG - 1) / SPREAD_SEG;
-  const int w = blockIdx.x / nb, sb = blockIdx.x - w * nb;
+  const int w = blk / nb, sb = blk - w * nb;
   const int k0 = sb * SPREAD_SEG, k1 = min(k0 + SPREAD_SEG, n);
   const GeoStep* gs = G.step + slot;
   const int bw = min(gs->bw[w], BMAX);
@@ -648,6 +670,10 @@ __global__ __launch_bounds__(SPREAD_THREADS) void k_ibm_spread(IbmParams p, GeoB
   }
 }
 
+__global__ __launch_bounds__(SPREAD_THREADS) void k_ibm_spread(IbmParams p, GeoBuffers G, int slot, FlowBuffers F) {
+  ibm_spread_body(p, G, slot, F, blockIdx.x);
+}
+
 void launch_ibm_flow(hipStream_t s, const IbmParams& p, const MrtParams& P, const GeoBuffers& G, int slot,
                      const FlowBuffers& F, const Grid& g, const void* src, bool f32) {
   if (p.W <= 0) return;
@@ -659,6 +685,66 @@ void launch_ibm_flow(hipStream_t s, const IbmParams& p, const MrtParams& P, cons
   hipLaunchKernelGGL(k_ibm_matvec, dim3(blk), dim3(256), 0, s, p, G, slot, F);
   const int nb = (p.SPW + SPREAD_SEG - 1) / SPREAD_SEG;
   hipLaunchKernelGGL(k_ibm_spread, dim3(p.W * nb), dim3(SPREAD_THREADS), 0, s, p, G, slot, F);
+}
+
+// ------------------------------------------------------- batched launches
+
+int spread_blocks(const IbmParams& p) { return p.W * ((p.SPW + SPREAD_SEG - 1) / SPREAD_SEG); }
+
+__global__ __launch_bounds__(GEO_THREADS) void kb_geo_wing(const EpDesc* __restrict__ eps,
+                                                           const int2* __restrict__ items, int k0) {
+  const int2 it = items[blockIdx.x];
+  const EpDesc& D = eps[it.x];
+  const int b = it.y / D.p.W, w = it.y - b * D.p.W;
+  geo_wing_body(D.p, D.G, D.st, ep_slot(D, k0), D.n0 + k0, k0, b, w);
+}
+
+__global__ __launch_bounds__(GEO_THREADS) void kb_geo_solve(const EpDesc* __restrict__ eps,
+                                                            const int2* __restrict__ items, int k0) {
+  const int2 it = items[blockIdx.x];
+  const EpDesc& D = eps[it.x];
+  const int b = it.y / D.p.W, w = it.y - b * D.p.W;
+  geo_solve_body(D.p, D.G, D.st, ep_slot(D, k0), b, w);
+}
+
+void launch_batch_geometry(hipStream_t s, const EpDesc* eps, const int2* items, int n_items, int k0,
+                           size_t lds_wing, size_t lds_solve) {
+  if (n_items <= 0) return;
+  hipLaunchKernelGGL(kb_geo_wing, dim3(n_items), dim3(GEO_THREADS), lds_wing, s, eps, items, k0);
+  hipLaunchKernelGGL(kb_geo_solve, dim3(n_items), dim3(GEO_THREADS), lds_solve, s, eps, items, k0);
+}
+
+template <typename S>
+__global__ __launch_bounds__(256) void kb_ibm_rhs(const EpDesc* __restrict__ eps, const int2* __restrict__ items,
+                                                  int k) {
+  const int2 it = items[blockIdx.x];
+  const EpDesc& D = eps[it.x];
+  const int cur = (D.cur0 + k) & 1;
+  ibm_rhs_body<S>(D.p, D.P, D.G, ep_slot(D, k), D.F, D.g, (const S*)D.fbuf[cur], it.y);
+}
+
+__global__ __launch_bounds__(256) void kb_ibm_matvec(const EpDesc* __restrict__ eps, const int2* __restrict__ items,
+                                                     int k) {
+  const int2 it = items[blockIdx.x];
+  const EpDesc& D = eps[it.x];
+  ibm_matvec_body(D.p, D.G, ep_slot(D, k), D.F, it.y);
+}
+
+__global__ __launch_bounds__(SPREAD_THREADS) void kb_ibm_spread(const EpDesc* __restrict__ eps,
+                                                                const int2* __restrict__ items, int k) {
+  const int2 it = items[blockIdx.x];
+  const EpDesc& D = eps[it.x];
+  ibm_spread_body(D.p, D.G, ep_slot(D, k), D.F, it.y);
+}
+
+void launch_batch_ibm_flow(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32) {
+  if (it.n_seg <= 0) return;
+  if (f32)
+    hipLaunchKernelGGL(kb_ibm_rhs<float>, dim3(it.n_seg), dim3(256), 0, s, eps, it.seg, k);
+  else
+    hipLaunchKernelGGL(kb_ibm_rhs<double>, dim3(it.n_seg), dim3(256), 0, s, eps, it.seg, k);
+  hipLaunchKernelGGL(kb_ibm_matvec, dim3(it.n_seg), dim3(256), 0, s, eps, it.seg, k);
+  hipLaunchKernelGGL(kb_ibm_spread, dim3(it.n_spread), dim3(SPREAD_THREADS), 0, s, eps, it.spread, k);
 }
 
 }  // namespace lbm

@@ -117,6 +117,41 @@ struct FieldSlot {
   double* rho; double* u; double* v;
 };
 
+// One episode of a batched advance: everything a kernel needs to run the
+// episode's part of a substep k of the call.  With ns steps per geometry
+// batch, substep k runs step n0 + k from fbuf[(cur0 + k) & 1] into the other
+// buffer with geometry slot ((bc0 + k/ns) & 1)*ns + k%ns.
+struct EpDesc {
+  Grid g;
+  MrtParams P;
+  IbmParams p;
+  GeoBuffers G;
+  FlowBuffers F;
+  RingBuffers R;
+  DevStep* st;
+  void* fbuf[2];
+  FieldSlot fslot[3];      // field slot of step n is fslot[n % 3]
+  int cur0, n0, ns, bc0;
+  int tiles_x;             // lattice row tiles
+  int bpw;                 // adjoint-collide blocks per wing
+  int nblk_adj;            // adjoint-collide blocks of the episode
+  int pad;
+};
+
+__host__ __device__ __forceinline__ int ep_slot(const EpDesc& D, int k) {
+  return ((D.bc0 + k / D.ns) & 1) * D.ns + k % D.ns;
+}
+
+// Work items of the batched launches: (episode, block of that episode's own
+// single-instance grid).
+struct BatchItems {
+  const int2* lattice; int n_lattice;
+  const int2* ring; int n_ring;
+  const int2* adj; int n_adj;
+  const int2* seg; int n_seg;        // k_ibm_rhs / k_ibm_matvec blocks
+  const int2* spread; int n_spread;
+};
+
 // Band position <-> segment in a wing of n segments: positions 0,1,2,3,...
 // hold segments 0, n-1, 1, n-2, ... so that segments facing each other on
 // the upper and lower surface are neighbours and S is a narrow band.
@@ -152,6 +187,19 @@ int adj_collide_blocks_per_wing(const IbmParams& p);
 void launch_geometry(hipStream_t s, const IbmParams& p, const GeoBuffers& G, DevStep* st, int slot0, int n0,
                      int fidx0, int count, bool with_inverse);
 size_t geo_solve_lds_bytes(const IbmParams& p);
+// Batched launches over the episodes of eps (device array); k is the substep.
+void launch_batch_lattice(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields);
+void launch_batch_bc_ring(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields);
+void launch_batch_adj_collide(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields);
+void launch_batch_ibm_flow(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32);
+// Geometry of substeps k0 .. k0+count-1 of every episode with wings; items
+// are (episode, b*W + w) for b < count.
+void launch_batch_geometry(hipStream_t s, const EpDesc* eps, const int2* items, int n_items, int k0,
+                           size_t lds_wing, size_t lds_solve);
+size_t geo_wing_lds_bytes(int spw);
+int bc_ring_blocks(const Grid& g);
+int lattice_tiles_x(const Grid& g);
+int spread_blocks(const IbmParams& p);
 // IBM flow kernels of one step.
 void launch_ibm_flow(hipStream_t s, const IbmParams& p, const MrtParams& P, const GeoBuffers& G, int slot,
                      const FlowBuffers& F, const Grid& g, const void* src, bool f32);

@@ -27,9 +27,9 @@ constexpr int LAT_BLOCK = 256;
 // follow the hardware's round-robin placement over the XCDs, which measured
 // faster than remapping tiles to XCD-contiguous runs.
 template <bool FIELDS>
-__global__ __launch_bounds__(LAT_BLOCK) void k_lattice(Grid g, MrtParams P, const double* __restrict__ src,
-                                                       double* __restrict__ dst, FieldSlot fs, int tiles_x) {
-  const int ty = blockIdx.x / tiles_x, tx = blockIdx.x - ty * tiles_x;
+__device__ __forceinline__ void lattice_body(const Grid& g, const MrtParams& P, const double* __restrict__ src,
+                                             double* __restrict__ dst, const FieldSlot& fs, int tiles_x, int blk) {
+  const int ty = blk / tiles_x, tx = blk - ty * tiles_x;
   const int i = tx * LAT_BLOCK + (int)threadIdx.x;
   const int j = 1 + ty;
   if (i < 1 || i > g.nx - 2) return;
@@ -49,14 +49,20 @@ __global__ __launch_bounds__(LAT_BLOCK) void k_lattice(Grid g, MrtParams P, cons
   }
 }
 
+template <bool FIELDS>
+__global__ __launch_bounds__(LAT_BLOCK) void k_lattice(Grid g, MrtParams P, const double* __restrict__ src,
+                                                       double* __restrict__ dst, FieldSlot fs, int tiles_x) {
+  lattice_body<FIELDS>(g, P, src, dst, fs, tiles_x, blockIdx.x);
+}
+
 // FP32 bulk step on deviations d = f - w: rho = 1 + sum d, j = sum e d (the
 // weights carry no momentum), m' = M d, and the equilibrium moments shifted
 // by M w.  Relaxation and the inverse transform are linear, so d* = M^-1 m*'
 // is the deviation of the post-collision populations.
 template <bool FIELDS>
-__global__ __launch_bounds__(LAT_BLOCK) void k_lattice_f32(Grid g, MrtParams P, const float* __restrict__ src,
-                                                           float* __restrict__ dst, FieldSlot fs, int tiles_x) {
-  const int ty = blockIdx.x / tiles_x, tx = blockIdx.x - ty * tiles_x;
+__device__ __forceinline__ void lattice_f32_body(const Grid& g, const MrtParams& P, const float* __restrict__ src,
+                                                 float* __restrict__ dst, const FieldSlot& fs, int tiles_x, int blk) {
+  const int ty = blk / tiles_x, tx = blk - ty * tiles_x;
   const int i = tx * LAT_BLOCK + (int)threadIdx.x;
   const int j = 1 + ty;
   if (i < 1 || i > g.nx - 2) return;
@@ -119,6 +125,12 @@ __global__ __launch_bounds__(LAT_BLOCK) void k_lattice_f32(Grid g, MrtParams P, 
     fs.u[o] = (double)jx / rd;
     fs.v[o] = (double)jy / rd;
   }
+}
+
+template <bool FIELDS>
+__global__ __launch_bounds__(LAT_BLOCK) void k_lattice_f32(Grid g, MrtParams P, const float* __restrict__ src,
+                                                           float* __restrict__ dst, FieldSlot fs, int tiles_x) {
+  lattice_f32_body<FIELDS>(g, P, src, dst, fs, tiles_x, blockIdx.x);
 }
 
 void launch_lattice(hipStream_t s, const Grid& g, const MrtParams& P, const void* src, void* dst, FieldSlot* fields,
@@ -216,16 +228,17 @@ __device__ void force_row(const IbmParams& p, const GeoStep* gs, DevStep* st, co
   }
 }
 
+// blk in [0, max(W,1)*bpw) is (wing, node block); nblk is the episode's block count.
 template <typename S, bool FIELDS>
-__global__ __launch_bounds__(ADJ_THREADS) void k_adj_collide(Grid g, MrtParams P, IbmParams p, GeoBuffers G, int slot,
-                                                             FlowBuffers F, DevStep* st, int n, int slot_row,
-                                                             int row_len, int bpw, const S* __restrict__ src,
-                                                             S* __restrict__ dst, FieldSlot fs) {
+__device__ __forceinline__ void adj_collide_body(const Grid& g, const MrtParams& P, const IbmParams& p,
+                                                 const GeoBuffers& G, int slot, const FlowBuffers& F, DevStep* st,
+                                                 int n, int slot_row, int row_len, int bpw, const S* __restrict__ src,
+                                                 S* __restrict__ dst, const FieldSlot& fs, int blk, int nblk) {
   __shared__ double red[3][ADJ_THREADS];
   __shared__ double s_raw[MAXW][3];
   __shared__ int s_last;
   const int t = threadIdx.x;
-  const int w = blockIdx.x / bpw, b = blockIdx.x - w * bpw;
+  const int w = blk / bpw, b = blk - w * bpw;
   const GeoStep* gs = G.step + slot;
   double fx = 0.0, fy = 0.0, fm = 0.0;
   const int cnt = p.W > 0 ? gs->cnt[w] : 0;
@@ -269,7 +282,7 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_collide(Grid g, MrtParams P
   if (t == 0) {
     for (int c = 0; c < 3; c++) F.partial[((size_t)w * bpw + b) * 3 + c] = red[c][0];
     __threadfence();
-    s_last = atomicAdd(&st->ticket, 1) == (int)gridDim.x - 1;
+    s_last = atomicAdd(&st->ticket, 1) == nblk - 1;
   }
   __syncthreads();
   if (!s_last) return;
@@ -285,6 +298,15 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_collide(Grid g, MrtParams P
   if (t != 0) return;
   force_row(p, gs, st, F, s_raw, n, slot_row, row_len);
   st->ticket = 0;
+}
+
+template <typename S, bool FIELDS>
+__global__ __launch_bounds__(ADJ_THREADS) void k_adj_collide(Grid g, MrtParams P, IbmParams p, GeoBuffers G, int slot,
+                                                             FlowBuffers F, DevStep* st, int n, int slot_row,
+                                                             int row_len, int bpw, const S* __restrict__ src,
+                                                             S* __restrict__ dst, FieldSlot fs) {
+  adj_collide_body<S, FIELDS>(g, P, p, G, slot, F, st, n, slot_row, row_len, bpw, src, dst, fs, blockIdx.x,
+                              gridDim.x);
 }
 
 template <typename S>
@@ -375,9 +397,10 @@ __device__ NodeVals lodi_right(const S* __restrict__ src, const Grid& g, const d
 // then the top and bottom walls (i = 0..nx-1), in the order and with the
 // overwrites of boundary_condition1 (boundary_condition.f90:403-853).
 template <typename S, bool FIELDS>
-__global__ __launch_bounds__(256) void k_bc_ring(Grid g, MrtParams P, RingBuffers R, const S* __restrict__ src,
-                                                 S* __restrict__ dst, FieldSlot fs) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void bc_ring_body(const Grid& g, const MrtParams& P, const RingBuffers& R,
+                                             const S* __restrict__ src, S* __restrict__ dst, const FieldSlot& fs,
+                                             int blk) {
+  const int t = blk * 256 + threadIdx.x;
   const int nx = g.nx, ny = g.ny, nside = ny - 2;
   const int total = 2 * nside + 2 * nx;
   if (t >= total) return;
@@ -426,6 +449,12 @@ __global__ __launch_bounds__(256) void k_bc_ring(Grid g, MrtParams P, RingBuffer
     fs.u[o] = b.u;
     fs.v[o] = b.v;
   }
+}
+
+template <typename S, bool FIELDS>
+__global__ __launch_bounds__(256) void k_bc_ring(Grid g, MrtParams P, RingBuffers R, const S* __restrict__ src,
+                                                 S* __restrict__ dst, FieldSlot fs) {
+  bc_ring_body<S, FIELDS>(g, P, R, src, dst, fs, blockIdx.x);
 }
 
 void launch_bc_ring(hipStream_t s, const Grid& g, const MrtParams& P, const RingBuffers& R, const void* src,
@@ -621,6 +650,80 @@ void launch_copy_ring_column(hipStream_t s, const Grid& g, const MrtParams& P, c
     hipLaunchKernelGGL(k_copy_ring_column<float>, grid, block, 0, s, g, P, (const float*)f, col, ring);
   else
     hipLaunchKernelGGL(k_copy_ring_column<double>, grid, block, 0, s, g, P, (const double*)f, col, ring);
+}
+
+// ------------------------------------------------------- batched launches
+
+int lattice_tiles_x(const Grid& g) { return (g.nx + LAT_BLOCK - 1) / LAT_BLOCK; }
+int bc_ring_blocks(const Grid& g) { return (2 * (g.ny - 2) + 2 * g.nx + 255) / 256; }
+
+template <bool F32, bool FIELDS>
+__global__ __launch_bounds__(LAT_BLOCK) void kb_lattice(const EpDesc* __restrict__ eps,
+                                                        const int2* __restrict__ items, int k) {
+  const int2 it = items[blockIdx.x];
+  const EpDesc& D = eps[it.x];
+  const int n = D.n0 + k, cur = (D.cur0 + k) & 1;
+  if (F32)
+    lattice_f32_body<FIELDS>(D.g, D.P, (const float*)D.fbuf[cur], (float*)D.fbuf[cur ^ 1], D.fslot[n % 3],
+                             D.tiles_x, it.y);
+  else
+    lattice_body<FIELDS>(D.g, D.P, (const double*)D.fbuf[cur], (double*)D.fbuf[cur ^ 1], D.fslot[n % 3],
+                         D.tiles_x, it.y);
+}
+
+template <typename S, bool FIELDS>
+__global__ __launch_bounds__(256) void kb_bc_ring(const EpDesc* __restrict__ eps, const int2* __restrict__ items,
+                                                  int k) {
+  const int2 it = items[blockIdx.x];
+  const EpDesc& D = eps[it.x];
+  const int n = D.n0 + k, cur = (D.cur0 + k) & 1;
+  bc_ring_body<S, FIELDS>(D.g, D.P, D.R, (const S*)D.fbuf[cur], (S*)D.fbuf[cur ^ 1], D.fslot[n % 3], it.y);
+}
+
+template <typename S, bool FIELDS>
+__global__ __launch_bounds__(ADJ_THREADS) void kb_adj_collide(const EpDesc* __restrict__ eps,
+                                                              const int2* __restrict__ items, int k) {
+  const int2 it = items[blockIdx.x];
+  const EpDesc& D = eps[it.x];
+  const int n = D.n0 + k, cur = (D.cur0 + k) & 1;
+  adj_collide_body<S, FIELDS>(D.g, D.P, D.p, D.G, ep_slot(D, k), D.F, D.st, n, k, 4 + 12 * D.p.W, D.bpw,
+                              (const S*)D.fbuf[cur], (S*)D.fbuf[cur ^ 1], D.fslot[n % 3], it.y, D.nblk_adj);
+}
+
+void launch_batch_lattice(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields) {
+  if (it.n_lattice <= 0) return;
+  dim3 grid(it.n_lattice), block(LAT_BLOCK);
+  if (f32) {
+    if (fields) hipLaunchKernelGGL((kb_lattice<true, true>), grid, block, 0, s, eps, it.lattice, k);
+    else hipLaunchKernelGGL((kb_lattice<true, false>), grid, block, 0, s, eps, it.lattice, k);
+  } else {
+    if (fields) hipLaunchKernelGGL((kb_lattice<false, true>), grid, block, 0, s, eps, it.lattice, k);
+    else hipLaunchKernelGGL((kb_lattice<false, false>), grid, block, 0, s, eps, it.lattice, k);
+  }
+}
+
+void launch_batch_bc_ring(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields) {
+  if (it.n_ring <= 0) return;
+  dim3 grid(it.n_ring), block(256);
+  if (f32) {
+    if (fields) hipLaunchKernelGGL((kb_bc_ring<float, true>), grid, block, 0, s, eps, it.ring, k);
+    else hipLaunchKernelGGL((kb_bc_ring<float, false>), grid, block, 0, s, eps, it.ring, k);
+  } else {
+    if (fields) hipLaunchKernelGGL((kb_bc_ring<double, true>), grid, block, 0, s, eps, it.ring, k);
+    else hipLaunchKernelGGL((kb_bc_ring<double, false>), grid, block, 0, s, eps, it.ring, k);
+  }
+}
+
+void launch_batch_adj_collide(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields) {
+  if (it.n_adj <= 0) return;
+  dim3 grid(it.n_adj), block(ADJ_THREADS);
+  if (f32) {
+    if (fields) hipLaunchKernelGGL((kb_adj_collide<float, true>), grid, block, 0, s, eps, it.adj, k);
+    else hipLaunchKernelGGL((kb_adj_collide<float, false>), grid, block, 0, s, eps, it.adj, k);
+  } else {
+    if (fields) hipLaunchKernelGGL((kb_adj_collide<double, true>), grid, block, 0, s, eps, it.adj, k);
+    else hipLaunchKernelGGL((kb_adj_collide<double, false>), grid, block, 0, s, eps, it.adj, k);
+  }
 }
 
 }  // namespace lbm

@@ -226,3 +226,60 @@ class TimedSolver(Solver):
         ms, n = ctypes.c_double(), ctypes.c_int32()
         self._check(fn(self._h, ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
+
+
+class Batch:
+    """Batched advance of several instances of the HIP library
+    (lbm_batch_advance): one launch per kernel phase over all instances."""
+
+    MAX_ROW = 4 + 12 * 8
+
+    def __init__(self, lib: Optional[ctypes.CDLL] = None):
+        if lib is None:
+            from ._native import load_library
+            lib = load_library()
+        self.lib = lib
+        lib.lbm_batch_create.restype = ctypes.c_int
+        lib.lbm_batch_create.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+        lib.lbm_batch_destroy.restype = ctypes.c_int
+        lib.lbm_batch_destroy.argtypes = [ctypes.c_void_p]
+        lib.lbm_batch_advance.restype = ctypes.c_int
+        lib.lbm_batch_advance.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p), ctypes.c_int32,
+                                          ctypes.c_int32, _dp, _dp, _dp]
+        lib.lbm_batch_last_error.restype = ctypes.c_int
+        lib.lbm_batch_last_error.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]
+        self._b = ctypes.c_void_p()
+        rc = lib.lbm_batch_create(ctypes.byref(self._b))
+        if rc != 0:
+            raise LbmError(rc, "lbm_batch_create failed")
+
+    def close(self) -> None:
+        if self._b:
+            self.lib.lbm_batch_destroy(self._b)
+            self._b = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def advance(self, solvers, steps: int):
+        """Advance every solver by `steps` steps; returns one AdvanceResult per
+        solver (row_sum, last_row, err; no per-step rows)."""
+        n = len(solvers)
+        hs = (ctypes.c_void_p * n)(*[s._h.value for s in solvers])
+        rs = np.zeros((n, self.MAX_ROW))
+        lr = np.zeros((n, self.MAX_ROW))
+        er = np.zeros(n)
+        rc = self.lib.lbm_batch_advance(self._b, hs, n, int(steps), _p(rs), _p(lr), _p(er))
+        if rc != 0:
+            buf = ctypes.create_string_buffer(512)
+            self.lib.lbm_batch_last_error(self._b, buf, 512)
+            raise LbmError(rc, buf.value.decode(errors="replace"))
+        out = []
+        for e, s in enumerate(solvers):
+            L = s.row_len
+            out.append(AdvanceResult(None, None, rs[e, :L].copy(), lr[e, :L].copy(), float(er[e]), int(steps)))
+        return out
+
