@@ -13,6 +13,18 @@ all data resident in HBM, the roofline of the fused lattice kernel measured
 with HIP events on the library's stream during the timed steps, and a CPU
 baseline (the FP64 oracle on the host cores, a bounded sample of the same
 workload).
+
+Two more workloads, each printing its own JSON line:
+  --mode batch  64 uneven episodes (lattices log-uniform 512x256 .. 2048x1024,
+                1-3 wings of chord 32-128, Re 100-500, seeded sinusoidal
+                motions), FP32-shifted, advanced together by lbm_batch_advance;
+                value = env-steps/s (1000-step STEPs completed per second over
+                all episodes), beside the plain FP64 and FP32 one-episode-at-a-
+                time GPU paths timed on a sample of the same episodes.
+  --mode serve  the episode server with continuous batching: Poisson arrivals,
+                each episode RESET 0 then STEPs of 1000 steps with per-wing
+                (domega, dxv, dyv) increments N(0, 0.1); p50/p99 STEP latency
+                (submission to FORCE reply) and aggregate env-steps/s.
 """
 from __future__ import annotations
 
@@ -32,6 +44,14 @@ BYTES_PER_NODE = {"fp64": 144.0,   # 9 population reads + 9 writes of 8 B
 
 def parse():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--mode", default="single", choices=["single", "batch", "serve"])
+    ap.add_argument("--episodes", type=int, default=None, help="batch: 64, serve: 32")
+    ap.add_argument("--substeps", type=int, default=1000, help="steps per STEP (env-step)")
+    ap.add_argument("--env-steps", type=int, default=4, help="serve: STEPs per episode")
+    ap.add_argument("--rate", type=float, default=40.0, help="serve: episode arrivals per second")
+    ap.add_argument("--chunk", type=int, default=100, help="serve: steps per scheduler iteration")
+    ap.add_argument("--hbm-gb", type=float, default=200.0, help="serve: HBM budget for episodes")
+    ap.add_argument("--base-sample", type=int, default=8, help="batch: episodes timed one at a time")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
@@ -79,6 +99,154 @@ def cpu_baseline(name, seed, budget_s):
             "kind": "port", "sample": f"{name} steps 2..{1 + steps} ({steps} steps, {t:.1f} s)"}
 
 
+def run_batch(args):
+    import numpy as np
+    from lbm_mutiwing_amd import (PRECISION_FP32_SHIFTED, PRECISION_FP64, Batch, Solver, random_episode,
+                                  seeded_motions, sinusoidal_motion_table)
+    n_ep = args.episodes or 64
+    K = args.substeps
+    rng = np.random.default_rng(args.seed)
+    cfgs = [random_episode(rng, PRECISION_FP32_SHIFTED) for _ in range(n_ep)]
+    warm = args.warmup
+    tables = [sinusoidal_motion_table(c, seeded_motions(c, args.seed + i), warm + K + 2) for i, c in enumerate(cfgs)]
+
+    def build(precision, idx):
+        out = []
+        for i in idx:
+            c = cfgs[i]
+            c.precision = precision
+            s = Solver(c)
+            s.set_motion_table(tables[i])
+            out.append(s)
+        return out
+
+    solvers = build(PRECISION_FP32_SHIFTED, range(n_ep))
+    b = Batch()
+    b.advance(solvers, warm)
+    t0 = time.perf_counter()
+    b.advance(solvers, K)
+    t_batch = time.perf_counter() - t0
+    nodes = sum(c.nx * c.ny for c in cfgs)
+    for s in solvers:
+        s.close()
+    b.close()
+    sample = list(range(min(args.base_sample, n_ep)))
+    ksample = max(50, K // 5)
+
+    def one_at_a_time(precision):
+        ss = build(precision, sample)
+        tot = 0.0
+        for s in ss:
+            s.advance(warm)
+            t = time.perf_counter()
+            s.advance(warm + ksample)
+            tot += (time.perf_counter() - t) * K / ksample
+            s.close()
+        return len(ss) / tot
+
+    base64 = one_at_a_time(PRECISION_FP64)
+    base32 = one_at_a_time(PRECISION_FP32_SHIFTED)
+    value = n_ep * K / 1000.0 / t_batch
+    return {
+        "metric": "env-steps/s (1000-step STEPs), 64 uneven episodes batched, FP32-shifted",
+        "value": round(value, 2), "unit": "env-steps/s", "n_gpus": 1, "steps": K, "warmup": warm,
+        "ms_per_step": round(t_batch * 1e3 / K, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded random episodes and sinusoidal motions)",
+        "config": {"workload": f"{n_ep} episodes, lattices 512x256..2048x1024 log-uniform, 1-3 wings, "
+                               f"chord 32-128, Re 100-500", "episodes": n_ep, "seed": args.seed,
+                   "total_nodes": nodes},
+        "aggregate_mlups": round(nodes * K / t_batch / 1e6, 1),
+        "one_at_a_time": {"fp64_env_steps_per_s": round(base64, 3), "fp32_env_steps_per_s": round(base32, 3),
+                          "sample": f"episodes 0..{len(sample) - 1}, {ksample} steps each, scaled to {K}"},
+        "speedup_vs_fp64_one_at_a_time": round(value / base64, 3),
+        "speedup_vs_fp32_one_at_a_time": round(value / base32, 3),
+    }
+
+
+def run_serve(args):
+    import numpy as np
+    from lbm_mutiwing_amd import MOTION_STREAM, PRECISION_FP32_SHIFTED, random_episode
+    from lbm_mutiwing_amd.server import EpisodeServer
+    n_ep = args.episodes or 32
+    K, S = args.substeps, args.env_steps
+    rng = np.random.default_rng(args.seed)
+    cfgs = [random_episode(rng, PRECISION_FP32_SHIFTED, MOTION_STREAM) for _ in range(n_ep)]
+    arrivals = np.cumsum(rng.exponential(1.0 / args.rate, n_ep))
+    noise = [rng.normal(0.0, 0.1, (S, 3 * c.wing_count)) for c in cfgs]
+
+    def next_command(i, k, reply):
+        """Client of episode i after its k-th reply: STEP k+1 (k < S) or STOP.
+        Per wing the target velocities of (rotation, x, y) are N(0, 0.1) around
+        the velocity that brings the wing back to its start pose within one
+        STEP, read from the motion columns of the FORCE reply."""
+        c = cfgs[i]
+        if k >= S:
+            return "STOP"
+        W = c.wing_count
+        row = np.array([float(t) for t in reply.split()[2:]])
+        T = K * c.dt
+        d = []
+        for w in range(W):
+            m = row[1 + 3 * W + 3 + 9 * w:]
+            x, xv, y, yv, a, om = m[0], m[1], m[3], m[4], m[6], m[7]
+            x0 = c.wing_x0 + w * c.wing_dx
+            tgt = noise[i][k, 3 * w:3 * w + 3] - np.array([a, x - x0, y - c.wing_y0]) / T
+            d += [tgt[0] - om, tgt[1] - xv, tgt[2] - yv]
+        return "STEP " + repr((k + 1) * K * c.dt) + "".join(" " + repr(float(v)) for v in d)
+    srv = EpisodeServer(int(args.hbm_gb * (1 << 30)), chunk=args.chunk)
+    for i, c in enumerate(cfgs):
+        srv.connect(i, c)
+    nxt = [0] * n_ep            # replies received so far per episode
+    pending = ["RESET 0.0"] * n_ep
+    busy = [False] * n_ep
+    finished = [False] * n_ep
+    lat = []
+    active_max = 0
+    t_start = time.perf_counter()
+    done_steps = 0
+    while True:
+        now = time.perf_counter() - t_start
+        for i in range(n_ep):
+            if not busy[i] and not finished[i] and pending[i] and arrivals[i] <= now:
+                srv.submit(i, pending[i])
+                busy[i] = True
+        active_max = max(active_max, len(srv.jobs) + len(srv.waiting))
+        for r in srv.poll():
+            i = r.client
+            busy[i] = False
+            cmd = pending[i].split(" ", 1)[0]
+            if cmd == "STOP":
+                finished[i] = True
+                continue
+            if not r.text.startswith("FORCE "):
+                raise RuntimeError(f"episode {i} ({cfgs[i].name}): {r.text}")
+            if cmd == "STEP":
+                lat.append(r.latency)
+                done_steps += 1
+            nxt[i] += 1
+            pending[i] = next_command(i, nxt[i] - 1, r.text)
+        if all(finished):
+            break
+        if srv.idle() and not any(busy):
+            time.sleep(0.0005)
+    wall = time.perf_counter() - t_start - float(arrivals[0])
+    srv.close()
+    lat_ms = np.array(lat) * 1e3
+    return {
+        "metric": "p50/p99 STEP latency and env-steps/s, episode server with continuous batching, FP32-shifted",
+        "value": round(done_steps / wall, 2), "unit": "env-steps/s", "n_gpus": 1, "steps": K, "warmup": 0,
+        "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded random episodes, Poisson arrivals, N(0, 0.1) velocity targets around a "
+                "return-to-start pose per STEP)",
+        "config": {"workload": f"{n_ep} episodes arriving at {args.rate}/s, RESET + {S} STEPs of {K} steps, "
+                               f"chunk {args.chunk}", "episodes": n_ep, "seed": args.seed},
+        "step_latency_ms": {"p50": round(float(np.percentile(lat_ms, 50)), 2),
+                            "p99": round(float(np.percentile(lat_ms, 99)), 2),
+                            "mean": round(float(lat_ms.mean()), 2), "n": int(lat_ms.size)},
+        "max_concurrent_jobs": active_max,
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -86,6 +254,12 @@ def main():
                           "(episodes are independent and shard one instance per GPU)"}))
         sys.exit(2)
     import torch  # noqa: F401  (device visibility check only)
+    if args.mode == "batch":
+        print(json.dumps(run_batch(args)), flush=True)
+        return
+    if args.mode == "serve":
+        print(json.dumps(run_serve(args)), flush=True)
+        return
     from lbm_mutiwing_amd import sinusoidal_motion_table
     from lbm_mutiwing_amd.solver import TimedSolver
 

@@ -184,14 +184,17 @@ int lbm_get_timing(lbm_handle *h, double *lattice_ms, int32_t *launches);
  *                        row_sum / last_row of lbm_advance for instance e at
  *                        e*LBM_MAX_FORCE_ROW (its row_len entries, rest zero)
  *   errs                 optional, count doubles
- * A step error of one instance fails the call; lbm_last_error on that instance
- * and lbm_batch_last_error name it. */
+ *   status               optional, count int32: 0, or the LBM_E_* code of an
+ *                        instance whose step failed (lbm_last_error on it)
+ * The outputs of the instances without error are filled in either case; the
+ * return value is 0 or the code of the first failing instance (also named by
+ * lbm_batch_last_error). */
 typedef struct lbm_batch lbm_batch;
 int lbm_batch_create(lbm_batch **out);
 int lbm_batch_destroy(lbm_batch *b);
 int lbm_batch_advance(lbm_batch *b, lbm_handle *const *hs, int32_t count,
                       int32_t steps, double *row_sums, double *last_rows,
-                      double *errs);
+                      double *errs, int32_t *status);
 int lbm_batch_last_error(const lbm_batch *b, char *buf, size_t n);
 
 /* Message of the last failed call on h (or of the last failed lbm_create when

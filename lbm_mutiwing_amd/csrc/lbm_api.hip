@@ -1056,7 +1056,7 @@ int lbm_batch_last_error(const lbm_batch* b, char* buf, size_t n) {
 }
 
 int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_t steps, double* row_sums,
-                      double* last_rows, double* errs) {
+                      double* last_rows, double* errs, int32_t* status) {
   if (!b) return LBM_E_INVALID;
   if (!hs || count < 1 || steps < 0) return bfail(b, LBM_E_INVALID, "need at least one instance and steps >= 0");
   const bool f32 = hs[0] && hs[0]->f32;
@@ -1079,6 +1079,8 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
     }
   }
   if (!wings) ns = 1;
+  if (status)
+    for (int e = 0; e < count; e++) status[e] = 0;
   if (steps == 0) {
     for (int e = 0; e < count; e++) {
       if (row_sums) std::fill(row_sums + (size_t)e * LBM_MAX_FORCE_ROW, row_sums + (size_t)(e + 1) * LBM_MAX_FORCE_ROW, 0.0);
@@ -1213,15 +1215,20 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
   }
   BCHECK(b, hipGetLastError());
   BCHECK(b, hipStreamSynchronize(b->stream));
-  int first_bad = -1;
+  int first_bad = -1, first_code = 0;
   for (int e = 0; e < count; e++) {
     lbm_handle* h = hs[e];
     DevStep st_host;
     BCHECK(b, hipMemcpy(&st_host, h->d_st, sizeof(DevStep), hipMemcpyDeviceToHost));
     if (st_host.error != 0) {
-      fail(h, st_host.error == DERR_MOTION ? LBM_E_STATE : (st_host.error == DERR_NOT_SPD ? LBM_E_NUMERIC : LBM_E_RANGE),
-           "step failed: %s (info %d)", dev_error_text(st_host.error), st_host.error_info);
-      if (first_bad < 0) first_bad = e;
+      const int code = fail(h, st_host.error == DERR_MOTION ? LBM_E_STATE
+                                   : (st_host.error == DERR_NOT_SPD ? LBM_E_NUMERIC : LBM_E_RANGE),
+                            "step failed: %s (info %d)", dev_error_text(st_host.error), st_host.error_info);
+      if (status) status[e] = code;
+      if (first_bad < 0) {
+        first_bad = e;
+        first_code = code;
+      }
       continue;
     }
     double errv = 0.0;
@@ -1240,8 +1247,7 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
                           hipMemcpyDeviceToHost));
     }
   }
-  if (first_bad >= 0)
-    return bfail(b, LBM_E_RANGE, "instance %d: %s", first_bad, hs[first_bad]->msg);
+  if (first_bad >= 0) return bfail(b, first_code, "instance %d: %s", first_bad, hs[first_bad]->msg);
   return 0;
 }
 

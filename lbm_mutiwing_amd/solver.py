@@ -245,7 +245,7 @@ class Batch:
         lib.lbm_batch_destroy.argtypes = [ctypes.c_void_p]
         lib.lbm_batch_advance.restype = ctypes.c_int
         lib.lbm_batch_advance.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p), ctypes.c_int32,
-                                          ctypes.c_int32, _dp, _dp, _dp]
+                                          ctypes.c_int32, _dp, _dp, _dp, _ip]
         lib.lbm_batch_last_error.restype = ctypes.c_int
         lib.lbm_batch_last_error.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]
         self._b = ctypes.c_void_p()
@@ -264,21 +264,28 @@ class Batch:
         except Exception:
             pass
 
-    def advance(self, solvers, steps: int):
+    def advance(self, solvers, steps: int, partial: bool = False):
         """Advance every solver by `steps` steps; returns one AdvanceResult per
-        solver (row_sum, last_row, err; no per-step rows)."""
+        solver (row_sum, last_row, err; no per-step rows).  A step error of any
+        instance raises LbmError, unless ``partial``: then the failing
+        instances get an LbmError object in their place."""
         n = len(solvers)
         hs = (ctypes.c_void_p * n)(*[s._h.value for s in solvers])
         rs = np.zeros((n, self.MAX_ROW))
         lr = np.zeros((n, self.MAX_ROW))
         er = np.zeros(n)
-        rc = self.lib.lbm_batch_advance(self._b, hs, n, int(steps), _p(rs), _p(lr), _p(er))
-        if rc != 0:
+        st = np.zeros(n, dtype=np.int32)
+        rc = self.lib.lbm_batch_advance(self._b, hs, n, int(steps), _p(rs), _p(lr), _p(er),
+                                        st.ctypes.data_as(_ip))
+        if rc != 0 and not (partial and np.any(st != 0)):
             buf = ctypes.create_string_buffer(512)
             self.lib.lbm_batch_last_error(self._b, buf, 512)
             raise LbmError(rc, buf.value.decode(errors="replace"))
         out = []
         for e, s in enumerate(solvers):
+            if st[e] != 0:
+                out.append(LbmError(int(st[e]), s._msg(s._h)))
+                continue
             L = s.row_len
             out.append(AdvanceResult(None, None, rs[e, :L].copy(), lr[e, :L].copy(), float(er[e]), int(steps)))
         return out
