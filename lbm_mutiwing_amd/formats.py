@@ -1,0 +1,258 @@
+"""On-disk formats of the reference run, written and read by the host.
+
+* ``force.dat`` / ``airfoil_motion.dat``: one ``(1000f15.8)`` record per step
+  (solution.f90:386-402);
+* ``motion_params.dat``: list-directed rows of 1 + 9*W columns, blank lines
+  skipped, time column non-decreasing (load_motion_table, mode.f90:104-178);
+* marker files ``boundary.dat``: ``(2f14.8)`` records, one per marker
+  (read_boundary, airfoil.f90:207-242);
+* the Tecplot flow snapshot ``F<n>.dat`` of SAVE / output (solution.f90:132-241):
+  the flow zone sampled every second node with velocity, pressure and
+  vorticity in ``(25e16.8)`` records, then the airfoil FE-triangle zone.
+
+The offline (file-mode) run loop of the reference (main.f90:374-441) is
+:func:`run_file_mode`: it drives a solver over a motion table and writes the
+two per-step files.
+"""
+from __future__ import annotations
+
+import os
+import re
+from typing import Iterable, List, Optional, TextIO
+
+import numpy as np
+
+OUTPUT_STRIDE_X = 2     # main.f90:38
+OUTPUT_STRIDE_Y = 2
+
+
+# ------------------------------------------------------------ edit descriptors
+
+def fortran_f(x: float, w: int, d: int) -> str:
+    """Fortran Fw.d: right-justified, w asterisks when the value does not fit."""
+    s = f"{float(x):{w}.{d}f}"
+    if len(s) > w:
+        return "*" * w
+    return s
+
+
+def fortran_e(x: float, w: int, d: int) -> str:
+    """Fortran Ew.d (0.ddddE+xx form), right-justified."""
+    x = float(x)
+    if x == 0.0:
+        body = "0." + "0" * d + "E+00"
+        body = ("-" if np.signbit(x) else "") + body
+    else:
+        m, e = f"{abs(x):.{d - 1}e}".split("e")
+        digits = m.replace(".", "")
+        ex = int(e) + 1
+        body = ("-" if x < 0 else "") + "0." + digits + (f"E{ex:+03d}" if abs(ex) < 100 else f"{ex:+04d}")
+    if len(body) > w:
+        return "*" * w
+    return body.rjust(w)
+
+
+def fortran_i(v: int, w: int, m: int = 1) -> str:
+    """Fortran Iw.m: at least m digits, right-justified in w columns."""
+    s = f"{abs(int(v)):0{m}d}"
+    s = ("-" if v < 0 else "") + s
+    return "*" * w if len(s) > w else s.rjust(w)
+
+
+def f15_8_record(values: Iterable[float]) -> str:
+    """One record of format '(1000f15.8)'."""
+    return "".join(fortran_f(v, 15, 8) for v in values)
+
+
+def e16_8_records(values: Iterable[float], per_line: int = 25) -> List[str]:
+    vals = list(values)
+    return ["".join(fortran_e(v, 16, 8) for v in vals[i:i + per_line]) for i in range(0, len(vals), per_line)]
+
+
+def f14_8_record(values: Iterable[float]) -> str:
+    return "".join(fortran_f(v, 14, 8) for v in values)
+
+
+def list_directed(*items) -> str:
+    """List-directed output in the usual style: a leading blank, strings as
+    they are, integers right-justified in 12 columns, reals with 17
+    significant digits."""
+    out = []
+    for it in items:
+        if isinstance(it, str):
+            out.append(it)
+        elif isinstance(it, (int, np.integer)):
+            out.append(f"{int(it):12d}")
+        else:
+            out.append(f"{float(it):25.17f}" if 1e-3 <= abs(float(it)) < 1e8 or float(it) == 0.0
+                       else f"{float(it):25.16E}")
+    return " " + "".join(out)
+
+
+# --------------------------------------------------------------- motion table
+
+class MotionFileError(ValueError):
+    pass
+
+
+def load_motion_params(path: str, wing_count: int) -> np.ndarray:
+    """load_motion_table (mode.f90:104-178): rows of 1 + 9*W columns."""
+    ncols = 1 + 9 * wing_count
+    if not os.path.exists(path):
+        raise MotionFileError(f"无法打开运动参数文件: {path}")
+    rows = []
+    with open(path) as fh:
+        for line in fh:
+            if not line.strip():
+                continue
+            toks = [t for t in re.split(r"[\s,]+", line.split("/", 1)[0].strip()) if t]
+            try:
+                vals = [float(t.replace("d", "e").replace("D", "E")) for t in toks[:ncols]]
+            except ValueError:
+                vals = []
+            if len(vals) < ncols:
+                raise MotionFileError(f"运动参数文件格式错误，列数应为: {ncols} {path}")
+            rows.append(vals)
+    if not rows:
+        raise MotionFileError(f"运动参数文件为空: {path}")
+    tab = np.array(rows, dtype=np.float64)
+    if np.any(np.diff(tab[:, 0]) < 0.0):
+        raise MotionFileError(f"运动参数文件时间列必须非递减: {path}")
+    return tab
+
+
+def write_motion_params(path: str, rows: np.ndarray) -> None:
+    with open(path, "w") as fh:
+        for r in rows:
+            fh.write(" ".join(repr(float(v)) for v in r) + "\n")
+
+
+# -------------------------------------------------------------------- markers
+
+def read_marker_file(path: str, points: Optional[int] = None) -> np.ndarray:
+    """read_boundary (airfoil.f90:229): (x, y) per record, chord units."""
+    out = []
+    with open(path) as fh:
+        for line in fh:
+            if not line.strip():
+                continue
+            x, y = line.split()[:2]
+            out.append((float(x), float(y)))
+            if points is not None and len(out) == points:
+                break
+    return np.array(out, dtype=np.float64)
+
+
+def write_marker_file(path: str, markers: np.ndarray) -> None:
+    with open(path, "w") as fh:
+        for x, y in markers:
+            fh.write(f14_8_record((x, y)) + "\n")
+
+
+# -------------------------------------------------------- per-step row files
+
+class RowFile:
+    """force.dat / airfoil_motion.dat: one '(1000f15.8)' record per row."""
+
+    def __init__(self, path: str, mode: str = "w"):
+        self.fh: TextIO = open(path, mode)
+
+    def write_rows(self, rows: np.ndarray) -> None:
+        for r in np.atleast_2d(rows):
+            self.fh.write(f15_8_record(r) + "\n")
+
+    def close(self) -> None:
+        self.fh.close()
+
+
+def run_file_mode(solver, table: np.ndarray, out_dir: str, chunk: int = 100) -> float:
+    """The offline run loop (main.f90:374-441): advance while
+    (n-1)*dt <= motion_end_time, writing force.dat and airfoil_motion.dat.
+    Returns the last erranalysis value."""
+    os.makedirs(out_dir, exist_ok=True)
+    end_time = solver.set_motion_table(table)
+    n_end = solver.cfg.n_steps_for(end_time)
+    ff = RowFile(os.path.join(out_dir, "force.dat"))
+    fm = RowFile(os.path.join(out_dir, "airfoil_motion.dat"))
+    err = 0.0
+    try:
+        while solver.state()["n_next"] <= n_end:
+            last = min(n_end, solver.state()["n_next"] + chunk - 1)
+            r = solver.advance(last, want_rows=True, want_motion=True)
+            ff.write_rows(r.force_rows)
+            fm.write_rows(r.motion_rows)
+            err = r.err
+    finally:
+        ff.close()
+        fm.close()
+    return err
+
+
+# ---------------------------------------------------------------- Tecplot
+
+def vorticity(u: np.ndarray, v: np.ndarray, uu: float, scale: float) -> np.ndarray:
+    """dv/dx - du/dy in units of U per chord (velocities in lattice units,
+    scale = chord per node), central inside and second-order one-sided on
+    the edges (solution.f90:183-207).  Arrays are [j, i]."""
+    ny, nx = u.shape
+    dvx = np.empty_like(v)
+    dvx[:, 1:-1] = (v[:, 2:] - v[:, :-2]) / (uu * 2.0 * scale)
+    dvx[:, -1] = (1.5 * v[:, -1] - 2.0 * v[:, -2] + 0.5 * v[:, -3]) / (uu * scale)
+    dvx[:, 0] = (-1.5 * v[:, 0] + 2.0 * v[:, 1] - 0.5 * v[:, 2]) / (uu * scale)
+    duy = np.empty_like(u)
+    duy[1:-1, :] = (u[2:, :] - u[:-2, :]) / (uu * 2.0 * scale)
+    duy[-1, :] = (1.5 * u[-1, :] - 2.0 * u[-2, :] + 0.5 * u[-3, :]) / (uu * scale)
+    duy[0, :] = (-1.5 * u[0, :] + 2.0 * u[1, :] - 0.5 * u[2, :]) / (uu * scale)
+    return dvx - duy
+
+
+def write_tecplot(path: str, cfg, cycle: int, u: np.ndarray, v: np.ndarray, pre: np.ndarray,
+                  xb1: np.ndarray, yb1: np.ndarray, movenum: int = 0) -> None:
+    """output (solution.f90:132-241) for one lattice: fields are [j, i]."""
+    ny, nx = u.shape
+    ox, oy = cfg.origin()
+    sc, uu = cfg.scale, cfg.uu
+    vor = vorticity(u, v, uu, sc)
+    out_nx = (nx - 1) // OUTPUT_STRIDE_X + 1
+    out_ny = (ny - 1) // OUTPUT_STRIDE_Y + 1
+    tsol = cycle * uu * sc
+    W, P = cfg.wing_count, cfg.points_per_wing
+    with open(path, "w") as fh:
+        fh.write('VARIABLES = "X" "Y" "U" "V" "Presure" "Vorticity" "body" "dvordt" \n')
+        fh.write(list_directed('ZONE T="Flow feild", I=', out_nx, ',J=', out_ny, ',SOLUTIONTIME=', tsol) + "\n")
+        jj = np.arange(0, ny, OUTPUT_STRIDE_Y)
+        ii = np.arange(0, nx, OUTPUT_STRIDE_X)
+        for j in jj:
+            y = oy + j * sc
+            for i in ii:
+                x = ox + i * sc + movenum * sc
+                fh.write(e16_8_records((x, y, u[j, i] / uu, v[j, i] / uu, pre[j, i], vor[j, i], 0.0, 0.0))[0] + "\n")
+        npts = W * P
+        fh.write(list_directed('ZONE T="airfoil", NODES=', npts + W, ' ELEMENTS,=', W * (P - 1),
+                               ', DATAPACKING=POINT, ZONETYPE=FETRIANGLE', ',SOLUTIONTIME=', tsol) + "\n")
+        for q in range(npts):
+            fh.write(f14_8_record((xb1[q] + movenum * sc, yb1[q], 255.0, 0, 0, 0, 0, 0)) + "\n")
+        for w in range(W):
+            j = w * P
+            k = j + (P - 1) // 2
+            fh.write(f14_8_record((0.5 * ((xb1[j] + movenum * sc) + (xb1[k] + movenum * sc)),
+                                   0.5 * (yb1[j] + yb1[k]), 255.0, 0, 0, 0, 0, 0)) + "\n")
+        for w in range(W):
+            j = w * P + 1
+            for k in range(j, j + P - 1):
+                fh.write("".join(fortran_i(val, 5, 3) for val in (k, k + 1, npts + w + 1)) + "\n")
+
+
+def tecplot_save_hook(out_dir: str):
+    """SAVE hook for protocol sessions: writes F<cycle>.dat into out_dir."""
+    os.makedirs(out_dir, exist_ok=True)
+
+    def hook(session, cycle):
+        s = session.solver
+        rho, u, v, pre = s.fields()
+        xb1, yb1, _, _ = s.markers()
+        name = f"F{cycle}.dat"
+        write_tecplot(os.path.join(out_dir, name), s.cfg, cycle, u, v, pre, xb1, yb1, s.state()["movenum"])
+        return name
+
+    return hook
