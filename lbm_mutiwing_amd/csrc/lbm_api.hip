@@ -675,11 +675,12 @@ int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motio
   HIPCHECK(h, hipMemsetAsync(&h->d_st->error, 0, sizeof(int), h->stream));
   HIPCHECK(h, hipStreamSynchronize(h->stream));
   const int ns = h->nslot;
-  int set = 0, slot = 0;
+  int set = 0, slot = 0, prev_set = 0, prev_slot = 0;
   for (int m = n0; m <= n_last; m++) {
     const int k = m - n0;
     if (h->W > 0 && k % ns == 0) {  // geometry of the next batch of steps, ahead of the flow
       const int cnt = std::min(ns, n_last - m + 1);
+      prev_set = set;
       set = (int)(h->batch_count++ & 1u);
       HIPCHECK(h, hipStreamWaitEvent(h->stream_geo, h->ev_rel[set], 0));
       launch_geometry(h->stream_geo, h->ip, h->G, h->d_st, set * ns, m, k, cnt, true);
@@ -695,11 +696,16 @@ int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motio
     const void* src = h->fbuf[h->cur];
     void* dst = h->fbuf[1 - h->cur];
     // IBM flow kernels on the second stream, concurrent with the full-lattice
-    // kernel; they start once the previous step has finished.
-    // The boundary ring reads only src and writes only ring nodes of dst, so
-    // it runs there as well.
+    // kernel; they start once the previous step has finished, after the
+    // previous step's force row.  The boundary ring reads only src and writes
+    // only ring nodes of dst, so it runs there as well.  A geometry slot set
+    // is released once the force row of its last step is written.
     HIPCHECK(h, hipEventRecord(h->ev_step, h->stream));
     HIPCHECK(h, hipStreamWaitEvent(h->stream_ibm, h->ev_step, 0));
+    if (k > 0) {
+      launch_force_row(h->stream_ibm, h->ip, h->G, prev_slot, h->F, h->d_st, m - 1, k - 1, h->row_len);
+      if (h->W > 0 && k % ns == 0) HIPCHECK(h, hipEventRecord(h->ev_rel[prev_set], h->stream_ibm));
+    }
     launch_ibm_flow(h->stream_ibm, h->ip, h->mrt, h->G, slot, h->F, h->g, src, h->f32);
     launch_bc_ring(h->stream_ibm, h->g, h->mrt, h->R, src, dst, fs, h->f32);
     HIPCHECK(h, hipEventRecord(h->ev_ibm, h->stream_ibm));
@@ -707,9 +713,8 @@ int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motio
     launch_lattice(h->stream, h->g, h->mrt, src, dst, fs, h->f32);
     if (h->timing) HIPCHECK(h, hipEventRecord(h->ev[2 * k + 1], h->stream));
     HIPCHECK(h, hipStreamWaitEvent(h->stream, h->ev_ibm, 0));
-    launch_adj_collide(h->stream, h->g, h->mrt, h->ip, h->G, slot, h->F, h->d_st, m, k, h->row_len, src, dst, fs,
-                       h->f32);
-    if (h->W > 0 && ((k + 1) % ns == 0 || m == n_last)) HIPCHECK(h, hipEventRecord(h->ev_rel[set], h->stream));
+    launch_adj_collide(h->stream, h->g, h->mrt, h->ip, h->G, slot, h->F, src, dst, fs, h->f32);
+    prev_slot = slot;
     if (want_fields) h->slot_step[s] = m;
     h->cur = 1 - h->cur;
     if (dr != 0) {  // move_mesh (move_mesh.f90:17-58, 171-214)
@@ -730,6 +735,10 @@ int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motio
       h->last_move_step = m;
     }
     if (h->flag2 > 0) h->flag2 = h->flag2 - 1;
+  }
+  if (nsteps > 0) {
+    launch_force_row(h->stream, h->ip, h->G, slot, h->F, h->d_st, n_last, nsteps - 1, h->row_len);
+    if (h->W > 0) HIPCHECK(h, hipEventRecord(h->ev_rel[set], h->stream));
   }
   if (nsteps > 0 && h->W > 0) h->last_slot = slot;
   h->n = std::max(n0, n_last + 1);
@@ -1121,8 +1130,7 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
     D.bc0 = (int)(h->batch_count & 1u);
     D.tiles_x = lattice_tiles_x(h->g);
     D.bpw = adj_collide_blocks_per_wing(h->ip);
-    D.nblk_adj = std::max(h->W, 1) * D.bpw;
-    D.pad = 0;
+    D.pad0 = D.pad1 = 0;
     if (h->W > 0) {
       lds_wing = std::max(lds_wing, geo_wing_lds_bytes(h->SPW));
       lds_solve = std::max(lds_solve, geo_solve_lds_bytes(h->ip));
@@ -1139,7 +1147,7 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
   off[1] = items.size();
   for (int e = 0; e < count; e++) add(e, bc_ring_blocks(desc[e].g));
   off[2] = items.size();
-  for (int e = 0; e < count; e++) add(e, desc[e].nblk_adj);
+  for (int e = 0; e < count; e++) add(e, desc[e].p.W * desc[e].bpw);
   off[3] = items.size();
   for (int e = 0; e < count; e++)
     if (desc[e].p.W > 0) add(e, (desc[e].p.nseg + 3) / 4);
@@ -1190,14 +1198,19 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
     const bool fields = k >= steps - 3;
     BCHECK(b, hipEventRecord(b->ev_step, b->stream));
     BCHECK(b, hipStreamWaitEvent(b->stream_ibm, b->ev_step, 0));
+    if (k > 0) {
+      launch_batch_force_row(b->stream_ibm, b->d_desc, count, k - 1);
+      if (wings && k % ns == 0) BCHECK(b, hipEventRecord(b->ev_rel[((k - 1) / ns) & 1], b->stream_ibm));
+    }
     launch_batch_ibm_flow(b->stream_ibm, b->d_desc, it, k, f32);
     launch_batch_bc_ring(b->stream_ibm, b->d_desc, it, k, f32, fields);
     BCHECK(b, hipEventRecord(b->ev_ibm, b->stream_ibm));
     launch_batch_lattice(b->stream, b->d_desc, it, k, f32, fields);
     BCHECK(b, hipStreamWaitEvent(b->stream, b->ev_ibm, 0));
     launch_batch_adj_collide(b->stream, b->d_desc, it, k, f32, fields);
-    if (wings && ((k + 1) % ns == 0 || k == steps - 1)) BCHECK(b, hipEventRecord(b->ev_rel[(k / ns) & 1], b->stream));
   }
+  launch_batch_force_row(b->stream, b->d_desc, count, steps - 1);
+  if (wings) BCHECK(b, hipEventRecord(b->ev_rel[((steps - 1) / ns) & 1], b->stream));
   // host state of every instance as if it had run the steps alone
   std::vector<char> have_err(count, 0);
   for (int e = 0; e < count; e++) {

@@ -33,8 +33,6 @@ struct WingMotionState {
 struct DevStep {
   int error;
   int error_info;
-  int ticket;              // completion counter of the adjoint-collide blocks
-  int pad;
   double force[MAXW][2];   // normalised per-wing force of the last step
   double moment[MAXW];
 };
@@ -134,8 +132,7 @@ struct EpDesc {
   int cur0, n0, ns, bc0;
   int tiles_x;             // lattice row tiles
   int bpw;                 // adjoint-collide blocks per wing
-  int nblk_adj;            // adjoint-collide blocks of the episode
-  int pad;
+  int pad0, pad1;
 };
 
 __host__ __device__ __forceinline__ int ep_slot(const EpDesc& D, int k) {
@@ -165,8 +162,10 @@ __host__ __device__ __forceinline__ int pos_of_seg(int k, int n) {
 void launch_lattice(hipStream_t s, const Grid& g, const MrtParams& P, const void* src, void* dst, FieldSlot* fields,
                     bool f32);
 void launch_adj_collide(hipStream_t s, const Grid& g, const MrtParams& P, const IbmParams& p, const GeoBuffers& G,
-                        int slot, const FlowBuffers& F, DevStep* st, int n, int row, int row_len, const void* src,
-                        void* dst, FieldSlot* fields, bool f32);
+                        int slot, const FlowBuffers& F, const void* src, void* dst, FieldSlot* fields, bool f32);
+// force row of step n (geometry slot, row index row) from the adjoint-collide partials
+void launch_force_row(hipStream_t s, const IbmParams& p, const GeoBuffers& G, int slot, const FlowBuffers& F,
+                      DevStep* st, int n, int row, int row_len);
 void launch_bc_ring(hipStream_t s, const Grid& g, const MrtParams& P, const RingBuffers& R, const void* src,
                     void* dst, FieldSlot* fields, bool f32);
 void launch_ring_moments(hipStream_t s, const Grid& g, const RingBuffers& R);
@@ -192,6 +191,7 @@ void launch_batch_lattice(hipStream_t s, const EpDesc* eps, const BatchItems& it
 void launch_batch_bc_ring(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields);
 void launch_batch_adj_collide(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields);
 void launch_batch_ibm_flow(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32);
+void launch_batch_force_row(hipStream_t s, const EpDesc* eps, int count, int k);
 // Geometry of substeps k0 .. k0+count-1 of every episode with wings; items
 // are (episode, b*W + w) for b < count.
 void launch_batch_geometry(hipStream_t s, const EpDesc* eps, const int2* items, int n_items, int k0,

@@ -159,17 +159,14 @@ void launch_lattice(hipStream_t s, const Grid& g, const MrtParams& P, const void
 // the body force 2 rho g (boundary_condition.f90:1645-1659, then
 // solution.f90:599-650), and reduce each wing's force and moment
 // (boundary_condition.f90:1662-1675).  Blocks are (wing, node block); each
-// writes its partial sums and the last block to finish adds them in block
-// order and writes the normalised force row (solution.f90:330-401).
+// writes its partial sums.  k_force_row adds them in block order and writes
+// the normalised force row (solution.f90:330-401); it runs off the lattice
+// stream's critical path, at the start of the next step's IBM chain.
 constexpr int ADJ_THREADS = 256;
 constexpr int ADJ_BLOCKS_PER_WING = 16;   // grid-stride over a wing's adjoint nodes
 
 int adj_collide_blocks_per_wing(const IbmParams& p) {
   return p.W > 0 ? std::min(ADJ_BLOCKS_PER_WING, (p.capw + ADJ_THREADS - 1) / ADJ_THREADS) : 1;
-}
-
-__device__ __forceinline__ double load_agent_d(const double* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ void force_row(const IbmParams& p, const GeoStep* gs, DevStep* st, const FlowBuffers& F,
@@ -228,20 +225,18 @@ __device__ void force_row(const IbmParams& p, const GeoStep* gs, DevStep* st, co
   }
 }
 
-// blk in [0, max(W,1)*bpw) is (wing, node block); nblk is the episode's block count.
+// blk in [0, W*bpw) is (wing, node block).
 template <typename S, bool FIELDS>
 __device__ __forceinline__ void adj_collide_body(const Grid& g, const MrtParams& P, const IbmParams& p,
-                                                 const GeoBuffers& G, int slot, const FlowBuffers& F, DevStep* st,
-                                                 int n, int slot_row, int row_len, int bpw, const S* __restrict__ src,
-                                                 S* __restrict__ dst, const FieldSlot& fs, int blk, int nblk) {
+                                                 const GeoBuffers& G, int slot, const FlowBuffers& F, int bpw,
+                                                 const S* __restrict__ src, S* __restrict__ dst, const FieldSlot& fs,
+                                                 int blk) {
   __shared__ double red[3][ADJ_THREADS];
-  __shared__ double s_raw[MAXW][3];
-  __shared__ int s_last;
   const int t = threadIdx.x;
   const int w = blk / bpw, b = blk - w * bpw;
   const GeoStep* gs = G.step + slot;
   double fx = 0.0, fy = 0.0, fm = 0.0;
-  const int cnt = p.W > 0 ? gs->cnt[w] : 0;
+  const int cnt = gs->cnt[w];
   for (int a = b * ADJ_THREADS + t; a < cnt; a += bpw * ADJ_THREADS) {
     const size_t la = (size_t)slot * p.cap + (size_t)w * p.capw + a;
     const int i = G.adj_ij[2 * la], j = G.adj_ij[2 * la + 1];
@@ -279,57 +274,65 @@ __device__ __forceinline__ void adj_collide_body(const Grid& g, const MrtParams&
       for (int c = 0; c < 3; c++) red[c][t] += red[c][t + h];
     __syncthreads();
   }
-  if (t == 0) {
+  if (t == 0)
     for (int c = 0; c < 3; c++) F.partial[((size_t)w * bpw + b) * 3 + c] = red[c][0];
-    __threadfence();
-    s_last = atomicAdd(&st->ticket, 1) == nblk - 1;
-  }
-  __syncthreads();
-  if (!s_last) return;
-  __threadfence();
-  // last block: per-wing sums of the block partials in block order
-  if (t < 3 * p.W) {
-    const int ww = t / 3, c = t - 3 * ww;
-    double r = 0.0;
-    for (int q = 0; q < bpw; q++) r = r + load_agent_d(&F.partial[((size_t)ww * bpw + q) * 3 + c]);
-    s_raw[ww][c] = r;
-  }
-  __syncthreads();
-  if (t != 0) return;
-  force_row(p, gs, st, F, s_raw, n, slot_row, row_len);
-  st->ticket = 0;
 }
 
 template <typename S, bool FIELDS>
 __global__ __launch_bounds__(ADJ_THREADS) void k_adj_collide(Grid g, MrtParams P, IbmParams p, GeoBuffers G, int slot,
-                                                             FlowBuffers F, DevStep* st, int n, int slot_row,
-                                                             int row_len, int bpw, const S* __restrict__ src,
+                                                             FlowBuffers F, int bpw, const S* __restrict__ src,
                                                              S* __restrict__ dst, FieldSlot fs) {
-  adj_collide_body<S, FIELDS>(g, P, p, G, slot, F, st, n, slot_row, row_len, bpw, src, dst, fs, blockIdx.x,
-                              gridDim.x);
+  adj_collide_body<S, FIELDS>(g, P, p, G, slot, F, bpw, src, dst, fs, blockIdx.x);
+}
+
+// Per-wing sums of the adjoint-collide partials in block order, then the
+// force row of step n into row slot_row (one 64-thread workgroup).
+__device__ __forceinline__ void force_row_body(const IbmParams& p, const GeoBuffers& G, int slot,
+                                               const FlowBuffers& F, DevStep* st, int n, int slot_row, int row_len,
+                                               int bpw) {
+  __shared__ double s_raw[MAXW][3];
+  const int t = threadIdx.x;
+  if (t < 3 * p.W) {
+    const int ww = t / 3, c = t - 3 * ww;
+    double r = 0.0;
+    for (int q = 0; q < bpw; q++) r = r + F.partial[((size_t)ww * bpw + q) * 3 + c];
+    s_raw[ww][c] = r;
+  }
+  __syncthreads();
+  if (t != 0) return;
+  force_row(p, G.step + slot, st, F, s_raw, n, slot_row, row_len);
+}
+
+__global__ __launch_bounds__(64) void k_force_row(IbmParams p, GeoBuffers G, int slot, FlowBuffers F, DevStep* st,
+                                                  int n, int slot_row, int row_len, int bpw) {
+  force_row_body(p, G, slot, F, st, n, slot_row, row_len, bpw);
+}
+
+void launch_force_row(hipStream_t s, const IbmParams& p, const GeoBuffers& G, int slot, const FlowBuffers& F,
+                      DevStep* st, int n, int row, int row_len) {
+  hipLaunchKernelGGL(k_force_row, dim3(1), dim3(64), 0, s, p, G, slot, F, st, n, row, row_len,
+                     adj_collide_blocks_per_wing(p));
 }
 
 template <typename S>
 void launch_adj_collide_t(hipStream_t s, const Grid& g, const MrtParams& P, const IbmParams& p, const GeoBuffers& G,
-                          int slot, const FlowBuffers& F, DevStep* st, int n, int row, int row_len, const S* src,
-                          S* dst, FieldSlot* fields) {
+                          int slot, const FlowBuffers& F, const S* src, S* dst, FieldSlot* fields) {
+  if (p.W <= 0) return;
   const int bpw = adj_collide_blocks_per_wing(p);
-  dim3 grid(std::max(p.W, 1) * bpw), block(ADJ_THREADS);
+  dim3 grid(p.W * bpw), block(ADJ_THREADS);
   if (fields)
-    hipLaunchKernelGGL((k_adj_collide<S, true>), grid, block, 0, s, g, P, p, G, slot, F, st, n, row, row_len, bpw,
-                       src, dst, *fields);
+    hipLaunchKernelGGL((k_adj_collide<S, true>), grid, block, 0, s, g, P, p, G, slot, F, bpw, src, dst, *fields);
   else
-    hipLaunchKernelGGL((k_adj_collide<S, false>), grid, block, 0, s, g, P, p, G, slot, F, st, n, row, row_len, bpw,
-                       src, dst, FieldSlot{});
+    hipLaunchKernelGGL((k_adj_collide<S, false>), grid, block, 0, s, g, P, p, G, slot, F, bpw, src, dst,
+                       FieldSlot{});
 }
 
 void launch_adj_collide(hipStream_t s, const Grid& g, const MrtParams& P, const IbmParams& p, const GeoBuffers& G,
-                        int slot, const FlowBuffers& F, DevStep* st, int n, int row, int row_len, const void* src,
-                        void* dst, FieldSlot* fields, bool f32) {
+                        int slot, const FlowBuffers& F, const void* src, void* dst, FieldSlot* fields, bool f32) {
   if (f32)
-    launch_adj_collide_t(s, g, P, p, G, slot, F, st, n, row, row_len, (const float*)src, (float*)dst, fields);
+    launch_adj_collide_t(s, g, P, p, G, slot, F, (const float*)src, (float*)dst, fields);
   else
-    launch_adj_collide_t(s, g, P, p, G, slot, F, st, n, row, row_len, (const double*)src, (double*)dst, fields);
+    launch_adj_collide_t(s, g, P, p, G, slot, F, (const double*)src, (double*)dst, fields);
 }
 
 // ------------------------------------------------------------------ far field
@@ -686,8 +689,18 @@ __global__ __launch_bounds__(ADJ_THREADS) void kb_adj_collide(const EpDesc* __re
   const int2 it = items[blockIdx.x];
   const EpDesc& D = eps[it.x];
   const int n = D.n0 + k, cur = (D.cur0 + k) & 1;
-  adj_collide_body<S, FIELDS>(D.g, D.P, D.p, D.G, ep_slot(D, k), D.F, D.st, n, k, 4 + 12 * D.p.W, D.bpw,
-                              (const S*)D.fbuf[cur], (S*)D.fbuf[cur ^ 1], D.fslot[n % 3], it.y, D.nblk_adj);
+  adj_collide_body<S, FIELDS>(D.g, D.P, D.p, D.G, ep_slot(D, k), D.F, D.bpw, (const S*)D.fbuf[cur],
+                              (S*)D.fbuf[cur ^ 1], D.fslot[n % 3], it.y);
+}
+
+// force row of substep k of every episode: one workgroup per episode
+__global__ __launch_bounds__(64) void kb_force_row(const EpDesc* __restrict__ eps, int k) {
+  const EpDesc& D = eps[blockIdx.x];
+  force_row_body(D.p, D.G, ep_slot(D, k), D.F, D.st, D.n0 + k, k, 4 + 12 * D.p.W, D.bpw);
+}
+
+void launch_batch_force_row(hipStream_t s, const EpDesc* eps, int count, int k) {
+  hipLaunchKernelGGL(kb_force_row, dim3(count), dim3(64), 0, s, eps, k);
 }
 
 void launch_batch_lattice(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields) {
