@@ -321,7 +321,7 @@ int alloc_ibm(lbm_handle* h, const std::vector<double>& hx, const std::vector<do
     return rc;
   const int bpw = adj_collide_blocks_per_wing(p);
   if ((rc = ialloc(h, &F.cvec, 2 * nseg)) || (rc = ialloc(h, &F.yvec, 2 * nseg)) ||
-      (rc = ialloc(h, &F.adj_g, 2 * cap)) || (rc = ialloc(h, &F.partial, 3 * W * (size_t)bpw)))
+      (rc = ialloc(h, &F.adj_g, 2 * cap)) || (rc = ialloc(h, &F.partial, 2 * 3 * MAXW * (size_t)bpw)))
     return rc;
   G.step_table = nullptr;
   G.stream_rows = h->d_stream;
@@ -702,18 +702,19 @@ int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motio
     // is released once the force row of its last step is written.
     HIPCHECK(h, hipEventRecord(h->ev_step, h->stream));
     HIPCHECK(h, hipStreamWaitEvent(h->stream_ibm, h->ev_step, 0));
+    launch_ibm_flow(h->stream_ibm, h->ip, h->mrt, h->G, slot, h->F, h->g, src, h->f32);
+    launch_bc_ring(h->stream_ibm, h->g, h->mrt, h->R, src, dst, fs, h->f32);
+    HIPCHECK(h, hipEventRecord(h->ev_ibm, h->stream_ibm));
+    // the previous step's force row (its partial sums are kept by step parity)
     if (k > 0) {
       launch_force_row(h->stream_ibm, h->ip, h->G, prev_slot, h->F, h->d_st, m - 1, k - 1, h->row_len);
       if (h->W > 0 && k % ns == 0) HIPCHECK(h, hipEventRecord(h->ev_rel[prev_set], h->stream_ibm));
     }
-    launch_ibm_flow(h->stream_ibm, h->ip, h->mrt, h->G, slot, h->F, h->g, src, h->f32);
-    launch_bc_ring(h->stream_ibm, h->g, h->mrt, h->R, src, dst, fs, h->f32);
-    HIPCHECK(h, hipEventRecord(h->ev_ibm, h->stream_ibm));
     if (h->timing) HIPCHECK(h, hipEventRecord(h->ev[2 * k], h->stream));
     launch_lattice(h->stream, h->g, h->mrt, src, dst, fs, h->f32);
     if (h->timing) HIPCHECK(h, hipEventRecord(h->ev[2 * k + 1], h->stream));
     HIPCHECK(h, hipStreamWaitEvent(h->stream, h->ev_ibm, 0));
-    launch_adj_collide(h->stream, h->g, h->mrt, h->ip, h->G, slot, h->F, src, dst, fs, h->f32);
+    launch_adj_collide(h->stream, h->g, h->mrt, h->ip, h->G, slot, h->F, m, src, dst, fs, h->f32);
     prev_slot = slot;
     if (want_fields) h->slot_step[s] = m;
     h->cur = 1 - h->cur;
@@ -737,6 +738,8 @@ int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motio
     if (h->flag2 > 0) h->flag2 = h->flag2 - 1;
   }
   if (nsteps > 0) {
+    HIPCHECK(h, hipEventRecord(h->ev_ibm, h->stream_ibm));   // the previous force row is done
+    HIPCHECK(h, hipStreamWaitEvent(h->stream, h->ev_ibm, 0));
     launch_force_row(h->stream, h->ip, h->G, slot, h->F, h->d_st, n_last, nsteps - 1, h->row_len);
     if (h->W > 0) HIPCHECK(h, hipEventRecord(h->ev_rel[set], h->stream));
   }
@@ -1198,17 +1201,19 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
     const bool fields = k >= steps - 3;
     BCHECK(b, hipEventRecord(b->ev_step, b->stream));
     BCHECK(b, hipStreamWaitEvent(b->stream_ibm, b->ev_step, 0));
+    launch_batch_ibm_flow(b->stream_ibm, b->d_desc, it, k, f32);
+    launch_batch_bc_ring(b->stream_ibm, b->d_desc, it, k, f32, fields);
+    BCHECK(b, hipEventRecord(b->ev_ibm, b->stream_ibm));
     if (k > 0) {
       launch_batch_force_row(b->stream_ibm, b->d_desc, count, k - 1);
       if (wings && k % ns == 0) BCHECK(b, hipEventRecord(b->ev_rel[((k - 1) / ns) & 1], b->stream_ibm));
     }
-    launch_batch_ibm_flow(b->stream_ibm, b->d_desc, it, k, f32);
-    launch_batch_bc_ring(b->stream_ibm, b->d_desc, it, k, f32, fields);
-    BCHECK(b, hipEventRecord(b->ev_ibm, b->stream_ibm));
     launch_batch_lattice(b->stream, b->d_desc, it, k, f32, fields);
     BCHECK(b, hipStreamWaitEvent(b->stream, b->ev_ibm, 0));
     launch_batch_adj_collide(b->stream, b->d_desc, it, k, f32, fields);
   }
+  BCHECK(b, hipEventRecord(b->ev_ibm, b->stream_ibm));
+  BCHECK(b, hipStreamWaitEvent(b->stream, b->ev_ibm, 0));
   launch_batch_force_row(b->stream, b->d_desc, count, steps - 1);
   if (wings) BCHECK(b, hipEventRecord(b->ev_rel[((steps - 1) / ns) & 1], b->stream));
   // host state of every instance as if it had run the steps alone

@@ -99,7 +99,7 @@ struct FlowBuffers {
   double* cvec;            // [nseg][2] right-hand side U_b - D u* by global band position
   double* yvec;            // [nseg][2] S^-1 c
   double* adj_g;           // [cap][2] velocity correction g at the adjoint nodes
-  double* partial;         // [W][blocks per wing][3] force partial sums
+  double* partial;         // [2][MAXW][blocks per wing][3] force partial sums by step parity
   double* rows;            // force rows of the current call
   double* mrows;           // airfoil_motion rows of the current call
   double* row_sum;         // sum of force rows
@@ -162,7 +162,8 @@ __host__ __device__ __forceinline__ int pos_of_seg(int k, int n) {
 void launch_lattice(hipStream_t s, const Grid& g, const MrtParams& P, const void* src, void* dst, FieldSlot* fields,
                     bool f32);
 void launch_adj_collide(hipStream_t s, const Grid& g, const MrtParams& P, const IbmParams& p, const GeoBuffers& G,
-                        int slot, const FlowBuffers& F, const void* src, void* dst, FieldSlot* fields, bool f32);
+                        int slot, const FlowBuffers& F, int n, const void* src, void* dst, FieldSlot* fields,
+                        bool f32);
 // force row of step n (geometry slot, row index row) from the adjoint-collide partials
 void launch_force_row(hipStream_t s, const IbmParams& p, const GeoBuffers& G, int slot, const FlowBuffers& F,
                       DevStep* st, int n, int row, int row_len);

@@ -169,8 +169,10 @@ int adj_collide_blocks_per_wing(const IbmParams& p) {
   return p.W > 0 ? std::min(ADJ_BLOCKS_PER_WING, (p.capw + ADJ_THREADS - 1) / ADJ_THREADS) : 1;
 }
 
-__device__ void force_row(const IbmParams& p, const GeoStep* gs, DevStep* st, const FlowBuffers& F,
-                          const double (*raw)[3], int n, int slot_row, int row_len) {
+// Normalised force row and airfoil_motion row of one step into LDS arrays
+// (one thread); the per-wing force and moment of the step go to st.
+__device__ void compute_rows(const IbmParams& p, const GeoStep* gs, DevStep* st, const double (*raw)[3], int n,
+                             double* row, double* mrow) {
   const bool ok = st->error == 0;
   const double UU = p.uu, scale = p.scale;
   double wf[MAXW][2], wmom[MAXW];
@@ -190,7 +192,6 @@ __device__ void force_row(const IbmParams& p, const GeoStep* gs, DevStep* st, co
   }
   for (int w = 0; w < p.W; w++) Mt = Mt + wmom[w];
   const double time = p.W > 0 ? gs->time : (double)(n - 1) * p.dt;
-  double* row = F.rows + (long long)slot_row * row_len;
   int col = 0;
   row[col++] = time;
   for (int w = 0; w < p.W; w++) {
@@ -213,8 +214,6 @@ __device__ void force_row(const IbmParams& p, const GeoStep* gs, DevStep* st, co
     row[col++] = m.omega;
     row[col++] = m.omgac;
   }
-  for (int c = 0; c < row_len; c++) F.row_sum[c] = F.row_sum[c] + row[c];
-  double* mrow = F.mrows + (long long)slot_row * (1 + 3 * p.W);
   int c = 0;
   mrow[c++] = time;
   for (int w = 0; w < p.W; w++) {
@@ -228,7 +227,7 @@ __device__ void force_row(const IbmParams& p, const GeoStep* gs, DevStep* st, co
 // blk in [0, W*bpw) is (wing, node block).
 template <typename S, bool FIELDS>
 __device__ __forceinline__ void adj_collide_body(const Grid& g, const MrtParams& P, const IbmParams& p,
-                                                 const GeoBuffers& G, int slot, const FlowBuffers& F, int bpw,
+                                                 const GeoBuffers& G, int slot, const FlowBuffers& F, int bpw, int n,
                                                  const S* __restrict__ src, S* __restrict__ dst, const FieldSlot& fs,
                                                  int blk) {
   __shared__ double red[3][ADJ_THREADS];
@@ -274,15 +273,17 @@ __device__ __forceinline__ void adj_collide_body(const Grid& g, const MrtParams&
       for (int c = 0; c < 3; c++) red[c][t] += red[c][t + h];
     __syncthreads();
   }
-  if (t == 0)
-    for (int c = 0; c < 3; c++) F.partial[((size_t)w * bpw + b) * 3 + c] = red[c][0];
+  if (t == 0) {
+    double* part = F.partial + (size_t)(n & 1) * MAXW * bpw * 3;
+    for (int c = 0; c < 3; c++) part[((size_t)w * bpw + b) * 3 + c] = red[c][0];
+  }
 }
 
 template <typename S, bool FIELDS>
 __global__ __launch_bounds__(ADJ_THREADS) void k_adj_collide(Grid g, MrtParams P, IbmParams p, GeoBuffers G, int slot,
-                                                             FlowBuffers F, int bpw, const S* __restrict__ src,
+                                                             FlowBuffers F, int bpw, int n, const S* __restrict__ src,
                                                              S* __restrict__ dst, FieldSlot fs) {
-  adj_collide_body<S, FIELDS>(g, P, p, G, slot, F, bpw, src, dst, fs, blockIdx.x);
+  adj_collide_body<S, FIELDS>(g, P, p, G, slot, F, bpw, n, src, dst, fs, blockIdx.x);
 }
 
 // Per-wing sums of the adjoint-collide partials in block order, then the
@@ -291,16 +292,26 @@ __device__ __forceinline__ void force_row_body(const IbmParams& p, const GeoBuff
                                                const FlowBuffers& F, DevStep* st, int n, int slot_row, int row_len,
                                                int bpw) {
   __shared__ double s_raw[MAXW][3];
+  __shared__ double s_row[LBM_MAX_FORCE_ROW];
+  __shared__ double s_mrow[1 + 3 * MAXW];
   const int t = threadIdx.x;
+  const double* part = F.partial + (size_t)(n & 1) * MAXW * bpw * 3;
   if (t < 3 * p.W) {
     const int ww = t / 3, c = t - 3 * ww;
     double r = 0.0;
-    for (int q = 0; q < bpw; q++) r = r + F.partial[((size_t)ww * bpw + q) * 3 + c];
+    for (int q = 0; q < bpw; q++) r = r + part[((size_t)ww * bpw + q) * 3 + c];
     s_raw[ww][c] = r;
   }
   __syncthreads();
-  if (t != 0) return;
-  force_row(p, G.step + slot, st, F, s_raw, n, slot_row, row_len);
+  if (t == 0) compute_rows(p, G.step + slot, st, s_raw, n, s_row, s_mrow);
+  __syncthreads();
+  double* row = F.rows + (long long)slot_row * row_len;
+  for (int c = t; c < row_len; c += blockDim.x) {
+    row[c] = s_row[c];
+    F.row_sum[c] = F.row_sum[c] + s_row[c];
+  }
+  double* mrow = F.mrows + (long long)slot_row * (1 + 3 * p.W);
+  for (int c = t; c < 1 + 3 * p.W; c += blockDim.x) mrow[c] = s_mrow[c];
 }
 
 __global__ __launch_bounds__(64) void k_force_row(IbmParams p, GeoBuffers G, int slot, FlowBuffers F, DevStep* st,
@@ -316,23 +327,24 @@ void launch_force_row(hipStream_t s, const IbmParams& p, const GeoBuffers& G, in
 
 template <typename S>
 void launch_adj_collide_t(hipStream_t s, const Grid& g, const MrtParams& P, const IbmParams& p, const GeoBuffers& G,
-                          int slot, const FlowBuffers& F, const S* src, S* dst, FieldSlot* fields) {
+                          int slot, const FlowBuffers& F, int n, const S* src, S* dst, FieldSlot* fields) {
   if (p.W <= 0) return;
   const int bpw = adj_collide_blocks_per_wing(p);
   dim3 grid(p.W * bpw), block(ADJ_THREADS);
   if (fields)
-    hipLaunchKernelGGL((k_adj_collide<S, true>), grid, block, 0, s, g, P, p, G, slot, F, bpw, src, dst, *fields);
+    hipLaunchKernelGGL((k_adj_collide<S, true>), grid, block, 0, s, g, P, p, G, slot, F, bpw, n, src, dst, *fields);
   else
-    hipLaunchKernelGGL((k_adj_collide<S, false>), grid, block, 0, s, g, P, p, G, slot, F, bpw, src, dst,
+    hipLaunchKernelGGL((k_adj_collide<S, false>), grid, block, 0, s, g, P, p, G, slot, F, bpw, n, src, dst,
                        FieldSlot{});
 }
 
 void launch_adj_collide(hipStream_t s, const Grid& g, const MrtParams& P, const IbmParams& p, const GeoBuffers& G,
-                        int slot, const FlowBuffers& F, const void* src, void* dst, FieldSlot* fields, bool f32) {
+                        int slot, const FlowBuffers& F, int n, const void* src, void* dst, FieldSlot* fields,
+                        bool f32) {
   if (f32)
-    launch_adj_collide_t(s, g, P, p, G, slot, F, (const float*)src, (float*)dst, fields);
+    launch_adj_collide_t(s, g, P, p, G, slot, F, n, (const float*)src, (float*)dst, fields);
   else
-    launch_adj_collide_t(s, g, P, p, G, slot, F, (const double*)src, (double*)dst, fields);
+    launch_adj_collide_t(s, g, P, p, G, slot, F, n, (const double*)src, (double*)dst, fields);
 }
 
 // ------------------------------------------------------------------ far field
@@ -689,7 +701,7 @@ __global__ __launch_bounds__(ADJ_THREADS) void kb_adj_collide(const EpDesc* __re
   const int2 it = items[blockIdx.x];
   const EpDesc& D = eps[it.x];
   const int n = D.n0 + k, cur = (D.cur0 + k) & 1;
-  adj_collide_body<S, FIELDS>(D.g, D.P, D.p, D.G, ep_slot(D, k), D.F, D.bpw, (const S*)D.fbuf[cur],
+  adj_collide_body<S, FIELDS>(D.g, D.P, D.p, D.G, ep_slot(D, k), D.F, D.bpw, n, (const S*)D.fbuf[cur],
                               (S*)D.fbuf[cur ^ 1], D.fslot[n % 3], it.y);
 }
 
