@@ -99,6 +99,11 @@ def cpu_baseline(name, seed, budget_s):
             "kind": "port", "sample": f"{name} steps 2..{1 + steps} ({steps} steps, {t:.1f} s)"}
 
 
+def note(msg):
+    """Progress line on stderr (stdout carries only the JSON result)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def run_batch(args):
     import numpy as np
     from lbm_mutiwing_amd import (PRECISION_FP32_SHIFTED, PRECISION_FP64, Batch, Solver, random_episode,
@@ -108,7 +113,8 @@ def run_batch(args):
     rng = np.random.default_rng(args.seed)
     cfgs = [random_episode(rng, PRECISION_FP32_SHIFTED) for _ in range(n_ep)]
     warm = args.warmup
-    tables = [sinusoidal_motion_table(c, seeded_motions(c, args.seed + i), warm + K + 2) for i, c in enumerate(cfgs)]
+    tables = [sinusoidal_motion_table(c, seeded_motions(c, args.seed + i), warm + 3 * K + 2)
+              for i, c in enumerate(cfgs)]
 
     def build(precision, idx):
         out = []
@@ -122,10 +128,15 @@ def run_batch(args):
 
     solvers = build(PRECISION_FP32_SHIFTED, range(n_ep))
     b = Batch()
+    note(f"batch: {n_ep} episodes built, warm-up of {warm} steps")
     b.advance(solvers, warm)
+    note(f"batch: untimed STEP of {K} steps")
+    b.advance(solvers, K)
+    note(f"batch: timed STEP of {K} steps")
     t0 = time.perf_counter()
     b.advance(solvers, K)
     t_batch = time.perf_counter() - t0
+    tile = b.tuning()
     nodes = sum(c.nx * c.ny for c in cfgs)
     for s in solvers:
         s.close()
@@ -134,6 +145,7 @@ def run_batch(args):
     ksample = max(50, K // 5)
 
     def one_at_a_time(precision):
+        note(f"batch: one episode at a time, precision {precision}")
         ss = build(precision, sample)
         tot = 0.0
         for s in ss:
@@ -156,6 +168,7 @@ def run_batch(args):
                                f"chord 32-128, Re 100-500", "episodes": n_ep, "seed": args.seed,
                    "total_nodes": nodes},
         "aggregate_mlups": round(nodes * K / t_batch / 1e6, 1),
+        "lattice_tile": tile, "timed": "second STEP of the same episodes",
         "one_at_a_time": {"fp64_env_steps_per_s": round(base64, 3), "fp32_env_steps_per_s": round(base32, 3),
                           "sample": f"episodes 0..{len(sample) - 1}, {ksample} steps each, scaled to {K}"},
         "speedup_vs_fp64_one_at_a_time": round(value / base64, 3),
@@ -217,6 +230,7 @@ def run_serve(args):
             cmd = pending[i].split(" ", 1)[0]
             if cmd == "STOP":
                 finished[i] = True
+                note(f"serve: episode {i} stopped ({sum(finished)}/{n_ep})")
                 continue
             if not r.text.startswith("FORCE "):
                 raise RuntimeError(f"episode {i} ({cfgs[i].name}): {r.text}")
@@ -276,6 +290,7 @@ def main():
     s.set_timing(True)
     s.advance(args.warmup + 2 * args.steps)
     lat_ms, launches = s.timing()
+    tile = s.tuning()
     st = s.state()
     nodes = cfg.nx * cfg.ny
     mlups = nodes * args.steps / dt / 1e6
@@ -309,7 +324,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_lattice (fused pull-stream + moments + MRT collide)",
-                     "kernel_avg_us": round(lat_avg_ms * 1e3, 2),
+                     "kernel_avg_us": round(lat_avg_ms * 1e3, 2), "tile": tile,
                      "algorithmic_bytes_per_launch": bpn * interior},
         "last_force_row": [round(x, 8) for x in r.last_row[:10]],
     }

@@ -188,7 +188,15 @@ int lbm_get_timing(lbm_handle *h, double *lattice_ms, int32_t *launches);
  *                        instance whose step failed (lbm_last_error on it)
  * The outputs of the instances without error are filled in either case; the
  * return value is 0 or the code of the first failing instance (also named by
- * lbm_batch_last_error). */
+ * lbm_batch_last_error).
+ * Launch graphs: an advance of at most 64 steps, and with wings at most one
+ * geometry batch (the 1-16 steps whose markers, adjoint lists and inverse
+ * IBM matrices are computed together), whose (instances, steps, work lists)
+ * were seen once before on this batch object is captured as a HIP graph on
+ * its second occurrence and replayed from then on (up to eight graphs are
+ * kept, least recently used evicted); other advances and first occurrences
+ * are issued launch by launch.  LBM_BATCH_GRAPHS=0 in the
+ * environment turns capture off.  Results are identical either way. */
 typedef struct lbm_batch lbm_batch;
 int lbm_batch_create(lbm_batch **out);
 int lbm_batch_destroy(lbm_batch *b);
@@ -196,6 +204,15 @@ int lbm_batch_advance(lbm_batch *b, lbm_handle *const *hs, int32_t count,
                       int32_t steps, double *row_sums, double *last_rows,
                       double *errs, int32_t *status);
 int lbm_batch_last_error(const lbm_batch *b, char *buf, size_t n);
+
+/* Autotuner result (no counterpart in the reference): the tile width of the
+ * fused lattice kernel (128, 256 or 512 nodes per workgroup), chosen by
+ * timing every width on the instance's own lattice at its first
+ * lbm_advance (0 before that; LBM_LATTICE_TILE=<width> in the environment
+ * fixes it).  lbm_batch_get_tuning gives the width a batch chose on its first
+ * lbm_batch_advance. */
+int lbm_get_tuning(const lbm_handle *h, int32_t *lattice_tile);
+int lbm_batch_get_tuning(const lbm_batch *b, int32_t *lattice_tile);
 
 /* Message of the last failed call on h (or of the last failed lbm_create when
  * h is NULL).  Always NUL-terminates when n > 0. */

@@ -107,7 +107,8 @@ struct lbm_handle {
   hipStream_t stream = nullptr;      // lattice kernels, copies
   hipStream_t stream_ibm = nullptr;  // flow part of the IBM, overlapped with the lattice kernel
   hipStream_t stream_geo = nullptr;  // IBM geometry of the coming steps
-  hipEvent_t ev_ibm = nullptr, ev_step = nullptr;
+  hipStream_t stream_bc = nullptr;   // boundary ring, concurrent with the IBM flow chain
+  hipEvent_t ev_ibm = nullptr, ev_step = nullptr, ev_bc = nullptr;
   hipEvent_t ev_geo[2] = {nullptr, nullptr};  // geometry of a slot set is ready
   hipEvent_t ev_rel[2] = {nullptr, nullptr};  // last step using a slot set has finished
   int device = 0;
@@ -150,6 +151,7 @@ struct lbm_handle {
   bool motion_ready = false;
   double motion_end_time = 0.0;
   std::vector<Frame> frames;         // per step of the current call
+  int lat_bx = 0;                    // lattice tile width (0: not tuned yet)
   // lattice-kernel timing (lbm_set_timing)
   bool timing = false;
   std::vector<hipEvent_t> ev;        // start/stop pairs of the current call
@@ -254,6 +256,8 @@ void free_all(lbm_handle* h) {
   }
   if (h->stream_ibm) (void)hipStreamDestroy(h->stream_ibm);
   if (h->stream_geo) (void)hipStreamDestroy(h->stream_geo);
+  if (h->stream_bc) (void)hipStreamDestroy(h->stream_bc);
+  if (h->ev_bc) (void)hipEventDestroy(h->ev_bc);
   for (auto e : h->ev) (void)hipEventDestroy(e);
   h->ev.clear();
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -285,6 +289,48 @@ int ensure_frames(lbm_handle* h, size_t steps) {
   h->frame_cap = cap;
   h->G.frame = h->d_frame;
   return 0;
+}
+
+// Lattice-kernel autotuner: every tile width of kLatticeTiles is timed on
+// launch() (three launches after one warm-up, HIP events on stream s) and the
+// fastest is returned.  LBM_LATTICE_TILE=<width> skips the timing.
+template <typename F>
+int autotune_tile(hipStream_t s, F launch) {
+  if (const char* e = getenv("LBM_LATTICE_TILE")) {
+    const int v = atoi(e);
+    for (int bx : kLatticeTiles)
+      if (bx == v) return v;
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return 256;
+  int best = 256;
+  float best_ms = 1e30f;
+  for (int bx : kLatticeTiles) {
+    launch(bx);
+    (void)hipEventRecord(e0, s);
+    for (int r = 0; r < 3; r++) launch(bx);
+    (void)hipEventRecord(e1, s);
+    (void)hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, e0, e1) == hipSuccess && ms < best_ms) {
+      best_ms = ms;
+      best = bx;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return best;
+}
+
+// Tile width of one instance, timed on its own buffers: the launches write
+// interior nodes of the idle population buffer, which the next step
+// overwrites entirely.
+void tune_instance(lbm_handle* h) {
+  if (h->lat_bx) return;
+  const void* src = h->fbuf[h->cur];
+  void* dst = h->fbuf[1 - h->cur];
+  h->lat_bx = autotune_tile(h->stream, [&](int bx) { launch_lattice(h->stream, h->g, h->mrt, src, dst, nullptr,
+                                                                     h->f32, bx); });
 }
 
 // Per-slot geometry arrays and flow buffers of the IBM.  The number of steps
@@ -439,6 +485,8 @@ int lbm_create(const lbm_config* cfg, lbm_handle** out) {
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&h->stream_ibm, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&h->stream_geo, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&h->stream_bc, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&h->ev_bc, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&h->ev_ibm, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&h->ev_step, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&h->ev_geo[0], hipEventDisableTiming) != hipSuccess ||
@@ -671,6 +719,7 @@ int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motio
     HIPCHECK(h, hipStreamSynchronize(h->stream_geo));
     HIPCHECK(h, hipMemcpy(h->d_frame, h->frames.data(), sizeof(Frame) * nsteps, hipMemcpyHostToDevice));
   }
+  if (nsteps > 0) tune_instance(h);
   HIPCHECK(h, hipMemsetAsync(h->d_rowsum, 0, sizeof(double) * h->row_len, h->stream));
   HIPCHECK(h, hipMemsetAsync(&h->d_st->error, 0, sizeof(int), h->stream));
   HIPCHECK(h, hipStreamSynchronize(h->stream));
@@ -695,15 +744,19 @@ int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motio
     FieldSlot* fs = want_fields ? &h->slot[s] : nullptr;
     const void* src = h->fbuf[h->cur];
     void* dst = h->fbuf[1 - h->cur];
-    // IBM flow kernels on the second stream, concurrent with the full-lattice
-    // kernel; they start once the previous step has finished, after the
-    // previous step's force row.  The boundary ring reads only src and writes
-    // only ring nodes of dst, so it runs there as well.  A geometry slot set
-    // is released once the force row of its last step is written.
+    // IBM flow kernels on the second stream and the boundary ring on a third,
+    // concurrent with the full-lattice kernel; they start once the previous
+    // step has finished (the ring reads only src and writes only ring nodes
+    // of dst).  The lattice stream waits for both through ev_ibm.  The
+    // previous step's force row follows on the IBM stream; a geometry slot
+    // set is released once the force row of its last step is written.
     HIPCHECK(h, hipEventRecord(h->ev_step, h->stream));
     HIPCHECK(h, hipStreamWaitEvent(h->stream_ibm, h->ev_step, 0));
+    HIPCHECK(h, hipStreamWaitEvent(h->stream_bc, h->ev_step, 0));
+    launch_bc_ring(h->stream_bc, h->g, h->mrt, h->R, src, dst, fs, h->f32);
+    HIPCHECK(h, hipEventRecord(h->ev_bc, h->stream_bc));
     launch_ibm_flow(h->stream_ibm, h->ip, h->mrt, h->G, slot, h->F, h->g, src, h->f32);
-    launch_bc_ring(h->stream_ibm, h->g, h->mrt, h->R, src, dst, fs, h->f32);
+    HIPCHECK(h, hipStreamWaitEvent(h->stream_ibm, h->ev_bc, 0));
     HIPCHECK(h, hipEventRecord(h->ev_ibm, h->stream_ibm));
     // the previous step's force row (its partial sums are kept by step parity)
     if (k > 0) {
@@ -711,7 +764,7 @@ int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motio
       if (h->W > 0 && k % ns == 0) HIPCHECK(h, hipEventRecord(h->ev_rel[prev_set], h->stream_ibm));
     }
     if (h->timing) HIPCHECK(h, hipEventRecord(h->ev[2 * k], h->stream));
-    launch_lattice(h->stream, h->g, h->mrt, src, dst, fs, h->f32);
+    launch_lattice(h->stream, h->g, h->mrt, src, dst, fs, h->f32, h->lat_bx);
     if (h->timing) HIPCHECK(h, hipEventRecord(h->ev[2 * k + 1], h->stream));
     HIPCHECK(h, hipStreamWaitEvent(h->stream, h->ev_ibm, 0));
     launch_adj_collide(h->stream, h->g, h->mrt, h->ip, h->G, slot, h->F, m, src, dst, fs, h->f32);
@@ -967,6 +1020,12 @@ int lbm_get_timing(lbm_handle* h, double* lattice_ms, int32_t* launches) {
   return 0;
 }
 
+int lbm_get_tuning(const lbm_handle* h, int32_t* lattice_tile) {
+  if (!h) return LBM_E_INVALID;
+  if (lattice_tile) *lattice_tile = h->lat_bx;
+  return 0;
+}
+
 int lbm_set_body_rel_pos(lbm_handle* h, int32_t wing, double pos) {
   if (!h || wing < 1 || wing > h->W) return fail(h, LBM_E_INVALID, "wing out of range");
   h->body_rel_pos[wing - 1] = pos;
@@ -978,15 +1037,36 @@ int lbm_set_body_rel_pos(lbm_handle* h, int32_t wing, double pos) {
 
 // ------------------------------------------------------------ batched advance
 
+// A captured batched advance: the launch sequence of one (composition,
+// steps) pair, replayed when the same pair comes again.
+// Longest advance that is captured as a graph; longer advances are issued
+// launch by launch.  Only advances that stay within one geometry batch are
+// captured: captures whose geometry stream waits on slot-release events
+// recorded inside the same capture (advances longer than one geometry batch)
+// ended the process with a host-side segmentation fault inside this call, for
+// a reason not yet found, so that form is not captured.
+constexpr int kMaxGraphSteps = 64;
+
+struct BatchGraph {
+  std::vector<long long> key;
+  int seen = 0;                      // eager runs of this key so far
+  hipGraphExec_t exec = nullptr;
+  unsigned long long last_use = 0;
+};
+
 struct lbm_batch {
-  hipStream_t stream = nullptr, stream_ibm = nullptr, stream_geo = nullptr;
-  hipEvent_t ev_ibm = nullptr, ev_step = nullptr;
+  hipStream_t stream = nullptr, stream_ibm = nullptr, stream_geo = nullptr, stream_bc = nullptr;
+  hipEvent_t ev_ibm = nullptr, ev_step = nullptr, ev_bc = nullptr, ev_start = nullptr;
+  std::vector<BatchGraph> graphs;
+  unsigned long long calls = 0;
+  bool use_graphs = true;
   hipEvent_t ev_geo[2] = {nullptr, nullptr}, ev_rel[2] = {nullptr, nullptr};
   int device = 0;
   EpDesc* d_desc = nullptr;
   size_t desc_cap = 0;
   int2* d_items = nullptr;
   size_t items_cap = 0;
+  int lat_bx = 0;                    // lattice tile width (tuned on the first call)
   char msg[512] = "";
 };
 
@@ -1007,11 +1087,15 @@ int bfail(lbm_batch* b, int code, const char* fmt, ...) {
   } while (0)
 
 void batch_free(lbm_batch* b) {
+  for (auto& g : b->graphs)
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  b->graphs.clear();
+  if (b->ev_start) (void)hipEventDestroy(b->ev_start);
   if (b->d_desc) (void)hipFree(b->d_desc);
   if (b->d_items) (void)hipFree(b->d_items);
-  for (hipEvent_t e : {b->ev_ibm, b->ev_step, b->ev_geo[0], b->ev_geo[1], b->ev_rel[0], b->ev_rel[1]})
+  for (hipEvent_t e : {b->ev_ibm, b->ev_step, b->ev_bc, b->ev_geo[0], b->ev_geo[1], b->ev_rel[0], b->ev_rel[1]})
     if (e) (void)hipEventDestroy(e);
-  for (hipStream_t s : {b->stream, b->stream_ibm, b->stream_geo})
+  for (hipStream_t s : {b->stream, b->stream_ibm, b->stream_geo, b->stream_bc})
     if (s) (void)hipStreamDestroy(s);
 }
 
@@ -1038,6 +1122,9 @@ int lbm_batch_create(lbm_batch** out) {
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&b->stream_ibm, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&b->stream_geo, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&b->stream_bc, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_bc, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_start, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&b->ev_ibm, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&b->ev_step, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&b->ev_geo[0], hipEventDisableTiming) != hipSuccess ||
@@ -1048,6 +1135,7 @@ int lbm_batch_create(lbm_batch** out) {
     delete b;
     return LBM_E_DEVICE;
   }
+  if (const char* e = getenv("LBM_BATCH_GRAPHS")) b->use_graphs = atoi(e) != 0;
   *out = b;
   return 0;
 }
@@ -1058,6 +1146,12 @@ int lbm_batch_destroy(lbm_batch* b) {
   (void)hipStreamSynchronize(b->stream_geo);
   batch_free(b);
   delete b;
+  return 0;
+}
+
+int lbm_batch_get_tuning(const lbm_batch* b, int32_t* lattice_tile) {
+  if (!b) return LBM_E_INVALID;
+  if (lattice_tile) *lattice_tile = b->lat_bx;
   return 0;
 }
 
@@ -1131,7 +1225,7 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
     D.n0 = h->n;
     D.ns = ns;
     D.bc0 = (int)(h->batch_count & 1u);
-    D.tiles_x = lattice_tiles_x(h->g);
+    D.tiles_x = 0;  // set once the tile width is known
     D.bpw = adj_collide_blocks_per_wing(h->ip);
     D.pad0 = D.pad1 = 0;
     if (h->W > 0) {
@@ -1139,11 +1233,48 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
       lds_solve = std::max(lds_solve, geo_solve_lds_bytes(h->ip));
     }
   }
-  // work lists: lattice, ring, adjoint, segment, spread, then the geometry
-  // lists of a full and of the last (shorter) geometry batch
   auto add = [&](int e, int nblk) {
     for (int q = 0; q < nblk; q++) items.push_back(make_int2(e, q));
   };
+  auto upload = [&]() -> int {
+    if ((size_t)count > b->desc_cap) {
+      if (b->d_desc) (void)hipFree(b->d_desc);
+      b->d_desc = nullptr;
+      BCHECK(b, hipMalloc((void**)&b->d_desc, sizeof(EpDesc) * count));
+      b->desc_cap = count;
+    }
+    if (items.size() > b->items_cap) {
+      if (b->d_items) (void)hipFree(b->d_items);
+      b->d_items = nullptr;
+      BCHECK(b, hipMalloc((void**)&b->d_items, sizeof(int2) * items.size()));
+      b->items_cap = items.size();
+    }
+    BCHECK(b, hipStreamSynchronize(b->stream));
+    BCHECK(b, hipMemcpy(b->d_desc, desc.data(), sizeof(EpDesc) * count, hipMemcpyHostToDevice));
+    BCHECK(b, hipMemcpy(b->d_items, items.data(), sizeof(int2) * items.size(), hipMemcpyHostToDevice));
+    return 0;
+  };
+  if (!b->lat_bx) {  // tile width of the batched lattice launch, timed on this composition
+    int rc = 0;
+    b->lat_bx = autotune_tile(b->stream, [&](int bx) {
+      if (rc) return;
+      items.clear();
+      for (int e = 0; e < count; e++) {
+        desc[e].tiles_x = lattice_tiles_x(desc[e].g, bx);
+        add(e, desc[e].tiles_x * (desc[e].g.ny - 2));
+      }
+      if ((rc = upload())) return;
+      BatchItems lt{};
+      lt.lattice = b->d_items;
+      lt.n_lattice = (int)items.size();
+      launch_batch_lattice(b->stream, b->d_desc, lt, 0, f32, false, bx);
+    });
+    if (rc) return rc;
+    items.clear();
+  }
+  for (int e = 0; e < count; e++) desc[e].tiles_x = lattice_tiles_x(desc[e].g, b->lat_bx);
+  // work lists: lattice, ring, adjoint, segment, spread, then the geometry
+  // lists of a full and of the last (shorter) geometry batch
   size_t off[8];
   off[0] = items.size();
   for (int e = 0; e < count; e++) add(e, desc[e].tiles_x * (desc[e].g.ny - 2));
@@ -1165,32 +1296,27 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
   for (int e = 0; e < count; e++)
     if (desc[e].p.W > 0) add(e, tail * desc[e].p.W);
   off[7] = items.size();
-  if ((size_t)count > b->desc_cap) {
-    if (b->d_desc) (void)hipFree(b->d_desc);
-    b->d_desc = nullptr;
-    BCHECK(b, hipMalloc((void**)&b->d_desc, sizeof(EpDesc) * count));
-    b->desc_cap = count;
+  {
+    const int rc = upload();
+    if (rc) return rc;
   }
-  if (items.size() > b->items_cap) {
-    if (b->d_items) (void)hipFree(b->d_items);
-    b->d_items = nullptr;
-    BCHECK(b, hipMalloc((void**)&b->d_items, sizeof(int2) * items.size()));
-    b->items_cap = items.size();
-  }
-  BCHECK(b, hipStreamSynchronize(b->stream));
-  BCHECK(b, hipMemcpy(b->d_desc, desc.data(), sizeof(EpDesc) * count, hipMemcpyHostToDevice));
-  BCHECK(b, hipMemcpy(b->d_items, items.data(), sizeof(int2) * items.size(), hipMemcpyHostToDevice));
   const int2* di = b->d_items;
   it.lattice = di + off[0]; it.n_lattice = (int)(off[1] - off[0]);
   it.ring = di + off[1]; it.n_ring = (int)(off[2] - off[1]);
   it.adj = di + off[2]; it.n_adj = (int)(off[3] - off[2]);
   it.seg = di + off[3]; it.n_seg = (int)(off[4] - off[3]);
   it.spread = di + off[4]; it.n_spread = (int)(off[5] - off[4]);
+  // The launch sequence of the call.  Every stream joins the lattice stream
+  // through events, so the sequence can be captured as one graph; the first
+  // two geometry batches wait on the start of the call (the previous call has
+  // completed), later ones on the release of their slot set.
+  auto issue = [&]() -> int {
+  BCHECK(b, hipEventRecord(b->ev_start, b->stream));
   for (int k = 0; k < steps; k++) {
     if (wings && k % ns == 0) {
       const int set = (k / ns) & 1;
       const bool last = k + ns >= steps;
-      BCHECK(b, hipStreamWaitEvent(b->stream_geo, b->ev_rel[set], 0));
+      BCHECK(b, hipStreamWaitEvent(b->stream_geo, k < 2 * ns ? b->ev_start : b->ev_rel[set], 0));
       if (last)
         launch_batch_geometry(b->stream_geo, b->d_desc, di + off[6], (int)(off[7] - off[6]), k, lds_wing, lds_solve);
       else
@@ -1201,21 +1327,72 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
     const bool fields = k >= steps - 3;
     BCHECK(b, hipEventRecord(b->ev_step, b->stream));
     BCHECK(b, hipStreamWaitEvent(b->stream_ibm, b->ev_step, 0));
+    BCHECK(b, hipStreamWaitEvent(b->stream_bc, b->ev_step, 0));
+    launch_batch_bc_ring(b->stream_bc, b->d_desc, it, k, f32, fields);
+    BCHECK(b, hipEventRecord(b->ev_bc, b->stream_bc));
     launch_batch_ibm_flow(b->stream_ibm, b->d_desc, it, k, f32);
-    launch_batch_bc_ring(b->stream_ibm, b->d_desc, it, k, f32, fields);
+    BCHECK(b, hipStreamWaitEvent(b->stream_ibm, b->ev_bc, 0));
     BCHECK(b, hipEventRecord(b->ev_ibm, b->stream_ibm));
     if (k > 0) {
       launch_batch_force_row(b->stream_ibm, b->d_desc, count, k - 1);
       if (wings && k % ns == 0) BCHECK(b, hipEventRecord(b->ev_rel[((k - 1) / ns) & 1], b->stream_ibm));
     }
-    launch_batch_lattice(b->stream, b->d_desc, it, k, f32, fields);
+    launch_batch_lattice(b->stream, b->d_desc, it, k, f32, fields, b->lat_bx);
     BCHECK(b, hipStreamWaitEvent(b->stream, b->ev_ibm, 0));
     launch_batch_adj_collide(b->stream, b->d_desc, it, k, f32, fields);
   }
   BCHECK(b, hipEventRecord(b->ev_ibm, b->stream_ibm));
   BCHECK(b, hipStreamWaitEvent(b->stream, b->ev_ibm, 0));
   launch_batch_force_row(b->stream, b->d_desc, count, steps - 1);
-  if (wings) BCHECK(b, hipEventRecord(b->ev_rel[((steps - 1) / ns) & 1], b->stream));
+  return 0;
+  };
+  // replay a captured graph of this (composition, steps) pair, capture it on
+  // its second occurrence, run eagerly otherwise
+  std::vector<long long> key = {count, steps, ns, f32, wings, b->lat_bx, (long long)(size_t)b->d_desc,
+                                (long long)(size_t)b->d_items, (long long)lds_wing, (long long)lds_solve};
+  for (int q = 0; q < 8; q++) key.push_back((long long)off[q]);
+  for (int e = 0; e < count; e++) key.push_back((long long)(size_t)hs[e]);
+  BatchGraph* gr = nullptr;
+  if (b->use_graphs && steps <= kMaxGraphSteps && (!wings || steps <= ns)) {
+    for (auto& g : b->graphs)
+      if (g.key == key) gr = &g;
+    if (!gr) {
+      if (b->graphs.size() >= 8) {  // evict the least recently used
+        auto lru = std::min_element(b->graphs.begin(), b->graphs.end(), [](const BatchGraph& x, const BatchGraph& y) {
+          return x.last_use < y.last_use;
+        });
+        if (lru->exec) (void)hipGraphExecDestroy(lru->exec);
+        b->graphs.erase(lru);
+      }
+      b->graphs.push_back(BatchGraph{key, 0, nullptr, 0});
+      gr = &b->graphs.back();
+    }
+    gr->last_use = ++b->calls;
+  }
+  if (gr && !gr->exec && gr->seen >= 1) {
+    hipGraph_t graph = nullptr;
+    BCHECK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = issue();
+    const hipError_t ec = hipStreamEndCapture(b->stream, &graph);
+    if (rc) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc;
+    }
+    if (ec != hipSuccess) return bfail(b, LBM_E_DEVICE, "hipStreamEndCapture: %s", hipGetErrorString(ec));
+    const hipError_t ei = hipGraphInstantiate(&gr->exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) {
+      gr->exec = nullptr;
+      return bfail(b, LBM_E_DEVICE, "hipGraphInstantiate: %s", hipGetErrorString(ei));
+    }
+  }
+  if (gr && gr->exec) {
+    BCHECK(b, hipGraphLaunch(gr->exec, b->stream));
+  } else {
+    const int rc = issue();
+    if (rc) return rc;
+    if (gr) gr->seen++;
+  }
   // host state of every instance as if it had run the steps alone
   std::vector<char> have_err(count, 0);
   for (int e = 0; e < count; e++) {

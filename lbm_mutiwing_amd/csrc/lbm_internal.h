@@ -130,7 +130,7 @@ struct EpDesc {
   void* fbuf[2];
   FieldSlot fslot[3];      // field slot of step n is fslot[n % 3]
   int cur0, n0, ns, bc0;
-  int tiles_x;             // lattice row tiles
+  int tiles_x;             // lattice row tiles of the batch's tile width
   int bpw;                 // adjoint-collide blocks per wing
   int pad0, pad1;
 };
@@ -159,8 +159,11 @@ __host__ __device__ __forceinline__ int pos_of_seg(int k, int n) {
 
 // lattice kernels (lbm_lattice.hip); populations are double (f32 = false) or
 // FP32 deviations from the weights (f32 = true)
+// bx: lattice tile width (one of kLatticeTiles)
+constexpr int kLatticeTiles[3] = {128, 256, 512};
 void launch_lattice(hipStream_t s, const Grid& g, const MrtParams& P, const void* src, void* dst, FieldSlot* fields,
-                    bool f32);
+                    bool f32, int bx);
+int lattice_tiles_x(const Grid& g, int bx);
 void launch_adj_collide(hipStream_t s, const Grid& g, const MrtParams& P, const IbmParams& p, const GeoBuffers& G,
                         int slot, const FlowBuffers& F, int n, const void* src, void* dst, FieldSlot* fields,
                         bool f32);
@@ -188,7 +191,8 @@ void launch_geometry(hipStream_t s, const IbmParams& p, const GeoBuffers& G, Dev
                      int fidx0, int count, bool with_inverse);
 size_t geo_solve_lds_bytes(const IbmParams& p);
 // Batched launches over the episodes of eps (device array); k is the substep.
-void launch_batch_lattice(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields);
+void launch_batch_lattice(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields,
+                          int bx);
 void launch_batch_bc_ring(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields);
 void launch_batch_adj_collide(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields);
 void launch_batch_ibm_flow(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32);
@@ -199,7 +203,6 @@ void launch_batch_geometry(hipStream_t s, const EpDesc* eps, const int2* items, 
                            size_t lds_wing, size_t lds_solve);
 size_t geo_wing_lds_bytes(int spw);
 int bc_ring_blocks(const Grid& g);
-int lattice_tiles_x(const Grid& g);
 int spread_blocks(const IbmParams& p);
 // IBM flow kernels of one step.
 void launch_ibm_flow(hipStream_t s, const IbmParams& p, const MrtParams& P, const GeoBuffers& G, int slot,

@@ -19,18 +19,17 @@
 
 namespace lbm {
 
-constexpr int LAT_BLOCK = 256;
-
-// One thread per node of a row tile; tiles start at i = 0 so that every
-// wavefront stores whole, aligned spans of each population plane (the ring
-// columns i = 0 and i = nx-1 are skipped and left to k_bc_ring).  Workgroups
-// follow the hardware's round-robin placement over the XCDs, which measured
-// faster than remapping tiles to XCD-contiguous runs.
-template <bool FIELDS>
+// One thread per node of a row tile of BX nodes; tiles start at i = 0 so
+// that every wavefront stores whole, aligned spans of each population plane
+// (the ring columns i = 0 and i = nx-1 are skipped and left to k_bc_ring).
+// Workgroups follow the hardware's round-robin placement over the XCDs, which
+// measured faster than remapping tiles to XCD-contiguous runs.  The tile
+// width is chosen per instance (or per batch) by timing (lattice_autotune).
+template <bool FIELDS, int BX>
 __device__ __forceinline__ void lattice_body(const Grid& g, const MrtParams& P, const double* __restrict__ src,
                                              double* __restrict__ dst, const FieldSlot& fs, int tiles_x, int blk) {
   const int ty = blk / tiles_x, tx = blk - ty * tiles_x;
-  const int i = tx * LAT_BLOCK + (int)threadIdx.x;
+  const int i = tx * BX + (int)threadIdx.x;
   const int j = 1 + ty;
   if (i < 1 || i > g.nx - 2) return;
   double f[9];
@@ -49,21 +48,21 @@ __device__ __forceinline__ void lattice_body(const Grid& g, const MrtParams& P, 
   }
 }
 
-template <bool FIELDS>
-__global__ __launch_bounds__(LAT_BLOCK) void k_lattice(Grid g, MrtParams P, const double* __restrict__ src,
-                                                       double* __restrict__ dst, FieldSlot fs, int tiles_x) {
-  lattice_body<FIELDS>(g, P, src, dst, fs, tiles_x, blockIdx.x);
+template <bool FIELDS, int BX>
+__global__ __launch_bounds__(BX) void k_lattice(Grid g, MrtParams P, const double* __restrict__ src,
+                                                double* __restrict__ dst, FieldSlot fs, int tiles_x) {
+  lattice_body<FIELDS, BX>(g, P, src, dst, fs, tiles_x, blockIdx.x);
 }
 
 // FP32 bulk step on deviations d = f - w: rho = 1 + sum d, j = sum e d (the
 // weights carry no momentum), m' = M d, and the equilibrium moments shifted
 // by M w.  Relaxation and the inverse transform are linear, so d* = M^-1 m*'
 // is the deviation of the post-collision populations.
-template <bool FIELDS>
+template <bool FIELDS, int BX>
 __device__ __forceinline__ void lattice_f32_body(const Grid& g, const MrtParams& P, const float* __restrict__ src,
                                                  float* __restrict__ dst, const FieldSlot& fs, int tiles_x, int blk) {
   const int ty = blk / tiles_x, tx = blk - ty * tiles_x;
-  const int i = tx * LAT_BLOCK + (int)threadIdx.x;
+  const int i = tx * BX + (int)threadIdx.x;
   const int j = 1 + ty;
   if (i < 1 || i > g.nx - 2) return;
   float d[9];
@@ -127,32 +126,45 @@ __device__ __forceinline__ void lattice_f32_body(const Grid& g, const MrtParams&
   }
 }
 
-template <bool FIELDS>
-__global__ __launch_bounds__(LAT_BLOCK) void k_lattice_f32(Grid g, MrtParams P, const float* __restrict__ src,
-                                                           float* __restrict__ dst, FieldSlot fs, int tiles_x) {
-  lattice_f32_body<FIELDS>(g, P, src, dst, fs, tiles_x, blockIdx.x);
+template <bool FIELDS, int BX>
+__global__ __launch_bounds__(BX) void k_lattice_f32(Grid g, MrtParams P, const float* __restrict__ src,
+                                                    float* __restrict__ dst, FieldSlot fs, int tiles_x) {
+  lattice_f32_body<FIELDS, BX>(g, P, src, dst, fs, tiles_x, blockIdx.x);
 }
 
-void launch_lattice(hipStream_t s, const Grid& g, const MrtParams& P, const void* src, void* dst, FieldSlot* fields,
-                    bool f32) {
-  const int tiles_x = (g.nx + LAT_BLOCK - 1) / LAT_BLOCK;
-  dim3 grid(tiles_x * (g.ny - 2)), block(LAT_BLOCK);
+int lattice_tiles_x(const Grid& g, int bx) { return (g.nx + bx - 1) / bx; }
+
+template <int BX>
+void launch_lattice_bx(hipStream_t s, const Grid& g, const MrtParams& P, const void* src, void* dst,
+                       const FieldSlot* fields, bool f32) {
+  const int tiles_x = lattice_tiles_x(g, BX);
+  dim3 grid(tiles_x * (g.ny - 2)), block(BX);
   const FieldSlot fs = fields ? *fields : FieldSlot{};
   if (f32) {
     const float* a = (const float*)src;
     float* b = (float*)dst;
     if (fields)
-      hipLaunchKernelGGL(k_lattice_f32<true>, grid, block, 0, s, g, P, a, b, fs, tiles_x);
+      hipLaunchKernelGGL((k_lattice_f32<true, BX>), grid, block, 0, s, g, P, a, b, fs, tiles_x);
     else
-      hipLaunchKernelGGL(k_lattice_f32<false>, grid, block, 0, s, g, P, a, b, fs, tiles_x);
+      hipLaunchKernelGGL((k_lattice_f32<false, BX>), grid, block, 0, s, g, P, a, b, fs, tiles_x);
   } else {
     const double* a = (const double*)src;
     double* b = (double*)dst;
     if (fields)
-      hipLaunchKernelGGL(k_lattice<true>, grid, block, 0, s, g, P, a, b, fs, tiles_x);
+      hipLaunchKernelGGL((k_lattice<true, BX>), grid, block, 0, s, g, P, a, b, fs, tiles_x);
     else
-      hipLaunchKernelGGL(k_lattice<false>, grid, block, 0, s, g, P, a, b, fs, tiles_x);
+      hipLaunchKernelGGL((k_lattice<false, BX>), grid, block, 0, s, g, P, a, b, fs, tiles_x);
   }
+}
+
+void launch_lattice(hipStream_t s, const Grid& g, const MrtParams& P, const void* src, void* dst, FieldSlot* fields,
+                    bool f32, int bx) {
+  if (bx == 128)
+    launch_lattice_bx<128>(s, g, P, src, dst, fields, f32);
+  else if (bx == 512)
+    launch_lattice_bx<512>(s, g, P, src, dst, fields, f32);
+  else
+    launch_lattice_bx<256>(s, g, P, src, dst, fields, f32);
 }
 
 // Re-collide the adjoint nodes with the IBM-corrected velocity u = u* + g and
@@ -669,21 +681,20 @@ void launch_copy_ring_column(hipStream_t s, const Grid& g, const MrtParams& P, c
 
 // ------------------------------------------------------- batched launches
 
-int lattice_tiles_x(const Grid& g) { return (g.nx + LAT_BLOCK - 1) / LAT_BLOCK; }
 int bc_ring_blocks(const Grid& g) { return (2 * (g.ny - 2) + 2 * g.nx + 255) / 256; }
 
-template <bool F32, bool FIELDS>
-__global__ __launch_bounds__(LAT_BLOCK) void kb_lattice(const EpDesc* __restrict__ eps,
-                                                        const int2* __restrict__ items, int k) {
+template <bool F32, bool FIELDS, int BX>
+__global__ __launch_bounds__(BX) void kb_lattice(const EpDesc* __restrict__ eps, const int2* __restrict__ items,
+                                                 int k) {
   const int2 it = items[blockIdx.x];
   const EpDesc& D = eps[it.x];
   const int n = D.n0 + k, cur = (D.cur0 + k) & 1;
   if (F32)
-    lattice_f32_body<FIELDS>(D.g, D.P, (const float*)D.fbuf[cur], (float*)D.fbuf[cur ^ 1], D.fslot[n % 3],
-                             D.tiles_x, it.y);
+    lattice_f32_body<FIELDS, BX>(D.g, D.P, (const float*)D.fbuf[cur], (float*)D.fbuf[cur ^ 1], D.fslot[n % 3],
+                                 D.tiles_x, it.y);
   else
-    lattice_body<FIELDS>(D.g, D.P, (const double*)D.fbuf[cur], (double*)D.fbuf[cur ^ 1], D.fslot[n % 3],
-                         D.tiles_x, it.y);
+    lattice_body<FIELDS, BX>(D.g, D.P, (const double*)D.fbuf[cur], (double*)D.fbuf[cur ^ 1], D.fslot[n % 3],
+                             D.tiles_x, it.y);
 }
 
 template <typename S, bool FIELDS>
@@ -715,16 +726,27 @@ void launch_batch_force_row(hipStream_t s, const EpDesc* eps, int count, int k) 
   hipLaunchKernelGGL(kb_force_row, dim3(count), dim3(64), 0, s, eps, k);
 }
 
-void launch_batch_lattice(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields) {
-  if (it.n_lattice <= 0) return;
-  dim3 grid(it.n_lattice), block(LAT_BLOCK);
+template <int BX>
+void launch_batch_lattice_bx(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields) {
+  dim3 grid(it.n_lattice), block(BX);
   if (f32) {
-    if (fields) hipLaunchKernelGGL((kb_lattice<true, true>), grid, block, 0, s, eps, it.lattice, k);
-    else hipLaunchKernelGGL((kb_lattice<true, false>), grid, block, 0, s, eps, it.lattice, k);
+    if (fields) hipLaunchKernelGGL((kb_lattice<true, true, BX>), grid, block, 0, s, eps, it.lattice, k);
+    else hipLaunchKernelGGL((kb_lattice<true, false, BX>), grid, block, 0, s, eps, it.lattice, k);
   } else {
-    if (fields) hipLaunchKernelGGL((kb_lattice<false, true>), grid, block, 0, s, eps, it.lattice, k);
-    else hipLaunchKernelGGL((kb_lattice<false, false>), grid, block, 0, s, eps, it.lattice, k);
+    if (fields) hipLaunchKernelGGL((kb_lattice<false, true, BX>), grid, block, 0, s, eps, it.lattice, k);
+    else hipLaunchKernelGGL((kb_lattice<false, false, BX>), grid, block, 0, s, eps, it.lattice, k);
   }
+}
+
+void launch_batch_lattice(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields,
+                          int bx) {
+  if (it.n_lattice <= 0) return;
+  if (bx == 128)
+    launch_batch_lattice_bx<128>(s, eps, it, k, f32, fields);
+  else if (bx == 512)
+    launch_batch_lattice_bx<512>(s, eps, it, k, f32, fields);
+  else
+    launch_batch_lattice_bx<256>(s, eps, it, k, f32, fields);
 }
 
 void launch_batch_bc_ring(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields) {

@@ -210,6 +210,14 @@ class Solver:
     def set_body_rel_pos(self, wing: int, pos: float) -> None:
         self._check(self._fn("set_body_rel_pos")(self._h, int(wing), float(pos)))
 
+    def tuning(self) -> int:
+        """Lattice tile width chosen by the autotuner (HIP library only)."""
+        fn = self.lib.lbm_get_tuning
+        fn.restype, fn.argtypes = ctypes.c_int, [ctypes.c_void_p, _ip]
+        v = ctypes.c_int32()
+        self._check(fn(self._h, ctypes.byref(v)))
+        return v.value
+
 
 class TimedSolver(Solver):
     """Solver with the lattice-kernel event timing of the HIP library."""
@@ -257,6 +265,13 @@ class Batch:
         if self._b:
             self.lib.lbm_batch_destroy(self._b)
             self._b = ctypes.c_void_p()
+
+    def tuning(self) -> int:
+        fn = self.lib.lbm_batch_get_tuning
+        fn.restype, fn.argtypes = ctypes.c_int, [ctypes.c_void_p, _ip]
+        v = ctypes.c_int32()
+        fn(self._b, ctypes.byref(v))
+        return v.value
 
     def __del__(self):
         try:
