@@ -17,10 +17,11 @@ two per-step files.
 from __future__ import annotations
 
 import os
-import re
 from typing import Iterable, List, Optional, TextIO
 
 import numpy as np
+
+from .protocol import list_items, read_reals
 
 OUTPUT_STRIDE_X = 2     # main.f90:38
 OUTPUT_STRIDE_Y = 2
@@ -105,12 +106,9 @@ def load_motion_params(path: str, wing_count: int) -> np.ndarray:
         for line in fh:
             if not line.strip():
                 continue
-            toks = [t for t in re.split(r"[\s,]+", line.split("/", 1)[0].strip()) if t]
-            try:
-                vals = [float(t.replace("d", "e").replace("D", "E")) for t in toks[:ncols]]
-            except ValueError:
-                vals = []
-            if len(vals) < ncols:
+            items, slash = list_items(line.rstrip("\n"))
+            vals = np.zeros(ncols)
+            if not read_reals(items, slash, vals, ncols):
                 raise MotionFileError(f"运动参数文件格式错误，列数应为: {ncols} {path}")
             rows.append(vals)
     if not rows:
@@ -206,9 +204,21 @@ def vorticity(u: np.ndarray, v: np.ndarray, uu: float, scale: float) -> np.ndarr
     return dvx - duy
 
 
+def body_map(nx: int, ny: int, adjoint_ij: np.ndarray) -> np.ndarray:
+    """The reference's body flags, [j, i]: 1 at the adjoint nodes of the last
+    step (1-based (i, j) pairs), 0 elsewhere (airfoil.f90:276-329)."""
+    body = np.zeros((ny, nx), dtype=np.int32)
+    ij = np.asarray(adjoint_ij, dtype=np.int64).reshape(-1, 2)
+    if ij.size:
+        body[ij[:, 1] - 1, ij[:, 0] - 1] = 1
+    return body
+
+
 def write_tecplot(path: str, cfg, cycle: int, u: np.ndarray, v: np.ndarray, pre: np.ndarray,
-                  xb1: np.ndarray, yb1: np.ndarray, movenum: int = 0) -> None:
-    """output (solution.f90:132-241) for one lattice: fields are [j, i]."""
+                  xb1: np.ndarray, yb1: np.ndarray, movenum: int = 0,
+                  body: Optional[np.ndarray] = None) -> None:
+    """output (solution.f90:132-241) for one lattice: fields are [j, i];
+    body is the flag map of :func:`body_map` (zeros when None)."""
     ny, nx = u.shape
     ox, oy = cfg.origin()
     sc, uu = cfg.scale, cfg.uu
@@ -226,7 +236,8 @@ def write_tecplot(path: str, cfg, cycle: int, u: np.ndarray, v: np.ndarray, pre:
             y = oy + j * sc
             for i in ii:
                 x = ox + i * sc + movenum * sc
-                fh.write(e16_8_records((x, y, u[j, i] / uu, v[j, i] / uu, pre[j, i], vor[j, i], 0.0, 0.0))[0] + "\n")
+                b = 0.0 if body is None else float(body[j, i])
+                fh.write(e16_8_records((x, y, u[j, i] / uu, v[j, i] / uu, pre[j, i], vor[j, i], b, 0.0))[0] + "\n")
         npts = W * P
         fh.write(list_directed('ZONE T="airfoil", NODES=', npts + W, ' ELEMENTS,=', W * (P - 1),
                                ', DATAPACKING=POINT, ZONETYPE=FETRIANGLE', ',SOLUTIONTIME=', tsol) + "\n")
@@ -251,8 +262,9 @@ def tecplot_save_hook(out_dir: str):
         s = session.solver
         rho, u, v, pre = s.fields()
         xb1, yb1, _, _ = s.markers()
+        body = body_map(s.cfg.nx, s.cfg.ny, s.ibm()[0])
         name = f"F{cycle}.dat"
-        write_tecplot(os.path.join(out_dir, name), s.cfg, cycle, u, v, pre, xb1, yb1, s.state()["movenum"])
+        write_tecplot(os.path.join(out_dir, name), s.cfg, cycle, u, v, pre, xb1, yb1, s.state()["movenum"], body)
         return name
 
     return hook

@@ -61,47 +61,111 @@ def format_force_reply(row: np.ndarray) -> str:
     return "FORCE " + str(len(row)) + "".join(" " + fortran_es24_16(v) for v in row)
 
 
-def _tokens(message: str) -> List[str]:
-    # list-directed input: values separated by blanks or commas; a slash ends the record
-    body = message.split("/", 1)[0]
-    return [t for t in re.split(r"[\s,]+", body.strip()) if t]
+_REAL = re.compile(r"^[+-]?(\d+\.?\d*|\.\d+)([eEdDqQ][+-]?\d+|[+-]\d+)?$")
+_REPEAT = re.compile(r"^(\d+)\*(.*)$", re.S)
+_SEPARATORS = " \t,;/"
 
 
-def _real(tok: str) -> float:
-    return float(tok.replace("d", "e").replace("D", "E"))
+def list_items(record: str) -> Tuple[List[Optional[str]], bool]:
+    """Items of a list-directed input record, as Fortran reads them: values
+    are separated by blanks (or tabs) around at most one comma or semicolon;
+    a comma that follows a separator gives a null item (None); ``r*c`` is r
+    copies of c and ``r*`` r null items; a quoted string is one item; a slash
+    ends the record.  Returns (items, ended_by_slash)."""
+    items: List[Optional[str]] = []
+    pos, n = 0, len(record)
+
+    def skip_blanks(p):
+        while p < n and record[p] in " \t":
+            p += 1
+        return p
+
+    while True:
+        pos = skip_blanks(pos)
+        if pos >= n:
+            return items, False
+        c = record[pos]
+        if c == "/":
+            return items, True
+        if c in ",;":                     # separator after a separator: null
+            items.append(None)
+            pos += 1
+            continue
+        if c in "'\"":
+            end = record.find(c, pos + 1)
+            end = n if end < 0 else end
+            tok, pos = record[pos + 1:end], end + 1
+            items.append(tok)
+        else:
+            end = pos
+            while end < n and record[end] not in _SEPARATORS:
+                end += 1
+            tok, pos = record[pos:end], end
+            m = _REPEAT.match(tok)
+            if m:
+                items.extend([m.group(2) if m.group(2) else None] * int(m.group(1)))
+            else:
+                items.append(tok)
+        pos = skip_blanks(pos)
+        if pos < n and record[pos] in ",;":
+            pos += 1
+
+
+def fortran_real(tok: str) -> float:
+    """A real in list-directed input: optional sign, digits with an optional
+    point, exponent letter E, D or Q, or a signed exponent without letter."""
+    m = _REAL.match(tok.strip())
+    if not m:
+        raise ValueError(tok)
+    mant, ex = m.group(1), m.group(2) or ""
+    sign = "-" if tok.strip().startswith("-") else ""
+    if ex and ex[0] in "eEdDqQ":
+        ex = ex[1:]
+    return float(f"{sign}{mant}e{ex}" if ex else f"{sign}{mant}")
+
+
+def read_reals(items: List[Optional[str]], slash: bool, row: np.ndarray, count: int) -> bool:
+    """Fill row[:count] from items as ``read(...) row(1:count)`` does: null
+    items keep the old value, a slash keeps the rest; False when the record
+    runs out or an item is not a real."""
+    for k in range(count):
+        if k >= len(items):
+            return slash
+        if items[k] is None:
+            continue
+        try:
+            row[k] = fortran_real(items[k])
+        except ValueError:
+            return False
+    return True
 
 
 def parse_command(message: str, wing_count: int) -> Tuple[int, np.ndarray, str]:
-    """parse_command_message (zmq_comm.f90:225-281): (command, row, error)."""
+    """parse_command_message (zmq_comm.f90:225-281): (command, row, error).
+    The keyword is the first list-directed item (at most 32 characters,
+    ASCII-uppercased).  RESET reads its time only when the message is longer
+    than the keyword (len_trim comparison, so leading blanks make the time
+    mandatory); STEP reads exactly 1 + 3*wing_count reals."""
     row = np.zeros(1 + 9 * 8)
-    toks = _tokens(message)
-    if not toks:
+    items, slash = list_items(message)
+    if not items and not slash:
         return CMD_INVALID, row, ERR_PARSE
-    key = toks[0].upper()
+    key = (items[0] or "") if items else ""
+    key = "".join(ch.upper() if "a" <= ch <= "z" else ch for ch in key[:32])
     if key in ("RESET", "START"):
-        if len(toks) == 1:
-            row[0] = 0.0
-        else:
-            try:
-                row[0] = _real(toks[1])
-            except ValueError:
+        if len(message.rstrip(" ")) != len(key.rstrip(" ")):
+            if not read_reals(items[1:], slash, row, 1):
                 return CMD_INVALID, row, ERR_RESET_TIME_MISSING
         return CMD_RESET, row, ""
     if key == "STEP":
-        need = 1 + 3 * wing_count
-        try:
-            vals = [_real(t) for t in toks[1:1 + need]]
-        except ValueError:
+        if not read_reals(items[1:], slash, row, 1 + 3 * wing_count):
             return CMD_INVALID, row, ERR_STEP_ARGS
-        if len(vals) < need:
-            return CMD_INVALID, row, ERR_STEP_ARGS
-        row[:need] = vals
         return CMD_STEP, row, ""
     if key in ("OUTPUT", "SAVE"):
         return CMD_OUTPUT, row, ""
     if key in ("STOP", "EXIT"):
         return CMD_STOP, row, ""
-    return CMD_INVALID, row, ERR_UNKNOWN + key
+    return CMD_INVALID, row, (ERR_UNKNOWN + key).rstrip(" ")
 
 
 def fortran_nint(x: float) -> int:

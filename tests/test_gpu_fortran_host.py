@@ -1,0 +1,199 @@
+"""GPU tests of the Fortran host program (lbm_mutiwing_amd/lbm_host) over
+liblbm_hip.so.
+
+File mode: the host reads its namelist, the marker files and
+motion_params.dat, runs the step loop and writes force.dat and
+airfoil_motion.dat (main.f90:235-441, solution.f90:386-402).  Every record is
+checked against the FP64 CPU oracle on the same inputs (tolerance: the
+'(f15.8)' rounding of 5e-9 plus 1e-9 relative, the force parity bar), and
+against the same run through the Python host and its formatter (identical
+text: same library, same GPU, same edit descriptors).
+
+Online mode: the host answers RESET / STEP / SAVE / STOP on a ZeroMQ REP
+socket (driver.f90:59-196).  The replies are checked against the protocol
+mirror (lbm_mutiwing_amd.protocol.Session) driven over the oracle (FORCE
+values within 1e-9 relative, all other replies identical) and over the GPU
+library (identical text), and the SAVE snapshot against the Python Tecplot
+writer (identical data records).
+"""
+import os
+import subprocess
+import uuid
+
+import numpy as np
+import pytest
+
+from lbm_mutiwing_amd import MOTION_STREAM, Solver, build, preset, seeded_motions, sinusoidal_motion_table
+from lbm_mutiwing_amd import formats
+from lbm_mutiwing_amd.protocol import Session, run_job_alone
+
+pytestmark = pytest.mark.gpu
+
+FORCE_TOL = 1e-9
+F15_8_HALF_ULP = 5.0e-9
+
+
+@pytest.fixture(scope="module")
+def host():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a visible MI355X"
+    if not os.path.exists(build.HOST):
+        build.build_host()
+    return build.HOST
+
+
+def _namelist(cfg, out_dir="out", extra=""):
+    ox, oy = cfg.origin()
+    vals = dict(nx=cfg.nx, ny=cfg.ny, scale=cfg.scale, ox=ox, oy=oy, uu=cfg.uu, re=cfg.re,
+                wing_count=cfg.wing_count, points_per_wing=cfg.points_per_wing, wing_x0=cfg.wing_x0,
+                wing_dx=cfg.wing_dx, wing_y0=cfg.wing_y0, xc=cfg.xc, yc=cfg.yc, xp=cfg.xp, yp=cfg.yp,
+                precision=cfg.precision)
+    body = ", ".join(f"{k}={v!r}" for k, v in vals.items())
+    s = ", ".join(repr(float(x)) for x in cfg.s)
+    return f"&lbm_case {body}, s={s}, boundary_dir='.', motion_file='motion_params.dat', " \
+           f"output_dir='{out_dir}' {extra} /\n"
+
+
+def _write_case(tmp, cfg, rows=None):
+    P = cfg.points_per_wing
+    m = cfg.marker_array()
+    for w in range(cfg.wing_count):
+        name = "boundary.dat" if w == 0 else f"boundary{w}.dat"
+        formats.write_marker_file(str(tmp / name), m[w * P:(w + 1) * P])
+    if rows is not None:
+        formats.write_motion_params(str(tmp / "motion_params.dat"), rows)
+    (tmp / "case.nml").write_text(_namelist(cfg))
+
+
+def _read_case(tmp, cfg):
+    """The configuration as the host sees it: markers read back from the
+    files ('(2f14.8)', exact for these 8-decimal markers)."""
+    P = cfg.points_per_wing
+    wm = [formats.read_marker_file(str(tmp / ("boundary.dat" if w == 0 else f"boundary{w}.dat")), P)
+          for w in range(cfg.wing_count)]
+    cfg.wing_markers = wm
+    cfg.markers = None
+    return cfg
+
+
+def _read_f15(path, ncols):
+    out = []
+    for line in open(path):
+        line = line.rstrip("\n")
+        assert len(line) == 15 * ncols, line
+        out.append([float(line[15 * k:15 * (k + 1)]) for k in range(ncols)])
+    return np.array(out)
+
+
+def test_file_mode_matches_oracle(host, oracle_mod, tmp_path):
+    cfg = preset("small")
+    n = 150
+    rows = sinusoidal_motion_table(cfg, seeded_motions(cfg, 21), n)
+    _write_case(tmp_path, cfg, rows)
+    print(f"fortran host: file mode, small 256x128, {n} steps", flush=True)
+    r = subprocess.run([host, "case.nml"], cwd=tmp_path, capture_output=True, timeout=300)
+    assert r.returncode == 0, r.stdout.decode() + r.stderr.decode()
+    cfg = _read_case(tmp_path, cfg)
+    table = formats.load_motion_params(str(tmp_path / "motion_params.dat"), cfg.wing_count)
+    fr = _read_f15(tmp_path / "out" / "force.dat", cfg.row_len)
+    mr = _read_f15(tmp_path / "out" / "airfoil_motion.dat", 1 + 3 * cfg.wing_count)
+    assert fr.shape[0] == n and mr.shape[0] == n
+
+    o = oracle_mod.OracleSolver(cfg)
+    o.set_motion_table(table)
+    ro = o.advance(n, want_rows=True, want_motion=True)
+    tol_f = F15_8_HALF_ULP * 1.0001 + FORCE_TOL * np.abs(ro.force_rows)
+    assert np.all(np.abs(fr - ro.force_rows) <= tol_f)
+    assert np.all(np.abs(mr - ro.motion_rows) <= F15_8_HALF_ULP * 1.0001 + 1e-12 * np.abs(ro.motion_rows))
+
+    # the same run through the Python host writes the same text
+    g = Solver(cfg)
+    formats.run_file_mode(g, table, str(tmp_path / "py"), chunk=100)
+    for name in ("force.dat", "airfoil_motion.dat"):
+        assert (tmp_path / "out" / name).read_text() == (tmp_path / "py" / name).read_text(), name
+
+
+def _commands(dt):
+    t = lambda k: repr(k * dt)  # noqa: E731
+    return ["STEP 0.1 0 0 0 0 0 0", "SAVE", "JUMP", "RESET 0.5", "RESET 0",
+            f"STEP {t(40)} 0.01 0.002 -0.003 0.0 0.001 0.002",
+            f"STEP {repr(40.5 * dt)} 0 0 0 0 0 0",
+            f"STEP {t(40)} 0 0 0 0 0 0",
+            f"STEP {t(20)} 0 0 0 0 0 0",
+            "SAVE",
+            f"STEP {t(90)} -0.01 0.003 0.001 0.002 0 0.001",
+            "RESET",
+            f"STEP {t(25)} 0.02 0 0 0 0 0",
+            "STOP"]
+
+
+def _mirror_replies(solver, commands, save_dir=None):
+    hook = formats.tecplot_save_hook(save_dir) if save_dir else None
+    sess = Session(solver, save_hook=hook)
+    out = []
+    for msg in commands:
+        reply, job = sess.receive(msg)
+        out.append(reply if job is None else run_job_alone(job))
+    return out
+
+
+def _force(reply):
+    toks = reply.split()
+    assert toks[0] == "FORCE" and len(toks) == 2 + int(toks[1])
+    return np.array([float(x) for x in toks[2:]])
+
+
+def _data_records(path):
+    return [ln for ln in open(path) if not ln.lstrip().startswith(("ZONE", "VARIABLES"))]
+
+
+def test_online_mode_session(host, oracle_mod, tmp_path):
+    from lbm_mutiwing_amd.build import zmq_library
+    from zmq_client import ReqClient
+    cfg = preset("small")
+    cfg.motion_source = MOTION_STREAM
+    _write_case(tmp_path, cfg)
+    cfg = _read_case(tmp_path, cfg)
+    cmds = _commands(cfg.dt)
+    endpoint = f"ipc:///tmp/lbm_host_{uuid.uuid4().hex[:12]}.ipc"
+    env = dict(os.environ, LBM_MOTION_SOURCE="ZMQ", LBM_ZMQ_ENDPOINT=endpoint)
+    print(f"fortran host: online mode on {endpoint}", flush=True)
+    log = open(tmp_path / "host.log", "w")
+    proc = subprocess.Popen([host, "case.nml"], cwd=tmp_path, env=env, stdout=log, stderr=subprocess.STDOUT)
+    client = ReqClient(zmq_library(), endpoint, timeout_ms=120000)
+    try:
+        got = []
+        for msg in cmds:
+            print(f"  -> {msg}", flush=True)
+            got.append(client.request(msg))
+        rc = proc.wait(timeout=60)
+    finally:
+        client.close()
+        if proc.poll() is None:
+            proc.kill()
+            proc.wait()
+        log.close()
+    assert rc == 0, (tmp_path / "host.log").read_text()
+
+    want_o = _mirror_replies(oracle_mod.OracleSolver(cfg), cmds)
+    want_g = _mirror_replies(Solver(cfg), cmds, save_dir=str(tmp_path / "py_save"))
+    assert [r.split()[0] for r in got] == [r.split()[0] for r in want_o]
+    for msg, a, b, c in zip(cmds, got, want_o, want_g):
+        if a.startswith("FORCE"):
+            fa, fb = _force(a), _force(b)
+            assert fa.shape == fb.shape, msg
+            # relative per value, floored at 1e-3 of the row's largest value
+            # (the first step's moments are rounding noise beside its forces)
+            floor = 1e-3 * np.max(np.abs(fb))
+            assert np.all(np.abs(fa - fb) <= FORCE_TOL * np.maximum(np.abs(fb), floor)), msg
+        else:
+            assert a == b, (msg, a, b)
+        assert a == c, (msg, a, c)
+
+    saved = [r for r in got if r.startswith("STATUS FLOW_SAVED")]
+    assert saved == ["STATUS FLOW_SAVED F41.dat"]
+    host_file = tmp_path / "out" / "episode_1" / "F41.dat"
+    py_file = tmp_path / "py_save" / "F41.dat"
+    assert _data_records(host_file) == _data_records(py_file)
+    for ep in (1, 2):
+        assert (tmp_path / "out" / f"force_episode_{ep}.dat").exists()
