@@ -329,8 +329,9 @@ void tune_instance(lbm_handle* h) {
   if (h->lat_bx) return;
   const void* src = h->fbuf[h->cur];
   void* dst = h->fbuf[1 - h->cur];
+  const AdjMask none{nullptr, nullptr, 0, 0};
   h->lat_bx = autotune_tile(h->stream, [&](int bx) { launch_lattice(h->stream, h->g, h->mrt, src, dst, nullptr,
-                                                                     h->f32, bx); });
+                                                                     h->f32, bx, none); });
 }
 
 // Per-slot geometry arrays and flow buffers of the IBM.  The number of steps
@@ -735,6 +736,7 @@ int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motio
       launch_geometry(h->stream_geo, h->ip, h->G, h->d_st, set * ns, m, k, cnt, true);
       HIPCHECK(h, hipEventRecord(h->ev_geo[set], h->stream_geo));
       HIPCHECK(h, hipStreamWaitEvent(h->stream_ibm, h->ev_geo[set], 0));
+      HIPCHECK(h, hipStreamWaitEvent(h->stream, h->ev_geo[set], 0));  // the lattice kernel reads the claim maps
     }
     slot = set * ns + k % ns;
     const int s = m % 3;
@@ -744,30 +746,33 @@ int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motio
     FieldSlot* fs = want_fields ? &h->slot[s] : nullptr;
     const void* src = h->fbuf[h->cur];
     void* dst = h->fbuf[1 - h->cur];
-    // IBM flow kernels on the second stream and the boundary ring on a third,
-    // concurrent with the full-lattice kernel; they start once the previous
-    // step has finished (the ring reads only src and writes only ring nodes
-    // of dst).  The lattice stream waits for both through ev_ibm.  The
-    // previous step's force row follows on the IBM stream; a geometry slot
-    // set is released once the force row of its last step is written.
+    // Every node of dst has exactly one writer this step: the full-lattice
+    // kernel (lattice stream) writes the interior nodes other than the
+    // adjoint nodes; the IBM flow kernels and the adjoint re-collision
+    // (second stream) write the adjoint nodes; the boundary ring (third
+    // stream) writes the ring.  All three read only src and run
+    // concurrently once the previous step has finished; the next step's
+    // lattice kernel waits for the other two through ev_ibm.  The previous
+    // step's force row follows on the IBM stream; a geometry slot set is
+    // released once the force row of its last step is written.
     HIPCHECK(h, hipEventRecord(h->ev_step, h->stream));
     HIPCHECK(h, hipStreamWaitEvent(h->stream_ibm, h->ev_step, 0));
     HIPCHECK(h, hipStreamWaitEvent(h->stream_bc, h->ev_step, 0));
     launch_bc_ring(h->stream_bc, h->g, h->mrt, h->R, src, dst, fs, h->f32);
     HIPCHECK(h, hipEventRecord(h->ev_bc, h->stream_bc));
-    launch_ibm_flow(h->stream_ibm, h->ip, h->mrt, h->G, slot, h->F, h->g, src, h->f32);
-    HIPCHECK(h, hipStreamWaitEvent(h->stream_ibm, h->ev_bc, 0));
-    HIPCHECK(h, hipEventRecord(h->ev_ibm, h->stream_ibm));
     // the previous step's force row (its partial sums are kept by step parity)
     if (k > 0) {
       launch_force_row(h->stream_ibm, h->ip, h->G, prev_slot, h->F, h->d_st, m - 1, k - 1, h->row_len);
       if (h->W > 0 && k % ns == 0) HIPCHECK(h, hipEventRecord(h->ev_rel[prev_set], h->stream_ibm));
     }
+    launch_ibm_flow(h->stream_ibm, h->ip, h->mrt, h->G, slot, h->F, h->g, src, h->f32);
+    launch_adj_collide(h->stream_ibm, h->g, h->mrt, h->ip, h->G, slot, h->F, m, src, dst, fs, h->f32);
+    HIPCHECK(h, hipStreamWaitEvent(h->stream_ibm, h->ev_bc, 0));
+    HIPCHECK(h, hipEventRecord(h->ev_ibm, h->stream_ibm));
     if (h->timing) HIPCHECK(h, hipEventRecord(h->ev[2 * k], h->stream));
-    launch_lattice(h->stream, h->g, h->mrt, src, dst, fs, h->f32, h->lat_bx);
+    launch_lattice(h->stream, h->g, h->mrt, src, dst, fs, h->f32, h->lat_bx, adj_mask(h->ip, h->G, slot));
     if (h->timing) HIPCHECK(h, hipEventRecord(h->ev[2 * k + 1], h->stream));
     HIPCHECK(h, hipStreamWaitEvent(h->stream, h->ev_ibm, 0));
-    launch_adj_collide(h->stream, h->g, h->mrt, h->ip, h->G, slot, h->F, m, src, dst, fs, h->f32);
     prev_slot = slot;
     if (want_fields) h->slot_step[s] = m;
     h->cur = 1 - h->cur;
@@ -1267,7 +1272,7 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
       BatchItems lt{};
       lt.lattice = b->d_items;
       lt.n_lattice = (int)items.size();
-      launch_batch_lattice(b->stream, b->d_desc, lt, 0, f32, false, bx);
+      launch_batch_lattice(b->stream, b->d_desc, lt, 0, f32, false, bx, false);
     });
     if (rc) return rc;
     items.clear();
@@ -1323,23 +1328,26 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
         launch_batch_geometry(b->stream_geo, b->d_desc, di + off[5], (int)(off[6] - off[5]), k, lds_wing, lds_solve);
       BCHECK(b, hipEventRecord(b->ev_geo[set], b->stream_geo));
       BCHECK(b, hipStreamWaitEvent(b->stream_ibm, b->ev_geo[set], 0));
+      BCHECK(b, hipStreamWaitEvent(b->stream, b->ev_geo[set], 0));  // lattice reads the claim maps
     }
+    // one writer per node, as in lbm_advance: lattice (all but the adjoint
+    // nodes), IBM stream (adjoint nodes), ring stream (boundary ring)
     const bool fields = k >= steps - 3;
     BCHECK(b, hipEventRecord(b->ev_step, b->stream));
     BCHECK(b, hipStreamWaitEvent(b->stream_ibm, b->ev_step, 0));
     BCHECK(b, hipStreamWaitEvent(b->stream_bc, b->ev_step, 0));
     launch_batch_bc_ring(b->stream_bc, b->d_desc, it, k, f32, fields);
     BCHECK(b, hipEventRecord(b->ev_bc, b->stream_bc));
-    launch_batch_ibm_flow(b->stream_ibm, b->d_desc, it, k, f32);
-    BCHECK(b, hipStreamWaitEvent(b->stream_ibm, b->ev_bc, 0));
-    BCHECK(b, hipEventRecord(b->ev_ibm, b->stream_ibm));
     if (k > 0) {
       launch_batch_force_row(b->stream_ibm, b->d_desc, count, k - 1);
       if (wings && k % ns == 0) BCHECK(b, hipEventRecord(b->ev_rel[((k - 1) / ns) & 1], b->stream_ibm));
     }
-    launch_batch_lattice(b->stream, b->d_desc, it, k, f32, fields, b->lat_bx);
+    launch_batch_ibm_flow(b->stream_ibm, b->d_desc, it, k, f32);
+    launch_batch_adj_collide(b->stream_ibm, b->d_desc, it, k, f32, fields);
+    BCHECK(b, hipStreamWaitEvent(b->stream_ibm, b->ev_bc, 0));
+    BCHECK(b, hipEventRecord(b->ev_ibm, b->stream_ibm));
+    launch_batch_lattice(b->stream, b->d_desc, it, k, f32, fields, b->lat_bx, true);
     BCHECK(b, hipStreamWaitEvent(b->stream, b->ev_ibm, 0));
-    launch_batch_adj_collide(b->stream, b->d_desc, it, k, f32, fields);
   }
   BCHECK(b, hipEventRecord(b->ev_ibm, b->stream_ibm));
   BCHECK(b, hipStreamWaitEvent(b->stream, b->ev_ibm, 0));

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <limits.h>
 #include <stdint.h>
 
 #include "../../include/lbm_hip.h"
@@ -94,6 +95,32 @@ struct GeoBuffers {
   const double* stream_rows;    // stream mode: start row, target row (2*LBM_MAX_MOTION_COLS)
 };
 
+// The adjoint nodes of one step, which the full-lattice kernel leaves to
+// k_adj_collide (their only writer): the wings' support rectangles and claim
+// maps in the step's geometry slot.  W = 0 selects no node.
+struct AdjMask {
+  const GeoStep* gs;
+  const int* map;          // claim maps of the slot, [W][2][map_cap]; owner INT_MAX = free
+  int W, map_cap;
+};
+
+__host__ __device__ __forceinline__ AdjMask adj_mask(const IbmParams& p, const GeoBuffers& G, int slot) {
+  return AdjMask{G.step + slot, G.map + (size_t)slot * p.W * 2 * (size_t)p.map_cap, p.W, p.map_cap};
+}
+
+// Is (i, j) (0-based) an adjoint node of the mask's step?  Rows outside every
+// rectangle cost a few scalar compares.
+__device__ __forceinline__ bool is_adjoint(const AdjMask& am, int i, int j) {
+  for (int w = 0; w < am.W; w++) {
+    const int* r = am.gs->rect[w];
+    const int i0 = r[0], i1 = r[1], j0 = r[2], j1 = r[3];
+    if (j < j0 || j > j1 || i < i0 || i > i1) continue;
+    const int* M = am.map + (size_t)w * 2 * am.map_cap;
+    if (M[(i - i0) + (i1 - i0 + 1) * (j - j0)] != INT_MAX) return true;
+  }
+  return false;
+}
+
 // Flow-dependent IBM buffers of the step being run.
 struct FlowBuffers {
   double* cvec;            // [nseg][2] right-hand side U_b - D u* by global band position
@@ -161,8 +188,9 @@ __host__ __device__ __forceinline__ int pos_of_seg(int k, int n) {
 // FP32 deviations from the weights (f32 = true)
 // bx: lattice tile width (one of kLatticeTiles)
 constexpr int kLatticeTiles[3] = {128, 256, 512};
+// mask: adjoint nodes of the step, skipped (written by k_adj_collide)
 void launch_lattice(hipStream_t s, const Grid& g, const MrtParams& P, const void* src, void* dst, FieldSlot* fields,
-                    bool f32, int bx);
+                    bool f32, int bx, const AdjMask& mask);
 int lattice_tiles_x(const Grid& g, int bx);
 void launch_adj_collide(hipStream_t s, const Grid& g, const MrtParams& P, const IbmParams& p, const GeoBuffers& G,
                         int slot, const FlowBuffers& F, int n, const void* src, void* dst, FieldSlot* fields,
@@ -191,8 +219,9 @@ void launch_geometry(hipStream_t s, const IbmParams& p, const GeoBuffers& G, Dev
                      int fidx0, int count, bool with_inverse);
 size_t geo_solve_lds_bytes(const IbmParams& p);
 // Batched launches over the episodes of eps (device array); k is the substep.
+// masked: skip each episode's adjoint nodes of substep k (its geometry must exist)
 void launch_batch_lattice(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields,
-                          int bx);
+                          int bx, bool masked);
 void launch_batch_bc_ring(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields);
 void launch_batch_adj_collide(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields);
 void launch_batch_ibm_flow(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32);

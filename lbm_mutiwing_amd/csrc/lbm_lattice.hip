@@ -4,10 +4,10 @@
 // (solution.f90:483-554) and colission (solution.f90:585-655) for every
 // interior node: 9 pulled loads and 9 stores of one SoA plane each, i.e. the
 // 144 B/node floor of a two-buffer FP64 D2Q9 step (72 B/node with FP32
-// deviation storage).  Nodes that the immersed boundary touches this step are
-// collided here with zero body force and collided again by k_adj_collide once
-// the IBM correction is known, so the full-lattice pass never waits for the
-// IBM chain.
+// deviation storage).  The nodes that the immersed boundary touches this step
+// (the adjoint nodes) are skipped: k_adj_collide collides them with the IBM
+// correction, on the IBM stream while the full-lattice pass runs, so neither
+// pass waits for the other.
 //
 // Every kernel that touches populations is a template on the storage type S:
 // double (FP64 values) or float (FP32 deviations d = f - w_k).  Only the bulk
@@ -27,11 +27,12 @@ namespace lbm {
 // width is chosen per instance (or per batch) by timing (lattice_autotune).
 template <bool FIELDS, int BX>
 __device__ __forceinline__ void lattice_body(const Grid& g, const MrtParams& P, const double* __restrict__ src,
-                                             double* __restrict__ dst, const FieldSlot& fs, int tiles_x, int blk) {
+                                             double* __restrict__ dst, const FieldSlot& fs, int tiles_x, int blk,
+                                             const AdjMask& am) {
   const int ty = blk / tiles_x, tx = blk - ty * tiles_x;
   const int i = tx * BX + (int)threadIdx.x;
   const int j = 1 + ty;
-  if (i < 1 || i > g.nx - 2) return;
+  if (i < 1 || i > g.nx - 2 || is_adjoint(am, i, j)) return;
   double f[9];
   pull_node(src, g, i, j, f);
   double rho, u, v;
@@ -50,8 +51,8 @@ __device__ __forceinline__ void lattice_body(const Grid& g, const MrtParams& P, 
 
 template <bool FIELDS, int BX>
 __global__ __launch_bounds__(BX) void k_lattice(Grid g, MrtParams P, const double* __restrict__ src,
-                                                double* __restrict__ dst, FieldSlot fs, int tiles_x) {
-  lattice_body<FIELDS, BX>(g, P, src, dst, fs, tiles_x, blockIdx.x);
+                                                double* __restrict__ dst, FieldSlot fs, int tiles_x, AdjMask am) {
+  lattice_body<FIELDS, BX>(g, P, src, dst, fs, tiles_x, blockIdx.x, am);
 }
 
 // FP32 bulk step on deviations d = f - w: rho = 1 + sum d, j = sum e d (the
@@ -60,11 +61,12 @@ __global__ __launch_bounds__(BX) void k_lattice(Grid g, MrtParams P, const doubl
 // is the deviation of the post-collision populations.
 template <bool FIELDS, int BX>
 __device__ __forceinline__ void lattice_f32_body(const Grid& g, const MrtParams& P, const float* __restrict__ src,
-                                                 float* __restrict__ dst, const FieldSlot& fs, int tiles_x, int blk) {
+                                                 float* __restrict__ dst, const FieldSlot& fs, int tiles_x, int blk,
+                                                 const AdjMask& am) {
   const int ty = blk / tiles_x, tx = blk - ty * tiles_x;
   const int i = tx * BX + (int)threadIdx.x;
   const int j = 1 + ty;
-  if (i < 1 || i > g.nx - 2) return;
+  if (i < 1 || i > g.nx - 2 || is_adjoint(am, i, j)) return;
   float d[9];
 #pragma unroll
   for (int k = 0; k < 9; k++) d[k] = src[(long long)k * g.ps + (long long)(j - EY(k)) * g.nx + (i - EX(k))];
@@ -128,15 +130,15 @@ __device__ __forceinline__ void lattice_f32_body(const Grid& g, const MrtParams&
 
 template <bool FIELDS, int BX>
 __global__ __launch_bounds__(BX) void k_lattice_f32(Grid g, MrtParams P, const float* __restrict__ src,
-                                                    float* __restrict__ dst, FieldSlot fs, int tiles_x) {
-  lattice_f32_body<FIELDS, BX>(g, P, src, dst, fs, tiles_x, blockIdx.x);
+                                                    float* __restrict__ dst, FieldSlot fs, int tiles_x, AdjMask am) {
+  lattice_f32_body<FIELDS, BX>(g, P, src, dst, fs, tiles_x, blockIdx.x, am);
 }
 
 int lattice_tiles_x(const Grid& g, int bx) { return (g.nx + bx - 1) / bx; }
 
 template <int BX>
 void launch_lattice_bx(hipStream_t s, const Grid& g, const MrtParams& P, const void* src, void* dst,
-                       const FieldSlot* fields, bool f32) {
+                       const FieldSlot* fields, bool f32, const AdjMask& am) {
   const int tiles_x = lattice_tiles_x(g, BX);
   dim3 grid(tiles_x * (g.ny - 2)), block(BX);
   const FieldSlot fs = fields ? *fields : FieldSlot{};
@@ -144,27 +146,27 @@ void launch_lattice_bx(hipStream_t s, const Grid& g, const MrtParams& P, const v
     const float* a = (const float*)src;
     float* b = (float*)dst;
     if (fields)
-      hipLaunchKernelGGL((k_lattice_f32<true, BX>), grid, block, 0, s, g, P, a, b, fs, tiles_x);
+      hipLaunchKernelGGL((k_lattice_f32<true, BX>), grid, block, 0, s, g, P, a, b, fs, tiles_x, am);
     else
-      hipLaunchKernelGGL((k_lattice_f32<false, BX>), grid, block, 0, s, g, P, a, b, fs, tiles_x);
+      hipLaunchKernelGGL((k_lattice_f32<false, BX>), grid, block, 0, s, g, P, a, b, fs, tiles_x, am);
   } else {
     const double* a = (const double*)src;
     double* b = (double*)dst;
     if (fields)
-      hipLaunchKernelGGL((k_lattice<true, BX>), grid, block, 0, s, g, P, a, b, fs, tiles_x);
+      hipLaunchKernelGGL((k_lattice<true, BX>), grid, block, 0, s, g, P, a, b, fs, tiles_x, am);
     else
-      hipLaunchKernelGGL((k_lattice<false, BX>), grid, block, 0, s, g, P, a, b, fs, tiles_x);
+      hipLaunchKernelGGL((k_lattice<false, BX>), grid, block, 0, s, g, P, a, b, fs, tiles_x, am);
   }
 }
 
 void launch_lattice(hipStream_t s, const Grid& g, const MrtParams& P, const void* src, void* dst, FieldSlot* fields,
-                    bool f32, int bx) {
+                    bool f32, int bx, const AdjMask& mask) {
   if (bx == 128)
-    launch_lattice_bx<128>(s, g, P, src, dst, fields, f32);
+    launch_lattice_bx<128>(s, g, P, src, dst, fields, f32, mask);
   else if (bx == 512)
-    launch_lattice_bx<512>(s, g, P, src, dst, fields, f32);
+    launch_lattice_bx<512>(s, g, P, src, dst, fields, f32, mask);
   else
-    launch_lattice_bx<256>(s, g, P, src, dst, fields, f32);
+    launch_lattice_bx<256>(s, g, P, src, dst, fields, f32, mask);
 }
 
 // Re-collide the adjoint nodes with the IBM-corrected velocity u = u* + g and
@@ -265,6 +267,7 @@ __device__ __forceinline__ void adj_collide_body(const Grid& g, const MrtParams&
 #pragma unroll
     for (int k = 0; k < 9; k++) Store<S>::st(dst, (long long)k * g.ps + o, out[k], P.w[k]);
     if (FIELDS) {
+      fs.rho[o] = rho;
       fs.u[o] = u;
       fs.v[o] = v;
     }
@@ -683,18 +686,21 @@ void launch_copy_ring_column(hipStream_t s, const Grid& g, const MrtParams& P, c
 
 int bc_ring_blocks(const Grid& g) { return (2 * (g.ny - 2) + 2 * g.nx + 255) / 256; }
 
+// masked = 0 (tile timing before any geometry exists): every interior node
 template <bool F32, bool FIELDS, int BX>
 __global__ __launch_bounds__(BX) void kb_lattice(const EpDesc* __restrict__ eps, const int2* __restrict__ items,
-                                                 int k) {
+                                                 int k, int masked) {
   const int2 it = items[blockIdx.x];
   const EpDesc& D = eps[it.x];
   const int n = D.n0 + k, cur = (D.cur0 + k) & 1;
+  AdjMask am = adj_mask(D.p, D.G, ep_slot(D, k));
+  if (!masked) am.W = 0;
   if (F32)
     lattice_f32_body<FIELDS, BX>(D.g, D.P, (const float*)D.fbuf[cur], (float*)D.fbuf[cur ^ 1], D.fslot[n % 3],
-                                 D.tiles_x, it.y);
+                                 D.tiles_x, it.y, am);
   else
     lattice_body<FIELDS, BX>(D.g, D.P, (const double*)D.fbuf[cur], (double*)D.fbuf[cur ^ 1], D.fslot[n % 3],
-                             D.tiles_x, it.y);
+                             D.tiles_x, it.y, am);
 }
 
 template <typename S, bool FIELDS>
@@ -727,26 +733,27 @@ void launch_batch_force_row(hipStream_t s, const EpDesc* eps, int count, int k) 
 }
 
 template <int BX>
-void launch_batch_lattice_bx(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields) {
+void launch_batch_lattice_bx(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields,
+                             int masked) {
   dim3 grid(it.n_lattice), block(BX);
   if (f32) {
-    if (fields) hipLaunchKernelGGL((kb_lattice<true, true, BX>), grid, block, 0, s, eps, it.lattice, k);
-    else hipLaunchKernelGGL((kb_lattice<true, false, BX>), grid, block, 0, s, eps, it.lattice, k);
+    if (fields) hipLaunchKernelGGL((kb_lattice<true, true, BX>), grid, block, 0, s, eps, it.lattice, k, masked);
+    else hipLaunchKernelGGL((kb_lattice<true, false, BX>), grid, block, 0, s, eps, it.lattice, k, masked);
   } else {
-    if (fields) hipLaunchKernelGGL((kb_lattice<false, true, BX>), grid, block, 0, s, eps, it.lattice, k);
-    else hipLaunchKernelGGL((kb_lattice<false, false, BX>), grid, block, 0, s, eps, it.lattice, k);
+    if (fields) hipLaunchKernelGGL((kb_lattice<false, true, BX>), grid, block, 0, s, eps, it.lattice, k, masked);
+    else hipLaunchKernelGGL((kb_lattice<false, false, BX>), grid, block, 0, s, eps, it.lattice, k, masked);
   }
 }
 
 void launch_batch_lattice(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields,
-                          int bx) {
+                          int bx, bool masked) {
   if (it.n_lattice <= 0) return;
   if (bx == 128)
-    launch_batch_lattice_bx<128>(s, eps, it, k, f32, fields);
+    launch_batch_lattice_bx<128>(s, eps, it, k, f32, fields, masked ? 1 : 0);
   else if (bx == 512)
-    launch_batch_lattice_bx<512>(s, eps, it, k, f32, fields);
+    launch_batch_lattice_bx<512>(s, eps, it, k, f32, fields, masked ? 1 : 0);
   else
-    launch_batch_lattice_bx<256>(s, eps, it, k, f32, fields);
+    launch_batch_lattice_bx<256>(s, eps, it, k, f32, fields, masked ? 1 : 0);
 }
 
 void launch_batch_bc_ring(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields) {

@@ -136,6 +136,25 @@ contains
         enddo
     end subroutine
 
+    ! Zone headers are one record each: integers in 12 columns, reals with 17
+    ! significant digits (F25.17 for 1e-3 <= |x| < 1e8 and zero, ES25.16
+    ! otherwise), the layout of list-directed output without its line breaks.
+    function int_text(v) result(t)
+        integer, intent(in) :: v
+        character(len=12) :: t
+        write(t, '(i12)') v
+    end function
+
+    function real_text(x) result(t)
+        real(c_double), intent(in) :: x
+        character(len=25) :: t
+        if (x == 0d0 .or. (abs(x) >= 1d-3 .and. abs(x) < 1d8)) then
+            write(t, '(f25.17)') x
+        else
+            write(t, '(es25.16)') x
+        endif
+    end function
+
     subroutine make_dir(path)
         character(len=*), intent(in) :: path
         integer :: stat
@@ -163,8 +182,8 @@ contains
         npts = wing_count * points_per_wing
         open(newunit=fu, file=trim(path), status='replace', action='write')
         write(fu, '(a)') 'VARIABLES = "X" "Y" "U" "V" "Presure" "Vorticity" "body" "dvordt" '
-        write(fu, *) 'ZONE T="Flow feild", I=', (nx - 1) / STRIDE_X + 1, ',J=', (ny - 1) / STRIDE_Y + 1, &
-                     ',SOLUTIONTIME=', tsol
+        write(fu, '(a)') ' ZONE T="Flow feild", I=' // int_text((nx - 1) / STRIDE_X + 1) // ',J=' // &
+            int_text((ny - 1) / STRIDE_Y + 1) // ',SOLUTIONTIME=' // real_text(tsol)
         do j = 1, ny, STRIDE_Y
             do i = 1, nx, STRIDE_X
                 if (i == nx) then
@@ -186,9 +205,9 @@ contains
                     u(i, j) / uu, v(i, j) / uu, pre(i, j), dvx - duy, dble(body(i, j)), 0.0d0
             enddo
         enddo
-        write(fu, *) 'ZONE T="airfoil", NODES=', npts + wing_count, ' ELEMENTS,=', &
-                     wing_count * (points_per_wing - 1), &
-                     ', DATAPACKING=POINT, ZONETYPE=FETRIANGLE', ',SOLUTIONTIME=', tsol
+        write(fu, '(a)') ' ZONE T="airfoil", NODES=' // int_text(npts + wing_count) // ' ELEMENTS,=' // &
+            int_text(wing_count * (points_per_wing - 1)) // ', DATAPACKING=POINT, ZONETYPE=FETRIANGLE' // &
+            ',SOLUTIONTIME=' // real_text(tsol)
         do k = 1, npts
             write(fu, '(8f14.8)') xb1(k) + shift, yb1(k), 255d0, 0d0, 0d0, 0d0, 0d0, 0d0
         enddo
