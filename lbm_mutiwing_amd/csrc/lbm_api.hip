@@ -8,7 +8,6 @@
 #include <cstring>
 #include <vector>
 
-#include "lbm_flow.h"
 #include "lbm_internal.h"
 
 using namespace lbm;
@@ -105,8 +104,11 @@ struct lbm_handle {
   GeoBuffers G;
   FlowBuffers F;
   RingBuffers R;
-  hipStream_t stream = nullptr;      // step kernels, copies
+  hipStream_t stream = nullptr;      // lattice kernels, copies
+  hipStream_t stream_ibm = nullptr;  // flow part of the IBM, overlapped with the lattice kernel
   hipStream_t stream_geo = nullptr;  // IBM geometry of the coming steps
+  hipStream_t stream_bc = nullptr;   // boundary ring, concurrent with the IBM flow chain
+  hipEvent_t ev_ibm = nullptr, ev_step = nullptr, ev_bc = nullptr;
   hipEvent_t ev_geo[2] = {nullptr, nullptr};  // geometry of a slot set is ready
   hipEvent_t ev_rel[2] = {nullptr, nullptr};  // last step using a slot set has finished
   int device = 0;
@@ -150,16 +152,6 @@ struct lbm_handle {
   double motion_end_time = 0.0;
   std::vector<Frame> frames;         // per step of the current call
   int lat_bx = 0;                    // lattice tile width (0: not tuned yet)
-  // step kernel: phase counters, launch counters, descriptor and work list
-  PhaseSync* d_phase = nullptr;
-  StepSync* d_sync = nullptr;
-  EpDesc* d_desc = nullptr;
-  int2* d_slots = nullptr;
-  size_t slots_cap = 0;
-  int2* d_ibm = nullptr;
-  size_t ibm_cap = 0;
-  int plan_tile = 0;                 // tile width of the uploaded work list (0: none)
-  int n_slots = 0, n_ibm = 0;
   // lattice-kernel timing (lbm_set_timing)
   bool timing = false;
   std::vector<hipEvent_t> ev;        // start/stop pairs of the current call
@@ -227,17 +219,7 @@ const char* dev_error_text(int code) {
     case DERR_WINGS_COUPLED: return "support boxes of two wings share a lattice node";
     case DERR_NOT_SPD: return "IBM matrix lost positive definiteness";
     case DERR_MOTION: return "motion source not initialised (call lbm_set_motion_table)";
-    case DERR_SYNC: return "an IBM work item of the step kernel timed out waiting for its inputs";
     default: return "unknown device error";
-  }
-}
-
-int dev_error_code(int code) {
-  switch (code) {
-    case DERR_MOTION: return LBM_E_STATE;
-    case DERR_NOT_SPD: return LBM_E_NUMERIC;
-    case DERR_SYNC: return LBM_E_DEVICE;
-    default: return LBM_E_RANGE;
   }
 }
 
@@ -266,16 +248,16 @@ void free_all(lbm_handle* h) {
   dfree(h->d_rows);
   dfree(h->d_mrows);
   dfree(h->d_rowsum);
-  dfree(h->d_phase);
-  dfree(h->d_sync);
-  dfree(h->d_desc);
-  dfree(h->d_slots);
-  dfree(h->d_ibm);
+  if (h->ev_ibm) (void)hipEventDestroy(h->ev_ibm);
+  if (h->ev_step) (void)hipEventDestroy(h->ev_step);
   for (int s = 0; s < 2; s++) {
     if (h->ev_geo[s]) (void)hipEventDestroy(h->ev_geo[s]);
     if (h->ev_rel[s]) (void)hipEventDestroy(h->ev_rel[s]);
   }
+  if (h->stream_ibm) (void)hipStreamDestroy(h->stream_ibm);
   if (h->stream_geo) (void)hipStreamDestroy(h->stream_geo);
+  if (h->stream_bc) (void)hipStreamDestroy(h->stream_bc);
+  if (h->ev_bc) (void)hipEventDestroy(h->ev_bc);
   for (auto e : h->ev) (void)hipEventDestroy(e);
   h->ev.clear();
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -309,29 +291,28 @@ int ensure_frames(lbm_handle* h, size_t steps) {
   return 0;
 }
 
-// Lattice tile autotuner: every tile shape of kTileWidths (256x1, 128x2 and
-// 64x4 nodes per 256-thread workgroup) is timed on launch() in three
-// interleaved rounds (per round and shape: one warm-up launch, then three
-// timed launches between HIP events on stream s); each shape keeps its
-// fastest round, and the fastest shape is returned, 256x1 on a tie within
-// 1 %.  LBM_LATTICE_TILE=<width> skips the timing.
+// Lattice-kernel autotuner: every tile width of kLatticeTiles is timed on
+// launch() in three interleaved rounds (per round and width: one warm-up
+// launch, then three timed launches between HIP events on stream s); each
+// width keeps its fastest round, and the fastest width is returned, 256 on a
+// tie within 1 %.  LBM_LATTICE_TILE=<width> skips the timing.
 template <typename F>
 int autotune_tile(hipStream_t s, F launch) {
   if (const char* e = getenv("LBM_LATTICE_TILE")) {
     const int v = atoi(e);
-    for (int bx : kTileWidths)
+    for (int bx : kLatticeTiles)
       if (bx == v) return v;
   }
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return 256;
-  constexpr int nt = sizeof(kTileWidths) / sizeof(kTileWidths[0]);
+  constexpr int nt = sizeof(kLatticeTiles) / sizeof(kLatticeTiles[0]);
   float best_ms[nt];
   for (int q = 0; q < nt; q++) best_ms[q] = 1e30f;
   for (int round = 0; round < 3; round++)
     for (int q = 0; q < nt; q++) {
-      launch(kTileWidths[q]);
+      launch(kLatticeTiles[q]);
       (void)hipEventRecord(e0, s);
-      for (int r = 0; r < 3; r++) launch(kTileWidths[q]);
+      for (int r = 0; r < 3; r++) launch(kLatticeTiles[q]);
       (void)hipEventRecord(e1, s);
       (void)hipEventSynchronize(e1);
       float ms = 0.f;
@@ -342,130 +323,26 @@ int autotune_tile(hipStream_t s, F launch) {
   int best = 256;
   float ref = 1e30f;
   for (int q = 0; q < nt; q++)
-    if (kTileWidths[q] == 256) ref = best_ms[q];
+    if (kLatticeTiles[q] == 256) ref = best_ms[q];
   float best_t = ref;
   for (int q = 0; q < nt; q++)
     if (best_ms[q] < 0.99f * ref && best_ms[q] < best_t) {
       best_t = best_ms[q];
-      best = kTileWidths[q];
+      best = kLatticeTiles[q];
     }
   return best;
 }
 
-// Step descriptor of instance h for a call that starts now (substep 0 is
-// step h->n), with ns steps per geometry batch and lattice tiles tile_w wide.
-EpDesc make_desc(lbm_handle* h, int ns, int tile_w) {
-  EpDesc D{};
-  D.g = h->g;
-  D.P = h->mrt;
-  D.p = h->ip;
-  D.G = h->G;
-  D.F = h->F;
-  D.R = h->R;
-  D.st = h->d_st;
-  D.fbuf[0] = h->fbuf[0];
-  D.fbuf[1] = h->fbuf[1];
-  for (int q = 0; q < 3; q++) D.fslot[q] = h->slot[q];
-  D.ph = h->d_phase;
-  D.cur0 = h->cur;
-  D.n0 = h->n;
-  D.ns = ns;
-  D.bc0 = (int)(h->batch_count & 1u);
-  D.tile_w = tile_w;
-  lattice_tiles(h->g, tile_w, &D.tiles_x);
-  D.bpw = adj_blocks_per_wing(h->ip);
-  for (int q = 0; q < NPHASE; q++) D.nph[q] = ibm_items(h->ip, q);
-  return D;
-}
-
-// Work list of one step-kernel launch over episodes desc[0..count).  The IBM
-// items are listed phase by phase (their ticket order); their slots sit in
-// the grid after growing runs of lattice tiles, one run per phase, so that
-// each phase's inputs are usually complete when its slots start.
-struct WorkList {
-  std::vector<int2> slots, ibm;
-};
-
-void build_work(const EpDesc* desc, int count, bool with_ibm, WorkList& wl) {
-  wl.slots.clear();
-  wl.ibm.clear();
-  std::vector<int2> lat;
-  for (int e = 0; e < count; e++) {
-    const int nt = lattice_tiles(desc[e].g, desc[e].tile_w, nullptr);
-    for (int t = 0; t < nt; t++) lat.push_back(work_item(e, SLOT_LATTICE, t));
-  }
-  if (!with_ibm) {
-    wl.slots = lat;
-    return;
-  }
-  size_t nph[NPHASE] = {0, 0, 0};
-  for (int ph = 0; ph < NPHASE; ph++)
-    for (int e = 0; e < count; e++) {
-      for (int q = 0; q < desc[e].nph[ph]; q++) wl.ibm.push_back(work_item(e, ph, q));
-      nph[ph] += (size_t)desc[e].nph[ph];
-    }
-  // a tenth of the lattice tiles before each later phase
-  const size_t run = std::max<size_t>(std::min<size_t>(lat.size(), 256), std::min<size_t>(4096, lat.size() / 10));
-  size_t l = 0;
-  auto ibm_slots = [&](size_t n) {
-    for (size_t q = 0; q < n; q++) wl.slots.push_back(work_item(0, SLOT_IBM, 0));
-  };
-  ibm_slots(nph[PH_RHS]);
-  for (int e = 0; e < count; e++) {
-    const int nr = bc_ring_blocks(desc[e].g);
-    for (int q = 0; q < nr; q++) wl.slots.push_back(work_item(e, SLOT_RING, q));
-  }
-  for (int ph = PH_MATVEC; ph < NPHASE; ph++) {
-    for (size_t q = 0; q < run && l < lat.size(); q++) wl.slots.push_back(lat[l++]);
-    ibm_slots(nph[ph]);
-  }
-  while (l < lat.size()) wl.slots.push_back(lat[l++]);
-}
-
-// Upload a work list into a pair of growable device arrays.
-int upload_work(const WorkList& wl, int2** d_slots, size_t* slots_cap, int2** d_ibm, size_t* ibm_cap,
-                char* msg) {
-  auto put = [&](const std::vector<int2>& v, int2** d, size_t* cap) -> hipError_t {
-    if (v.size() > *cap) {
-      if (*d) (void)hipFree(*d);
-      *d = nullptr;
-      *cap = 0;
-      hipError_t e = hipMalloc((void**)d, sizeof(int2) * std::max<size_t>(v.size(), 1));
-      if (e != hipSuccess) return e;
-      *cap = std::max<size_t>(v.size(), 1);
-    }
-    return v.empty() ? hipSuccess : hipMemcpy(*d, v.data(), sizeof(int2) * v.size(), hipMemcpyHostToDevice);
-  };
-  hipError_t e = put(wl.slots, d_slots, slots_cap);
-  if (e == hipSuccess) e = put(wl.ibm, d_ibm, ibm_cap);
-  if (e != hipSuccess) {
-    snprintf(msg, 512, "work list upload: %s", hipGetErrorString(e));
-    return LBM_E_DEVICE;
-  }
-  return 0;
-}
-
-// Tile shape of one instance, timed on its own buffers with lattice-only
-// launches of the step kernel: they write interior nodes of the idle
-// population buffer, which the next step overwrites entirely.
-int tune_instance(lbm_handle* h) {
-  if (h->lat_bx) return 0;
-  WorkList wl;
-  int rc = 0;
-  h->lat_bx = autotune_tile(h->stream, [&](int bx) {
-    if (rc) return;
-    const EpDesc D = make_desc(h, h->nslot, bx);
-    build_work(&D, 1, false, wl);
-    if ((rc = upload_work(wl, &h->d_slots, &h->slots_cap, &h->d_ibm, &h->ibm_cap, h->msg))) return;
-    if (hipMemcpy(h->d_desc, &D, sizeof(EpDesc), hipMemcpyHostToDevice) != hipSuccess) {
-      rc = fail(h, LBM_E_DEVICE, "descriptor upload failed");
-      return;
-    }
-    const StepItems it{h->d_slots, (int)wl.slots.size(), h->d_ibm, 0};
-    launch_step(h->stream, h->d_desc, it, h->d_sync, 0, h->nslot, h->f32, false, false);
-  });
-  h->plan_tile = 0;  // the uploaded list is a timing list
-  return rc;
+// Tile width of one instance, timed on its own buffers: the launches write
+// interior nodes of the idle population buffer, which the next step
+// overwrites entirely.
+void tune_instance(lbm_handle* h) {
+  if (h->lat_bx) return;
+  const void* src = h->fbuf[h->cur];
+  void* dst = h->fbuf[1 - h->cur];
+  const AdjMask none{nullptr, nullptr, 0, 0};
+  h->lat_bx = autotune_tile(h->stream, [&](int bx) { launch_lattice(h->stream, h->g, h->mrt, src, dst, nullptr,
+                                                                     h->f32, bx, none); });
 }
 
 // Per-slot geometry arrays and flow buffers of the IBM.  The number of steps
@@ -500,9 +377,9 @@ int alloc_ibm(lbm_handle* h, const std::vector<double>& hx, const std::vector<do
       (rc = ialloc(h, &G.sinv, h->W > 0 ? ns * W * spw * spw : 1)) ||
       (rc = ialloc(h, &G.map, h->W > 0 ? ns * W * 2 * (size_t)p.map_cap : 1)))
     return rc;
-  const int nb = adj_blocks_per_wing(p);
+  const int bpw = adj_collide_blocks_per_wing(p);
   if ((rc = ialloc(h, &F.cvec, 2 * nseg)) || (rc = ialloc(h, &F.yvec, 2 * nseg)) ||
-      (rc = ialloc(h, &F.partial, 2 * 3 * MAXW * (size_t)nb)))
+      (rc = ialloc(h, &F.adj_g, 2 * cap)) || (rc = ialloc(h, &F.partial, 2 * 3 * MAXW * (size_t)bpw)))
     return rc;
   G.step_table = nullptr;
   G.stream_rows = h->d_stream;
@@ -618,7 +495,12 @@ int lbm_create(const lbm_config* cfg, lbm_handle** out) {
 
   int rc = 0;
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&h->stream_ibm, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&h->stream_geo, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&h->stream_bc, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&h->ev_bc, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&h->ev_ibm, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&h->ev_step, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&h->ev_geo[0], hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&h->ev_geo[1], hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&h->ev_rel[0], hipEventDisableTiming) != hipSuccess ||
@@ -663,16 +545,7 @@ int lbm_create(const lbm_config* cfg, lbm_handle** out) {
   ALLOC(h->d_stream, 2 * LBM_MAX_MOTION_COLS);
   ALLOC(h->d_st, 1);
   ALLOC(h->d_rowsum, h->row_len);
-  ALLOC(h->d_phase, 1);
-  ALLOC(h->d_sync, 1);
-  ALLOC(h->d_desc, 1);
 #undef ALLOC
-  if (hipMemset(h->d_phase, 0, sizeof(PhaseSync)) != hipSuccess ||
-      hipMemset(h->d_sync, 0, sizeof(StepSync)) != hipSuccess) {
-    free_all(h);
-    delete h;
-    return fail(nullptr, LBM_E_DEVICE, "counter initialisation failed");
-  }
   // markers: read_boundary (airfoil.f90:229-241) plus the flow-centre offset
   std::vector<double> hx(h->npts), hy(h->npts);
   for (int q = 0; q < h->npts; q++) {
@@ -704,7 +577,7 @@ int lbm_destroy(lbm_handle* h) {
 int lbm_reset(lbm_handle* h) {
   if (!h) return LBM_E_INVALID;
   HIPCHECK(h, hipStreamSynchronize(h->stream_geo));
-  HIPCHECK(h, hipStreamSynchronize(h->stream));
+  HIPCHECK(h, hipStreamSynchronize(h->stream_ibm));
   h->cur = 0;
   launch_init_fields(h->stream, h->g, h->mrt, h->cfg.uu, h->feq, h->fbuf[0], h->R, h->slot[0], h->f32);
   launch_ring_moments(h->stream, h->g, h->R);
@@ -858,32 +731,16 @@ int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motio
     HIPCHECK(h, hipStreamSynchronize(h->stream_geo));
     HIPCHECK(h, hipMemcpy(h->d_frame, h->frames.data(), sizeof(Frame) * nsteps, hipMemcpyHostToDevice));
   }
-  if (nsteps > 0) {
-    rc = tune_instance(h);
-    if (rc) return rc;
-  }
+  if (nsteps > 0) tune_instance(h);
   HIPCHECK(h, hipMemsetAsync(h->d_rowsum, 0, sizeof(double) * h->row_len, h->stream));
   HIPCHECK(h, hipMemsetAsync(&h->d_st->error, 0, sizeof(int), h->stream));
   HIPCHECK(h, hipStreamSynchronize(h->stream));
-  const int ns = h->nslot;
-  // the step descriptor of the call and the instance's work list
-  EpDesc D = make_desc(h, ns, h->lat_bx);
-  if (nsteps > 0 && h->plan_tile != h->lat_bx) {
-    WorkList wl;
-    build_work(&D, 1, true, wl);
-    rc = upload_work(wl, &h->d_slots, &h->slots_cap, &h->d_ibm, &h->ibm_cap, h->msg);
-    if (rc) return rc;
-    h->n_slots = (int)wl.slots.size();
-    h->n_ibm = (int)wl.ibm.size();
-    h->plan_tile = h->lat_bx;
-  }
-  if (nsteps > 0) HIPCHECK(h, hipMemcpy(h->d_desc, &D, sizeof(EpDesc), hipMemcpyHostToDevice));
-  const StepItems items{h->d_slots, h->n_slots, h->d_ibm, h->n_ibm};
   // Geometry batches of ns steps alternate between two slot sets.  Batch b
   // of the call is computed on the geometry stream one batch ahead of the
   // flow: it is issued when batch b-1 starts, right after its slot set has
-  // been released by the last step of batch b-2, so it runs while batch b-1
-  // steps.  Only the first batch of a call is waited for.
+  // been released by the force row of the last step of batch b-2, so it runs
+  // while batch b-1 steps.  Only the first batch of a call is waited for.
+  const int ns = h->nslot;
   const unsigned bc0 = h->batch_count;
   const int nbatch = h->W > 0 ? (nsteps + ns - 1) / ns : 0;
   auto set_of = [&](int b) { return (int)((bc0 + (unsigned)b) & 1u); };
@@ -897,27 +754,55 @@ int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motio
   };
   rc = issue_geometry(0);
   if (rc) return rc;
-  // One step-kernel launch per step on one stream: the lattice tiles, the
-  // boundary ring and the IBM chain of the step with its force row.
-  int set = 0, slot = 0, prev_set = 0;
+  int set = 0, slot = 0, prev_set = 0, prev_slot = 0;
   for (int m = n0; m <= n_last; m++) {
     const int k = m - n0;
     if (h->W > 0 && k % ns == 0) {
       prev_set = set;
       set = set_of(k / ns);
-      if (k > 0) HIPCHECK(h, hipEventRecord(h->ev_rel[prev_set], h->stream));  // batch k/ns-1 has run
-      rc = issue_geometry(k / ns + 1);
-      if (rc) return rc;
-      HIPCHECK(h, hipStreamWaitEvent(h->stream, h->ev_geo[set], 0));
+      HIPCHECK(h, hipStreamWaitEvent(h->stream_ibm, h->ev_geo[set], 0));
+      HIPCHECK(h, hipStreamWaitEvent(h->stream, h->ev_geo[set], 0));  // the lattice kernel reads the claim maps
     }
     slot = set * ns + k % ns;
     const int s = m % 3;
     double brp = h->W > 0 ? h->body_rel_pos[0] : 0.0;
     const int dr = brp >= 1.0 ? 1 : (brp <= -1.0 ? -1 : 0);
     const bool want_fields = (m >= n_last - 2) || dr != 0;
+    FieldSlot* fs = want_fields ? &h->slot[s] : nullptr;
+    const void* src = h->fbuf[h->cur];
+    void* dst = h->fbuf[1 - h->cur];
+    // Every node of dst has exactly one writer this step: the full-lattice
+    // kernel (lattice stream) writes the interior nodes other than the
+    // adjoint nodes; the IBM flow kernels and the adjoint re-collision
+    // (second stream) write the adjoint nodes; the boundary ring (third
+    // stream) writes the ring.  All three read only src and run
+    // concurrently once the previous step has finished; the next step's
+    // lattice kernel waits for the other two through ev_ibm.  The previous
+    // step's force row follows on the IBM stream; a geometry slot set is
+    // released once the force row of its last step is written.
+    HIPCHECK(h, hipEventRecord(h->ev_step, h->stream));
+    HIPCHECK(h, hipStreamWaitEvent(h->stream_ibm, h->ev_step, 0));
+    HIPCHECK(h, hipStreamWaitEvent(h->stream_bc, h->ev_step, 0));
+    launch_bc_ring(h->stream_bc, h->g, h->mrt, h->R, src, dst, fs, h->f32);
+    HIPCHECK(h, hipEventRecord(h->ev_bc, h->stream_bc));
+    // the previous step's force row (its partial sums are kept by step parity)
+    if (k > 0) {
+      launch_force_row(h->stream_ibm, h->ip, h->G, prev_slot, h->F, h->d_st, m - 1, k - 1, h->row_len);
+      if (h->W > 0 && k % ns == 0) HIPCHECK(h, hipEventRecord(h->ev_rel[prev_set], h->stream_ibm));
+    }
+    if (h->W > 0 && k % ns == 0) {
+      rc = issue_geometry(k / ns + 1);
+      if (rc) return rc;
+    }
+    launch_ibm_flow(h->stream_ibm, h->ip, h->mrt, h->G, slot, h->F, h->g, src, h->f32);
+    launch_adj_collide(h->stream_ibm, h->g, h->mrt, h->ip, h->G, slot, h->F, m, src, dst, fs, h->f32);
+    HIPCHECK(h, hipStreamWaitEvent(h->stream_ibm, h->ev_bc, 0));
+    HIPCHECK(h, hipEventRecord(h->ev_ibm, h->stream_ibm));
     if (h->timing) HIPCHECK(h, hipEventRecord(h->ev[2 * k], h->stream));
-    launch_step(h->stream, h->d_desc, items, h->d_sync, k, ns, h->f32, want_fields, true);
+    launch_lattice(h->stream, h->g, h->mrt, src, dst, fs, h->f32, h->lat_bx, adj_mask(h->ip, h->G, slot));
     if (h->timing) HIPCHECK(h, hipEventRecord(h->ev[2 * k + 1], h->stream));
+    HIPCHECK(h, hipStreamWaitEvent(h->stream, h->ev_ibm, 0));
+    prev_slot = slot;
     if (want_fields) h->slot_step[s] = m;
     h->cur = 1 - h->cur;
     if (dr != 0) {  // move_mesh (move_mesh.f90:17-58, 171-214)
@@ -931,11 +816,6 @@ int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motio
       launch_shift_x(h->stream, h->g, dr, cs.u, h->spare.u, 1, false);
       launch_shift_x(h->stream, h->g, dr, cs.v, h->spare.v, 1, false);
       std::swap(h->slot[s], h->spare);
-      // the descriptor follows the extra buffer flip and the swapped slot
-      D.cur0 ^= 1;
-      D.fslot[s] = h->slot[s];
-      HIPCHECK(h, hipStreamSynchronize(h->stream));
-      HIPCHECK(h, hipMemcpy(h->d_desc, &D, sizeof(EpDesc), hipMemcpyHostToDevice));
       h->body_rel_pos[0] = h->body_rel_pos[0] - (double)dr;
       h->movenum += dr;
       update_body_extra(h);
@@ -944,7 +824,12 @@ int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motio
     }
     if (h->flag2 > 0) h->flag2 = h->flag2 - 1;
   }
-  if (nsteps > 0 && h->W > 0) HIPCHECK(h, hipEventRecord(h->ev_rel[set], h->stream));
+  if (nsteps > 0) {
+    HIPCHECK(h, hipEventRecord(h->ev_ibm, h->stream_ibm));   // the previous force row is done
+    HIPCHECK(h, hipStreamWaitEvent(h->stream, h->ev_ibm, 0));
+    launch_force_row(h->stream, h->ip, h->G, slot, h->F, h->d_st, n_last, nsteps - 1, h->row_len);
+    if (h->W > 0) HIPCHECK(h, hipEventRecord(h->ev_rel[set], h->stream));
+  }
   h->batch_count = bc0 + (unsigned)nbatch;
   if (nsteps > 0 && h->W > 0) h->last_slot = slot;
   h->n = std::max(n0, n_last + 1);
@@ -970,8 +855,8 @@ int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motio
   DevStep st_host;
   HIPCHECK(h, hipMemcpy(&st_host, h->d_st, sizeof(DevStep), hipMemcpyDeviceToHost));
   if (st_host.error != 0)
-    return fail(h, dev_error_code(st_host.error), "step failed: %s (info %d)", dev_error_text(st_host.error),
-                st_host.error_info);
+    return fail(h, st_host.error == DERR_MOTION ? LBM_E_STATE : (st_host.error == DERR_NOT_SPD ? LBM_E_NUMERIC : LBM_E_RANGE),
+                "step failed: %s (info %d)", dev_error_text(st_host.error), st_host.error_info);
   if (h->timing)
     for (int i = 0; i < nsteps; i++) {
       float ms = 0.f;
@@ -1205,8 +1090,8 @@ struct BatchGraph {
 };
 
 struct lbm_batch {
-  hipStream_t stream = nullptr, stream_geo = nullptr;
-  hipEvent_t ev_start = nullptr;
+  hipStream_t stream = nullptr, stream_ibm = nullptr, stream_geo = nullptr, stream_bc = nullptr;
+  hipEvent_t ev_ibm = nullptr, ev_step = nullptr, ev_bc = nullptr, ev_start = nullptr;
   std::vector<BatchGraph> graphs;
   unsigned long long calls = 0;
   bool use_graphs = true;
@@ -1214,13 +1099,8 @@ struct lbm_batch {
   int device = 0;
   EpDesc* d_desc = nullptr;
   size_t desc_cap = 0;
-  int2* d_geo = nullptr;             // geometry work items
-  size_t geo_cap = 0;
-  int2* d_slots = nullptr;           // step-kernel work list
-  size_t slots_cap = 0;
-  int2* d_ibm = nullptr;
-  size_t ibm_cap = 0;
-  StepSync* d_sync = nullptr;
+  int2* d_items = nullptr;
+  size_t items_cap = 0;
   int lat_bx = 0;                    // lattice tile width (tuned on the first call)
   char msg[512] = "";
 };
@@ -1246,11 +1126,11 @@ void batch_free(lbm_batch* b) {
     if (g.exec) (void)hipGraphExecDestroy(g.exec);
   b->graphs.clear();
   if (b->ev_start) (void)hipEventDestroy(b->ev_start);
-  for (void* p : {(void*)b->d_desc, (void*)b->d_geo, (void*)b->d_slots, (void*)b->d_ibm, (void*)b->d_sync})
-    if (p) (void)hipFree(p);
-  for (hipEvent_t e : {b->ev_geo[0], b->ev_geo[1], b->ev_rel[0], b->ev_rel[1]})
+  if (b->d_desc) (void)hipFree(b->d_desc);
+  if (b->d_items) (void)hipFree(b->d_items);
+  for (hipEvent_t e : {b->ev_ibm, b->ev_step, b->ev_bc, b->ev_geo[0], b->ev_geo[1], b->ev_rel[0], b->ev_rel[1]})
     if (e) (void)hipEventDestroy(e);
-  for (hipStream_t s : {b->stream, b->stream_geo})
+  for (hipStream_t s : {b->stream, b->stream_ibm, b->stream_geo, b->stream_bc})
     if (s) (void)hipStreamDestroy(s);
 }
 
@@ -1275,14 +1155,17 @@ int lbm_batch_create(lbm_batch** out) {
   lbm_batch* b = new lbm_batch();
   (void)hipGetDevice(&b->device);
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&b->stream_ibm, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&b->stream_geo, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&b->stream_bc, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_bc, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&b->ev_start, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_ibm, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_step, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&b->ev_geo[0], hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&b->ev_geo[1], hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&b->ev_rel[0], hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&b->ev_rel[1], hipEventDisableTiming) != hipSuccess ||
-      hipMalloc((void**)&b->d_sync, sizeof(StepSync)) != hipSuccess ||
-      hipMemset(b->d_sync, 0, sizeof(StepSync)) != hipSuccess) {
+      hipEventCreateWithFlags(&b->ev_rel[1], hipEventDisableTiming) != hipSuccess) {
     batch_free(b);
     delete b;
     return LBM_E_DEVICE;
@@ -1349,6 +1232,8 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
   }
   // per-instance call set-up: rows, frames (constant without moves), counters
   std::vector<EpDesc> desc(count);
+  std::vector<int2> items;
+  BatchItems it{};
   size_t lds_wing = 0, lds_solve = 0;
   for (int e = 0; e < count; e++) {
     lbm_handle* h = hs[e];
@@ -1356,124 +1241,169 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
     if (!rc) rc = ensure_frames(h, (size_t)steps);
     if (rc) return bfail(b, rc, "instance %d: %s", e, h->msg);
     BCHECK(b, hipStreamSynchronize(h->stream_geo));
-    BCHECK(b, hipStreamSynchronize(h->stream));
     h->frames.assign((size_t)steps, h->fr);
     BCHECK(b, hipMemcpy(h->d_frame, h->frames.data(), sizeof(Frame) * steps, hipMemcpyHostToDevice));
     BCHECK(b, hipMemset(h->d_rowsum, 0, sizeof(double) * h->row_len));
     BCHECK(b, hipMemset(&h->d_st->error, 0, sizeof(int)));
-    desc[e] = make_desc(h, ns, 256);  // tile width set below
+    EpDesc& D = desc[e];
+    D.g = h->g;
+    D.P = h->mrt;
+    D.p = h->ip;
+    D.G = h->G;
+    D.F = h->F;
+    D.R = h->R;
+    D.st = h->d_st;
+    D.fbuf[0] = h->fbuf[0];
+    D.fbuf[1] = h->fbuf[1];
+    for (int q = 0; q < 3; q++) D.fslot[q] = h->slot[q];
+    D.cur0 = h->cur;
+    D.n0 = h->n;
+    D.ns = ns;
+    D.bc0 = (int)(h->batch_count & 1u);
+    D.tiles_x = 0;  // set once the tile width is known
+    D.bpw = adj_collide_blocks_per_wing(h->ip);
+    D.pad0 = D.pad1 = 0;
     if (h->W > 0) {
       lds_wing = std::max(lds_wing, geo_wing_lds_bytes(h->SPW));
       lds_solve = std::max(lds_solve, geo_solve_lds_bytes(h->ip));
     }
   }
-  auto upload_desc = [&]() -> int {
+  auto add = [&](int e, int nblk) {
+    for (int q = 0; q < nblk; q++) items.push_back(make_int2(e, q));
+  };
+  auto upload = [&]() -> int {
     if ((size_t)count > b->desc_cap) {
       if (b->d_desc) (void)hipFree(b->d_desc);
       b->d_desc = nullptr;
-      b->desc_cap = 0;
       BCHECK(b, hipMalloc((void**)&b->d_desc, sizeof(EpDesc) * count));
       b->desc_cap = count;
     }
+    if (items.size() > b->items_cap) {
+      if (b->d_items) (void)hipFree(b->d_items);
+      b->d_items = nullptr;
+      BCHECK(b, hipMalloc((void**)&b->d_items, sizeof(int2) * items.size()));
+      b->items_cap = items.size();
+    }
     BCHECK(b, hipStreamSynchronize(b->stream));
     BCHECK(b, hipMemcpy(b->d_desc, desc.data(), sizeof(EpDesc) * count, hipMemcpyHostToDevice));
+    BCHECK(b, hipMemcpy(b->d_items, items.data(), sizeof(int2) * items.size(), hipMemcpyHostToDevice));
     return 0;
   };
-  WorkList wl;
-  if (!b->lat_bx) {  // tile shape of the batch, timed on this composition with lattice-only launches
+  if (!b->lat_bx) {  // tile width of the batched lattice launch, timed on this composition
     int rc = 0;
     b->lat_bx = autotune_tile(b->stream, [&](int bx) {
       if (rc) return;
+      items.clear();
       for (int e = 0; e < count; e++) {
-        desc[e].tile_w = bx;
-        lattice_tiles(desc[e].g, bx, &desc[e].tiles_x);
+        desc[e].tiles_x = lattice_tiles_x(desc[e].g, bx);
+        add(e, desc[e].tiles_x * (desc[e].g.ny - 2));
       }
-      if ((rc = upload_desc())) return;
-      build_work(desc.data(), count, false, wl);
-      if ((rc = upload_work(wl, &b->d_slots, &b->slots_cap, &b->d_ibm, &b->ibm_cap, b->msg))) return;
-      const StepItems lt{b->d_slots, (int)wl.slots.size(), b->d_ibm, 0};
-      launch_step(b->stream, b->d_desc, lt, b->d_sync, 0, ns, f32, false, false);
+      if ((rc = upload())) return;
+      BatchItems lt{};
+      lt.lattice = b->d_items;
+      lt.n_lattice = (int)items.size();
+      launch_batch_lattice(b->stream, b->d_desc, lt, 0, f32, false, bx, false);
     });
     if (rc) return rc;
+    items.clear();
   }
-  for (int e = 0; e < count; e++) {
-    desc[e].tile_w = b->lat_bx;
-    lattice_tiles(desc[e].g, b->lat_bx, &desc[e].tiles_x);
-  }
-  {
-    const int rc = upload_desc();
-    if (rc) return rc;
-  }
-  build_work(desc.data(), count, true, wl);
-  {
-    const int rc = upload_work(wl, &b->d_slots, &b->slots_cap, &b->d_ibm, &b->ibm_cap, b->msg);
-    if (rc) return rc;
-  }
-  const StepItems items{b->d_slots, (int)wl.slots.size(), b->d_ibm, (int)wl.ibm.size()};
-  // geometry work items of a full and of the last (shorter) geometry batch:
-  // (episode, b*W + w)
-  std::vector<int2> gitems;
+  for (int e = 0; e < count; e++) desc[e].tiles_x = lattice_tiles_x(desc[e].g, b->lat_bx);
+  // work lists: lattice, ring, adjoint, segment, spread, then the geometry
+  // lists of a full and of the last (shorter) geometry batch
+  size_t off[8];
+  off[0] = items.size();
+  for (int e = 0; e < count; e++) add(e, desc[e].tiles_x * (desc[e].g.ny - 2));
+  off[1] = items.size();
+  for (int e = 0; e < count; e++) add(e, bc_ring_blocks(desc[e].g));
+  off[2] = items.size();
+  for (int e = 0; e < count; e++) add(e, desc[e].p.W * desc[e].bpw);
+  off[3] = items.size();
+  for (int e = 0; e < count; e++)
+    if (desc[e].p.W > 0) add(e, (desc[e].p.nseg + 3) / 4);
+  off[4] = items.size();
+  for (int e = 0; e < count; e++)
+    if (desc[e].p.W > 0) add(e, spread_blocks(desc[e].p));
+  off[5] = items.size();
   const int tail = steps % ns == 0 ? ns : steps % ns;
   for (int e = 0; e < count; e++)
-    for (int q = 0; q < std::min(ns, steps) * desc[e].p.W; q++) gitems.push_back(make_int2(e, q));
-  const size_t g_full = gitems.size();
+    if (desc[e].p.W > 0) add(e, std::min(ns, steps) * desc[e].p.W);
+  off[6] = items.size();
   for (int e = 0; e < count; e++)
-    for (int q = 0; q < tail * desc[e].p.W; q++) gitems.push_back(make_int2(e, q));
-  if (gitems.size() > b->geo_cap) {
-    if (b->d_geo) (void)hipFree(b->d_geo);
-    b->d_geo = nullptr;
-    b->geo_cap = 0;
-    BCHECK(b, hipMalloc((void**)&b->d_geo, sizeof(int2) * gitems.size()));
-    b->geo_cap = gitems.size();
+    if (desc[e].p.W > 0) add(e, tail * desc[e].p.W);
+  off[7] = items.size();
+  {
+    const int rc = upload();
+    if (rc) return rc;
   }
-  if (!gitems.empty())
-    BCHECK(b, hipMemcpy(b->d_geo, gitems.data(), sizeof(int2) * gitems.size(), hipMemcpyHostToDevice));
-  const int2* dg = b->d_geo;
-  // The launch sequence of the call: one step-kernel launch per substep on
-  // the batch stream, the geometry on its own stream joined through events,
-  // so the sequence can be captured as one graph.  Geometry batch bb is
-  // issued one batch ahead of the flow (when batch bb-1 starts): the first
-  // two wait on the start of the call (the previous call has completed),
-  // later ones on the release of their slot set by the last substep of
-  // batch bb-2.
+  const int2* di = b->d_items;
+  it.lattice = di + off[0]; it.n_lattice = (int)(off[1] - off[0]);
+  it.ring = di + off[1]; it.n_ring = (int)(off[2] - off[1]);
+  it.adj = di + off[2]; it.n_adj = (int)(off[3] - off[2]);
+  it.seg = di + off[3]; it.n_seg = (int)(off[4] - off[3]);
+  it.spread = di + off[4]; it.n_spread = (int)(off[5] - off[4]);
+  // The launch sequence of the call.  Every stream joins the lattice stream
+  // through events, so the sequence can be captured as one graph.  Geometry
+  // batch bb is issued one batch ahead of the flow (when batch bb-1 starts):
+  // the first two wait on the start of the call (the previous call has
+  // completed), later ones on the release of their slot set by the force row
+  // of the last step of batch bb-2.
   auto issue_geometry = [&](int bb) -> int {
     const int k0 = bb * ns;
     if (!wings || k0 >= steps) return 0;
     const int es = bb & 1;
     BCHECK(b, hipStreamWaitEvent(b->stream_geo, bb < 2 ? b->ev_start : b->ev_rel[es], 0));
     if (k0 + ns >= steps)
-      launch_batch_geometry(b->stream_geo, b->d_desc, dg + g_full, (int)(gitems.size() - g_full), k0, lds_wing,
-                            lds_solve);
+      launch_batch_geometry(b->stream_geo, b->d_desc, di + off[6], (int)(off[7] - off[6]), k0, lds_wing, lds_solve);
     else
-      launch_batch_geometry(b->stream_geo, b->d_desc, dg, (int)g_full, k0, lds_wing, lds_solve);
+      launch_batch_geometry(b->stream_geo, b->d_desc, di + off[5], (int)(off[6] - off[5]), k0, lds_wing, lds_solve);
     BCHECK(b, hipEventRecord(b->ev_geo[es], b->stream_geo));
     return 0;
   };
   auto issue = [&]() -> int {
-    BCHECK(b, hipEventRecord(b->ev_start, b->stream));
-    {
-      const int rc = issue_geometry(0);
+  BCHECK(b, hipEventRecord(b->ev_start, b->stream));
+  {
+    const int rc = issue_geometry(0);
+    if (rc) return rc;
+  }
+  for (int k = 0; k < steps; k++) {
+    if (wings && k % ns == 0) {
+      const int es = (k / ns) & 1;
+      BCHECK(b, hipStreamWaitEvent(b->stream_ibm, b->ev_geo[es], 0));
+      BCHECK(b, hipStreamWaitEvent(b->stream, b->ev_geo[es], 0));  // lattice reads the claim maps
+    }
+    // one writer per node, as in lbm_advance: lattice (all but the adjoint
+    // nodes), IBM stream (adjoint nodes), ring stream (boundary ring)
+    const bool fields = k >= steps - 3;
+    BCHECK(b, hipEventRecord(b->ev_step, b->stream));
+    BCHECK(b, hipStreamWaitEvent(b->stream_ibm, b->ev_step, 0));
+    BCHECK(b, hipStreamWaitEvent(b->stream_bc, b->ev_step, 0));
+    launch_batch_bc_ring(b->stream_bc, b->d_desc, it, k, f32, fields);
+    BCHECK(b, hipEventRecord(b->ev_bc, b->stream_bc));
+    if (k > 0) {
+      launch_batch_force_row(b->stream_ibm, b->d_desc, count, k - 1);
+      if (wings && k % ns == 0) BCHECK(b, hipEventRecord(b->ev_rel[((k - 1) / ns) & 1], b->stream_ibm));
+    }
+    if (wings && k % ns == 0) {
+      const int rc = issue_geometry(k / ns + 1);
       if (rc) return rc;
     }
-    for (int k = 0; k < steps; k++) {
-      if (wings && k % ns == 0) {
-        const int es = (k / ns) & 1;
-        if (k > 0) BCHECK(b, hipEventRecord(b->ev_rel[es ^ 1], b->stream));  // batch k/ns-1 has run
-        const int rc = issue_geometry(k / ns + 1);
-        if (rc) return rc;
-        BCHECK(b, hipStreamWaitEvent(b->stream, b->ev_geo[es], 0));
-      }
-      launch_step(b->stream, b->d_desc, items, b->d_sync, k, ns, f32, k >= steps - 3, true);
-    }
-    return 0;
+    launch_batch_ibm_flow(b->stream_ibm, b->d_desc, it, k, f32);
+    launch_batch_adj_collide(b->stream_ibm, b->d_desc, it, k, f32, fields);
+    BCHECK(b, hipStreamWaitEvent(b->stream_ibm, b->ev_bc, 0));
+    BCHECK(b, hipEventRecord(b->ev_ibm, b->stream_ibm));
+    launch_batch_lattice(b->stream, b->d_desc, it, k, f32, fields, b->lat_bx, true);
+    BCHECK(b, hipStreamWaitEvent(b->stream, b->ev_ibm, 0));
+  }
+  BCHECK(b, hipEventRecord(b->ev_ibm, b->stream_ibm));
+  BCHECK(b, hipStreamWaitEvent(b->stream, b->ev_ibm, 0));
+  launch_batch_force_row(b->stream, b->d_desc, count, steps - 1);
+  return 0;
   };
   // replay a captured graph of this (composition, steps) pair, capture it on
   // its second occurrence, run eagerly otherwise
   std::vector<long long> key = {count, steps, ns, f32, wings, b->lat_bx, (long long)(size_t)b->d_desc,
-                                (long long)(size_t)b->d_slots, (long long)(size_t)b->d_ibm,
-                                (long long)(size_t)b->d_geo, (long long)items.n_slots, (long long)items.n_ibm,
-                                (long long)g_full, (long long)gitems.size(), (long long)lds_wing,
-                                (long long)lds_solve};
+                                (long long)(size_t)b->d_items, (long long)lds_wing, (long long)lds_solve};
+  for (int q = 0; q < 8; q++) key.push_back((long long)off[q]);
   for (int e = 0; e < count; e++) key.push_back((long long)(size_t)hs[e]);
   BatchGraph* gr = nullptr;
   if (b->use_graphs && steps <= kMaxGraphSteps && (!wings || steps <= ns)) {
@@ -1539,8 +1469,9 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
     DevStep st_host;
     BCHECK(b, hipMemcpy(&st_host, h->d_st, sizeof(DevStep), hipMemcpyDeviceToHost));
     if (st_host.error != 0) {
-      const int code = fail(h, dev_error_code(st_host.error), "step failed: %s (info %d)",
-                            dev_error_text(st_host.error), st_host.error_info);
+      const int code = fail(h, st_host.error == DERR_MOTION ? LBM_E_STATE
+                                   : (st_host.error == DERR_NOT_SPD ? LBM_E_NUMERIC : LBM_E_RANGE),
+                            "step failed: %s (info %d)", dev_error_text(st_host.error), st_host.error_info);
       if (status) status[e] = code;
       if (first_bad < 0) {
         first_bad = e;

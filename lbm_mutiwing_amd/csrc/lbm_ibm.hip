@@ -18,17 +18,21 @@
 //     ordered zig-zag (0, n-1, 1, n-2, ...), which puts segments that face
 //     each other on the two surfaces next to each other and makes S a narrow
 //     band; the band is factored S = L D L^T and inverted explicitly.
-// Flow (per step, work items of the step kernel, lbm_flow.h and
-// lbm_lattice.hip): c = U_b - D u*, with u* pulled from the previous step's
-// populations at every box cell; y = S^-1 c as a dense mat-vec; g = D^T y at
-// every adjoint node; the adjoint-node re-collision applies g and the Guo
-// forcing 2 rho g and reduces the forces.
+// Flow (per step, three short kernels overlapping the lattice kernel):
+//   c = U_b - D u*, with u* pulled from the previous step's populations at
+//   every box cell; y = S^-1 c as a dense mat-vec; g = D^T y at every
+//   adjoint node.  The adjoint-node re-collision (lbm_lattice.hip) applies g,
+//   the Guo forcing 2 rho g and reduces the forces.
 #include <algorithm>
 #include <climits>
 
 #include "lbm_internal.h"
 
 namespace lbm {
+
+__device__ __forceinline__ void raise_error(DevStep* st, int code, int info) {
+  if (atomicCAS(&st->error, 0, code) == 0) st->error_info = info;
+}
 
 // Load that bypasses the CU's vector cache (values written by atomics or by
 // other workgroups of the same launch).
@@ -452,17 +456,9 @@ __device__ __forceinline__ void geo_solve_body(const IbmParams& p, const GeoBuff
   if (t == 0) {
     s_skip = load_agent(&st->error) != 0;
     if (w == 0) {
-      int tot = 0, b0 = INT_MAX, b1 = INT_MIN;
-      for (int q = 0; q < p.W; q++) {
-        tot += gs->cnt[q];
-        if (gs->rect[q][1] >= gs->rect[q][0]) {
-          b0 = min(b0, gs->rect[q][2]);
-          b1 = max(b1, gs->rect[q][3]);
-        }
-      }
+      int tot = 0;
+      for (int q = 0; q < p.W; q++) tot += gs->cnt[q];
       gs->adj_n = tot;
-      gs->band[0] = b0;
-      gs->band[1] = b1;
     }
   }
   // nodes claimed by two wings couple their systems: not supported
@@ -534,8 +530,166 @@ void launch_geometry(hipStream_t s, const IbmParams& p, const GeoBuffers& G, Dev
     hipLaunchKernelGGL(k_geo_solve, grid, dim3(GEO_THREADS), geo_solve_lds_bytes(p), s, p, G, st, slot0);
 }
 
+// ------------------------------------------------------------- flow part
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+// c = U_b - D u* (boundary_condition.f90:1609-1629): one wave per segment,
+// lane = box cell; u* of the cell is pulled from the previous step's
+// populations with the lattice kernel's arithmetic.
+template <typename S>
+__device__ __forceinline__ void ibm_rhs_body(const IbmParams& p, const MrtParams& P, const GeoBuffers& G, int slot,
+                                             const FlowBuffers& F, const Grid& g, const S* __restrict__ src, int blk) {
+  const int gk = blk * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (gk >= p.nseg) return;
+  const size_t e = (size_t)slot * p.nseg + gk;
+  const int* bx = G.box + e * 4;
+  double su = 0.0, sv = 0.0;
+  if (lane < BOXC) {
+    const int ci = lane / BOXW, cj = lane - ci * BOXW;
+    const int i = bx[0] + ci, j = bx[2] + cj;
+    if (i <= bx[1] && j <= bx[3]) {
+      double f[9];
+      pull_node_s(src, g, i, j, P.w, f);
+      double rho, u, v;
+      node_moments(f, rho, u, v);
+      const double phi = G.phx[e * BOXW + ci] * G.phy[e * BOXW + cj];
+      su = phi * u;
+      sv = phi * v;
+    }
+  }
+  su = wave_sum(su);
+  sv = wave_sum(sv);
+  if (lane == 0) {
+    const int w = gk / p.SPW, k = gk - w * p.SPW;
+    const int gp = w * p.SPW + pos_of_seg(k, p.SPW);
+    F.cvec[2 * gp] = G.ub[e] - su;
+    F.cvec[2 * gp + 1] = G.vb[e] - sv;
+  }
+}
+
+template <typename S>
+__global__ __launch_bounds__(256) void k_ibm_rhs(IbmParams p, MrtParams P, GeoBuffers G, int slot, FlowBuffers F,
+                                                 Grid g, const S* __restrict__ src) {
+  ibm_rhs_body<S>(p, P, G, slot, F, g, src, blockIdx.x);
+}
+
+// y = S^-1 c, one wave per row of the inverse.
+__device__ __forceinline__ void ibm_matvec_body(const IbmParams& p, const GeoBuffers& G, int slot,
+                                                const FlowBuffers& F, int blk) {
+  const int gp = blk * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (gp >= p.nseg) return;
+  const int n = p.SPW, w = gp / n, r = gp - w * n;
+  const double* row = G.sinv + (((size_t)slot * p.W + w) * n + r) * (size_t)n;
+  const double* c = F.cvec + (size_t)w * n * 2;
+  double a0 = 0.0, a1 = 0.0;
+#pragma unroll 4
+  for (int q = lane; q < n; q += 64) {
+    const double s = row[q];
+    a0 += s * c[2 * q];
+    a1 += s * c[2 * q + 1];
+  }
+  a0 = wave_sum(a0);
+  a1 = wave_sum(a1);
+  if (lane == 0) {
+    F.yvec[2 * gp] = a0;
+    F.yvec[2 * gp + 1] = a1;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_ibm_matvec(IbmParams p, GeoBuffers G, int slot, FlowBuffers F) {
+  ibm_matvec_body(p, G, slot, F, blockIdx.x);
+}
+
+// g = D^T y at the adjoint nodes (boundary_condition.f90:1639-1651).  A
+// workgroup takes the nodes claimed by SPREAD_SEG consecutive segments; every
+// segment whose box can hold them lies within +-bw band positions of the
+// claiming segment, and those positions are staged in LDS.
+constexpr int SPREAD_SEG = 16;
+constexpr int SPREAD_THREADS = 128;
+constexpr int SPREAD_STAGE = 2 * SPREAD_SEG + 2 * BMAX + 2;
+
+__device__ __forceinline__ void ibm_spread_body(const IbmParams& p, const GeoBuffers& G, int slot,
+                                                const FlowBuffers& F, int blk) {
+  __shared__ int s_box[SPREAD_STAGE][4];
+  __shared__ double s_ph[SPREAD_STAGE][2 * BOXW];
+  __shared__ double s_y[SPREAD_STAGE][2];
+  const int n = p.SPW, t = threadIdx.x;
+  const int nb = (n + SPREAD_SEG - 1) / SPREAD_SEG;
+  const int w = blk / nb, sb = blk - w * nb;
+  const int k0 = sb * SPREAD_SEG, k1 = min(k0 + SPREAD_SEG, n);
+  const GeoStep* gs = G.step + slot;
+  const int bw = min(gs->bw[w], BMAX);
+  int plo = INT_MAX, phi = -1;
+  for (int k = k0; k < k1; k++) {
+    const int q = pos_of_seg(k, n);
+    plo = min(plo, q);
+    phi = max(phi, q);
+  }
+  plo = max(0, plo - bw);
+  phi = min(n - 1, phi + bw);
+  const int ns = min(phi - plo + 1, SPREAD_STAGE);
+  const size_t base = (size_t)slot * p.nseg + (size_t)w * n;
+  for (int e = t; e < ns; e += SPREAD_THREADS) {
+    const int q = plo + e, k = seg_of_pos(q, n);
+    const int* bx = G.box + (base + k) * 4;
+#pragma unroll
+    for (int c = 0; c < 4; c++) s_box[e][c] = bx[c];
+#pragma unroll
+    for (int c = 0; c < BOXW; c++) {
+      s_ph[e][c] = G.phx[(base + k) * BOXW + c];
+      s_ph[e][BOXW + c] = G.phy[(base + k) * BOXW + c];
+    }
+    s_y[e][0] = F.yvec[2 * ((size_t)w * n + q)];
+    s_y[e][1] = F.yvec[2 * ((size_t)w * n + q) + 1];
+  }
+  __syncthreads();
+  const int* soff = G.seg_off + ((size_t)slot * p.W + w) * (n + 1);
+  const int a0 = soff[k0], a1 = soff[k1];
+  const size_t lb = (size_t)slot * p.cap + (size_t)w * p.capw;
+  const int* aij = G.adj_ij + lb * 2;
+  const int* aseg = G.adj_seg + lb;
+  double* gout = F.adj_g + ((size_t)w * p.capw) * 2;
+  for (int a = a0 + t; a < a1; a += SPREAD_THREADS) {
+    const int i = aij[2 * a], j = aij[2 * a + 1];
+    const int p0 = pos_of_seg(aseg[a], n);
+    const int qa = max(0, p0 - bw), qb = min(n - 1, p0 + bw);
+    double gx = 0.0, gy = 0.0;
+    for (int q = qa; q <= qb; q++) {
+      const int e = q - plo;
+      if (i < s_box[e][0] || i > s_box[e][1] || j < s_box[e][2] || j > s_box[e][3]) continue;
+      const double phi_ = s_ph[e][i - s_box[e][0]] * s_ph[e][BOXW + j - s_box[e][2]];
+      gx = gx + phi_ * s_y[e][0];
+      gy = gy + phi_ * s_y[e][1];
+    }
+    gout[2 * a] = gx;
+    gout[2 * a + 1] = gy;
+  }
+}
+
+__global__ __launch_bounds__(SPREAD_THREADS) void k_ibm_spread(IbmParams p, GeoBuffers G, int slot, FlowBuffers F) {
+  ibm_spread_body(p, G, slot, F, blockIdx.x);
+}
+
+void launch_ibm_flow(hipStream_t s, const IbmParams& p, const MrtParams& P, const GeoBuffers& G, int slot,
+                     const FlowBuffers& F, const Grid& g, const void* src, bool f32) {
+  if (p.W <= 0) return;
+  const int blk = (p.nseg + 3) / 4;
+  if (f32)
+    hipLaunchKernelGGL(k_ibm_rhs<float>, dim3(blk), dim3(256), 0, s, p, P, G, slot, F, g, (const float*)src);
+  else
+    hipLaunchKernelGGL(k_ibm_rhs<double>, dim3(blk), dim3(256), 0, s, p, P, G, slot, F, g, (const double*)src);
+  hipLaunchKernelGGL(k_ibm_matvec, dim3(blk), dim3(256), 0, s, p, G, slot, F);
+  const int nb = (p.SPW + SPREAD_SEG - 1) / SPREAD_SEG;
+  hipLaunchKernelGGL(k_ibm_spread, dim3(p.W * nb), dim3(SPREAD_THREADS), 0, s, p, G, slot, F);
+}
 
 // ------------------------------------------------------- batched launches
+
+int spread_blocks(const IbmParams& p) { return p.W * ((p.SPW + SPREAD_SEG - 1) / SPREAD_SEG); }
 
 __global__ __launch_bounds__(GEO_THREADS) void kb_geo_wing(const EpDesc* __restrict__ eps,
                                                            const int2* __restrict__ items, int k0) {
@@ -558,6 +712,39 @@ void launch_batch_geometry(hipStream_t s, const EpDesc* eps, const int2* items, 
   if (n_items <= 0) return;
   hipLaunchKernelGGL(kb_geo_wing, dim3(n_items), dim3(GEO_THREADS), lds_wing, s, eps, items, k0);
   hipLaunchKernelGGL(kb_geo_solve, dim3(n_items), dim3(GEO_THREADS), lds_solve, s, eps, items, k0);
+}
+
+template <typename S>
+__global__ __launch_bounds__(256) void kb_ibm_rhs(const EpDesc* __restrict__ eps, const int2* __restrict__ items,
+                                                  int k) {
+  const int2 it = items[blockIdx.x];
+  const EpDesc& D = eps[it.x];
+  const int cur = (D.cur0 + k) & 1;
+  ibm_rhs_body<S>(D.p, D.P, D.G, ep_slot(D, k), D.F, D.g, (const S*)D.fbuf[cur], it.y);
+}
+
+__global__ __launch_bounds__(256) void kb_ibm_matvec(const EpDesc* __restrict__ eps, const int2* __restrict__ items,
+                                                     int k) {
+  const int2 it = items[blockIdx.x];
+  const EpDesc& D = eps[it.x];
+  ibm_matvec_body(D.p, D.G, ep_slot(D, k), D.F, it.y);
+}
+
+__global__ __launch_bounds__(SPREAD_THREADS) void kb_ibm_spread(const EpDesc* __restrict__ eps,
+                                                                const int2* __restrict__ items, int k) {
+  const int2 it = items[blockIdx.x];
+  const EpDesc& D = eps[it.x];
+  ibm_spread_body(D.p, D.G, ep_slot(D, k), D.F, it.y);
+}
+
+void launch_batch_ibm_flow(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32) {
+  if (it.n_seg <= 0) return;
+  if (f32)
+    hipLaunchKernelGGL(kb_ibm_rhs<float>, dim3(it.n_seg), dim3(256), 0, s, eps, it.seg, k);
+  else
+    hipLaunchKernelGGL(kb_ibm_rhs<double>, dim3(it.n_seg), dim3(256), 0, s, eps, it.seg, k);
+  hipLaunchKernelGGL(kb_ibm_matvec, dim3(it.n_seg), dim3(256), 0, s, eps, it.seg, k);
+  hipLaunchKernelGGL(kb_ibm_spread, dim3(it.n_spread), dim3(SPREAD_THREADS), 0, s, eps, it.spread, k);
 }
 
 }  // namespace lbm

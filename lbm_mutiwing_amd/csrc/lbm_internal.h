@@ -24,7 +24,6 @@ enum DevError : int {
   DERR_WINGS_COUPLED = 4,  // supports of two wings share a node
   DERR_NOT_SPD = 5,        // non-positive pivot in the band factorisation
   DERR_MOTION = 6,         // motion source not ready
-  DERR_SYNC = 7,           // an IBM work item waited too long for its inputs
 };
 
 struct WingMotionState {
@@ -38,10 +37,6 @@ struct DevStep {
   double force[MAXW][2];   // normalised per-wing force of the last step
   double moment[MAXW];
 };
-
-__device__ __forceinline__ void raise_error(DevStep* st, int code, int info) {
-  if (atomicCAS(&st->error, 0, code) == 0) st->error_info = info;
-}
 
 // Rigid-frame state of one step, known on the host before the step runs
 // (movenum and update_body_extra_from_rel, main.f90:393-411, mode.f90:473-495).
@@ -61,7 +56,6 @@ struct GeoStep {
   int cnt[MAXW];           // adjoint nodes per wing
   int bw[MAXW];            // band half-width of S per wing (zig-zag order)
   int rect[MAXW][4];       // i0, i1, j0, j1 (0-based) of the union of the wing's boxes
-  int band[2];             // rows j0..j1 spanned by all the rectangles (j0 > j1: none)
   Frame frame;
 };
 
@@ -101,8 +95,8 @@ struct GeoBuffers {
   const double* stream_rows;    // stream mode: start row, target row (2*LBM_MAX_MOTION_COLS)
 };
 
-// The adjoint nodes of one step, which the lattice tiles leave to the IBM
-// adjoint items (their only writer): the wings' support rectangles and claim
+// The adjoint nodes of one step, which the full-lattice kernel leaves to
+// k_adj_collide (their only writer): the wings' support rectangles and claim
 // maps in the step's geometry slot.  W = 0 selects no node.
 struct AdjMask {
   const GeoStep* gs;
@@ -131,7 +125,8 @@ __device__ __forceinline__ bool is_adjoint(const AdjMask& am, int i, int j) {
 struct FlowBuffers {
   double* cvec;            // [nseg][2] right-hand side U_b - D u* by global band position
   double* yvec;            // [nseg][2] S^-1 c
-  double* partial;         // [2][MAXW][segment blocks per wing][3] force partial sums by step parity
+  double* adj_g;           // [cap][2] velocity correction g at the adjoint nodes
+  double* partial;         // [2][MAXW][blocks per wing][3] force partial sums by step parity
   double* rows;            // force rows of the current call
   double* mrows;           // airfoil_motion rows of the current call
   double* row_sum;         // sum of force rows
@@ -145,24 +140,6 @@ struct RingBuffers {
 
 struct FieldSlot {
   double* rho; double* u; double* v;
-};
-
-// Phases of the IBM work items of a step (lbm_lattice.hip).
-enum IbmPhase : int { PH_RHS = 0, PH_MATVEC = 1, PH_ADJ = 2, NPHASE = 3 };
-
-// Completion counts of the IBM work items of one episode in the running step
-// kernel (rhs, matvec, adjoint); the episode's last adjoint item writes the
-// force row and returns them to zero.
-struct PhaseSync {
-  unsigned done[4];
-};
-
-// Ticket and completion counts of the IBM slots of one step-kernel launch;
-// the last IBM slot to finish returns them to zero.
-struct StepSync {
-  unsigned ticket;
-  unsigned done;
-  unsigned pad[2];
 };
 
 // One episode of a batched advance: everything a kernel needs to run the
@@ -179,40 +156,25 @@ struct EpDesc {
   DevStep* st;
   void* fbuf[2];
   FieldSlot fslot[3];      // field slot of step n is fslot[n % 3]
-  PhaseSync* ph;           // IBM phase counters of the episode
   int cur0, n0, ns, bc0;
-  int tile_w, tiles_x;     // lattice tile width (256, 128 or 64 nodes x 256/width rows), tiles per row
-  int bpw;                 // adjoint blocks (segment blocks) per wing
-  int nph[NPHASE];         // IBM work items per phase (rhs, matvec, adjoint)
+  int tiles_x;             // lattice row tiles of the batch's tile width
+  int bpw;                 // adjoint-collide blocks per wing
   int pad0, pad1;
 };
 
 __host__ __device__ __forceinline__ int ep_slot(const EpDesc& D, int k) {
   return ((D.bc0 + k / D.ns) & 1) * D.ns + k % D.ns;
 }
-// the same with kq = k / ns and kr = k % ns given
-__host__ __device__ __forceinline__ int ep_slot_qr(const EpDesc& D, int kq, int kr) {
-  return ((D.bc0 + kq) & 1) * D.ns + kr;
-}
 
-// The step kernel: one launch advances every episode of a work list by one
-// step.  Grid slot b runs slots[b] = (episode, kind << 24 | index): a lattice
-// tile, a boundary-ring block, or an IBM slot.  An IBM slot takes the next
-// ticket of the launch and runs ibm[ticket] = (episode, phase << 24 | index);
-// the IBM items are listed phase by phase (rhs, matvec, adjoint), so an item
-// only ever waits for items with lower tickets, which have all started.
-enum SlotKind : int { SLOT_LATTICE = 0, SLOT_RING = 1, SLOT_IBM = 2 };
-constexpr int STEP_THREADS = 256;
-constexpr int kTileWidths[3] = {256, 128, 64};
-
-struct StepItems {
-  const int2* slots; int n_slots;
-  const int2* ibm; int n_ibm;
+// Work items of the batched launches: (episode, block of that episode's own
+// single-instance grid).
+struct BatchItems {
+  const int2* lattice; int n_lattice;
+  const int2* ring; int n_ring;
+  const int2* adj; int n_adj;
+  const int2* seg; int n_seg;        // k_ibm_rhs / k_ibm_matvec blocks
+  const int2* spread; int n_spread;
 };
-
-__host__ __device__ __forceinline__ int2 work_item(int episode, int kind, int index) {
-  return make_int2(episode, (kind << 24) | index);
-}
 
 // Band position <-> segment in a wing of n segments: positions 0,1,2,3,...
 // hold segments 0, n-1, 1, n-2, ... so that segments facing each other on
@@ -222,17 +184,22 @@ __host__ __device__ __forceinline__ int pos_of_seg(int k, int n) {
   return k < (n + 1) / 2 ? 2 * k : 2 * (n - 1 - k) + 1;
 }
 
-// Step kernel (lbm_lattice.hip): one launch per step over the work list it;
-// k is the substep of the call, ns the steps per geometry batch of the
-// episodes' descriptors; fields: write rho/u/v of the step; masked:
-// skip the adjoint nodes (off only for lattice-only tile timing, where no
-// geometry exists yet).
-void launch_step(hipStream_t s, const EpDesc* eps, const StepItems& it, StepSync* sync, int k, int ns, bool f32,
-                 bool fields, bool masked);
-// Work-item counts of one episode (host side).
-int lattice_tiles(const Grid& g, int tile_w, int* tiles_x);
-int bc_ring_blocks(const Grid& g);
-int ibm_items(const IbmParams& p, int phase);
+// lattice kernels (lbm_lattice.hip); populations are double (f32 = false) or
+// FP32 deviations from the weights (f32 = true)
+// bx: lattice tile width (one of kLatticeTiles)
+constexpr int kLatticeTiles[3] = {128, 256, 512};
+// mask: adjoint nodes of the step, skipped (written by k_adj_collide)
+void launch_lattice(hipStream_t s, const Grid& g, const MrtParams& P, const void* src, void* dst, FieldSlot* fields,
+                    bool f32, int bx, const AdjMask& mask);
+int lattice_tiles_x(const Grid& g, int bx);
+void launch_adj_collide(hipStream_t s, const Grid& g, const MrtParams& P, const IbmParams& p, const GeoBuffers& G,
+                        int slot, const FlowBuffers& F, int n, const void* src, void* dst, FieldSlot* fields,
+                        bool f32);
+// force row of step n (geometry slot, row index row) from the adjoint-collide partials
+void launch_force_row(hipStream_t s, const IbmParams& p, const GeoBuffers& G, int slot, const FlowBuffers& F,
+                      DevStep* st, int n, int row, int row_len);
+void launch_bc_ring(hipStream_t s, const Grid& g, const MrtParams& P, const RingBuffers& R, const void* src,
+                    void* dst, FieldSlot* fields, bool f32);
 void launch_ring_moments(hipStream_t s, const Grid& g, const RingBuffers& R);
 void launch_init_fields(hipStream_t s, const Grid& g, const MrtParams& P, double uu, const double feq[9], void* f,
                         const RingBuffers& R, FieldSlot slot, bool f32);
@@ -244,16 +211,30 @@ int err_blocks(const Grid& g);
 void launch_shift_x(hipStream_t s, const Grid& g, int dr, const void* f, void* fshift, int planes, bool f32);
 void launch_copy_ring_column(hipStream_t s, const Grid& g, const MrtParams& P, const void* f, int col, double* ring,
                              bool f32);
+int adj_collide_blocks_per_wing(const IbmParams& p);
 
 // IBM geometry kernels (lbm_ibm.hip), one launch covers `count` consecutive
 // steps n0.. into slots slot0..; frames are indexed from fidx0.
 void launch_geometry(hipStream_t s, const IbmParams& p, const GeoBuffers& G, DevStep* st, int slot0, int n0,
                      int fidx0, int count, bool with_inverse);
 size_t geo_solve_lds_bytes(const IbmParams& p);
-// Geometry of substeps k0 .. k0+count-1 of every episode with wings (device
-// descriptor array eps); items are (episode, b*W + w) for b < count.
+// Batched launches over the episodes of eps (device array); k is the substep.
+// masked: skip each episode's adjoint nodes of substep k (its geometry must exist)
+void launch_batch_lattice(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields,
+                          int bx, bool masked);
+void launch_batch_bc_ring(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields);
+void launch_batch_adj_collide(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields);
+void launch_batch_ibm_flow(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32);
+void launch_batch_force_row(hipStream_t s, const EpDesc* eps, int count, int k);
+// Geometry of substeps k0 .. k0+count-1 of every episode with wings; items
+// are (episode, b*W + w) for b < count.
 void launch_batch_geometry(hipStream_t s, const EpDesc* eps, const int2* items, int n_items, int k0,
                            size_t lds_wing, size_t lds_solve);
 size_t geo_wing_lds_bytes(int spw);
+int bc_ring_blocks(const Grid& g);
+int spread_blocks(const IbmParams& p);
+// IBM flow kernels of one step.
+void launch_ibm_flow(hipStream_t s, const IbmParams& p, const MrtParams& P, const GeoBuffers& G, int slot,
+                     const FlowBuffers& F, const Grid& g, const void* src, bool f32);
 
 }  // namespace lbm

@@ -1,47 +1,40 @@
-// lbm_lattice.hip - the step kernel of the time step on gfx950.
+// lbm_lattice.hip - full-lattice kernels of the time step on gfx950.
 //
-// One launch of k_step advances every episode of a work list by one step.
-// Its workgroups run three kinds of work item, all reading only the step's
-// source populations and each writing its own nodes of the destination:
-//   lattice tiles  streaming (solution.f90:440-466), physical_variables
-//                  (solution.f90:483-554) and colission (solution.f90:585-655)
-//                  fused per interior node: 9 pulled loads and 9 stores of
-//                  one SoA plane each, the 144 B/node floor of a two-buffer
-//                  FP64 D2Q9 step (72 B/node with FP32 deviation storage);
-//                  the adjoint nodes of the step are skipped;
-//   ring blocks    the far-field and wall boundary (boundary_condition1,
-//                  boundary_condition.f90:288-858);
-//   IBM items      c = U_b - D u*, y = S^-1 c, g = D^T y (lbm_flow.h), the
-//                  re-collision of the adjoint nodes with u* + g and the Guo
-//                  forcing, the per-wing force reduction and the force row
-//                  (boundary_condition.f90:1512-1692, solution.f90:330-401).
-// The IBM items depend on one another phase by phase; they take tickets in
-// phase order and wait on per-episode completion counters, while the lattice
-// tiles around them keep the memory system busy.  One launch per step on one
-// stream leaves no cross-stream synchronisation in the step loop.
+// k_lattice fuses streaming (solution.f90:440-466), physical_variables
+// (solution.f90:483-554) and colission (solution.f90:585-655) for every
+// interior node: 9 pulled loads and 9 stores of one SoA plane each, i.e. the
+// 144 B/node floor of a two-buffer FP64 D2Q9 step (72 B/node with FP32
+// deviation storage).  The nodes that the immersed boundary touches this step
+// (the adjoint nodes) are skipped: k_adj_collide collides them with the IBM
+// correction, on the IBM stream while the full-lattice pass runs, so neither
+// pass waits for the other.
 //
-// Every body that touches populations is a template on the storage type S:
+// Every kernel that touches populations is a template on the storage type S:
 // double (FP64 values) or float (FP32 deviations d = f - w_k).  Only the bulk
 // collision of the FP32 storage computes in FP32; the boundary ring, the
 // adjoint nodes and all reductions compute in FP64 on loaded f = w_k + d.
 #include <algorithm>
-#include <type_traits>
 
-#include "lbm_flow.h"
 #include "lbm_internal.h"
 
 namespace lbm {
 
-// One lattice node (i, j), 0-based and interior, of a tile.  The populations
-// are loaded before the adjoint-node test, so the test's loads overlap theirs;
-// an adjoint node is then left to the IBM items.
-template <bool FIELDS>
-__device__ __forceinline__ void lattice_node(const Grid& g, const MrtParams& P, const double* __restrict__ src,
-                                             double* __restrict__ dst, const FieldSlot& fs, int i, int j,
+// One thread per node of a row tile of BX nodes; tiles start at i = 0 so
+// that every wavefront stores whole, aligned spans of each population plane
+// (the ring columns i = 0 and i = nx-1 are skipped and left to k_bc_ring).
+// Workgroups follow the hardware's round-robin placement over the XCDs, which
+// measured faster than remapping tiles to XCD-contiguous runs.  The tile
+// width is chosen per instance (or per batch) by timing (lattice_autotune).
+template <bool FIELDS, int BX>
+__device__ __forceinline__ void lattice_body(const Grid& g, const MrtParams& P, const double* __restrict__ src,
+                                             double* __restrict__ dst, const FieldSlot& fs, int tiles_x, int blk,
                                              const AdjMask& am) {
+  const int ty = blk / tiles_x, tx = blk - ty * tiles_x;
+  const int i = tx * BX + (int)threadIdx.x;
+  const int j = 1 + ty;
+  if (i < 1 || i > g.nx - 2 || is_adjoint(am, i, j)) return;
   double f[9];
   pull_node(src, g, i, j, f);
-  if (is_adjoint(am, i, j)) return;
   double rho, u, v;
   node_moments(f, rho, u, v);
   double out[9];
@@ -56,18 +49,27 @@ __device__ __forceinline__ void lattice_node(const Grid& g, const MrtParams& P, 
   }
 }
 
+template <bool FIELDS, int BX>
+__global__ __launch_bounds__(BX) void k_lattice(Grid g, MrtParams P, const double* __restrict__ src,
+                                                double* __restrict__ dst, FieldSlot fs, int tiles_x, AdjMask am) {
+  lattice_body<FIELDS, BX>(g, P, src, dst, fs, tiles_x, blockIdx.x, am);
+}
+
 // FP32 bulk step on deviations d = f - w: rho = 1 + sum d, j = sum e d (the
 // weights carry no momentum), m' = M d, and the equilibrium moments shifted
 // by M w.  Relaxation and the inverse transform are linear, so d* = M^-1 m*'
 // is the deviation of the post-collision populations.
-template <bool FIELDS>
-__device__ __forceinline__ void lattice_f32_node(const Grid& g, const MrtParams& P, const float* __restrict__ src,
-                                                 float* __restrict__ dst, const FieldSlot& fs, int i, int j,
+template <bool FIELDS, int BX>
+__device__ __forceinline__ void lattice_f32_body(const Grid& g, const MrtParams& P, const float* __restrict__ src,
+                                                 float* __restrict__ dst, const FieldSlot& fs, int tiles_x, int blk,
                                                  const AdjMask& am) {
+  const int ty = blk / tiles_x, tx = blk - ty * tiles_x;
+  const int i = tx * BX + (int)threadIdx.x;
+  const int j = 1 + ty;
+  if (i < 1 || i > g.nx - 2 || is_adjoint(am, i, j)) return;
   float d[9];
 #pragma unroll
   for (int k = 0; k < 9; k++) d[k] = src[(long long)k * g.ps + (long long)(j - EY(k)) * g.nx + (i - EX(k))];
-  if (is_adjoint(am, i, j)) return;
   const float rp = d[0] + d[1] + d[2] + d[3] + d[4] + d[5] + d[6] + d[7] + d[8];
   const float jx = d[1] - d[3] + d[5] - d[6] - d[7] + d[8];
   const float jy = d[2] - d[4] + d[5] + d[6] - d[7] - d[8];
@@ -126,12 +128,60 @@ __device__ __forceinline__ void lattice_f32_node(const Grid& g, const MrtParams&
   }
 }
 
-// The adjoint nodes claimed by one block of SPREAD_SEG segments of a wing:
-// g = D^T y at each node (lbm_flow.h), re-collision with the IBM-corrected
-// velocity u = u* + g and the body force 2 rho g (boundary_condition.f90:
-// 1645-1659, then solution.f90:599-650), and the block's partial sums of the
-// wing's force and moment (boundary_condition.f90:1662-1675).  The force row
-// adds the partials in block order (solution.f90:330-401).
+template <bool FIELDS, int BX>
+__global__ __launch_bounds__(BX) void k_lattice_f32(Grid g, MrtParams P, const float* __restrict__ src,
+                                                    float* __restrict__ dst, FieldSlot fs, int tiles_x, AdjMask am) {
+  lattice_f32_body<FIELDS, BX>(g, P, src, dst, fs, tiles_x, blockIdx.x, am);
+}
+
+int lattice_tiles_x(const Grid& g, int bx) { return (g.nx + bx - 1) / bx; }
+
+template <int BX>
+void launch_lattice_bx(hipStream_t s, const Grid& g, const MrtParams& P, const void* src, void* dst,
+                       const FieldSlot* fields, bool f32, const AdjMask& am) {
+  const int tiles_x = lattice_tiles_x(g, BX);
+  dim3 grid(tiles_x * (g.ny - 2)), block(BX);
+  const FieldSlot fs = fields ? *fields : FieldSlot{};
+  if (f32) {
+    const float* a = (const float*)src;
+    float* b = (float*)dst;
+    if (fields)
+      hipLaunchKernelGGL((k_lattice_f32<true, BX>), grid, block, 0, s, g, P, a, b, fs, tiles_x, am);
+    else
+      hipLaunchKernelGGL((k_lattice_f32<false, BX>), grid, block, 0, s, g, P, a, b, fs, tiles_x, am);
+  } else {
+    const double* a = (const double*)src;
+    double* b = (double*)dst;
+    if (fields)
+      hipLaunchKernelGGL((k_lattice<true, BX>), grid, block, 0, s, g, P, a, b, fs, tiles_x, am);
+    else
+      hipLaunchKernelGGL((k_lattice<false, BX>), grid, block, 0, s, g, P, a, b, fs, tiles_x, am);
+  }
+}
+
+void launch_lattice(hipStream_t s, const Grid& g, const MrtParams& P, const void* src, void* dst, FieldSlot* fields,
+                    bool f32, int bx, const AdjMask& mask) {
+  if (bx == 128)
+    launch_lattice_bx<128>(s, g, P, src, dst, fields, f32, mask);
+  else if (bx == 512)
+    launch_lattice_bx<512>(s, g, P, src, dst, fields, f32, mask);
+  else
+    launch_lattice_bx<256>(s, g, P, src, dst, fields, f32, mask);
+}
+
+// Re-collide the adjoint nodes with the IBM-corrected velocity u = u* + g and
+// the body force 2 rho g (boundary_condition.f90:1645-1659, then
+// solution.f90:599-650), and reduce each wing's force and moment
+// (boundary_condition.f90:1662-1675).  Blocks are (wing, node block); each
+// writes its partial sums.  k_force_row adds them in block order and writes
+// the normalised force row (solution.f90:330-401); it runs off the lattice
+// stream's critical path, at the start of the next step's IBM chain.
+constexpr int ADJ_THREADS = 256;
+constexpr int ADJ_BLOCKS_PER_WING = 16;   // grid-stride over a wing's adjoint nodes
+
+int adj_collide_blocks_per_wing(const IbmParams& p) {
+  return p.W > 0 ? std::min(ADJ_BLOCKS_PER_WING, (p.capw + ADJ_THREADS - 1) / ADJ_THREADS) : 1;
+}
 
 // Normalised force row and airfoil_motion row of one step into LDS arrays
 // (one thread); the per-wing force and moment of the step go to st.
@@ -188,34 +238,27 @@ __device__ void compute_rows(const IbmParams& p, const GeoStep* gs, DevStep* st,
   }
 }
 
-// blk in [0, W*nb) is (wing, segment block); nb = adj_blocks_per_wing.
+// blk in [0, W*bpw) is (wing, node block).
 template <typename S, bool FIELDS>
-__device__ __forceinline__ void ibm_adj_body(const Grid& g, const MrtParams& P, const IbmParams& p,
-                                             const GeoBuffers& G, int slot, const FlowBuffers& F, int nb, int n,
-                                             const S* __restrict__ src, S* __restrict__ dst, const FieldSlot& fs,
-                                             int blk) {
-  __shared__ SpreadStage stage;
-  __shared__ double red[3][STEP_THREADS];
-  if (p.W <= 0) return;  // nothing to re-collide; the item only ends the step
+__device__ __forceinline__ void adj_collide_body(const Grid& g, const MrtParams& P, const IbmParams& p,
+                                                 const GeoBuffers& G, int slot, const FlowBuffers& F, int bpw, int n,
+                                                 const S* __restrict__ src, S* __restrict__ dst, const FieldSlot& fs,
+                                                 int blk) {
+  __shared__ double red[3][ADJ_THREADS];
   const int t = threadIdx.x;
-  const int w = blk / nb, sb = blk - w * nb;
-  const SpreadBlock B = spread_stage<STEP_THREADS>(p, G, slot, F, w, sb, stage);
-  __syncthreads();
+  const int w = blk / bpw, b = blk - w * bpw;
   const GeoStep* gs = G.step + slot;
-  const int* soff = G.seg_off + ((size_t)slot * p.W + w) * (p.SPW + 1);
-  const int a0 = soff[B.k0], a1 = soff[B.k1];
-  const size_t lb = (size_t)slot * p.cap + (size_t)w * p.capw;
-  const int* aij = G.adj_ij + lb * 2;
-  const int* aseg = G.adj_seg + lb;
   double fx = 0.0, fy = 0.0, fm = 0.0;
-  for (int a = a0 + t; a < a1; a += STEP_THREADS) {
-    const int i = aij[2 * a], j = aij[2 * a + 1];
-    double gx, gy;
-    spread_g(B, stage, p.SPW, i, j, aseg[a], gx, gy);
+  const int cnt = gs->cnt[w];
+  for (int a = b * ADJ_THREADS + t; a < cnt; a += bpw * ADJ_THREADS) {
+    const size_t la = (size_t)slot * p.cap + (size_t)w * p.capw + a;
+    const int i = G.adj_ij[2 * la], j = G.adj_ij[2 * la + 1];
     double f[9];
     pull_node_s(src, g, i, j, P.w, f);
     double rho, us, vs;
     node_moments(f, rho, us, vs);
+    const size_t ga = (size_t)w * p.capw + a;
+    const double gx = F.adj_g[2 * ga], gy = F.adj_g[2 * ga + 1];
     const double u = us + gx, v = vs + gy;
     const double t1 = 2.0 * rho * gx, t2 = 2.0 * rho * gy;
     double out[9];
@@ -240,19 +283,26 @@ __device__ __forceinline__ void ibm_adj_body(const Grid& g, const MrtParams& P, 
   red[1][t] = fy;
   red[2][t] = fm;
   __syncthreads();
-  for (int h = STEP_THREADS / 2; h > 0; h >>= 1) {
+  for (int h = ADJ_THREADS / 2; h > 0; h >>= 1) {
     if (t < h)
       for (int c = 0; c < 3; c++) red[c][t] += red[c][t + h];
     __syncthreads();
   }
   if (t == 0) {
-    double* part = F.partial + (size_t)(n & 1) * MAXW * nb * 3;
-    for (int c = 0; c < 3; c++) part[((size_t)w * nb + sb) * 3 + c] = red[c][0];
+    double* part = F.partial + (size_t)(n & 1) * MAXW * bpw * 3;
+    for (int c = 0; c < 3; c++) part[((size_t)w * bpw + b) * 3 + c] = red[c][0];
   }
 }
 
-// Per-wing sums of the adjoint-block partials in block order, then the
-// force row of step n into row slot_row.
+template <typename S, bool FIELDS>
+__global__ __launch_bounds__(ADJ_THREADS) void k_adj_collide(Grid g, MrtParams P, IbmParams p, GeoBuffers G, int slot,
+                                                             FlowBuffers F, int bpw, int n, const S* __restrict__ src,
+                                                             S* __restrict__ dst, FieldSlot fs) {
+  adj_collide_body<S, FIELDS>(g, P, p, G, slot, F, bpw, n, src, dst, fs, blockIdx.x);
+}
+
+// Per-wing sums of the adjoint-collide partials in block order, then the
+// force row of step n into row slot_row (one 64-thread workgroup).
 __device__ __forceinline__ void force_row_body(const IbmParams& p, const GeoBuffers& G, int slot,
                                                const FlowBuffers& F, DevStep* st, int n, int slot_row, int row_len,
                                                int bpw) {
@@ -277,6 +327,39 @@ __device__ __forceinline__ void force_row_body(const IbmParams& p, const GeoBuff
   }
   double* mrow = F.mrows + (long long)slot_row * (1 + 3 * p.W);
   for (int c = t; c < 1 + 3 * p.W; c += blockDim.x) mrow[c] = s_mrow[c];
+}
+
+__global__ __launch_bounds__(64) void k_force_row(IbmParams p, GeoBuffers G, int slot, FlowBuffers F, DevStep* st,
+                                                  int n, int slot_row, int row_len, int bpw) {
+  force_row_body(p, G, slot, F, st, n, slot_row, row_len, bpw);
+}
+
+void launch_force_row(hipStream_t s, const IbmParams& p, const GeoBuffers& G, int slot, const FlowBuffers& F,
+                      DevStep* st, int n, int row, int row_len) {
+  hipLaunchKernelGGL(k_force_row, dim3(1), dim3(64), 0, s, p, G, slot, F, st, n, row, row_len,
+                     adj_collide_blocks_per_wing(p));
+}
+
+template <typename S>
+void launch_adj_collide_t(hipStream_t s, const Grid& g, const MrtParams& P, const IbmParams& p, const GeoBuffers& G,
+                          int slot, const FlowBuffers& F, int n, const S* src, S* dst, FieldSlot* fields) {
+  if (p.W <= 0) return;
+  const int bpw = adj_collide_blocks_per_wing(p);
+  dim3 grid(p.W * bpw), block(ADJ_THREADS);
+  if (fields)
+    hipLaunchKernelGGL((k_adj_collide<S, true>), grid, block, 0, s, g, P, p, G, slot, F, bpw, n, src, dst, *fields);
+  else
+    hipLaunchKernelGGL((k_adj_collide<S, false>), grid, block, 0, s, g, P, p, G, slot, F, bpw, n, src, dst,
+                       FieldSlot{});
+}
+
+void launch_adj_collide(hipStream_t s, const Grid& g, const MrtParams& P, const IbmParams& p, const GeoBuffers& G,
+                        int slot, const FlowBuffers& F, int n, const void* src, void* dst, FieldSlot* fields,
+                        bool f32) {
+  if (f32)
+    launch_adj_collide_t(s, g, P, p, G, slot, F, n, (const float*)src, (float*)dst, fields);
+  else
+    launch_adj_collide_t(s, g, P, p, G, slot, F, n, (const double*)src, (double*)dst, fields);
 }
 
 // ------------------------------------------------------------------ far field
@@ -395,6 +478,31 @@ __device__ __forceinline__ void bc_ring_body(const Grid& g, const MrtParams& P, 
     fs.rho[o] = b.rho;
     fs.u[o] = b.u;
     fs.v[o] = b.v;
+  }
+}
+
+template <typename S, bool FIELDS>
+__global__ __launch_bounds__(256) void k_bc_ring(Grid g, MrtParams P, RingBuffers R, const S* __restrict__ src,
+                                                 S* __restrict__ dst, FieldSlot fs) {
+  bc_ring_body<S, FIELDS>(g, P, R, src, dst, fs, blockIdx.x);
+}
+
+void launch_bc_ring(hipStream_t s, const Grid& g, const MrtParams& P, const RingBuffers& R, const void* src,
+                    void* dst, FieldSlot* fields, bool f32) {
+  const int total = 2 * (g.ny - 2) + 2 * g.nx;
+  dim3 grid((total + 255) / 256), block(256);
+  const FieldSlot fs = fields ? *fields : FieldSlot{};
+  if (f32) {
+    if (fields)
+      hipLaunchKernelGGL((k_bc_ring<float, true>), grid, block, 0, s, g, P, R, (const float*)src, (float*)dst, fs);
+    else
+      hipLaunchKernelGGL((k_bc_ring<float, false>), grid, block, 0, s, g, P, R, (const float*)src, (float*)dst, fs);
+  } else {
+    if (fields)
+      hipLaunchKernelGGL((k_bc_ring<double, true>), grid, block, 0, s, g, P, R, (const double*)src, (double*)dst, fs);
+    else
+      hipLaunchKernelGGL((k_bc_ring<double, false>), grid, block, 0, s, g, P, R, (const double*)src, (double*)dst,
+                         fs);
   }
 }
 
@@ -574,161 +682,101 @@ void launch_copy_ring_column(hipStream_t s, const Grid& g, const MrtParams& P, c
     hipLaunchKernelGGL(k_copy_ring_column<double>, grid, block, 0, s, g, P, (const double*)f, col, ring);
 }
 
-// -------------------------------------------------------------- step kernel
+// ------------------------------------------------------- batched launches
 
 int bc_ring_blocks(const Grid& g) { return (2 * (g.ny - 2) + 2 * g.nx + 255) / 256; }
 
-int lattice_tiles(const Grid& g, int tile_w, int* tiles_x) {
-  const int th = STEP_THREADS / tile_w;
-  const int tx = (g.nx + tile_w - 1) / tile_w, ty = (g.ny - 2 + th - 1) / th;
-  if (tiles_x) *tiles_x = tx;
-  return tx * ty;
-}
-
-// rhs and matvec items cover IBM_GROUP blocks of four segments (rows) each,
-// one segment (row) per wave, so that few items carry the phase's fences.
-#ifndef LBM_IBM_GROUP
-#define LBM_IBM_GROUP 8
-#endif
-constexpr int IBM_GROUP = LBM_IBM_GROUP;
-
-int ibm_items(const IbmParams& p, int phase) {
-  switch (phase) {
-    case PH_RHS:
-    case PH_MATVEC: return p.W > 0 ? ((p.nseg + 3) / 4 + IBM_GROUP - 1) / IBM_GROUP : 0;
-    default: return p.W > 0 ? p.W * adj_blocks_per_wing(p) : 1;
-  }
-}
-
-// Wait until *c reaches target.  Thread 0 polls with relaxed agent-scope
-// loads, which read the counter coherently without invalidating any cache
-// (an acquire per poll would drop the L1 and L2 lines the lattice tiles of
-// the same launch are using), then takes one acquire fence; the barrier
-// hands the acquired view to the other waves.  The counted items have lower
-// tickets, so they have all started; the poll is still bounded and raises
-// DERR_SYNC (and proceeds) if it ever runs out.
-__device__ __forceinline__ void wait_count(unsigned* c, unsigned target, DevStep* st, int info) {
-  if (threadIdx.x == 0) {
-    unsigned spins = 0;
-    while (__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-      __builtin_amdgcn_s_sleep(8);
-      if (++spins == (1u << 24)) {
-        raise_error(st, DERR_SYNC, info);
-        break;
-      }
-    }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  }
-  __syncthreads();
-}
-
-// Count one finished item: the barrier completes every wave's writes, then
-// one release increment makes them visible at agent scope.  With ACQUIRE the
-// increment also acquires the writes of the items counted before it, and the
-// result says whether this was the last item of target.
-template <bool ACQUIRE>
-__device__ __forceinline__ bool count_done(unsigned* c, unsigned target) {
-  __shared__ int s_last;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const unsigned old = ACQUIRE ? __hip_atomic_fetch_add(c, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT)
-                                 : __hip_atomic_fetch_add(c, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    s_last = old + 1 == target;
-  }
-  __syncthreads();
-  return s_last != 0;
+// masked = 0 (tile timing before any geometry exists): every interior node
+template <bool F32, bool FIELDS, int BX>
+__global__ __launch_bounds__(BX) void kb_lattice(const EpDesc* __restrict__ eps, const int2* __restrict__ items,
+                                                 int k, int masked) {
+  const int2 it = items[blockIdx.x];
+  const EpDesc& D = eps[it.x];
+  const int n = D.n0 + k, cur = (D.cur0 + k) & 1;
+  AdjMask am = adj_mask(D.p, D.G, ep_slot(D, k));
+  if (!masked) am.W = 0;
+  if (F32)
+    lattice_f32_body<FIELDS, BX>(D.g, D.P, (const float*)D.fbuf[cur], (float*)D.fbuf[cur ^ 1], D.fslot[n % 3],
+                                 D.tiles_x, it.y, am);
+  else
+    lattice_body<FIELDS, BX>(D.g, D.P, (const double*)D.fbuf[cur], (double*)D.fbuf[cur ^ 1], D.fslot[n % 3],
+                             D.tiles_x, it.y, am);
 }
 
 template <typename S, bool FIELDS>
-__device__ __forceinline__ void ibm_item(const EpDesc& D, int phase, int idx, int k, DevStep* st) {
-  const int n = D.n0 + k, cur = (D.cur0 + k) & 1, slot = ep_slot(D, k);
-  const S* src = (const S*)D.fbuf[cur];
-  S* dst = (S*)D.fbuf[cur ^ 1];
-  unsigned* done = D.ph->done;
-  if (phase > PH_RHS) wait_count(&done[phase - 1], (unsigned)D.nph[phase - 1], st, phase);
-  switch (phase) {
-    case PH_RHS:
-      for (int r = 0; r < IBM_GROUP; r++) ibm_rhs_body<S>(D.p, D.P, D.G, slot, D.F, D.g, src, idx * IBM_GROUP + r);
-      break;
-    case PH_MATVEC:
-      for (int r = 0; r < IBM_GROUP; r++) ibm_matvec_body(D.p, D.G, slot, D.F, idx * IBM_GROUP + r);
-      break;
-    default:
-      ibm_adj_body<S, FIELDS>(D.g, D.P, D.p, D.G, slot, D.F, D.bpw, n, src, dst, D.fslot[n % 3], idx);
-      break;
-  }
-  if (phase != PH_ADJ) {
-    count_done<false>(&done[phase], (unsigned)D.nph[phase]);
-  } else if (count_done<true>(&done[phase], (unsigned)D.nph[phase])) {
-    // the episode's last item of the step: the force row, then its counters
-    // are free again
-    force_row_body(D.p, D.G, slot, D.F, D.st, n, k, 4 + 12 * D.p.W, D.bpw);
-    __syncthreads();
-    if (threadIdx.x < NPHASE) __hip_atomic_store(&done[threadIdx.x], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
+__global__ __launch_bounds__(256) void kb_bc_ring(const EpDesc* __restrict__ eps, const int2* __restrict__ items,
+                                                  int k) {
+  const int2 it = items[blockIdx.x];
+  const EpDesc& D = eps[it.x];
+  const int n = D.n0 + k, cur = (D.cur0 + k) & 1;
+  bc_ring_body<S, FIELDS>(D.g, D.P, D.R, (const S*)D.fbuf[cur], (S*)D.fbuf[cur ^ 1], D.fslot[n % 3], it.y);
 }
 
-template <bool F32, bool FIELDS>
-__global__ __launch_bounds__(STEP_THREADS) void k_step(const EpDesc* __restrict__ eps, StepItems it,
-                                                       StepSync* sync, int k, int kq, int kr, int masked) {
-  using S = typename std::conditional<F32, float, double>::type;
-  const int2 sl = it.slots[blockIdx.x];
-  const int kind = sl.y >> 24, idx = sl.y & 0xffffff;
-  if (kind == SLOT_LATTICE) {
-    const EpDesc& D = eps[sl.x];
-    const int tw = D.tile_w, th = STEP_THREADS / tw, ty = idx / D.tiles_x, tx = idx - ty * D.tiles_x;
-    const int i = tx * tw + (int)threadIdx.x % tw;
-    const int j0 = 1 + ty * th, j = j0 + (int)threadIdx.x / tw;
-    if (i < 1 || i > D.g.nx - 2 || j > D.g.ny - 2) return;
-    const int n = D.n0 + k, cur = (D.cur0 + k) & 1;
-    // only tiles in the rows of the step's wing rectangles test their nodes
-    AdjMask am{nullptr, nullptr, 0, 0};
-    if (masked && D.p.W > 0) {
-      const int slot = ep_slot_qr(D, kq, kr);
-      const int* band = D.G.step[slot].band;
-      if (j0 + th - 1 >= band[0] && j0 <= band[1]) am = adj_mask(D.p, D.G, slot);
-    }
-    if (F32)
-      lattice_f32_node<FIELDS>(D.g, D.P, (const float*)D.fbuf[cur], (float*)D.fbuf[cur ^ 1], D.fslot[n % 3], i, j,
-                               am);
-    else
-      lattice_node<FIELDS>(D.g, D.P, (const double*)D.fbuf[cur], (double*)D.fbuf[cur ^ 1], D.fslot[n % 3], i, j,
-                           am);
-    return;
-  }
-  if (kind == SLOT_RING) {
-    const EpDesc& D = eps[sl.x];
-    const int n = D.n0 + k, cur = (D.cur0 + k) & 1;
-    bc_ring_body<S, FIELDS>(D.g, D.P, D.R, (const S*)D.fbuf[cur], (S*)D.fbuf[cur ^ 1], D.fslot[n % 3], idx);
-    return;
-  }
-  // IBM slot: the next IBM item in ticket order
-  __shared__ unsigned s_ticket;
-  if (threadIdx.x == 0) s_ticket = __hip_atomic_fetch_add(&sync->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __syncthreads();
-  const int2 im = it.ibm[s_ticket];
-  const EpDesc& D = eps[im.x];
-  ibm_item<S, FIELDS>(D, im.y >> 24, im.y & 0xffffff, k, D.st);
-  // the last IBM slot of the launch returns the launch counters to zero
-  __syncthreads();
-  if (threadIdx.x == 0 &&
-      __hip_atomic_fetch_add(&sync->done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == (unsigned)it.n_ibm - 1) {
-    __hip_atomic_store(&sync->ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(&sync->done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
+template <typename S, bool FIELDS>
+__global__ __launch_bounds__(ADJ_THREADS) void kb_adj_collide(const EpDesc* __restrict__ eps,
+                                                              const int2* __restrict__ items, int k) {
+  const int2 it = items[blockIdx.x];
+  const EpDesc& D = eps[it.x];
+  const int n = D.n0 + k, cur = (D.cur0 + k) & 1;
+  adj_collide_body<S, FIELDS>(D.g, D.P, D.p, D.G, ep_slot(D, k), D.F, D.bpw, n, (const S*)D.fbuf[cur],
+                              (S*)D.fbuf[cur ^ 1], D.fslot[n % 3], it.y);
 }
 
-void launch_step(hipStream_t s, const EpDesc* eps, const StepItems& it, StepSync* sync, int k, int ns, bool f32,
-                 bool fields, bool masked) {
-  if (it.n_slots <= 0) return;
-  dim3 grid(it.n_slots), block(STEP_THREADS);
-  const int mk = masked ? 1 : 0, kq = k / ns, kr = k % ns;
+// force row of substep k of every episode: one workgroup per episode
+__global__ __launch_bounds__(64) void kb_force_row(const EpDesc* __restrict__ eps, int k) {
+  const EpDesc& D = eps[blockIdx.x];
+  force_row_body(D.p, D.G, ep_slot(D, k), D.F, D.st, D.n0 + k, k, 4 + 12 * D.p.W, D.bpw);
+}
+
+void launch_batch_force_row(hipStream_t s, const EpDesc* eps, int count, int k) {
+  hipLaunchKernelGGL(kb_force_row, dim3(count), dim3(64), 0, s, eps, k);
+}
+
+template <int BX>
+void launch_batch_lattice_bx(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields,
+                             int masked) {
+  dim3 grid(it.n_lattice), block(BX);
   if (f32) {
-    if (fields) hipLaunchKernelGGL((k_step<true, true>), grid, block, 0, s, eps, it, sync, k, kq, kr, mk);
-    else hipLaunchKernelGGL((k_step<true, false>), grid, block, 0, s, eps, it, sync, k, kq, kr, mk);
+    if (fields) hipLaunchKernelGGL((kb_lattice<true, true, BX>), grid, block, 0, s, eps, it.lattice, k, masked);
+    else hipLaunchKernelGGL((kb_lattice<true, false, BX>), grid, block, 0, s, eps, it.lattice, k, masked);
   } else {
-    if (fields) hipLaunchKernelGGL((k_step<false, true>), grid, block, 0, s, eps, it, sync, k, kq, kr, mk);
-    else hipLaunchKernelGGL((k_step<false, false>), grid, block, 0, s, eps, it, sync, k, kq, kr, mk);
+    if (fields) hipLaunchKernelGGL((kb_lattice<false, true, BX>), grid, block, 0, s, eps, it.lattice, k, masked);
+    else hipLaunchKernelGGL((kb_lattice<false, false, BX>), grid, block, 0, s, eps, it.lattice, k, masked);
+  }
+}
+
+void launch_batch_lattice(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields,
+                          int bx, bool masked) {
+  if (it.n_lattice <= 0) return;
+  if (bx == 128)
+    launch_batch_lattice_bx<128>(s, eps, it, k, f32, fields, masked ? 1 : 0);
+  else if (bx == 512)
+    launch_batch_lattice_bx<512>(s, eps, it, k, f32, fields, masked ? 1 : 0);
+  else
+    launch_batch_lattice_bx<256>(s, eps, it, k, f32, fields, masked ? 1 : 0);
+}
+
+void launch_batch_bc_ring(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields) {
+  if (it.n_ring <= 0) return;
+  dim3 grid(it.n_ring), block(256);
+  if (f32) {
+    if (fields) hipLaunchKernelGGL((kb_bc_ring<float, true>), grid, block, 0, s, eps, it.ring, k);
+    else hipLaunchKernelGGL((kb_bc_ring<float, false>), grid, block, 0, s, eps, it.ring, k);
+  } else {
+    if (fields) hipLaunchKernelGGL((kb_bc_ring<double, true>), grid, block, 0, s, eps, it.ring, k);
+    else hipLaunchKernelGGL((kb_bc_ring<double, false>), grid, block, 0, s, eps, it.ring, k);
+  }
+}
+
+void launch_batch_adj_collide(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields) {
+  if (it.n_adj <= 0) return;
+  dim3 grid(it.n_adj), block(ADJ_THREADS);
+  if (f32) {
+    if (fields) hipLaunchKernelGGL((kb_adj_collide<float, true>), grid, block, 0, s, eps, it.adj, k);
+    else hipLaunchKernelGGL((kb_adj_collide<float, false>), grid, block, 0, s, eps, it.adj, k);
+  } else {
+    if (fields) hipLaunchKernelGGL((kb_adj_collide<double, true>), grid, block, 0, s, eps, it.adj, k);
+    else hipLaunchKernelGGL((kb_adj_collide<double, false>), grid, block, 0, s, eps, it.adj, k);
   }
 }
 
