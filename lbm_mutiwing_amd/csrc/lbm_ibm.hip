@@ -456,9 +456,17 @@ __device__ __forceinline__ void geo_solve_body(const IbmParams& p, const GeoBuff
   if (t == 0) {
     s_skip = load_agent(&st->error) != 0;
     if (w == 0) {
-      int tot = 0;
-      for (int q = 0; q < p.W; q++) tot += gs->cnt[q];
+      int tot = 0, b0 = INT_MAX, b1 = INT_MIN;
+      for (int q = 0; q < p.W; q++) {
+        tot += gs->cnt[q];
+        if (gs->rect[q][1] >= gs->rect[q][0]) {
+          b0 = min(b0, gs->rect[q][2]);
+          b1 = max(b1, gs->rect[q][3]);
+        }
+      }
       gs->adj_n = tot;
+      gs->band[0] = b0;
+      gs->band[1] = b1;
     }
   }
   // nodes claimed by two wings couple their systems: not supported

@@ -56,6 +56,7 @@ struct GeoStep {
   int cnt[MAXW];           // adjoint nodes per wing
   int bw[MAXW];            // band half-width of S per wing (zig-zag order)
   int rect[MAXW][4];       // i0, i1, j0, j1 (0-based) of the union of the wing's boxes
+  int band[2];             // rows j0..j1 spanned by all the rectangles (j0 > j1: none)
   Frame frame;
 };
 
@@ -108,9 +109,10 @@ __host__ __device__ __forceinline__ AdjMask adj_mask(const IbmParams& p, const G
   return AdjMask{G.step + slot, G.map + (size_t)slot * p.W * 2 * (size_t)p.map_cap, p.W, p.map_cap};
 }
 
-// Is (i, j) (0-based) an adjoint node of the mask's step?  Rows outside every
-// rectangle cost a few scalar compares.
+// Is (i, j) (0-based) an adjoint node of the mask's step?  Rows outside the
+// band of the wings' rectangles cost one scalar compare.
 __device__ __forceinline__ bool is_adjoint(const AdjMask& am, int i, int j) {
+  if (am.W == 0 || j < am.gs->band[0] || j > am.gs->band[1]) return false;
   for (int w = 0; w < am.W; w++) {
     const int* r = am.gs->rect[w];
     const int i0 = r[0], i1 = r[1], j0 = r[2], j1 = r[3];
