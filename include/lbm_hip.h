@@ -195,8 +195,11 @@ int lbm_get_timing(lbm_handle *h, double *lattice_ms, int32_t *launches);
  * were seen once before on this batch object is captured as a HIP graph on
  * its second occurrence and replayed from then on (up to eight graphs are
  * kept, least recently used evicted); other advances and first occurrences
- * are issued launch by launch.  LBM_BATCH_GRAPHS=0 in the
- * environment turns capture off.  Results are identical either way. */
+ * are issued launch by launch.  LBM_BATCH_GRAPHS in the environment, read
+ * by lbm_batch_create: 0 turns capture off; 2 also replays, in every longer
+ * advance, all geometry batches but the first and the last two from one
+ * captured graph of one geometry batch (16 steps without wings) per
+ * composition; 1 is the default.  Results are identical in every mode. */
 typedef struct lbm_batch lbm_batch;
 int lbm_batch_create(lbm_batch **out);
 int lbm_batch_destroy(lbm_batch *b);

@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <vector>
 
 #include "lbm_internal.h"
@@ -1072,14 +1073,14 @@ int lbm_set_body_rel_pos(lbm_handle* h, int32_t wing, double pos) {
 
 // ------------------------------------------------------------ batched advance
 
-// A captured batched advance: the launch sequence of one (composition,
-// steps) pair, replayed when the same pair comes again.
-// Longest advance that is captured as a graph; longer advances are issued
-// launch by launch.  Only advances that stay within one geometry batch are
-// captured: captures whose geometry stream waits on slot-release events
-// recorded inside the same capture (advances longer than one geometry batch)
-// ended the process with a host-side segmentation fault inside this call, for
-// a reason not yet found, so that form is not captured.
+// A captured batched launch sequence: a whole short advance of one
+// (composition, steps) pair, or one geometry batch of a long advance of one
+// composition, replayed when the same key comes again.
+// Longest advance that is captured whole; it must also stay within one
+// geometry batch.  A capture whose geometry stream waits on slot-release
+// events recorded inside the same capture (a whole advance over several
+// geometry batches) ended the process with a host-side segmentation fault
+// inside this call, for a reason not found; no capture here has that form.
 constexpr int kMaxGraphSteps = 64;
 
 struct BatchGraph {
@@ -1094,13 +1095,14 @@ struct lbm_batch {
   hipEvent_t ev_ibm = nullptr, ev_step = nullptr, ev_bc = nullptr, ev_start = nullptr;
   std::vector<BatchGraph> graphs;
   unsigned long long calls = 0;
-  bool use_graphs = true;
+  int graph_mode = 1;                // 0 none, 1 short advances, 2 also the middle batches of long ones
   hipEvent_t ev_geo[2] = {nullptr, nullptr}, ev_rel[2] = {nullptr, nullptr};
   int device = 0;
   EpDesc* d_desc = nullptr;
   size_t desc_cap = 0;
   int2* d_items = nullptr;
   size_t items_cap = 0;
+  int* d_kbase = nullptr;            // device step base of the batched launches
   int lat_bx = 0;                    // lattice tile width (tuned on the first call)
   char msg[512] = "";
 };
@@ -1128,6 +1130,7 @@ void batch_free(lbm_batch* b) {
   if (b->ev_start) (void)hipEventDestroy(b->ev_start);
   if (b->d_desc) (void)hipFree(b->d_desc);
   if (b->d_items) (void)hipFree(b->d_items);
+  if (b->d_kbase) (void)hipFree(b->d_kbase);
   for (hipEvent_t e : {b->ev_ibm, b->ev_step, b->ev_bc, b->ev_geo[0], b->ev_geo[1], b->ev_rel[0], b->ev_rel[1]})
     if (e) (void)hipEventDestroy(e);
   for (hipStream_t s : {b->stream, b->stream_ibm, b->stream_geo, b->stream_bc})
@@ -1165,12 +1168,13 @@ int lbm_batch_create(lbm_batch** out) {
       hipEventCreateWithFlags(&b->ev_geo[0], hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&b->ev_geo[1], hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&b->ev_rel[0], hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&b->ev_rel[1], hipEventDisableTiming) != hipSuccess) {
+      hipEventCreateWithFlags(&b->ev_rel[1], hipEventDisableTiming) != hipSuccess ||
+      hipMalloc((void**)&b->d_kbase, 256) != hipSuccess) {
     batch_free(b);
     delete b;
     return LBM_E_DEVICE;
   }
-  if (const char* e = getenv("LBM_BATCH_GRAPHS")) b->use_graphs = atoi(e) != 0;
+  if (const char* e = getenv("LBM_BATCH_GRAPHS")) b->graph_mode = std::max(0, std::min(2, atoi(e)));
   *out = b;
   return 0;
 }
@@ -1291,6 +1295,7 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
   };
   if (!b->lat_bx) {  // tile width of the batched lattice launch, timed on this composition
     int rc = 0;
+    launch_step_base(b->stream, b->d_kbase, 0, false);  // the timed launches run substep 0
     b->lat_bx = autotune_tile(b->stream, [&](int bx) {
       if (rc) return;
       items.clear();
@@ -1300,6 +1305,7 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
       }
       if ((rc = upload())) return;
       BatchItems lt{};
+      lt.kbase = b->d_kbase;
       lt.lattice = b->d_items;
       lt.n_lattice = (int)items.size();
       launch_batch_lattice(b->stream, b->d_desc, lt, 0, f32, false, bx, false);
@@ -1341,91 +1347,112 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
   it.adj = di + off[2]; it.n_adj = (int)(off[3] - off[2]);
   it.seg = di + off[3]; it.n_seg = (int)(off[4] - off[3]);
   it.spread = di + off[4]; it.n_spread = (int)(off[5] - off[4]);
-  // The launch sequence of the call.  Every stream joins the lattice stream
-  // through events, so the sequence can be captured as one graph.  Geometry
-  // batch bb is issued one batch ahead of the flow (when batch bb-1 starts):
-  // the first two wait on the start of the call (the previous call has
-  // completed), later ones on the release of their slot set by the force row
-  // of the last step of batch bb-2.
-  auto issue_geometry = [&](int bb) -> int {
+  it.kbase = b->d_kbase;
+  // The launch sequence of the call, in segments of whole geometry batches
+  // (pl steps; 16 without wings), each ending with the force row of its last
+  // step.  At the start of a segment every earlier launch has joined the
+  // lattice stream, so the segment's first two geometry batches wait on
+  // ev_start, recorded there; later ones wait on the release of their slot
+  // set by the force row of the last step of batch bb-2.  Geometry batch bb
+  // is issued one batch ahead of the flow (at the first step of batch bb-1).
+  // Launches take the substep relative to the device step base kb0.
+  const int pl = wings ? ns : 16;
+  auto issue_geometry = [&](int bb, int seg_bb, int kb0) -> int {
     const int k0 = bb * ns;
     if (!wings || k0 >= steps) return 0;
     const int es = bb & 1;
-    BCHECK(b, hipStreamWaitEvent(b->stream_geo, bb < 2 ? b->ev_start : b->ev_rel[es], 0));
-    if (k0 + ns >= steps)
-      launch_batch_geometry(b->stream_geo, b->d_desc, di + off[6], (int)(off[7] - off[6]), k0, lds_wing, lds_solve);
-    else
-      launch_batch_geometry(b->stream_geo, b->d_desc, di + off[5], (int)(off[6] - off[5]), k0, lds_wing, lds_solve);
+    BCHECK(b, hipStreamWaitEvent(b->stream_geo, bb < seg_bb + 2 ? b->ev_start : b->ev_rel[es], 0));
+    const int q = k0 + ns >= steps ? 6 : 5;  // the last (shorter) batch has its own list
+    launch_batch_geometry(b->stream_geo, b->d_desc, b->d_kbase, di + off[q], (int)(off[q + 1] - off[q]), k0 - kb0,
+                          lds_wing, lds_solve);
     BCHECK(b, hipEventRecord(b->ev_geo[es], b->stream_geo));
     return 0;
   };
-  auto issue = [&]() -> int {
-  BCHECK(b, hipEventRecord(b->ev_start, b->stream));
-  {
-    const int rc = issue_geometry(0);
-    if (rc) return rc;
-  }
-  for (int k = 0; k < steps; k++) {
-    if (wings && k % ns == 0) {
-      const int es = (k / ns) & 1;
-      BCHECK(b, hipStreamWaitEvent(b->stream_ibm, b->ev_geo[es], 0));
-      BCHECK(b, hipStreamWaitEvent(b->stream, b->ev_geo[es], 0));  // lattice reads the claim maps
-    }
-    // one writer per node, as in lbm_advance: lattice (all but the adjoint
-    // nodes), IBM stream (adjoint nodes), ring stream (boundary ring)
-    const bool fields = k >= steps - 3;
-    BCHECK(b, hipEventRecord(b->ev_step, b->stream));
-    BCHECK(b, hipStreamWaitEvent(b->stream_ibm, b->ev_step, 0));
-    BCHECK(b, hipStreamWaitEvent(b->stream_bc, b->ev_step, 0));
-    launch_batch_bc_ring(b->stream_bc, b->d_desc, it, k, f32, fields);
-    BCHECK(b, hipEventRecord(b->ev_bc, b->stream_bc));
-    if (k > 0) {
-      launch_batch_force_row(b->stream_ibm, b->d_desc, count, k - 1);
-      if (wings && k % ns == 0) BCHECK(b, hipEventRecord(b->ev_rel[((k - 1) / ns) & 1], b->stream_ibm));
-    }
-    if (wings && k % ns == 0) {
-      const int rc = issue_geometry(k / ns + 1);
+  auto issue_steps = [&](int k_begin, int k_end, int kb0) -> int {
+    const int seg_bb = k_begin / pl;
+    BCHECK(b, hipEventRecord(b->ev_start, b->stream));
+    if (k_begin == 0) {
+      const int rc = issue_geometry(0, 0, kb0);
       if (rc) return rc;
     }
-    launch_batch_ibm_flow(b->stream_ibm, b->d_desc, it, k, f32);
-    launch_batch_adj_collide(b->stream_ibm, b->d_desc, it, k, f32, fields);
-    BCHECK(b, hipStreamWaitEvent(b->stream_ibm, b->ev_bc, 0));
-    BCHECK(b, hipEventRecord(b->ev_ibm, b->stream_ibm));
-    launch_batch_lattice(b->stream, b->d_desc, it, k, f32, fields, b->lat_bx, true);
-    BCHECK(b, hipStreamWaitEvent(b->stream, b->ev_ibm, 0));
-  }
-  BCHECK(b, hipEventRecord(b->ev_ibm, b->stream_ibm));
-  BCHECK(b, hipStreamWaitEvent(b->stream, b->ev_ibm, 0));
-  launch_batch_force_row(b->stream, b->d_desc, count, steps - 1);
-  return 0;
-  };
-  // replay a captured graph of this (composition, steps) pair, capture it on
-  // its second occurrence, run eagerly otherwise
-  std::vector<long long> key = {count, steps, ns, f32, wings, b->lat_bx, (long long)(size_t)b->d_desc,
-                                (long long)(size_t)b->d_items, (long long)lds_wing, (long long)lds_solve};
-  for (int q = 0; q < 8; q++) key.push_back((long long)off[q]);
-  for (int e = 0; e < count; e++) key.push_back((long long)(size_t)hs[e]);
-  BatchGraph* gr = nullptr;
-  if (b->use_graphs && steps <= kMaxGraphSteps && (!wings || steps <= ns)) {
-    for (auto& g : b->graphs)
-      if (g.key == key) gr = &g;
-    if (!gr) {
-      if (b->graphs.size() >= 8) {  // evict the least recently used
-        auto lru = std::min_element(b->graphs.begin(), b->graphs.end(), [](const BatchGraph& x, const BatchGraph& y) {
-          return x.last_use < y.last_use;
-        });
-        if (lru->exec) (void)hipGraphExecDestroy(lru->exec);
-        b->graphs.erase(lru);
+    for (int k = k_begin; k < k_end; k++) {
+      const int kk = k - kb0;
+      // the geometry of a batch starting at k_begin > 0 joined before the segment
+      if (wings && k % ns == 0 && (k > k_begin || k == 0)) {
+        const int es = (k / ns) & 1;
+        BCHECK(b, hipStreamWaitEvent(b->stream_ibm, b->ev_geo[es], 0));
+        BCHECK(b, hipStreamWaitEvent(b->stream, b->ev_geo[es], 0));  // lattice reads the claim maps
       }
-      b->graphs.push_back(BatchGraph{key, 0, nullptr, 0});
-      gr = &b->graphs.back();
+      // one writer per node, as in lbm_advance: lattice (all but the adjoint
+      // nodes), IBM stream (adjoint nodes), ring stream (boundary ring)
+      const bool fields = k >= steps - 3;
+      BCHECK(b, hipEventRecord(b->ev_step, b->stream));
+      BCHECK(b, hipStreamWaitEvent(b->stream_ibm, b->ev_step, 0));
+      BCHECK(b, hipStreamWaitEvent(b->stream_bc, b->ev_step, 0));
+      launch_batch_bc_ring(b->stream_bc, b->d_desc, it, kk, f32, fields);
+      BCHECK(b, hipEventRecord(b->ev_bc, b->stream_bc));
+      if (k > k_begin) {
+        launch_batch_force_row(b->stream_ibm, b->d_desc, b->d_kbase, count, kk - 1);
+        if (wings && k % ns == 0) BCHECK(b, hipEventRecord(b->ev_rel[((k - 1) / ns) & 1], b->stream_ibm));
+      }
+      if (wings && k % ns == 0) {
+        const int rc = issue_geometry(k / ns + 1, seg_bb, kb0);
+        if (rc) return rc;
+      }
+      launch_batch_ibm_flow(b->stream_ibm, b->d_desc, it, kk, f32);
+      launch_batch_adj_collide(b->stream_ibm, b->d_desc, it, kk, f32, fields);
+      BCHECK(b, hipStreamWaitEvent(b->stream_ibm, b->ev_bc, 0));
+      BCHECK(b, hipEventRecord(b->ev_ibm, b->stream_ibm));
+      launch_batch_lattice(b->stream, b->d_desc, it, kk, f32, fields, b->lat_bx, true);
+      BCHECK(b, hipStreamWaitEvent(b->stream, b->ev_ibm, 0));
     }
-    gr->last_use = ++b->calls;
-  }
-  if (gr && !gr->exec && gr->seen >= 1) {
+    // the geometry of the batch after the segment was issued inside it
+    if (wings && k_end < steps) BCHECK(b, hipStreamWaitEvent(b->stream, b->ev_geo[(k_end / ns) & 1], 0));
+    BCHECK(b, hipEventRecord(b->ev_ibm, b->stream_ibm));
+    BCHECK(b, hipStreamWaitEvent(b->stream, b->ev_ibm, 0));
+    launch_batch_force_row(b->stream, b->d_desc, b->d_kbase, count, k_end - 1 - kb0);
+    return 0;
+  };
+  // Captured graphs.  A call of at most kMaxGraphSteps steps within one
+  // geometry batch is captured whole (on the second call of the same
+  // composition and length).  With graph_mode 2 a longer call runs its first
+  // batch and its last two eagerly, and every batch in between (no field
+  // writes, the next batch a full one) by replaying one graph of pl steps
+  // that also computes the geometry of the next batch and ends by advancing
+  // the device step base by pl; the graph depends only on the composition.
+  // Off by default: the graph executor spreads the nodes over its own queues,
+  // and 64 uneven episodes ran 9 % slower that way than from the four streams
+  // launch by launch.
+  // The key: the composition, the device lists and, for a whole advance,
+  // its length and the last geometry list (a period graph uses the full
+  // list only).
+  std::vector<long long> key = {count, ns, f32, wings, b->lat_bx, (long long)(size_t)b->d_desc,
+                                (long long)(size_t)b->d_items, (long long)lds_wing, (long long)lds_solve};
+  for (int q = 0; q < 7; q++) key.push_back((long long)off[q]);
+  for (int e = 0; e < count; e++) key.push_back((long long)(size_t)hs[e]);
+  auto find_graph = [&](long long tag) -> BatchGraph* {
+    std::vector<long long> kq = key;
+    kq.push_back(tag);
+    if (tag > 0) kq.push_back((long long)off[7]);
+    for (auto& g : b->graphs)
+      if (g.key == kq) {
+        g.last_use = ++b->calls;
+        return &g;
+      }
+    if (b->graphs.size() >= 8) {  // evict the least recently used
+      auto lru = std::min_element(b->graphs.begin(), b->graphs.end(), [](const BatchGraph& x, const BatchGraph& y) {
+        return x.last_use < y.last_use;
+      });
+      if (lru->exec) (void)hipGraphExecDestroy(lru->exec);
+      b->graphs.erase(lru);
+    }
+    b->graphs.push_back(BatchGraph{kq, 0, nullptr, ++b->calls});
+    return &b->graphs.back();
+  };
+  auto capture = [&](BatchGraph* gr, const std::function<int()>& body) -> int {
     hipGraph_t graph = nullptr;
     BCHECK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    const int rc = issue();
+    const int rc = body();
     const hipError_t ec = hipStreamEndCapture(b->stream, &graph);
     if (rc) {
       if (graph) (void)hipGraphDestroy(graph);
@@ -1438,13 +1465,51 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
       gr->exec = nullptr;
       return bfail(b, LBM_E_DEVICE, "hipGraphInstantiate: %s", hipGetErrorString(ei));
     }
+    return 0;
+  };
+  // periods replayed from the period graph: 1 .. J
+  int J = 0;
+  if (b->graph_mode >= 2 && steps > pl) {
+    J = (steps - 1) / pl - 2;
+    while (J > 0 && (J + 1) * pl > steps - 3) J--;
   }
-  if (gr && gr->exec) {
-    BCHECK(b, hipGraphLaunch(gr->exec, b->stream));
-  } else {
-    const int rc = issue();
+  if (J >= 1) {
+    const int kc = (J + 1) * pl;
+    launch_step_base(b->stream, b->d_kbase, 0, false);
+    int rc = issue_steps(0, pl, 0);
     if (rc) return rc;
-    if (gr) gr->seen++;
+    launch_step_base(b->stream, b->d_kbase, pl, true);
+    BatchGraph* gr = find_graph(-pl);
+    if (!gr->exec) {
+      rc = capture(gr, [&]() -> int {
+        const int r = issue_steps(pl, 2 * pl, pl);
+        if (!r) launch_step_base(b->stream, b->d_kbase, pl, true);
+        return r;
+      });
+      if (rc) return rc;
+    }
+    for (int j = 1; j <= J; j++) BCHECK(b, hipGraphLaunch(gr->exec, b->stream));
+    gr->seen += J;
+    rc = issue_steps(kc, steps, kc);
+    if (rc) return rc;
+  } else {
+    auto whole = [&]() -> int {
+      launch_step_base(b->stream, b->d_kbase, 0, false);
+      return issue_steps(0, steps, 0);
+    };
+    BatchGraph* gr = nullptr;
+    if (b->graph_mode >= 1 && steps <= kMaxGraphSteps && (!wings || steps <= ns)) gr = find_graph(steps);
+    if (gr && !gr->exec && gr->seen >= 1) {
+      const int rc = capture(gr, whole);
+      if (rc) return rc;
+    }
+    if (gr && gr->exec) {
+      BCHECK(b, hipGraphLaunch(gr->exec, b->stream));
+    } else {
+      const int rc = whole();
+      if (rc) return rc;
+      if (gr) gr->seen++;
+    }
   }
   // host state of every instance as if it had run the steps alone
   std::vector<char> have_err(count, 0);

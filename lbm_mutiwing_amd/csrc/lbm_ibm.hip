@@ -700,59 +700,67 @@ void launch_ibm_flow(hipStream_t s, const IbmParams& p, const MrtParams& P, cons
 int spread_blocks(const IbmParams& p) { return p.W * ((p.SPW + SPREAD_SEG - 1) / SPREAD_SEG); }
 
 __global__ __launch_bounds__(GEO_THREADS) void kb_geo_wing(const EpDesc* __restrict__ eps,
-                                                           const int2* __restrict__ items, int k0) {
+                                                           const int2* __restrict__ items,
+                                                           const int* __restrict__ kb, int kk0) {
   const int2 it = items[blockIdx.x];
   const EpDesc& D = eps[it.x];
+  const int k0 = kb[0] + kk0;
   const int b = it.y / D.p.W, w = it.y - b * D.p.W;
   geo_wing_body(D.p, D.G, D.st, ep_slot(D, k0), D.n0 + k0, k0, b, w);
 }
 
 __global__ __launch_bounds__(GEO_THREADS) void kb_geo_solve(const EpDesc* __restrict__ eps,
-                                                            const int2* __restrict__ items, int k0) {
+                                                            const int2* __restrict__ items,
+                                                            const int* __restrict__ kb, int kk0) {
   const int2 it = items[blockIdx.x];
   const EpDesc& D = eps[it.x];
+  const int k0 = kb[0] + kk0;
   const int b = it.y / D.p.W, w = it.y - b * D.p.W;
   geo_solve_body(D.p, D.G, D.st, ep_slot(D, k0), b, w);
 }
 
-void launch_batch_geometry(hipStream_t s, const EpDesc* eps, const int2* items, int n_items, int k0,
-                           size_t lds_wing, size_t lds_solve) {
+void launch_batch_geometry(hipStream_t s, const EpDesc* eps, const int* kb, const int2* items, int n_items,
+                           int k0, size_t lds_wing, size_t lds_solve) {
   if (n_items <= 0) return;
-  hipLaunchKernelGGL(kb_geo_wing, dim3(n_items), dim3(GEO_THREADS), lds_wing, s, eps, items, k0);
-  hipLaunchKernelGGL(kb_geo_solve, dim3(n_items), dim3(GEO_THREADS), lds_solve, s, eps, items, k0);
+  hipLaunchKernelGGL(kb_geo_wing, dim3(n_items), dim3(GEO_THREADS), lds_wing, s, eps, items, kb, k0);
+  hipLaunchKernelGGL(kb_geo_solve, dim3(n_items), dim3(GEO_THREADS), lds_solve, s, eps, items, kb, k0);
 }
 
 template <typename S>
 __global__ __launch_bounds__(256) void kb_ibm_rhs(const EpDesc* __restrict__ eps, const int2* __restrict__ items,
-                                                  int k) {
+                                                  const int* __restrict__ kb, int kk) {
   const int2 it = items[blockIdx.x];
   const EpDesc& D = eps[it.x];
+  const int k = kb[0] + kk;
   const int cur = (D.cur0 + k) & 1;
   ibm_rhs_body<S>(D.p, D.P, D.G, ep_slot(D, k), D.F, D.g, (const S*)D.fbuf[cur], it.y);
 }
 
 __global__ __launch_bounds__(256) void kb_ibm_matvec(const EpDesc* __restrict__ eps, const int2* __restrict__ items,
-                                                     int k) {
+                                                     const int* __restrict__ kb, int kk) {
   const int2 it = items[blockIdx.x];
   const EpDesc& D = eps[it.x];
+  const int k = kb[0] + kk;
   ibm_matvec_body(D.p, D.G, ep_slot(D, k), D.F, it.y);
 }
 
 __global__ __launch_bounds__(SPREAD_THREADS) void kb_ibm_spread(const EpDesc* __restrict__ eps,
-                                                                const int2* __restrict__ items, int k) {
+                                                                const int2* __restrict__ items,
+                                                                const int* __restrict__ kb, int kk) {
   const int2 it = items[blockIdx.x];
   const EpDesc& D = eps[it.x];
+  const int k = kb[0] + kk;
   ibm_spread_body(D.p, D.G, ep_slot(D, k), D.F, it.y);
 }
 
 void launch_batch_ibm_flow(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32) {
   if (it.n_seg <= 0) return;
   if (f32)
-    hipLaunchKernelGGL(kb_ibm_rhs<float>, dim3(it.n_seg), dim3(256), 0, s, eps, it.seg, k);
+    hipLaunchKernelGGL(kb_ibm_rhs<float>, dim3(it.n_seg), dim3(256), 0, s, eps, it.seg, it.kbase, k);
   else
-    hipLaunchKernelGGL(kb_ibm_rhs<double>, dim3(it.n_seg), dim3(256), 0, s, eps, it.seg, k);
-  hipLaunchKernelGGL(kb_ibm_matvec, dim3(it.n_seg), dim3(256), 0, s, eps, it.seg, k);
-  hipLaunchKernelGGL(kb_ibm_spread, dim3(it.n_spread), dim3(SPREAD_THREADS), 0, s, eps, it.spread, k);
+    hipLaunchKernelGGL(kb_ibm_rhs<double>, dim3(it.n_seg), dim3(256), 0, s, eps, it.seg, it.kbase, k);
+  hipLaunchKernelGGL(kb_ibm_matvec, dim3(it.n_seg), dim3(256), 0, s, eps, it.seg, it.kbase, k);
+  hipLaunchKernelGGL(kb_ibm_spread, dim3(it.n_spread), dim3(SPREAD_THREADS), 0, s, eps, it.spread, it.kbase, k);
 }
 
 }  // namespace lbm

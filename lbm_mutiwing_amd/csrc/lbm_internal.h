@@ -171,6 +171,7 @@ __host__ __device__ __forceinline__ int ep_slot(const EpDesc& D, int k) {
 // Work items of the batched launches: (episode, block of that episode's own
 // single-instance grid).
 struct BatchItems {
+  const int* kbase;                  // device step base: a launch for step k passes k - *kbase
   const int2* lattice; int n_lattice;
   const int2* ring; int n_ring;
   const int2* adj; int n_adj;
@@ -220,18 +221,22 @@ int adj_collide_blocks_per_wing(const IbmParams& p);
 void launch_geometry(hipStream_t s, const IbmParams& p, const GeoBuffers& G, DevStep* st, int slot0, int n0,
                      int fidx0, int count, bool with_inverse);
 size_t geo_solve_lds_bytes(const IbmParams& p);
-// Batched launches over the episodes of eps (device array); k is the substep.
-// masked: skip each episode's adjoint nodes of substep k (its geometry must exist)
+// Batched launches over the episodes of eps (device array).  The substep of a
+// launch is *it.kbase + k (the base lives on the device, so that a captured
+// graph of one geometry batch replays for every batch of a long advance).
+// masked: skip each episode's adjoint nodes of the substep (its geometry must exist)
 void launch_batch_lattice(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields,
                           int bx, bool masked);
 void launch_batch_bc_ring(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields);
 void launch_batch_adj_collide(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields);
 void launch_batch_ibm_flow(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32);
-void launch_batch_force_row(hipStream_t s, const EpDesc* eps, int count, int k);
-// Geometry of substeps k0 .. k0+count-1 of every episode with wings; items
-// are (episode, b*W + w) for b < count.
-void launch_batch_geometry(hipStream_t s, const EpDesc* eps, const int2* items, int n_items, int k0,
-                           size_t lds_wing, size_t lds_solve);
+void launch_batch_force_row(hipStream_t s, const EpDesc* eps, const int* kb, int count, int k);
+// *p = v (add = false) or *p += v (add = true), one thread, in stream order
+void launch_step_base(hipStream_t s, int* p, int v, bool add);
+// Geometry of substeps *kb + k0 .. *kb + k0 + count - 1 of every episode with
+// wings; items are (episode, b*W + w) for b < count.
+void launch_batch_geometry(hipStream_t s, const EpDesc* eps, const int* kb, const int2* items, int n_items,
+                           int k0, size_t lds_wing, size_t lds_solve);
 size_t geo_wing_lds_bytes(int spw);
 int bc_ring_blocks(const Grid& g);
 int spread_blocks(const IbmParams& p);

@@ -689,9 +689,10 @@ int bc_ring_blocks(const Grid& g) { return (2 * (g.ny - 2) + 2 * g.nx + 255) / 2
 // masked = 0 (tile timing before any geometry exists): every interior node
 template <bool F32, bool FIELDS, int BX>
 __global__ __launch_bounds__(BX) void kb_lattice(const EpDesc* __restrict__ eps, const int2* __restrict__ items,
-                                                 int k, int masked) {
+                                                 const int* __restrict__ kb, int kk, int masked) {
   const int2 it = items[blockIdx.x];
   const EpDesc& D = eps[it.x];
+  const int k = kb[0] + kk;
   const int n = D.n0 + k, cur = (D.cur0 + k) & 1;
   AdjMask am = adj_mask(D.p, D.G, ep_slot(D, k));
   if (!masked) am.W = 0;
@@ -705,31 +706,44 @@ __global__ __launch_bounds__(BX) void kb_lattice(const EpDesc* __restrict__ eps,
 
 template <typename S, bool FIELDS>
 __global__ __launch_bounds__(256) void kb_bc_ring(const EpDesc* __restrict__ eps, const int2* __restrict__ items,
-                                                  int k) {
+                                                  const int* __restrict__ kb, int kk) {
   const int2 it = items[blockIdx.x];
   const EpDesc& D = eps[it.x];
+  const int k = kb[0] + kk;
   const int n = D.n0 + k, cur = (D.cur0 + k) & 1;
   bc_ring_body<S, FIELDS>(D.g, D.P, D.R, (const S*)D.fbuf[cur], (S*)D.fbuf[cur ^ 1], D.fslot[n % 3], it.y);
 }
 
 template <typename S, bool FIELDS>
 __global__ __launch_bounds__(ADJ_THREADS) void kb_adj_collide(const EpDesc* __restrict__ eps,
-                                                              const int2* __restrict__ items, int k) {
+                                                              const int2* __restrict__ items,
+                                                              const int* __restrict__ kb, int kk) {
   const int2 it = items[blockIdx.x];
   const EpDesc& D = eps[it.x];
+  const int k = kb[0] + kk;
   const int n = D.n0 + k, cur = (D.cur0 + k) & 1;
   adj_collide_body<S, FIELDS>(D.g, D.P, D.p, D.G, ep_slot(D, k), D.F, D.bpw, n, (const S*)D.fbuf[cur],
                               (S*)D.fbuf[cur ^ 1], D.fslot[n % 3], it.y);
 }
 
 // force row of substep k of every episode: one workgroup per episode
-__global__ __launch_bounds__(64) void kb_force_row(const EpDesc* __restrict__ eps, int k) {
+__global__ __launch_bounds__(64) void kb_force_row(const EpDesc* __restrict__ eps, const int* __restrict__ kb,
+                                                   int kk) {
   const EpDesc& D = eps[blockIdx.x];
+  const int k = kb[0] + kk;
   force_row_body(D.p, D.G, ep_slot(D, k), D.F, D.st, D.n0 + k, k, 4 + 12 * D.p.W, D.bpw);
 }
 
-void launch_batch_force_row(hipStream_t s, const EpDesc* eps, int count, int k) {
-  hipLaunchKernelGGL(kb_force_row, dim3(count), dim3(64), 0, s, eps, k);
+void launch_batch_force_row(hipStream_t s, const EpDesc* eps, const int* kb, int count, int k) {
+  hipLaunchKernelGGL(kb_force_row, dim3(count), dim3(64), 0, s, eps, kb, k);
+}
+
+__global__ __launch_bounds__(64) void k_step_base(int* __restrict__ p, int v, int add) {
+  if (threadIdx.x == 0) p[0] = add ? p[0] + v : v;
+}
+
+void launch_step_base(hipStream_t s, int* p, int v, bool add) {
+  hipLaunchKernelGGL(k_step_base, dim3(1), dim3(64), 0, s, p, v, add ? 1 : 0);
 }
 
 template <int BX>
@@ -737,11 +751,11 @@ void launch_batch_lattice_bx(hipStream_t s, const EpDesc* eps, const BatchItems&
                              int masked) {
   dim3 grid(it.n_lattice), block(BX);
   if (f32) {
-    if (fields) hipLaunchKernelGGL((kb_lattice<true, true, BX>), grid, block, 0, s, eps, it.lattice, k, masked);
-    else hipLaunchKernelGGL((kb_lattice<true, false, BX>), grid, block, 0, s, eps, it.lattice, k, masked);
+    if (fields) hipLaunchKernelGGL((kb_lattice<true, true, BX>), grid, block, 0, s, eps, it.lattice, it.kbase, k, masked);
+    else hipLaunchKernelGGL((kb_lattice<true, false, BX>), grid, block, 0, s, eps, it.lattice, it.kbase, k, masked);
   } else {
-    if (fields) hipLaunchKernelGGL((kb_lattice<false, true, BX>), grid, block, 0, s, eps, it.lattice, k, masked);
-    else hipLaunchKernelGGL((kb_lattice<false, false, BX>), grid, block, 0, s, eps, it.lattice, k, masked);
+    if (fields) hipLaunchKernelGGL((kb_lattice<false, true, BX>), grid, block, 0, s, eps, it.lattice, it.kbase, k, masked);
+    else hipLaunchKernelGGL((kb_lattice<false, false, BX>), grid, block, 0, s, eps, it.lattice, it.kbase, k, masked);
   }
 }
 
@@ -760,11 +774,11 @@ void launch_batch_bc_ring(hipStream_t s, const EpDesc* eps, const BatchItems& it
   if (it.n_ring <= 0) return;
   dim3 grid(it.n_ring), block(256);
   if (f32) {
-    if (fields) hipLaunchKernelGGL((kb_bc_ring<float, true>), grid, block, 0, s, eps, it.ring, k);
-    else hipLaunchKernelGGL((kb_bc_ring<float, false>), grid, block, 0, s, eps, it.ring, k);
+    if (fields) hipLaunchKernelGGL((kb_bc_ring<float, true>), grid, block, 0, s, eps, it.ring, it.kbase, k);
+    else hipLaunchKernelGGL((kb_bc_ring<float, false>), grid, block, 0, s, eps, it.ring, it.kbase, k);
   } else {
-    if (fields) hipLaunchKernelGGL((kb_bc_ring<double, true>), grid, block, 0, s, eps, it.ring, k);
-    else hipLaunchKernelGGL((kb_bc_ring<double, false>), grid, block, 0, s, eps, it.ring, k);
+    if (fields) hipLaunchKernelGGL((kb_bc_ring<double, true>), grid, block, 0, s, eps, it.ring, it.kbase, k);
+    else hipLaunchKernelGGL((kb_bc_ring<double, false>), grid, block, 0, s, eps, it.ring, it.kbase, k);
   }
 }
 
@@ -772,11 +786,11 @@ void launch_batch_adj_collide(hipStream_t s, const EpDesc* eps, const BatchItems
   if (it.n_adj <= 0) return;
   dim3 grid(it.n_adj), block(ADJ_THREADS);
   if (f32) {
-    if (fields) hipLaunchKernelGGL((kb_adj_collide<float, true>), grid, block, 0, s, eps, it.adj, k);
-    else hipLaunchKernelGGL((kb_adj_collide<float, false>), grid, block, 0, s, eps, it.adj, k);
+    if (fields) hipLaunchKernelGGL((kb_adj_collide<float, true>), grid, block, 0, s, eps, it.adj, it.kbase, k);
+    else hipLaunchKernelGGL((kb_adj_collide<float, false>), grid, block, 0, s, eps, it.adj, it.kbase, k);
   } else {
-    if (fields) hipLaunchKernelGGL((kb_adj_collide<double, true>), grid, block, 0, s, eps, it.adj, k);
-    else hipLaunchKernelGGL((kb_adj_collide<double, false>), grid, block, 0, s, eps, it.adj, k);
+    if (fields) hipLaunchKernelGGL((kb_adj_collide<double, true>), grid, block, 0, s, eps, it.adj, it.kbase, k);
+    else hipLaunchKernelGGL((kb_adj_collide<double, false>), grid, block, 0, s, eps, it.adj, it.kbase, k);
   }
 }
 

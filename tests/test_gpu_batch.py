@@ -93,29 +93,33 @@ def test_batch_pending_move_is_rejected(lib):
     b.close()
 
 
-SPANS_GEOMETRY_BATCHES = pytest.param(
-    PRECISION_FP64, 40, marks=pytest.mark.skip(
-        reason="40 steps span three geometry batches of 16 steps; capturing that form ended the process "
-               "with a host-side segmentation fault inside lbm_batch_advance, cause not found, so the "
-               "library no longer captures advances longer than one geometry batch"))
-
-
-@pytest.mark.parametrize("precision,steps", [(PRECISION_FP64, 12), (PRECISION_FP32_SHIFTED, 12),
-                                             SPANS_GEOMETRY_BATCHES])
-def test_batch_graph_replays(lib, precision, steps):
-    """Same composition and step count four times: the first call runs
-    eagerly, the second captures a graph and launches it, the third and fourth
-    replay it.  Each replay is checked against the instances run alone, with
-    eager work and synchronisations of the host in between."""
-    alone = [make(sp, precision, 400) for sp in SPECS]
-    mixed = [make(sp, precision, 400) for sp in SPECS]
+@pytest.mark.parametrize("precision,steps,specs", [
+    (PRECISION_FP64, 12, SPECS), (PRECISION_FP32_SHIFTED, 12, SPECS),   # captured whole
+    (PRECISION_FP64, 100, SPECS), (PRECISION_FP32_SHIFTED, 100, SPECS),  # batches 1-4 from one graph
+    (PRECISION_FP64, 70, SPECS[2:3]),                                   # no wings: 16-step periods
+])
+def test_batch_graph_replays(lib, precision, steps, specs, monkeypatch):
+    """Same composition and step count four times, graph mode 2.  A short
+    advance runs eagerly first, is captured on the second call and replayed on
+    the third and fourth; a long one replays the graph of one geometry batch
+    for its middle batches from the first call on.  Each call is checked against the
+    instances run alone, with eager work and synchronisations of the host in
+    between."""
+    n_table = 4 * steps + 50
+    alone = [make(sp, precision, n_table) for sp in specs]
+    mixed = [make(sp, precision, n_table) for sp in specs]
+    monkeypatch.setenv("LBM_BATCH_GRAPHS", "2")   # read when the batch object is created
     b = Batch(lib)
     for call in range(4):
-        print(f"graph replay: precision {precision} steps {steps} call {call}", flush=True)
+        print(f"graph replay: precision {precision} steps {steps} instances {len(specs)} call {call}", flush=True)
         ra = [s.advance(s.state()["n_next"] + steps - 1) for s in alone]
         rb = b.advance(mixed, steps)
         for e, (sa, sb) in enumerate(zip(alone, mixed)):
+            assert sa.state() == sb.state(), (call, e)
             assert np.array_equal(ra[e].last_row, rb[e].last_row), (call, e)
             assert np.array_equal(ra[e].row_sum, rb[e].row_sum), (call, e)
+            assert ra[e].err == rb[e].err, (call, e)
             assert np.array_equal(sa.populations()[0], sb.populations()[0]), (call, e)
+            for x, y in zip(sa.fields()[:3], sb.fields()[:3]):
+                assert np.array_equal(x, y), (call, e)
     b.close()
