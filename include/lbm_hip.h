@@ -160,6 +160,14 @@ int lbm_get_markers(lbm_handle *h, double *xb1, double *yb1, double *xb2,
 int lbm_get_ibm(lbm_handle *h, int32_t capacity, int32_t *adjoint_ij,
                 int32_t *wing_id, double *wing_force, double *wing_moment);
 
+/* Restart hook (readflowfield, solution.f90:1721-1851, and the restart branch
+ * of main.f90:343-347): the next step is n_next and the frame shift is
+ * movenum; the field history of erranalysis is dropped (err reads 0 for the
+ * next two steps) and flag2 is 1, as after init.  Call it before
+ * lbm_set_populations, which records the moments of f as the fields of step
+ * n_next - 1. */
+int lbm_set_step(lbm_handle *h, int32_t n_next, int32_t movenum);
+
 /* Rigid-body displacement hook: body_rel_pos(wing) (main.f90:77).  In the
  * reference only the (disabled) dynamics solver moves it; setting |pos| >= 1
  * for wing 1 triggers move_mesh after the next step (main.f90:393-411). */

@@ -1062,6 +1062,22 @@ int lbm_get_tuning(const lbm_handle* h, int32_t* lattice_tile) {
   return 0;
 }
 
+int lbm_set_step(lbm_handle* h, int32_t n_next, int32_t movenum) {
+  if (!h) return LBM_E_INVALID;
+  if (n_next < 1) return fail(h, LBM_E_INVALID, "n_next must be at least 1");
+  HIPCHECK(h, hipStreamSynchronize(h->stream_geo));
+  HIPCHECK(h, hipStreamSynchronize(h->stream_ibm));
+  HIPCHECK(h, hipStreamSynchronize(h->stream));
+  h->n = n_next;
+  h->movenum = movenum;
+  h->flag2 = 1;  // main.f90:349 holds for a restart too
+  h->last_move_step = -1;
+  h->last_slot = -1;
+  for (int s = 0; s < 3; s++) h->slot_step[s] = -1;
+  update_body_extra(h);
+  return 0;
+}
+
 int lbm_set_body_rel_pos(lbm_handle* h, int32_t wing, double pos) {
   if (!h || wing < 1 || wing > h->W) return fail(h, LBM_E_INVALID, "wing out of range");
   h->body_rel_pos[wing - 1] = pos;

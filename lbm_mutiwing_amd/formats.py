@@ -8,7 +8,9 @@
   (read_boundary, airfoil.f90:207-242);
 * the Tecplot flow snapshot ``F<n>.dat`` of SAVE / output (solution.f90:132-241):
   the flow zone sampled every second node with velocity, pressure and
-  vorticity in ``(25e16.8)`` records, then the airfoil FE-triangle zone.
+  vorticity in ``(25e16.8)`` records, then the airfoil FE-triangle zone;
+* the restart checkpoint ``saved_flow_feild.dat`` (saveflowfield /
+  readflowfield, solution.f90:1616-1851): :func:`write_checkpoint`.
 
 The offline (file-mode) run loop of the reference (main.f90:374-441) is
 :func:`run_file_mode`: it drives a solver over a motion table and writes the
@@ -17,6 +19,8 @@ two per-step files.
 from __future__ import annotations
 
 import os
+import struct
+from dataclasses import dataclass
 from typing import Iterable, List, Optional, TextIO
 
 import numpy as np
@@ -268,3 +272,96 @@ def tecplot_save_hook(out_dir: str):
         return name
 
     return hook
+
+
+# ------------------------------------------------------------ checkpoint
+
+RING_NAMES = ("left2", "f0_left", "right2", "f0_right", "up2", "f0_top", "down2", "f0_bottom")
+
+
+@dataclass
+class Checkpoint:
+    """Contents of a restart checkpoint.  u, v: (ny, nx); f: (9, ny, nx);
+    body_rel: (11, W), the rigid-body arrays in file order (body_rel_a, _a_prev,
+    _v, _v_prev, _s, _s_prev, _pos, _theta_prev, _theta, _temp_theta_prev,
+    _temp_theta); ring: the previous-step boundary populations and the f0
+    ring, ``left2``/``f0_left``/``right2``/``f0_right`` (9, ny) (rows 2..ny-1
+    are stored) and ``up2``/``f0_top``/``down2``/``f0_bottom`` (9, nx)."""
+    cycle: int
+    movenum: int
+    body_rel: np.ndarray
+    u: np.ndarray
+    v: np.ndarray
+    f: np.ndarray
+    ring: dict
+
+
+def checkpoint_bytes(wing_count: int, nx: int, ny: int) -> int:
+    return 12 + 88 * wing_count + 88 * nx * ny + 288 * (ny - 2) + 288 * nx
+
+
+def ring_from_populations(f: np.ndarray) -> dict:
+    """The ring records as init leaves them, from the populations f
+    (9, ny, nx) just after init (solution.f90:55-63)."""
+    return {"left2": f[:, :, 0].copy(), "f0_left": f[:, :, 0].copy(), "right2": f[:, :, -1].copy(),
+            "f0_right": f[:, :, -1].copy(), "up2": f[:, -1, :].copy(), "f0_top": f[:, -1, :].copy(),
+            "down2": f[:, 0, :].copy(), "f0_bottom": f[:, 0, :].copy()}
+
+
+def write_checkpoint(path: str, ck: Checkpoint) -> None:
+    """saveflowfield (solution.f90:1660-1686): int32 step, int32 wing count,
+    11*W doubles, int32 movenum, then (u, v, f1..f9) per node with i fastest,
+    then per row j = 2..ny-1 (leftf2, f0(1,j), rightf2, f0(nx,j)), then per
+    column i = 1..nx (upf2, f0(i,ny), downf2, f0(i,1)); 9 doubles each, no
+    record markers (form='binary')."""
+    ny, nx = ck.u.shape
+    W = ck.body_rel.shape[1]
+    node = np.empty((ny, nx, 11))
+    node[:, :, 0] = ck.u
+    node[:, :, 1] = ck.v
+    node[:, :, 2:] = np.moveaxis(ck.f, 0, -1)
+    r = ck.ring
+    rows = np.stack([r["left2"].T, r["f0_left"].T, r["right2"].T, r["f0_right"].T], axis=1)[1:ny - 1]
+    cols = np.stack([r["up2"].T, r["f0_top"].T, r["down2"].T, r["f0_bottom"].T], axis=1)
+    with open(path, "wb") as fh:
+        fh.write(struct.pack("<ii", ck.cycle, W))
+        fh.write(np.ascontiguousarray(ck.body_rel, dtype="<f8").tobytes())
+        fh.write(struct.pack("<i", ck.movenum))
+        fh.write(node.astype("<f8").tobytes())
+        fh.write(np.ascontiguousarray(rows, dtype="<f8").tobytes())
+        fh.write(np.ascontiguousarray(cols, dtype="<f8").tobytes())
+
+
+def read_checkpoint(path: str, wing_count: int, nx: int, ny: int) -> Checkpoint:
+    """readflowfield (solution.f90:1721-1787) for a case of wing_count wings
+    on an nx x ny lattice; raises ValueError on a size or wing-count
+    mismatch."""
+    data = open(path, "rb").read()
+    if len(data) != checkpoint_bytes(wing_count, nx, ny):
+        raise ValueError(f"checkpoint size {len(data)} does not fit {nx}x{ny} with {wing_count} wings")
+    cycle, W = struct.unpack_from("<ii", data, 0)
+    if W != wing_count:
+        raise ValueError("wing count of the checkpoint differs from the case")
+    o = 8
+    body_rel = np.frombuffer(data, "<f8", 11 * W, o).reshape(11, W).copy()
+    o += 88 * W
+    (movenum,) = struct.unpack_from("<i", data, o)
+    o += 4
+    node = np.frombuffer(data, "<f8", 11 * nx * ny, o).reshape(ny, nx, 11)
+    o += 88 * nx * ny
+    rows = np.frombuffer(data, "<f8", 36 * (ny - 2), o).reshape(ny - 2, 4, 9)
+    o += 288 * (ny - 2)
+    cols = np.frombuffer(data, "<f8", 36 * nx, o).reshape(nx, 4, 9)
+    ring = {}
+    for q, name in enumerate(("left2", "f0_left", "right2", "f0_right")):
+        a = np.zeros((9, ny))
+        a[:, 1:ny - 1] = rows[:, q, :].T
+        ring[name] = a
+    for q, name in enumerate(("up2", "f0_top", "down2", "f0_bottom")):
+        ring[name] = cols[:, q, :].T.copy()
+    # the f0 ring's corners are the end columns' rows 1 and ny
+    ring["f0_left"][:, 0], ring["f0_left"][:, -1] = ring["f0_bottom"][:, 0], ring["f0_top"][:, 0]
+    ring["f0_right"][:, 0], ring["f0_right"][:, -1] = ring["f0_bottom"][:, -1], ring["f0_top"][:, -1]
+    return Checkpoint(cycle, movenum, body_rel, node[:, :, 0].copy(), node[:, :, 1].copy(),
+                      np.moveaxis(node[:, :, 2:], -1, 0).copy(), ring)
+

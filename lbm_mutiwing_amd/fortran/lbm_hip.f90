@@ -123,6 +123,12 @@ module lbm_hip
             real(c_double), intent(out) :: wing_force(*), wing_moment(*)
         end function
 
+        integer(c_int) function lbm_set_step(h, n_next, movenum) bind(C, name='lbm_set_step')
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: h
+            integer(c_int32_t), value :: n_next, movenum
+        end function
+
         integer(c_int) function lbm_set_body_rel_pos(h, wing, pos) bind(C, name='lbm_set_body_rel_pos')
             import :: c_int, c_int32_t, c_ptr, c_double
             type(c_ptr), value :: h

@@ -12,6 +12,9 @@
 !                          (FORCE / STATUS / ERROR); per-episode rows go to
 !                          force_episode_<k>.dat and airfoil_motion_episode_<k>.dat,
 !                          SAVE writes episode_<k>/F<cycle>.dat
+! File mode also writes the restart checkpoint saved_flow_feild.dat and its
+! copy every checkpoint_every steps (main.f90:436-439) and, with
+! restart = .true., continues from one (main.f90:343-347, the goon branch).
 ! The reference fixes the case at compile time (main.f90:4-159,
 ! ini_mesh.f90:15-67); here it is read from the namelist /lbm_case/ in the
 ! file named by the first argument (default lbm_case.nml).  All solver work
@@ -27,15 +30,17 @@ program lbm_host
 
     ! ---- case (namelist /lbm_case/)
     integer :: nx = 0, ny = 0, wing_count = 2, points_per_wing = 0, precision = 0
-    integer :: rows_per_call = 100, print_every = 100
+    integer :: rows_per_call = 100, print_every = 100, checkpoint_every = 100000
+    logical :: restart = .false.
     real(c_double) :: scale = 0.01d0, ox = 0d0, oy = 0d0, uu = 0.05d0, re = 1000d0
     real(c_double) :: s(9) = [0d0, 1.4d0, 1.4d0, 0d0, 1.2d0, 0d0, 1.2d0, 0d0, 0d0]
     real(c_double) :: wing_x0 = 0d0, wing_dx = 1.5d0, wing_y0 = 0.5d0
     real(c_double) :: xc = 0d0, yc = 0d0, xp = 0d0, yp = 0d0
     character(len=256) :: boundary_dir = '.', motion_file = 'motion_params.dat', output_dir = 'output'
+    character(len=256) :: restart_dir = ''
     namelist /lbm_case/ nx, ny, scale, ox, oy, uu, re, s, wing_count, points_per_wing, &
         wing_x0, wing_dx, wing_y0, xc, yc, xp, yp, precision, boundary_dir, motion_file, &
-        output_dir, rows_per_call, print_every
+        output_dir, rows_per_call, print_every, checkpoint_every, restart, restart_dir
 
     ! per-step row files: force.dat and airfoil_motion.dat (file mode) or the
     ! current episode's force_episode_<k>.dat / airfoil_motion_episode_<k>.dat
@@ -156,6 +161,8 @@ contains
     subroutine run_file_mode()
         real(c_double), allocatable :: table(:, :)
         real(c_double) :: end_time, dt, rsum(LBM_MAX_FORCE_ROW), lrow(LBM_MAX_FORCE_ROW), err
+        real(c_double) :: brp(LBM_MAX_WINGS)
+        type(checkpoint_ring) :: ring
         integer :: nrows, n_end, n_next, n_last, steps
         integer(c_int) :: rc
 
@@ -171,22 +178,105 @@ contains
             n_end = n_end + 1
         enddo
 
+        brp = 0d0
+        if (restart) then
+            call restore_checkpoint(ring, brp, n_next)
+        else
+            call init_ring(ring)
+            n_next = 1
+        endif
         open(UNIT_FORCE, file=trim(output_dir) // '/force.dat', status='replace', action='write')
         open(UNIT_MOTION, file=trim(output_dir) // '/airfoil_motion.dat', status='replace', action='write')
-        n_next = 1
         do while (n_next <= n_end)
             n_last = min(n_end, n_next + rows_per_call - 1)
+            if (checkpoint_every > 0) n_last = min(n_last, ((n_next - 1) / checkpoint_every + 1) * checkpoint_every)
             call advance_to(n_last, rsum, lrow, steps, err, rc)
             call check(rc, 'lbm_advance')
             if (print_every > 0) then
                 if (n_last / print_every > (n_next - 1) / print_every) &
                     write(*, '(a,i0,a,es12.5)') 'step ', n_last, '  err ', err
             endif
+            if (checkpoint_every > 0) then
+                if (mod(n_last, checkpoint_every) == 0) call save_checkpoint(n_last, ring, brp)
+            endif
             n_next = n_last + 1
         enddo
         close(UNIT_FORCE)
         close(UNIT_MOTION)
         write(*, '(a,i0,a)') 'file mode done: ', n_end, ' steps'
+    end subroutine
+
+    ! The boundary records of a fresh start: the populations init leaves.
+    subroutine init_ring(ring)
+        type(checkpoint_ring), intent(out) :: ring
+        real(c_double), allocatable :: f(:, :, :), f0(:, :, :)
+        allocate(f(nx, ny, 9), f0(nx, ny, 9))
+        call check(lbm_get_populations(h, f, f0), 'lbm_get_populations')
+        call ring_from_populations(nx, ny, f, ring)
+    end subroutine
+
+    ! saveflowfield (solution.f90:1616-1719): the same checkpoint twice.
+    subroutine save_checkpoint(n, ring, brp)
+        integer, intent(in) :: n
+        type(checkpoint_ring), intent(inout) :: ring
+        real(c_double), intent(in) :: brp(:)
+        real(c_double), allocatable :: f(:, :, :), f0(:, :, :), uf(:, :), vf(:, :), rho(:, :), pre(:, :)
+        integer(c_int32_t) :: n_next, movenum, flag2, rl, nadj
+        character(len=32), parameter :: names(2) = [character(len=32) :: 'saved_flow_feild', 'saved_flow_feild_copy']
+        integer :: q
+
+        allocate(f(nx, ny, 9), f0(nx, ny, 9), uf(nx, ny), vf(nx, ny), rho(nx, ny), pre(nx, ny))
+        call check(lbm_get_state(h, n_next, movenum, flag2, rl, nadj), 'lbm_get_state')
+        call check(lbm_get_fields(h, rho, uf, vf, pre), 'lbm_get_fields')
+        call check(lbm_get_populations(h, f, f0), 'lbm_get_populations')
+        ! the f0 ring columns as the library holds them (move_mesh shifts them)
+        ring%f0_left = f0(1, :, :)
+        ring%f0_right = f0(nx, :, :)
+        do q = 1, 2
+            call write_checkpoint(trim(output_dir) // '/' // trim(names(q)) // '.dat', n, wing_count, brp, &
+                                  int(movenum), nx, ny, uf, vf, f, ring, ok)
+            if (.not. ok) call die('cannot write ' // trim(output_dir) // '/' // trim(names(q)) // '.dat')
+        enddo
+    end subroutine
+
+    ! readflowfield (solution.f90:1721-1787): the checkpoint in restart_dir
+    ! (default output_dir), its copy if that fails; the next step is the
+    ! saved step + 1 (main.f90:343-347).
+    subroutine restore_checkpoint(ring, brp, n_next)
+        type(checkpoint_ring), intent(out) :: ring
+        real(c_double), intent(out) :: brp(:)
+        integer, intent(out) :: n_next
+        real(c_double), allocatable :: f(:, :, :), f0(:, :, :), uf(:, :), vf(:, :)
+        character(len=512) :: dir
+        integer :: cycle, movenum, w
+
+        dir = restart_dir
+        if (len_trim(dir) == 0) dir = output_dir
+        allocate(f(nx, ny, 9), f0(nx, ny, 9), uf(nx, ny), vf(nx, ny))
+        brp = 0d0
+        call read_checkpoint(trim(dir) // '/saved_flow_feild.dat', wing_count, nx, ny, cycle, brp, movenum, &
+                             uf, vf, f, ring, text, ok)
+        if (ok) then
+            write(*, '(a)') 'sucess!'
+        else
+            write(*, '(a)') trim(text)
+            write(*, '(a)') '启动备用存档！'
+            call read_checkpoint(trim(dir) // '/saved_flow_feild_copy.dat', wing_count, nx, ny, cycle, brp, &
+                                 movenum, uf, vf, f, ring, text, ok)
+            if (.not. ok) call die(text)
+            write(*, '(a)') '读取保存的流场数据成功!'
+        endif
+        f0 = 0d0
+        f0(1, :, :) = ring%f0_left
+        f0(nx, :, :) = ring%f0_right
+        f0(:, 1, :) = ring%f0_bottom
+        f0(:, ny, :) = ring%f0_top
+        call check(lbm_set_step(h, int(cycle + 1, c_int32_t), int(movenum, c_int32_t)), 'lbm_set_step')
+        do w = 1, wing_count
+            if (brp(w) /= 0d0) call check(lbm_set_body_rel_pos(h, int(w, c_int32_t), brp(w)), 'lbm_set_body_rel_pos')
+        enddo
+        call check(lbm_set_populations(h, f, f0), 'lbm_set_populations')
+        n_next = cycle + 1
     end subroutine
 
     ! ---------------------------------------------------------- online mode

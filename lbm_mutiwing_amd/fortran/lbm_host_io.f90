@@ -12,11 +12,24 @@
 !   F<cycle>.dat            Tecplot: the flow zone sampled every second node,
 !                           then the airfoil FE-triangle zone
 !                           (solution.f90:132-241)
+!   saved_flow_feild.dat    restart checkpoint, unformatted stream (the
+!   (+ _copy)               reference's form='binary'): see write_checkpoint
+!                           (solution.f90:1616-1851)
 module lbm_host_io
     use iso_c_binding
     implicit none
     private
     public :: marker_file_name, read_markers, read_motion_table, write_rows, write_tecplot, make_dir
+    public :: checkpoint_ring, ring_from_populations, write_checkpoint, read_checkpoint, checkpoint_bytes
+
+    ! The boundary records of the checkpoint: the previous-step boundary
+    ! populations leftf2/rightf2 (rows 2..ny-1), upf2/downf2 (all columns),
+    ! which the live step never changes after init (solution.f90:60-63), and
+    ! the f0 ring (main.f90:174, solution.f90:1704-1715).  (ny, 9) and (nx, 9).
+    type :: checkpoint_ring
+        real(c_double), allocatable :: left2(:, :), f0_left(:, :), right2(:, :), f0_right(:, :)
+        real(c_double), allocatable :: up2(:, :), f0_top(:, :), down2(:, :), f0_bottom(:, :)
+    end type
 
     integer, parameter :: STRIDE_X = 2, STRIDE_Y = 2   ! main.f90:38
     integer, parameter :: LINE_BYTES = 8192
@@ -224,6 +237,162 @@ contains
             enddo
         enddo
         close(fu)
+    end subroutine
+
+    ! The ring records as init leaves them: f (nx, ny, 9) just after init, where
+    ! f = f0 = the equilibrium everywhere (solution.f90:55-63).
+    subroutine ring_from_populations(nx, ny, f, ring)
+        integer, intent(in) :: nx, ny
+        real(c_double), intent(in) :: f(nx, ny, 9)
+        type(checkpoint_ring), intent(out) :: ring
+        allocate(ring%left2(ny, 9), ring%f0_left(ny, 9), ring%right2(ny, 9), ring%f0_right(ny, 9))
+        allocate(ring%up2(nx, 9), ring%f0_top(nx, 9), ring%down2(nx, 9), ring%f0_bottom(nx, 9))
+        ring%left2 = f(1, :, :)
+        ring%f0_left = f(1, :, :)
+        ring%right2 = f(nx, :, :)
+        ring%f0_right = f(nx, :, :)
+        ring%up2 = f(:, ny, :)
+        ring%f0_top = f(:, ny, :)
+        ring%down2 = f(:, 1, :)
+        ring%f0_bottom = f(:, 1, :)
+    end subroutine
+
+    integer(c_int64_t) function checkpoint_bytes(wing_count, nx, ny)
+        integer, intent(in) :: wing_count, nx, ny
+        checkpoint_bytes = 12_c_int64_t + 88_c_int64_t * wing_count + 88_c_int64_t * nx * ny + &
+                           288_c_int64_t * (ny - 2) + 288_c_int64_t * nx
+    end function
+
+    ! Checkpoint as saveflowfield writes it (solution.f90:1660-1686): the
+    ! step, the wing count, the eleven rigid-body arrays of wing_count
+    ! doubles (body_rel_a, _a_prev, _v, _v_prev, _s, _s_prev, _pos,
+    ! _theta_prev, _theta, _temp_theta_prev, _temp_theta; only body_rel_pos
+    ! is state here, the rest of the rigid-body solver is disabled), movenum,
+    ! then u, v, f(1:9) node by node (i fastest), then per row j = 2..ny-1
+    ! leftf2, f0(1,j), rightf2, f0(nx,j), then per column i = 1..nx upf2,
+    ! f0(i,ny), downf2, f0(i,1).  Integers are 4 bytes, reals 8, no record
+    ! markers.
+    subroutine write_checkpoint(path, cycle, wing_count, body_rel_pos, movenum, nx, ny, u, v, f, ring, ok)
+        character(len=*), intent(in) :: path
+        integer, intent(in) :: cycle, wing_count, movenum, nx, ny
+        real(c_double), intent(in) :: body_rel_pos(*), u(nx, ny), v(nx, ny), f(nx, ny, 9)
+        type(checkpoint_ring), intent(in) :: ring
+        logical, intent(out) :: ok
+        real(c_double), allocatable :: rel(:, :), buf(:, :)
+        integer :: fu, ios, i, j
+
+        ok = .false.
+        open(newunit=fu, file=trim(path), access='stream', form='unformatted', status='replace', &
+             action='write', iostat=ios)
+        if (ios /= 0) return
+        allocate(rel(max(1, wing_count), 11), buf(11, nx))
+        rel = 0d0
+        if (wing_count > 0) rel(1:wing_count, 7) = body_rel_pos(1:wing_count)
+        write(fu, iostat=ios) int(cycle, c_int32_t), int(wing_count, c_int32_t)
+        if (wing_count > 0 .and. ios == 0) write(fu, iostat=ios) rel(1:wing_count, :)
+        if (ios == 0) write(fu, iostat=ios) int(movenum, c_int32_t)
+        do j = 1, ny
+            if (ios /= 0) exit
+            do i = 1, nx
+                buf(1, i) = u(i, j)
+                buf(2, i) = v(i, j)
+                buf(3:11, i) = f(i, j, :)
+            enddo
+            write(fu, iostat=ios) buf
+        enddo
+        do j = 2, ny - 1
+            if (ios /= 0) exit
+            write(fu, iostat=ios) ring%left2(j, :), ring%f0_left(j, :), ring%right2(j, :), ring%f0_right(j, :)
+        enddo
+        do i = 1, nx
+            if (ios /= 0) exit
+            write(fu, iostat=ios) ring%up2(i, :), ring%f0_top(i, :), ring%down2(i, :), ring%f0_bottom(i, :)
+        enddo
+        close(fu)
+        ok = ios == 0
+    end subroutine
+
+    ! readflowfield (solution.f90:1721-1787): the checkpoint of a case of
+    ! wing_count wings on an nx x ny lattice.  A missing file, a file of the
+    ! wrong size or another wing count fails with a message.
+    subroutine read_checkpoint(path, wing_count, nx, ny, cycle, body_rel_pos, movenum, u, v, f, ring, msg, ok)
+        character(len=*), intent(in) :: path
+        integer, intent(in) :: wing_count, nx, ny
+        integer, intent(out) :: cycle, movenum
+        real(c_double), intent(out) :: body_rel_pos(*), u(nx, ny), v(nx, ny), f(nx, ny, 9)
+        type(checkpoint_ring), intent(out) :: ring
+        character(len=*), intent(out) :: msg
+        logical, intent(out) :: ok
+        real(c_double), allocatable :: rel(:, :), buf(:, :)
+        integer(c_int32_t) :: i4, w4, m4
+        integer(c_int64_t) :: fsize
+        logical :: exists
+        integer :: fu, ios, i, j
+
+        ok = .false.
+        msg = ''
+        inquire(file=trim(path), exist=exists, size=fsize)
+        if (.not. exists) then
+            msg = '无法打开续算存档文件: ' // trim(path)
+            return
+        endif
+        if (fsize /= checkpoint_bytes(wing_count, nx, ny)) then
+            msg = '续算存档文件大小与网格不一致: ' // trim(path)
+            return
+        endif
+        open(newunit=fu, file=trim(path), access='stream', form='unformatted', status='old', &
+             action='read', iostat=ios)
+        if (ios /= 0) then
+            msg = '无法打开续算存档文件: ' // trim(path)
+            return
+        endif
+        read(fu, iostat=ios) i4, w4
+        if (ios == 0 .and. w4 /= wing_count) then
+            close(fu)
+            msg = '续算文件中的翼数量与当前编译设置不一致'
+            return
+        endif
+        allocate(rel(max(1, wing_count), 11), buf(11, nx))
+        rel = 0d0
+        if (ios == 0 .and. wing_count > 0) read(fu, iostat=ios) rel(1:wing_count, :)
+        if (ios == 0) read(fu, iostat=ios) m4
+        do j = 1, ny
+            if (ios /= 0) exit
+            read(fu, iostat=ios) buf
+            do i = 1, nx
+                u(i, j) = buf(1, i)
+                v(i, j) = buf(2, i)
+                f(i, j, :) = buf(3:11, i)
+            enddo
+        enddo
+        allocate(ring%left2(ny, 9), ring%f0_left(ny, 9), ring%right2(ny, 9), ring%f0_right(ny, 9))
+        allocate(ring%up2(nx, 9), ring%f0_top(nx, 9), ring%down2(nx, 9), ring%f0_bottom(nx, 9))
+        ring%left2 = 0d0
+        ring%f0_left = 0d0
+        ring%right2 = 0d0
+        ring%f0_right = 0d0
+        do j = 2, ny - 1
+            if (ios /= 0) exit
+            read(fu, iostat=ios) ring%left2(j, :), ring%f0_left(j, :), ring%right2(j, :), ring%f0_right(j, :)
+        enddo
+        do i = 1, nx
+            if (ios /= 0) exit
+            read(fu, iostat=ios) ring%up2(i, :), ring%f0_top(i, :), ring%down2(i, :), ring%f0_bottom(i, :)
+        enddo
+        close(fu)
+        if (ios /= 0) then
+            msg = '续算存档文件读取失败: ' // trim(path)
+            return
+        endif
+        ! the corners of the f0 ring are the end columns' rows 1 and ny
+        ring%f0_left(1, :) = ring%f0_bottom(1, :)
+        ring%f0_left(ny, :) = ring%f0_top(1, :)
+        ring%f0_right(1, :) = ring%f0_bottom(nx, :)
+        ring%f0_right(ny, :) = ring%f0_top(nx, :)
+        cycle = i4
+        movenum = m4
+        if (wing_count > 0) body_rel_pos(1:wing_count) = rel(1:wing_count, 7)
+        ok = .true.
     end subroutine
 
 end module lbm_host_io

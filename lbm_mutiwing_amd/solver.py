@@ -38,6 +38,7 @@ _PROTOS = {
     "get_markers": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _dp]),
     "get_ibm": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _ip, _ip, _dp, _dp]),
     "set_body_rel_pos": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_double]),
+    "set_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
     "last_error": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]),
 }
 
@@ -206,6 +207,12 @@ class Solver:
         self._check(self._fn("get_ibm")(self._h, n, ij.ctypes.data_as(_ip), wid.ctypes.data_as(_ip),
                                         _p(wf), _p(wm)))
         return ij[: 2 * n].reshape(n, 2), wid[:n], wf.reshape(8, 2), wm
+
+    def set_step(self, n_next: int, movenum: int = 0) -> None:
+        """Restart: the next step is ``n_next``, the frame shift ``movenum``
+        (readflowfield, solution.f90:1721-1851).  Call before
+        :meth:`set_populations`."""
+        self._check(self._fn("set_step")(self._h, int(n_next), int(movenum)))
 
     def set_body_rel_pos(self, wing: int, pos: float) -> None:
         self._check(self._fn("set_body_rel_pos")(self._h, int(wing), float(pos)))

@@ -1304,6 +1304,18 @@ int orc_get_ibm_system(orc_t *o, double *a, double *rhs, double *sol)
     return 0;
 }
 
+/* Restart hook (readflowfield, solution.f90:1721-1851, main.f90:343-349):
+ * next step, frame shift, flag2 = 1. */
+int orc_set_step(orc_t *o, int32_t n_next, int32_t movenum)
+{
+    if (!o || n_next < 1) return LBM_E_INVALID;
+    o->n = n_next;
+    o->movenum = movenum;
+    o->flag2 = 1;
+    update_body_extra(o);
+    return 0;
+}
+
 int orc_set_body_rel_pos(orc_t *o, int32_t wing, double pos)
 {
     if (!o || wing < 1 || wing > o->W) return LBM_E_INVALID;

@@ -42,6 +42,7 @@ int orc_get_markers(orc_t *o, double *xb1, double *yb1, double *xb2,
 int orc_get_ibm(orc_t *o, int32_t capacity, int32_t *adjoint_ij,
                 int32_t *wing_id, double *wing_force, double *wing_moment);
 int orc_set_body_rel_pos(orc_t *o, int32_t wing, double pos);
+int orc_set_step(orc_t *o, int32_t n_next, int32_t movenum);
 int orc_last_error(const orc_t *o, char *buf, size_t n);
 
 /* Test hooks exposing single reference routines. */

@@ -107,3 +107,78 @@ def test_host_links_library_and_zmq(host):
     out = subprocess.run(["ldd", host], capture_output=True, timeout=60).stdout.decode()
     assert "liblbm_hip.so" in out and "libzmq" in out
     assert "not found" not in out
+
+
+# ------------------------------------------------------------ checkpoint I/O
+
+@pytest.fixture(scope="module")
+def checkpoint_check(tmp_path_factory, hip_lib_path):
+    out = str(tmp_path_factory.mktemp("fortran") / "checkpoint_check")
+    return build.fortran_program(os.path.join(HERE, "fortran", "checkpoint_check.f90"), out)
+
+
+def test_checkpoint_layout_of_fortran_writer(checkpoint_check, tmp_path):
+    """The Fortran writer's bytes, read back by the Python reader, hold the
+    driver's known values at the places saveflowfield writes them
+    (solution.f90:1660-1686)."""
+    from lbm_mutiwing_amd import formats
+    nx, ny, W = 7, 5, 2
+    path = tmp_path / "ck.dat"
+    r = subprocess.run([checkpoint_check, "write", str(path), str(nx), str(ny), str(W), "300", "-2"],
+                       capture_output=True, timeout=60)
+    assert r.returncode == 0, r.stdout.decode() + r.stderr.decode()
+    assert path.stat().st_size == formats.checkpoint_bytes(W, nx, ny)
+    ck = formats.read_checkpoint(str(path), W, nx, ny)
+    assert (ck.cycle, ck.movenum) == (300, -2)
+    expect_rel = np.zeros((11, W))
+    expect_rel[6] = [0.5, 1.0]
+    assert np.array_equal(ck.body_rel, expect_rel)
+    i = np.arange(1, nx + 1)[None, :]
+    j = np.arange(1, ny + 1)[:, None]
+    assert np.array_equal(ck.u, i + j / 1024.0)
+    assert np.array_equal(ck.v, -i - j / 2048.0)
+    k = np.arange(1, 10)[:, None, None]
+    assert np.array_equal(ck.f, k + i[None] / 1024.0 + j[None] / 1048576.0)
+    kk = np.arange(1, 10)[:, None]
+    for q, name in enumerate(("left2", "f0_left", "right2", "f0_right")):
+        want = 16.0 * (q + 1) + kk + np.arange(1, ny + 1)[None, :] / 1024.0
+        assert np.array_equal(ck.ring[name][:, 1:ny - 1], want[:, 1:ny - 1]), name
+    for q, name in enumerate(("up2", "f0_top", "down2", "f0_bottom")):
+        want = 16.0 * (q + 5) + kk + np.arange(1, nx + 1)[None, :] / 1024.0
+        assert np.array_equal(ck.ring[name], want), name
+
+
+def test_checkpoint_python_fortran_round_trip(checkpoint_check, tmp_path):
+    """A checkpoint written by the Python mirror, read and written again by
+    the Fortran host's routines, comes back byte for byte."""
+    from lbm_mutiwing_amd import formats
+    rng = np.random.default_rng(11)
+    nx, ny, W = 9, 6, 3
+    f = rng.normal(size=(9, ny, nx))
+    ring = formats.ring_from_populations(rng.normal(size=(9, ny, nx)))
+    ck = formats.Checkpoint(cycle=12345, movenum=3, body_rel=np.zeros((11, W)), u=rng.normal(size=(ny, nx)),
+                            v=rng.normal(size=(ny, nx)), f=f, ring=ring)
+    ck.body_rel[6] = [0.25, -0.5, 0.0]
+    a, b = tmp_path / "a.dat", tmp_path / "b.dat"
+    formats.write_checkpoint(str(a), ck)
+    r = subprocess.run([checkpoint_check, "copy", str(a), str(b), str(nx), str(ny), str(W)],
+                       capture_output=True, timeout=60)
+    assert r.returncode == 0, r.stdout.decode()
+    assert a.read_bytes() == b.read_bytes()
+    back = formats.read_checkpoint(str(b), W, nx, ny)
+    assert np.array_equal(back.f, f) and back.cycle == 12345 and back.movenum == 3
+
+
+def test_checkpoint_reader_rejects_mismatch(checkpoint_check, tmp_path):
+    path = tmp_path / "ck.dat"
+    subprocess.run([checkpoint_check, "write", str(path), "7", "5", "2", "1", "0"], check=True, timeout=60)
+    r = subprocess.run([checkpoint_check, "copy", str(path), str(tmp_path / "o.dat"), "7", "6", "2"],
+                       capture_output=True, timeout=60)
+    assert r.returncode == 1 and "续算存档文件大小与网格不一致".encode() in r.stdout
+    # the right size, but the wing-count field says 3
+    data = bytearray(path.read_bytes())
+    data[4:8] = (3).to_bytes(4, "little")
+    path.write_bytes(bytes(data))
+    r = subprocess.run([checkpoint_check, "copy", str(path), str(tmp_path / "o.dat"), "7", "5", "2"],
+                       capture_output=True, timeout=60)
+    assert r.returncode == 1 and "续算文件中的翼数量与当前编译设置不一致".encode() in r.stdout

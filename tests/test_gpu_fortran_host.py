@@ -193,3 +193,57 @@ def test_online_mode_session(host, oracle_mod, tmp_path):
     assert host_file.read_text() == py_file.read_text()
     for ep in (1, 2):
         assert (tmp_path / "out" / f"force_episode_{ep}.dat").exists()
+
+
+def test_file_mode_checkpoint_and_restart(host, tmp_path):
+    """saveflowfield / readflowfield (solution.f90:1616-1851, main.f90:343-347,
+    436-439): the checkpoint written at step 100 holds the state the library
+    has after 100 steps, and a restarted run writes for steps 101..150 the
+    same force and motion records as the uninterrupted run; with the main
+    checkpoint damaged the restart falls back to the copy."""
+    cfg = preset("small")
+    n = 150
+    rows = sinusoidal_motion_table(cfg, seeded_motions(cfg, 23), n)
+    _write_case(tmp_path, cfg, rows)
+    (tmp_path / "full.nml").write_text(_namelist(cfg, out_dir="full", extra=", checkpoint_every=100"))
+    print("fortran host: file mode with a checkpoint at step 100", flush=True)
+    r = subprocess.run([host, "full.nml"], cwd=tmp_path, capture_output=True, timeout=300)
+    assert r.returncode == 0, r.stdout.decode() + r.stderr.decode()
+    ckp = tmp_path / "full" / "saved_flow_feild.dat"
+    assert ckp.read_bytes() == (tmp_path / "full" / "saved_flow_feild_copy.dat").read_bytes()
+
+    # the checkpoint against the library's own state after 100 steps
+    cfg = _read_case(tmp_path, cfg)
+    table = formats.load_motion_params(str(tmp_path / "motion_params.dat"), cfg.wing_count)
+    g = Solver(cfg)
+    g.set_motion_table(table)
+    f_init, _ = g.populations()
+    g.advance(100)
+    ck = formats.read_checkpoint(str(ckp), cfg.wing_count, cfg.nx, cfg.ny)
+    _, u, v, _ = g.fields()
+    f, f0 = g.populations()
+    assert (ck.cycle, ck.movenum) == (100, 0)
+    assert np.array_equal(ck.u, u) and np.array_equal(ck.v, v) and np.array_equal(ck.f, f)
+    ring0 = formats.ring_from_populations(f_init)
+    for name in ("left2", "right2"):
+        assert np.array_equal(ck.ring[name][:, 1:-1], ring0[name][:, 1:-1]), name
+    for name in ("up2", "down2", "f0_top", "f0_bottom"):
+        assert np.array_equal(ck.ring[name], ring0[name]), name
+    assert np.array_equal(ck.ring["f0_left"][:, 1:-1], f0[:, 1:-1, 0])
+    assert np.array_equal(ck.ring["f0_right"][:, 1:-1], f0[:, 1:-1, -1])
+
+    full_f = (tmp_path / "full" / "force.dat").read_text().splitlines()
+    full_m = (tmp_path / "full" / "airfoil_motion.dat").read_text().splitlines()
+    assert len(full_f) == n
+    for name, damage in (("rs", False), ("rs_copy", True)):
+        if damage:  # a truncated main checkpoint: the copy is read instead
+            ckp.write_bytes(ckp.read_bytes()[:1000])
+        (tmp_path / f"{name}.nml").write_text(
+            _namelist(cfg, out_dir=name, extra=", checkpoint_every=100, restart=.true., restart_dir='full'"))
+        print(f"fortran host: restart {name}", flush=True)
+        r = subprocess.run([host, f"{name}.nml"], cwd=tmp_path, capture_output=True, timeout=300)
+        out = r.stdout.decode()
+        assert r.returncode == 0, out + r.stderr.decode()
+        assert ("启动备用存档" in out) == damage, out
+        assert (tmp_path / name / "force.dat").read_text().splitlines() == full_f[100:], name
+        assert (tmp_path / name / "airfoil_motion.dat").read_text().splitlines() == full_m[100:], name
