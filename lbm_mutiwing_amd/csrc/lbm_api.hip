@@ -756,6 +756,12 @@ int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motio
   rc = issue_geometry(0);
   if (rc) return rc;
   int set = 0, slot = 0, prev_set = 0, prev_slot = 0;
+  // step_ev completes when the lattice stream has finished the previous
+  // step: the call's start here, then the stop event bound to each lattice
+  // launch (no event packet between two lattice kernels), or a recorded
+  // event after a move_mesh
+  HIPCHECK(h, hipEventRecord(h->ev_step, h->stream));
+  hipEvent_t step_ev = h->ev_step;
   for (int m = n0; m <= n_last; m++) {
     const int k = m - n0;
     if (h->W > 0 && k % ns == 0) {
@@ -781,9 +787,8 @@ int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motio
     // lattice kernel waits for the other two through ev_ibm.  The previous
     // step's force row follows on the IBM stream; a geometry slot set is
     // released once the force row of its last step is written.
-    HIPCHECK(h, hipEventRecord(h->ev_step, h->stream));
-    HIPCHECK(h, hipStreamWaitEvent(h->stream_ibm, h->ev_step, 0));
-    HIPCHECK(h, hipStreamWaitEvent(h->stream_bc, h->ev_step, 0));
+    HIPCHECK(h, hipStreamWaitEvent(h->stream_ibm, step_ev, 0));
+    HIPCHECK(h, hipStreamWaitEvent(h->stream_bc, step_ev, 0));
     launch_bc_ring(h->stream_bc, h->g, h->mrt, h->R, src, dst, fs, h->f32);
     HIPCHECK(h, hipEventRecord(h->ev_bc, h->stream_bc));
     // the previous step's force row (its partial sums are kept by step parity)
@@ -799,9 +804,10 @@ int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motio
     launch_adj_collide(h->stream_ibm, h->g, h->mrt, h->ip, h->G, slot, h->F, m, src, dst, fs, h->f32);
     HIPCHECK(h, hipStreamWaitEvent(h->stream_ibm, h->ev_bc, 0));
     HIPCHECK(h, hipEventRecord(h->ev_ibm, h->stream_ibm));
-    if (h->timing) HIPCHECK(h, hipEventRecord(h->ev[2 * k], h->stream));
-    launch_lattice(h->stream, h->g, h->mrt, src, dst, fs, h->f32, h->lat_bx, adj_mask(h->ip, h->G, slot));
-    if (h->timing) HIPCHECK(h, hipEventRecord(h->ev[2 * k + 1], h->stream));
+    step_ev = h->timing ? h->ev[2 * k + 1] : h->ev_step;
+    launch_lattice(h->stream, h->g, h->mrt, src, dst, fs, h->f32, h->lat_bx, adj_mask(h->ip, h->G, slot),
+                   h->timing ? h->ev[2 * k] : nullptr, step_ev);
+    HIPCHECK(h, hipGetLastError());
     HIPCHECK(h, hipStreamWaitEvent(h->stream, h->ev_ibm, 0));
     prev_slot = slot;
     if (want_fields) h->slot_step[s] = m;
@@ -822,6 +828,8 @@ int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motio
       update_body_extra(h);
       h->flag2 = 3;
       h->last_move_step = m;
+      HIPCHECK(h, hipEventRecord(h->ev_step, h->stream));  // the next step reads the shifted lattice
+      step_ev = h->ev_step;
     }
     if (h->flag2 > 0) h->flag2 = h->flag2 - 1;
   }

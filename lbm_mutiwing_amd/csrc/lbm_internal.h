@@ -192,8 +192,10 @@ __host__ __device__ __forceinline__ int pos_of_seg(int k, int n) {
 // bx: lattice tile width (one of kLatticeTiles)
 constexpr int kLatticeTiles[3] = {128, 256, 512};
 // mask: adjoint nodes of the step, skipped (written by k_adj_collide)
+// t0 / t1 (optional): events bound to the launch itself, completing at its
+// start and at its end (no separate event packets in the stream)
 void launch_lattice(hipStream_t s, const Grid& g, const MrtParams& P, const void* src, void* dst, FieldSlot* fields,
-                    bool f32, int bx, const AdjMask& mask);
+                    bool f32, int bx, const AdjMask& mask, hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr);
 int lattice_tiles_x(const Grid& g, int bx);
 void launch_adj_collide(hipStream_t s, const Grid& g, const MrtParams& P, const IbmParams& p, const GeoBuffers& G,
                         int slot, const FlowBuffers& F, int n, const void* src, void* dst, FieldSlot* fields,

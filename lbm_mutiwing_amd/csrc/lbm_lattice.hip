@@ -15,6 +15,8 @@
 // adjoint nodes and all reductions compute in FP64 on loaded f = w_k + d.
 #include <algorithm>
 
+#include <hip/hip_ext.h>
+
 #include "lbm_internal.h"
 
 namespace lbm {
@@ -138,7 +140,7 @@ int lattice_tiles_x(const Grid& g, int bx) { return (g.nx + bx - 1) / bx; }
 
 template <int BX>
 void launch_lattice_bx(hipStream_t s, const Grid& g, const MrtParams& P, const void* src, void* dst,
-                       const FieldSlot* fields, bool f32, const AdjMask& am) {
+                       const FieldSlot* fields, bool f32, const AdjMask& am, hipEvent_t t0, hipEvent_t t1) {
   const int tiles_x = lattice_tiles_x(g, BX);
   dim3 grid(tiles_x * (g.ny - 2)), block(BX);
   const FieldSlot fs = fields ? *fields : FieldSlot{};
@@ -146,27 +148,27 @@ void launch_lattice_bx(hipStream_t s, const Grid& g, const MrtParams& P, const v
     const float* a = (const float*)src;
     float* b = (float*)dst;
     if (fields)
-      hipLaunchKernelGGL((k_lattice_f32<true, BX>), grid, block, 0, s, g, P, a, b, fs, tiles_x, am);
+      hipExtLaunchKernelGGL((k_lattice_f32<true, BX>), grid, block, 0, s, t0, t1, 0, g, P, a, b, fs, tiles_x, am);
     else
-      hipLaunchKernelGGL((k_lattice_f32<false, BX>), grid, block, 0, s, g, P, a, b, fs, tiles_x, am);
+      hipExtLaunchKernelGGL((k_lattice_f32<false, BX>), grid, block, 0, s, t0, t1, 0, g, P, a, b, fs, tiles_x, am);
   } else {
     const double* a = (const double*)src;
     double* b = (double*)dst;
     if (fields)
-      hipLaunchKernelGGL((k_lattice<true, BX>), grid, block, 0, s, g, P, a, b, fs, tiles_x, am);
+      hipExtLaunchKernelGGL((k_lattice<true, BX>), grid, block, 0, s, t0, t1, 0, g, P, a, b, fs, tiles_x, am);
     else
-      hipLaunchKernelGGL((k_lattice<false, BX>), grid, block, 0, s, g, P, a, b, fs, tiles_x, am);
+      hipExtLaunchKernelGGL((k_lattice<false, BX>), grid, block, 0, s, t0, t1, 0, g, P, a, b, fs, tiles_x, am);
   }
 }
 
 void launch_lattice(hipStream_t s, const Grid& g, const MrtParams& P, const void* src, void* dst, FieldSlot* fields,
-                    bool f32, int bx, const AdjMask& mask) {
+                    bool f32, int bx, const AdjMask& mask, hipEvent_t t0, hipEvent_t t1) {
   if (bx == 128)
-    launch_lattice_bx<128>(s, g, P, src, dst, fields, f32, mask);
+    launch_lattice_bx<128>(s, g, P, src, dst, fields, f32, mask, t0, t1);
   else if (bx == 512)
-    launch_lattice_bx<512>(s, g, P, src, dst, fields, f32, mask);
+    launch_lattice_bx<512>(s, g, P, src, dst, fields, f32, mask, t0, t1);
   else
-    launch_lattice_bx<256>(s, g, P, src, dst, fields, f32, mask);
+    launch_lattice_bx<256>(s, g, P, src, dst, fields, f32, mask, t0, t1);
 }
 
 // Re-collide the adjoint nodes with the IBM-corrected velocity u = u* + g and
