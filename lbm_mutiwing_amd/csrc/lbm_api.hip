@@ -1392,7 +1392,9 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
     BCHECK(b, hipEventRecord(b->ev_geo[es], b->stream_geo));
     return 0;
   };
-  auto issue_steps = [&](int k_begin, int k_end, int kb0) -> int {
+  // bind: the lattice launches carry the step event as their own completion
+  // event (not inside a graph capture, where steps record it explicitly)
+  auto issue_steps = [&](int k_begin, int k_end, int kb0, bool bind) -> int {
     const int seg_bb = k_begin / pl;
     BCHECK(b, hipEventRecord(b->ev_start, b->stream));
     if (k_begin == 0) {
@@ -1410,7 +1412,8 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
       // one writer per node, as in lbm_advance: lattice (all but the adjoint
       // nodes), IBM stream (adjoint nodes), ring stream (boundary ring)
       const bool fields = k >= steps - 3;
-      BCHECK(b, hipEventRecord(b->ev_step, b->stream));
+      // ev_step: the lattice stream has finished the previous step
+      if (!bind || k == k_begin) BCHECK(b, hipEventRecord(b->ev_step, b->stream));
       BCHECK(b, hipStreamWaitEvent(b->stream_ibm, b->ev_step, 0));
       BCHECK(b, hipStreamWaitEvent(b->stream_bc, b->ev_step, 0));
       launch_batch_bc_ring(b->stream_bc, b->d_desc, it, kk, f32, fields);
@@ -1427,7 +1430,8 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
       launch_batch_adj_collide(b->stream_ibm, b->d_desc, it, kk, f32, fields);
       BCHECK(b, hipStreamWaitEvent(b->stream_ibm, b->ev_bc, 0));
       BCHECK(b, hipEventRecord(b->ev_ibm, b->stream_ibm));
-      launch_batch_lattice(b->stream, b->d_desc, it, kk, f32, fields, b->lat_bx, true);
+      launch_batch_lattice(b->stream, b->d_desc, it, kk, f32, fields, b->lat_bx, true, bind ? b->ev_step : nullptr);
+      BCHECK(b, hipGetLastError());
       BCHECK(b, hipStreamWaitEvent(b->stream, b->ev_ibm, 0));
     }
     // the geometry of the batch after the segment was issued inside it
@@ -1500,13 +1504,13 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
   if (J >= 1) {
     const int kc = (J + 1) * pl;
     launch_step_base(b->stream, b->d_kbase, 0, false);
-    int rc = issue_steps(0, pl, 0);
+    int rc = issue_steps(0, pl, 0, true);
     if (rc) return rc;
     launch_step_base(b->stream, b->d_kbase, pl, true);
     BatchGraph* gr = find_graph(-pl);
     if (!gr->exec) {
       rc = capture(gr, [&]() -> int {
-        const int r = issue_steps(pl, 2 * pl, pl);
+        const int r = issue_steps(pl, 2 * pl, pl, false);
         if (!r) launch_step_base(b->stream, b->d_kbase, pl, true);
         return r;
       });
@@ -1514,23 +1518,23 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
     }
     for (int j = 1; j <= J; j++) BCHECK(b, hipGraphLaunch(gr->exec, b->stream));
     gr->seen += J;
-    rc = issue_steps(kc, steps, kc);
+    rc = issue_steps(kc, steps, kc, true);
     if (rc) return rc;
   } else {
-    auto whole = [&]() -> int {
+    auto whole = [&](bool bind) -> int {
       launch_step_base(b->stream, b->d_kbase, 0, false);
-      return issue_steps(0, steps, 0);
+      return issue_steps(0, steps, 0, bind);
     };
     BatchGraph* gr = nullptr;
     if (b->graph_mode >= 1 && steps <= kMaxGraphSteps && (!wings || steps <= ns)) gr = find_graph(steps);
     if (gr && !gr->exec && gr->seen >= 1) {
-      const int rc = capture(gr, whole);
+      const int rc = capture(gr, [&]() { return whole(false); });
       if (rc) return rc;
     }
     if (gr && gr->exec) {
       BCHECK(b, hipGraphLaunch(gr->exec, b->stream));
     } else {
-      const int rc = whole();
+      const int rc = whole(true);
       if (rc) return rc;
       if (gr) gr->seen++;
     }

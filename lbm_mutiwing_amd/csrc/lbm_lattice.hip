@@ -750,26 +750,34 @@ void launch_step_base(hipStream_t s, int* p, int v, bool add) {
 
 template <int BX>
 void launch_batch_lattice_bx(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields,
-                             int masked) {
+                             int masked, hipEvent_t t1) {
   dim3 grid(it.n_lattice), block(BX);
   if (f32) {
-    if (fields) hipLaunchKernelGGL((kb_lattice<true, true, BX>), grid, block, 0, s, eps, it.lattice, it.kbase, k, masked);
-    else hipLaunchKernelGGL((kb_lattice<true, false, BX>), grid, block, 0, s, eps, it.lattice, it.kbase, k, masked);
+    if (fields)
+      hipExtLaunchKernelGGL((kb_lattice<true, true, BX>), grid, block, 0, s, nullptr, t1, 0, eps, it.lattice, it.kbase,
+                            k, masked);
+    else
+      hipExtLaunchKernelGGL((kb_lattice<true, false, BX>), grid, block, 0, s, nullptr, t1, 0, eps, it.lattice,
+                            it.kbase, k, masked);
   } else {
-    if (fields) hipLaunchKernelGGL((kb_lattice<false, true, BX>), grid, block, 0, s, eps, it.lattice, it.kbase, k, masked);
-    else hipLaunchKernelGGL((kb_lattice<false, false, BX>), grid, block, 0, s, eps, it.lattice, it.kbase, k, masked);
+    if (fields)
+      hipExtLaunchKernelGGL((kb_lattice<false, true, BX>), grid, block, 0, s, nullptr, t1, 0, eps, it.lattice,
+                            it.kbase, k, masked);
+    else
+      hipExtLaunchKernelGGL((kb_lattice<false, false, BX>), grid, block, 0, s, nullptr, t1, 0, eps, it.lattice,
+                            it.kbase, k, masked);
   }
 }
 
 void launch_batch_lattice(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields,
-                          int bx, bool masked) {
+                          int bx, bool masked, hipEvent_t t1) {
   if (it.n_lattice <= 0) return;
   if (bx == 128)
-    launch_batch_lattice_bx<128>(s, eps, it, k, f32, fields, masked ? 1 : 0);
+    launch_batch_lattice_bx<128>(s, eps, it, k, f32, fields, masked ? 1 : 0, t1);
   else if (bx == 512)
-    launch_batch_lattice_bx<512>(s, eps, it, k, f32, fields, masked ? 1 : 0);
+    launch_batch_lattice_bx<512>(s, eps, it, k, f32, fields, masked ? 1 : 0, t1);
   else
-    launch_batch_lattice_bx<256>(s, eps, it, k, f32, fields, masked ? 1 : 0);
+    launch_batch_lattice_bx<256>(s, eps, it, k, f32, fields, masked ? 1 : 0, t1);
 }
 
 void launch_batch_bc_ring(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields) {
