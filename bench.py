@@ -136,7 +136,7 @@ def run_batch(args):
     t0 = time.perf_counter()
     b.advance(solvers, K)
     t_batch = time.perf_counter() - t0
-    tile = b.tuning()
+    tile, rows = b.tuning_full()
     nodes = sum(c.nx * c.ny for c in cfgs)
     for s in solvers:
         s.close()
@@ -168,7 +168,7 @@ def run_batch(args):
                                f"chord 32-128, Re 100-500", "episodes": n_ep, "seed": args.seed,
                    "total_nodes": nodes},
         "aggregate_mlups": round(nodes * K / t_batch / 1e6, 1),
-        "lattice_tile": tile, "timed": "second STEP of the same episodes",
+        "lattice_tile": tile, "lattice_tiles_per_workgroup": rows, "timed": "second STEP of the same episodes",
         "one_at_a_time": {"fp64_env_steps_per_s": round(base64, 3), "fp32_env_steps_per_s": round(base32, 3),
                           "sample": f"episodes 0..{len(sample) - 1}, {ksample} steps each, scaled to {K}"},
         "speedup_vs_fp64_one_at_a_time": round(value / base64, 3),

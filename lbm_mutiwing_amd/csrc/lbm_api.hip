@@ -292,28 +292,23 @@ int ensure_frames(lbm_handle* h, size_t steps) {
   return 0;
 }
 
-// Lattice-kernel autotuner: every tile width of kLatticeTiles is timed on
-// launch() in three interleaved rounds (per round and width: one warm-up
-// launch, then three timed launches between HIP events on stream s); each
-// width keeps its fastest round, and the fastest width is returned, 256 on a
-// tie within 1 %.  LBM_LATTICE_TILE=<width> skips the timing.
+// Lattice-kernel autotuner: every candidate launch configuration (cands[0]
+// is the default) is timed on launch() in three interleaved rounds (per
+// round and candidate: one warm-up launch, then three timed launches between
+// HIP events on stream s); each candidate keeps its fastest round, and the
+// fastest candidate is returned, the default unless another is more than
+// 1 % faster.
 template <typename F>
-int autotune_tile(hipStream_t s, F launch) {
-  if (const char* e = getenv("LBM_LATTICE_TILE")) {
-    const int v = atoi(e);
-    for (int bx : kLatticeTiles)
-      if (bx == v) return v;
-  }
+int autotune(hipStream_t s, const std::vector<int>& cands, F launch) {
+  if (cands.size() == 1) return cands[0];
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return 256;
-  constexpr int nt = sizeof(kLatticeTiles) / sizeof(kLatticeTiles[0]);
-  float best_ms[nt];
-  for (int q = 0; q < nt; q++) best_ms[q] = 1e30f;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return cands[0];
+  std::vector<float> best_ms(cands.size(), 1e30f);
   for (int round = 0; round < 3; round++)
-    for (int q = 0; q < nt; q++) {
-      launch(kLatticeTiles[q]);
+    for (size_t q = 0; q < cands.size(); q++) {
+      launch(cands[q]);
       (void)hipEventRecord(e0, s);
-      for (int r = 0; r < 3; r++) launch(kLatticeTiles[q]);
+      for (int r = 0; r < 3; r++) launch(cands[q]);
       (void)hipEventRecord(e1, s);
       (void)hipEventSynchronize(e1);
       float ms = 0.f;
@@ -321,17 +316,25 @@ int autotune_tile(hipStream_t s, F launch) {
     }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  int best = 256;
-  float ref = 1e30f;
-  for (int q = 0; q < nt; q++)
-    if (kLatticeTiles[q] == 256) ref = best_ms[q];
-  float best_t = ref;
-  for (int q = 0; q < nt; q++)
-    if (best_ms[q] < 0.99f * ref && best_ms[q] < best_t) {
+  int best = cands[0];
+  float best_t = best_ms[0];
+  for (size_t q = 1; q < cands.size(); q++)
+    if (best_ms[q] < 0.99f * best_ms[0] && best_ms[q] < best_t) {
       best_t = best_ms[q];
-      best = kLatticeTiles[q];
+      best = cands[q];
     }
   return best;
+}
+
+// LBM_LATTICE_TILE=<width> in the environment fixes the tile width (one of
+// kLatticeTiles), 0 otherwise.
+int env_tile() {
+  if (const char* e = getenv("LBM_LATTICE_TILE")) {
+    const int v = atoi(e);
+    for (int bx : kLatticeTiles)
+      if (bx == v) return v;
+  }
+  return 0;
 }
 
 // Tile width of one instance, timed on its own buffers: the launches write
@@ -342,8 +345,10 @@ void tune_instance(lbm_handle* h) {
   const void* src = h->fbuf[h->cur];
   void* dst = h->fbuf[1 - h->cur];
   const AdjMask none{nullptr, nullptr, 0, 0};
-  h->lat_bx = autotune_tile(h->stream, [&](int bx) { launch_lattice(h->stream, h->g, h->mrt, src, dst, nullptr,
-                                                                     h->f32, bx, none); });
+  const int fixed = env_tile();
+  h->lat_bx = autotune(h->stream, fixed ? std::vector<int>{fixed} : std::vector<int>{256, 128, 512}, [&](int bx) {
+    launch_lattice(h->stream, h->g, h->mrt, src, dst, nullptr, h->f32, bx, none);
+  });
 }
 
 // Per-slot geometry arrays and flow buffers of the IBM.  The number of steps
@@ -1128,6 +1133,7 @@ struct lbm_batch {
   size_t items_cap = 0;
   int* d_kbase = nullptr;            // device step base of the batched launches
   int lat_bx = 0;                    // lattice tile width (tuned on the first call)
+  int lat_rows = 1;                  // lattice tiles per work item (tuned with the width)
   char msg[512] = "";
 };
 
@@ -1212,9 +1218,10 @@ int lbm_batch_destroy(lbm_batch* b) {
   return 0;
 }
 
-int lbm_batch_get_tuning(const lbm_batch* b, int32_t* lattice_tile) {
+int lbm_batch_get_tuning(const lbm_batch* b, int32_t* lattice_tile, int32_t* rows_per_item) {
   if (!b) return LBM_E_INVALID;
   if (lattice_tile) *lattice_tile = b->lat_bx;
+  if (rows_per_item) *rows_per_item = b->lat_rows;
   return 0;
 }
 
@@ -1317,24 +1324,35 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
     BCHECK(b, hipMemcpy(b->d_items, items.data(), sizeof(int2) * items.size(), hipMemcpyHostToDevice));
     return 0;
   };
-  if (!b->lat_bx) {  // tile width of the batched lattice launch, timed on this composition
+  if (!b->lat_bx) {  // tile width and tiles per item of the batched lattice launch, timed on this composition
     int rc = 0;
     launch_step_base(b->stream, b->d_kbase, 0, false);  // the timed launches run substep 0
-    b->lat_bx = autotune_tile(b->stream, [&](int bx) {
+    const int fixed_bx = env_tile();
+    int fixed_rows = 0;
+    if (const char* e = getenv("LBM_BATCH_ROWS")) fixed_rows = atoi(e);
+    std::vector<int> cands;  // bx * 8 + rows
+    for (int bx : {256, 128, 512})
+      for (int rows : {1, 2, 4})
+        if ((!fixed_bx || bx == fixed_bx) && (fixed_rows < 1 || rows == fixed_rows)) cands.push_back(bx * 8 + rows);
+    if (cands.empty()) cands.push_back((fixed_bx ? fixed_bx : 256) * 8 + 1);
+    const int best = autotune(b->stream, cands, [&](int c) {
       if (rc) return;
+      const int bx = c / 8, rows = c % 8;
       items.clear();
       for (int e = 0; e < count; e++) {
         desc[e].tiles_x = lattice_tiles_x(desc[e].g, bx);
-        add(e, desc[e].tiles_x * (desc[e].g.ny - 2));
+        add(e, (desc[e].tiles_x * (desc[e].g.ny - 2) + rows - 1) / rows);
       }
       if ((rc = upload())) return;
       BatchItems lt{};
       lt.kbase = b->d_kbase;
       lt.lattice = b->d_items;
       lt.n_lattice = (int)items.size();
-      launch_batch_lattice(b->stream, b->d_desc, lt, 0, f32, false, bx, false);
+      launch_batch_lattice(b->stream, b->d_desc, lt, 0, f32, false, bx, rows, false);
     });
     if (rc) return rc;
+    b->lat_bx = best / 8;
+    b->lat_rows = best % 8;
     items.clear();
   }
   for (int e = 0; e < count; e++) desc[e].tiles_x = lattice_tiles_x(desc[e].g, b->lat_bx);
@@ -1342,7 +1360,7 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
   // lists of a full and of the last (shorter) geometry batch
   size_t off[8];
   off[0] = items.size();
-  for (int e = 0; e < count; e++) add(e, desc[e].tiles_x * (desc[e].g.ny - 2));
+  for (int e = 0; e < count; e++) add(e, (desc[e].tiles_x * (desc[e].g.ny - 2) + b->lat_rows - 1) / b->lat_rows);
   off[1] = items.size();
   for (int e = 0; e < count; e++) add(e, bc_ring_blocks(desc[e].g));
   off[2] = items.size();
@@ -1430,7 +1448,8 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
       launch_batch_adj_collide(b->stream_ibm, b->d_desc, it, kk, f32, fields);
       BCHECK(b, hipStreamWaitEvent(b->stream_ibm, b->ev_bc, 0));
       BCHECK(b, hipEventRecord(b->ev_ibm, b->stream_ibm));
-      launch_batch_lattice(b->stream, b->d_desc, it, kk, f32, fields, b->lat_bx, true, bind ? b->ev_step : nullptr);
+      launch_batch_lattice(b->stream, b->d_desc, it, kk, f32, fields, b->lat_bx, b->lat_rows, true,
+                           bind ? b->ev_step : nullptr);
       BCHECK(b, hipGetLastError());
       BCHECK(b, hipStreamWaitEvent(b->stream, b->ev_ibm, 0));
     }
@@ -1454,7 +1473,7 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
   // The key: the composition, the device lists and, for a whole advance,
   // its length and the last geometry list (a period graph uses the full
   // list only).
-  std::vector<long long> key = {count, ns, f32, wings, b->lat_bx, (long long)(size_t)b->d_desc,
+  std::vector<long long> key = {count, ns, f32, wings, b->lat_bx, b->lat_rows, (long long)(size_t)b->d_desc,
                                 (long long)(size_t)b->d_items, (long long)lds_wing, (long long)lds_solve};
   for (int q = 0; q < 7; q++) key.push_back((long long)off[q]);
   for (int e = 0; e < count; e++) key.push_back((long long)(size_t)hs[e]);

@@ -227,9 +227,10 @@ size_t geo_solve_lds_bytes(const IbmParams& p);
 // launch is *it.kbase + k (the base lives on the device, so that a captured
 // graph of one geometry batch replays for every batch of a long advance).
 // masked: skip each episode's adjoint nodes of the substep (its geometry must exist)
+// rows: tiles per work item (1, 2 or 4; the items were built for it);
 // t1 (optional): an event bound to the launch, completing at its end
 void launch_batch_lattice(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields,
-                          int bx, bool masked, hipEvent_t t1 = nullptr);
+                          int bx, int rows, bool masked, hipEvent_t t1 = nullptr);
 void launch_batch_bc_ring(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields);
 void launch_batch_adj_collide(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields);
 void launch_batch_ibm_flow(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32);

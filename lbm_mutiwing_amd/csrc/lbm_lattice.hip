@@ -688,8 +688,12 @@ void launch_copy_ring_column(hipStream_t s, const Grid& g, const MrtParams& P, c
 
 int bc_ring_blocks(const Grid& g) { return (2 * (g.ny - 2) + 2 * g.nx + 255) / 256; }
 
-// masked = 0 (tile timing before any geometry exists): every interior node
-template <bool F32, bool FIELDS, int BX>
+// masked = 0 (tile timing before any geometry exists): every interior node.
+// A work item is a group of ROWS consecutive tiles of one episode (in
+// row-major tile order), so that the chain of dependent loads a workgroup
+// needs before its first population load (item, then descriptor, then
+// buffer pointers) is paid once per ROWS tiles.
+template <bool F32, bool FIELDS, int BX, int ROWS>
 __global__ __launch_bounds__(BX) void kb_lattice(const EpDesc* __restrict__ eps, const int2* __restrict__ items,
                                                  const int* __restrict__ kb, int kk, int masked) {
   const int2 it = items[blockIdx.x];
@@ -698,12 +702,20 @@ __global__ __launch_bounds__(BX) void kb_lattice(const EpDesc* __restrict__ eps,
   const int n = D.n0 + k, cur = (D.cur0 + k) & 1;
   AdjMask am = adj_mask(D.p, D.G, ep_slot(D, k));
   if (!masked) am.W = 0;
-  if (F32)
-    lattice_f32_body<FIELDS, BX>(D.g, D.P, (const float*)D.fbuf[cur], (float*)D.fbuf[cur ^ 1], D.fslot[n % 3],
-                                 D.tiles_x, it.y, am);
-  else
-    lattice_body<FIELDS, BX>(D.g, D.P, (const double*)D.fbuf[cur], (double*)D.fbuf[cur ^ 1], D.fslot[n % 3],
-                             D.tiles_x, it.y, am);
+  const int ntiles = D.tiles_x * (D.g.ny - 2);
+  const Grid g = D.g;
+  const FieldSlot fs = D.fslot[n % 3];
+#pragma unroll 1
+  for (int r = 0; r < ROWS; r++) {
+    const int blk = it.y * ROWS + r;
+    if (blk >= ntiles) break;
+    if (F32)
+      lattice_f32_body<FIELDS, BX>(g, D.P, (const float*)D.fbuf[cur], (float*)D.fbuf[cur ^ 1], fs, D.tiles_x, blk,
+                                   am);
+    else
+      lattice_body<FIELDS, BX>(g, D.P, (const double*)D.fbuf[cur], (double*)D.fbuf[cur ^ 1], fs, D.tiles_x, blk,
+                               am);
+  }
 }
 
 template <typename S, bool FIELDS>
@@ -748,36 +760,47 @@ void launch_step_base(hipStream_t s, int* p, int v, bool add) {
   hipLaunchKernelGGL(k_step_base, dim3(1), dim3(64), 0, s, p, v, add ? 1 : 0);
 }
 
-template <int BX>
+template <int BX, int ROWS>
 void launch_batch_lattice_bx(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields,
                              int masked, hipEvent_t t1) {
   dim3 grid(it.n_lattice), block(BX);
   if (f32) {
     if (fields)
-      hipExtLaunchKernelGGL((kb_lattice<true, true, BX>), grid, block, 0, s, nullptr, t1, 0, eps, it.lattice, it.kbase,
-                            k, masked);
+      hipExtLaunchKernelGGL((kb_lattice<true, true, BX, ROWS>), grid, block, 0, s, nullptr, t1, 0, eps, it.lattice,
+                            it.kbase, k, masked);
     else
-      hipExtLaunchKernelGGL((kb_lattice<true, false, BX>), grid, block, 0, s, nullptr, t1, 0, eps, it.lattice,
+      hipExtLaunchKernelGGL((kb_lattice<true, false, BX, ROWS>), grid, block, 0, s, nullptr, t1, 0, eps, it.lattice,
                             it.kbase, k, masked);
   } else {
     if (fields)
-      hipExtLaunchKernelGGL((kb_lattice<false, true, BX>), grid, block, 0, s, nullptr, t1, 0, eps, it.lattice,
+      hipExtLaunchKernelGGL((kb_lattice<false, true, BX, ROWS>), grid, block, 0, s, nullptr, t1, 0, eps, it.lattice,
                             it.kbase, k, masked);
     else
-      hipExtLaunchKernelGGL((kb_lattice<false, false, BX>), grid, block, 0, s, nullptr, t1, 0, eps, it.lattice,
+      hipExtLaunchKernelGGL((kb_lattice<false, false, BX, ROWS>), grid, block, 0, s, nullptr, t1, 0, eps, it.lattice,
                             it.kbase, k, masked);
   }
 }
 
+template <int BX>
+void launch_batch_lattice_rows(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields,
+                               int rows, int masked, hipEvent_t t1) {
+  if (rows == 4)
+    launch_batch_lattice_bx<BX, 4>(s, eps, it, k, f32, fields, masked, t1);
+  else if (rows == 2)
+    launch_batch_lattice_bx<BX, 2>(s, eps, it, k, f32, fields, masked, t1);
+  else
+    launch_batch_lattice_bx<BX, 1>(s, eps, it, k, f32, fields, masked, t1);
+}
+
 void launch_batch_lattice(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields,
-                          int bx, bool masked, hipEvent_t t1) {
+                          int bx, int rows, bool masked, hipEvent_t t1) {
   if (it.n_lattice <= 0) return;
   if (bx == 128)
-    launch_batch_lattice_bx<128>(s, eps, it, k, f32, fields, masked ? 1 : 0, t1);
+    launch_batch_lattice_rows<128>(s, eps, it, k, f32, fields, rows, masked ? 1 : 0, t1);
   else if (bx == 512)
-    launch_batch_lattice_bx<512>(s, eps, it, k, f32, fields, masked ? 1 : 0, t1);
+    launch_batch_lattice_rows<512>(s, eps, it, k, f32, fields, rows, masked ? 1 : 0, t1);
   else
-    launch_batch_lattice_bx<256>(s, eps, it, k, f32, fields, masked ? 1 : 0, t1);
+    launch_batch_lattice_rows<256>(s, eps, it, k, f32, fields, rows, masked ? 1 : 0, t1);
 }
 
 void launch_batch_bc_ring(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields) {

@@ -274,11 +274,16 @@ class Batch:
             self._b = ctypes.c_void_p()
 
     def tuning(self) -> int:
+        """Lattice tile width chosen by the batch's autotuner."""
+        return self.tuning_full()[0]
+
+    def tuning_full(self):
+        """(tile width, tiles per workgroup) chosen by the batch's autotuner."""
         fn = self.lib.lbm_batch_get_tuning
-        fn.restype, fn.argtypes = ctypes.c_int, [ctypes.c_void_p, _ip]
-        v = ctypes.c_int32()
-        fn(self._b, ctypes.byref(v))
-        return v.value
+        fn.restype, fn.argtypes = ctypes.c_int, [ctypes.c_void_p, _ip, _ip]
+        v, r = ctypes.c_int32(), ctypes.c_int32()
+        fn(self._b, ctypes.byref(v), ctypes.byref(r))
+        return v.value, r.value
 
     def __del__(self):
         try:

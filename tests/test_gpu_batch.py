@@ -73,6 +73,25 @@ def test_batch_equals_alone(lib, precision):
     b.close()
 
 
+@pytest.mark.parametrize("tile,rows", [(128, 4), (256, 2), (512, 4)])
+def test_batch_tiles_per_workgroup(lib, tile, rows, monkeypatch):
+    """Work items of several lattice tiles (crossing row ends: 2-4 tiles per
+    row here) give the same bits as the instances run alone."""
+    monkeypatch.setenv("LBM_LATTICE_TILE", str(tile))
+    monkeypatch.setenv("LBM_BATCH_ROWS", str(rows))
+    steps = 20
+    alone = [make(sp, PRECISION_FP32_SHIFTED, 100) for sp in SPECS]
+    mixed = [make(sp, PRECISION_FP32_SHIFTED, 100) for sp in SPECS]
+    ra = [s.advance(steps) for s in alone]
+    b = Batch(lib)
+    rb = b.advance(mixed, steps)
+    assert b.tuning_full() == (tile, rows)
+    for e, (sa, sb) in enumerate(zip(alone, mixed)):
+        assert np.array_equal(sa.populations()[0], sb.populations()[0]), e
+        assert np.array_equal(ra[e].last_row, rb[e].last_row), e
+    b.close()
+
+
 def test_batch_rejects_mixed_precision(lib):
     a = make(SPECS[0], PRECISION_FP64, 50)
     c = make(SPECS[1], PRECISION_FP32_SHIFTED, 50)
