@@ -376,22 +376,39 @@ __device__ __forceinline__ double s_entry(const int* box, const double* phx, con
 
 // In-place S = L D L^T of the lower band (band[r*(bw+1) + d] = S(r, r-d)) by
 // one wave: afterwards band[k*(bw+1)] = D_k, band[(k+i)*(bw+1) + i] = L(k+i, k),
-// invd[k] = 1/D_k.
+// invd[k] = 1/D_k.  Step k updates S(k+i, k+j) -= (S(k+i, k) / D_k) S(k+j, k)
+// for 1 <= j <= i <= bw: the bw(bw+1)/2 updates write distinct entries and
+// read only column k, so each lane takes up to five (i, j) pairs of its own
+// and a step costs one round of LDS updates.
+constexpr int LDLT_PAIRS_PER_LANE = (BMAX * (BMAX + 1) / 2 + 63) / 64;
+
 __device__ void band_ldlt(double* band, double* invd, int n, int bw, int lane, DevStep* st, int info) {
   const int Dd = bw + 1;
+  const int npairs = bw * (bw + 1) / 2;
+  int pi[LDLT_PAIRS_PER_LANE], pj[LDLT_PAIRS_PER_LANE];  // pair q = i(i-1)/2 + j-1; i = 0: none
+#pragma unroll
+  for (int u = 0; u < LDLT_PAIRS_PER_LANE; u++) {
+    const int q = lane + 64 * u;
+    int i = 0, j = 0;
+    if (q < npairs) {
+      i = 1;
+      while ((i + 1) * i / 2 <= q) i++;
+      j = q - i * (i - 1) / 2 + 1;
+    }
+    pi[u] = i;
+    pj[u] = j;
+  }
   for (int k = 0; k < n; k++) {
     const double d = band[k * Dd];
     if (lane == 0 && !(d > 0.0)) raise_error(st, DERR_NOT_SPD, info + k);
     const double rd = 1.0 / (d > 0.0 ? d : 1.0);
-    const int i = lane + 1;
-    const bool act = i <= bw && k + i < n;
-    double li = 0.0;
-    if (act) li = band[(k + i) * Dd + i] * rd;
+#pragma unroll
+    for (int u = 0; u < LDLT_PAIRS_PER_LANE; u++) {
+      const int i = pi[u], j = pj[u];
+      if (i > 0 && k + i < n) band[(k + i) * Dd + (i - j)] -= band[(k + i) * Dd + i] * rd * band[(k + j) * Dd + j];
+    }
     wave_sync();
-    if (act)
-      for (int j = 1; j <= i; j++) band[(k + i) * Dd + (i - j)] -= li * band[(k + j) * Dd + j];
-    wave_sync();
-    if (act) band[(k + i) * Dd + i] = li;
+    if (lane >= 1 && lane <= bw && k + lane < n) band[(k + lane) * Dd + lane] = band[(k + lane) * Dd + lane] * rd;
     if (lane == 0) invd[k] = rd;
     wave_sync();
   }
