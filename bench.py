@@ -136,7 +136,7 @@ def run_batch(args):
     t0 = time.perf_counter()
     b.advance(solvers, K)
     t_batch = time.perf_counter() - t0
-    tile, rows = b.tuning_full()
+    tile, rows, waves = b.tuning_full()
     nodes = sum(c.nx * c.ny for c in cfgs)
     for s in solvers:
         s.close()
@@ -168,7 +168,7 @@ def run_batch(args):
                                f"chord 32-128, Re 100-500", "episodes": n_ep, "seed": args.seed,
                    "total_nodes": nodes},
         "aggregate_mlups": round(nodes * K / t_batch / 1e6, 1),
-        "lattice_tile": tile, "lattice_tiles_per_workgroup": rows, "timed": "second STEP of the same episodes",
+        "lattice_tile": tile, "lattice_tiles_per_workgroup": rows, "lattice_waves_per_cu_cap": waves, "timed": "second STEP of the same episodes",
         "one_at_a_time": {"fp64_env_steps_per_s": round(base64, 3), "fp32_env_steps_per_s": round(base32, 3),
                           "sample": f"episodes 0..{len(sample) - 1}, {ksample} steps each, scaled to {K}"},
         "speedup_vs_fp64_one_at_a_time": round(value / base64, 3),
@@ -290,7 +290,7 @@ def main():
     s.set_timing(True)
     s.advance(args.warmup + 2 * args.steps)
     lat_ms, launches = s.timing()
-    tile = s.tuning()
+    tile, waves = s.tuning_full()
     st = s.state()
     nodes = cfg.nx * cfg.ny
     mlups = nodes * args.steps / dt / 1e6
@@ -324,7 +324,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_lattice (fused pull-stream + moments + MRT collide)",
-                     "kernel_avg_us": round(lat_avg_ms * 1e3, 2), "tile": tile,
+                     "kernel_avg_us": round(lat_avg_ms * 1e3, 2), "tile": tile, "waves_per_cu_cap": waves,
                      "algorithmic_bytes_per_launch": bpn * interior},
         "last_force_row": [round(x, 8) for x in r.last_row[:10]],
     }

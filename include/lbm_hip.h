@@ -217,16 +217,18 @@ int lbm_batch_advance(lbm_batch *b, lbm_handle *const *hs, int32_t count,
 int lbm_batch_last_error(const lbm_batch *b, char *buf, size_t n);
 
 /* Autotuner result (no counterpart in the reference): the tile width of the
- * fused lattice kernel (128, 256 or 512 nodes per workgroup), chosen by
- * timing every width on the instance's own lattice at its first
- * lbm_advance (0 before that; LBM_LATTICE_TILE=<width> in the environment
- * fixes it).  lbm_batch_get_tuning gives the width and the number of tiles
- * per workgroup (1, 2 or 4) a batch chose together on its first
- * lbm_batch_advance (LBM_BATCH_ROWS=<n> fixes the latter); either pointer may
- * be NULL. */
-int lbm_get_tuning(const lbm_handle *h, int32_t *lattice_tile);
+ * fused lattice kernel (128, 256 or 512 nodes per workgroup) and its cap on
+ * resident waves per CU (0 none, 24 or 16; set through unused dynamic LDS),
+ * chosen by timing every pair on the instance's own lattice at its first
+ * lbm_advance (0 before that; LBM_LATTICE_TILE=<width> and
+ * LBM_LATTICE_WAVES=<n> in the environment fix them).  lbm_batch_get_tuning
+ * gives the width and the number of tiles per workgroup (1, 2 or 4) a batch
+ * chose together on its first lbm_batch_advance (LBM_BATCH_ROWS=<n> fixes
+ * the latter), then the wave cap for that shape.  Any pointer may be NULL. */
+int lbm_get_tuning(const lbm_handle *h, int32_t *lattice_tile,
+                   int32_t *waves_per_cu);
 int lbm_batch_get_tuning(const lbm_batch *b, int32_t *lattice_tile,
-                         int32_t *rows_per_item);
+                         int32_t *rows_per_item, int32_t *waves_per_cu);
 
 /* Message of the last failed call on h (or of the last failed lbm_create when
  * h is NULL).  Always NUL-terminates when n > 0. */

@@ -153,6 +153,8 @@ struct lbm_handle {
   double motion_end_time = 0.0;
   std::vector<Frame> frames;         // per step of the current call
   int lat_bx = 0;                    // lattice tile width (0: not tuned yet)
+  int lat_waves = 0;                 // lattice waves per CU cap (0: none)
+  size_t lat_lds = 0;                // dynamic LDS per lattice workgroup that sets the cap
   // lattice-kernel timing (lbm_set_timing)
   bool timing = false;
   std::vector<hipEvent_t> ev;        // start/stop pairs of the current call
@@ -337,18 +339,45 @@ int env_tile() {
   return 0;
 }
 
-// Tile width of one instance, timed on its own buffers: the launches write
-// interior nodes of the idle population buffer, which the next step
-// overwrites entirely.
+// LBM_LATTICE_WAVES=<n> in the environment fixes the cap on lattice waves
+// per CU (0: none), -1 otherwise.
+int env_waves() {
+  if (const char* e = getenv("LBM_LATTICE_WAVES")) return std::max(0, std::min(63, atoi(e)));
+  return -1;
+}
+
+int lds_per_cu(int device) {
+  int v = 0;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device) != hipSuccess) v = 0;
+  return v > 0 ? v : 65536;
+}
+
+// Caps on the lattice waves per CU the autotuner tries (0: none): fewer
+// resident lattice waves leave room for the IBM and ring kernels beside it.
+constexpr int kLatticeWaves[3] = {0, 24, 16};
+
+// Tile width and wave cap of one instance, timed on its own buffers: the
+// launches write interior nodes of the idle population buffer, which the
+// next step overwrites entirely.  Candidates are bx * 64 + waves.
 void tune_instance(lbm_handle* h) {
   if (h->lat_bx) return;
   const void* src = h->fbuf[h->cur];
   void* dst = h->fbuf[1 - h->cur];
   const AdjMask none{nullptr, nullptr, 0, 0};
-  const int fixed = env_tile();
-  h->lat_bx = autotune(h->stream, fixed ? std::vector<int>{fixed} : std::vector<int>{256, 128, 512}, [&](int bx) {
-    launch_lattice(h->stream, h->g, h->mrt, src, dst, nullptr, h->f32, bx, none);
+  const int fixed = env_tile(), fw = env_waves(), lds_cu = lds_per_cu(h->device);
+  std::vector<int> cands;
+  for (int bx : {256, 128, 512})
+    for (int wv : kLatticeWaves)
+      if ((!fixed || bx == fixed) && (fw < 0 || wv == fw) && lattice_lds_for_waves(bx, wv, lds_cu) >= 0)
+        cands.push_back(bx * 64 + wv);
+  if (cands.empty()) cands.push_back((fixed ? fixed : 256) * 64);
+  const int best = autotune(h->stream, cands, [&](int c) {
+    launch_lattice(h->stream, h->g, h->mrt, src, dst, nullptr, h->f32, c / 64, none, nullptr, nullptr,
+                   (size_t)lattice_lds_for_waves(c / 64, c % 64, lds_cu));
   });
+  h->lat_bx = best / 64;
+  h->lat_waves = best % 64;
+  h->lat_lds = (size_t)lattice_lds_for_waves(h->lat_bx, h->lat_waves, lds_cu);
 }
 
 // Per-slot geometry arrays and flow buffers of the IBM.  The number of steps
@@ -811,7 +840,7 @@ int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motio
     HIPCHECK(h, hipEventRecord(h->ev_ibm, h->stream_ibm));
     step_ev = h->timing ? h->ev[2 * k + 1] : h->ev_step;
     launch_lattice(h->stream, h->g, h->mrt, src, dst, fs, h->f32, h->lat_bx, adj_mask(h->ip, h->G, slot),
-                   h->timing ? h->ev[2 * k] : nullptr, step_ev);
+                   h->timing ? h->ev[2 * k] : nullptr, step_ev, h->lat_lds);
     HIPCHECK(h, hipGetLastError());
     HIPCHECK(h, hipStreamWaitEvent(h->stream, h->ev_ibm, 0));
     prev_slot = slot;
@@ -1069,9 +1098,10 @@ int lbm_get_timing(lbm_handle* h, double* lattice_ms, int32_t* launches) {
   return 0;
 }
 
-int lbm_get_tuning(const lbm_handle* h, int32_t* lattice_tile) {
+int lbm_get_tuning(const lbm_handle* h, int32_t* lattice_tile, int32_t* waves_per_cu) {
   if (!h) return LBM_E_INVALID;
   if (lattice_tile) *lattice_tile = h->lat_bx;
+  if (waves_per_cu) *waves_per_cu = h->lat_waves;
   return 0;
 }
 
@@ -1134,6 +1164,8 @@ struct lbm_batch {
   int* d_kbase = nullptr;            // device step base of the batched launches
   int lat_bx = 0;                    // lattice tile width (tuned on the first call)
   int lat_rows = 1;                  // lattice tiles per work item (tuned with the width)
+  int lat_waves = 0;                 // lattice waves per CU cap (0: none), tuned last
+  size_t lat_lds = 0;
   char msg[512] = "";
 };
 
@@ -1218,10 +1250,11 @@ int lbm_batch_destroy(lbm_batch* b) {
   return 0;
 }
 
-int lbm_batch_get_tuning(const lbm_batch* b, int32_t* lattice_tile, int32_t* rows_per_item) {
+int lbm_batch_get_tuning(const lbm_batch* b, int32_t* lattice_tile, int32_t* rows_per_item, int32_t* waves_per_cu) {
   if (!b) return LBM_E_INVALID;
   if (lattice_tile) *lattice_tile = b->lat_bx;
   if (rows_per_item) *rows_per_item = b->lat_rows;
+  if (waves_per_cu) *waves_per_cu = b->lat_waves;
   return 0;
 }
 
@@ -1353,6 +1386,27 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
     if (rc) return rc;
     b->lat_bx = best / 8;
     b->lat_rows = best % 8;
+    // then the wave cap for that shape
+    const int fw = env_waves(), lds_cu = lds_per_cu(b->device);
+    std::vector<int> wc;
+    for (int wv : kLatticeWaves)
+      if ((fw < 0 || wv == fw) && lattice_lds_for_waves(b->lat_bx, wv, lds_cu) >= 0) wc.push_back(wv);
+    if (wc.empty()) wc.push_back(0);
+    items.clear();
+    for (int e = 0; e < count; e++) {
+      desc[e].tiles_x = lattice_tiles_x(desc[e].g, b->lat_bx);
+      add(e, (desc[e].tiles_x * (desc[e].g.ny - 2) + b->lat_rows - 1) / b->lat_rows);
+    }
+    if ((rc = upload())) return rc;
+    b->lat_waves = autotune(b->stream, wc, [&](int wv) {
+      BatchItems lt{};
+      lt.kbase = b->d_kbase;
+      lt.lattice = b->d_items;
+      lt.n_lattice = (int)items.size();
+      launch_batch_lattice(b->stream, b->d_desc, lt, 0, f32, false, b->lat_bx, b->lat_rows, false, nullptr,
+                           (size_t)lattice_lds_for_waves(b->lat_bx, wv, lds_cu));
+    });
+    b->lat_lds = (size_t)lattice_lds_for_waves(b->lat_bx, b->lat_waves, lds_cu);
     items.clear();
   }
   for (int e = 0; e < count; e++) desc[e].tiles_x = lattice_tiles_x(desc[e].g, b->lat_bx);
@@ -1449,7 +1503,7 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
       BCHECK(b, hipStreamWaitEvent(b->stream_ibm, b->ev_bc, 0));
       BCHECK(b, hipEventRecord(b->ev_ibm, b->stream_ibm));
       launch_batch_lattice(b->stream, b->d_desc, it, kk, f32, fields, b->lat_bx, b->lat_rows, true,
-                           bind ? b->ev_step : nullptr);
+                           bind ? b->ev_step : nullptr, b->lat_lds);
       BCHECK(b, hipGetLastError());
       BCHECK(b, hipStreamWaitEvent(b->stream, b->ev_ibm, 0));
     }
@@ -1473,7 +1527,7 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
   // The key: the composition, the device lists and, for a whole advance,
   // its length and the last geometry list (a period graph uses the full
   // list only).
-  std::vector<long long> key = {count, ns, f32, wings, b->lat_bx, b->lat_rows, (long long)(size_t)b->d_desc,
+  std::vector<long long> key = {count, ns, f32, wings, b->lat_bx, b->lat_rows, b->lat_waves, (long long)(size_t)b->d_desc,
                                 (long long)(size_t)b->d_items, (long long)lds_wing, (long long)lds_solve};
   for (int q = 0; q < 7; q++) key.push_back((long long)off[q]);
   for (int e = 0; e < count; e++) key.push_back((long long)(size_t)hs[e]);

@@ -193,9 +193,20 @@ __host__ __device__ __forceinline__ int pos_of_seg(int k, int n) {
 constexpr int kLatticeTiles[3] = {128, 256, 512};
 // mask: adjoint nodes of the step, skipped (written by k_adj_collide)
 // t0 / t1 (optional): events bound to the launch itself, completing at its
-// start and at its end (no separate event packets in the stream)
+// start and at its end (no separate event packets in the stream); lds:
+// dynamic LDS per workgroup, which the kernel does not use: it caps the
+// workgroups per CU (lattice_lds_for_waves)
 void launch_lattice(hipStream_t s, const Grid& g, const MrtParams& P, const void* src, void* dst, FieldSlot* fields,
-                    bool f32, int bx, const AdjMask& mask, hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr);
+                    bool f32, int bx, const AdjMask& mask, hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr,
+                    size_t lds = 0);
+// Dynamic LDS that lets at most `waves` waves of bx-thread workgroups share a
+// CU of lds_cu bytes of LDS: 0 for waves = 0 (no cap), -1 when no size does.
+inline long long lattice_lds_for_waves(int bx, int waves, int lds_cu) {
+  if (waves <= 0) return 0;
+  const long long wg = waves / (bx / 64) > 0 ? waves / (bx / 64) : 1;
+  const long long bytes = ((lds_cu / (wg + 1) + 1) + 511) / 512 * 512;
+  return wg * bytes <= lds_cu ? bytes : -1;
+}
 int lattice_tiles_x(const Grid& g, int bx);
 void launch_adj_collide(hipStream_t s, const Grid& g, const MrtParams& P, const IbmParams& p, const GeoBuffers& G,
                         int slot, const FlowBuffers& F, int n, const void* src, void* dst, FieldSlot* fields,
@@ -230,7 +241,7 @@ size_t geo_solve_lds_bytes(const IbmParams& p);
 // rows: tiles per work item (1, 2 or 4; the items were built for it);
 // t1 (optional): an event bound to the launch, completing at its end
 void launch_batch_lattice(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields,
-                          int bx, int rows, bool masked, hipEvent_t t1 = nullptr);
+                          int bx, int rows, bool masked, hipEvent_t t1 = nullptr, size_t lds = 0);
 void launch_batch_bc_ring(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields);
 void launch_batch_adj_collide(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields);
 void launch_batch_ibm_flow(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32);

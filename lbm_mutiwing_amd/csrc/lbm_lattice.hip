@@ -140,7 +140,8 @@ int lattice_tiles_x(const Grid& g, int bx) { return (g.nx + bx - 1) / bx; }
 
 template <int BX>
 void launch_lattice_bx(hipStream_t s, const Grid& g, const MrtParams& P, const void* src, void* dst,
-                       const FieldSlot* fields, bool f32, const AdjMask& am, hipEvent_t t0, hipEvent_t t1) {
+                       const FieldSlot* fields, bool f32, const AdjMask& am, hipEvent_t t0, hipEvent_t t1,
+                       size_t lds) {
   const int tiles_x = lattice_tiles_x(g, BX);
   dim3 grid(tiles_x * (g.ny - 2)), block(BX);
   const FieldSlot fs = fields ? *fields : FieldSlot{};
@@ -148,27 +149,27 @@ void launch_lattice_bx(hipStream_t s, const Grid& g, const MrtParams& P, const v
     const float* a = (const float*)src;
     float* b = (float*)dst;
     if (fields)
-      hipExtLaunchKernelGGL((k_lattice_f32<true, BX>), grid, block, 0, s, t0, t1, 0, g, P, a, b, fs, tiles_x, am);
+      hipExtLaunchKernelGGL((k_lattice_f32<true, BX>), grid, block, lds, s, t0, t1, 0, g, P, a, b, fs, tiles_x, am);
     else
-      hipExtLaunchKernelGGL((k_lattice_f32<false, BX>), grid, block, 0, s, t0, t1, 0, g, P, a, b, fs, tiles_x, am);
+      hipExtLaunchKernelGGL((k_lattice_f32<false, BX>), grid, block, lds, s, t0, t1, 0, g, P, a, b, fs, tiles_x, am);
   } else {
     const double* a = (const double*)src;
     double* b = (double*)dst;
     if (fields)
-      hipExtLaunchKernelGGL((k_lattice<true, BX>), grid, block, 0, s, t0, t1, 0, g, P, a, b, fs, tiles_x, am);
+      hipExtLaunchKernelGGL((k_lattice<true, BX>), grid, block, lds, s, t0, t1, 0, g, P, a, b, fs, tiles_x, am);
     else
-      hipExtLaunchKernelGGL((k_lattice<false, BX>), grid, block, 0, s, t0, t1, 0, g, P, a, b, fs, tiles_x, am);
+      hipExtLaunchKernelGGL((k_lattice<false, BX>), grid, block, lds, s, t0, t1, 0, g, P, a, b, fs, tiles_x, am);
   }
 }
 
 void launch_lattice(hipStream_t s, const Grid& g, const MrtParams& P, const void* src, void* dst, FieldSlot* fields,
-                    bool f32, int bx, const AdjMask& mask, hipEvent_t t0, hipEvent_t t1) {
+                    bool f32, int bx, const AdjMask& mask, hipEvent_t t0, hipEvent_t t1, size_t lds) {
   if (bx == 128)
-    launch_lattice_bx<128>(s, g, P, src, dst, fields, f32, mask, t0, t1);
+    launch_lattice_bx<128>(s, g, P, src, dst, fields, f32, mask, t0, t1, lds);
   else if (bx == 512)
-    launch_lattice_bx<512>(s, g, P, src, dst, fields, f32, mask, t0, t1);
+    launch_lattice_bx<512>(s, g, P, src, dst, fields, f32, mask, t0, t1, lds);
   else
-    launch_lattice_bx<256>(s, g, P, src, dst, fields, f32, mask, t0, t1);
+    launch_lattice_bx<256>(s, g, P, src, dst, fields, f32, mask, t0, t1, lds);
 }
 
 // Re-collide the adjoint nodes with the IBM-corrected velocity u = u* + g and
@@ -762,45 +763,45 @@ void launch_step_base(hipStream_t s, int* p, int v, bool add) {
 
 template <int BX, int ROWS>
 void launch_batch_lattice_bx(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields,
-                             int masked, hipEvent_t t1) {
+                             int masked, hipEvent_t t1, size_t lds) {
   dim3 grid(it.n_lattice), block(BX);
   if (f32) {
     if (fields)
-      hipExtLaunchKernelGGL((kb_lattice<true, true, BX, ROWS>), grid, block, 0, s, nullptr, t1, 0, eps, it.lattice,
+      hipExtLaunchKernelGGL((kb_lattice<true, true, BX, ROWS>), grid, block, lds, s, nullptr, t1, 0, eps, it.lattice,
                             it.kbase, k, masked);
     else
-      hipExtLaunchKernelGGL((kb_lattice<true, false, BX, ROWS>), grid, block, 0, s, nullptr, t1, 0, eps, it.lattice,
+      hipExtLaunchKernelGGL((kb_lattice<true, false, BX, ROWS>), grid, block, lds, s, nullptr, t1, 0, eps, it.lattice,
                             it.kbase, k, masked);
   } else {
     if (fields)
-      hipExtLaunchKernelGGL((kb_lattice<false, true, BX, ROWS>), grid, block, 0, s, nullptr, t1, 0, eps, it.lattice,
+      hipExtLaunchKernelGGL((kb_lattice<false, true, BX, ROWS>), grid, block, lds, s, nullptr, t1, 0, eps, it.lattice,
                             it.kbase, k, masked);
     else
-      hipExtLaunchKernelGGL((kb_lattice<false, false, BX, ROWS>), grid, block, 0, s, nullptr, t1, 0, eps, it.lattice,
+      hipExtLaunchKernelGGL((kb_lattice<false, false, BX, ROWS>), grid, block, lds, s, nullptr, t1, 0, eps, it.lattice,
                             it.kbase, k, masked);
   }
 }
 
 template <int BX>
 void launch_batch_lattice_rows(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields,
-                               int rows, int masked, hipEvent_t t1) {
+                               int rows, int masked, hipEvent_t t1, size_t lds) {
   if (rows == 4)
-    launch_batch_lattice_bx<BX, 4>(s, eps, it, k, f32, fields, masked, t1);
+    launch_batch_lattice_bx<BX, 4>(s, eps, it, k, f32, fields, masked, t1, lds);
   else if (rows == 2)
-    launch_batch_lattice_bx<BX, 2>(s, eps, it, k, f32, fields, masked, t1);
+    launch_batch_lattice_bx<BX, 2>(s, eps, it, k, f32, fields, masked, t1, lds);
   else
-    launch_batch_lattice_bx<BX, 1>(s, eps, it, k, f32, fields, masked, t1);
+    launch_batch_lattice_bx<BX, 1>(s, eps, it, k, f32, fields, masked, t1, lds);
 }
 
 void launch_batch_lattice(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields,
-                          int bx, int rows, bool masked, hipEvent_t t1) {
+                          int bx, int rows, bool masked, hipEvent_t t1, size_t lds) {
   if (it.n_lattice <= 0) return;
   if (bx == 128)
-    launch_batch_lattice_rows<128>(s, eps, it, k, f32, fields, rows, masked ? 1 : 0, t1);
+    launch_batch_lattice_rows<128>(s, eps, it, k, f32, fields, rows, masked ? 1 : 0, t1, lds);
   else if (bx == 512)
-    launch_batch_lattice_rows<512>(s, eps, it, k, f32, fields, rows, masked ? 1 : 0, t1);
+    launch_batch_lattice_rows<512>(s, eps, it, k, f32, fields, rows, masked ? 1 : 0, t1, lds);
   else
-    launch_batch_lattice_rows<256>(s, eps, it, k, f32, fields, rows, masked ? 1 : 0, t1);
+    launch_batch_lattice_rows<256>(s, eps, it, k, f32, fields, rows, masked ? 1 : 0, t1, lds);
 }
 
 void launch_batch_bc_ring(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields) {

@@ -219,11 +219,16 @@ class Solver:
 
     def tuning(self) -> int:
         """Lattice tile width chosen by the autotuner (HIP library only)."""
+        return self.tuning_full()[0]
+
+    def tuning_full(self):
+        """(tile width, cap on lattice waves per CU, 0 = none) chosen by the
+        autotuner (HIP library only)."""
         fn = self.lib.lbm_get_tuning
-        fn.restype, fn.argtypes = ctypes.c_int, [ctypes.c_void_p, _ip]
-        v = ctypes.c_int32()
-        self._check(fn(self._h, ctypes.byref(v)))
-        return v.value
+        fn.restype, fn.argtypes = ctypes.c_int, [ctypes.c_void_p, _ip, _ip]
+        v, w = ctypes.c_int32(), ctypes.c_int32()
+        self._check(fn(self._h, ctypes.byref(v), ctypes.byref(w)))
+        return v.value, w.value
 
 
 class TimedSolver(Solver):
@@ -278,12 +283,13 @@ class Batch:
         return self.tuning_full()[0]
 
     def tuning_full(self):
-        """(tile width, tiles per workgroup) chosen by the batch's autotuner."""
+        """(tile width, tiles per workgroup, cap on lattice waves per CU)
+        chosen by the batch's autotuner."""
         fn = self.lib.lbm_batch_get_tuning
-        fn.restype, fn.argtypes = ctypes.c_int, [ctypes.c_void_p, _ip, _ip]
-        v, r = ctypes.c_int32(), ctypes.c_int32()
-        fn(self._b, ctypes.byref(v), ctypes.byref(r))
-        return v.value, r.value
+        fn.restype, fn.argtypes = ctypes.c_int, [ctypes.c_void_p, _ip, _ip, _ip]
+        v, r, w = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        fn(self._b, ctypes.byref(v), ctypes.byref(r), ctypes.byref(w))
+        return v.value, r.value, w.value
 
     def __del__(self):
         try:

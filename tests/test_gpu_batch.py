@@ -73,19 +73,21 @@ def test_batch_equals_alone(lib, precision):
     b.close()
 
 
-@pytest.mark.parametrize("tile,rows", [(128, 4), (256, 2), (512, 4)])
-def test_batch_tiles_per_workgroup(lib, tile, rows, monkeypatch):
+@pytest.mark.parametrize("tile,rows,waves", [(128, 4, 16), (256, 2, 0), (512, 4, 24)])
+def test_batch_tiles_per_workgroup(lib, tile, rows, waves, monkeypatch):
     """Work items of several lattice tiles (crossing row ends: 2-4 tiles per
-    row here) give the same bits as the instances run alone."""
+    row here), with and without a cap on lattice waves per CU, give the same
+    bits as the instances run alone."""
     monkeypatch.setenv("LBM_LATTICE_TILE", str(tile))
     monkeypatch.setenv("LBM_BATCH_ROWS", str(rows))
+    monkeypatch.setenv("LBM_LATTICE_WAVES", str(waves))
     steps = 20
     alone = [make(sp, PRECISION_FP32_SHIFTED, 100) for sp in SPECS]
     mixed = [make(sp, PRECISION_FP32_SHIFTED, 100) for sp in SPECS]
     ra = [s.advance(steps) for s in alone]
     b = Batch(lib)
     rb = b.advance(mixed, steps)
-    assert b.tuning_full() == (tile, rows)
+    assert b.tuning_full() == (tile, rows, waves)
     for e, (sa, sb) in enumerate(zip(alone, mixed)):
         assert np.array_equal(sa.populations()[0], sb.populations()[0]), e
         assert np.array_equal(ra[e].last_row, rb[e].last_row), e

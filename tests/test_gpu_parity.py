@@ -153,6 +153,26 @@ def test_run_to_run_bitwise_deterministic(lib):
     assert np.array_equal(out[0][1], out[1][1])
 
 
+@pytest.mark.parametrize("tile,waves", [(128, 16), (256, 24), (512, 0)])
+def test_tuning_choices_bitwise_equal(lib, tile, waves, monkeypatch):
+    """Every tile width and wave cap the autotuner may pick computes the same
+    bits (the launch shape changes, the arithmetic per node does not)."""
+    cfg = preset("small")
+    rows = sinusoidal_motion_table(cfg, seeded_motions(cfg, 21), 100)
+    out = []
+    for t, w in ((tile, waves), (256, 0)):
+        monkeypatch.setenv("LBM_LATTICE_TILE", str(t))
+        monkeypatch.setenv("LBM_LATTICE_WAVES", str(w))
+        s = Solver(cfg)
+        s.set_motion_table(rows)
+        r = s.advance(60, want_rows=True)
+        assert s.tuning_full() == (t, w)
+        out.append((s.populations()[0], r.force_rows))
+        s.close()
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+
+
 def test_reset_restores_initial_state(lib):
     cfg = preset("small")
     rows = sinusoidal_motion_table(cfg, seeded_motions(cfg, 2), 200)
