@@ -354,7 +354,7 @@ int lds_per_cu(int device) {
 
 // Caps on the lattice waves per CU the autotuner tries (0: none): fewer
 // resident lattice waves leave room for the IBM and ring kernels beside it.
-constexpr int kLatticeWaves[3] = {0, 24, 16};
+constexpr int kLatticeWaves[5] = {0, 24, 16, 12, 8};
 
 // Tile width and wave cap of one instance, timed on its own buffers: the
 // launches write interior nodes of the idle population buffer, which the
