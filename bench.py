@@ -25,6 +25,13 @@ Two more workloads, each printing its own JSON line:
                 each episode RESET 0 then STEPs of 1000 steps with per-wing
                 (domega, dxv, dyv) increments N(0, 0.1); p50/p99 STEP latency
                 (submission to FORCE reply) and aggregate env-steps/s.
+  --mode mix    the default-scale served mix: 16 episodes on the reference's
+                default 40x20-chord domain (4001 x 2001, chord 100, Re 1000,
+                two wings), all connected at once, RESET 0 then STEPs of 1000
+                steps with seeded per-wing increments N(0, 0.1), served in
+                FP32-shifted storage; the same commands are then served in
+                FP64 and every FORCE value of the FP32 run is compared with
+                the FP64 one.
 """
 from __future__ import annotations
 
@@ -44,13 +51,14 @@ BYTES_PER_NODE = {"fp64": 144.0,   # 9 population reads + 9 writes of 8 B
 
 def parse():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--mode", default="single", choices=["single", "batch", "serve"])
-    ap.add_argument("--episodes", type=int, default=None, help="batch: 64, serve: 32")
+    ap.add_argument("--mode", default="single", choices=["single", "batch", "serve", "mix"])
+    ap.add_argument("--episodes", type=int, default=None, help="batch: 64, serve: 32, mix: 16")
     ap.add_argument("--substeps", type=int, default=1000, help="steps per STEP (env-step)")
     ap.add_argument("--env-steps", type=int, default=4, help="serve: STEPs per episode")
     ap.add_argument("--rate", type=float, default=40.0, help="serve: episode arrivals per second")
     ap.add_argument("--chunk", type=int, default=100, help="serve: steps per scheduler iteration")
     ap.add_argument("--hbm-gb", type=float, default=200.0, help="serve: HBM budget for episodes")
+    ap.add_argument("--spacing", type=float, default=2.0, help="serve: wing spacing in chords")
     ap.add_argument("--base-sample", type=int, default=8, help="batch: episodes timed one at a time")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
@@ -183,28 +191,19 @@ def run_serve(args):
     n_ep = args.episodes or 32
     K, S = args.substeps, args.env_steps
     rng = np.random.default_rng(args.seed)
-    cfgs = [random_episode(rng, PRECISION_FP32_SHIFTED, MOTION_STREAM) for _ in range(n_ep)]
+    # wings 2 chords apart: velocity targets drawn over many STEPs must not
+    # bring two wings' supports onto a common node, which this solver rejects
+    cfgs = [random_episode(rng, PRECISION_FP32_SHIFTED, MOTION_STREAM, spacing=args.spacing) for _ in range(n_ep)]
     arrivals = np.cumsum(rng.exponential(1.0 / args.rate, n_ep))
     noise = [rng.normal(0.0, 0.1, (S, 3 * c.wing_count)) for c in cfgs]
 
     def next_command(i, k, reply):
-        """Client of episode i after its k-th reply: STEP k+1 (k < S) or STOP.
-        Per wing the target velocities of (rotation, x, y) are N(0, 0.1) around
-        the velocity that brings the wing back to its start pose within one
-        STEP, read from the motion columns of the FORCE reply."""
+        """Client of episode i after its k-th reply: STEP k+1 (k < S) or STOP,
+        with N(0, 0.1) velocity targets around a return to the start pose."""
         c = cfgs[i]
         if k >= S:
             return "STOP"
-        W = c.wing_count
-        row = np.array([float(t) for t in reply.split()[2:]])
-        T = K * c.dt
-        d = []
-        for w in range(W):
-            m = row[1 + 3 * W + 3 + 9 * w:]
-            x, xv, y, yv, a, om = m[0], m[1], m[3], m[4], m[6], m[7]
-            x0 = c.wing_x0 + w * c.wing_dx
-            tgt = noise[i][k, 3 * w:3 * w + 3] - np.array([a, x - x0, y - c.wing_y0]) / T
-            d += [tgt[0] - om, tgt[1] - xv, tgt[2] - yv]
+        d = _return_to_start(c, noise[i][k], reply, K * c.dt)
         return "STEP " + repr((k + 1) * K * c.dt) + "".join(" " + repr(float(v)) for v in d)
     srv = EpisodeServer(int(args.hbm_gb * (1 << 30)), chunk=args.chunk)
     for i, c in enumerate(cfgs):
@@ -261,6 +260,132 @@ def run_serve(args):
     }
 
 
+def _force_values(reply):
+    toks = reply.split()
+    if not toks or toks[0] != "FORCE":
+        raise RuntimeError(f"expected a FORCE reply, got: {reply[:200]}")
+    return [float(x) for x in toks[2:]]
+
+
+def _return_to_start(cfg, noise_k, reply, T):
+    """STEP values for the next segment of one episode: per wing the target
+    velocities of (rotation, x, y) are noise_k around the velocity that brings
+    the wing back to its start pose within one STEP, read from the motion
+    columns of the FORCE reply; the STEP carries target minus current."""
+    import numpy as np
+    W = cfg.wing_count
+    row = np.array([float(t) for t in reply.split()[2:]])
+    d = []
+    for w in range(W):
+        m = row[1 + 3 * W + 3 + 9 * w:]
+        x, xv, y, yv, a, om = m[0], m[1], m[3], m[4], m[6], m[7]
+        x0 = cfg.wing_x0 + w * cfg.wing_dx
+        tgt = noise_k[3 * w:3 * w + 3] - np.array([a, x - x0, y - cfg.wing_y0]) / T
+        d += [tgt[0] - om, tgt[1] - xv, tgt[2] - yv]
+    return d
+
+
+def run_mix(args):
+    """Default-scale served mix: seeded command streams served in FP32-shifted
+    storage, then the same command strings served in FP64; every FORCE value
+    of the FP32 run is compared with the FP64 one (relative, floored at 1e-3
+    of the reply's largest value: the first steps' moments are rounding noise
+    beside the forces)."""
+    import numpy as np
+    from lbm_mutiwing_amd import MOTION_STREAM, PRECISION_FP32_SHIFTED, PRECISION_FP64, preset
+    from lbm_mutiwing_amd.server import EpisodeServer
+    n_ep = args.episodes or 16
+    K, S = args.substeps, args.env_steps
+    base = preset("c5")
+    rng = np.random.default_rng(args.seed)
+    noise = [rng.normal(0.0, 0.1, (S, 3 * base.wing_count)) for _ in range(n_ep)]
+    T = K * base.dt
+
+    def serve(precision, script=None):
+        """Serve every episode's commands; without a script the STEPs are
+        built from the replies (return to start plus noise) and recorded."""
+        srv = EpisodeServer(int(args.hbm_gb * (1 << 30)), chunk=args.chunk)
+        for i in range(n_ep):
+            cfg = preset("c5")
+            cfg.precision = precision
+            cfg.motion_source = MOTION_STREAM
+            srv.connect(i, cfg)
+        sent = [["RESET 0.0"] for _ in range(n_ep)]
+        replies = [[] for _ in range(n_ep)]
+        lat = []
+        t0 = time.perf_counter()
+        for i in range(n_ep):
+            srv.submit(i, "RESET 0.0")
+        done = 0
+        while done < n_ep:
+            for r in srv.poll():
+                i = r.client
+                if not r.text.startswith(("FORCE ", "STATUS STOPPING")):
+                    raise RuntimeError(f"episode {i}: {r.text}")
+                if sent[i][-1].startswith("STEP"):
+                    lat.append(r.latency)
+                replies[i].append(r.text)
+                k = len(replies[i]) - 1          # STEPs answered so far
+                if sent[i][-1] == "STOP":
+                    done += 1
+                    continue
+                if script is not None:
+                    nxt = script[i][len(sent[i])]
+                elif k >= S:
+                    nxt = "STOP"
+                else:
+                    d = _return_to_start(base, noise[i][k], r.text, T)
+                    nxt = "STEP " + repr((k + 1) * T) + "".join(" " + repr(float(v)) for v in d)
+                sent[i].append(nxt)
+                srv.submit(i, nxt)
+            if srv.idle():
+                time.sleep(0.0005)
+        wall = time.perf_counter() - t0
+        srv.close()
+        note(f"mix: precision {precision} served in {wall:.1f} s")
+        return sent, replies, np.array(lat) * 1e3, wall
+
+    cmds, r32, lat32, wall32 = serve(PRECISION_FP32_SHIFTED)
+    _, r64, _, wall64 = serve(PRECISION_FP64, script=cmds)
+    # per value, relative with a floor of 1e-3 of the reply's largest value;
+    # and per force/moment column, against that column's largest magnitude
+    # over the episode's STEPs
+    W = base.wing_count
+    nf = 3 * W + 3                      # averaged (Fx, Fy, M) per wing and total
+    errs, col_errs = [], []
+    for i in range(n_ep):
+        fa = np.array([_force_values(a) for a in r32[i][1:S + 1]])
+        fb = np.array([_force_values(b) for b in r64[i][1:S + 1]])
+        for ra, rb in zip(fa, fb):
+            floor = 1e-3 * np.max(np.abs(rb))
+            errs.append(float(np.max(np.abs(ra - rb) / np.maximum(np.abs(rb), floor))))
+        scale = np.max(np.abs(fb[:, 1:1 + nf]), axis=0)
+        col_errs.append(np.max(np.abs(fa[:, 1:1 + nf] - fb[:, 1:1 + nf]), axis=0) / np.maximum(scale, 1e-300))
+    errs = np.array(errs)
+    col_errs = np.array(col_errs)
+    env_steps = n_ep * S
+    return {
+        "metric": "env-steps/s and p50/p99 STEP latency, default-scale served mix, FP32-shifted; FORCE error vs FP64",
+        "value": round(env_steps / wall32, 3), "unit": "env-steps/s", "n_gpus": 1, "steps": K, "warmup": 0,
+        "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded N(0, 0.1) velocity targets around a return-to-start pose per STEP)",
+        "config": {"workload": f"{n_ep} episodes of 4001x2001 (chord 100, Re 1000, 2 wings x {base.points_per_wing} "
+                               f"markers), all connected at once, RESET + {S} STEPs of {K} steps, chunk {args.chunk}",
+                   "episodes": n_ep, "seed": args.seed},
+        "step_latency_ms": {"p50": round(float(np.percentile(lat32, 50)), 2),
+                            "p99": round(float(np.percentile(lat32, 99)), 2), "n": int(lat32.size)},
+        "fp64_env_steps_per_s": round(env_steps / wall64, 3),
+        "force_rel_err_vs_fp64": {"max": float(errs.max()), "median": float(np.median(errs)), "n": int(errs.size),
+                                  "bar": 1e-3},
+        "force_err_vs_fp64_per_column": {"max": float(col_errs.max()), "median": float(np.median(col_errs)),
+                                         "max_by_column": [float(v) for v in col_errs.max(axis=0)],
+                                         "columns": "Fx, Fy, M per wing, then the totals; error over the "
+                                                    "column's largest |value| in the episode"},
+        "note": "STEP commands carry velocity targets only; as in the reference, nothing in the online loop "
+                "moves body_rel_pos, so these STEPs trigger no move_mesh (move_mesh is covered by the parity tests)",
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -273,6 +398,9 @@ def main():
         return
     if args.mode == "serve":
         print(json.dumps(run_serve(args)), flush=True)
+        return
+    if args.mode == "mix":
+        print(json.dumps(run_mix(args)), flush=True)
         return
     from lbm_mutiwing_amd import sinusoidal_motion_table
     from lbm_mutiwing_amd.solver import TimedSolver

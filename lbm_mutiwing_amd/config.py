@@ -274,32 +274,34 @@ def preset(name: str, precision: int = PRECISION_FP64) -> SolverConfig:
 
 
 def episode_config(nx: int, wings: int, chord: int, re: float, precision: int = PRECISION_FP64,
-                   motion_source: int = MOTION_FILE, name: str = "episode") -> SolverConfig:
+                   motion_source: int = MOTION_FILE, name: str = "episode", spacing: float = 1.5) -> SolverConfig:
     """One episode of the batched / served workloads: an nx x nx/2 lattice,
     ``wings`` NACA0012 wings of ``chord`` lattice units (markers about one
-    lattice spacing apart) 1.5 chords apart, the first pivot 1.5 chords from
-    the inflow edge, centred vertically on y = 0.5."""
+    lattice spacing apart) ``spacing`` chords apart, the first pivot 1.5
+    chords from the inflow edge, centred vertically on y = 0.5."""
     ny = nx // 2
     c = float(chord)
     cfg = SolverConfig(nx=nx, ny=ny, scale=1.0 / c, uu=0.05, re=float(re),
                        s=(0.0, 1.4, 1.4, 0.0, 1.2, 0.0, 1.2, 0.0, 0.0), wing_count=wings,
                        markers=naca0012_markers(max(16, int(chord) - 2)), ox=-1.5,
                        oy=0.5 - 0.5 * (ny - 1) / c, motion_source=motion_source, precision=precision, name=name)
+    cfg.wing_dx = float(spacing)
     return cfg
 
 
 def random_episode(rng: np.random.Generator, precision: int = PRECISION_FP32_SHIFTED,
-                   motion_source: int = MOTION_FILE, min_nx: int = 512, max_nx: int = 2048) -> SolverConfig:
+                   motion_source: int = MOTION_FILE, min_nx: int = 512, max_nx: int = 2048,
+                   spacing: float = 1.5) -> SolverConfig:
     """Episode of the uneven batched workload: lattice log-uniform from
     min_nx x min_nx/2 to max_nx x max_nx/2 (2:1), 1-3 wings of chord 32-128
-    lattice units (capped so that the wings and 1.5 chords of margin fit),
-    Re uniform in [100, 500]."""
+    lattice units ``spacing`` chords apart (chord capped so that the wings
+    and 1.5 chords of margin fit), Re uniform in [100, 500]."""
     nx = int(round(math.exp(rng.uniform(math.log(min_nx), math.log(max_nx)))))
     nx -= nx % 2
     wings = int(rng.integers(1, 4))
-    cmax = max(32, min(128, int(nx / (1.5 * (wings - 1) + 4.0))))
+    cmax = max(32, min(128, int(nx / (spacing * (wings - 1) + 4.0))))
     chord = int(rng.integers(32, cmax + 1))
     re = float(rng.uniform(100.0, 500.0))
     return episode_config(nx, wings, chord, re, precision, motion_source,
-                          name=f"ep{nx}x{nx // 2}w{wings}c{chord}")
+                          name=f"ep{nx}x{nx // 2}w{wings}c{chord}", spacing=spacing)
 
