@@ -195,6 +195,29 @@ def test_wing_at_edge_is_an_error(lib):
         s.advance(2)
 
 
+def test_c1_2000_steps(lib, oracle_mod):
+    """The 1024x512 two-wing case (chord 64, Re 200, seeded pitch/heave),
+    2000 steps: rho/u/v within 1e-10 relative L2 of the FP64 oracle after
+    1000 and after 2000 steps, and the per-wing forces and moments averaged
+    over steps 1001-2000 within 1e-9 relative."""
+    cfg = preset("c1")
+    n = 2000
+    rows = sinusoidal_motion_table(cfg, seeded_motions(cfg, 17), n + 2)
+    g, o = pair(cfg, oracle_mod, rows)
+    for n_last in (1000, n):
+        print(f"c1 parity: advancing to step {n_last}", flush=True)
+        rg, ro = g.advance(n_last, want_rows=True), o.advance(n_last, want_rows=True)
+        errs = [rel(a, b) for a, b in zip(g.fields()[:3], o.fields()[:3])]
+        print(f"c1 parity: step {n_last}, rho/u/v relative L2 {errs}", flush=True)
+        assert max(errs) < FIELD_TOL, n_last
+    W = cfg.wing_count
+    avg_g = rg.force_rows[:, 1:1 + 3 * W + 3].mean(0)
+    avg_o = ro.force_rows[:, 1:1 + 3 * W + 3].mean(0)
+    ferr = np.abs(avg_g - avg_o) / np.maximum(np.abs(avg_o), 1e-3 * np.max(np.abs(avg_o)))
+    print(f"c1 parity: averaged forces and moments, largest relative difference {ferr.max():.3e}", flush=True)
+    assert np.all(ferr <= FORCE_TOL)
+
+
 @pytest.mark.parametrize("name", ["c1"])
 def test_c1_short_parity(lib, oracle_mod, name):
     cfg = preset(name)
