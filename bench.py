@@ -85,26 +85,48 @@ def workload(name, seed, precision="fp64"):
     return cfg, mot
 
 
-def cpu_baseline(name, seed, budget_s):
+def cpu_baseline(name, seed, budget_s, precision):
     """The FP64 oracle (C restatement of the reference CPU path, OpenMP over
-    the host cores) on the first steps of the same workload."""
-    from lbm_mutiwing_amd import sinusoidal_motion_table
+    the host cores) on the first steps of the same workload, timed; then the
+    GPU path over the same steps from a fresh instance, with its error against
+    the oracle's fields and force rows."""
+    import numpy as np
+    from lbm_mutiwing_amd import Solver, sinusoidal_motion_table
     from oracle import oracle
     cores = min(16, os.cpu_count() or 1)
     oracle.set_threads(cores)
     cfg, mot = workload(name, seed)
+    table = sinusoidal_motion_table(cfg, mot, 64)
     o = oracle.OracleSolver(cfg)
-    o.set_motion_table(sinusoidal_motion_table(cfg, mot, 64))
-    o.advance(1)  # first step outside the timing (page faults of fresh buffers)
+    o.set_motion_table(table)
+    r1 = o.advance(1, want_rows=True)  # first step outside the timing (page faults of fresh buffers)
+    rows = [r1.force_rows]
     steps, t = 0, 0.0
     while t < budget_s and steps < 50:
         t0 = time.perf_counter()
-        o.advance(2 + steps)
+        r = o.advance(2 + steps, want_rows=True)
         t += time.perf_counter() - t0
+        rows.append(r.force_rows)
         steps += 1
+    fo = o.fields()[:3]
     o.close()
-    return {"value": cfg.nx * cfg.ny * steps / t / 1e6, "unit": "MLUPS", "cores": cores,
-            "kind": "port", "sample": f"{name} steps 2..{1 + steps} ({steps} steps, {t:.1f} s)"}
+    ro = np.concatenate(rows)
+    gcfg, _ = workload(name, seed, precision)
+    g = Solver(gcfg)
+    g.set_motion_table(table)
+    rg = g.advance(1 + steps, want_rows=True)
+    fg = g.fields()[:3]
+    g.close()
+
+    def rel(a, b):
+        return float(np.linalg.norm(a - b) / max(float(np.linalg.norm(b)), 1e-300))
+    W = cfg.wing_count
+    fr = slice(1, 1 + 3 * W + 3)
+    err = {"steps": 1 + steps, "rho": rel(fg[0], fo[0]), "u": rel(fg[1], fo[1]), "v": rel(fg[2], fo[2]),
+           "force_rows": rel(rg.force_rows[:, fr], ro[:, fr]),
+           "bar": "1e-10 fields, 1e-9 forces (FP64); 1e-4 fields, 1e-3 forces (FP32-shifted)"}
+    return ({"value": cfg.nx * cfg.ny * steps / t / 1e6, "unit": "MLUPS", "cores": cores,
+             "kind": "port", "sample": f"{name} steps 2..{1 + steps} ({steps} steps, {t:.1f} s)"}, err)
 
 
 def note(msg):
@@ -466,7 +488,8 @@ def main():
         torch.cuda.synchronize()
         out["copy_calibration_bytes"] = 2 * n * 8
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(args.config, args.seed, args.cpu_seconds)
+        out["cpu_baseline"], out["error_vs_fp64_oracle"] = cpu_baseline(args.config, args.seed, args.cpu_seconds,
+                                                                        args.precision)
     print(json.dumps(out), flush=True)
 
 
