@@ -195,6 +195,20 @@ def test_wing_at_edge_is_an_error(lib):
         s.advance(2)
 
 
+def test_wings_sharing_a_node_are_an_error(lib):
+    """Two wings whose support boxes reach a common lattice node would couple
+    their IBM systems (the reference solves them as one); this solver keeps
+    one system per wing and reports the step as failed instead of solving a
+    different system."""
+    from lbm_mutiwing_amd import LbmError
+    cfg = preset("small")
+    cfg.wing_dx = 1.05     # 0.05 chord (1.6 nodes) between trailing and leading edge
+    s = Solver(cfg)
+    s.set_motion_table(sinusoidal_motion_table(cfg, [WingMotion(), WingMotion()], 10))
+    with pytest.raises(LbmError, match="share a lattice node"):
+        s.advance(2)
+
+
 def test_c1_2000_steps(lib, oracle_mod):
     """The 1024x512 two-wing case (chord 64, Re 200, seeded pitch/heave),
     2000 steps: rho/u/v within 1e-10 relative L2 of the FP64 oracle after
