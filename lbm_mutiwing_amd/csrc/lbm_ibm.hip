@@ -461,8 +461,8 @@ __device__ void inverse_column(const double* band, const double* invd, int n, in
   }
 }
 
-// One workgroup per (step, wing): adjoint total, wing-coupling check, band
-// assembly of S, L D L^T and the explicit inverse (one column per thread).
+// One workgroup per (step, wing): adjoint total, band assembly of S,
+// L D L^T and the explicit inverse (one column per thread).
 __device__ __forceinline__ void geo_solve_body(const IbmParams& p, const GeoBuffers& G, DevStep* st, int slot0, int b,
                                                int w) {
   extern __shared__ double lds_s[];
@@ -484,23 +484,6 @@ __device__ __forceinline__ void geo_solve_body(const IbmParams& p, const GeoBuff
       gs->adj_n = tot;
       gs->band[0] = b0;
       gs->band[1] = b1;
-    }
-  }
-  // nodes claimed by two wings couple their systems: not supported
-  const int* rw = gs->rect[w];
-  for (int w2 = w + 1; w2 < p.W; w2++) {
-    const int* r2 = gs->rect[w2];
-    const int ia = max(rw[0], r2[0]), ib = min(rw[1], r2[1]);
-    const int ja = max(rw[2], r2[2]), jb = min(rw[3], r2[3]);
-    if (ia > ib || ja > jb) continue;
-    const int* M1 = G.map + ((size_t)slot * p.W + w) * 2 * (size_t)p.map_cap;
-    const int* M2 = G.map + ((size_t)slot * p.W + w2) * 2 * (size_t)p.map_cap;
-    const int w1w = rw[1] - rw[0] + 1, w2w = r2[1] - r2[0] + 1;
-    const int ow = ib - ia + 1, oc = ow * (jb - ja + 1);
-    for (int c = t; c < oc; c += GEO_THREADS) {
-      const int i = ia + c % ow, j = ja + c / ow;
-      if (M1[(i - rw[0]) + w1w * (j - rw[2])] != INT_MAX && M2[(i - r2[0]) + w2w * (j - r2[2])] != INT_MAX)
-        raise_error(st, DERR_WINGS_COUPLED, w2);
     }
   }
   __syncthreads();
@@ -546,13 +529,378 @@ __global__ __launch_bounds__(GEO_THREADS) void k_geo_solve(IbmParams p, GeoBuffe
 
 size_t geo_solve_lds_bytes(const IbmParams& p) { return (size_t)p.SPW * (BMAX + 2) * sizeof(double); }
 
+// ------------------------------------------------------------ coupled wings
+//
+// When the support boxes of wings a < b share lattice nodes, the IBM system
+// is the one over both wings' segments (boundary_condition.f90:1558-1640 over
+// all segments): S = [[S_aa, C], [C^T, S_bb]], with C = E_a C0 E_b^T nonzero
+// only between the p segments of a and q segments of b whose boxes meet.  The
+// shared nodes belong to a (its segments claim first, airfoil.f90:288-333).
+// With u = blockdiag(S_aa, S_bb)^-1 c (the per-wing solves), the Schur
+// complement on b and Woodbury give
+//   w   = C0^T u_a[K_a]
+//   v   = u_b - S_bb^-1[:, K_b] w                     (= S_bb^-1 (c_b - C^T u_a))
+//   y_b = v + S_bb^-1[:, K_b] M v[K_b],   M = (I - H G_bb)^-1 H,  H = C0^T G_aa C0
+//   y_a = u_a - S_aa^-1[:, K_a] C0 y_b[K_b]
+// with G_ww = S_ww^-1[K_w, K_w].  The geometry pass finds the pair, K_a,
+// K_b, C0 and M (k_geo_couple) and drops the shared nodes from b's adjoint
+// list; the flow pass applies the correction after the per-wing mat-vecs
+// (k_ibm_couple), and the spread adds b's coupling segments at a's nodes.
+constexpr int CPL_THREADS = 256;
+constexpr int CPL_MAX_SPW = 2048;   // segments per wing the coupling pass handles
+
+__device__ __forceinline__ bool boxes_meet(const int* a, const int* b) {
+  return a[0] <= b[1] && b[0] <= a[1] && a[2] <= b[3] && b[2] <= a[3];
+}
+
+// S(k, l) for segments of two different wings: s_entry with the boxes and
+// weights of each segment given separately.
+__device__ double s_cross(const int* bk, const double* pxk, const double* pyk, const int* bl, const double* pxl,
+                          const double* pyl) {
+  const int ia = max(bk[0], bl[0]), ib = min(bk[1], bl[1]);
+  const int ja = max(bk[2], bl[2]), jb = min(bk[3], bl[3]);
+  if (ia > ib || ja > jb) return 0.0;
+  double sx = 0.0, sy = 0.0;
+  for (int i = ia; i <= ib; i++) sx += pxk[i - bk[0]] * pxl[i - bl[0]];
+  for (int j = ja; j <= jb; j++) sy += pyk[j - bk[2]] * pyl[j - bl[2]];
+  return sx * sy;
+}
+
+// exclusive prefix sum of v over the workgroup (CPL_THREADS entries of s);
+// returns the total
+__device__ int block_scan(int* s, int v) {
+  const int t = threadIdx.x;
+  s[t] = v;
+  __syncthreads();
+  for (int d = 1; d < CPL_THREADS; d <<= 1) {
+    const int x = t >= d ? s[t - d] : 0;
+    __syncthreads();
+    s[t] += x;
+    __syncthreads();
+  }
+  const int total = s[CPL_THREADS - 1];
+  const int incl = s[t];
+  __syncthreads();
+  s[t] = incl - v;
+  __syncthreads();
+  return total;
+}
+
+// One workgroup per step, after the per-wing solves of the step.
+__device__ __forceinline__ void geo_couple_body(const IbmParams& p, const GeoBuffers& G, DevStep* st, int slot) {
+  __shared__ int s_flag[MAXW * MAXW];
+  __shared__ int s_cnt[2], s_skip;
+  __shared__ int s_k[2][CPL_K];
+  __shared__ double s_c0[CPL_K * CPL_K];        // C0 (p x q)
+  __shared__ double s_x[CPL_K * CPL_K];         // G_aa C0 (p x q), then G_bb (q x q)
+  __shared__ double s_h[CPL_K * CPL_K];         // H (q x q)
+  __shared__ double s_aug[CPL_K * 2 * CPL_K];   // [I - H G_bb | H], row stride 2q
+  __shared__ double s_f[CPL_K];
+  __shared__ int s_piv;
+  __shared__ int s_scan[CPL_THREADS];
+  __shared__ int s_seg[CPL_MAX_SPW + 1];
+  const int t = threadIdx.x;
+  const int W = p.W, n = p.SPW;
+  GeoStep* gs = G.step + slot;
+  for (int q = t; q < MAXW * MAXW; q += CPL_THREADS) s_flag[q] = 0;
+  if (t == 0) {
+    gs->cpl_on = 0;
+    s_cnt[0] = s_cnt[1] = 0;
+    s_skip = load_agent(&st->error) != 0;
+  }
+  __syncthreads();
+  if (s_skip) return;
+  // pairs of wings whose claim maps share a cell
+  for (int w = 0; w < W; w++)
+    for (int w2 = w + 1; w2 < W; w2++) {
+      const int* r1 = gs->rect[w];
+      const int* r2 = gs->rect[w2];
+      const int ia = max(r1[0], r2[0]), ib = min(r1[1], r2[1]);
+      const int ja = max(r1[2], r2[2]), jb = min(r1[3], r2[3]);
+      if (ia > ib || ja > jb) continue;
+      const int* M1 = G.map + ((size_t)slot * W + w) * 2 * (size_t)p.map_cap;
+      const int* M2 = G.map + ((size_t)slot * W + w2) * 2 * (size_t)p.map_cap;
+      const int w1w = r1[1] - r1[0] + 1, w2w = r2[1] - r2[0] + 1;
+      const int ow = ib - ia + 1, oc = ow * (jb - ja + 1);
+      for (int c = t; c < oc; c += CPL_THREADS) {
+        const int i = ia + c % ow, j = ja + c / ow;
+        if (M1[(i - r1[0]) + w1w * (j - r1[2])] != INT_MAX && M2[(i - r2[0]) + w2w * (j - r2[2])] != INT_MAX)
+          s_flag[w * MAXW + w2] = 1;
+      }
+    }
+  __syncthreads();
+  int npair = 0, a = -1, b = -1;
+  for (int w = 0; w < W; w++)
+    for (int w2 = w + 1; w2 < W; w2++)
+      if (s_flag[w * MAXW + w2]) {
+        npair++;
+        a = w;
+        b = w2;
+      }
+  if (npair == 0) return;
+  if (npair > 1 || n > CPL_MAX_SPW) {
+    if (t == 0) raise_error(st, DERR_WINGS_COUPLED, npair > 1 ? 1000 + npair : n);
+    return;
+  }
+  // the coupling segments of each wing
+  const size_t sa0 = (size_t)slot * p.nseg + (size_t)a * n, sb0 = (size_t)slot * p.nseg + (size_t)b * n;
+  const int* boxa = G.box + sa0 * 4;
+  const int* boxb = G.box + sb0 * 4;
+  for (int side = 0; side < 2; side++) {
+    const int* mine = side ? boxb : boxa;
+    const int* other = side ? boxa : boxb;
+    const int* ro = gs->rect[side ? a : b];
+    for (int k = t; k < n; k += CPL_THREADS) {
+      const int* bk = mine + 4 * k;
+      if (!boxes_meet(bk, ro)) continue;
+      bool meet = false;
+      for (int l = 0; l < n && !meet; l++) meet = boxes_meet(bk, other + 4 * l);
+      if (meet) {
+        const int idx = atomicAdd(&s_cnt[side], 1);
+        if (idx < CPL_K) s_k[side][idx] = k;
+      }
+    }
+  }
+  __syncthreads();
+  const int pa = s_cnt[0], qb = s_cnt[1];
+  if (pa > CPL_K || qb > CPL_K || pa == 0 || qb == 0) {
+    if (t == 0) raise_error(st, DERR_WINGS_COUPLED, 2000 + max(pa, qb));
+    return;
+  }
+  if (t < 2) {  // ascending segment order
+    int* ks = s_k[t];
+    const int m = t ? qb : pa;
+    for (int x = 1; x < m; x++) {
+      const int v = ks[x];
+      int y = x - 1;
+      while (y >= 0 && ks[y] > v) {
+        ks[y + 1] = ks[y];
+        y--;
+      }
+      ks[y + 1] = v;
+    }
+  }
+  __syncthreads();
+  // C0, then G_aa C0, H = C0^T G_aa C0, G_bb, and [I - H G_bb | H]
+  for (int e = t; e < pa * qb; e += CPL_THREADS) {
+    const int i = e / qb, j = e - i * qb;
+    const int k = s_k[0][i], l = s_k[1][j];
+    s_c0[e] = s_cross(boxa + 4 * k, G.phx + (sa0 + k) * BOXW, G.phy + (sa0 + k) * BOXW, boxb + 4 * l,
+                      G.phx + (sb0 + l) * BOXW, G.phy + (sb0 + l) * BOXW);
+  }
+  __syncthreads();
+  const double* SA = G.sinv + ((size_t)slot * W + a) * (size_t)n * n;
+  const double* SB = G.sinv + ((size_t)slot * W + b) * (size_t)n * n;
+  for (int e = t; e < pa * qb; e += CPL_THREADS) {
+    const int i = e / qb, j = e - i * qb;
+    const size_t ri = (size_t)pos_of_seg(s_k[0][i], n) * n;
+    double acc = 0.0;
+    for (int m = 0; m < pa; m++) acc += SA[ri + pos_of_seg(s_k[0][m], n)] * s_c0[m * qb + j];
+    s_x[e] = acc;
+  }
+  __syncthreads();
+  for (int e = t; e < qb * qb; e += CPL_THREADS) {
+    const int i = e / qb, j = e - i * qb;
+    double acc = 0.0;
+    for (int m = 0; m < pa; m++) acc += s_c0[m * qb + i] * s_x[m * qb + j];
+    s_h[e] = acc;
+  }
+  __syncthreads();
+  for (int e = t; e < qb * qb; e += CPL_THREADS) {
+    const int i = e / qb, j = e - i * qb;
+    s_x[e] = SB[(size_t)pos_of_seg(s_k[1][i], n) * n + pos_of_seg(s_k[1][j], n)];
+  }
+  __syncthreads();
+  const int q2 = 2 * qb;
+  for (int e = t; e < qb * qb; e += CPL_THREADS) {
+    const int i = e / qb, j = e - i * qb;
+    double acc = i == j ? 1.0 : 0.0;
+    for (int m = 0; m < qb; m++) acc -= s_h[i * qb + m] * s_x[m * qb + j];
+    s_aug[i * q2 + j] = acc;
+    s_aug[i * q2 + qb + j] = s_h[e];
+  }
+  __syncthreads();
+  // Gauss-Jordan with partial pivoting: the right half becomes M
+  for (int c = 0; c < qb; c++) {
+    if (t == 0) {
+      int r = c;
+      for (int x = c + 1; x < qb; x++)
+        if (fabs(s_aug[x * q2 + c]) > fabs(s_aug[r * q2 + c])) r = x;
+      s_piv = s_aug[r * q2 + c] != 0.0 ? r : -1;
+    }
+    __syncthreads();
+    const int r = s_piv;
+    if (r < 0) {
+      if (t == 0) raise_error(st, DERR_WINGS_COUPLED, 3000 + c);
+      return;
+    }
+    if (r != c)
+      for (int e = t; e < q2; e += CPL_THREADS) {
+        const double x = s_aug[c * q2 + e];
+        s_aug[c * q2 + e] = s_aug[r * q2 + e];
+        s_aug[r * q2 + e] = x;
+      }
+    __syncthreads();
+    const double inv = 1.0 / s_aug[c * q2 + c];
+    __syncthreads();
+    for (int e = t; e < q2; e += CPL_THREADS) s_aug[c * q2 + e] *= inv;
+    if (t < qb) s_f[t] = t == c ? 0.0 : s_aug[t * q2 + c];
+    __syncthreads();
+    for (int e = t; e < qb * q2; e += CPL_THREADS) {
+      const int rr = e / q2, cc = e - rr * q2;
+      if (rr != c) s_aug[e] -= s_f[rr] * s_aug[c * q2 + cc];
+    }
+    __syncthreads();
+  }
+  double* C0 = G.cpl + (size_t)slot * 2 * CPL_K * CPL_K;
+  double* Mo = C0 + CPL_K * CPL_K;
+  for (int e = t; e < pa * qb; e += CPL_THREADS) C0[e] = s_c0[e];
+  for (int e = t; e < qb * qb; e += CPL_THREADS) Mo[e] = s_aug[(e / qb) * q2 + qb + (e % qb)];
+  // b's adjoint list without the nodes a claims, in order; new segment offsets
+  for (int k = t; k <= n; k += CPL_THREADS) s_seg[k] = 0;
+  __syncthreads();
+  const int* ra = gs->rect[a];
+  const int raw = ra[1] - ra[0] + 1;
+  const int* MA = G.map + ((size_t)slot * W + a) * 2 * (size_t)p.map_cap;
+  const size_t lb = (size_t)slot * p.cap + (size_t)b * p.capw;
+  int* aij = G.adj_ij + lb * 2;
+  int* aseg = G.adj_seg + lb;
+  const int cntb = gs->cnt[b];
+  int kept = 0;
+  for (int c0 = 0; c0 < cntb; c0 += CPL_THREADS) {
+    const int e = c0 + t;
+    int keep = 0, i = 0, j = 0, k = 0;
+    if (e < cntb) {
+      i = aij[2 * e];
+      j = aij[2 * e + 1];
+      k = aseg[e];
+      const bool in_a = i >= ra[0] && i <= ra[1] && j >= ra[2] && j <= ra[3] &&
+                        MA[(i - ra[0]) + raw * (j - ra[2])] != INT_MAX;
+      keep = !in_a;
+    }
+    const int total = block_scan(s_scan, keep);
+    if (keep) {
+      const int dst = kept + s_scan[t];
+      aij[2 * dst] = i;
+      aij[2 * dst + 1] = j;
+      aseg[dst] = k;
+      atomicAdd(&s_seg[k], 1);
+    }
+    kept += total;
+    __syncthreads();
+  }
+  int* soff = G.seg_off + ((size_t)slot * W + b) * (n + 1);
+  if (t == 0) {
+    int run = 0;
+    for (int k = 0; k < n; k++) {
+      const int c = s_seg[k];
+      soff[k] = run;
+      run += c;
+    }
+    soff[n] = run;
+    gs->adj_n -= cntb - kept;
+    gs->cnt[b] = kept;
+    gs->cpl_a = a;
+    gs->cpl_b = b;
+    gs->cpl_p = pa;
+    gs->cpl_q = qb;
+    for (int x = 0; x < pa; x++) gs->cpl_ka[x] = s_k[0][x];
+    for (int x = 0; x < qb; x++) gs->cpl_kb[x] = s_k[1][x];
+    gs->cpl_on = 1;
+  }
+}
+
+__global__ __launch_bounds__(CPL_THREADS) void k_geo_couple(IbmParams p, GeoBuffers G, DevStep* st, int slot0) {
+  geo_couple_body(p, G, st, slot0 + blockIdx.x);
+}
+
+// The coupling correction of the flow pass (one workgroup), after the
+// per-wing mat-vecs: F.yvec holds u = blockdiag(S_ww)^-1 c on entry and the
+// solution of the coupled system on exit (both components).
+__device__ __forceinline__ void ibm_couple_body(const IbmParams& p, const GeoBuffers& G, int slot,
+                                                const FlowBuffers& F) {
+  const GeoStep* gs = G.step + slot;
+  if (!gs->cpl_on) return;
+  __shared__ int s_pa[CPL_K], s_pb[CPL_K];
+  __shared__ double s_v[2 * CPL_K];
+  const int t = threadIdx.x, n = p.SPW;
+  const int a = gs->cpl_a, b = gs->cpl_b, pa = gs->cpl_p, qb = gs->cpl_q;
+  const double* C0 = G.cpl + (size_t)slot * 2 * CPL_K * CPL_K;
+  const double* Mm = C0 + CPL_K * CPL_K;
+  const double* SA = G.sinv + ((size_t)slot * p.W + a) * (size_t)n * n;
+  const double* SB = G.sinv + ((size_t)slot * p.W + b) * (size_t)n * n;
+  double* ya = F.yvec + 2 * (size_t)a * n;
+  double* yb = F.yvec + 2 * (size_t)b * n;
+  if (t < pa) s_pa[t] = pos_of_seg(gs->cpl_ka[t], n);
+  if (t < qb) s_pb[t] = pos_of_seg(gs->cpl_kb[t], n);
+  __syncthreads();
+  // w = C0^T u_a[K_a]
+  if (t < 2 * qb) {
+    const int j = t >> 1, c = t & 1;
+    double acc = 0.0;
+    for (int i = 0; i < pa; i++) acc += C0[i * qb + j] * ya[2 * s_pa[i] + c];
+    s_v[t] = acc;
+  }
+  __syncthreads();
+  // v = u_b - S_bb^-1[:, K_b] w
+  for (int e = t; e < 2 * n; e += CPL_THREADS) {
+    const int r = e >> 1, c = e & 1;
+    const double* row = SB + (size_t)r * n;
+    double acc = yb[e];
+    for (int j = 0; j < qb; j++) acc -= row[s_pb[j]] * s_v[2 * j + c];
+    yb[e] = acc;
+  }
+  __syncthreads();
+  // t = M v[K_b]
+  double tv = 0.0;
+  if (t < 2 * qb) {
+    const int i = t >> 1, c = t & 1;
+    for (int j = 0; j < qb; j++) tv += Mm[i * qb + j] * yb[2 * s_pb[j] + c];
+  }
+  __syncthreads();
+  if (t < 2 * qb) s_v[t] = tv;
+  __syncthreads();
+  // y_b = v + S_bb^-1[:, K_b] t
+  for (int e = t; e < 2 * n; e += CPL_THREADS) {
+    const int r = e >> 1, c = e & 1;
+    const double* row = SB + (size_t)r * n;
+    double acc = yb[e];
+    for (int j = 0; j < qb; j++) acc += row[s_pb[j]] * s_v[2 * j + c];
+    yb[e] = acc;
+  }
+  __syncthreads();
+  // z = C0 y_b[K_b]
+  double zv = 0.0;
+  if (t < 2 * pa) {
+    const int i = t >> 1, c = t & 1;
+    for (int j = 0; j < qb; j++) zv += C0[i * qb + j] * yb[2 * s_pb[j] + c];
+  }
+  __syncthreads();
+  if (t < 2 * pa) s_v[t] = zv;
+  __syncthreads();
+  // y_a = u_a - S_aa^-1[:, K_a] z
+  for (int e = t; e < 2 * n; e += CPL_THREADS) {
+    const int r = e >> 1, c = e & 1;
+    const double* row = SA + (size_t)r * n;
+    double acc = ya[e];
+    for (int i = 0; i < pa; i++) acc -= row[s_pa[i]] * s_v[2 * i + c];
+    ya[e] = acc;
+  }
+}
+
+__global__ __launch_bounds__(CPL_THREADS) void k_ibm_couple(IbmParams p, GeoBuffers G, int slot, FlowBuffers F) {
+  ibm_couple_body(p, G, slot, F);
+}
+
 void launch_geometry(hipStream_t s, const IbmParams& p, const GeoBuffers& G, DevStep* st, int slot0, int n0,
                      int fidx0, int count, bool with_inverse) {
   if (p.W <= 0 || count <= 0) return;
   dim3 grid(count, p.W);
   hipLaunchKernelGGL(k_geo_wing, grid, dim3(GEO_THREADS), geo_wing_lds_bytes(p.SPW), s, p, G, st, slot0, n0, fidx0);
-  if (with_inverse)
+  if (with_inverse) {
     hipLaunchKernelGGL(k_geo_solve, grid, dim3(GEO_THREADS), geo_solve_lds_bytes(p), s, p, G, st, slot0);
+    if (p.W >= 2) hipLaunchKernelGGL(k_geo_couple, dim3(count), dim3(CPL_THREADS), 0, s, p, G, st, slot0);
+  }
 }
 
 // ------------------------------------------------------------- flow part
@@ -674,6 +1022,7 @@ __device__ __forceinline__ void ibm_spread_body(const IbmParams& p, const GeoBuf
   __syncthreads();
   const int* soff = G.seg_off + ((size_t)slot * p.W + w) * (n + 1);
   const int a0 = soff[k0], a1 = soff[k1];
+  const bool cpl = gs->cpl_on && gs->cpl_a == w;
   const size_t lb = (size_t)slot * p.cap + (size_t)w * p.capw;
   const int* aij = G.adj_ij + lb * 2;
   const int* aseg = G.adj_seg + lb;
@@ -689,6 +1038,19 @@ __device__ __forceinline__ void ibm_spread_body(const IbmParams& p, const GeoBuf
       const double phi_ = s_ph[e][i - s_box[e][0]] * s_ph[e][BOXW + j - s_box[e][2]];
       gx = gx + phi_ * s_y[e][0];
       gy = gy + phi_ * s_y[e][1];
+    }
+    if (cpl) {  // the other wing's coupling segments whose boxes hold the node
+      const int wb = gs->cpl_b;
+      for (int x = 0; x < gs->cpl_q; x++) {
+        const int l = gs->cpl_kb[x];
+        const size_t sl = (size_t)slot * p.nseg + (size_t)wb * n + l;
+        const int* bl = G.box + sl * 4;
+        if (i < bl[0] || i > bl[1] || j < bl[2] || j > bl[3]) continue;
+        const double phi_ = G.phx[sl * BOXW + (i - bl[0])] * G.phy[sl * BOXW + (j - bl[2])];
+        const size_t yq = (size_t)wb * n + pos_of_seg(l, n);
+        gx = gx + phi_ * F.yvec[2 * yq];
+        gy = gy + phi_ * F.yvec[2 * yq + 1];
+      }
     }
     gout[2 * a] = gx;
     gout[2 * a + 1] = gy;
@@ -708,6 +1070,7 @@ void launch_ibm_flow(hipStream_t s, const IbmParams& p, const MrtParams& P, cons
   else
     hipLaunchKernelGGL(k_ibm_rhs<double>, dim3(blk), dim3(256), 0, s, p, P, G, slot, F, g, (const double*)src);
   hipLaunchKernelGGL(k_ibm_matvec, dim3(blk), dim3(256), 0, s, p, G, slot, F);
+  if (p.W >= 2) hipLaunchKernelGGL(k_ibm_couple, dim3(1), dim3(CPL_THREADS), 0, s, p, G, slot, F);
   const int nb = (p.SPW + SPREAD_SEG - 1) / SPREAD_SEG;
   hipLaunchKernelGGL(k_ibm_spread, dim3(p.W * nb), dim3(SPREAD_THREADS), 0, s, p, G, slot, F);
 }
@@ -736,11 +1099,24 @@ __global__ __launch_bounds__(GEO_THREADS) void kb_geo_solve(const EpDesc* __rest
   geo_solve_body(D.p, D.G, D.st, ep_slot(D, k0), b, w);
 }
 
+// coupling pass of step b of each episode with two or more wings: the
+// workgroup of its (b, wing 0) item
+__global__ __launch_bounds__(CPL_THREADS) void kb_geo_couple(const EpDesc* __restrict__ eps,
+                                                             const int2* __restrict__ items,
+                                                             const int* __restrict__ kb, int kk0) {
+  const int2 it = items[blockIdx.x];
+  const EpDesc& D = eps[it.x];
+  if (D.p.W < 2 || it.y % D.p.W != 0) return;
+  const int k0 = kb[0] + kk0;
+  geo_couple_body(D.p, D.G, D.st, ep_slot(D, k0) + it.y / D.p.W);
+}
+
 void launch_batch_geometry(hipStream_t s, const EpDesc* eps, const int* kb, const int2* items, int n_items,
                            int k0, size_t lds_wing, size_t lds_solve) {
   if (n_items <= 0) return;
   hipLaunchKernelGGL(kb_geo_wing, dim3(n_items), dim3(GEO_THREADS), lds_wing, s, eps, items, kb, k0);
   hipLaunchKernelGGL(kb_geo_solve, dim3(n_items), dim3(GEO_THREADS), lds_solve, s, eps, items, kb, k0);
+  hipLaunchKernelGGL(kb_geo_couple, dim3(n_items), dim3(CPL_THREADS), 0, s, eps, items, kb, k0);
 }
 
 template <typename S>
@@ -770,6 +1146,18 @@ __global__ __launch_bounds__(SPREAD_THREADS) void kb_ibm_spread(const EpDesc* __
   ibm_spread_body(D.p, D.G, ep_slot(D, k), D.F, it.y);
 }
 
+// coupling correction of each episode with two or more wings: the
+// workgroup of its first segment item
+__global__ __launch_bounds__(CPL_THREADS) void kb_ibm_couple(const EpDesc* __restrict__ eps,
+                                                             const int2* __restrict__ items,
+                                                             const int* __restrict__ kb, int kk) {
+  const int2 it = items[blockIdx.x];
+  const EpDesc& D = eps[it.x];
+  if (it.y != 0 || D.p.W < 2) return;
+  const int k = kb[0] + kk;
+  ibm_couple_body(D.p, D.G, ep_slot(D, k), D.F);
+}
+
 void launch_batch_ibm_flow(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32) {
   if (it.n_seg <= 0) return;
   if (f32)
@@ -777,6 +1165,7 @@ void launch_batch_ibm_flow(hipStream_t s, const EpDesc* eps, const BatchItems& i
   else
     hipLaunchKernelGGL(kb_ibm_rhs<double>, dim3(it.n_seg), dim3(256), 0, s, eps, it.seg, it.kbase, k);
   hipLaunchKernelGGL(kb_ibm_matvec, dim3(it.n_seg), dim3(256), 0, s, eps, it.seg, it.kbase, k);
+  hipLaunchKernelGGL(kb_ibm_couple, dim3(it.n_seg), dim3(CPL_THREADS), 0, s, eps, it.seg, it.kbase, k);
   hipLaunchKernelGGL(kb_ibm_spread, dim3(it.n_spread), dim3(SPREAD_THREADS), 0, s, eps, it.spread, it.kbase, k);
 }
 

@@ -195,17 +195,65 @@ def test_wing_at_edge_is_an_error(lib):
         s.advance(2)
 
 
-def test_wings_sharing_a_node_are_an_error(lib):
-    """Two wings whose support boxes reach a common lattice node would couple
-    their IBM systems (the reference solves them as one); this solver keeps
-    one system per wing and reports the step as failed instead of solving a
-    different system."""
-    from lbm_mutiwing_amd import LbmError
+def _coupled_case(dx=1.05):
     cfg = preset("small")
-    cfg.wing_dx = 1.05     # 0.05 chord (1.6 nodes) between trailing and leading edge
+    cfg.wing_dx = dx       # 0.05 chord (1.6 nodes) between trailing and leading edge
+    return cfg
+
+
+def test_coupled_wings_match_oracle(lib, oracle_mod):
+    """Two wings whose support boxes share lattice nodes: one IBM system over
+    both wings' segments, the shared nodes owned by the first wing, as the
+    reference solves it (boundary_condition.f90:1558-1640, airfoil.f90:288-333).
+    Fields within 1e-10, forces within 1e-9, identical adjoint list and owners."""
+    cfg = _coupled_case()
+    n = 200
+    motions = [WingMotion(pitch=0.05, heave=0.02, freq=0.25, phase_p=0.0, phase_h=0.5),
+               WingMotion(pitch=0.08, heave=0.03, freq=0.25, phase_p=1.0, phase_h=2.0)]
+    rows = sinusoidal_motion_table(cfg, motions, n + 2)
+    g, o = pair(cfg, oracle_mod, rows)
+    rg, ro = g.advance(n, want_rows=True), o.advance(n, want_rows=True)
+    ig, io = g.ibm(), o.ibm()
+    assert np.array_equal(ig[0], io[0]) and np.array_equal(ig[1], io[1])  # adjoint list and owners
+    assert (io[1] == 2).any() and (io[1] == 1).any()
+    errs = [rel(a, b) for a, b in zip(g.fields()[:3], o.fields()[:3])]
+    print(f"coupled wings: rho/u/v relative L2 {errs}", flush=True)
+    assert max(errs) < FIELD_TOL
+    W = cfg.wing_count
+    fr = slice(1, 1 + 3 * W + 3)
+    assert rel(rg.force_rows[:, fr], ro.force_rows[:, fr]) < FORCE_TOL
+    assert rel(rg.row_sum, ro.row_sum) < FORCE_TOL
+
+
+def test_coupled_wings_batch_equals_alone(lib):
+    """A coupled instance advanced in a batch ends bitwise as alone."""
+    from lbm_mutiwing_amd import Batch
+    cfgs = [_coupled_case(), preset("small")]
+    motions = [WingMotion(pitch=0.05), WingMotion(pitch=0.08, heave=0.03)]
+    alone, mixed = [], []
+    for c in cfgs:
+        for lst in (alone, mixed):
+            s = Solver(c)
+            s.set_motion_table(sinusoidal_motion_table(c, motions, 80))
+            lst.append(s)
+    ra = [s.advance(40) for s in alone]
+    b = Batch(lib)
+    rb = b.advance(mixed, 40)
+    for e in range(2):
+        assert np.array_equal(alone[e].populations()[0], mixed[e].populations()[0]), e
+        assert np.array_equal(ra[e].row_sum, rb[e].row_sum), e
+    b.close()
+
+
+def test_three_coupled_wings_are_an_error(lib):
+    """Only one coupled pair per step is solved; three wings in a chain are
+    reported as a failed step."""
+    from lbm_mutiwing_amd import LbmError
+    cfg = _coupled_case()
+    cfg.wing_count = 3
     s = Solver(cfg)
-    s.set_motion_table(sinusoidal_motion_table(cfg, [WingMotion(), WingMotion()], 10))
-    with pytest.raises(LbmError, match="share a lattice node"):
+    s.set_motion_table(sinusoidal_motion_table(cfg, [WingMotion(), WingMotion(), WingMotion()], 10))
+    with pytest.raises(LbmError, match="more than two wings"):
         s.advance(2)
 
 
