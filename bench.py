@@ -58,7 +58,7 @@ def parse():
     ap.add_argument("--rate", type=float, default=40.0, help="serve: episode arrivals per second")
     ap.add_argument("--chunk", type=int, default=100, help="serve: steps per scheduler iteration")
     ap.add_argument("--hbm-gb", type=float, default=200.0, help="serve: HBM budget for episodes")
-    ap.add_argument("--spacing", type=float, default=2.0, help="serve: wing spacing in chords")
+    ap.add_argument("--spacing", type=float, default=1.5, help="serve: wing spacing in chords")
     ap.add_argument("--base-sample", type=int, default=8, help="batch: episodes timed one at a time")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
@@ -213,8 +213,6 @@ def run_serve(args):
     n_ep = args.episodes or 32
     K, S = args.substeps, args.env_steps
     rng = np.random.default_rng(args.seed)
-    # wings 2 chords apart: velocity targets drawn over many STEPs must not
-    # bring two wings' supports onto a common node, which this solver rejects
     cfgs = [random_episode(rng, PRECISION_FP32_SHIFTED, MOTION_STREAM, spacing=args.spacing) for _ in range(n_ep)]
     arrivals = np.cumsum(rng.exponential(1.0 / args.rate, n_ep))
     noise = [rng.normal(0.0, 0.1, (S, 3 * c.wing_count)) for c in cfgs]
