@@ -173,6 +173,40 @@ int lbm_set_step(lbm_handle *h, int32_t n_next, int32_t movenum);
  * for wing 1 triggers move_mesh after the next step (main.f90:393-411). */
 int lbm_set_body_rel_pos(lbm_handle *h, int32_t wing, double pos);
 
+/* Rigid-body state body_rel_* of all wings, 11*wing_count doubles in the
+ * order of the restart file (solution.f90:1663-1665): a, a_prev, v, v_prev,
+ * s, s_prev, pos, theta_prev, theta, temp_theta_prev, temp_theta, each
+ * wing_count long.  lbm_reset zeroes it (driver.f90:218-228); setting it
+ * recomputes body_extra_* (mode.f90:473-495), as readflowfield does. */
+#define LBM_BODY_REL_FIELDS 11
+int lbm_get_body_rel(lbm_handle *h, double *rel);
+int lbm_set_body_rel(lbm_handle *h, const double *rel);
+
+/* Rigid-body coupling of wing 1 (boundary_condition.f90:1694-1719, rk4
+ * :3168-3202; switches and constants main.f90:98, 122-125, 141-151).  When
+ * enabled, each step's IBM force drives wing 1 along x: body_rel_a(1) =
+ * Fx / m with Fx the raw x force summed over all wings and m = solid_den *
+ * area / scale^2 (closed_boundary) or solid_den / scale^2; v, s and pos
+ * follow by RK4 (rk4 != 0, the reference's if_rk4) or explicit Euler, and
+ * theta by the pitch equation with p1 = par1*UU^2*scale^2, p2 = par2*scale^4
+ * and the moment of wing 1.  body_extra_* follow, so the next step's markers,
+ * their velocity and the force row's motion columns move with the body, and
+ * |body_rel_pos(1)| >= 1 triggers move_mesh.  A step then depends on the
+ * previous step's force: lbm_advance runs such an instance one step at a
+ * time (geometry not ahead of the flow), and lbm_batch_advance rejects it.
+ * enable = 0 (the reference's setting) restores the prescribed-motion path. */
+typedef struct lbm_dynamics {
+    int32_t enable;            /* solve_dynamic_equations (main.f90:124-125)   */
+    int32_t closed_boundary;   /* main.f90:122-123                             */
+    int32_t rk4;               /* if_rk4 (boundary_condition.f90:1527-1529)    */
+    int32_t reserved;          /* 0                                            */
+    double solid_den;          /* main.f90:145                                 */
+    double area;               /* area_airfoil (main.f90:147)                  */
+    double par1, par2;         /* main.f90:150-151                             */
+    double alphad;             /* radians (main.f90:98)                        */
+} lbm_dynamics;
+int lbm_set_dynamics(lbm_handle *h, const lbm_dynamics *d);
+
 /* Measurement hook (no counterpart in the reference): when enabled, every
  * launch of the fused lattice kernel is bracketed by HIP events on the
  * instance's stream; lbm_get_timing returns the summed kernel time (ms) and

@@ -147,7 +147,8 @@ struct lbm_handle {
   int movenum = 0;
   int flag2 = 1;
   int last_move_step = -1;           // step after which move_mesh last ran
-  double body_rel_pos[MAXW] = {};
+  double rel[REL_FIELDS][MAXW] = {};  // body_rel_* (restart-file order)
+  lbm_dynamics dyn{};                // rigid-body coupling switch and constants
   Frame fr{};                        // movenum and body_extra_* of the next step
   bool motion_ready = false;
   double motion_end_time = 0.0;
@@ -201,7 +202,7 @@ int ialloc(lbm_handle* h, T** p, size_t count) {
 }
 
 void update_body_extra(lbm_handle* h) {  // update_body_extra_from_rel (mode.f90:473-495)
-  double ap = 0.0;
+  double av = 0.0, as = 0.0, ap = 0.0;
   h->fr.movenum = h->movenum;
   for (int w = 0; w < MAXW; w++) {
     h->fr.body_extra_v[w] = 0.0;
@@ -209,7 +210,11 @@ void update_body_extra(lbm_handle* h) {  // update_body_extra_from_rel (mode.f90
     h->fr.body_extra_pos[w] = 0.0;
   }
   for (int w = 0; w < h->W; w++) {
-    ap = ap + h->body_rel_pos[w];
+    av = av + h->rel[REL_V][w];
+    as = as + h->rel[REL_S][w];
+    ap = ap + h->rel[REL_POS][w];
+    h->fr.body_extra_v[w] = av;
+    h->fr.body_extra_s[w] = as;
     h->fr.body_extra_pos[w] = ap;
   }
 }
@@ -628,7 +633,8 @@ int lbm_reset(lbm_handle* h) {
     HIPCHECK(h, hipMemcpy(h->d_stream, z.data(), sizeof(double) * z.size(), hipMemcpyHostToDevice));
     h->motion_ready = true;
   }
-  for (int w = 0; w < MAXW; w++) h->body_rel_pos[w] = 0.0;
+  for (int q = 0; q < REL_FIELDS; q++)
+    for (int w = 0; w < MAXW; w++) h->rel[q][w] = 0.0;
   h->n = 1;
   h->movenum = 0;
   h->flag2 = 1;
@@ -729,11 +735,120 @@ int lbm_build_stream_target_row(int32_t W, const double* start, double target_ti
   return 0;
 }
 
+}  // extern "C"
+
+namespace {
+
+// move_mesh (move_mesh.f90:17-58, 171-214) after step m, whose fields are in
+// slot s; the next step reads the shifted lattice after ev_step.
+int apply_move(lbm_handle* h, int dr, int s, int m) {
+  launch_shift_x(h->stream, h->g, dr, h->fbuf[h->cur], h->fbuf[1 - h->cur], 9, h->f32);
+  h->cur = 1 - h->cur;
+  launch_copy_ring_column(h->stream, h->g, h->mrt, h->fbuf[h->cur], dr == 1 ? 0 : h->g.nx - 1,
+                          dr == 1 ? h->R.left : h->R.right, h->f32);
+  launch_ring_moments(h->stream, h->g, h->R);
+  FieldSlot& cs = h->slot[s];
+  launch_shift_x(h->stream, h->g, dr, cs.rho, h->spare.rho, 1, false);
+  launch_shift_x(h->stream, h->g, dr, cs.u, h->spare.u, 1, false);
+  launch_shift_x(h->stream, h->g, dr, cs.v, h->spare.v, 1, false);
+  std::swap(h->slot[s], h->spare);
+  h->rel[REL_POS][0] = h->rel[REL_POS][0] - (double)dr;
+  h->movenum += dr;
+  update_body_extra(h);
+  h->flag2 = 3;
+  h->last_move_step = m;
+  HIPCHECK(h, hipEventRecord(h->ev_step, h->stream));
+  return 0;
+}
+
+int step_error(lbm_handle* h, const DevStep& st) {
+  return fail(h, st.error == DERR_MOTION ? LBM_E_STATE : (st.error == DERR_NOT_SPD ? LBM_E_NUMERIC : LBM_E_RANGE),
+              "step failed: %s (info %d)", dev_error_text(st.error), st.error_info);
+}
+
+int finish_advance(lbm_handle* h, int nsteps, int n_last, bool timed, double* force_rows, double* motion_rows,
+                   double* row_sum, double* last_row, double* err);
+
+// Advance of an instance with rigid-body coupling, one step at a time: the
+// geometry of step m needs the body state left by the force of step m-1,
+// and the move_mesh check after step m reads that state on the host.  Each
+// step runs the same kernels as lbm_advance (fields written every step, so
+// that a move can shift them), with its force row, and the body update in
+// it, right after the adjoint collision on the IBM stream.
+int advance_dynamic(lbm_handle* h, int32_t n_last, double* force_rows, double* motion_rows, double* row_sum,
+                    double* last_row, double* err) {
+  const int n0 = h->n;
+  const int nsteps = n_last >= n0 ? n_last - n0 + 1 : 0;
+  int rc = ensure_rows(h, (size_t)std::max(nsteps, 1));
+  if (rc) return rc;
+  rc = ensure_frames(h, 1);
+  if (rc) return rc;
+  if (nsteps > 0) tune_instance(h);
+  HIPCHECK(h, hipMemsetAsync(h->d_rowsum, 0, sizeof(double) * h->row_len, h->stream));
+  HIPCHECK(h, hipMemsetAsync(&h->d_st->error, 0, sizeof(int), h->stream));
+  HIPCHECK(h, hipStreamSynchronize(h->stream));
+  const int ns = h->nslot;
+  for (int m = n0; m <= n_last; m++) {
+    const int k = m - n0;
+    const int set = (int)(h->batch_count & 1u), slot = set * ns, s = m % 3;
+    HIPCHECK(h, hipMemcpy(h->d_frame, &h->fr, sizeof(Frame), hipMemcpyHostToDevice));
+    HIPCHECK(h, hipMemcpy(h->d_st->rel, h->rel, sizeof(h->rel), hipMemcpyHostToDevice));
+    launch_geometry(h->stream_geo, h->ip, h->G, h->d_st, slot, m, 0, 1, true);
+    HIPCHECK(h, hipEventRecord(h->ev_geo[set], h->stream_geo));
+    h->batch_count++;
+    HIPCHECK(h, hipStreamWaitEvent(h->stream_ibm, h->ev_geo[set], 0));
+    HIPCHECK(h, hipStreamWaitEvent(h->stream, h->ev_geo[set], 0));
+    FieldSlot* fs = &h->slot[s];
+    const void* src = h->fbuf[h->cur];
+    void* dst = h->fbuf[1 - h->cur];
+    launch_bc_ring(h->stream_bc, h->g, h->mrt, h->R, src, dst, fs, h->f32);
+    HIPCHECK(h, hipEventRecord(h->ev_bc, h->stream_bc));
+    launch_ibm_flow(h->stream_ibm, h->ip, h->mrt, h->G, slot, h->F, h->g, src, h->f32);
+    launch_adj_collide(h->stream_ibm, h->g, h->mrt, h->ip, h->G, slot, h->F, m, src, dst, fs, h->f32);
+    launch_force_row(h->stream_ibm, h->ip, h->G, slot, h->F, h->d_st, m, k, h->row_len);
+    HIPCHECK(h, hipStreamWaitEvent(h->stream_ibm, h->ev_bc, 0));
+    HIPCHECK(h, hipEventRecord(h->ev_ibm, h->stream_ibm));
+    launch_lattice(h->stream, h->g, h->mrt, src, dst, fs, h->f32, h->lat_bx, adj_mask(h->ip, h->G, slot), nullptr,
+                   nullptr, h->lat_lds);
+    HIPCHECK(h, hipGetLastError());
+    HIPCHECK(h, hipStreamWaitEvent(h->stream, h->ev_ibm, 0));
+    h->slot_step[s] = m;
+    h->cur = 1 - h->cur;
+    h->last_slot = slot;
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    DevStep st;
+    HIPCHECK(h, hipMemcpy(&st, h->d_st, sizeof(DevStep), hipMemcpyDeviceToHost));
+    if (st.error != 0) {
+      h->n = m + 1;
+      return step_error(h, st);
+    }
+    for (int q = 0; q < REL_FIELDS; q++)
+      for (int w = 0; w < MAXW; w++) h->rel[q][w] = st.rel[q][w];
+    update_body_extra(h);
+    // the move_mesh check of the run loop (main.f90:393-412)
+    const double brp = h->rel[REL_POS][0];
+    const int dr = brp >= 1.0 ? 1 : (brp <= -1.0 ? -1 : 0);
+    if (dr != 0) {
+      rc = apply_move(h, dr, s, m);
+      if (rc) return rc;
+      HIPCHECK(h, hipStreamSynchronize(h->stream));  // the next step's ring and IBM streams read the shifted lattice
+    }
+    if (h->flag2 > 0) h->flag2 = h->flag2 - 1;
+  }
+  h->n = std::max(n0, n_last + 1);
+  return finish_advance(h, nsteps, n_last, false, force_rows, motion_rows, row_sum, last_row, err);
+}
+
+}  // namespace
+
+extern "C" {
+
 int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motion_rows, double* row_sum,
                 double* last_row, double* err) {
   if (!h) return LBM_E_INVALID;
   if (h->cfg.motion_source == LBM_MOTION_FILE && !h->motion_ready && h->W > 0)
     return fail(h, LBM_E_STATE, "motion table not loaded");
+  if (h->dyn.enable && h->W > 0) return advance_dynamic(h, n_last, force_rows, motion_rows, row_sum, last_row, err);
   const int n0 = h->n;
   const int nsteps = n_last >= n0 ? n_last - n0 + 1 : 0;
   int rc = ensure_rows(h, (size_t)std::max(nsteps, 1));
@@ -750,7 +865,7 @@ int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motio
   // body_rel_pos(1), which only this host changes (main.f90:393-411).
   if (nsteps > 0) {
     h->frames.resize(nsteps);
-    double brp0 = h->W > 0 ? h->body_rel_pos[0] : 0.0;
+    double brp0 = h->W > 0 ? h->rel[REL_POS][0] : 0.0;
     Frame f = h->fr;
     for (int k = 0; k < nsteps; k++) {
       h->frames[k] = f;
@@ -761,7 +876,7 @@ int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motio
         double ap = brp0;
         f.body_extra_pos[0] = ap;
         for (int w = 1; w < h->W; w++) {
-          ap = ap + h->body_rel_pos[w];
+          ap = ap + h->rel[REL_POS][w];
           f.body_extra_pos[w] = ap;
         }
       }
@@ -809,7 +924,7 @@ int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motio
     }
     slot = set * ns + k % ns;
     const int s = m % 3;
-    double brp = h->W > 0 ? h->body_rel_pos[0] : 0.0;
+    double brp = h->W > 0 ? h->rel[REL_POS][0] : 0.0;
     const int dr = brp >= 1.0 ? 1 : (brp <= -1.0 ? -1 : 0);
     const bool want_fields = (m >= n_last - 2) || dr != 0;
     FieldSlot* fs = want_fields ? &h->slot[s] : nullptr;
@@ -849,23 +964,9 @@ int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motio
     prev_slot = slot;
     if (want_fields) h->slot_step[s] = m;
     h->cur = 1 - h->cur;
-    if (dr != 0) {  // move_mesh (move_mesh.f90:17-58, 171-214)
-      launch_shift_x(h->stream, h->g, dr, h->fbuf[h->cur], h->fbuf[1 - h->cur], 9, h->f32);
-      h->cur = 1 - h->cur;
-      launch_copy_ring_column(h->stream, h->g, h->mrt, h->fbuf[h->cur], dr == 1 ? 0 : h->g.nx - 1,
-                              dr == 1 ? h->R.left : h->R.right, h->f32);
-      launch_ring_moments(h->stream, h->g, h->R);
-      FieldSlot& cs = h->slot[s];
-      launch_shift_x(h->stream, h->g, dr, cs.rho, h->spare.rho, 1, false);
-      launch_shift_x(h->stream, h->g, dr, cs.u, h->spare.u, 1, false);
-      launch_shift_x(h->stream, h->g, dr, cs.v, h->spare.v, 1, false);
-      std::swap(h->slot[s], h->spare);
-      h->body_rel_pos[0] = h->body_rel_pos[0] - (double)dr;
-      h->movenum += dr;
-      update_body_extra(h);
-      h->flag2 = 3;
-      h->last_move_step = m;
-      HIPCHECK(h, hipEventRecord(h->ev_step, h->stream));  // the next step reads the shifted lattice
+    if (dr != 0) {
+      rc = apply_move(h, dr, s, m);
+      if (rc) return rc;
       step_ev = h->ev_step;
     }
     if (h->flag2 > 0) h->flag2 = h->flag2 - 1;
@@ -879,6 +980,17 @@ int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motio
   h->batch_count = bc0 + (unsigned)nbatch;
   if (nsteps > 0 && h->W > 0) h->last_slot = slot;
   h->n = std::max(n0, n_last + 1);
+  return finish_advance(h, nsteps, n_last, h->timing, force_rows, motion_rows, row_sum, last_row, err);
+}
+
+}  // extern "C"
+
+namespace {
+
+// Common end of an advance over steps n_last-nsteps+1 .. n_last: the
+// erranalysis of the last step, the device error check and the outputs.
+int finish_advance(lbm_handle* h, int nsteps, int n_last, bool timed, double* force_rows, double* motion_rows,
+                   double* row_sum, double* last_row, double* err) {
   // erranalysis of the last step (solution.f90:77-130): u of step n-1 against step n-2
   double errv = 0.0;
   const int ml = n_last;
@@ -900,10 +1012,8 @@ int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motio
   HIPCHECK(h, hipStreamSynchronize(h->stream));
   DevStep st_host;
   HIPCHECK(h, hipMemcpy(&st_host, h->d_st, sizeof(DevStep), hipMemcpyDeviceToHost));
-  if (st_host.error != 0)
-    return fail(h, st_host.error == DERR_MOTION ? LBM_E_STATE : (st_host.error == DERR_NOT_SPD ? LBM_E_NUMERIC : LBM_E_RANGE),
-                "step failed: %s (info %d)", dev_error_text(st_host.error), st_host.error_info);
-  if (h->timing)
+  if (st_host.error != 0) return step_error(h, st_host);
+  if (timed)
     for (int i = 0; i < nsteps; i++) {
       float ms = 0.f;
       HIPCHECK(h, hipEventElapsedTime(&ms, h->ev[2 * i], h->ev[2 * i + 1]));
@@ -929,6 +1039,10 @@ int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motio
   }
   return 0;
 }
+
+}  // namespace
+
+extern "C" {
 
 int lbm_get_state(lbm_handle* h, int32_t* n_next, int32_t* movenum, int32_t* flag2, int32_t* row_len,
                   int32_t* adjoint_n) {
@@ -1126,8 +1240,40 @@ int lbm_set_step(lbm_handle* h, int32_t n_next, int32_t movenum) {
 
 int lbm_set_body_rel_pos(lbm_handle* h, int32_t wing, double pos) {
   if (!h || wing < 1 || wing > h->W) return fail(h, LBM_E_INVALID, "wing out of range");
-  h->body_rel_pos[wing - 1] = pos;
+  h->rel[REL_POS][wing - 1] = pos;
   update_body_extra(h);
+  return 0;
+}
+
+int lbm_get_body_rel(lbm_handle* h, double* rel) {
+  if (!h || !rel) return fail(h, LBM_E_INVALID, "null argument");
+  for (int q = 0; q < REL_FIELDS; q++)
+    for (int w = 0; w < h->W; w++) rel[q * h->W + w] = h->rel[q][w];
+  return 0;
+}
+
+int lbm_set_body_rel(lbm_handle* h, const double* rel) {
+  if (!h || !rel) return fail(h, LBM_E_INVALID, "null argument");
+  for (int q = 0; q < REL_FIELDS; q++)
+    for (int w = 0; w < h->W; w++) h->rel[q][w] = rel[q * h->W + w];
+  update_body_extra(h);
+  return 0;
+}
+
+int lbm_set_dynamics(lbm_handle* h, const lbm_dynamics* d) {
+  if (!h || !d) return fail(h, LBM_E_INVALID, "null argument");
+  if (d->enable && h->W == 0) return fail(h, LBM_E_INVALID, "rigid-body coupling needs at least one wing");
+  if (d->enable && !(d->solid_den > 0.0 && (!d->closed_boundary || d->area > 0.0)))
+    return fail(h, LBM_E_INVALID, "solid_den and area must be positive");
+  h->dyn = *d;
+  IbmParams& p = h->ip;
+  const double sc = h->cfg.scale, uu = h->cfg.uu;
+  p.dyn_on = d->enable ? 1 : 0;
+  p.dyn_rk4 = d->rk4 ? 1 : 0;
+  p.dyn_mass = d->closed_boundary ? d->solid_den * d->area / (sc * sc) : d->solid_den / (sc * sc);
+  p.dyn_p1 = d->par1 * uu * uu * sc * sc;
+  p.dyn_p2 = d->par2 * (sc * sc * sc * sc);
+  p.dyn_alphad = d->alphad;
   return 0;
 }
 
@@ -1283,8 +1429,10 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
     if (h->device != b->device) return bfail(b, LBM_E_INVALID, "instance %d lives on another device", e);
     if (h->cfg.motion_source == LBM_MOTION_FILE && !h->motion_ready && h->W > 0)
       return bfail(b, LBM_E_STATE, "instance %d: motion table not loaded", e);
-    if (h->W > 0 && (h->body_rel_pos[0] >= 1.0 || h->body_rel_pos[0] <= -1.0))
+    if (h->W > 0 && (h->rel[REL_POS][0] >= 1.0 || h->rel[REL_POS][0] <= -1.0))
       return bfail(b, LBM_E_STATE, "instance %d has a pending move_mesh; advance it alone", e);
+    if (h->dyn.enable)
+      return bfail(b, LBM_E_STATE, "instance %d has rigid-body coupling on; advance it alone", e);
     if (h->W > 0) {
       wings = true;
       ns = std::min(ns, h->nslot);

@@ -31,12 +31,21 @@ struct WingMotionState {
   double angle, omega, omgac, sa, sau, saac, sb, sbv, sbac;
 };
 
-// Error state and last-step forces (one instance per solver).
+// Fields of the rigid-body state body_rel_*, in restart-file order
+// (solution.f90:1663-1665).
+enum BodyRel : int {
+  REL_A, REL_A_PREV, REL_V, REL_V_PREV, REL_S, REL_S_PREV, REL_POS, REL_THETA_PREV, REL_THETA,
+  REL_TEMP_THETA_PREV, REL_TEMP_THETA, REL_FIELDS
+};
+
+// Error state, last-step forces and, with rigid-body coupling, the body
+// state that each step's force row advances (one instance per solver).
 struct DevStep {
   int error;
   int error_info;
   double force[MAXW][2];   // normalised per-wing force of the last step
   double moment[MAXW];
+  double rel[REL_FIELDS][MAXW];
 };
 
 // Rigid-frame state of one step, known on the host before the step runs
@@ -80,6 +89,11 @@ struct IbmParams {
   int ncols;
   int pad;
   double motion_dt;
+  // rigid-body coupling of wing 1 (boundary_condition.f90:1694-1719)
+  int dyn_on, dyn_rk4;
+  double dyn_mass;         // solid_den*area/scale^2 (closed boundary) or solid_den/scale^2
+  double dyn_p1, dyn_p2;   // par1*UU^2*scale^2, par2*scale^4
+  double dyn_alphad;
 };
 
 // Per-slot geometry arrays; slot s of an array X of per-slot length L starts

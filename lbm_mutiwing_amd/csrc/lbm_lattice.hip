@@ -186,8 +186,53 @@ int adj_collide_blocks_per_wing(const IbmParams& p) {
   return p.W > 0 ? std::min(ADJ_BLOCKS_PER_WING, (p.capw + ADJ_THREADS - 1) / ADJ_THREADS) : 1;
 }
 
+// Rigid-body update of wing 1 from the step's raw force (boundary_condition.
+// f90:1694-1719): fx = x force summed over all wings, mom = moment of wing 1.
+// RK4 (:3168-3202) with constant acceleration over the step, and the pitch
+// equation theta'' = p1 (theta + alphad) + p2 mom; or explicit Euler.
+__device__ void body_step(const IbmParams& p, double fx, double mom, double (*rel)[MAXW]) {
+  double* a = &rel[REL_A][0];
+  rel[REL_A_PREV][0] = *a;
+  *a = fx / p.dyn_mass;
+  const double v1 = rel[REL_V][0], s1 = rel[REL_S][0];
+  const double th1 = rel[REL_THETA][0], om1 = rel[REL_TEMP_THETA][0];
+  rel[REL_V_PREV][0] = v1;
+  rel[REL_S_PREV][0] = s1;
+  rel[REL_THETA_PREV][0] = th1;
+  rel[REL_TEMP_THETA_PREV][0] = om1;
+  const double acc = *a, p1 = p.dyn_p1, p2 = p.dyn_p2, ad = p.dyn_alphad;
+  if (!p.dyn_rk4) {
+    rel[REL_V][0] = acc + v1;
+    rel[REL_S][0] = s1 + v1;
+    rel[REL_POS][0] = rel[REL_POS][0] + v1;
+    const double th = th1 + om1;
+    rel[REL_THETA][0] = th;
+    rel[REL_TEMP_THETA][0] = om1 + p1 * (th + ad) + p2 * mom;
+    return;
+  }
+  // stage slopes of (s, v): (v1, a), (v1 + a/2, a), (v1 + a/2, a), (v1 + a, a)
+  const double ds = (v1 + 2.0 * (v1 + 0.5 * acc) + 2.0 * (v1 + 0.5 * acc) + (v1 + acc)) / 6.0;
+  rel[REL_S][0] = s1 + ds;
+  rel[REL_V][0] = v1 + (acc + 2.0 * acc + 2.0 * acc + acc) / 6.0;
+  rel[REL_POS][0] = rel[REL_POS][0] + ds;
+  // (theta, omega): q1 = omega slope at the start, r_i = omega' at stage i
+  const double pm = p2 * mom;
+  const double r1 = p1 * (th1 + ad) + pm;
+  const double q2 = om1 + 0.5 * r1;
+  const double r2 = p1 * (th1 + ad + 0.5 * om1) + pm;
+  const double q3 = om1 + 0.5 * r2;
+  const double r3 = p1 * (th1 + ad + 0.5 * q2) + pm;
+  const double q4 = om1 + r3;
+  const double r4 = p1 * (th1 + ad + q3) + pm;
+  rel[REL_THETA][0] = th1 + (om1 + 2.0 * q2 + 2.0 * q3 + q4) / 6.0;
+  rel[REL_TEMP_THETA][0] = om1 + (r1 + 2.0 * r2 + 2.0 * r3 + r4) / 6.0;
+}
+
 // Normalised force row and airfoil_motion row of one step into LDS arrays
-// (one thread); the per-wing force and moment of the step go to st.
+// (one thread); the per-wing force and moment of the step go to st.  With
+// rigid-body coupling the step's force first advances st->rel, and the
+// motion columns carry the body state after that update, as the reference's
+// row does (its IBM routine updates body_extra_* before the row is written).
 __device__ void compute_rows(const IbmParams& p, const GeoStep* gs, DevStep* st, const double (*raw)[3], int n,
                              double* row, double* mrow) {
   const bool ok = st->error == 0;
@@ -208,6 +253,23 @@ __device__ void compute_rows(const IbmParams& p, const GeoStep* gs, DevStep* st,
     F1 = F1 + wf[w][1];
   }
   for (int w = 0; w < p.W; w++) Mt = Mt + wmom[w];
+  double be_v[MAXW], be_s[MAXW];
+  for (int w = 0; w < p.W; w++) {
+    be_v[w] = gs->frame.body_extra_v[w];
+    be_s[w] = gs->frame.body_extra_s[w];
+  }
+  if (p.dyn_on && ok && p.W > 0) {
+    double fx = 0.0;
+    for (int w = 0; w < p.W; w++) fx = fx + raw[w][0];
+    body_step(p, fx, raw[0][2], st->rel);
+    double av = 0.0, as = 0.0;   // update_body_extra_from_rel (mode.f90:473-495)
+    for (int w = 0; w < p.W; w++) {
+      av = av + st->rel[REL_V][w];
+      as = as + st->rel[REL_S][w];
+      be_v[w] = av;
+      be_s[w] = as;
+    }
+  }
   const double time = p.W > 0 ? gs->time : (double)(n - 1) * p.dt;
   int col = 0;
   row[col++] = time;
@@ -221,7 +283,7 @@ __device__ void compute_rows(const IbmParams& p, const GeoStep* gs, DevStep* st,
   row[col++] = Mt;
   for (int w = 0; w < p.W; w++) {
     const WingMotionState& m = gs->wing[w];
-    row[col++] = p.wing_x0 + (double)w * p.wing_dx + m.sa + gs->frame.body_extra_s[w] / (1.0 / scale);
+    row[col++] = p.wing_x0 + (double)w * p.wing_dx + m.sa + be_s[w] / (1.0 / scale);
     row[col++] = m.sau;
     row[col++] = m.saac;
     row[col++] = p.wing_y0 + m.sb;
@@ -236,8 +298,8 @@ __device__ void compute_rows(const IbmParams& p, const GeoStep* gs, DevStep* st,
   for (int w = 0; w < p.W; w++) {
     const WingMotionState& m = gs->wing[w];
     mrow[c++] = 0.0;
-    mrow[c++] = gs->frame.body_extra_v[w] / UU;
-    mrow[c++] = p.wing_x0 + (double)w * p.wing_dx + m.sa + gs->frame.body_extra_s[w] / (1.0 / scale);
+    mrow[c++] = be_v[w] / UU;
+    mrow[c++] = p.wing_x0 + (double)w * p.wing_dx + m.sa + be_s[w] / (1.0 / scale);
   }
 }
 

@@ -38,9 +38,30 @@ _PROTOS = {
     "get_markers": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _dp]),
     "get_ibm": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _ip, _ip, _dp, _dp]),
     "set_body_rel_pos": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_double]),
+    "get_body_rel": (ctypes.c_int, [ctypes.c_void_p, _dp]),
+    "set_body_rel": (ctypes.c_int, [ctypes.c_void_p, _dp]),
+    "set_dynamics": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "set_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
     "last_error": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]),
 }
+
+
+BODY_REL_FIELDS = ("a", "a_prev", "v", "v_prev", "s", "s_prev", "pos", "theta_prev", "theta", "temp_theta_prev",
+                   "temp_theta")
+
+
+class Dynamics(ctypes.Structure):
+    """lbm_dynamics: rigid-body coupling of wing 1 (boundary_condition.f90:
+    1694-1719; constants main.f90:98, 122-151, whose values are the
+    defaults here, with the solver switched on)."""
+    _fields_ = [("enable", ctypes.c_int32), ("closed_boundary", ctypes.c_int32), ("rk4", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("solid_den", ctypes.c_double), ("area", ctypes.c_double),
+                ("par1", ctypes.c_double), ("par2", ctypes.c_double), ("alphad", ctypes.c_double)]
+
+    def __init__(self, enable=True, closed_boundary=True, rk4=True, solid_den=10.0, area=6.807946476488794e-2,
+                 par1=0.0, par2=0.0, alphad=10.0 * 3.1415926535897932384626433832795 / 180.0):
+        super().__init__(int(bool(enable)), int(bool(closed_boundary)), int(bool(rk4)), 0, float(solid_den),
+                         float(area), float(par1), float(par2), float(alphad))
 
 
 def bind(lib: ctypes.CDLL, prefix: str) -> None:
@@ -216,6 +237,23 @@ class Solver:
 
     def set_body_rel_pos(self, wing: int, pos: float) -> None:
         self._check(self._fn("set_body_rel_pos")(self._h, int(wing), float(pos)))
+
+    def body_rel(self) -> np.ndarray:
+        """body_rel_* as an (11, wing_count) array, rows in restart-file
+        order (BODY_REL_FIELDS)."""
+        out = np.zeros(11 * max(self.cfg.wing_count, 1))
+        self._check(self._fn("get_body_rel")(self._h, _p(out)))
+        return out[: 11 * self.cfg.wing_count].reshape(11, self.cfg.wing_count)
+
+    def set_body_rel(self, rel) -> None:
+        a = np.ascontiguousarray(np.asarray(rel, dtype=np.float64).reshape(11, self.cfg.wing_count))
+        self._check(self._fn("set_body_rel")(self._h, _p(a)))
+
+    def set_dynamics(self, dyn: Optional[Dynamics]) -> None:
+        """Switch the rigid-body coupling on (a :class:`Dynamics`) or off
+        (None)."""
+        d = dyn if dyn is not None else Dynamics(enable=False)
+        self._check(self._fn("set_dynamics")(self._h, ctypes.byref(d)))
 
     def tuning(self) -> int:
         """Lattice tile width chosen by the autotuner (HIP library only)."""

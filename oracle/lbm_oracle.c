@@ -38,6 +38,9 @@
 
 #define MAXW LBM_MAX_WINGS
 #define PAI 3.1415926535897932384626433832795
+/* fields of the rigid-body state, restart-file order (solution.f90:1663-1665) */
+enum { REL_A, REL_A_PREV, REL_V, REL_V_PREV, REL_S, REL_S_PREV, REL_POS, REL_THETA_PREV, REL_THETA,
+       REL_TEMP_THETA_PREV, REL_TEMP_THETA };
 
 struct orc {
     lbm_config cfg;
@@ -56,7 +59,9 @@ struct orc {
     int *adj_i, *adj_j, *adj_w;
     double angle[MAXW], omega[MAXW], omgac[MAXW], sa[MAXW], sau[MAXW],
         saac[MAXW], sb[MAXW], sbv[MAXW], sbac[MAXW];
-    double body_rel_pos[MAXW], body_rel_v[MAXW], body_rel_s[MAXW];
+    /* body_rel_* in restart-file order (solution.f90:1663-1665) */
+    double rel[LBM_BODY_REL_FIELDS][MAXW];
+    lbm_dynamics dyn;
     double body_extra_v[MAXW], body_extra_s[MAXW], body_extra_pos[MAXW];
     int movenum, flag2, n;
     double *table;
@@ -347,9 +352,9 @@ static void update_body_extra(orc_t *o)
         o->body_extra_pos[w] = 0.0;
     }
     for (int w = 0; w < o->W; w++) {
-        av = av + o->body_rel_v[w];
-        as = as + o->body_rel_s[w];
-        ap = ap + o->body_rel_pos[w];
+        av = av + o->rel[REL_V][w];
+        as = as + o->rel[REL_S][w];
+        ap = ap + o->rel[REL_POS][w];
         o->body_extra_v[w] = av;
         o->body_extra_s[w] = as;
         o->body_extra_pos[w] = ap;
@@ -993,6 +998,14 @@ static int immersed_boundary(orc_t *o)
     }
     free(ub); free(vb); free(amap); free(rows); free(rlen); free(us); free(vs);
     free(gx); free(gy);
+    if (o->dyn.enable && o->W > 0) { /* :1677-1719, then update_body_extra_from_rel (:1731) */
+        double fx = 0.0, st[LBM_BODY_REL_FIELDS];
+        for (int w = 0; w < o->W; w++) fx = fx + o->wing_force[w][0];
+        for (int q = 0; q < LBM_BODY_REL_FIELDS; q++) st[q] = o->rel[q][0];
+        orc_body_step(&o->dyn, scale, UU, fx, o->wing_moment[0], st);
+        for (int q = 0; q < LBM_BODY_REL_FIELDS; q++) o->rel[q][0] = st[q];
+        update_body_extra(o);
+    }
     return 0;
 }
 
@@ -1109,7 +1122,7 @@ int orc_reset(orc_t *o)
         o->stream_ready = 1;
     }
     for (int w = 0; w < MAXW; w++) {
-        o->body_rel_pos[w] = o->body_rel_v[w] = o->body_rel_s[w] = 0.0;
+        for (int q = 0; q < LBM_BODY_REL_FIELDS; q++) o->rel[q][w] = 0.0;
         o->wing_force[w][0] = o->wing_force[w][1] = 0.0;
         o->wing_moment[w] = 0.0;
     }
@@ -1161,15 +1174,15 @@ static int one_step(orc_t *o, double *frow, double *mrow)
     force_rows(o, time, frow, mrow);
     boundary_condition1(o);
     /* move_mesh check (main.f90:393-412) */
-    if (o->W > 0 && o->body_rel_pos[0] >= 1.0) {
+    if (o->W > 0 && o->rel[REL_POS][0] >= 1.0) {
         move_mesh(o, 1);
-        o->body_rel_pos[0] = o->body_rel_pos[0] - 1.0;
+        o->rel[REL_POS][0] = o->rel[REL_POS][0] - 1.0;
         update_body_extra(o);
         o->movenum = o->movenum + 1;
         o->flag2 = 3;
-    } else if (o->W > 0 && o->body_rel_pos[0] <= -1.0) {
+    } else if (o->W > 0 && o->rel[REL_POS][0] <= -1.0) {
         move_mesh(o, -1);
-        o->body_rel_pos[0] = o->body_rel_pos[0] + 1.0;
+        o->rel[REL_POS][0] = o->rel[REL_POS][0] + 1.0;
         update_body_extra(o);
         o->movenum = o->movenum - 1;
         o->flag2 = 3;
@@ -1319,9 +1332,96 @@ int orc_set_step(orc_t *o, int32_t n_next, int32_t movenum)
 int orc_set_body_rel_pos(orc_t *o, int32_t wing, double pos)
 {
     if (!o || wing < 1 || wing > o->W) return LBM_E_INVALID;
-    o->body_rel_pos[wing - 1] = pos;
+    o->rel[REL_POS][wing - 1] = pos;
     update_body_extra(o);
     return 0;
+}
+
+int orc_get_body_rel(orc_t *o, double *rel)
+{
+    if (!o || !rel) return LBM_E_INVALID;
+    for (int q = 0; q < LBM_BODY_REL_FIELDS; q++)
+        for (int w = 0; w < o->W; w++) rel[q * o->W + w] = o->rel[q][w];
+    return 0;
+}
+
+/* readflowfield's body_rel_* (solution.f90:1721-1851) */
+int orc_set_body_rel(orc_t *o, const double *rel)
+{
+    if (!o || !rel) return LBM_E_INVALID;
+    for (int q = 0; q < LBM_BODY_REL_FIELDS; q++)
+        for (int w = 0; w < o->W; w++) o->rel[q][w] = rel[q * o->W + w];
+    update_body_extra(o);
+    return 0;
+}
+
+int orc_set_dynamics(orc_t *o, const lbm_dynamics *d)
+{
+    if (!o || !d) return LBM_E_INVALID;
+    if (d->enable && o->W == 0) return fail(o, LBM_E_INVALID, "rigid-body coupling needs at least one wing");
+    if (d->enable && !(d->solid_den > 0.0 && (!d->closed_boundary || d->area > 0.0)))
+        return fail(o, LBM_E_INVALID, "solid_den and area must be positive");
+    o->dyn = *d;
+    return 0;
+}
+
+/* rk4 (boundary_condition.f90:3168-3202): s1..a1 are wing 1's s_prev, s,
+ * pos, v_prev, v, a; jd1..jsd its theta_prev, theta, temp_theta_prev,
+ * temp_theta; mom the moment of wing 1.  The k terms are evaluated in the
+ * reference's order, which reuses the same names for the pitch equation. */
+static void rk4(double s1, double *s, double *s2, double v1, double *v, double a1, double jd1, double *jd,
+                double jsd1, double *jsd, double p1, double p2, double alphad, double mom)
+{
+    double k11 = v1;
+    double k21 = a1;
+    double k12 = v1 + 0.5 * k21;
+    double k22 = a1;
+    double k13 = v1 + 0.5 * k22;
+    double k23 = a1;
+    double k14 = v1 + k23;
+    double k24 = a1;
+    *s = s1 + (k11 + 2.0 * k12 + 2.0 * k13 + k14) / 6.0;
+    *v = v1 + (k21 + 2.0 * k22 + 2.0 * k23 + k24) / 6.0;
+    *s2 = *s2 + (k11 + 2.0 * k12 + 2.0 * k13 + k14) / 6.0;
+    k11 = jsd1;
+    k21 = p1 * (jd1 + alphad) + p2 * mom;
+    k12 = jsd1 + 0.5 * k21;
+    k22 = p1 * (jd1 + alphad + 0.5 * k11) + p2 * mom;
+    k13 = jsd1 + 0.5 * k22;
+    k23 = p1 * (jd1 + alphad + 0.5 * k12) + p2 * mom;
+    k14 = jsd1 + k23;
+    k24 = p1 * (jd1 + alphad + k13) + p2 * mom;
+    *jd = jd1 + (k11 + 2.0 * k12 + 2.0 * k13 + k14) / 6.0;
+    *jsd = jsd1 + (k21 + 2.0 * k22 + 2.0 * k23 + k24) / 6.0;
+}
+
+/* The dynamics block of immersed_boundary_Wu (boundary_condition.f90:
+ * 1694-1719) on wing 1's state st (11 fields, restart order): fx is force(1),
+ * the raw x force summed over the wings (:1677-1680), mom wing_moment(1). */
+void orc_body_step(const lbm_dynamics *d, double scale, double uu, double fx, double mom, double *st)
+{
+    st[REL_A_PREV] = st[REL_A];
+    if (!d->closed_boundary)
+        st[REL_A] = fx / (d->solid_den / (scale * scale));
+    else
+        st[REL_A] = fx / (d->solid_den * d->area / (scale * scale));
+    st[REL_V_PREV] = st[REL_V];
+    st[REL_S_PREV] = st[REL_S];
+    st[REL_THETA_PREV] = st[REL_THETA];
+    st[REL_TEMP_THETA_PREV] = st[REL_TEMP_THETA];
+    if (!d->rk4) {
+        st[REL_V] = st[REL_A] + st[REL_V];
+        st[REL_S] = st[REL_S] + st[REL_V_PREV];
+        st[REL_POS] = st[REL_POS] + st[REL_V_PREV];
+        st[REL_THETA] = st[REL_THETA_PREV] + st[REL_TEMP_THETA_PREV];
+        st[REL_TEMP_THETA] = st[REL_TEMP_THETA_PREV] +
+                             d->par1 * uu * uu * scale * scale * (st[REL_THETA] + d->alphad) +
+                             d->par2 * (scale * scale * scale * scale) * mom;
+    } else {
+        rk4(st[REL_S_PREV], &st[REL_S], &st[REL_POS], st[REL_V_PREV], &st[REL_V], st[REL_A], st[REL_THETA_PREV],
+            &st[REL_THETA], st[REL_TEMP_THETA_PREV], &st[REL_TEMP_THETA], d->par1 * uu * uu * scale * scale,
+            d->par2 * (scale * scale * scale * scale), d->alphad, mom);
+    }
 }
 
 /* ------------------------------------------------------------ test hooks */

@@ -42,6 +42,9 @@ int orc_get_markers(orc_t *o, double *xb1, double *yb1, double *xb2,
 int orc_get_ibm(orc_t *o, int32_t capacity, int32_t *adjoint_ij,
                 int32_t *wing_id, double *wing_force, double *wing_moment);
 int orc_set_body_rel_pos(orc_t *o, int32_t wing, double pos);
+int orc_get_body_rel(orc_t *o, double *rel);
+int orc_set_body_rel(orc_t *o, const double *rel);
+int orc_set_dynamics(orc_t *o, const lbm_dynamics *d);
 int orc_set_step(orc_t *o, int32_t n_next, int32_t movenum);
 int orc_last_error(const orc_t *o, char *buf, size_t n);
 
@@ -49,6 +52,8 @@ int orc_last_error(const orc_t *o, char *buf, size_t n);
 void orc_mrt_matrices(double m[81], double minv[81]);  /* matm, nizhen */
 void orc_weights(double w[9]);                          /* D2Q9model   */
 double orc_delta_kernel(double disr);                   /* smooth=1 branch */
+/* The rigid-body update of one step on wing 1's 11 body_rel fields. */
+void orc_body_step(const lbm_dynamics *d, double scale, double uu, double fx, double mom, double *st);
 /* IBM system of the last step: A (nseg x nseg, column-major), rhs (nseg x 2)
  * and solution comj (nseg x 2); nseg = wing_count*(points_per_wing-1). */
 int orc_get_ibm_system(orc_t *o, double *a, double *rhs, double *sol);

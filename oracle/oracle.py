@@ -49,10 +49,19 @@ def load():
         lib.orc_get_ibm_nodes.argtypes = [ctypes.c_void_p, ctypes.c_int32, _dp, _dp, _dp, _dp, _dp]
         lib.orc_get_rates.restype = ctypes.c_int
         lib.orc_get_rates.argtypes = [ctypes.c_void_p, _dp]
+        lib.orc_body_step.restype = None
+        lib.orc_body_step.argtypes = [ctypes.c_void_p] + [ctypes.c_double] * 4 + [_dp]
         lib.orc_get_ibm_system.restype = ctypes.c_int
         lib.orc_get_ibm_system.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
         _lib = lib
     return _lib
+
+
+def body_step(dyn, scale: float, uu: float, fx: float, mom: float, st):
+    """One rigid-body update of wing 1's 11 body_rel fields (in place)."""
+    import numpy as np
+    assert st.dtype == np.float64 and st.shape == (11,) and st.flags.c_contiguous
+    load().orc_body_step(ctypes.byref(dyn), scale, uu, fx, mom, st.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
 
 
 def set_threads(n: int) -> int:
