@@ -33,6 +33,9 @@ FIELDS = {
     "restart": ("logical", False),
     "boundary_dir": ("str", "."), "motion_file": ("str", "motion_params.dat"), "output_dir": ("str", "output"),
     "restart_dir": ("str", ""),
+    "solve_dynamic_equations": ("logical", False), "closed_boundary": ("logical", True), "if_rk4": ("logical", True),
+    "solid_den": ("real", 10.0), "area_airfoil": ("real", 6.807946476488794e-2), "par1": ("real", 0.0),
+    "par2": ("real", 0.0), "alphad": ("real", 10.0 * 3.1415926535897932384626433832795 / 180.0),
 }
 
 _TOKEN = re.compile(r"'(?:[^']|'')*'|\"(?:[^\"]|\"\")*\"|[^\s,=]+=?|=")
@@ -87,6 +90,17 @@ def read_case(path: str) -> Dict[str, object]:
             index += 1
         i += 1
     return case
+
+
+def case_dynamics(case: Dict[str, object]):
+    """The rigid-body coupling of the case (a solver.Dynamics), or None when
+    solve_dynamic_equations is off (the reference's setting, main.f90:124-125)."""
+    from .solver import Dynamics
+    if not case["solve_dynamic_equations"]:
+        return None
+    return Dynamics(enable=True, closed_boundary=case["closed_boundary"], rk4=case["if_rk4"],
+                    solid_den=case["solid_den"], area=case["area_airfoil"], par1=case["par1"], par2=case["par2"],
+                    alphad=case["alphad"])
 
 
 def case_config(case: Dict[str, object], motion_source: int = MOTION_FILE) -> SolverConfig:

@@ -29,6 +29,12 @@ module lbm_hip
         type(c_ptr) :: xb0, yb0
     end type
 
+    ! rigid-body coupling of wing 1 (include/lbm_hip.h, lbm_dynamics)
+    type, bind(C) :: lbm_dynamics
+        integer(c_int32_t) :: enable, closed_boundary, rk4, reserved
+        real(c_double) :: solid_den, area, par1, par2, alphad
+    end type
+
     interface
         integer(c_int) function lbm_create(cfg, h) bind(C, name='lbm_create')
             import :: c_int, c_ptr, lbm_config
@@ -134,6 +140,24 @@ module lbm_hip
             type(c_ptr), value :: h
             integer(c_int32_t), value :: wing
             real(c_double), value :: pos
+        end function
+
+        integer(c_int) function lbm_get_body_rel(h, rel) bind(C, name='lbm_get_body_rel')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: h
+            real(c_double), intent(out) :: rel(*)
+        end function
+
+        integer(c_int) function lbm_set_body_rel(h, rel) bind(C, name='lbm_set_body_rel')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: h
+            real(c_double), intent(in) :: rel(*)
+        end function
+
+        integer(c_int) function lbm_set_dynamics(h, d) bind(C, name='lbm_set_dynamics')
+            import :: c_int, c_ptr, lbm_dynamics
+            type(c_ptr), value :: h
+            type(lbm_dynamics), intent(in) :: d
         end function
 
         integer(c_int) function lbm_last_error(h, buf, n) bind(C, name='lbm_last_error')

@@ -38,15 +38,22 @@ program lbm_host
     real(c_double) :: xc = 0d0, yc = 0d0, xp = 0d0, yp = 0d0
     character(len=256) :: boundary_dir = '.', motion_file = 'motion_params.dat', output_dir = 'output'
     character(len=256) :: restart_dir = ''
+    ! rigid-body coupling of wing 1: the reference's compile-time switches and
+    ! constants (main.f90:98, 122-151; if_rk4 boundary_condition.f90:1527-1529)
+    logical :: solve_dynamic_equations = .false., closed_boundary = .true., if_rk4 = .true.
+    real(c_double) :: solid_den = 10d0, area_airfoil = 6.807946476488794d-2, par1 = 0d0, par2 = 0d0
+    real(c_double) :: alphad = 10.00d0 * 3.1415926535897932384626433832795d0 / 180.0d0
     namelist /lbm_case/ nx, ny, scale, ox, oy, uu, re, s, wing_count, points_per_wing, &
         wing_x0, wing_dx, wing_y0, xc, yc, xp, yp, precision, boundary_dir, motion_file, &
-        output_dir, rows_per_call, print_every, checkpoint_every, restart, restart_dir
+        output_dir, rows_per_call, print_every, checkpoint_every, restart, restart_dir, &
+        solve_dynamic_equations, closed_boundary, if_rk4, solid_den, area_airfoil, par1, par2, alphad
 
     ! per-step row files: force.dat and airfoil_motion.dat (file mode) or the
     ! current episode's force_episode_<k>.dat / airfoil_motion_episode_<k>.dat
     integer, parameter :: UNIT_FORCE = 101, UNIT_MOTION = 102
 
     type(lbm_config) :: cfg
+    type(lbm_dynamics) :: dyn
     type(c_ptr) :: h = c_null_ptr
     real(c_double), allocatable, target :: xb0(:), yb0(:)
     character(len=512) :: case_file, text, path
@@ -103,6 +110,18 @@ program lbm_host
         cfg%motion_source = LBM_MOTION_FILE
     endif
     if (lbm_create(cfg, h) /= LBM_OK) call die('lbm_create: ' // trim(lbm_error_text(c_null_ptr)))
+    if (solve_dynamic_equations) then
+        dyn%enable = 1
+        dyn%closed_boundary = merge(1, 0, closed_boundary)
+        dyn%rk4 = merge(1, 0, if_rk4)
+        dyn%reserved = 0
+        dyn%solid_den = solid_den
+        dyn%area = area_airfoil
+        dyn%par1 = par1
+        dyn%par2 = par2
+        dyn%alphad = alphad
+        call check(lbm_set_dynamics(h, dyn), 'lbm_set_dynamics')
+    endif
 
     call make_dir(output_dir)
     if (online) then
@@ -161,7 +180,6 @@ contains
     subroutine run_file_mode()
         real(c_double), allocatable :: table(:, :)
         real(c_double) :: end_time, dt, rsum(LBM_MAX_FORCE_ROW), lrow(LBM_MAX_FORCE_ROW), err
-        real(c_double) :: brp(LBM_MAX_WINGS)
         type(checkpoint_ring) :: ring
         integer :: nrows, n_end, n_next, n_last, steps
         integer(c_int) :: rc
@@ -178,9 +196,8 @@ contains
             n_end = n_end + 1
         enddo
 
-        brp = 0d0
         if (restart) then
-            call restore_checkpoint(ring, brp, n_next)
+            call restore_checkpoint(ring, n_next)
         else
             call init_ring(ring)
             n_next = 1
@@ -197,7 +214,7 @@ contains
                     write(*, '(a,i0,a,es12.5)') 'step ', n_last, '  err ', err
             endif
             if (checkpoint_every > 0) then
-                if (mod(n_last, checkpoint_every) == 0) call save_checkpoint(n_last, ring, brp)
+                if (mod(n_last, checkpoint_every) == 0) call save_checkpoint(n_last, ring)
             endif
             n_next = n_last + 1
         enddo
@@ -216,11 +233,11 @@ contains
     end subroutine
 
     ! saveflowfield (solution.f90:1616-1719): the same checkpoint twice.
-    subroutine save_checkpoint(n, ring, brp)
+    subroutine save_checkpoint(n, ring)
         integer, intent(in) :: n
         type(checkpoint_ring), intent(inout) :: ring
-        real(c_double), intent(in) :: brp(:)
         real(c_double), allocatable :: f(:, :, :), f0(:, :, :), uf(:, :), vf(:, :), rho(:, :), pre(:, :)
+        real(c_double) :: rel(wing_count, 11)
         integer(c_int32_t) :: n_next, movenum, flag2, rl, nadj
         character(len=32), parameter :: names(2) = [character(len=32) :: 'saved_flow_feild', 'saved_flow_feild_copy']
         integer :: q
@@ -229,11 +246,12 @@ contains
         call check(lbm_get_state(h, n_next, movenum, flag2, rl, nadj), 'lbm_get_state')
         call check(lbm_get_fields(h, rho, uf, vf, pre), 'lbm_get_fields')
         call check(lbm_get_populations(h, f, f0), 'lbm_get_populations')
+        if (wing_count > 0) call check(lbm_get_body_rel(h, rel), 'lbm_get_body_rel')
         ! the f0 ring columns as the library holds them (move_mesh shifts them)
         ring%f0_left = f0(1, :, :)
         ring%f0_right = f0(nx, :, :)
         do q = 1, 2
-            call write_checkpoint(trim(output_dir) // '/' // trim(names(q)) // '.dat', n, wing_count, brp, &
+            call write_checkpoint(trim(output_dir) // '/' // trim(names(q)) // '.dat', n, wing_count, rel, &
                                   int(movenum), nx, ny, uf, vf, f, ring, ok)
             if (.not. ok) call die('cannot write ' // trim(output_dir) // '/' // trim(names(q)) // '.dat')
         enddo
@@ -242,26 +260,25 @@ contains
     ! readflowfield (solution.f90:1721-1787): the checkpoint in restart_dir
     ! (default output_dir), its copy if that fails; the next step is the
     ! saved step + 1 (main.f90:343-347).
-    subroutine restore_checkpoint(ring, brp, n_next)
+    subroutine restore_checkpoint(ring, n_next)
         type(checkpoint_ring), intent(out) :: ring
-        real(c_double), intent(out) :: brp(:)
         integer, intent(out) :: n_next
         real(c_double), allocatable :: f(:, :, :), f0(:, :, :), uf(:, :), vf(:, :)
+        real(c_double) :: rel(wing_count, 11)
         character(len=512) :: dir
-        integer :: cycle, movenum, w
+        integer :: cycle, movenum
 
         dir = restart_dir
         if (len_trim(dir) == 0) dir = output_dir
         allocate(f(nx, ny, 9), f0(nx, ny, 9), uf(nx, ny), vf(nx, ny))
-        brp = 0d0
-        call read_checkpoint(trim(dir) // '/saved_flow_feild.dat', wing_count, nx, ny, cycle, brp, movenum, &
+        call read_checkpoint(trim(dir) // '/saved_flow_feild.dat', wing_count, nx, ny, cycle, rel, movenum, &
                              uf, vf, f, ring, text, ok)
         if (ok) then
             write(*, '(a)') 'sucess!'
         else
             write(*, '(a)') trim(text)
             write(*, '(a)') '启动备用存档！'
-            call read_checkpoint(trim(dir) // '/saved_flow_feild_copy.dat', wing_count, nx, ny, cycle, brp, &
+            call read_checkpoint(trim(dir) // '/saved_flow_feild_copy.dat', wing_count, nx, ny, cycle, rel, &
                                  movenum, uf, vf, f, ring, text, ok)
             if (.not. ok) call die(text)
             write(*, '(a)') '读取保存的流场数据成功!'
@@ -272,9 +289,7 @@ contains
         f0(:, 1, :) = ring%f0_bottom
         f0(:, ny, :) = ring%f0_top
         call check(lbm_set_step(h, int(cycle + 1, c_int32_t), int(movenum, c_int32_t)), 'lbm_set_step')
-        do w = 1, wing_count
-            if (brp(w) /= 0d0) call check(lbm_set_body_rel_pos(h, int(w, c_int32_t), brp(w)), 'lbm_set_body_rel_pos')
-        enddo
+        if (wing_count > 0) call check(lbm_set_body_rel(h, rel), 'lbm_set_body_rel')
         call check(lbm_set_populations(h, f, f0), 'lbm_set_populations')
         n_next = cycle + 1
     end subroutine

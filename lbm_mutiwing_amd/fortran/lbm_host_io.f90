@@ -265,31 +265,28 @@ contains
 
     ! Checkpoint as saveflowfield writes it (solution.f90:1660-1686): the
     ! step, the wing count, the eleven rigid-body arrays of wing_count
-    ! doubles (body_rel_a, _a_prev, _v, _v_prev, _s, _s_prev, _pos,
-    ! _theta_prev, _theta, _temp_theta_prev, _temp_theta; only body_rel_pos
-    ! is state here, the rest of the rigid-body solver is disabled), movenum,
-    ! then u, v, f(1:9) node by node (i fastest), then per row j = 2..ny-1
-    ! leftf2, f0(1,j), rightf2, f0(nx,j), then per column i = 1..nx upf2,
-    ! f0(i,ny), downf2, f0(i,1).  Integers are 4 bytes, reals 8, no record
-    ! markers.
-    subroutine write_checkpoint(path, cycle, wing_count, body_rel_pos, movenum, nx, ny, u, v, f, ring, ok)
+    ! doubles (body_rel(:, 1..11) = body_rel_a, _a_prev, _v, _v_prev, _s,
+    ! _s_prev, _pos, _theta_prev, _theta, _temp_theta_prev, _temp_theta, the
+    ! layout of lbm_get_body_rel), movenum, then u, v, f(1:9) node by node
+    ! (i fastest), then per row j = 2..ny-1 leftf2, f0(1,j), rightf2,
+    ! f0(nx,j), then per column i = 1..nx upf2, f0(i,ny), downf2, f0(i,1).
+    ! Integers are 4 bytes, reals 8, no record markers.
+    subroutine write_checkpoint(path, cycle, wing_count, body_rel, movenum, nx, ny, u, v, f, ring, ok)
         character(len=*), intent(in) :: path
         integer, intent(in) :: cycle, wing_count, movenum, nx, ny
-        real(c_double), intent(in) :: body_rel_pos(*), u(nx, ny), v(nx, ny), f(nx, ny, 9)
+        real(c_double), intent(in) :: body_rel(wing_count, 11), u(nx, ny), v(nx, ny), f(nx, ny, 9)
         type(checkpoint_ring), intent(in) :: ring
         logical, intent(out) :: ok
-        real(c_double), allocatable :: rel(:, :), buf(:, :)
+        real(c_double), allocatable :: buf(:, :)
         integer :: fu, ios, i, j
 
         ok = .false.
         open(newunit=fu, file=trim(path), access='stream', form='unformatted', status='replace', &
              action='write', iostat=ios)
         if (ios /= 0) return
-        allocate(rel(max(1, wing_count), 11), buf(11, nx))
-        rel = 0d0
-        if (wing_count > 0) rel(1:wing_count, 7) = body_rel_pos(1:wing_count)
+        allocate(buf(11, nx))
         write(fu, iostat=ios) int(cycle, c_int32_t), int(wing_count, c_int32_t)
-        if (wing_count > 0 .and. ios == 0) write(fu, iostat=ios) rel(1:wing_count, :)
+        if (wing_count > 0 .and. ios == 0) write(fu, iostat=ios) body_rel
         if (ios == 0) write(fu, iostat=ios) int(movenum, c_int32_t)
         do j = 1, ny
             if (ios /= 0) exit
@@ -315,15 +312,15 @@ contains
     ! readflowfield (solution.f90:1721-1787): the checkpoint of a case of
     ! wing_count wings on an nx x ny lattice.  A missing file, a file of the
     ! wrong size or another wing count fails with a message.
-    subroutine read_checkpoint(path, wing_count, nx, ny, cycle, body_rel_pos, movenum, u, v, f, ring, msg, ok)
+    subroutine read_checkpoint(path, wing_count, nx, ny, cycle, body_rel, movenum, u, v, f, ring, msg, ok)
         character(len=*), intent(in) :: path
         integer, intent(in) :: wing_count, nx, ny
         integer, intent(out) :: cycle, movenum
-        real(c_double), intent(out) :: body_rel_pos(*), u(nx, ny), v(nx, ny), f(nx, ny, 9)
+        real(c_double), intent(out) :: body_rel(wing_count, 11), u(nx, ny), v(nx, ny), f(nx, ny, 9)
         type(checkpoint_ring), intent(out) :: ring
         character(len=*), intent(out) :: msg
         logical, intent(out) :: ok
-        real(c_double), allocatable :: rel(:, :), buf(:, :)
+        real(c_double), allocatable :: buf(:, :)
         integer(c_int32_t) :: i4, w4, m4
         integer(c_int64_t) :: fsize
         logical :: exists
@@ -352,9 +349,9 @@ contains
             msg = '续算文件中的翼数量与当前编译设置不一致'
             return
         endif
-        allocate(rel(max(1, wing_count), 11), buf(11, nx))
-        rel = 0d0
-        if (ios == 0 .and. wing_count > 0) read(fu, iostat=ios) rel(1:wing_count, :)
+        allocate(buf(11, nx))
+        body_rel = 0d0
+        if (ios == 0 .and. wing_count > 0) read(fu, iostat=ios) body_rel
         if (ios == 0) read(fu, iostat=ios) m4
         do j = 1, ny
             if (ios /= 0) exit
@@ -391,7 +388,6 @@ contains
         ring%f0_right(ny, :) = ring%f0_top(nx, :)
         cycle = i4
         movenum = m4
-        if (wing_count > 0) body_rel_pos(1:wing_count) = rel(1:wing_count, 7)
         ok = .true.
     end subroutine
 

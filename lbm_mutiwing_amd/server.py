@@ -75,15 +75,20 @@ class EpisodeServer:
         self.clock = clock
         self.save_hook = save_hook
         self.configs: Dict[object, SolverConfig] = {}
+        self.dynamics: Dict[object, object] = {}   # client -> solver.Dynamics (rigid-body coupling) or None
         self.sessions: Dict[object, Session] = {}
         self.waiting: deque = deque()          # (client, message, submitted) first RESETs
         self.jobs: List = []
         self.replies: List[Reply] = []
 
     # ---------------------------------------------------------------- clients
-    def connect(self, client, cfg: SolverConfig) -> None:
-        """Register a client and the configuration of its episodes."""
+    def connect(self, client, cfg: SolverConfig, dynamics=None) -> None:
+        """Register a client and the configuration of its episodes; with
+        ``dynamics`` (a solver.Dynamics) its episodes couple wing 1 to the
+        flow.  Such an episode steps one step at a time, so the scheduler
+        advances it on its own, beside the batched launches."""
         self.configs[client] = cfg
+        self.dynamics[client] = dynamics
 
     def submit(self, client, message: str, submitted: Optional[float] = None) -> None:
         t = self.clock() if submitted is None else submitted
@@ -131,6 +136,8 @@ class EpisodeServer:
                 return
             self.waiting.popleft()
             solver = Solver(cfg, self.lib)
+            if self.dynamics.get(client) is not None:
+                solver.set_dynamics(self.dynamics[client])
             sess = Session(solver, self.save_hook)
             self.sessions[client] = sess
             self.used += need
@@ -147,10 +154,22 @@ class EpisodeServer:
         self._admit()
         if self.jobs:
             groups: Dict[Tuple[int, int], list] = {}
+            alone = []
             for job in self.jobs:
                 c = min(self.chunk, job.remaining())
-                groups.setdefault((job.session.cfg.precision, c), []).append(job)
+                if self.dynamics.get(job.tag) is not None:
+                    alone.append((job, c))
+                else:
+                    groups.setdefault((job.session.cfg.precision, c), []).append(job)
             failed = []
+            for j, c in alone:
+                s = j.session.solver
+                try:
+                    r = s.advance(s.state()["n_next"] + c - 1)
+                except LbmError as e:
+                    failed.append((j, str(e)))
+                else:
+                    j.add(r)
             for (_, c), jobs in groups.items():
                 results = self.batch.advance([j.session.solver for j in jobs], c, partial=True)
                 for j, r in zip(jobs, results):

@@ -112,10 +112,11 @@ class RouterFront:
 
 
 def serve(front: RouterFront, cfg_for_client, hbm_budget: int, chunk: int, save_dir: Optional[str] = None,
-          exit_after: int = 0) -> int:
+          exit_after: int = 0, dynamics=None) -> int:
     """Run the episode server behind ``front`` until ``exit_after`` clients
-    have had their STOP answered (0: forever).  Returns the number of replies
-    sent."""
+    have had their STOP answered (0: forever); every client's episodes use
+    ``dynamics`` (rigid-body coupling, a solver.Dynamics, or None).  Returns
+    the number of replies sent."""
     from . import formats
     from .server import EpisodeServer
     hooks: Dict[bytes, object] = {}
@@ -139,7 +140,7 @@ def serve(front: RouterFront, cfg_for_client, hbm_budget: int, chunk: int, save_
                     ident, text = got
                     if ident not in order:
                         order[ident] = len(order) + 1
-                        srv.connect(ident, cfg_for_client(ident))
+                        srv.connect(ident, cfg_for_client(ident), dynamics)
                         if save_dir:
                             hooks[ident] = formats.tecplot_save_hook(
                                 os.path.join(save_dir, f"client_{order[ident]}"))
@@ -156,7 +157,7 @@ def serve(front: RouterFront, cfg_for_client, hbm_budget: int, chunk: int, save_
 
 def main(argv=None) -> int:
     from .build import zmq_library
-    from .case import case_config, read_case
+    from .case import case_config, case_dynamics, read_case
     from .config import MOTION_STREAM
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("case", help="namelist file with the group /lbm_case/")
@@ -171,7 +172,7 @@ def main(argv=None) -> int:
     print(f"episode server on {args.endpoint}: RESET / STEP / SAVE / STOP from any number of clients", flush=True)
     try:
         serve(front, lambda ident: base, int(args.hbm_gb * (1 << 30)), args.chunk,
-              save_dir=str(case["output_dir"]), exit_after=args.exit_after)
+              save_dir=str(case["output_dir"]), exit_after=args.exit_after, dynamics=case_dynamics(case))
     finally:
         front.close()
     return 0

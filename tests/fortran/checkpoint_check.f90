@@ -3,7 +3,7 @@
 !       writes a checkpoint of known values (all exact in binary):
 !       u = i + j/1024, v = -i - j/2048, f(k) = k + i/1024 + j/2^20,
 !       ring record r (1..8 in file order) = 16 r + k + (i or j)/1024,
-!       body_rel_pos(w) = w/2
+!       body_rel(w, q) = q + w/8
 !   checkpoint_check copy <in> <out> <nx> <ny> <wings>
 !       read_checkpoint <in>, then write_checkpoint <out>; prints the
 !       error text and exits 1 if the read fails
@@ -14,7 +14,7 @@ program checkpoint_check
     character(len=512) :: mode, path, path2, arg, msg
     integer :: nx, ny, w, cycle, movenum, i, j, k
     real(c_double), allocatable :: u(:, :), v(:, :), f(:, :, :)
-    real(c_double) :: brp(16)
+    real(c_double), allocatable :: rel(:, :)
     type(checkpoint_ring) :: ring
     logical :: ok
 
@@ -52,23 +52,26 @@ program checkpoint_check
                 ring%f0_bottom(i, k) = 128d0 + k + i / 1024d0
             enddo
         enddo
-        do i = 1, 16
-            brp(i) = 0.5d0 * i
+        allocate(rel(w, 11))
+        do k = 1, 11
+            do i = 1, w
+                rel(i, k) = dble(k) + dble(i) / 8d0
+            enddo
         enddo
-        call write_checkpoint(path, cycle, w, brp, movenum, nx, ny, u, v, f, ring, ok)
+        call write_checkpoint(path, cycle, w, rel, movenum, nx, ny, u, v, f, ring, ok)
         if (.not. ok) stop 1
     else
         call get_command_argument(3, path2)
         call get_command_argument(4, arg); read(arg, *) nx
         call get_command_argument(5, arg); read(arg, *) ny
         call get_command_argument(6, arg); read(arg, *) w
-        allocate(u(nx, ny), v(nx, ny), f(nx, ny, 9))
-        call read_checkpoint(path, w, nx, ny, cycle, brp, movenum, u, v, f, ring, msg, ok)
+        allocate(u(nx, ny), v(nx, ny), f(nx, ny, 9), rel(w, 11))
+        call read_checkpoint(path, w, nx, ny, cycle, rel, movenum, u, v, f, ring, msg, ok)
         if (.not. ok) then
             write(*, '(a)') trim(msg)
             stop 1
         endif
-        call write_checkpoint(path2, cycle, w, brp, movenum, nx, ny, u, v, f, ring, ok)
+        call write_checkpoint(path2, cycle, w, rel, movenum, nx, ny, u, v, f, ring, ok)
         if (.not. ok) stop 1
     endif
 end program checkpoint_check

@@ -130,8 +130,7 @@ def test_checkpoint_layout_of_fortran_writer(checkpoint_check, tmp_path):
     assert path.stat().st_size == formats.checkpoint_bytes(W, nx, ny)
     ck = formats.read_checkpoint(str(path), W, nx, ny)
     assert (ck.cycle, ck.movenum) == (300, -2)
-    expect_rel = np.zeros((11, W))
-    expect_rel[6] = [0.5, 1.0]
+    expect_rel = np.arange(1, 12)[:, None] + np.arange(1, W + 1)[None, :] / 8.0
     assert np.array_equal(ck.body_rel, expect_rel)
     i = np.arange(1, nx + 1)[None, :]
     j = np.arange(1, ny + 1)[:, None]
@@ -158,7 +157,7 @@ def test_checkpoint_python_fortran_round_trip(checkpoint_check, tmp_path):
     ring = formats.ring_from_populations(rng.normal(size=(9, ny, nx)))
     ck = formats.Checkpoint(cycle=12345, movenum=3, body_rel=np.zeros((11, W)), u=rng.normal(size=(ny, nx)),
                             v=rng.normal(size=(ny, nx)), f=f, ring=ring)
-    ck.body_rel[6] = [0.25, -0.5, 0.0]
+    ck.body_rel[:] = rng.normal(size=(11, W))
     a, b = tmp_path / "a.dat", tmp_path / "b.dat"
     formats.write_checkpoint(str(a), ck)
     r = subprocess.run([checkpoint_check, "copy", str(a), str(b), str(nx), str(ny), str(W)],

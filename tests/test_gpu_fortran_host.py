@@ -247,3 +247,45 @@ def test_file_mode_checkpoint_and_restart(host, tmp_path):
         assert ("启动备用存档" in out) == damage, out
         assert (tmp_path / name / "force.dat").read_text().splitlines() == full_f[100:], name
         assert (tmp_path / name / "airfoil_motion.dat").read_text().splitlines() == full_m[100:], name
+
+
+def test_file_mode_dynamics_checkpoint_and_restart(host, tmp_path):
+    """Rigid-body coupling switched on from the namelist (the reference's
+    solve_dynamic_equations and constants, main.f90:98, 122-151): the host's
+    force.dat equals the library's rows with the same coupling, the
+    checkpoint at step 60 holds the library's full body state
+    (solution.f90:1663-1665), and a restart from it continues with the same
+    records as the uninterrupted run."""
+    from lbm_mutiwing_amd.solver import Dynamics
+    cfg = preset("small")
+    n = 100                                   # one checkpoint, at step 60
+    rows = sinusoidal_motion_table(cfg, seeded_motions(cfg, 23), n)
+    _write_case(tmp_path, cfg, rows)
+    dyn_nml = ", solve_dynamic_equations=.true., solid_den=50.0, par1=-2.0, par2=50.0"
+    (tmp_path / "full.nml").write_text(_namelist(cfg, out_dir="full", extra=dyn_nml + ", checkpoint_every=60"))
+    print("fortran host: file mode with rigid-body coupling, checkpoint at step 60", flush=True)
+    r = subprocess.run([host, "full.nml"], cwd=tmp_path, capture_output=True, timeout=300)
+    assert r.returncode == 0, r.stdout.decode() + r.stderr.decode()
+
+    cfg = _read_case(tmp_path, cfg)
+    table = formats.load_motion_params(str(tmp_path / "motion_params.dat"), cfg.wing_count)
+    g = Solver(cfg)
+    g.set_motion_table(table)
+    g.set_dynamics(Dynamics(solid_den=50.0, par1=-2.0, par2=50.0))
+    r60 = g.advance(60, want_rows=True)
+    ck = formats.read_checkpoint(str(tmp_path / "full" / "saved_flow_feild.dat"), cfg.wing_count, cfg.nx, cfg.ny)
+    assert ck.cycle == 60
+    assert np.array_equal(ck.body_rel, g.body_rel())
+    assert ck.body_rel[2, 0] != 0.0 and ck.body_rel[4, 0] != 0.0     # v and s of wing 1 moved
+    r_end = g.advance(n, want_rows=True)
+    lib_rows = np.concatenate([r60.force_rows, r_end.force_rows])
+    fr = _read_f15(tmp_path / "full" / "force.dat", cfg.row_len)
+    assert np.all(np.abs(fr - lib_rows) <= F15_8_HALF_ULP * 1.01)
+
+    full_f = (tmp_path / "full" / "force.dat").read_text().splitlines()
+    (tmp_path / "rs.nml").write_text(
+        _namelist(cfg, out_dir="rs", extra=dyn_nml + ", checkpoint_every=60, restart=.true., restart_dir='full'"))
+    print("fortran host: restart with rigid-body coupling", flush=True)
+    r = subprocess.run([host, "rs.nml"], cwd=tmp_path, capture_output=True, timeout=300)
+    assert r.returncode == 0, r.stdout.decode() + r.stderr.decode()
+    assert (tmp_path / "rs" / "force.dat").read_text().splitlines() == full_f[60:]

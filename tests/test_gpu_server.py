@@ -29,12 +29,12 @@ def script(cfg, seed, steps=3, seg=40):
     return cmds
 
 
-def serve(lib, clients, budget, chunk=16, start_after=None):
+def serve(lib, clients, budget, chunk=16, start_after=None, dynamics=None):
     """clients: {name: (cfg, cmds)}; a client submits its next command as soon
     as it has the previous reply; client c starts after start_after[c] polls."""
     srv = EpisodeServer(budget, chunk=chunk, lib=lib)
     for name, (cfg, _) in clients.items():
-        srv.connect(name, cfg)
+        srv.connect(name, cfg, (dynamics or {}).get(name))
     nxt = {name: 0 for name in clients}
     busy = {name: False for name in clients}
     replies = {name: [] for name in clients}
@@ -78,6 +78,33 @@ def test_replies_same_alone_and_mixed(lib, precision):
         assert mixed[n] == alone[n], n
         assert alone[n][0].startswith("FORCE ") and alone[n][-2].startswith("STATUS FLOW_SAVED F")
         assert alone[n][-1] == "STATUS STOPPING"
+
+
+def test_dynamics_client_same_alone_and_mixed(lib):
+    """A client whose episodes couple wing 1 to the flow steps on its own
+    beside the batch: its replies do not depend on the other clients, and
+    equal those of its solver advanced directly."""
+    from lbm_mutiwing_amd.protocol import Session, run_job_alone
+    from lbm_mutiwing_amd.solver import Dynamics, Solver
+    dyn = Dynamics(solid_den=50.0, par1=-2.0, par2=50.0)
+    cfgs = {"d": episode_config(256, 2, 32, 100.0, PRECISION_FP64, MOTION_STREAM),
+            "p": episode_config(320, 1, 40, 300.0, PRECISION_FP64, MOTION_STREAM)}
+    clients = {n: (cfg, script(cfg, 10 + i)) for i, (n, cfg) in enumerate(cfgs.items())}
+    big = 1 << 40
+    alone = serve(lib, {"d": clients["d"]}, big, dynamics={"d": dyn})[0]["d"]
+    mixed, _ = serve(lib, clients, big, start_after={"p": 2}, dynamics={"d": dyn})
+    assert mixed["d"] == alone
+    plain = serve(lib, {"d": clients["d"]}, big)[0]["d"]
+    assert plain[1] != alone[1]                # the coupling changes the forces
+    s = Solver(cfgs["d"], lib)
+    s.set_dynamics(dyn)
+    sess = Session(s)
+    direct = []
+    for cmd in clients["d"][1][:-2]:
+        reply, job = sess.receive(cmd, 0.0)
+        direct.append(run_job_alone(job, 16) if job is not None else reply)   # the chunks of serve()
+    assert direct == alone[:-2]
+    s.close()
 
 
 def test_admission_waits_for_hbm(lib):

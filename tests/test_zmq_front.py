@@ -36,6 +36,19 @@ def test_read_case_forms(tmp_path):
         read_case(str(p))
 
 
+def test_read_case_dynamics(tmp_path):
+    from lbm_mutiwing_amd.case import case_dynamics
+    p = tmp_path / "case.nml"
+    p.write_text("&lbm_case nx=64, ny=32 /\n")
+    assert case_dynamics(read_case(str(p))) is None          # the reference's setting
+    p.write_text("&lbm_case nx=64, ny=32, solve_dynamic_equations=.true., if_rk4=.false.,\n"
+                 "  solid_den=2.5d1, par1=-1.0 /\n")
+    d = case_dynamics(read_case(str(p)))
+    assert (d.enable, d.closed_boundary, d.rk4) == (1, 1, 0)
+    assert d.solid_den == 25.0 and d.area == 6.807946476488794e-2 and d.par1 == -1.0 and d.par2 == 0.0
+    assert d.alphad == 10.0 * 3.1415926535897932384626433832795 / 180.0
+
+
 @pytest.mark.skipif(not _have_zmq(), reason="libzmq not in this image")
 def test_router_front_envelopes():
     from lbm_mutiwing_amd.zmq_server import RouterFront
