@@ -32,6 +32,12 @@ Two more workloads, each printing its own JSON line:
                 FP32-shifted storage; the same commands are then served in
                 FP64 and every FORCE value of the FP32 run is compared with
                 the FP64 one.
+  --mode dynamics  the single-instance workload with wing 1 coupled to the
+                flow (the reference's rigid-body solver switched on with its
+                own constants: solid density 10, closed boundary, RK4); each
+                step's geometry waits for the previous step's force, so the
+                steps run one at a time; the same instance's prescribed-motion
+                step is timed beside it.
 """
 from __future__ import annotations
 
@@ -51,7 +57,7 @@ BYTES_PER_NODE = {"fp64": 144.0,   # 9 population reads + 9 writes of 8 B
 
 def parse():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--mode", default="single", choices=["single", "batch", "serve", "mix"])
+    ap.add_argument("--mode", default="single", choices=["single", "batch", "serve", "mix", "dynamics"])
     ap.add_argument("--episodes", type=int, default=None, help="batch: 64, serve: 32, mix: 16")
     ap.add_argument("--substeps", type=int, default=1000, help="steps per STEP (env-step)")
     ap.add_argument("--env-steps", type=int, default=4, help="serve: STEPs per episode")
@@ -406,6 +412,43 @@ def run_mix(args):
     }
 
 
+def run_dynamics(args):
+    from lbm_mutiwing_amd import sinusoidal_motion_table
+    from lbm_mutiwing_amd.solver import BODY_REL_FIELDS, Dynamics, Solver
+    cfg, mot = workload(args.config, args.seed, args.precision)
+    steps, warmup = args.steps, args.warmup
+    table = sinusoidal_motion_table(cfg, mot, warmup + 2 * steps + 2)
+    out_ms = {}
+    for name, dyn in (("prescribed", None), ("coupled", Dynamics())):
+        s = Solver(cfg)
+        s.set_motion_table(table)
+        s.set_dynamics(dyn)
+        s.advance(warmup)
+        t0 = time.perf_counter()
+        r = s.advance(warmup + steps)
+        out_ms[name] = (time.perf_counter() - t0) * 1e3 / steps
+        note(f"dynamics: {name} {out_ms[name]:.4f} ms/step")
+        if dyn is not None:
+            rel = s.body_rel()[:, 0]
+            body = {k: float(rel[i]) for i, k in enumerate(BODY_REL_FIELDS) if k in ("a", "v", "s", "pos")}
+            movenum = s.state()["movenum"]
+        s.close()
+    nodes = cfg.nx * cfg.ny
+    return {
+        "metric": f"MLUPS (whole time step incl. IBM and rigid-body coupling), {args.precision.upper()} single instance",
+        "value": round(nodes / (out_ms["coupled"] * 1e-3) / 1e6, 1),
+        "unit": "MLUPS", "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "ms_per_step": round(out_ms["coupled"], 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64" if args.precision == "fp64" else "f32",
+        "data": "synthetic (seeded sinusoidal pitch/heave motion table, generated NACA0012 markers)",
+        "config": {"workload": f"{args.config}: {cfg.nx}x{cfg.ny}, {cfg.wing_count} wings x {cfg.points_per_wing} "
+                               "markers, wing 1 coupled (solid density 10, closed boundary, RK4)",
+                   "seed": args.seed},
+        "prescribed_ms_per_step": round(out_ms["prescribed"], 4),
+        "body_after_warmup_plus_steps": body, "movenum": movenum,
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -421,6 +464,9 @@ def main():
         return
     if args.mode == "mix":
         print(json.dumps(run_mix(args)), flush=True)
+        return
+    if args.mode == "dynamics":
+        print(json.dumps(run_dynamics(args)), flush=True)
         return
     from lbm_mutiwing_amd import sinusoidal_motion_table
     from lbm_mutiwing_amd.solver import TimedSolver

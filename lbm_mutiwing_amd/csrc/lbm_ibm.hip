@@ -374,16 +374,15 @@ __device__ __forceinline__ double s_entry(const int* box, const double* phx, con
   return sx * sy;
 }
 
-// In-place S = L D L^T of the lower band (band[r*(bw+1) + d] = S(r, r-d)) by
-// one wave: afterwards band[k*(bw+1)] = D_k, band[(k+i)*(bw+1) + i] = L(k+i, k),
-// invd[k] = 1/D_k.  Step k updates S(k+i, k+j) -= (S(k+i, k) / D_k) S(k+j, k)
+// In-place S = L D L^T of the lower band (band[r*Dd + d] = S(r, r-d), row
+// stride Dd > bw) by one wave: afterwards band[k*Dd] = D_k,
+// band[(k+i)*Dd + i] = L(k+i, k), invd[k] = 1/D_k.  Step k updates S(k+i, k+j) -= (S(k+i, k) / D_k) S(k+j, k)
 // for 1 <= j <= i <= bw: the bw(bw+1)/2 updates write distinct entries and
 // read only column k, so each lane takes up to five (i, j) pairs of its own
 // and a step costs one round of LDS updates.
 constexpr int LDLT_PAIRS_PER_LANE = (BMAX * (BMAX + 1) / 2 + 63) / 64;
 
-__device__ void band_ldlt(double* band, double* invd, int n, int bw, int lane, DevStep* st, int info) {
-  const int Dd = bw + 1;
+__device__ void band_ldlt(double* band, double* invd, int n, int bw, int Dd, int lane, DevStep* st, int info) {
   const int npairs = bw * (bw + 1) / 2;
   int pi[LDLT_PAIRS_PER_LANE], pj[LDLT_PAIRS_PER_LANE];  // pair q = i(i-1)/2 + j-1; i = 0: none
 #pragma unroll
@@ -417,10 +416,13 @@ __device__ void band_ldlt(double* band, double* invd, int n, int bw, int lane, D
 // Column col of S^-1 = L^-T D^-1 L^-1 e_col: forward substitution with a
 // sliding window of the last BW values, t = D^-1 z written to the output
 // column, then backward substitution over the stored t (read ahead in chunks).
-// BW is a compile-time bound >= the band half-width bw.
+// The band is stored with the compile-time row stride BW + 1 (BW >= bw), its
+// entries beyond bw and BW rows past the last one zero: every load of a row
+// has a constant offset and no guard, so the loads of a row issue together
+// (a term with a zero L entry subtracts an exact zero).
 template <int BW>
-__device__ void inverse_column(const double* band, const double* invd, int n, int bw, int col, double* S) {
-  const int Dd = bw + 1;
+__device__ void inverse_column(const double* band, const double* invd, int n, int col, double* S) {
+  constexpr int Dd = BW + 1;
   double win[BW];
 #pragma unroll
   for (int d = 0; d < BW; d++) win[d] = 0.0;
@@ -428,8 +430,7 @@ __device__ void inverse_column(const double* band, const double* invd, int n, in
     double z = m == col ? 1.0 : 0.0;
     const double* Lr = band + m * Dd;
 #pragma unroll
-    for (int d = 1; d <= BW; d++)
-      if (d <= bw) z -= Lr[d] * win[d - 1];
+    for (int d = 1; d <= BW; d++) z -= Lr[d] * win[d - 1];
 #pragma unroll
     for (int d = BW - 1; d >= 1; d--) win[d] = win[d - 1];
     win[0] = z;
@@ -450,9 +451,9 @@ __device__ void inverse_column(const double* band, const double* invd, int n, in
       const int m = m0 - u;
       if (m < 0) break;
       double x = tv[u];
+      const double* Lc = band + m * Dd;   // L(m + d, m) = Lc[d * (Dd + 1)]
 #pragma unroll
-      for (int d = 1; d <= BW; d++)
-        if (d <= bw && m + d < n) x -= band[(m + d) * Dd + d] * win[d - 1];
+      for (int d = 1; d <= BW; d++) x -= Lc[d * (Dd + 1)] * win[d - 1];
 #pragma unroll
       for (int d = BW - 1; d >= 1; d--) win[d] = win[d - 1];
       win[0] = x;
@@ -493,32 +494,35 @@ __device__ __forceinline__ void geo_solve_body(const IbmParams& p, const GeoBuff
     if (t == 0) raise_error(st, DERR_BAND, bw);
     return;
   }
-  const int Dd = bw + 1;
-  double* band = lds_s;
-  double* invd = band + (size_t)n * Dd;
+  // row stride: the compile-time band bound of the inverse below
+  const int BWc = bw <= 8 ? 8 : (bw <= 12 ? 12 : (bw <= 16 ? 16 : BMAX));
+  const int Dd = BWc + 1;
+  double* band = lds_s;                          // (n + BWc) rows of Dd
+  double* invd = band + (size_t)(n + BWc) * Dd;
   const size_t base = (size_t)slot * p.nseg + (size_t)w * n;
   const int* box = G.box + base * 4;
   const double* phx = G.phx + base * BOXW;
   const double* phy = G.phy + base * BOXW;
-  for (int e = t; e < n * Dd; e += GEO_THREADS) {
+  for (int e = t; e < (n + BWc) * Dd; e += GEO_THREADS) {
     const int pp = e / Dd, d = e - pp * Dd;
-    band[e] = pp - d >= 0 ? s_entry(box, phx, phy, seg_of_pos(pp, n), seg_of_pos(pp - d, n)) : 0.0;
+    band[e] = pp < n && d <= bw && pp - d >= 0 ? s_entry(box, phx, phy, seg_of_pos(pp, n), seg_of_pos(pp - d, n))
+                                               : 0.0;
   }
   __syncthreads();
-  if (wave == 0) band_ldlt(band, invd, n, bw, lane, st, w * 65536);
+  if (wave == 0) band_ldlt(band, invd, n, bw, Dd, lane, st, w * 65536);
   __syncthreads();
   double* S = G.sinv + ((size_t)slot * p.W + w) * (size_t)n * n;
   for (int col0 = 0; col0 < n; col0 += GEO_THREADS) {
     const int col = col0 + t;
     if (col < n) {
-      if (bw <= 8)
-        inverse_column<8>(band, invd, n, bw, col, S);
-      else if (bw <= 12)
-        inverse_column<12>(band, invd, n, bw, col, S);
-      else if (bw <= 16)
-        inverse_column<16>(band, invd, n, bw, col, S);
+      if (BWc == 8)
+        inverse_column<8>(band, invd, n, col, S);
+      else if (BWc == 12)
+        inverse_column<12>(band, invd, n, col, S);
+      else if (BWc == 16)
+        inverse_column<16>(band, invd, n, col, S);
       else
-        inverse_column<BMAX>(band, invd, n, bw, col, S);
+        inverse_column<BMAX>(band, invd, n, col, S);
     }
   }
 }
@@ -527,7 +531,10 @@ __global__ __launch_bounds__(GEO_THREADS) void k_geo_solve(IbmParams p, GeoBuffe
   geo_solve_body(p, G, st, slot0, blockIdx.x, blockIdx.y);
 }
 
-size_t geo_solve_lds_bytes(const IbmParams& p) { return (size_t)p.SPW * (BMAX + 2) * sizeof(double); }
+// band of (SPW + BMAX) rows of stride BMAX + 1 at most, and 1/D
+size_t geo_solve_lds_bytes(const IbmParams& p) {
+  return ((size_t)(p.SPW + BMAX) * (BMAX + 1) + p.SPW) * sizeof(double);
+}
 
 // ------------------------------------------------------------ coupled wings
 //
