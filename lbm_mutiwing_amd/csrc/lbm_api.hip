@@ -149,6 +149,7 @@ struct lbm_handle {
   int last_move_step = -1;           // step after which move_mesh last ran
   double rel[REL_FIELDS][MAXW] = {};  // body_rel_* (restart-file order)
   lbm_dynamics dyn{};                // rigid-body coupling switch and constants
+  DevStep* h_st = nullptr;           // pinned host copy of DevStep (coupled steps)
   Frame fr{};                        // movenum and body_extra_* of the next step
   bool motion_ready = false;
   double motion_end_time = 0.0;
@@ -251,6 +252,8 @@ void free_all(lbm_handle* h) {
   for (void* p : h->ibm_allocs) dfree(p);
   h->ibm_allocs.clear();
   dfree(h->d_frame);
+  if (h->h_st) (void)hipHostFree(h->h_st);
+  h->h_st = nullptr;
   dfree(h->d_table);
   dfree(h->d_stream);
   dfree(h->d_st);
@@ -428,6 +431,7 @@ int alloc_ibm(lbm_handle* h, const std::vector<double>& hx, const std::vector<do
   G.stream_rows = h->d_stream;
   G.frame = nullptr;
   F.row_sum = h->d_rowsum;
+  F.frame_out = nullptr;
   HIPCHECK(h, hipMemset(G.step, 0, ns * sizeof(GeoStep)));
   if (h->npts > 0) {
     HIPCHECK(h, hipMemcpy(m0, hx.data(), sizeof(double) * h->npts, hipMemcpyHostToDevice));
@@ -774,7 +778,10 @@ int finish_advance(lbm_handle* h, int nsteps, int n_last, bool timed, double* fo
 // and the move_mesh check after step m reads that state on the host.  Each
 // step runs the same kernels as lbm_advance (fields written every step, so
 // that a move can shift them), with its force row, and the body update in
-// it, right after the adjoint collision on the IBM stream.
+// it, right after the adjoint collision on the IBM stream.  The lattice
+// kernel does not wait for the step's geometry: it writes every interior
+// node, beside the geometry solve, and the adjoint collision overwrites the
+// adjoint nodes (populations and fields) after it.
 int advance_dynamic(lbm_handle* h, int32_t n_last, double* force_rows, double* motion_rows, double* row_sum,
                     double* last_row, double* err) {
   const int n0 = h->n;
@@ -787,37 +794,48 @@ int advance_dynamic(lbm_handle* h, int32_t n_last, double* force_rows, double* m
   HIPCHECK(h, hipMemsetAsync(h->d_rowsum, 0, sizeof(double) * h->row_len, h->stream));
   HIPCHECK(h, hipMemsetAsync(&h->d_st->error, 0, sizeof(int), h->stream));
   HIPCHECK(h, hipStreamSynchronize(h->stream));
+  if (!h->h_st) HIPCHECK(h, hipHostMalloc((void**)&h->h_st, sizeof(DevStep), hipHostMallocDefault));
   const int ns = h->nslot;
+  // the force row of each step writes the next step's frame and body state
+  // on the device; the host uploads them at the start and after a move
+  bool upload = true;
+  FlowBuffers F = h->F;
+  F.frame_out = h->d_frame;
   for (int m = n0; m <= n_last; m++) {
     const int k = m - n0;
     const int set = (int)(h->batch_count & 1u), slot = set * ns, s = m % 3;
-    HIPCHECK(h, hipMemcpy(h->d_frame, &h->fr, sizeof(Frame), hipMemcpyHostToDevice));
-    HIPCHECK(h, hipMemcpy(h->d_st->rel, h->rel, sizeof(h->rel), hipMemcpyHostToDevice));
+    if (upload) {
+      HIPCHECK(h, hipMemcpy(h->d_frame, &h->fr, sizeof(Frame), hipMemcpyHostToDevice));
+      HIPCHECK(h, hipMemcpy(h->d_st->rel, h->rel, sizeof(h->rel), hipMemcpyHostToDevice));
+      upload = false;
+    }
     launch_geometry(h->stream_geo, h->ip, h->G, h->d_st, slot, m, 0, 1, true);
     HIPCHECK(h, hipEventRecord(h->ev_geo[set], h->stream_geo));
     h->batch_count++;
-    HIPCHECK(h, hipStreamWaitEvent(h->stream_ibm, h->ev_geo[set], 0));
-    HIPCHECK(h, hipStreamWaitEvent(h->stream, h->ev_geo[set], 0));
     FieldSlot* fs = &h->slot[s];
     const void* src = h->fbuf[h->cur];
     void* dst = h->fbuf[1 - h->cur];
+    AdjMask every_node = adj_mask(h->ip, h->G, slot);
+    every_node.W = 0;
+    launch_lattice(h->stream, h->g, h->mrt, src, dst, fs, h->f32, h->lat_bx, every_node, nullptr, nullptr, h->lat_lds);
+    HIPCHECK(h, hipGetLastError());
+    HIPCHECK(h, hipEventRecord(h->ev_step, h->stream));
     launch_bc_ring(h->stream_bc, h->g, h->mrt, h->R, src, dst, fs, h->f32);
     HIPCHECK(h, hipEventRecord(h->ev_bc, h->stream_bc));
+    HIPCHECK(h, hipStreamWaitEvent(h->stream_ibm, h->ev_geo[set], 0));
     launch_ibm_flow(h->stream_ibm, h->ip, h->mrt, h->G, slot, h->F, h->g, src, h->f32);
+    HIPCHECK(h, hipStreamWaitEvent(h->stream_ibm, h->ev_step, 0));   // the lattice kernel's writes first
     launch_adj_collide(h->stream_ibm, h->g, h->mrt, h->ip, h->G, slot, h->F, m, src, dst, fs, h->f32);
-    launch_force_row(h->stream_ibm, h->ip, h->G, slot, h->F, h->d_st, m, k, h->row_len);
+    launch_force_row(h->stream_ibm, h->ip, h->G, slot, F, h->d_st, m, k, h->row_len);
     HIPCHECK(h, hipStreamWaitEvent(h->stream_ibm, h->ev_bc, 0));
     HIPCHECK(h, hipEventRecord(h->ev_ibm, h->stream_ibm));
-    launch_lattice(h->stream, h->g, h->mrt, src, dst, fs, h->f32, h->lat_bx, adj_mask(h->ip, h->G, slot), nullptr,
-                   nullptr, h->lat_lds);
-    HIPCHECK(h, hipGetLastError());
     HIPCHECK(h, hipStreamWaitEvent(h->stream, h->ev_ibm, 0));
     h->slot_step[s] = m;
     h->cur = 1 - h->cur;
     h->last_slot = slot;
+    HIPCHECK(h, hipMemcpyAsync(h->h_st, h->d_st, sizeof(DevStep), hipMemcpyDeviceToHost, h->stream));
     HIPCHECK(h, hipStreamSynchronize(h->stream));
-    DevStep st;
-    HIPCHECK(h, hipMemcpy(&st, h->d_st, sizeof(DevStep), hipMemcpyDeviceToHost));
+    const DevStep& st = *h->h_st;
     if (st.error != 0) {
       h->n = m + 1;
       return step_error(h, st);
@@ -832,6 +850,7 @@ int advance_dynamic(lbm_handle* h, int32_t n_last, double* force_rows, double* m
       rc = apply_move(h, dr, s, m);
       if (rc) return rc;
       HIPCHECK(h, hipStreamSynchronize(h->stream));  // the next step's ring and IBM streams read the shifted lattice
+      upload = true;                                 // pos and movenum changed on the host
     }
     if (h->flag2 > 0) h->flag2 = h->flag2 - 1;
   }

@@ -154,6 +154,7 @@ struct FlowBuffers {
   double* rows;            // force rows of the current call
   double* mrows;           // airfoil_motion rows of the current call
   double* row_sum;         // sum of force rows
+  Frame* frame_out;        // rigid-body coupling: the frame of the next step, or null
 };
 
 struct RingBuffers {

@@ -383,7 +383,25 @@ __device__ __forceinline__ void force_row_body(const IbmParams& p, const GeoBuff
     s_raw[ww][c] = r;
   }
   __syncthreads();
-  if (t == 0) compute_rows(p, G.step + slot, st, s_raw, n, s_row, s_mrow);
+  if (t == 0) {
+    compute_rows(p, G.step + slot, st, s_raw, n, s_row, s_mrow);
+    if (p.dyn_on && F.frame_out && st->error == 0) {
+      // the next step's frame from the updated body state
+      // (update_body_extra_from_rel, mode.f90:473-495); movenum changes only
+      // with a move, which the host applies
+      Frame f = G.step[slot].frame;
+      double av = 0.0, as = 0.0, ap = 0.0;
+      for (int w = 0; w < p.W; w++) {
+        av = av + st->rel[REL_V][w];
+        as = as + st->rel[REL_S][w];
+        ap = ap + st->rel[REL_POS][w];
+        f.body_extra_v[w] = av;
+        f.body_extra_s[w] = as;
+        f.body_extra_pos[w] = ap;
+      }
+      *F.frame_out = f;
+    }
+  }
   __syncthreads();
   double* row = F.rows + (long long)slot_row * row_len;
   for (int c = t; c < row_len; c += blockDim.x) {
