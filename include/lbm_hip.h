@@ -221,7 +221,8 @@ int lbm_get_timing(lbm_handle *h, double *lattice_ms, int32_t *launches);
  * list of (instance, block) items, so many small, uneven lattices fill the GPU
  * together.  Each instance ends in exactly the state lbm_advance would leave
  * it in.  All instances must share the device and the precision, and none may
- * have a pending move_mesh (|body_rel_pos(1)| >= 1).
+ * have a pending move_mesh (|body_rel_pos(1)| >= 1) or the rigid-body
+ * coupling switched on (lbm_set_dynamics; such an instance steps alone).
  *   row_sums, last_rows  optional, count x LBM_MAX_FORCE_ROW doubles: the
  *                        row_sum / last_row of lbm_advance for instance e at
  *                        e*LBM_MAX_FORCE_ROW (its row_len entries, rest zero)

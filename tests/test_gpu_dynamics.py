@@ -109,3 +109,29 @@ def test_batch_rejects_dynamics(lib):
     with pytest.raises(LbmError, match="rigid-body coupling"):
         bt.advance([s], 5)
     bt.close()
+
+
+def test_dynamics_fp32_storage_within_fp32_bar(lib, oracle_mod):
+    """FP32-shifted populations with the coupling on: the body update itself
+    runs in FP64 on the FP32 run's forces; fields within 1e-4 and the body
+    state within 1e-3 of the FP64 oracle (the FP32 bars), same moves."""
+    from lbm_mutiwing_amd import PRECISION_FP32_SHIFTED
+    cfg = preset("small")
+    o = oracle_mod.OracleSolver(cfg)
+    cfg32 = preset("small", precision=PRECISION_FP32_SHIFTED)
+    g = Solver(cfg32)
+    for s, c in ((g, cfg32), (o, cfg)):
+        _start(s, c, True)
+    print("dynamics fp32-shifted", flush=True)
+    rg, ro = g.advance(N, want_rows=True), o.advance(N, want_rows=True)
+    assert g.state()["movenum"] == o.state()["movenum"] >= 2
+    bg, bo = g.body_rel()[:, 0], o.body_rel()[:, 0]
+    scale = np.maximum(np.abs(bo), 1e-12)
+    for name in ("v", "s", "pos"):
+        i = BODY_REL_FIELDS.index(name)
+        assert abs(bg[i] - bo[i]) <= 1e-3 * scale[i], (name, bg[i], bo[i])
+    for a, b in zip(g.fields()[:3], o.fields()[:3]):
+        assert rel(a, b) < 1e-4
+    avg_g = rg.force_rows[20:, 1:10].mean(0)
+    avg_o = ro.force_rows[20:, 1:10].mean(0)
+    assert np.all(np.abs(avg_g - avg_o) <= 1e-3 * np.maximum(np.abs(avg_o).max(), 1e-3))
