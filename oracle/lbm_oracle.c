@@ -751,6 +751,40 @@ static void bc_extrapolate(orc_t *o, int ib, int jb, int in, int jn)
 
 /* boundary_condition1 (boundary_condition.f90:288-858), ref_freestream=1,
  * periodical_bc=0, single block. */
+/* LODI update of one far-field node (boundary_condition.f90:413-483 left,
+ * :495-567 right): r, u, v hold rho, u, v at the boundary node and at the
+ * next two nodes inward; out receives the boundary node's updated rho, u, v.
+ * The u update divides by the updated rho, as the reference does. */
+static void lodi_node(double cs, int right, const double *r, const double *u, const double *v, double *out)
+{
+    double rhox, ux, vx, lx1, lx2, lx3;
+    if (!right) {
+        rhox = (-3.0 * r[0] + 4.0 * r[1] - r[2]) * 0.5;
+        ux = (-3.0 * u[0] + 4.0 * u[1] - u[2]) * 0.5;
+        vx = (-3.0 * v[0] + 4.0 * v[1] - v[2]) * 0.5;
+        lx1 = (u[0] - cs) * (cs * cs * rhox - cs * r[0] * ux);
+        lx2 = u[0] < 0.0 ? u[0] * vx : 0.0;
+        lx3 = 0.0;
+    } else {
+        rhox = (3.0 * r[0] - 4.0 * r[1] + r[2]) * 0.5;
+        ux = (3.0 * u[0] - 4.0 * u[1] + u[2]) * 0.5;
+        vx = (3.0 * v[0] - 4.0 * v[1] + v[2]) * 0.5;
+        lx1 = 0.0;
+        lx2 = u[0] >= 0.0 ? u[0] * vx : 0.0;
+        lx3 = (u[0] + cs) * (cs * cs * rhox + cs * r[0] * ux);
+    }
+    out[0] = r[0] - (lx1 + lx3) * 1.5;
+    out[1] = u[0] + (lx1 - lx3) / (2.0 * cs * out[0]);
+    out[2] = v[0] - lx2;
+}
+
+int orc_lodi_node(const orc_t *o, int32_t right, const double *r, const double *u, const double *v, double *out)
+{
+    if (!o || !r || !u || !v || !out) return LBM_E_INVALID;
+    lodi_node(o->cs, right != 0, r, u, v, out);
+    return 0;
+}
+
 static void boundary_condition1(orc_t *o)
 {
     const int nx = o->nx, ny = o->ny;
@@ -765,16 +799,15 @@ static void boundary_condition1(orc_t *o)
         tv[j] = o->v[IDX(1, j)];
     }
     for (int j = 2; j <= ny - 1; j++) {
-        double rhox = (-3.0 * tr[j] + 4.0 * o->rho[IDX(2, j)] - o->rho[IDX(3, j)]) * 0.5;
-        double ux = (-3.0 * tu[j] + 4.0 * o->u[IDX(2, j)] - o->u[IDX(3, j)]) * 0.5;
-        double vx = (-3.0 * tv[j] + 4.0 * o->v[IDX(2, j)] - o->v[IDX(3, j)]) * 0.5;
-        double lx1 = (tu[j] - cs) * (cs * cs * rhox - cs * tr[j] * ux);
-        double lx2 = tu[j] < 0.0 ? tu[j] * vx : 0.0;
-        double lx3 = 0.0;
+        const double r[3] = {tr[j], o->rho[IDX(2, j)], o->rho[IDX(3, j)]};
+        const double uu[3] = {tu[j], o->u[IDX(2, j)], o->u[IDX(3, j)]};
+        const double vv[3] = {tv[j], o->v[IDX(2, j)], o->v[IDX(3, j)]};
+        double out[3];
+        lodi_node(cs, 0, r, uu, vv, out);
         size_t id = IDX(1, j);
-        o->rho[id] = o->rho[id] - (lx1 + lx3) * 1.5;
-        o->u[id] = o->u[id] + (lx1 - lx3) / (2.0 * cs * o->rho[id]);
-        o->v[id] = o->v[id] - lx2;
+        o->rho[id] = out[0];
+        o->u[id] = out[1];
+        o->v[id] = out[2];
     }
     for (int j = 2; j <= ny - 1; j++) bc_extrapolate(o, 1, j, 2, j);
     /* right (boundary_condition.f90:495-567) */
@@ -784,16 +817,15 @@ static void boundary_condition1(orc_t *o)
         tv[j] = o->v[IDX(nx, j)];
     }
     for (int j = 2; j <= ny - 1; j++) {
-        double rhox = (3.0 * tr[j] - 4.0 * o->rho[IDX(nx - 1, j)] + o->rho[IDX(nx - 2, j)]) * 0.5;
-        double ux = (3.0 * tu[j] - 4.0 * o->u[IDX(nx - 1, j)] + o->u[IDX(nx - 2, j)]) * 0.5;
-        double vx = (3.0 * tv[j] - 4.0 * o->v[IDX(nx - 1, j)] + o->v[IDX(nx - 2, j)]) * 0.5;
-        double lx1 = 0.0;
-        double lx2 = tu[j] >= 0.0 ? tu[j] * vx : 0.0;
-        double lx3 = (tu[j] + cs) * (cs * cs * rhox + cs * tr[j] * ux);
+        const double r[3] = {tr[j], o->rho[IDX(nx - 1, j)], o->rho[IDX(nx - 2, j)]};
+        const double uu[3] = {tu[j], o->u[IDX(nx - 1, j)], o->u[IDX(nx - 2, j)]};
+        const double vv[3] = {tv[j], o->v[IDX(nx - 1, j)], o->v[IDX(nx - 2, j)]};
+        double out[3];
+        lodi_node(cs, 1, r, uu, vv, out);
         size_t id = IDX(nx, j);
-        o->rho[id] = o->rho[id] - (lx1 + lx3) * 1.5;
-        o->u[id] = o->u[id] + (lx1 - lx3) / (2.0 * cs * o->rho[id]);
-        o->v[id] = o->v[id] - lx2;
+        o->rho[id] = out[0];
+        o->u[id] = out[1];
+        o->v[id] = out[2];
     }
     for (int j = 2; j <= ny - 1; j++) bc_extrapolate(o, nx, j, nx - 1, j);
     /* top wall (boundary_condition.f90:657-723) */

@@ -64,6 +64,9 @@ int orc_bc_node(orc_t *o, double rb, double ub, double vb, double rn, double un,
 int orc_get_ibm_nodes(orc_t *o, int32_t capacity, double *us, double *vs, double *u,
                       double *v, double *rho);
 int orc_get_rates(orc_t *o, double *s);
+/* LODI update of one far-field node (right = 0: the left boundary): rho, u,
+ * v at the boundary node and the next two inward -> its updated rho, u, v. */
+int orc_lodi_node(const orc_t *o, int32_t right, const double *r, const double *u, const double *v, double *out);
 /* Threads used by the lattice loops (OpenMP); 1 when built without it. */
 int orc_set_threads(int n);
 

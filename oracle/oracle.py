@@ -49,6 +49,8 @@ def load():
         lib.orc_get_ibm_nodes.argtypes = [ctypes.c_void_p, ctypes.c_int32, _dp, _dp, _dp, _dp, _dp]
         lib.orc_get_rates.restype = ctypes.c_int
         lib.orc_get_rates.argtypes = [ctypes.c_void_p, _dp]
+        lib.orc_lodi_node.restype = ctypes.c_int
+        lib.orc_lodi_node.argtypes = [ctypes.c_void_p, ctypes.c_int32, _dp, _dp, _dp, _dp]
         lib.orc_body_step.restype = None
         lib.orc_body_step.argtypes = [ctypes.c_void_p] + [ctypes.c_double] * 4 + [_dp]
         lib.orc_get_ibm_system.restype = ctypes.c_int
@@ -111,6 +113,13 @@ class OracleSolver(Solver):
         arrs = [np.zeros(max(n, 1)) for _ in range(5)]
         self.lib.orc_get_ibm_nodes(self._h, n, *[self._dp(a) for a in arrs])
         return [a[:n] for a in arrs]
+
+    def lodi(self, right, r, u, v):
+        import numpy as np
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (r, u, v)]
+        out = np.zeros(3)
+        self.lib.orc_lodi_node(self._h, int(right), *[self._dp(x) for x in a], self._dp(out))
+        return out
 
     def rates(self):
         import numpy as np

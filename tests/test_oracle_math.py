@@ -159,6 +159,92 @@ def test_streaming_is_a_pull_permutation(oracle_mod):
     assert np.allclose(u[1:-1, 1:-1], ((E[:, 0, None, None] * fs).sum(0) / r)[1:-1, 1:-1], atol=1e-15)
 
 
+# sound speed as the reference computes it: sqrt of the single-precision
+# literal 3.0, promoted, over 3d0 (boundary_condition.f90:304)
+CS_REF = float(np.float32(np.sqrt(np.float32(3.0)))) / 3.0
+# the density step uses the exact lattice c^2 = 1/3 (its coefficient is the
+# literal 1.5), the wave amplitudes and the velocity step the rounded c
+C0_SQ = 1.0 / 3.0
+
+
+def _lodi_textbook(r, u, v, left):
+    """Locally one-dimensional inviscid characteristic update of a far-field
+    node (Poinsot & Lele 1992), written from the textbook: p = c^2 rho, the
+    x derivatives from the parabola through the node and the next two
+    inward (Lagrange weights in exact arithmetic), wave amplitudes L1 = (u - c)(p_x - rho c u_x),
+    L3 = u v_x (shear), L5 = (u + c)(p_x + rho c u_x); the waves entering the
+    domain are set to zero (left boundary: L5, and L3 when u >= 0; right
+    boundary: L1, and L3 when u < 0); one unit time step of
+    rho_t = -(L1 + L5) / (2 c0^2), u_t = (L1 - L5) / (2 rho c), v_t = -L3,
+    the u step divided by the new rho (the reference's order)."""
+    c = CS_REF
+    from fractions import Fraction as Fr
+    x = [Fr(0), Fr(1), Fr(2)] if left else [Fr(0), Fr(-1), Fr(-2)]
+    # weights of the derivative at x[0] of the Lagrange parabola, in exact arithmetic
+    wts = []
+    for a_ in range(3):
+        others = [x[b_] for b_ in range(3) if b_ != a_]
+        den = (x[a_] - others[0]) * (x[a_] - others[1])
+        wts.append(float(((x[0] - others[0]) + (x[0] - others[1])) / den))
+    rx, ux, vx = (sum(w_ * f_ for w_, f_ in zip(wts, f)) for f in (r, u, v))
+    px = c * c * rx
+    L1 = (u[0] - c) * (px - r[0] * c * ux)
+    L5 = (u[0] + c) * (px + r[0] * c * ux)
+    L3 = u[0] * vx
+    if left:
+        L5 = 0.0
+        L3 = L3 if u[0] < 0.0 else 0.0
+    else:
+        L1 = 0.0
+        L3 = L3 if u[0] >= 0.0 else 0.0
+    rho = r[0] - (L1 + L5) / (2.0 * C0_SQ)
+    return np.array([rho, u[0] + (L1 - L5) / (2.0 * rho * c), v[0] - L3])
+
+
+@pytest.mark.parametrize("left", [True, False])
+def test_lodi_far_field_is_the_textbook_characteristic_update(oracle_mod, left):
+    o = oracle_mod.OracleSolver(_cfg_nowings())
+    rng = np.random.default_rng(5 if left else 6)
+    for _ in range(200):
+        r = 1.0 + 0.05 * rng.normal(size=3)
+        u = 0.05 * rng.normal(size=3)
+        v = 0.05 * rng.normal(size=3)
+        got = o.lodi(0 if left else 1, r, u, v)
+        want = _lodi_textbook(r, u, v, left)
+        assert np.allclose(got, want, rtol=0, atol=1e-14), (r, u, v, got, want)
+
+
+def test_lodi_left_and_right_are_mirror_images(oracle_mod):
+    """Reflecting x (u -> -u, node order kept inward) maps the right
+    boundary's update onto the left one's, bit for bit."""
+    o = oracle_mod.OracleSolver(_cfg_nowings())
+    rng = np.random.default_rng(8)
+    for _ in range(500):
+        r = 1.0 + 0.05 * rng.normal(size=3)
+        u = 0.05 * rng.normal(size=3)
+        v = 0.05 * rng.normal(size=3)
+        a = o.lodi(0, r, u, v)
+        b = o.lodi(1, r, -u, v)
+        assert a[0] == b[0] and a[1] == -b[1] and a[2] == b[2], (r, u, v, a, b)
+
+
+def test_lodi_passes_a_simple_wave_entering_unchanged(oracle_mod):
+    """At the left (inflow) boundary a right-running acoustic wave
+    (p_x = rho c u_x) carries no outgoing characteristic, and the node is
+    left as it is; a left-running one (p_x = -rho c u_x) leaves through it."""
+    o = oracle_mod.OracleSolver(_cfg_nowings())
+    c = CS_REF
+    eps = 1e-3
+    r = np.array([1.0, 1.0 + eps, 1.0 + 2 * eps])
+    u_in = 0.05 + np.array([0.0, 1.0, 2.0]) * c * eps      # c^2 rho_x = c rho u_x at rho = 1
+    out = o.lodi(0, r, u_in, np.zeros(3))
+    assert abs(out[0] - 1.0) < 1e-17 and abs(out[1] - 0.05) < 1e-16 and out[2] == 0.0
+    u_out = 0.05 - np.array([0.0, 1.0, 2.0]) * c * eps
+    out = o.lodi(0, r, u_out, np.zeros(3))
+    L1 = (0.05 - c) * (2 * c * c * eps)
+    assert out[0] == pytest.approx(1.0 - 1.5 * L1, rel=1e-14)
+
+
 def test_walls_and_uniform_far_field(oracle_mod):
     o = oracle_mod.OracleSolver(_cfg_nowings())
     o.advance(3)
