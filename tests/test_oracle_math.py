@@ -309,6 +309,40 @@ def test_ibm_enforces_no_slip_at_every_segment(oracle_mod):
         fy = -(2 * rho[sel] * (v[sel] - vs[sel])).sum() * 2 / (cfg.uu ** 2 / s)
         assert wf[w - 1, 0] == pytest.approx(fx, rel=1e-9, abs=1e-12)
         assert wf[w - 1, 1] == pytest.approx(fy, rel=1e-9, abs=1e-12)
+        # moment about the wing's pivot (its origin; the motion offsets are
+        # zero here), lattice units, normalised by U^2 c^2 / 2 with the
+        # reference's sign (solution.f90:341-349)
+        x0 = (x[sel] - (cfg.xc + cfg.wing_x0 + (w - 1) * cfg.wing_dx)) / s
+        y0 = (y[sel] - (cfg.yc + cfg.wing_y0)) / s
+        t1 = 2 * rho[sel] * (u[sel] - us[sel])
+        t2 = 2 * rho[sel] * (v[sel] - vs[sel])
+        mom = (-x0 * t2 + y0 * t1).sum()
+        assert wm[w - 1] == pytest.approx(-2 * mom / (cfg.uu ** 2 / s ** 2), rel=1e-9, abs=1e-12)
+
+
+def test_move_mesh_shifts_the_lattice_one_column(oracle_mod):
+    """move_mesh(+1) after a step (body_rel_pos(1) >= 1, main.f90:393-404):
+    every field moves one column towards i = 1, the last column stays, the
+    frame counter advances and flag2 opens its window."""
+    cfg = preset("small")
+    rows = sinusoidal_motion_table(cfg, [WingMotion(), WingMotion()], 8)
+    rng = np.random.default_rng(4)
+    f = W_EXACT[:, None, None] * (1 + 0.01 * rng.normal(size=(9, cfg.ny, cfg.nx)))
+    a, b = oracle_mod.OracleSolver(cfg), oracle_mod.OracleSolver(cfg)
+    for o in (a, b):
+        o.set_motion_table(rows)
+        o.set_populations(f, f)
+    b.set_body_rel_pos(1, 1.0)
+    a.advance(1)
+    b.advance(1)
+    fa, f0a = a.populations()
+    fb, f0b = b.populations()
+    assert np.array_equal(fb[:, :, :-1], fa[:, :, 1:]) and np.array_equal(fb[:, :, -1], fa[:, :, -1])
+    for qa, qb in zip(a.fields()[:3], b.fields()[:3]):
+        assert np.array_equal(qb[:, :-1], qa[:, 1:]) and np.array_equal(qb[:, -1], qa[:, -1])
+    sa, sb = a.state(), b.state()
+    assert (sa["movenum"], sb["movenum"]) == (0, 1) and sb["flag2"] == 2
+    assert b.body_rel()[6, 0] == 0.0
 
 
 def test_adjoint_list_is_first_claim_union_of_boxes(oracle_mod):
