@@ -486,6 +486,15 @@ def main():
     lat_ms, launches = s.timing()
     tile, waves = s.tuning_full()
     st = s.state()
+    # the lattice kernel's memory pattern alone (pull of nine planes, no
+    # arithmetic), timed on the same device: the ceiling of that pattern
+    import ctypes
+    ceil_us = ctypes.c_double()
+    fn = s.lib.lbm_measure_pull_copy
+    fn.restype = ctypes.c_int
+    fn.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]
+    if fn(cfg.nx, cfg.ny, cfg.precision, 20, ctypes.byref(ceil_us)) != 0:
+        raise RuntimeError("lbm_measure_pull_copy failed")
     nodes = cfg.nx * cfg.ny
     mlups = nodes * args.steps / dt / 1e6
     ms_step = dt * 1e3 / args.steps
@@ -520,6 +529,12 @@ def main():
                      "kernel": "k_lattice (fused pull-stream + moments + MRT collide)",
                      "kernel_avg_us": round(lat_avg_ms * 1e3, 2), "tile": tile, "waves_per_cu_cap": waves,
                      "algorithmic_bytes_per_launch": bpn * interior},
+        "pull_copy_ceiling": {"avg_us": round(ceil_us.value, 2),
+                              "achieved": round(bpn * interior / (ceil_us.value * 1e-6) / 1e9, 1), "unit": "GB/s",
+                              "frac": round(bpn * interior / (ceil_us.value * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
+                              "kernel_over_ceiling": round(lat_avg_ms * 1e3 / ceil_us.value, 4),
+                              "what": "the lattice kernel's loads and stores alone (nine pulled planes in, nine out, "
+                                      "no arithmetic), same lattice and storage, 20 launches"},
         "last_force_row": [round(x, 8) for x in r.last_row[:10]],
     }
     s.close()

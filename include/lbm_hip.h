@@ -214,6 +214,15 @@ int lbm_set_dynamics(lbm_handle *h, const lbm_dynamics *d);
 int lbm_set_timing(lbm_handle *h, int32_t enable);
 int lbm_get_timing(lbm_handle *h, double *lattice_ms, int32_t *launches);
 
+/* Measurement hook (no counterpart in the reference): the average time of
+ * `launches` launches of the lattice kernel's memory pattern alone (each
+ * interior node of an nx x ny lattice pulls its nine populations from its
+ * neighbours and writes them, in the storage of `precision`, with no
+ * arithmetic), on the current device, in its own buffers, which it
+ * allocates and frees.  The ceiling the lattice kernel's time is compared
+ * with. */
+int lbm_measure_pull_copy(int32_t nx, int32_t ny, int32_t precision, int32_t launches, double *avg_us);
+
 /* Batched advance (no counterpart in the reference, which runs one episode
  * per process; for hosts that serve many episodes on one GPU).  Advances each
  * of the count instances in hs by `steps` steps from its own next step, with

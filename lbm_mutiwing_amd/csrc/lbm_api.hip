@@ -1234,6 +1234,13 @@ int lbm_get_timing(lbm_handle* h, double* lattice_ms, int32_t* launches) {
   return 0;
 }
 
+int lbm_measure_pull_copy(int32_t nx, int32_t ny, int32_t precision, int32_t launches, double* avg_us) {
+  if (nx < 8 || ny < 8 || launches < 1 || !avg_us ||
+      (precision != LBM_PRECISION_FP64 && precision != LBM_PRECISION_FP32_SHIFTED))
+    return LBM_E_INVALID;
+  return measure_pull_copy(nx, ny, precision == LBM_PRECISION_FP32_SHIFTED, launches, avg_us);
+}
+
 int lbm_get_tuning(const lbm_handle* h, int32_t* lattice_tile, int32_t* waves_per_cu) {
   if (!h) return LBM_E_INVALID;
   if (lattice_tile) *lattice_tile = h->lat_bx;

@@ -908,4 +908,54 @@ void launch_batch_adj_collide(hipStream_t s, const EpDesc* eps, const BatchItems
   }
 }
 
+// The lattice kernel's memory pattern alone: every interior node pulls its
+// nine populations from the neighbours of the source planes and writes them
+// to the destination planes, with no moment or collision arithmetic.  Its
+// time is the ceiling of that access pattern on the device.
+template <typename T, int BX>
+__global__ __launch_bounds__(BX) void k_pull_copy(const T* __restrict__ src, T* __restrict__ dst, int nx, int ny) {
+  const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y;
+  if (i < 1 || i > nx - 2 || j < 1 || j > ny - 2) return;
+  const size_t ps = (size_t)nx * ny;
+  T f[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) f[k] = src[k * ps + (size_t)(j - EY(k)) * nx + (i - EX(k))];
+#pragma unroll
+  for (int k = 0; k < 9; k++) dst[k * ps + (size_t)j * nx + i] = f[k];
+}
+
+int measure_pull_copy(int nx, int ny, bool f32, int launches, double* avg_us) {
+  const size_t ps = (size_t)nx * ny, bytes = 9 * ps * (f32 ? 4 : 8);
+  void *a = nullptr, *b = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = 0;
+  float ms = 0.f;
+  const dim3 grid((nx + 255) / 256, ny), block(256);
+  auto launch = [&](const void* s, void* d) {
+    if (f32)
+      hipLaunchKernelGGL((k_pull_copy<float, 256>), grid, block, 0, 0, (const float*)s, (float*)d, nx, ny);
+    else
+      hipLaunchKernelGGL((k_pull_copy<double, 256>), grid, block, 0, 0, (const double*)s, (double*)d, nx, ny);
+  };
+  if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess || hipMemset(a, 0, bytes) != hipSuccess ||
+      hipMemset(b, 0, bytes) != hipSuccess || hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    rc = LBM_E_DEVICE;
+  } else {
+    for (int r = 0; r < 3; r++) launch(r & 1 ? b : a, r & 1 ? a : b);
+    (void)hipEventRecord(e0, 0);
+    for (int r = 0; r < launches; r++) launch(r & 1 ? b : a, r & 1 ? a : b);
+    (void)hipEventRecord(e1, 0);
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess ||
+        hipGetLastError() != hipSuccess)
+      rc = LBM_E_DEVICE;
+    else
+      *avg_us = 1000.0 * ms / launches;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  return rc;
+}
+
 }  // namespace lbm
