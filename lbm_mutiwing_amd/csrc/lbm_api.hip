@@ -226,7 +226,7 @@ const char* dev_error_text(int code) {
     case DERR_MAP: return "support rectangle of a wing exceeds its claim map";
     case DERR_BAND: return "IBM band half-width exceeds the solver limit";
     case DERR_WINGS_COUPLED:
-      return "the supports of more than two wings, or of more than 32 segments of one wing, share lattice nodes";
+      return "more than 32 segments of one wing, or 64 in all, couple wings whose supports share lattice nodes";
     case DERR_NOT_SPD: return "IBM matrix lost positive definiteness";
     case DERR_MOTION: return "motion source not initialised (call lbm_set_motion_table)";
     default: return "unknown device error";
@@ -405,7 +405,7 @@ int alloc_ibm(lbm_handle* h, const std::vector<double>& hx, const std::vector<do
   p.map_cap = h->W > 0 ? side * side : 1;
   const size_t per_slot = sizeof(GeoStep) + 2 * npts * 8 + 4 * nseg * 8 + 4 * nseg * 4 + 2 * BOXW * nseg * 8 +
                           W * (spw + 1) * 4 + 3 * cap * 4 + W * spw * spw * 8 + W * 2 * (size_t)p.map_cap * 4 +
-                          2 * CPL_K * CPL_K * 8;
+                          (size_t)CPL_KT * CPL_KT * 8;
   h->nslot = h->W > 0 ? (int)std::max<size_t>(1, std::min<size_t>(16, (256u << 20) / per_slot)) : 1;
   const size_t ns = 2 * (size_t)h->nslot;
   int rc = 0;
@@ -421,7 +421,7 @@ int alloc_ibm(lbm_handle* h, const std::vector<double>& hx, const std::vector<do
       (rc = ialloc(h, &G.adj_ij, ns * cap * 2)) || (rc = ialloc(h, &G.adj_seg, ns * cap)) ||
       (rc = ialloc(h, &G.sinv, h->W > 0 ? ns * W * spw * spw : 1)) ||
       (rc = ialloc(h, &G.map, h->W > 0 ? ns * W * 2 * (size_t)p.map_cap : 1)) ||
-      (rc = ialloc(h, &G.cpl, h->W > 1 ? ns * 2 * CPL_K * CPL_K : 1)))
+      (rc = ialloc(h, &G.cpl, h->W > 1 ? ns * CPL_KT * CPL_KT : 1)))
     return rc;
   const int bpw = adj_collide_blocks_per_wing(p);
   if ((rc = ialloc(h, &F.cvec, 2 * nseg)) || (rc = ialloc(h, &F.yvec, 2 * nseg)) ||

@@ -538,21 +538,20 @@ size_t geo_solve_lds_bytes(const IbmParams& p) {
 
 // ------------------------------------------------------------ coupled wings
 //
-// When the support boxes of wings a < b share lattice nodes, the IBM system
-// is the one over both wings' segments (boundary_condition.f90:1558-1640 over
-// all segments): S = [[S_aa, C], [C^T, S_bb]], with C = E_a C0 E_b^T nonzero
-// only between the p segments of a and q segments of b whose boxes meet.  The
-// shared nodes belong to a (its segments claim first, airfoil.f90:288-333).
-// With u = blockdiag(S_aa, S_bb)^-1 c (the per-wing solves), the Schur
-// complement on b and Woodbury give
-//   w   = C0^T u_a[K_a]
-//   v   = u_b - S_bb^-1[:, K_b] w                     (= S_bb^-1 (c_b - C^T u_a))
-//   y_b = v + S_bb^-1[:, K_b] M v[K_b],   M = (I - H G_bb)^-1 H,  H = C0^T G_aa C0
-//   y_a = u_a - S_aa^-1[:, K_a] C0 y_b[K_b]
-// with G_ww = S_ww^-1[K_w, K_w].  The geometry pass finds the pair, K_a,
-// K_b, C0 and M (k_geo_couple) and drops the shared nodes from b's adjoint
-// list; the flow pass applies the correction after the per-wing mat-vecs
-// (k_ibm_couple), and the spread adds b's coupling segments at a's nodes.
+// When the support boxes of wings share lattice nodes, the IBM system is the
+// one over all the wings' segments (boundary_condition.f90:1558-1640 over all
+// segments): S = B + P O P^T, with B = blockdiag(S_ww) the per-wing systems,
+// P the selection of the coupling segments K (the segments whose boxes meet a
+// box of another wing) and O (|K| x |K|) the cross-wing entries among them
+// (zero within a wing).  A shared node belongs to the lowest of its wings
+// (its segments claim first, airfoil.f90:288-333).  Woodbury in the form that
+// needs no inverse of O gives, with u = B^-1 c (the per-wing solves),
+//   y = u - B^-1 P M P^T u,   M = (I + O G)^-1 O,   G = P^T B^-1 P
+// (G block-diagonal: G_ww = S_ww^-1[K_w, K_w]).  The geometry pass finds K,
+// forms M by Gauss-Jordan in LDS (k_geo_couple) and drops each wing's nodes
+// that a lower wing owns from its adjoint list; the flow pass applies the
+// correction after the per-wing mat-vecs (k_ibm_couple), and the spread adds
+// the coupling segments of higher wings at a wing's nodes.
 constexpr int CPL_THREADS = 256;
 constexpr int CPL_MAX_SPW = 2048;   // segments per wing the coupling pass handles
 
@@ -593,26 +592,31 @@ __device__ int block_scan(int* s, int v) {
   return total;
 }
 
+// Dynamic LDS of the coupling pass: G, then [I + O G | O] as two K x K blocks.
+size_t geo_couple_lds_bytes() { return (size_t)3 * CPL_KT * CPL_KT * sizeof(double); }
+
 // One workgroup per step, after the per-wing solves of the step.
 __device__ __forceinline__ void geo_couple_body(const IbmParams& p, const GeoBuffers& G, DevStep* st, int slot) {
+  extern __shared__ double lds_c[];
+  double* sG = lds_c;                           // G (K x K, block-diagonal)
+  double* sL = sG + CPL_KT * CPL_KT;            // I + O G, row stride K
+  double* sR = sL + CPL_KT * CPL_KT;            // O, then M
   __shared__ int s_flag[MAXW * MAXW];
-  __shared__ int s_cnt[2], s_skip;
-  __shared__ int s_k[2][CPL_K];
-  __shared__ double s_c0[CPL_K * CPL_K];        // C0 (p x q)
-  __shared__ double s_x[CPL_K * CPL_K];         // G_aa C0 (p x q), then G_bb (q x q)
-  __shared__ double s_h[CPL_K * CPL_K];         // H (q x q)
-  __shared__ double s_aug[CPL_K * 2 * CPL_K];   // [I - H G_bb | H], row stride 2q
-  __shared__ double s_f[CPL_K];
-  __shared__ int s_piv;
+  __shared__ int s_cnt[MAXW], s_skip, s_piv;
+  __shared__ int s_kw[MAXW][CPL_K];
+  __shared__ int s_off[MAXW + 1];
+  __shared__ int s_k[CPL_KT], s_w[CPL_KT], s_pos[CPL_KT];
+  __shared__ double s_f[CPL_KT];
   __shared__ int s_scan[CPL_THREADS];
   __shared__ int s_seg[CPL_MAX_SPW + 1];
   const int t = threadIdx.x;
   const int W = p.W, n = p.SPW;
   GeoStep* gs = G.step + slot;
   for (int q = t; q < MAXW * MAXW; q += CPL_THREADS) s_flag[q] = 0;
+  if (t < MAXW) s_cnt[t] = 0;
   if (t == 0) {
     gs->cpl_on = 0;
-    s_cnt[0] = s_cnt[1] = 0;
+    gs->cpl_n = 0;
     s_skip = load_agent(&st->error) != 0;
   }
   __syncthreads();
@@ -636,47 +640,50 @@ __device__ __forceinline__ void geo_couple_body(const IbmParams& p, const GeoBuf
       }
     }
   __syncthreads();
-  int npair = 0, a = -1, b = -1;
-  for (int w = 0; w < W; w++)
-    for (int w2 = w + 1; w2 < W; w2++)
-      if (s_flag[w * MAXW + w2]) {
-        npair++;
-        a = w;
-        b = w2;
-      }
-  if (npair == 0) return;
-  if (npair > 1 || n > CPL_MAX_SPW) {
-    if (t == 0) raise_error(st, DERR_WINGS_COUPLED, npair > 1 ? 1000 + npair : n);
+  bool any = false;
+  for (int q = 0; q < MAXW * MAXW; q++) any = any || s_flag[q] != 0;
+  if (!any) return;
+  if (n > CPL_MAX_SPW) {
+    if (t == 0) raise_error(st, DERR_WINGS_COUPLED, n);
     return;
   }
-  // the coupling segments of each wing
-  const size_t sa0 = (size_t)slot * p.nseg + (size_t)a * n, sb0 = (size_t)slot * p.nseg + (size_t)b * n;
-  const int* boxa = G.box + sa0 * 4;
-  const int* boxb = G.box + sb0 * 4;
-  for (int side = 0; side < 2; side++) {
-    const int* mine = side ? boxb : boxa;
-    const int* other = side ? boxa : boxb;
-    const int* ro = gs->rect[side ? a : b];
+  // the coupling segments of each wing: boxes that meet a box of a wing it shares nodes with
+  for (int w = 0; w < W; w++) {
+    const size_t s0 = (size_t)slot * p.nseg + (size_t)w * n;
     for (int k = t; k < n; k += CPL_THREADS) {
-      const int* bk = mine + 4 * k;
-      if (!boxes_meet(bk, ro)) continue;
+      const int* bk = G.box + (s0 + k) * 4;
       bool meet = false;
-      for (int l = 0; l < n && !meet; l++) meet = boxes_meet(bk, other + 4 * l);
+      for (int w2 = 0; w2 < W && !meet; w2++) {
+        if (w2 == w || !s_flag[min(w, w2) * MAXW + max(w, w2)] || !boxes_meet(bk, gs->rect[w2])) continue;
+        const int* ob = G.box + ((size_t)slot * p.nseg + (size_t)w2 * n) * 4;
+        for (int l = 0; l < n && !meet; l++) meet = boxes_meet(bk, ob + 4 * l);
+      }
       if (meet) {
-        const int idx = atomicAdd(&s_cnt[side], 1);
-        if (idx < CPL_K) s_k[side][idx] = k;
+        const int idx = atomicAdd(&s_cnt[w], 1);
+        if (idx < CPL_K) s_kw[w][idx] = k;
       }
     }
   }
   __syncthreads();
-  const int pa = s_cnt[0], qb = s_cnt[1];
-  if (pa > CPL_K || qb > CPL_K || pa == 0 || qb == 0) {
-    if (t == 0) raise_error(st, DERR_WINGS_COUPLED, 2000 + max(pa, qb));
+  if (t == 0) {
+    int run = 0, over = 0;
+    for (int w = 0; w < W; w++) {
+      s_off[w] = run;
+      if (s_cnt[w] > CPL_K) over = max(over, s_cnt[w]);
+      run += min(s_cnt[w], CPL_K);
+    }
+    s_off[W] = run;
+    if (over || run > CPL_KT) s_piv = -2 - max(over, run);
+    else s_piv = 0;
+  }
+  __syncthreads();
+  if (s_piv < 0) {
+    if (t == 0) raise_error(st, DERR_WINGS_COUPLED, 2000 - s_piv - 2);
     return;
   }
-  if (t < 2) {  // ascending segment order
-    int* ks = s_k[t];
-    const int m = t ? qb : pa;
+  if (t < W) {  // ascending segment order per wing, then the flat list
+    int* ks = s_kw[t];
+    const int m = s_cnt[t];
     for (int x = 1; x < m; x++) {
       const int v = ks[x];
       int y = x - 1;
@@ -686,54 +693,45 @@ __device__ __forceinline__ void geo_couple_body(const IbmParams& p, const GeoBuf
       }
       ks[y + 1] = v;
     }
+    for (int x = 0; x < m; x++) {
+      s_k[s_off[t] + x] = ks[x];
+      s_w[s_off[t] + x] = t;
+      s_pos[s_off[t] + x] = pos_of_seg(ks[x], n);
+    }
   }
   __syncthreads();
-  // C0, then G_aa C0, H = C0^T G_aa C0, G_bb, and [I - H G_bb | H]
-  for (int e = t; e < pa * qb; e += CPL_THREADS) {
-    const int i = e / qb, j = e - i * qb;
-    const int k = s_k[0][i], l = s_k[1][j];
-    s_c0[e] = s_cross(boxa + 4 * k, G.phx + (sa0 + k) * BOXW, G.phy + (sa0 + k) * BOXW, boxb + 4 * l,
-                      G.phx + (sb0 + l) * BOXW, G.phy + (sb0 + l) * BOXW);
+  const int K = s_off[W];
+  // G (from the per-wing inverses) and O (cross-wing entries of S)
+  for (int e = t; e < K * K; e += CPL_THREADS) {
+    const int i = e / K, j = e - i * K;
+    const int wi = s_w[i], wj = s_w[j];
+    double gv = 0.0, ov = 0.0;
+    if (wi == wj) {
+      gv = G.sinv[((size_t)slot * W + wi) * (size_t)n * n + (size_t)s_pos[i] * n + s_pos[j]];
+    } else {
+      const size_t si = (size_t)slot * p.nseg + (size_t)wi * n + s_k[i];
+      const size_t sj = (size_t)slot * p.nseg + (size_t)wj * n + s_k[j];
+      ov = s_cross(G.box + si * 4, G.phx + si * BOXW, G.phy + si * BOXW, G.box + sj * 4, G.phx + sj * BOXW,
+                   G.phy + sj * BOXW);
+    }
+    sG[e] = gv;
+    sR[e] = ov;
   }
   __syncthreads();
-  const double* SA = G.sinv + ((size_t)slot * W + a) * (size_t)n * n;
-  const double* SB = G.sinv + ((size_t)slot * W + b) * (size_t)n * n;
-  for (int e = t; e < pa * qb; e += CPL_THREADS) {
-    const int i = e / qb, j = e - i * qb;
-    const size_t ri = (size_t)pos_of_seg(s_k[0][i], n) * n;
-    double acc = 0.0;
-    for (int m = 0; m < pa; m++) acc += SA[ri + pos_of_seg(s_k[0][m], n)] * s_c0[m * qb + j];
-    s_x[e] = acc;
-  }
-  __syncthreads();
-  for (int e = t; e < qb * qb; e += CPL_THREADS) {
-    const int i = e / qb, j = e - i * qb;
-    double acc = 0.0;
-    for (int m = 0; m < pa; m++) acc += s_c0[m * qb + i] * s_x[m * qb + j];
-    s_h[e] = acc;
-  }
-  __syncthreads();
-  for (int e = t; e < qb * qb; e += CPL_THREADS) {
-    const int i = e / qb, j = e - i * qb;
-    s_x[e] = SB[(size_t)pos_of_seg(s_k[1][i], n) * n + pos_of_seg(s_k[1][j], n)];
-  }
-  __syncthreads();
-  const int q2 = 2 * qb;
-  for (int e = t; e < qb * qb; e += CPL_THREADS) {
-    const int i = e / qb, j = e - i * qb;
+  for (int e = t; e < K * K; e += CPL_THREADS) {   // L = I + O G
+    const int i = e / K, j = e - i * K;
     double acc = i == j ? 1.0 : 0.0;
-    for (int m = 0; m < qb; m++) acc -= s_h[i * qb + m] * s_x[m * qb + j];
-    s_aug[i * q2 + j] = acc;
-    s_aug[i * q2 + qb + j] = s_h[e];
+    for (int m = 0; m < K; m++) acc += sR[i * K + m] * sG[m * K + j];
+    sL[e] = acc;
   }
   __syncthreads();
-  // Gauss-Jordan with partial pivoting: the right half becomes M
-  for (int c = 0; c < qb; c++) {
+  // Gauss-Jordan with partial pivoting on [L | R]: R becomes M = L^-1 O
+  for (int c = 0; c < K; c++) {
     if (t == 0) {
       int r = c;
-      for (int x = c + 1; x < qb; x++)
-        if (fabs(s_aug[x * q2 + c]) > fabs(s_aug[r * q2 + c])) r = x;
-      s_piv = s_aug[r * q2 + c] != 0.0 ? r : -1;
+      for (int x = c + 1; x < K; x++)
+        if (fabs(sL[x * K + c]) > fabs(sL[r * K + c])) r = x;
+      s_piv = sL[r * K + c] != 0.0 ? r : -1;
     }
     __syncthreads();
     const int r = s_piv;
@@ -742,77 +740,94 @@ __device__ __forceinline__ void geo_couple_body(const IbmParams& p, const GeoBuf
       return;
     }
     if (r != c)
-      for (int e = t; e < q2; e += CPL_THREADS) {
-        const double x = s_aug[c * q2 + e];
-        s_aug[c * q2 + e] = s_aug[r * q2 + e];
-        s_aug[r * q2 + e] = x;
+      for (int e = t; e < 2 * K; e += CPL_THREADS) {
+        double* A = e < K ? sL : sR;
+        const int cc = e < K ? e : e - K;
+        const double x = A[c * K + cc];
+        A[c * K + cc] = A[r * K + cc];
+        A[r * K + cc] = x;
       }
     __syncthreads();
-    const double inv = 1.0 / s_aug[c * q2 + c];
+    const double inv = 1.0 / sL[c * K + c];
     __syncthreads();
-    for (int e = t; e < q2; e += CPL_THREADS) s_aug[c * q2 + e] *= inv;
-    if (t < qb) s_f[t] = t == c ? 0.0 : s_aug[t * q2 + c];
+    for (int e = t; e < 2 * K; e += CPL_THREADS) {
+      if (e < K) sL[c * K + e] *= inv;
+      else sR[c * K + e - K] *= inv;
+    }
+    if (t < K) s_f[t] = t == c ? 0.0 : sL[t * K + c];
     __syncthreads();
-    for (int e = t; e < qb * q2; e += CPL_THREADS) {
-      const int rr = e / q2, cc = e - rr * q2;
-      if (rr != c) s_aug[e] -= s_f[rr] * s_aug[c * q2 + cc];
+    for (int e = t; e < 2 * K * K; e += CPL_THREADS) {
+      const int rr = e / (2 * K), cc = e - rr * 2 * K;
+      if (rr == c) continue;
+      if (cc < K) sL[rr * K + cc] -= s_f[rr] * sL[c * K + cc];
+      else sR[rr * K + cc - K] -= s_f[rr] * sR[c * K + cc - K];
     }
     __syncthreads();
   }
-  double* C0 = G.cpl + (size_t)slot * 2 * CPL_K * CPL_K;
-  double* Mo = C0 + CPL_K * CPL_K;
-  for (int e = t; e < pa * qb; e += CPL_THREADS) C0[e] = s_c0[e];
-  for (int e = t; e < qb * qb; e += CPL_THREADS) Mo[e] = s_aug[(e / qb) * q2 + qb + (e % qb)];
-  // b's adjoint list without the nodes a claims, in order; new segment offsets
-  for (int k = t; k <= n; k += CPL_THREADS) s_seg[k] = 0;
-  __syncthreads();
-  const int* ra = gs->rect[a];
-  const int raw = ra[1] - ra[0] + 1;
-  const int* MA = G.map + ((size_t)slot * W + a) * 2 * (size_t)p.map_cap;
-  const size_t lb = (size_t)slot * p.cap + (size_t)b * p.capw;
-  int* aij = G.adj_ij + lb * 2;
-  int* aseg = G.adj_seg + lb;
-  const int cntb = gs->cnt[b];
-  int kept = 0;
-  for (int c0 = 0; c0 < cntb; c0 += CPL_THREADS) {
-    const int e = c0 + t;
-    int keep = 0, i = 0, j = 0, k = 0;
-    if (e < cntb) {
-      i = aij[2 * e];
-      j = aij[2 * e + 1];
-      k = aseg[e];
-      const bool in_a = i >= ra[0] && i <= ra[1] && j >= ra[2] && j <= ra[3] &&
-                        MA[(i - ra[0]) + raw * (j - ra[2])] != INT_MAX;
-      keep = !in_a;
+  double* Mo = G.cpl + (size_t)slot * CPL_KT * CPL_KT;
+  for (int e = t; e < K * K; e += CPL_THREADS) Mo[e] = sR[e];
+  // each wing's adjoint list without the nodes a lower wing owns, in order;
+  // new segment offsets
+  for (int b = 1; b < W; b++) {
+    bool lower = false;
+    for (int a = 0; a < b; a++) lower = lower || s_flag[a * MAXW + b];
+    if (!lower) continue;
+    for (int k = t; k <= n; k += CPL_THREADS) s_seg[k] = 0;
+    __syncthreads();
+    const size_t lb = (size_t)slot * p.cap + (size_t)b * p.capw;
+    int* aij = G.adj_ij + lb * 2;
+    int* aseg = G.adj_seg + lb;
+    const int cntb = gs->cnt[b];
+    int kept = 0;
+    for (int c0 = 0; c0 < cntb; c0 += CPL_THREADS) {
+      const int e = c0 + t;
+      int keep = 0, i = 0, j = 0, k = 0;
+      if (e < cntb) {
+        i = aij[2 * e];
+        j = aij[2 * e + 1];
+        k = aseg[e];
+        bool owned = false;
+        for (int a = 0; a < b && !owned; a++) {
+          if (!s_flag[a * MAXW + b]) continue;
+          const int* ra = gs->rect[a];
+          const int* MA = G.map + ((size_t)slot * W + a) * 2 * (size_t)p.map_cap;
+          owned = i >= ra[0] && i <= ra[1] && j >= ra[2] && j <= ra[3] &&
+                  MA[(i - ra[0]) + (ra[1] - ra[0] + 1) * (j - ra[2])] != INT_MAX;
+        }
+        keep = !owned;
+      }
+      const int total = block_scan(s_scan, keep);
+      if (keep) {
+        const int dst = kept + s_scan[t];
+        aij[2 * dst] = i;
+        aij[2 * dst + 1] = j;
+        aseg[dst] = k;
+        atomicAdd(&s_seg[k], 1);
+      }
+      kept += total;
+      __syncthreads();
     }
-    const int total = block_scan(s_scan, keep);
-    if (keep) {
-      const int dst = kept + s_scan[t];
-      aij[2 * dst] = i;
-      aij[2 * dst + 1] = j;
-      aseg[dst] = k;
-      atomicAdd(&s_seg[k], 1);
+    int* soff = G.seg_off + ((size_t)slot * W + b) * (n + 1);
+    if (t == 0) {
+      int run = 0;
+      for (int k = 0; k < n; k++) {
+        const int c = s_seg[k];
+        soff[k] = run;
+        run += c;
+      }
+      soff[n] = run;
+      gs->adj_n -= cntb - kept;
+      gs->cnt[b] = kept;
     }
-    kept += total;
     __syncthreads();
   }
-  int* soff = G.seg_off + ((size_t)slot * W + b) * (n + 1);
   if (t == 0) {
-    int run = 0;
-    for (int k = 0; k < n; k++) {
-      const int c = s_seg[k];
-      soff[k] = run;
-      run += c;
+    gs->cpl_n = K;
+    for (int w = 0; w <= W; w++) gs->cpl_off[w] = s_off[w];
+    for (int x = 0; x < K; x++) {
+      gs->cpl_k[x] = s_k[x];
+      gs->cpl_w[x] = s_w[x];
     }
-    soff[n] = run;
-    gs->adj_n -= cntb - kept;
-    gs->cnt[b] = kept;
-    gs->cpl_a = a;
-    gs->cpl_b = b;
-    gs->cpl_p = pa;
-    gs->cpl_q = qb;
-    for (int x = 0; x < pa; x++) gs->cpl_ka[x] = s_k[0][x];
-    for (int x = 0; x < qb; x++) gs->cpl_kb[x] = s_k[1][x];
     gs->cpl_on = 1;
   }
 }
@@ -823,75 +838,45 @@ __global__ __launch_bounds__(CPL_THREADS) void k_geo_couple(IbmParams p, GeoBuff
 
 // The coupling correction of the flow pass (one workgroup), after the
 // per-wing mat-vecs: F.yvec holds u = blockdiag(S_ww)^-1 c on entry and the
-// solution of the coupled system on exit (both components).
+// solution of the coupled system on exit (both components):
+// z = u[K], r = M z, y_w = u_w - S_ww^-1[:, K_w] r_w.
 __device__ __forceinline__ void ibm_couple_body(const IbmParams& p, const GeoBuffers& G, int slot,
                                                 const FlowBuffers& F) {
   const GeoStep* gs = G.step + slot;
   if (!gs->cpl_on) return;
-  __shared__ int s_pa[CPL_K], s_pb[CPL_K];
-  __shared__ double s_v[2 * CPL_K];
-  const int t = threadIdx.x, n = p.SPW;
-  const int a = gs->cpl_a, b = gs->cpl_b, pa = gs->cpl_p, qb = gs->cpl_q;
-  const double* C0 = G.cpl + (size_t)slot * 2 * CPL_K * CPL_K;
-  const double* Mm = C0 + CPL_K * CPL_K;
-  const double* SA = G.sinv + ((size_t)slot * p.W + a) * (size_t)n * n;
-  const double* SB = G.sinv + ((size_t)slot * p.W + b) * (size_t)n * n;
-  double* ya = F.yvec + 2 * (size_t)a * n;
-  double* yb = F.yvec + 2 * (size_t)b * n;
-  if (t < pa) s_pa[t] = pos_of_seg(gs->cpl_ka[t], n);
-  if (t < qb) s_pb[t] = pos_of_seg(gs->cpl_kb[t], n);
+  __shared__ int s_pos[CPL_KT], s_w[CPL_KT];
+  __shared__ double s_z[2 * CPL_KT], s_r[2 * CPL_KT];
+  const int t = threadIdx.x, n = p.SPW, K = gs->cpl_n;
+  const double* Mm = G.cpl + (size_t)slot * CPL_KT * CPL_KT;
+  if (t < K) {
+    s_pos[t] = pos_of_seg(gs->cpl_k[t], n);
+    s_w[t] = gs->cpl_w[t];
+  }
   __syncthreads();
-  // w = C0^T u_a[K_a]
-  if (t < 2 * qb) {
-    const int j = t >> 1, c = t & 1;
+  if (t < 2 * K) {
+    const int i = t >> 1, c = t & 1;
+    s_z[t] = F.yvec[2 * ((size_t)s_w[i] * n + s_pos[i]) + c];
+  }
+  __syncthreads();
+  if (t < 2 * K) {
+    const int i = t >> 1, c = t & 1;
     double acc = 0.0;
-    for (int i = 0; i < pa; i++) acc += C0[i * qb + j] * ya[2 * s_pa[i] + c];
-    s_v[t] = acc;
+    for (int j = 0; j < K; j++) acc += Mm[i * K + j] * s_z[2 * j + c];
+    s_r[t] = acc;
   }
   __syncthreads();
-  // v = u_b - S_bb^-1[:, K_b] w
-  for (int e = t; e < 2 * n; e += CPL_THREADS) {
-    const int r = e >> 1, c = e & 1;
-    const double* row = SB + (size_t)r * n;
-    double acc = yb[e];
-    for (int j = 0; j < qb; j++) acc -= row[s_pb[j]] * s_v[2 * j + c];
-    yb[e] = acc;
-  }
-  __syncthreads();
-  // t = M v[K_b]
-  double tv = 0.0;
-  if (t < 2 * qb) {
-    const int i = t >> 1, c = t & 1;
-    for (int j = 0; j < qb; j++) tv += Mm[i * qb + j] * yb[2 * s_pb[j] + c];
-  }
-  __syncthreads();
-  if (t < 2 * qb) s_v[t] = tv;
-  __syncthreads();
-  // y_b = v + S_bb^-1[:, K_b] t
-  for (int e = t; e < 2 * n; e += CPL_THREADS) {
-    const int r = e >> 1, c = e & 1;
-    const double* row = SB + (size_t)r * n;
-    double acc = yb[e];
-    for (int j = 0; j < qb; j++) acc += row[s_pb[j]] * s_v[2 * j + c];
-    yb[e] = acc;
-  }
-  __syncthreads();
-  // z = C0 y_b[K_b]
-  double zv = 0.0;
-  if (t < 2 * pa) {
-    const int i = t >> 1, c = t & 1;
-    for (int j = 0; j < qb; j++) zv += C0[i * qb + j] * yb[2 * s_pb[j] + c];
-  }
-  __syncthreads();
-  if (t < 2 * pa) s_v[t] = zv;
-  __syncthreads();
-  // y_a = u_a - S_aa^-1[:, K_a] z
-  for (int e = t; e < 2 * n; e += CPL_THREADS) {
-    const int r = e >> 1, c = e & 1;
-    const double* row = SA + (size_t)r * n;
-    double acc = ya[e];
-    for (int i = 0; i < pa; i++) acc -= row[s_pa[i]] * s_v[2 * i + c];
-    ya[e] = acc;
+  for (int w = 0; w < p.W; w++) {
+    const int k0 = gs->cpl_off[w], k1 = gs->cpl_off[w + 1];
+    if (k0 == k1) continue;
+    const double* SW = G.sinv + ((size_t)slot * p.W + w) * (size_t)n * n;
+    double* yw = F.yvec + 2 * (size_t)w * n;
+    for (int e = t; e < 2 * n; e += CPL_THREADS) {
+      const int r = e >> 1, c = e & 1;
+      const double* row = SW + (size_t)r * n;
+      double acc = yw[e];
+      for (int x = k0; x < k1; x++) acc -= row[s_pos[x]] * s_r[2 * x + c];
+      yw[e] = acc;
+    }
   }
 }
 
@@ -906,7 +891,8 @@ void launch_geometry(hipStream_t s, const IbmParams& p, const GeoBuffers& G, Dev
   hipLaunchKernelGGL(k_geo_wing, grid, dim3(GEO_THREADS), geo_wing_lds_bytes(p.SPW), s, p, G, st, slot0, n0, fidx0);
   if (with_inverse) {
     hipLaunchKernelGGL(k_geo_solve, grid, dim3(GEO_THREADS), geo_solve_lds_bytes(p), s, p, G, st, slot0);
-    if (p.W >= 2) hipLaunchKernelGGL(k_geo_couple, dim3(count), dim3(CPL_THREADS), 0, s, p, G, st, slot0);
+    if (p.W >= 2)
+      hipLaunchKernelGGL(k_geo_couple, dim3(count), dim3(CPL_THREADS), geo_couple_lds_bytes(), s, p, G, st, slot0);
   }
 }
 
@@ -1029,7 +1015,7 @@ __device__ __forceinline__ void ibm_spread_body(const IbmParams& p, const GeoBuf
   __syncthreads();
   const int* soff = G.seg_off + ((size_t)slot * p.W + w) * (n + 1);
   const int a0 = soff[k0], a1 = soff[k1];
-  const bool cpl = gs->cpl_on && gs->cpl_a == w;
+  const int cpl0 = gs->cpl_on ? gs->cpl_off[w + 1] : 0, cpl1 = gs->cpl_on ? gs->cpl_n : 0;
   const size_t lb = (size_t)slot * p.cap + (size_t)w * p.capw;
   const int* aij = G.adj_ij + lb * 2;
   const int* aseg = G.adj_seg + lb;
@@ -1046,18 +1032,16 @@ __device__ __forceinline__ void ibm_spread_body(const IbmParams& p, const GeoBuf
       gx = gx + phi_ * s_y[e][0];
       gy = gy + phi_ * s_y[e][1];
     }
-    if (cpl) {  // the other wing's coupling segments whose boxes hold the node
-      const int wb = gs->cpl_b;
-      for (int x = 0; x < gs->cpl_q; x++) {
-        const int l = gs->cpl_kb[x];
-        const size_t sl = (size_t)slot * p.nseg + (size_t)wb * n + l;
-        const int* bl = G.box + sl * 4;
-        if (i < bl[0] || i > bl[1] || j < bl[2] || j > bl[3]) continue;
-        const double phi_ = G.phx[sl * BOXW + (i - bl[0])] * G.phy[sl * BOXW + (j - bl[2])];
-        const size_t yq = (size_t)wb * n + pos_of_seg(l, n);
-        gx = gx + phi_ * F.yvec[2 * yq];
-        gy = gy + phi_ * F.yvec[2 * yq + 1];
-      }
+    // the coupling segments of higher wings whose boxes hold the node
+    for (int x = cpl0; x < cpl1; x++) {
+      const int l = gs->cpl_k[x], wb = gs->cpl_w[x];
+      const size_t sl = (size_t)slot * p.nseg + (size_t)wb * n + l;
+      const int* bl = G.box + sl * 4;
+      if (i < bl[0] || i > bl[1] || j < bl[2] || j > bl[3]) continue;
+      const double phi_ = G.phx[sl * BOXW + (i - bl[0])] * G.phy[sl * BOXW + (j - bl[2])];
+      const size_t yq = (size_t)wb * n + pos_of_seg(l, n);
+      gx = gx + phi_ * F.yvec[2 * yq];
+      gy = gy + phi_ * F.yvec[2 * yq + 1];
     }
     gout[2 * a] = gx;
     gout[2 * a + 1] = gy;
@@ -1123,7 +1107,7 @@ void launch_batch_geometry(hipStream_t s, const EpDesc* eps, const int* kb, cons
   if (n_items <= 0) return;
   hipLaunchKernelGGL(kb_geo_wing, dim3(n_items), dim3(GEO_THREADS), lds_wing, s, eps, items, kb, k0);
   hipLaunchKernelGGL(kb_geo_solve, dim3(n_items), dim3(GEO_THREADS), lds_solve, s, eps, items, kb, k0);
-  hipLaunchKernelGGL(kb_geo_couple, dim3(n_items), dim3(CPL_THREADS), 0, s, eps, items, kb, k0);
+  hipLaunchKernelGGL(kb_geo_couple, dim3(n_items), dim3(CPL_THREADS), geo_couple_lds_bytes(), s, eps, items, kb, k0);
 }
 
 template <typename S>

@@ -14,7 +14,8 @@ constexpr int MAXW = LBM_MAX_WINGS;
 constexpr int BMAX = 24;          // largest IBM band half-width the solver accepts
 constexpr int BOXW = 6;           // nodes per axis of a delta support box (|r| <= 2.5)
 constexpr int BOXC = BOXW * BOXW; // cells of a support box
-constexpr int CPL_K = 32;         // most segments per wing in the coupling of two wings
+constexpr int CPL_K = 32;         // most coupling segments of one wing
+constexpr int CPL_KT = 64;        // most coupling segments over all wings
 
 // Error codes raised by kernels into DevStep::error (first one wins).
 enum DevError : int {
@@ -22,7 +23,7 @@ enum DevError : int {
   DERR_BOX_EDGE = 1,       // support box within 3 nodes of the lattice edge
   DERR_MAP = 2,            // a wing's support rectangle exceeds its claim map
   DERR_BAND = 3,           // IBM band wider than BMAX
-  DERR_WINGS_COUPLED = 4,  // coupling beyond one pair of wings / CPL_K segments
+  DERR_WINGS_COUPLED = 4,  // more coupling segments than CPL_K of a wing / CPL_KT in all
   DERR_NOT_SPD = 5,        // non-positive pivot in the band factorisation
   DERR_MOTION = 6,         // motion source not ready
 };
@@ -67,12 +68,14 @@ struct GeoStep {
   int bw[MAXW];            // band half-width of S per wing (zig-zag order)
   int rect[MAXW][4];       // i0, i1, j0, j1 (0-based) of the union of the wing's boxes
   int band[2];             // rows j0..j1 spanned by all the rectangles (j0 > j1: none)
-  // Coupled pair: wings cpl_a < cpl_b whose support boxes share lattice
-  // nodes (cpl_on = 1).  cpl_ka / cpl_kb are the wing-local segments whose
-  // boxes meet a box of the other wing (ascending); the shared nodes belong to
-  // cpl_a, whose segments claim them first.
-  int cpl_on, cpl_a, cpl_b, cpl_p, cpl_q;
-  int cpl_ka[CPL_K], cpl_kb[CPL_K];
+  // Coupled wings (cpl_on = 1): wings whose support boxes share lattice
+  // nodes.  cpl_k[cpl_off[w] .. cpl_off[w+1]) are the wing-local segments of
+  // wing w whose boxes meet a box of another wing (ascending), cpl_w their
+  // wing; cpl_n = cpl_off[W] in all.  A shared node belongs to the lowest of
+  // its wings, whose segments claim it first.
+  int cpl_on, cpl_n;
+  int cpl_off[MAXW + 1];
+  int cpl_k[CPL_KT], cpl_w[CPL_KT];
   Frame frame;
 };
 
@@ -111,7 +114,7 @@ struct GeoBuffers {
   int* adj_ij;                  // [slots][cap][2] (i, j) 0-based
   int* adj_seg;                 // [slots][cap] claiming segment (wing-local)
   double* sinv;                 // [slots][W][SPW][SPW] inverse of S = D D^T, zig-zag order
-  double* cpl;                  // [slots][2][CPL_K][CPL_K] coupling C0 (p x q) and M (q x q)
+  double* cpl;                  // [slots][CPL_KT][CPL_KT] coupling matrix M (cpl_n x cpl_n)
   int* map;                     // [slots][W][2][map_cap] claim maps (owner, lowest position)
   const Frame* frame;           // per step of the current call
   const double* step_table;     // file mode: step_count * ncols

@@ -45,7 +45,7 @@ def device_bytes(cfg: SolverConfig) -> int:
         r = float(max((m[:, 0] + cfg.xc - cfg.xp) ** 2 + (m[:, 1] + cfg.yc - cfg.yp) ** 2)) ** 0.5
         side = math.ceil(2.0 * r / cfg.scale) + 12
         per_slot = W * spw * spw * 8 + W * 2 * side * side * 4 + 36 * W * spw * 12 + W * spw * 160
-        per_slot += 2 * 32 * 32 * 8 if W > 1 else 0     # coupled-pair matrices C0 and M
+        per_slot += 64 * 64 * 8 if W > 1 else 0         # coupling matrix M
         nslot = max(1, min(16, (256 << 20) // per_slot))
         total += 2 * nslot * per_slot
     return int(total)

@@ -245,15 +245,39 @@ def test_coupled_wings_batch_equals_alone(lib):
     b.close()
 
 
-def test_three_coupled_wings_are_an_error(lib):
-    """Only one coupled pair per step is solved; three wings in a chain are
-    reported as a failed step."""
-    from lbm_mutiwing_amd import LbmError
+def test_three_coupled_wings_match_oracle(lib, oracle_mod):
+    """Three wings in a chain (the supports of wings 1-2 and 2-3 share
+    nodes): one system over all the segments, each shared node owned by the
+    lowest of its wings, as the oracle's dense LU over all three."""
     cfg = _coupled_case()
     cfg.wing_count = 3
+    n = 120
+    motions = [WingMotion(pitch=0.05, heave=0.02, freq=0.25, phase_p=0.0, phase_h=0.5),
+               WingMotion(pitch=0.08, heave=0.03, freq=0.25, phase_p=1.0, phase_h=2.0),
+               WingMotion(pitch=0.06, heave=0.02, freq=0.25, phase_p=2.0, phase_h=1.0)]
+    rows = sinusoidal_motion_table(cfg, motions, n + 2)
+    g, o = pair(cfg, oracle_mod, rows)
+    print("three coupled wings: 120 steps", flush=True)
+    rg, ro = g.advance(n, want_rows=True), o.advance(n, want_rows=True)
+    ig, io = g.ibm(), o.ibm()
+    assert np.array_equal(ig[0], io[0]) and np.array_equal(ig[1], io[1])
+    assert all((io[1] == w).any() for w in (1, 2, 3))
+    errs = [rel(a, b) for a, b in zip(g.fields()[:3], o.fields()[:3])]
+    print(f"three coupled wings: rho/u/v relative L2 {errs}", flush=True)
+    assert max(errs) < FIELD_TOL
+    fr = slice(1, 1 + 3 * cfg.wing_count + 3)
+    assert rel(rg.force_rows[:, fr], ro.force_rows[:, fr]) < FORCE_TOL
+    assert rel(rg.row_sum, ro.row_sum) < FORCE_TOL
+
+
+def test_too_many_coupling_segments_are_an_error(lib):
+    """Wings overlapping over most of their chord couple more than 32
+    segments of a wing: reported as a failed step."""
+    from lbm_mutiwing_amd import LbmError
+    cfg = _coupled_case(dx=0.2)
     s = Solver(cfg)
-    s.set_motion_table(sinusoidal_motion_table(cfg, [WingMotion(), WingMotion(), WingMotion()], 10))
-    with pytest.raises(LbmError, match="more than two wings"):
+    s.set_motion_table(sinusoidal_motion_table(cfg, [WingMotion(), WingMotion()], 10))
+    with pytest.raises(LbmError, match="more than 32 segments"):
         s.advance(2)
 
 
