@@ -43,6 +43,16 @@ def test_config_struct_layout_matches_header():
     assert ctypes.sizeof(LbmConfigC) == LbmConfigC.yb0.offset + 8
 
 
+def test_dynamics_struct_layout_matches_header():
+    from lbm_mutiwing_amd.solver import Dynamics
+    txt = open(os.path.join(ROOT, "include", "lbm_hip.h")).read()
+    body = txt[txt.index("typedef struct lbm_dynamics {"):txt.index("} lbm_dynamics;")]
+    fields = re.findall(r"^\s*(int32_t|double)\s+([\w, ]+);", body, re.M)
+    names = [n.strip() for _, group in fields for n in group.split(",")]
+    assert names == [f[0] for f in Dynamics._fields_]
+    assert Dynamics.solid_den.offset == 16 and Dynamics.alphad.offset == 48 and ctypes.sizeof(Dynamics) == 56
+
+
 def test_naca_generator_reproduces_reference_marker_file():
     ref = np.loadtxt(os.path.join(ROOT, "tests", "golden", "boundary.dat"))
     gen = naca0012_markers(101)
