@@ -10,9 +10,10 @@ force row, i.e. the reference's whole run-loop body.
 
 Prints one JSON line: value = whole-job MLUPS (Nx*Ny*steps / s / 1e6) with
 all data resident in HBM, the roofline of the fused lattice kernel measured
-with HIP events on the library's stream during the timed steps, and a CPU
-baseline (the FP64 oracle on the host cores, a bounded sample of the same
-workload).
+with HIP events on the library's stream during the timed steps, the ceiling
+of that kernel's memory pattern, and a CPU baseline (the FP64 oracle on the
+host cores, a bounded sample of the same workload; beside it the same on one
+core, and the 1024x512 workload on all cores and on one).
 
 Two more workloads, each printing its own JSON line:
   --mode batch  64 uneven episodes (lattices log-uniform 512x256 .. 2048x1024,
@@ -131,8 +132,35 @@ def cpu_baseline(name, seed, budget_s, precision):
     err = {"steps": 1 + steps, "rho": rel(fg[0], fo[0]), "u": rel(fg[1], fo[1]), "v": rel(fg[2], fo[2]),
            "force_rows": rel(rg.force_rows[:, fr], ro[:, fr]),
            "bar": "1e-10 fields, 1e-9 forces (FP64); 1e-4 fields, 1e-3 forces (FP32-shifted)"}
-    return ({"value": cfg.nx * cfg.ny * steps / t / 1e6, "unit": "MLUPS", "cores": cores,
-             "kind": "port", "sample": f"{name} steps 2..{1 + steps} ({steps} steps, {t:.1f} s)"}, err)
+    base = {"value": cfg.nx * cfg.ny * steps / t / 1e6, "unit": "MLUPS", "cores": cores,
+            "kind": "port", "sample": f"{name} steps 2..{1 + steps} ({steps} steps, {t:.1f} s)"}
+    base["single_core"] = _oracle_rate(name, seed, 1, min(8.0, budget_s), 4)
+    other = "c1" if name != "c1" else "c2"
+    base[f"{other}_all_cores"] = _oracle_rate(other, seed, cores, min(5.0, budget_s), 50)
+    base[f"{other}_single_core"] = _oracle_rate(other, seed, 1, min(5.0, budget_s), 10)
+    oracle.set_threads(cores)
+    return base, err
+
+
+def _oracle_rate(name, seed, threads, budget_s, max_steps):
+    """MLUPS of the FP64 oracle on `threads` host cores over the first steps
+    of a workload (step 1 untimed), stopping after budget_s or max_steps."""
+    from lbm_mutiwing_amd import sinusoidal_motion_table
+    from oracle import oracle
+    oracle.set_threads(threads)
+    cfg, mot = workload(name, seed)
+    o = oracle.OracleSolver(cfg)
+    o.set_motion_table(sinusoidal_motion_table(cfg, mot, max_steps + 4))
+    o.advance(1)
+    steps, t = 0, 0.0
+    while t < budget_s and steps < max_steps:
+        t0 = time.perf_counter()
+        o.advance(2 + steps)
+        t += time.perf_counter() - t0
+        steps += 1
+    o.close()
+    return {"value": round(cfg.nx * cfg.ny * steps / t / 1e6, 3), "cores": threads,
+            "sample": f"{name} steps 2..{1 + steps} ({steps} steps, {t:.1f} s)"}
 
 
 def note(msg):
