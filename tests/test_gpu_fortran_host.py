@@ -289,3 +289,38 @@ def test_file_mode_dynamics_checkpoint_and_restart(host, tmp_path):
     r = subprocess.run([host, "rs.nml"], cwd=tmp_path, capture_output=True, timeout=300)
     assert r.returncode == 0, r.stdout.decode() + r.stderr.decode()
     assert (tmp_path / "rs" / "force.dat").read_text().splitlines() == full_f[60:]
+
+
+def test_python_file_mode_restarts_from_fortran_checkpoint(host, tmp_path):
+    """The Python host's file mode (lbm_mutiwing_amd/file_mode.py) on the same
+    case file: a full run writes the Fortran host's force.dat as text, and a
+    restart from the checkpoint the Fortran host wrote at step 60 continues
+    with the uninterrupted run's records (coupling on, so the body state
+    travels through the file too)."""
+    from lbm_mutiwing_amd import file_mode
+    from lbm_mutiwing_amd.case import read_case
+    cfg = preset("small")
+    n = 100
+    rows = sinusoidal_motion_table(cfg, seeded_motions(cfg, 31), n)
+    _write_case(tmp_path, cfg, rows)
+    dyn_nml = ", solve_dynamic_equations=.true., solid_den=50.0, par1=-2.0, par2=50.0, checkpoint_every=60"
+    (tmp_path / "full.nml").write_text(_namelist(cfg, out_dir="full", extra=dyn_nml))
+    print("fortran host: file mode with coupling, checkpoint at step 60", flush=True)
+    r = subprocess.run([host, "full.nml"], cwd=tmp_path, capture_output=True, timeout=300)
+    assert r.returncode == 0, r.stdout.decode() + r.stderr.decode()
+    full_f = (tmp_path / "full" / "force.dat").read_text().splitlines()
+    (tmp_path / "py.nml").write_text(_namelist(cfg, out_dir="py", extra=dyn_nml))
+    (tmp_path / "pyrs.nml").write_text(
+        _namelist(cfg, out_dir="pyrs", extra=dyn_nml + ", restart=.true., restart_dir='full'"))
+    cwd = os.getcwd()
+    os.chdir(tmp_path)
+    try:
+        print("python host: the same case, then a restart from the Fortran checkpoint", flush=True)
+        assert file_mode.run_case(read_case("py.nml")) == n
+        assert file_mode.run_case(read_case("pyrs.nml")) == n
+    finally:
+        os.chdir(cwd)
+    assert (tmp_path / "py" / "force.dat").read_text().splitlines() == full_f
+    assert (tmp_path / "pyrs" / "force.dat").read_text().splitlines() == full_f[60:]
+    assert (tmp_path / "py" / "saved_flow_feild.dat").read_bytes() == \
+        (tmp_path / "full" / "saved_flow_feild.dat").read_bytes()
