@@ -426,7 +426,8 @@ __device__ void inverse_column(const double* band, const double* invd, int n, in
   double win[BW];
 #pragma unroll
   for (int d = 0; d < BW; d++) win[d] = 0.0;
-  for (int m = 0; m < n; m++) {
+  // rows above col give z = 0 (L is unit lower triangular): start at col
+  for (int m = col; m < n; m++) {
     double z = m == col ? 1.0 : 0.0;
     const double* Lr = band + m * Dd;
 #pragma unroll
