@@ -72,10 +72,12 @@ def restore_checkpoint(solver: Solver, ck_dir: str) -> Tuple[dict, int]:
     return ck.ring, ck.cycle + 1
 
 
-def run_case(case: dict, lib=None) -> int:
-    """Run the case to the end of its motion table; returns the last step."""
+def run_case(case: dict, lib=None, solver_cls=Solver) -> int:
+    """Run the case to the end of its motion table; returns the last step.
+    ``solver_cls`` is the host class bound to the library (the HIP one
+    unless a test drives another implementation of the same interface)."""
     cfg = case_config(case, MOTION_FILE)
-    solver = Solver(cfg, lib)
+    solver = solver_cls(cfg) if lib is None else solver_cls(cfg, lib)
     try:
         dyn = case_dynamics(case)
         if dyn is not None:
