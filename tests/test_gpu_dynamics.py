@@ -135,3 +135,28 @@ def test_dynamics_fp32_storage_within_fp32_bar(lib, oracle_mod):
     avg_g = rg.force_rows[20:, 1:10].mean(0)
     avg_o = ro.force_rows[20:, 1:10].mean(0)
     assert np.all(np.abs(avg_g - avg_o) <= 1e-3 * np.maximum(np.abs(avg_o).max(), 1e-3))
+
+
+def test_dynamics_c1_match_oracle(lib, oracle_mod):
+    """The 1024x512 two-wing workload (chord 64) with wing 1 coupled, 200
+    steps from a drift of 0.02 nodes per step half a node from a move: the
+    same bars as the small case."""
+    cfg = preset("c1")
+    rows = sinusoidal_motion_table(cfg, seeded_motions(cfg, 12), 204)
+    g, o = Solver(cfg), oracle_mod.OracleSolver(cfg)
+    for s in (g, o):
+        s.set_motion_table(rows)
+        s.set_dynamics(Dynamics(solid_den=40.0, par1=-2.0, par2=50.0))
+        rel0 = np.zeros((11, cfg.wing_count))
+        rel0[BODY_REL_FIELDS.index("v"), 0] = 0.02
+        rel0[BODY_REL_FIELDS.index("pos"), 0] = 0.5
+        s.set_body_rel(rel0)
+    print("dynamics c1: 200 steps", flush=True)
+    rg, ro = g.advance(200, want_rows=True), o.advance(200, want_rows=True)
+    assert g.state()["movenum"] == o.state()["movenum"] >= 1
+    assert np.allclose(g.body_rel(), o.body_rel(), rtol=1e-9, atol=1e-14)
+    for a, b in zip(g.fields()[:3], o.fields()[:3]):
+        assert rel(a, b) < FIELD_TOL
+    avg_g = rg.force_rows[50:, 1:10].mean(0)
+    avg_o = ro.force_rows[50:, 1:10].mean(0)
+    assert np.all(np.abs(avg_g - avg_o) <= FORCE_TOL * np.maximum(np.abs(avg_o), 1e-3))
