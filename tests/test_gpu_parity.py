@@ -226,20 +226,23 @@ def test_coupled_wings_match_oracle(lib, oracle_mod):
 
 
 def test_coupled_wings_batch_equals_alone(lib):
-    """A coupled instance advanced in a batch ends bitwise as alone."""
+    """Coupled instances (a pair, and a chain of three) advanced in a batch
+    end bitwise as alone."""
     from lbm_mutiwing_amd import Batch
-    cfgs = [_coupled_case(), preset("small")]
-    motions = [WingMotion(pitch=0.05), WingMotion(pitch=0.08, heave=0.03)]
+    chain = _coupled_case()
+    chain.wing_count = 3
+    cfgs = [_coupled_case(), preset("small"), chain]
+    motions = [WingMotion(pitch=0.05), WingMotion(pitch=0.08, heave=0.03), WingMotion(pitch=0.06)]
     alone, mixed = [], []
     for c in cfgs:
         for lst in (alone, mixed):
             s = Solver(c)
-            s.set_motion_table(sinusoidal_motion_table(c, motions, 80))
+            s.set_motion_table(sinusoidal_motion_table(c, motions[:c.wing_count], 80))
             lst.append(s)
     ra = [s.advance(40) for s in alone]
     b = Batch(lib)
     rb = b.advance(mixed, 40)
-    for e in range(2):
+    for e in range(len(cfgs)):
         assert np.array_equal(alone[e].populations()[0], mixed[e].populations()[0]), e
         assert np.array_equal(ra[e].row_sum, rb[e].row_sum), e
     b.close()
