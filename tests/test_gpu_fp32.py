@@ -113,3 +113,28 @@ def test_fp32_move_mesh(lib, oracle_mod):
     assert g.state()["movenum"] == o.state()["movenum"] == -1
     errs = [rel(a, b) for a, b in zip(g.fields()[:3], o.fields()[:3])]
     assert max(errs) < FIELD_TOL_FP32
+
+
+def test_coupled_wings_fp32_within_bars(lib, oracle_mod):
+    """Three wings whose supports chain together (the coupled solve over all
+    their segments) with FP32-shifted populations, against the FP64 oracle."""
+    from lbm_mutiwing_amd import WingMotion
+    cfg = preset("small")
+    cfg.wing_dx = 1.05
+    cfg.wing_count = 3
+    cfg32 = preset("small", precision=PRECISION_FP32_SHIFTED)
+    cfg32.wing_dx = 1.05
+    cfg32.wing_count = 3
+    motions = [WingMotion(pitch=0.05, heave=0.02, freq=0.25), WingMotion(pitch=0.08, heave=0.03, freq=0.25),
+               WingMotion(pitch=0.06, heave=0.02, freq=0.25)]
+    rows = sinusoidal_motion_table(cfg, motions, 202)
+    g, o = Solver(cfg32), oracle_mod.OracleSolver(cfg)
+    for s in (g, o):
+        s.set_motion_table(rows)
+    print("fp32 three coupled wings: 200 steps", flush=True)
+    rg, ro = g.advance(200, want_rows=True), o.advance(200, want_rows=True)
+    for a, b in zip(g.fields()[:3], o.fields()[:3]):
+        assert rel(a, b) < FIELD_TOL_FP32
+    avg_g = rg.force_rows[50:, 1:13].mean(0)
+    avg_o = ro.force_rows[50:, 1:13].mean(0)
+    assert np.all(np.abs(avg_g - avg_o) <= FORCE_TOL_FP32 * np.maximum(np.abs(avg_o).max(), 1e-3))
