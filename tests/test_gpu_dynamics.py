@@ -160,3 +160,22 @@ def test_dynamics_c1_match_oracle(lib, oracle_mod):
     avg_g = rg.force_rows[50:, 1:10].mean(0)
     avg_o = ro.force_rows[50:, 1:10].mean(0)
     assert np.all(np.abs(avg_g - avg_o) <= FORCE_TOL * np.maximum(np.abs(avg_o), 1e-3))
+
+
+def test_dynamics_with_coupled_wings_match_oracle(lib, oracle_mod):
+    """Wing 1 coupled to the flow while the supports of wings 1 and 2 share
+    nodes: the body moves with the force of the coupled system."""
+    cfg = preset("small")
+    cfg.wing_dx = 1.05
+    g, o = Solver(cfg), oracle_mod.OracleSolver(cfg)
+    for s in (g, o):
+        _start(s, cfg, True)
+    print("dynamics with coupled wings", flush=True)
+    rg, ro = g.advance(N, want_rows=True), o.advance(N, want_rows=True)
+    ig, io = g.ibm(), o.ibm()
+    assert np.array_equal(ig[0], io[0]) and np.array_equal(ig[1], io[1])
+    assert g.state()["movenum"] == o.state()["movenum"] >= 1
+    assert np.allclose(g.body_rel(), o.body_rel(), rtol=1e-9, atol=1e-14)
+    for a, b in zip(g.fields()[:3], o.fields()[:3]):
+        assert rel(a, b) < FIELD_TOL
+    assert rel(rg.row_sum, ro.row_sum) < FORCE_TOL
