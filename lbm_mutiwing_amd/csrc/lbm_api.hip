@@ -773,6 +773,8 @@ int step_error(lbm_handle* h, const DevStep& st) {
 int finish_advance(lbm_handle* h, int nsteps, int n_last, bool timed, double* force_rows, double* motion_rows,
                    double* row_sum, double* last_row, double* err);
 
+constexpr int DYN_LATTICE_WAVES = 4;
+
 // Advance of an instance with rigid-body coupling, one step at a time: the
 // geometry of step m needs the body state left by the force of step m-1,
 // and the move_mesh check after step m reads that state on the host.  Each
@@ -817,7 +819,11 @@ int advance_dynamic(lbm_handle* h, int32_t n_last, double* force_rows, double* m
     void* dst = h->fbuf[1 - h->cur];
     AdjMask every_node = adj_mask(h->ip, h->G, slot);
     every_node.W = 0;
-    launch_lattice(h->stream, h->g, h->mrt, src, dst, fs, h->f32, h->lat_bx, every_node, nullptr, nullptr, h->lat_lds);
+    // a tighter wave cap than the tuned one: the step's geometry beside the
+    // lattice kernel is the critical path here
+    const int dyn_lds = lattice_lds_for_waves(h->lat_bx, DYN_LATTICE_WAVES, lds_per_cu(h->device));
+    launch_lattice(h->stream, h->g, h->mrt, src, dst, fs, h->f32, h->lat_bx, every_node, nullptr, nullptr,
+                   dyn_lds >= 0 ? (size_t)dyn_lds : h->lat_lds);
     HIPCHECK(h, hipGetLastError());
     HIPCHECK(h, hipEventRecord(h->ev_step, h->stream));
     launch_bc_ring(h->stream_bc, h->g, h->mrt, h->R, src, dst, fs, h->f32);
