@@ -126,6 +126,8 @@ struct lbm_handle {
   int slot_step[3] = {-1, -1, -1};
   double* err_partial = nullptr;
   double* err_out = nullptr;
+  double* mass_rows = nullptr;       // per-row sums of lbm_get_mass (ny), then the total
+
   // IBM / motion device memory
   std::vector<void*> ibm_allocs;     // geometry slots, flow buffers, markers
   Frame* d_frame = nullptr;
@@ -245,6 +247,7 @@ void free_all(lbm_handle* h) {
   dfree(h->spare.u);
   dfree(h->spare.v);
   dfree(h->err_partial);
+  dfree(h->mass_rows);
   dfree(h->err_out);
   dfree(h->R.left);
   dfree(h->R.right);
@@ -586,6 +589,7 @@ int lbm_create(const lbm_config* cfg, lbm_handle** out) {
   ALLOC(h->spare.v, np);
   ALLOC(h->err_partial, 2 * (size_t)err_blocks(h->g));
   ALLOC(h->err_out, 1);
+  ALLOC(h->mass_rows, (size_t)cfg->ny + 1);
   ALLOC(h->R.left, 9 * (size_t)cfg->ny);
   ALLOC(h->R.right, 9 * (size_t)cfg->ny);
   ALLOC(h->R.mom, 6 * (size_t)cfg->ny);
@@ -1102,6 +1106,17 @@ int lbm_get_fields(lbm_handle* h, double* rho, double* u, double* v, double* pre
     HIPCHECK(h, hipMemcpy(r.data(), h->slot[s].rho, np * sizeof(double), hipMemcpyDeviceToHost));
     for (size_t q = 0; q < np; q++) pre[q] = r[q] / 3.0;
   }
+  return 0;
+}
+
+int lbm_get_mass(lbm_handle* h, double* mass) {
+  if (!h || !mass) return LBM_E_INVALID;
+  HIPCHECK(h, hipStreamSynchronize(h->stream));
+  double* total = h->mass_rows + h->g.ny;
+  launch_mass(h->stream, h->g, h->mrt, h->fbuf[h->cur], h->f32, h->mass_rows, total);
+  HIPCHECK(h, hipGetLastError());
+  HIPCHECK(h, hipStreamSynchronize(h->stream));
+  HIPCHECK(h, hipMemcpy(mass, total, sizeof(double), hipMemcpyDeviceToHost));
   return 0;
 }
 

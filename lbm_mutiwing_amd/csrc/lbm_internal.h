@@ -250,6 +250,10 @@ void launch_err_partial(hipStream_t s, const Grid& g, FieldSlot cur, FieldSlot p
                         int nblocks);
 void launch_err_final(hipStream_t s, const double* partial, int nblocks, double* out);
 int err_blocks(const Grid& g);
+// sum over every node of its nine populations (FP64): rows (ny doubles) gets
+// the per-row sums, out[0] the total; nothing else is written
+void launch_mass(hipStream_t s, const Grid& g, const MrtParams& P, const void* f, bool f32, double* rows,
+                 double* out);
 int measure_pull_copy(int nx, int ny, bool f32, int launches, double* avg_us);
 void launch_shift_x(hipStream_t s, const Grid& g, int dr, const void* f, void* fshift, int planes, bool f32);
 void launch_copy_ring_column(hipStream_t s, const Grid& g, const MrtParams& P, const void* f, int col, double* ring,

@@ -722,6 +722,59 @@ void launch_err_final(hipStream_t s, const double* partial, int nblocks, double*
   hipLaunchKernelGGL(k_err_final, dim3(1), dim3(64), 0, s, partial, nblocks, out);
 }
 
+// Lattice mass (the FP64 mass-conservation check; the reference has none, the
+// reduction follows the pattern of erranalysis, solution.f90:77-130): the sum
+// over every node of its nine populations, in FP64 whatever the storage.  One
+// workgroup per lattice row writes the row's sum (threads take every
+// MASS_THREADS-th node, then a fixed tree in LDS); k_mass_final adds the rows
+// the same way.  The order never changes, so the value is reproducible.
+constexpr int MASS_THREADS = 256;
+
+__device__ __forceinline__ double block_sum_fixed(double a, double* s) {
+  s[threadIdx.x] = a;
+  __syncthreads();
+  for (int w = MASS_THREADS / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  return s[0];
+}
+
+template <typename S>
+__global__ __launch_bounds__(MASS_THREADS) void k_mass_rows(Grid g, MrtParams P, const S* __restrict__ f,
+                                                            double* __restrict__ rows) {
+  __shared__ double s[MASS_THREADS];
+  const int j = blockIdx.x;
+  double a = 0.0;
+  for (int i = threadIdx.x; i < g.nx; i += MASS_THREADS) {
+    const long long o = (long long)j * g.nx + i;
+    double node = 0.0;
+#pragma unroll
+    for (int k = 0; k < 9; k++) node += Store<S>::ld(f, (long long)k * g.ps + o, P.w[k]);
+    a += node;
+  }
+  const double t = block_sum_fixed(a, s);
+  if (threadIdx.x == 0) rows[j] = t;
+}
+
+__global__ __launch_bounds__(MASS_THREADS) void k_mass_final(const double* __restrict__ rows, int ny,
+                                                             double* __restrict__ out) {
+  __shared__ double s[MASS_THREADS];
+  double a = 0.0;
+  for (int j = threadIdx.x; j < ny; j += MASS_THREADS) a += rows[j];
+  const double t = block_sum_fixed(a, s);
+  if (threadIdx.x == 0) out[0] = t;
+}
+
+void launch_mass(hipStream_t s, const Grid& g, const MrtParams& P, const void* f, bool f32, double* rows,
+                 double* out) {
+  if (f32)
+    hipLaunchKernelGGL(k_mass_rows<float>, dim3(g.ny), dim3(MASS_THREADS), 0, s, g, P, (const float*)f, rows);
+  else
+    hipLaunchKernelGGL(k_mass_rows<double>, dim3(g.ny), dim3(MASS_THREADS), 0, s, g, P, (const double*)f, rows);
+  hipLaunchKernelGGL(k_mass_final, dim3(1), dim3(MASS_THREADS), 0, s, rows, g.ny, out);
+}
+
 // move_mesh column shift (move_mesh.f90:29-57 dr=+1, 185-213 dr=-1): out(i) =
 // in(i+dr) where that column exists, else the column is kept.  Populations
 // and fields are shifted as stored (the FP32 deviation of a shifted value is

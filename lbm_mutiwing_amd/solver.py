@@ -42,6 +42,7 @@ _PROTOS = {
     "set_body_rel": (ctypes.c_int, [ctypes.c_void_p, _dp]),
     "set_dynamics": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "set_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
+    "get_mass": (ctypes.c_int, [ctypes.c_void_p, _dp]),
     "last_error": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]),
 }
 
@@ -210,6 +211,13 @@ class Solver:
         fa = np.ascontiguousarray(f, dtype=np.float64).reshape(-1)
         f0a = None if f0 is None else np.ascontiguousarray(f0, dtype=np.float64).reshape(-1)
         self._check(self._fn("set_populations")(self._h, _p(fa), _p(f0a)))
+
+    def mass(self) -> float:
+        """Sum over every node of its nine populations after the last step,
+        reduced in FP64 (lbm_get_mass)."""
+        m = ctypes.c_double()
+        self._check(self._fn("get_mass")(self._h, ctypes.byref(m)))
+        return m.value
 
     def markers(self):
         npts = self.cfg.wing_count * self.cfg.points_per_wing

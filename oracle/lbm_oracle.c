@@ -1279,6 +1279,29 @@ int orc_get_populations(orc_t *o, double *f, double *f0)
     return 0;
 }
 
+/* Total mass: the sum over every node of its nine populations f (no
+ * counterpart in the reference, which has no conservation check; the
+ * lattice-wide FP64 reduction follows the pattern of erranalysis,
+ * solution.f90:77-130).  Compensated (Neumaier) summation, so the value is
+ * the exact sum to a few ulp whatever the lattice size. */
+int orc_get_mass(orc_t *o, double *mass)
+{
+    if (!o || !mass) return LBM_E_INVALID;
+    double s = 0.0, c = 0.0;
+    for (size_t q = 0; q < o->np; q++)
+        for (int k = 0; k < 9; k++) {
+            double x = o->f[(size_t)k * o->np + q];
+            double t = s + x;
+            if (fabs(s) >= fabs(x))
+                c += (s - t) + x;
+            else
+                c += (x - t) + s;
+            s = t;
+        }
+    *mass = s + c;
+    return 0;
+}
+
 int orc_set_populations(orc_t *o, const double *f, const double *f0)
 {
     if (!o) return LBM_E_INVALID;

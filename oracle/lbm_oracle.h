@@ -37,6 +37,8 @@ int orc_get_state(orc_t *o, int32_t *n_next, int32_t *movenum, int32_t *flag2,
 int orc_get_fields(orc_t *o, double *rho, double *u, double *v, double *pre);
 int orc_get_populations(orc_t *o, double *f, double *f0);
 int orc_set_populations(orc_t *o, const double *f, const double *f0);
+/* Sum of f over every node and population, compensated (lbm_get_mass). */
+int orc_get_mass(orc_t *o, double *mass);
 int orc_get_markers(orc_t *o, double *xb1, double *yb1, double *xb2,
                     double *yb2);
 int orc_get_ibm(orc_t *o, int32_t capacity, int32_t *adjoint_ij,

@@ -24,39 +24,55 @@ def build(force: bool = False) -> str:
     return LIB_PATH
 
 
+def configure(lib):
+    """Attach argument/result types to every entry point of an oracle build."""
+    bind(lib, "orc_")
+    lib.orc_set_threads.restype = ctypes.c_int
+    lib.orc_set_threads.argtypes = [ctypes.c_int]
+    lib.orc_delta_kernel.restype = ctypes.c_double
+    lib.orc_delta_kernel.argtypes = [ctypes.c_double]
+    _dp = ctypes.POINTER(ctypes.c_double)
+    lib.orc_mrt_matrices.restype = None
+    lib.orc_mrt_matrices.argtypes = [_dp, _dp]
+    lib.orc_weights.restype = None
+    lib.orc_weights.argtypes = [_dp]
+    lib.orc_collide_node.restype = ctypes.c_int
+    lib.orc_collide_node.argtypes = [ctypes.c_void_p, _dp, ctypes.c_double, ctypes.c_double,
+                                     ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]
+    lib.orc_bc_node.restype = ctypes.c_int
+    lib.orc_bc_node.argtypes = [ctypes.c_void_p] + [ctypes.c_double] * 6 + [_dp, _dp]
+    lib.orc_get_ibm_nodes.restype = ctypes.c_int
+    lib.orc_get_ibm_nodes.argtypes = [ctypes.c_void_p, ctypes.c_int32, _dp, _dp, _dp, _dp, _dp]
+    lib.orc_get_rates.restype = ctypes.c_int
+    lib.orc_get_rates.argtypes = [ctypes.c_void_p, _dp]
+    lib.orc_lodi_node.restype = ctypes.c_int
+    lib.orc_lodi_node.argtypes = [ctypes.c_void_p, ctypes.c_int32, _dp, _dp, _dp, _dp]
+    lib.orc_body_step.restype = None
+    lib.orc_body_step.argtypes = [ctypes.c_void_p] + [ctypes.c_double] * 4 + [_dp]
+    lib.orc_get_ibm_system.restype = ctypes.c_int
+    lib.orc_get_ibm_system.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
+    lib.orc_get_mass.restype = ctypes.c_int
+    lib.orc_get_mass.argtypes = [ctypes.c_void_p, _dp]
+    return lib
+
+
 def load():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        lib = ctypes.CDLL(LIB_PATH)
-        bind(lib, "orc_")
-        lib.orc_set_threads.restype = ctypes.c_int
-        lib.orc_set_threads.argtypes = [ctypes.c_int]
-        lib.orc_delta_kernel.restype = ctypes.c_double
-        lib.orc_delta_kernel.argtypes = [ctypes.c_double]
-        _dp = ctypes.POINTER(ctypes.c_double)
-        lib.orc_mrt_matrices.restype = None
-        lib.orc_mrt_matrices.argtypes = [_dp, _dp]
-        lib.orc_weights.restype = None
-        lib.orc_weights.argtypes = [_dp]
-        lib.orc_collide_node.restype = ctypes.c_int
-        lib.orc_collide_node.argtypes = [ctypes.c_void_p, _dp, ctypes.c_double, ctypes.c_double,
-                                         ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]
-        lib.orc_bc_node.restype = ctypes.c_int
-        lib.orc_bc_node.argtypes = [ctypes.c_void_p] + [ctypes.c_double] * 6 + [_dp, _dp]
-        lib.orc_get_ibm_nodes.restype = ctypes.c_int
-        lib.orc_get_ibm_nodes.argtypes = [ctypes.c_void_p, ctypes.c_int32, _dp, _dp, _dp, _dp, _dp]
-        lib.orc_get_rates.restype = ctypes.c_int
-        lib.orc_get_rates.argtypes = [ctypes.c_void_p, _dp]
-        lib.orc_lodi_node.restype = ctypes.c_int
-        lib.orc_lodi_node.argtypes = [ctypes.c_void_p, ctypes.c_int32, _dp, _dp, _dp, _dp]
-        lib.orc_body_step.restype = None
-        lib.orc_body_step.argtypes = [ctypes.c_void_p] + [ctypes.c_double] * 4 + [_dp]
-        lib.orc_get_ibm_system.restype = ctypes.c_int
-        lib.orc_get_ibm_system.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
-        _lib = lib
+        _lib = configure(ctypes.CDLL(LIB_PATH))
     return _lib
+
+
+def build_variant(source: str, out_path: str) -> str:
+    """Compile an edited copy of the oracle source (a planted mistake, for
+    the tests that show a check fails on it) with the flags of the Makefile;
+    the copy includes lbm_oracle.h from this directory."""
+    cmd = ["gcc", "-O2", "-fPIC", "-fopenmp", "-ffp-contract=off", "-fno-fast-math", "-w",
+           "-iquote", HERE, "-shared", "-o", out_path, source, "-lm"]
+    subprocess.check_call(cmd)
+    return out_path
 
 
 def body_step(dyn, scale: float, uu: float, fx: float, mom: float, st):
@@ -73,8 +89,8 @@ def set_threads(n: int) -> int:
 class OracleSolver(Solver):
     prefix = "orc_"
 
-    def __init__(self, cfg):
-        super().__init__(cfg, lib=load())
+    def __init__(self, cfg, lib=None):
+        super().__init__(cfg, lib=lib if lib is not None else load())
 
     def ibm_system(self):
         import numpy as np
