@@ -143,6 +143,21 @@ struct lbm_handle {
   int nslot = 1;                     // steps per geometry batch (slots per set)
   unsigned batch_count = 0;          // geometry batches issued (selects the slot set)
   int last_slot = -1;                // slot of the last step run (-1: none since reset)
+  // Geometry of the steps after the last call: the rest of that call's last
+  // geometry batch, launched after its own steps with its own error state.
+  // The next call uses it when it starts at n_first, nothing the geometry
+  // depends on has changed (epoch) and its frames are the predicted ones.
+  struct GeoCarry {
+    bool valid = false;
+    int n_first = 0;                 // first step covered
+    int offset = 0;                  // slot of n_first inside its slot set
+    int count = 0;                   // steps covered
+    unsigned epoch = 0;
+    std::vector<Frame> frames;       // frames the geometry was computed with
+  } carry;
+  unsigned geo_epoch = 0;            // bumped by every change of the motion source or of the body state
+  bool carry_on = true;              // LBM_GEO_CARRY=0 in the environment turns the carry off
+  DevStep* d_st_carry = nullptr;     // error state of the carried geometry
   // host state
   int W, P, SPW, nseg, npts, row_len;
   int n = 1;
@@ -248,6 +263,7 @@ void free_all(lbm_handle* h) {
   dfree(h->spare.v);
   dfree(h->err_partial);
   dfree(h->mass_rows);
+  dfree(h->d_st_carry);
   dfree(h->err_out);
   dfree(h->R.left);
   dfree(h->R.right);
@@ -542,6 +558,10 @@ int lbm_create(const lbm_config* cfg, lbm_handle** out) {
     return rc;
   }
   update_body_extra(h);
+  {
+    const char* e = std::getenv("LBM_GEO_CARRY");
+    h->carry_on = !(e && e[0] == '0');
+  }
 
   int rc = 0;
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
@@ -595,6 +615,7 @@ int lbm_create(const lbm_config* cfg, lbm_handle** out) {
   ALLOC(h->R.mom, 6 * (size_t)cfg->ny);
   ALLOC(h->d_stream, 2 * LBM_MAX_MOTION_COLS);
   ALLOC(h->d_st, 1);
+  ALLOC(h->d_st_carry, 1);
   ALLOC(h->d_rowsum, h->row_len);
 #undef ALLOC
   // markers: read_boundary (airfoil.f90:229-241) plus the flow-centre offset
@@ -648,6 +669,7 @@ int lbm_reset(lbm_handle* h) {
   h->flag2 = 1;
   h->last_move_step = -1;
   h->last_slot = -1;
+  h->geo_epoch++;
   update_body_extra(h);
   return 0;
 }
@@ -702,6 +724,7 @@ int lbm_set_motion_table(lbm_handle* h, const double* rows, int32_t nrows, int32
   HIPCHECK(h, hipMemcpy(h->d_table, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice));
   h->G.step_table = h->d_table;
   h->ip.step_count = (int)count;
+  h->geo_epoch++;
   h->motion_end_time = end;
   h->motion_ready = true;
   if (end_time) *end_time = end;
@@ -719,6 +742,7 @@ int lbm_set_stream_segment(lbm_handle* h, const double* start_row, const double*
   HIPCHECK(h, hipStreamSynchronize(h->stream_geo));
   HIPCHECK(h, hipMemcpy(h->d_stream, z.data(), sizeof(double) * z.size(), hipMemcpyHostToDevice));
   h->motion_ready = true;
+  h->geo_epoch++;
   return 0;
 }
 
@@ -801,6 +825,7 @@ int advance_dynamic(lbm_handle* h, int32_t n_last, double* force_rows, double* m
   HIPCHECK(h, hipMemsetAsync(&h->d_st->error, 0, sizeof(int), h->stream));
   HIPCHECK(h, hipStreamSynchronize(h->stream));
   if (!h->h_st) HIPCHECK(h, hipHostMalloc((void**)&h->h_st, sizeof(DevStep), hipHostMallocDefault));
+  h->carry.valid = false;   // these steps reuse the slot sets
   const int ns = h->nslot;
   // the force row of each step writes the next step's frame and body state
   // on the device; the host uploads them at the start and after a move
@@ -880,9 +905,12 @@ int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motio
   if (h->dyn.enable && h->W > 0) return advance_dynamic(h, n_last, force_rows, motion_rows, row_sum, last_row, err);
   const int n0 = h->n;
   const int nsteps = n_last >= n0 ? n_last - n0 + 1 : 0;
+  const int ns = h->nslot;
   int rc = ensure_rows(h, (size_t)std::max(nsteps, 1));
   if (rc) return rc;
-  rc = ensure_frames(h, (size_t)std::max(nsteps, 1));
+  // frames of the call's steps and of the rest of its last geometry batch
+  const int span = h->W > 0 ? std::max(nsteps, 1) + ns : std::max(nsteps, 1);
+  rc = ensure_frames(h, (size_t)span);
   if (rc) return rc;
   if (h->timing)
     while (h->ev.size() < 2 * (size_t)nsteps) {
@@ -891,12 +919,16 @@ int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motio
       h->ev.push_back(e);
     }
   // The frame of every step is known now: moves are fully determined by
-  // body_rel_pos(1), which only this host changes (main.f90:393-411).
+  // body_rel_pos(1), which only this host changes (main.f90:393-411).  The
+  // prediction runs on past the call's last step for the geometry computed
+  // ahead of the next call.
+  bool carried = false;
+  int r = 0;  // slot offset of step n0 inside its slot set
   if (nsteps > 0) {
-    h->frames.resize(nsteps);
+    h->frames.resize(span);
     double brp0 = h->W > 0 ? h->rel[REL_POS][0] : 0.0;
     Frame f = h->fr;
-    for (int k = 0; k < nsteps; k++) {
+    for (int k = 0; k < span; k++) {
       h->frames[k] = f;
       const int dr = brp0 >= 1.0 ? 1 : (brp0 <= -1.0 ? -1 : 0);
       if (dr != 0) {
@@ -911,7 +943,19 @@ int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motio
       }
     }
     HIPCHECK(h, hipStreamSynchronize(h->stream_geo));
-    HIPCHECK(h, hipMemcpy(h->d_frame, h->frames.data(), sizeof(Frame) * nsteps, hipMemcpyHostToDevice));
+    const lbm_handle::GeoCarry& c = h->carry;
+    if (h->W > 0 && c.valid && c.epoch == h->geo_epoch && c.n_first == n0) {
+      const int m = std::min(c.count, nsteps);
+      carried = std::memcmp(c.frames.data(), h->frames.data(), sizeof(Frame) * m) == 0;
+      if (carried) {
+        int cerr = 0;
+        HIPCHECK(h, hipMemcpy(&cerr, &h->d_st_carry->error, sizeof(int), hipMemcpyDeviceToHost));
+        carried = cerr == 0;
+      }
+      if (carried) r = c.offset;
+    }
+    h->carry.valid = false;
+    HIPCHECK(h, hipMemcpy(h->d_frame, h->frames.data(), sizeof(Frame) * span, hipMemcpyHostToDevice));
   }
   if (nsteps > 0) tune_instance(h);
   HIPCHECK(h, hipMemsetAsync(h->d_rowsum, 0, sizeof(double) * h->row_len, h->stream));
@@ -921,21 +965,43 @@ int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motio
   // of the call is computed on the geometry stream one batch ahead of the
   // flow: it is issued when batch b-1 starts, right after its slot set has
   // been released by the force row of the last step of batch b-2, so it runs
-  // while batch b-1 steps.  Only the first batch of a call is waited for.
-  const int ns = h->nslot;
-  const unsigned bc0 = h->batch_count;
-  const int nbatch = h->W > 0 ? (nsteps + ns - 1) / ns : 0;
+  // while batch b-1 steps.  Step k of the call is at kv = k + r of the
+  // batch sequence: r = 0, or the offset of the call's first step in the
+  // batch carried from the previous call, which is then batch 0 and in the
+  // slot set that call used last.  Only the first batch of a call, when not
+  // carried, is waited for.  The last batch is computed in full: its steps
+  // after n_last (with their own error state, so that a step the caller has
+  // not asked for raises nothing) become the carry of the next call.
+  const unsigned bc0 = carried ? h->batch_count - 1u : h->batch_count;
+  const int nbatch = h->W > 0 ? (nsteps + r + ns - 1) / ns : 0;
+  bool carry_pending = false;
   auto set_of = [&](int b) { return (int)((bc0 + (unsigned)b) & 1u); };
   auto issue_geometry = [&](int b) -> int {
     if (b >= nbatch) return 0;
-    const int gs = set_of(b), k0 = b * ns;
+    const int gs = set_of(b), k0 = b * ns - r;   // first step of the batch, call-relative (b >= 1 when r > 0)
+    const int own = std::min(ns, nsteps - k0);
     HIPCHECK(h, hipStreamWaitEvent(h->stream_geo, h->ev_rel[gs], 0));
-    launch_geometry(h->stream_geo, h->ip, h->G, h->d_st, gs * ns, n0 + k0, k0, std::min(ns, nsteps - k0), true);
+    launch_geometry(h->stream_geo, h->ip, h->G, h->d_st, gs * ns, n0 + k0, k0, own, true);
     HIPCHECK(h, hipEventRecord(h->ev_geo[gs], h->stream_geo));
+    const int rest = ns - own;
+    if (b == nbatch - 1 && rest > 0 && h->carry_on) {
+      HIPCHECK(h, hipMemsetAsync(&h->d_st_carry->error, 0, sizeof(int), h->stream_geo));
+      launch_geometry(h->stream_geo, h->ip, h->G, h->d_st_carry, gs * ns + own, n0 + k0 + own, k0 + own, rest,
+                      true);
+      lbm_handle::GeoCarry& c = h->carry;
+      c.n_first = n0 + k0 + own;
+      c.offset = own;
+      c.count = rest;
+      c.epoch = h->geo_epoch;
+      c.frames.assign(h->frames.begin() + (k0 + own), h->frames.begin() + (k0 + own + rest));
+      carry_pending = true;   // valid once the call has completed
+    }
     return 0;
   };
-  rc = issue_geometry(0);
-  if (rc) return rc;
+  if (!carried) {
+    rc = issue_geometry(0);
+    if (rc) return rc;
+  }
   int set = 0, slot = 0, prev_set = 0, prev_slot = 0;
   // step_ev completes when the lattice stream has finished the previous
   // step: the call's start here, then the stop event bound to each lattice
@@ -944,14 +1010,15 @@ int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motio
   HIPCHECK(h, hipEventRecord(h->ev_step, h->stream));
   hipEvent_t step_ev = h->ev_step;
   for (int m = n0; m <= n_last; m++) {
-    const int k = m - n0;
-    if (h->W > 0 && k % ns == 0) {
+    const int k = m - n0, kv = k + r;
+    const bool batch_start = h->W > 0 && (k == 0 || kv % ns == 0);
+    if (batch_start) {
       prev_set = set;
-      set = set_of(k / ns);
+      set = set_of(kv / ns);
       HIPCHECK(h, hipStreamWaitEvent(h->stream_ibm, h->ev_geo[set], 0));
       HIPCHECK(h, hipStreamWaitEvent(h->stream, h->ev_geo[set], 0));  // the lattice kernel reads the claim maps
     }
-    slot = set * ns + k % ns;
+    slot = set * ns + kv % ns;
     const int s = m % 3;
     double brp = h->W > 0 ? h->rel[REL_POS][0] : 0.0;
     const int dr = brp >= 1.0 ? 1 : (brp <= -1.0 ? -1 : 0);
@@ -975,10 +1042,10 @@ int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motio
     // the previous step's force row (its partial sums are kept by step parity)
     if (k > 0) {
       launch_force_row(h->stream_ibm, h->ip, h->G, prev_slot, h->F, h->d_st, m - 1, k - 1, h->row_len);
-      if (h->W > 0 && k % ns == 0) HIPCHECK(h, hipEventRecord(h->ev_rel[prev_set], h->stream_ibm));
+      if (batch_start) HIPCHECK(h, hipEventRecord(h->ev_rel[prev_set], h->stream_ibm));
     }
-    if (h->W > 0 && k % ns == 0) {
-      rc = issue_geometry(k / ns + 1);
+    if (batch_start) {
+      rc = issue_geometry(kv / ns + 1);
       if (rc) return rc;
     }
     launch_ibm_flow(h->stream_ibm, h->ip, h->mrt, h->G, slot, h->F, h->g, src, h->f32);
@@ -1009,7 +1076,9 @@ int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motio
   h->batch_count = bc0 + (unsigned)nbatch;
   if (nsteps > 0 && h->W > 0) h->last_slot = slot;
   h->n = std::max(n0, n_last + 1);
-  return finish_advance(h, nsteps, n_last, h->timing, force_rows, motion_rows, row_sum, last_row, err);
+  rc = finish_advance(h, nsteps, n_last, h->timing, force_rows, motion_rows, row_sum, last_row, err);
+  if (rc == 0 && carry_pending) h->carry.valid = true;
+  return rc;
 }
 
 }  // extern "C"
@@ -1281,6 +1350,7 @@ int lbm_set_step(lbm_handle* h, int32_t n_next, int32_t movenum) {
   h->last_move_step = -1;
   h->last_slot = -1;
   for (int s = 0; s < 3; s++) h->slot_step[s] = -1;
+  h->geo_epoch++;
   update_body_extra(h);
   return 0;
 }
@@ -1288,6 +1358,7 @@ int lbm_set_step(lbm_handle* h, int32_t n_next, int32_t movenum) {
 int lbm_set_body_rel_pos(lbm_handle* h, int32_t wing, double pos) {
   if (!h || wing < 1 || wing > h->W) return fail(h, LBM_E_INVALID, "wing out of range");
   h->rel[REL_POS][wing - 1] = pos;
+  h->geo_epoch++;
   update_body_extra(h);
   return 0;
 }
@@ -1303,6 +1374,7 @@ int lbm_set_body_rel(lbm_handle* h, const double* rel) {
   if (!h || !rel) return fail(h, LBM_E_INVALID, "null argument");
   for (int q = 0; q < REL_FIELDS; q++)
     for (int w = 0; w < h->W; w++) h->rel[q][w] = rel[q * h->W + w];
+  h->geo_epoch++;
   update_body_extra(h);
   return 0;
 }
@@ -1313,6 +1385,7 @@ int lbm_set_dynamics(lbm_handle* h, const lbm_dynamics* d) {
   if (d->enable && !(d->solid_den > 0.0 && (!d->closed_boundary || d->area > 0.0)))
     return fail(h, LBM_E_INVALID, "solid_den and area must be positive");
   h->dyn = *d;
+  h->geo_epoch++;
   IbmParams& p = h->ip;
   const double sc = h->cfg.scale, uu = h->cfg.uu;
   p.dyn_on = d->enable ? 1 : 0;
@@ -1507,6 +1580,7 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
     if (!rc) rc = ensure_frames(h, (size_t)steps);
     if (rc) return bfail(b, rc, "instance %d: %s", e, h->msg);
     BCHECK(b, hipStreamSynchronize(h->stream_geo));
+    h->carry.valid = false;   // the batched steps reuse the slot sets
     h->frames.assign((size_t)steps, h->fr);
     BCHECK(b, hipMemcpy(h->d_frame, h->frames.data(), sizeof(Frame) * steps, hipMemcpyHostToDevice));
     BCHECK(b, hipMemset(h->d_rowsum, 0, sizeof(double) * h->row_len));
