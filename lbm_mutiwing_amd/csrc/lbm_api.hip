@@ -167,6 +167,17 @@ struct lbm_handle {
   double rel[REL_FIELDS][MAXW] = {};  // body_rel_* (restart-file order)
   lbm_dynamics dyn{};                // rigid-body coupling switch and constants
   DevStep* h_st = nullptr;           // pinned host copy of DevStep (coupled steps)
+  // pinned host memory the end of a call copies its outputs into (one
+  // stream synchronisation per call), and the staging of the frames
+  struct HostOut {
+    DevStep st;
+    double err;
+    double last_row[LBM_MAX_FORCE_ROW];
+    double row_sum[LBM_MAX_FORCE_ROW];
+    int carry_err;                   // error state of the carried geometry
+  };
+  HostOut* h_out = nullptr;
+  Frame* h_frames = nullptr;         // frame_cap entries
   Frame fr{};                        // movenum and body_extra_* of the next step
   bool motion_ready = false;
   double motion_end_time = 0.0;
@@ -273,6 +284,10 @@ void free_all(lbm_handle* h) {
   dfree(h->d_frame);
   if (h->h_st) (void)hipHostFree(h->h_st);
   h->h_st = nullptr;
+  if (h->h_out) (void)hipHostFree(h->h_out);
+  h->h_out = nullptr;
+  if (h->h_frames) (void)hipHostFree(h->h_frames);
+  h->h_frames = nullptr;
   dfree(h->d_table);
   dfree(h->d_stream);
   dfree(h->d_st);
@@ -310,13 +325,19 @@ int ensure_rows(lbm_handle* h, size_t steps) {
   return 0;
 }
 
+// Device frames of a call's steps and their pinned staging; the caller has
+// synchronised the geometry stream, the only reader of both.
 int ensure_frames(lbm_handle* h, size_t steps) {
   if (steps <= h->frame_cap) return 0;
   dfree(h->d_frame);
   h->d_frame = nullptr;
+  if (h->h_frames) (void)hipHostFree(h->h_frames);
+  h->h_frames = nullptr;
+  h->frame_cap = 0;
   size_t cap = std::max<size_t>(steps, 1024);
   int rc = dalloc(h, &h->d_frame, cap);
   if (rc) return rc;
+  HIPCHECK(h, hipHostMalloc((void**)&h->h_frames, sizeof(Frame) * cap, hipHostMallocDefault));
   h->frame_cap = cap;
   h->G.frame = h->d_frame;
   return 0;
@@ -616,6 +637,12 @@ int lbm_create(const lbm_config* cfg, lbm_handle** out) {
   ALLOC(h->d_stream, 2 * LBM_MAX_MOTION_COLS);
   ALLOC(h->d_st, 1);
   ALLOC(h->d_st_carry, 1);
+  if (hipHostMalloc((void**)&h->h_out, sizeof(lbm_handle::HostOut), hipHostMallocDefault) != hipSuccess) {
+    free_all(h);
+    delete h;
+    return fail(nullptr, LBM_E_DEVICE, "pinned host allocation failed");
+  }
+  std::memset(h->h_out, 0, sizeof(lbm_handle::HostOut));
   ALLOC(h->d_rowsum, h->row_len);
 #undef ALLOC
   // markers: read_boundary (airfoil.f90:229-241) plus the flow-centre offset
@@ -816,6 +843,7 @@ int advance_dynamic(lbm_handle* h, int32_t n_last, double* force_rows, double* m
                     double* last_row, double* err) {
   const int n0 = h->n;
   const int nsteps = n_last >= n0 ? n_last - n0 + 1 : 0;
+  HIPCHECK(h, hipStreamSynchronize(h->stream_geo));   // geometry carried by a previous call reads the frames
   int rc = ensure_rows(h, (size_t)std::max(nsteps, 1));
   if (rc) return rc;
   rc = ensure_frames(h, 1);
@@ -906,6 +934,9 @@ int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motio
   const int n0 = h->n;
   const int nsteps = n_last >= n0 ? n_last - n0 + 1 : 0;
   const int ns = h->nslot;
+  // the geometry stream may still run the previous call's carried geometry,
+  // which reads the frames
+  HIPCHECK(h, hipStreamSynchronize(h->stream_geo));
   int rc = ensure_rows(h, (size_t)std::max(nsteps, 1));
   if (rc) return rc;
   // frames of the call's steps and of the rest of its last geometry batch
@@ -925,11 +956,11 @@ int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motio
   bool carried = false;
   int r = 0;  // slot offset of step n0 inside its slot set
   if (nsteps > 0) {
-    h->frames.resize(span);
+    Frame* fr = h->h_frames;
     double brp0 = h->W > 0 ? h->rel[REL_POS][0] : 0.0;
     Frame f = h->fr;
     for (int k = 0; k < span; k++) {
-      h->frames[k] = f;
+      fr[k] = f;
       const int dr = brp0 >= 1.0 ? 1 : (brp0 <= -1.0 ? -1 : 0);
       if (dr != 0) {
         brp0 = brp0 - (double)dr;
@@ -942,25 +973,22 @@ int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motio
         }
       }
     }
-    HIPCHECK(h, hipStreamSynchronize(h->stream_geo));
     const lbm_handle::GeoCarry& c = h->carry;
-    if (h->W > 0 && c.valid && c.epoch == h->geo_epoch && c.n_first == n0) {
+    if (h->W > 0 && c.valid && c.epoch == h->geo_epoch && c.n_first == n0 && h->h_out->carry_err == 0) {
       const int m = std::min(c.count, nsteps);
-      carried = std::memcmp(c.frames.data(), h->frames.data(), sizeof(Frame) * m) == 0;
-      if (carried) {
-        int cerr = 0;
-        HIPCHECK(h, hipMemcpy(&cerr, &h->d_st_carry->error, sizeof(int), hipMemcpyDeviceToHost));
-        carried = cerr == 0;
-      }
+      carried = std::memcmp(c.frames.data(), fr, sizeof(Frame) * m) == 0;
       if (carried) r = c.offset;
     }
     h->carry.valid = false;
-    HIPCHECK(h, hipMemcpy(h->d_frame, h->frames.data(), sizeof(Frame) * span, hipMemcpyHostToDevice));
+    HIPCHECK(h, hipMemcpyAsync(h->d_frame, fr, sizeof(Frame) * span, hipMemcpyHostToDevice, h->stream_geo));
   }
   if (nsteps > 0) tune_instance(h);
+  // the call's counters are cleared on the lattice stream before ev_step,
+  // which every other stream of the call waits for before its first kernel
   HIPCHECK(h, hipMemsetAsync(h->d_rowsum, 0, sizeof(double) * h->row_len, h->stream));
   HIPCHECK(h, hipMemsetAsync(&h->d_st->error, 0, sizeof(int), h->stream));
-  HIPCHECK(h, hipStreamSynchronize(h->stream));
+  HIPCHECK(h, hipEventRecord(h->ev_step, h->stream));
+  HIPCHECK(h, hipStreamWaitEvent(h->stream_geo, h->ev_step, 0));
   // Geometry batches of ns steps alternate between two slot sets.  Batch b
   // of the call is computed on the geometry stream one batch ahead of the
   // flow: it is issued when batch b-1 starts, right after its slot set has
@@ -988,12 +1016,14 @@ int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motio
       HIPCHECK(h, hipMemsetAsync(&h->d_st_carry->error, 0, sizeof(int), h->stream_geo));
       launch_geometry(h->stream_geo, h->ip, h->G, h->d_st_carry, gs * ns + own, n0 + k0 + own, k0 + own, rest,
                       true);
+      HIPCHECK(h, hipMemcpyAsync(&h->h_out->carry_err, &h->d_st_carry->error, sizeof(int), hipMemcpyDeviceToHost,
+                                 h->stream_geo));
       lbm_handle::GeoCarry& c = h->carry;
       c.n_first = n0 + k0 + own;
       c.offset = own;
       c.count = rest;
       c.epoch = h->geo_epoch;
-      c.frames.assign(h->frames.begin() + (k0 + own), h->frames.begin() + (k0 + own + rest));
+      c.frames.assign(h->h_frames + (k0 + own), h->h_frames + (k0 + own + rest));
       carry_pending = true;   // valid once the call has completed
     }
     return 0;
@@ -1004,10 +1034,9 @@ int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motio
   }
   int set = 0, slot = 0, prev_set = 0, prev_slot = 0;
   // step_ev completes when the lattice stream has finished the previous
-  // step: the call's start here, then the stop event bound to each lattice
-  // launch (no event packet between two lattice kernels), or a recorded
-  // event after a move_mesh
-  HIPCHECK(h, hipEventRecord(h->ev_step, h->stream));
+  // step: the call's start (recorded above), then the stop event bound to
+  // each lattice launch (no event packet between two lattice kernels), or a
+  // recorded event after a move_mesh
   hipEvent_t step_ev = h->ev_step;
   for (int m = n0; m <= n_last; m++) {
     const int k = m - n0, kv = k + r;
@@ -1022,8 +1051,12 @@ int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motio
     const int s = m % 3;
     double brp = h->W > 0 ? h->rel[REL_POS][0] : 0.0;
     const int dr = brp >= 1.0 ? 1 : (brp <= -1.0 ? -1 : 0);
+    // fields: rho, u, v of the last step (and of a step before a move, which
+    // shifts them); u, v of the two steps before it, for erranalysis
     const bool want_fields = (m >= n_last - 2) || dr != 0;
-    FieldSlot* fs = want_fields ? &h->slot[s] : nullptr;
+    FieldSlot vel = h->slot[s];
+    if (!(m == n_last || dr != 0)) vel.rho = nullptr;
+    FieldSlot* fs = want_fields ? &vel : nullptr;
     const void* src = h->fbuf[h->cur];
     void* dst = h->fbuf[1 - h->cur];
     // Every node of dst has exactly one writer this step: the full-lattice
@@ -1034,9 +1067,16 @@ int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motio
     // concurrently once the previous step has finished; the next step's
     // lattice kernel waits for the other two through ev_ibm.  The previous
     // step's force row follows on the IBM stream; a geometry slot set is
-    // released once the force row of its last step is written.
+    // released once the force row of its last step is written.  The host
+    // issues the lattice kernel first (the waits on step_ev before it see
+    // the previous step's binding of the event) and the geometry of the
+    // next batch last, so that the step's longest kernel is queued first.
     HIPCHECK(h, hipStreamWaitEvent(h->stream_ibm, step_ev, 0));
     HIPCHECK(h, hipStreamWaitEvent(h->stream_bc, step_ev, 0));
+    step_ev = h->timing ? h->ev[2 * k + 1] : h->ev_step;
+    launch_lattice(h->stream, h->g, h->mrt, src, dst, fs, h->f32, h->lat_bx, adj_mask(h->ip, h->G, slot),
+                   h->timing ? h->ev[2 * k] : nullptr, step_ev, h->lat_lds);
+    HIPCHECK(h, hipGetLastError());
     launch_bc_ring(h->stream_bc, h->g, h->mrt, h->R, src, dst, fs, h->f32);
     HIPCHECK(h, hipEventRecord(h->ev_bc, h->stream_bc));
     // the previous step's force row (its partial sums are kept by step parity)
@@ -1044,19 +1084,15 @@ int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motio
       launch_force_row(h->stream_ibm, h->ip, h->G, prev_slot, h->F, h->d_st, m - 1, k - 1, h->row_len);
       if (batch_start) HIPCHECK(h, hipEventRecord(h->ev_rel[prev_set], h->stream_ibm));
     }
-    if (batch_start) {
-      rc = issue_geometry(kv / ns + 1);
-      if (rc) return rc;
-    }
     launch_ibm_flow(h->stream_ibm, h->ip, h->mrt, h->G, slot, h->F, h->g, src, h->f32);
     launch_adj_collide(h->stream_ibm, h->g, h->mrt, h->ip, h->G, slot, h->F, m, src, dst, fs, h->f32);
     HIPCHECK(h, hipStreamWaitEvent(h->stream_ibm, h->ev_bc, 0));
     HIPCHECK(h, hipEventRecord(h->ev_ibm, h->stream_ibm));
-    step_ev = h->timing ? h->ev[2 * k + 1] : h->ev_step;
-    launch_lattice(h->stream, h->g, h->mrt, src, dst, fs, h->f32, h->lat_bx, adj_mask(h->ip, h->G, slot),
-                   h->timing ? h->ev[2 * k] : nullptr, step_ev, h->lat_lds);
-    HIPCHECK(h, hipGetLastError());
     HIPCHECK(h, hipStreamWaitEvent(h->stream, h->ev_ibm, 0));
+    if (batch_start) {
+      rc = issue_geometry(kv / ns + 1);
+      if (rc) return rc;
+    }
     prev_slot = slot;
     if (want_fields) h->slot_step[s] = m;
     h->cur = 1 - h->cur;
@@ -1107,10 +1143,21 @@ int finish_advance(lbm_handle* h, int nsteps, int n_last, bool timed, double* fo
     }
   }
   HIPCHECK(h, hipGetLastError());
+  // the outputs into pinned memory behind the step, then one synchronisation
+  lbm_handle::HostOut* ho = h->h_out;
+  HIPCHECK(h, hipMemcpyAsync(&ho->st, h->d_st, sizeof(DevStep), hipMemcpyDeviceToHost, h->stream));
+  const bool read_err = have_err && h->last_move_step != ml - 1;
+  if (read_err) HIPCHECK(h, hipMemcpyAsync(&ho->err, h->err_out, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  if (nsteps > 0) {
+    if (last_row)
+      HIPCHECK(h, hipMemcpyAsync(ho->last_row, h->d_rows + (size_t)(nsteps - 1) * h->row_len,
+                                 sizeof(double) * h->row_len, hipMemcpyDeviceToHost, h->stream));
+    if (row_sum)
+      HIPCHECK(h, hipMemcpyAsync(ho->row_sum, h->d_rowsum, sizeof(double) * h->row_len, hipMemcpyDeviceToHost,
+                                 h->stream));
+  }
   HIPCHECK(h, hipStreamSynchronize(h->stream));
-  DevStep st_host;
-  HIPCHECK(h, hipMemcpy(&st_host, h->d_st, sizeof(DevStep), hipMemcpyDeviceToHost));
-  if (st_host.error != 0) return step_error(h, st_host);
+  if (ho->st.error != 0) return step_error(h, ho->st);
   if (timed)
     for (int i = 0; i < nsteps; i++) {
       float ms = 0.f;
@@ -1118,22 +1165,17 @@ int finish_advance(lbm_handle* h, int nsteps, int n_last, bool timed, double* fo
       h->lattice_ms += ms;
       h->lattice_launches++;
     }
-  if (have_err && h->last_move_step != ml - 1) HIPCHECK(h, hipMemcpy(&errv, h->err_out, sizeof(double), hipMemcpyDeviceToHost));
+  if (read_err) errv = ho->err;
   if (err) *err = errv;
   if (nsteps > 0) {
     if (force_rows)
       HIPCHECK(h, hipMemcpy(force_rows, h->d_rows, sizeof(double) * h->row_len * nsteps, hipMemcpyDeviceToHost));
     if (motion_rows)
       HIPCHECK(h, hipMemcpy(motion_rows, h->d_mrows, sizeof(double) * (1 + 3 * h->W) * nsteps, hipMemcpyDeviceToHost));
-    if (last_row)
-      HIPCHECK(h, hipMemcpy(last_row, h->d_rows + (size_t)(nsteps - 1) * h->row_len, sizeof(double) * h->row_len,
-                            hipMemcpyDeviceToHost));
-  }
-  if (row_sum) {
-    if (nsteps > 0)
-      HIPCHECK(h, hipMemcpy(row_sum, h->d_rowsum, sizeof(double) * h->row_len, hipMemcpyDeviceToHost));
-    else
-      for (int c = 0; c < h->row_len; c++) row_sum[c] = 0.0;
+    if (last_row) std::memcpy(last_row, ho->last_row, sizeof(double) * h->row_len);
+    if (row_sum) std::memcpy(row_sum, ho->row_sum, sizeof(double) * h->row_len);
+  } else if (row_sum) {
+    for (int c = 0; c < h->row_len; c++) row_sum[c] = 0.0;
   }
   return 0;
 }
@@ -1576,10 +1618,10 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
   size_t lds_wing = 0, lds_solve = 0;
   for (int e = 0; e < count; e++) {
     lbm_handle* h = hs[e];
+    BCHECK(b, hipStreamSynchronize(h->stream_geo));
     int rc = ensure_rows(h, (size_t)steps);
     if (!rc) rc = ensure_frames(h, (size_t)steps);
     if (rc) return bfail(b, rc, "instance %d: %s", e, h->msg);
-    BCHECK(b, hipStreamSynchronize(h->stream_geo));
     h->carry.valid = false;   // the batched steps reuse the slot sets
     h->frames.assign((size_t)steps, h->fr);
     BCHECK(b, hipMemcpy(h->d_frame, h->frames.data(), sizeof(Frame) * steps, hipMemcpyHostToDevice));

@@ -45,7 +45,7 @@ __device__ __forceinline__ void lattice_body(const Grid& g, const MrtParams& P, 
 #pragma unroll
   for (int k = 0; k < 9; k++) dst[(long long)k * g.ps + o] = out[k];
   if (FIELDS) {
-    fs.rho[o] = rho;
+    if (fs.rho) fs.rho[o] = rho;   // null: velocities only (erranalysis)
     fs.u[o] = u;
     fs.v[o] = v;
   }
@@ -124,7 +124,7 @@ __device__ __forceinline__ void lattice_f32_body(const Grid& g, const MrtParams&
   for (int k = 0; k < 9; k++) dst[(long long)k * g.ps + o] = out[k];
   if (FIELDS) {
     const double rd = 1.0 + (double)rp;
-    fs.rho[o] = rd;
+    if (fs.rho) fs.rho[o] = rd;
     fs.u[o] = (double)jx / rd;
     fs.v[o] = (double)jy / rd;
   }
@@ -332,7 +332,7 @@ __device__ __forceinline__ void adj_collide_body(const Grid& g, const MrtParams&
 #pragma unroll
     for (int k = 0; k < 9; k++) Store<S>::st(dst, (long long)k * g.ps + o, out[k], P.w[k]);
     if (FIELDS) {
-      fs.rho[o] = rho;
+      if (fs.rho) fs.rho[o] = rho;
       fs.u[o] = u;
       fs.v[o] = v;
     }
@@ -558,7 +558,7 @@ __device__ __forceinline__ void bc_ring_body(const Grid& g, const MrtParams& P, 
 #pragma unroll
   for (int k = 0; k < 9; k++) Store<S>::st(dst, (long long)k * g.ps + o, out[k], w[k]);
   if (FIELDS) {
-    fs.rho[o] = b.rho;
+    if (fs.rho) fs.rho[o] = b.rho;
     fs.u[o] = b.u;
     fs.v[o] = b.v;
   }
@@ -670,28 +670,43 @@ void launch_moments_all(hipStream_t s, const Grid& g, const MrtParams& P, const 
     hipLaunchKernelGGL(k_moments_all<double>, dim3(2048), dim3(256), 0, s, g, P, (const double*)f, slot);
 }
 
-// erranalysis (solution.f90:77-130): two fixed-order reductions over the
-// interior; one partial per block, summed in block order by k_err_final.
-constexpr int ERR_ROWS_PER_BLOCK = 8;
+// erranalysis (solution.f90:77-130): sqrt(sum (u-u0)^2 + (v-v0)^2) /
+// (sqrt(sum u^2 + v^2) + 1e-30) over the interior.  One workgroup per
+// interior row writes the row's two sums (threads take every
+// ERR_THREADS-th node, then a fixed tree in LDS); k_err_final adds the rows
+// the same way.  The order never changes, so the value is reproducible.
+constexpr int ERR_THREADS = 256;
 
-int err_blocks(const Grid& g) { return (g.ny - 2 + ERR_ROWS_PER_BLOCK - 1) / ERR_ROWS_PER_BLOCK; }
+int err_blocks(const Grid& g) { return g.ny - 2; }
 
-__global__ __launch_bounds__(256) void k_err_partial(Grid g, FieldSlot cur, FieldSlot prev, double* partial) {
-  __shared__ double s1[256], s2[256];
+__global__ __launch_bounds__(ERR_THREADS) void k_err_partial(Grid g, FieldSlot cur, FieldSlot prev, double* partial) {
+  __shared__ double s1[ERR_THREADS], s2[ERR_THREADS];
   double a1 = 0.0, a2 = 0.0;
-  const int j0 = 1 + blockIdx.x * ERR_ROWS_PER_BLOCK;
-  for (int j = j0; j < j0 + ERR_ROWS_PER_BLOCK && j <= g.ny - 2; j++)
-    for (int i = 1 + threadIdx.x; i <= g.nx - 2; i += blockDim.x) {
+  const int j = 1 + blockIdx.x;
+  constexpr int U = 4;   // nodes per thread whose loads are issued together
+  for (int i0 = 1 + threadIdx.x; i0 <= g.nx - 2; i0 += U * ERR_THREADS) {
+    double cu[U], cv[U], pu[U], pv[U];
+#pragma unroll
+    for (int q = 0; q < U; q++) {
+      const int i = i0 + q * ERR_THREADS;
       const long long o = (long long)j * g.nx + i;
-      const double u = cur.u[o], v = cur.v[o];
-      const double du = u - prev.u[o], dv = v - prev.v[o];
-      a1 = a1 + du * du + dv * dv;
-      a2 = a2 + u * u + v * v;
+      const bool in = i <= g.nx - 2;
+      cu[q] = in ? cur.u[o] : 0.0;
+      cv[q] = in ? cur.v[o] : 0.0;
+      pu[q] = in ? prev.u[o] : 0.0;
+      pv[q] = in ? prev.v[o] : 0.0;
     }
+#pragma unroll
+    for (int q = 0; q < U; q++) {
+      const double du = cu[q] - pu[q], dv = cv[q] - pv[q];
+      a1 = a1 + du * du + dv * dv;
+      a2 = a2 + cu[q] * cu[q] + cv[q] * cv[q];
+    }
+  }
   s1[threadIdx.x] = a1;
   s2[threadIdx.x] = a2;
   __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
+  for (int w = ERR_THREADS / 2; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) {
       s1[threadIdx.x] += s1[threadIdx.x + w];
       s2[threadIdx.x] += s2[threadIdx.x + w];
@@ -705,21 +720,31 @@ __global__ __launch_bounds__(256) void k_err_partial(Grid g, FieldSlot cur, Fiel
 }
 
 void launch_err_partial(hipStream_t s, const Grid& g, FieldSlot cur, FieldSlot prev, double* partial, int nblocks) {
-  hipLaunchKernelGGL(k_err_partial, dim3(nblocks), dim3(256), 0, s, g, cur, prev, partial);
+  hipLaunchKernelGGL(k_err_partial, dim3(nblocks), dim3(ERR_THREADS), 0, s, g, cur, prev, partial);
 }
 
-__global__ void k_err_final(const double* partial, int nblocks, double* out) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__global__ __launch_bounds__(ERR_THREADS) void k_err_final(const double* partial, int nblocks, double* out) {
+  __shared__ double s1[ERR_THREADS], s2[ERR_THREADS];
   double t1 = 0.0, t2 = 0.0;
-  for (int b = 0; b < nblocks; b++) {
+  for (int b = threadIdx.x; b < nblocks; b += ERR_THREADS) {
     t1 += partial[2 * b];
     t2 += partial[2 * b + 1];
   }
-  out[0] = sqrt(t1) / (sqrt(t2) + 1e-30);
+  s1[threadIdx.x] = t1;
+  s2[threadIdx.x] = t2;
+  __syncthreads();
+  for (int w = ERR_THREADS / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      s1[threadIdx.x] += s1[threadIdx.x + w];
+      s2[threadIdx.x] += s2[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = sqrt(s1[0]) / (sqrt(s2[0]) + 1e-30);
 }
 
 void launch_err_final(hipStream_t s, const double* partial, int nblocks, double* out) {
-  hipLaunchKernelGGL(k_err_final, dim3(1), dim3(64), 0, s, partial, nblocks, out);
+  hipLaunchKernelGGL(k_err_final, dim3(1), dim3(ERR_THREADS), 0, s, partial, nblocks, out);
 }
 
 // Lattice mass (the FP64 mass-conservation check; the reference has none, the

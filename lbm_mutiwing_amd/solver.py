@@ -177,9 +177,13 @@ class Solver:
         keys = ("n_next", "movenum", "flag2", "row_len", "adjoint_n")
         return dict(zip(keys, (v.value for v in vals)))
 
+    def _n_next(self) -> int:
+        n = ctypes.c_int32()
+        self._check(self._fn("get_state")(self._h, ctypes.byref(n), None, None, None, None))
+        return n.value
+
     def advance(self, n_last: int, want_rows: bool = False, want_motion: bool = False) -> AdvanceResult:
-        st = self.state()
-        nsteps = max(0, n_last - st["n_next"] + 1)
+        nsteps = max(0, n_last - self._n_next() + 1)
         rows = np.zeros((max(nsteps, 1), self.row_len)) if want_rows else None
         mrows = np.zeros((max(nsteps, 1), 1 + 3 * self.cfg.wing_count)) if want_motion else None
         rsum = np.zeros(self.row_len)
