@@ -102,6 +102,12 @@ module lbm_hip
             real(c_double), intent(out) :: rho(*), u(*), v(*), pre(*)
         end function
 
+        integer(c_int) function lbm_get_mass(h, mass) bind(C, name='lbm_get_mass')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: h
+            real(c_double), intent(out) :: mass
+        end function
+
         integer(c_int) function lbm_get_populations(h, f, f0) bind(C, name='lbm_get_populations')
             import :: c_int, c_ptr, c_double
             type(c_ptr), value :: h
