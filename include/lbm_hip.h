@@ -158,6 +158,23 @@ int lbm_set_populations(lbm_handle *h, const double *f, const double *f0);
  * population buffer once (604 MB at 4096 x 2048 in FP64). */
 int lbm_get_mass(lbm_handle *h, double *mass);
 
+/* Allocation-free variant of lbm_get_mass, with no counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops).  f points to caller-owned device
+ * populations in the layout of this library (9 planes of nx*ny values,
+ * population k of node (i, j) at i + nx*j + nx*ny*k): FP64 values for
+ * LBM_PRECISION_FP64, FP32 deviations f - w_k for
+ * LBM_PRECISION_FP32_SHIFTED.  The kernels are those of lbm_get_mass (same
+ * value, bit for bit), enqueued on `stream` (a hipStream_t; NULL is the null
+ * stream) with no synchronisation: *mass is a device address, written when
+ * the stream reaches the call.  The per-row sums go to a fixed device array
+ * of the library (one per GPU, 32768 doubles, held until the process ends),
+ * so ny may be at most 32768 (LBM_E_RANGE otherwise), nothing is allocated
+ * or freed, and calls that overlap on one GPU race.  Errors are reported by
+ * lbm_last_error(NULL, ...). */
+int lbm_get_mass_noalloc(const void *f, int32_t nx, int32_t ny, int32_t precision,
+                         double *mass, void *stream);
+
 /* Markers after the last step: xb1/yb1 (wing_count*points_per_wing) and
  * segment midpoints xb2/yb2 (airfoil.f90:261-270); any may be NULL. */
 int lbm_get_markers(lbm_handle *h, double *xb1, double *yb1, double *xb2,

@@ -1231,6 +1231,38 @@ int lbm_get_mass(lbm_handle* h, double* mass) {
   return 0;
 }
 
+int lbm_get_mass_noalloc(const void* f, int32_t nx, int32_t ny, int32_t precision, double* mass, void* stream) {
+  if (!f || !mass) return LBM_E_INVALID;
+  if (nx < 1 || ny < 1) return LBM_E_INVALID;
+  if (precision != LBM_PRECISION_FP64 && precision != LBM_PRECISION_FP32_SHIFTED) return LBM_E_INVALID;
+  if (ny > mass_scratch_rows()) {
+    snprintf(g_create_error, sizeof(g_create_error), "lbm_get_mass_noalloc: ny = %d exceeds the scratch of %d rows",
+             ny, mass_scratch_rows());
+    return LBM_E_RANGE;
+  }
+  double* rows = nullptr;
+  hipError_t e = mass_scratch(&rows);
+  if (e != hipSuccess) {
+    snprintf(g_create_error, sizeof(g_create_error), "lbm_get_mass_noalloc: %s", hipGetErrorString(e));
+    return LBM_E_DEVICE;
+  }
+  Grid g;
+  g.nx = nx;
+  g.ny = ny;
+  g.ps = (long long)nx * ny;
+  MrtParams P{};
+  P.w[0] = (double)(4.0f / 9.0f);  // the weights of lbm_create (D2Q9model.f90:24-32)
+  for (int k = 1; k < 5; k++) P.w[k] = (double)(1.0f / 9.0f);
+  for (int k = 5; k < 9; k++) P.w[k] = (double)(1.0f / 36.0f);
+  launch_mass((hipStream_t)stream, g, P, f, precision == LBM_PRECISION_FP32_SHIFTED, rows, mass);
+  e = hipGetLastError();
+  if (e != hipSuccess) {
+    snprintf(g_create_error, sizeof(g_create_error), "lbm_get_mass_noalloc: %s", hipGetErrorString(e));
+    return LBM_E_DEVICE;
+  }
+  return 0;
+}
+
 int lbm_get_populations(lbm_handle* h, double* f, double* f0) {
   if (!h) return LBM_E_INVALID;
   const size_t np = (size_t)h->g.nx * h->g.ny;

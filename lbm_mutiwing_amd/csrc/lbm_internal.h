@@ -254,6 +254,10 @@ int err_blocks(const Grid& g);
 // the per-row sums, out[0] the total; nothing else is written
 void launch_mass(hipStream_t s, const Grid& g, const MrtParams& P, const void* f, bool f32, double* rows,
                  double* out);
+// device address of this GPU's file-scope scratch array of the allocation-free
+// mass (room for mass_scratch_rows() row sums), looked up once per device
+hipError_t mass_scratch(double** out);
+int mass_scratch_rows();
 int measure_pull_copy(int nx, int ny, bool f32, int launches, double* avg_us);
 void launch_shift_x(hipStream_t s, const Grid& g, int dr, const void* f, void* fshift, int planes, bool f32);
 void launch_copy_ring_column(hipStream_t s, const Grid& g, const MrtParams& P, const void* f, int col, double* ring,

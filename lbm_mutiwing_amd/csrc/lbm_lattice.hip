@@ -14,6 +14,7 @@
 // collision of the FP32 storage computes in FP32; the boundary ring, the
 // adjoint nodes and all reductions compute in FP64 on loaded f = w_k + d.
 #include <algorithm>
+#include <mutex>
 
 #include <hip/hip_ext.h>
 
@@ -798,6 +799,38 @@ void launch_mass(hipStream_t s, const Grid& g, const MrtParams& P, const void* f
   else
     hipLaunchKernelGGL(k_mass_rows<double>, dim3(g.ny), dim3(MASS_THREADS), 0, s, g, P, (const double*)f, rows);
   hipLaunchKernelGGL(k_mass_final, dim3(1), dim3(MASS_THREADS), 0, s, rows, g.ny, out);
+}
+
+// The lattice mass of caller-owned populations with no allocation: the row
+// sums live in a file-scope device array, one copy per GPU, whose device
+// address is looked up once per device (a host reference to the array's name
+// is not a device address).  Every call on a GPU shares that GPU's array, so
+// calls that overlap on one GPU race.
+constexpr int MASS_SCRATCH_ROWS = 32768;
+static __device__ __align__(256) double scratch[MASS_SCRATCH_ROWS];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxDevices] = {};
+}  // namespace
+
+int mass_scratch_rows() { return MASS_SCRATCH_ROWS; }
+
+hipError_t mass_scratch(double** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch[dev] = (double*)p;
+  }
+  *out = g_scratch[dev];
+  return hipSuccess;
 }
 
 // move_mesh column shift (move_mesh.f90:29-57 dr=+1, 185-213 dr=-1): out(i) =

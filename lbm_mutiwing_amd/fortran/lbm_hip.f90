@@ -108,6 +108,14 @@ module lbm_hip
             real(c_double), intent(out) :: mass
         end function
 
+        ! allocation-free variant: f and mass are device addresses, stream a hipStream_t
+        integer(c_int) function lbm_get_mass_noalloc(f, nx, ny, precision, mass, stream) &
+                bind(C, name='lbm_get_mass_noalloc')
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: f, mass, stream
+            integer(c_int32_t), value :: nx, ny, precision
+        end function
+
         integer(c_int) function lbm_get_populations(h, f, f0) bind(C, name='lbm_get_populations')
             import :: c_int, c_ptr, c_double
             type(c_ptr), value :: h
