@@ -14,7 +14,11 @@ life of the episode.  The scheduler:
   depends only on its own segment: a reply is bitwise the same alone and in
   any mix;
 * answers a job with the segment-averaged FORCE reply as soon as it reaches
-  its target step, recording the latency from submission to reply.
+  its target step, recording the latency from submission to reply;
+* advances the episodes that couple wing 1 to the flow (one step at a time,
+  lbm_batch_advance rejects them) on a worker thread while the batched
+  launches run, so their steps overlap on the device; each instance is still
+  used by one thread at a time.
 
 The transport is not part of this module: ``submit`` takes the command text
 and ``poll`` returns reply texts, so a ZeroMQ REP loop, a test or a benchmark
@@ -25,12 +29,18 @@ from __future__ import annotations
 import math
 import time
 from collections import deque
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass
 from typing import Callable, Dict, List, Optional, Tuple
 
 from .config import PRECISION_FP32_SHIFTED, SolverConfig
 from .protocol import CMD_RESET, CMD_STOP, Session, parse_command
 from .solver import Batch, LbmError, Solver
+
+
+# lbm_batch_advance captures advances of at most this many steps as HIP
+# graphs (include/lbm_hip.h, "Launch graphs")
+GRAPH_MAX_STEPS = 64
 
 
 def device_bytes(cfg: SolverConfig) -> int:
@@ -80,6 +90,9 @@ class EpisodeServer:
         self.waiting: deque = deque()          # (client, message, submitted) first RESETs
         self.jobs: List = []
         self.replies: List[Reply] = []
+        # per client, when it stopped: its episode's mass (lbm_get_mass) and movenum
+        self.final: Dict[object, Tuple[float, int]] = {}
+        self._pool: Optional[ThreadPoolExecutor] = None
 
     # ---------------------------------------------------------------- clients
     def connect(self, client, cfg: SolverConfig, dynamics=None) -> None:
@@ -124,6 +137,10 @@ class EpisodeServer:
         sess = self.sessions.pop(client, None)
         if sess is not None:
             self.used -= device_bytes(sess.cfg)
+            try:
+                self.final[client] = (sess.solver.mass(), sess.solver.state()["movenum"])
+            except LbmError:
+                pass
             sess.solver.close()
 
     # ------------------------------------------------------------- scheduling
@@ -162,14 +179,29 @@ class EpisodeServer:
                 else:
                     groups.setdefault((job.session.cfg.precision, c), []).append(job)
             failed = []
-            for j, c in alone:
-                s = j.session.solver
-                try:
-                    r = s.advance(s.state()["n_next"] + c - 1)
-                except LbmError as e:
-                    failed.append((j, str(e)))
-                else:
-                    j.add(r)
+
+            def step_alone():
+                out = []
+                for j, c in alone:
+                    s = j.session.solver
+                    try:
+                        r = s.advance(s.state()["n_next"] + c - 1)
+                    except LbmError as e:
+                        out.append((j, str(e)))
+                    else:
+                        j.add(r)
+                return out
+
+            pending = None
+            # a batched advance of at most GRAPH_MAX_STEPS steps may be
+            # captured as a HIP graph, and a capture fails when another
+            # thread synchronises meanwhile: overlap only longer chunks
+            if alone and groups and min(c for _, c in groups) > GRAPH_MAX_STEPS:
+                if self._pool is None:
+                    self._pool = ThreadPoolExecutor(max_workers=1)
+                pending = self._pool.submit(step_alone)
+            else:
+                failed += step_alone()
             for (_, c), jobs in groups.items():
                 results = self.batch.advance([j.session.solver for j in jobs], c, partial=True)
                 for j, r in zip(jobs, results):
@@ -177,6 +209,8 @@ class EpisodeServer:
                         failed.append((j, str(r)))
                     else:
                         j.add(r)
+            if pending is not None:
+                failed += pending.result()
             for j, msg in failed:
                 self.jobs.remove(j)
                 self._reply(j.tag, "ERROR " + msg, j.arrival)
@@ -191,6 +225,9 @@ class EpisodeServer:
         return not self.jobs and not self.waiting
 
     def close(self) -> None:
+        if self._pool is not None:
+            self._pool.shutdown(wait=True)
+            self._pool = None
         for c in list(self.sessions):
             self._release(c)
         self.batch.close()
