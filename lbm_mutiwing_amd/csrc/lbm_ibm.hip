@@ -376,10 +376,12 @@ __device__ __forceinline__ double s_entry(const int* box, const double* phx, con
 
 // In-place S = L D L^T of the lower band (band[r*Dd + d] = S(r, r-d), row
 // stride Dd > bw) by one wave: afterwards band[k*Dd] = D_k,
-// band[(k+i)*Dd + i] = L(k+i, k), invd[k] = 1/D_k.  Step k updates S(k+i, k+j) -= (S(k+i, k) / D_k) S(k+j, k)
-// for 1 <= j <= i <= bw: the bw(bw+1)/2 updates write distinct entries and
+// band[(k+i)*Dd + i] = L(k+i, k), invd[k] = 1/D_k.  Step k updates
+// S(k+i, k+j) -= (S(k+i, k) / D_k) S(k+j, k) for 1 <= j <= i <= bw with the
+// column k not yet scaled: the bw(bw+1)/2 updates write distinct entries and
 // read only column k, so each lane takes up to five (i, j) pairs of its own
-// and a step costs one round of LDS updates.
+// and a step costs one round of LDS updates and one wave barrier.  Column k
+// is scaled by 1/D_k after the last step (it is not read again).
 constexpr int LDLT_PAIRS_PER_LANE = (BMAX * (BMAX + 1) / 2 + 63) / 64;
 
 __device__ void band_ldlt(double* band, double* invd, int n, int bw, int Dd, int lane, DevStep* st, int info) {
@@ -406,59 +408,145 @@ __device__ void band_ldlt(double* band, double* invd, int n, int bw, int Dd, int
       const int i = pi[u], j = pj[u];
       if (i > 0 && k + i < n) band[(k + i) * Dd + (i - j)] -= band[(k + i) * Dd + i] * rd * band[(k + j) * Dd + j];
     }
-    wave_sync();
-    if (lane >= 1 && lane <= bw && k + lane < n) band[(k + lane) * Dd + lane] = band[(k + lane) * Dd + lane] * rd;
     if (lane == 0) invd[k] = rd;
     wave_sync();
   }
+  // L(k + i, k) = S(k + i, k) / D_k
+  for (int e = lane; e < n * bw; e += 64) {
+    const int k = e / bw, i = 1 + e - (e / bw) * bw;
+    if (k + i < n) band[(k + i) * Dd + i] = band[(k + i) * Dd + i] * invd[k];
+  }
+  wave_sync();
 }
 
-// Column col of S^-1 = L^-T D^-1 L^-1 e_col: forward substitution with a
-// sliding window of the last BW values, t = D^-1 z written to the output
-// column, then backward substitution over the stored t (read ahead in chunks).
-// The band is stored with the compile-time row stride BW + 1 (BW >= bw), its
-// entries beyond bw and BW rows past the last one zero: every load of a row
-// has a constant offset and no guard, so the loads of a row issue together
-// (a term with a zero L entry subtracts an exact zero).
+// Column col of S^-1 = L^-T D^-1 L^-1 e_col (col >= n: a lane with no column,
+// which runs along and stores nothing): forward substitution, t = D^-1 z
+// written to the output column, then backward substitution over the stored t.
+// Every lane of a wave walks the same rows at the same time (a lane whose
+// column starts later carries exact zeros until it does), so the band loads
+// are broadcasts and the lanes' stores of a row are contiguous.  The rows are
+// taken in blocks of BW with the last BW values in a register ring indexed at
+// compile time (no window shifts), and each row's BW products are summed by
+// fused multiply-adds into three partial sums.  The band is stored with the
+// compile-time row stride BW + 1 (BW >= bw), its entries beyond bw and BW rows
+// past the last one zero, so the loads of a row have constant offsets and no
+// guards (a term with a zero L entry adds an exact zero).
 template <int BW>
-__device__ void inverse_column(const double* band, const double* invd, int n, int col, double* S) {
+__device__ __forceinline__ double band_dot(const double (&l)[BW + 1], const double (&w)[BW], int r, double a0) {
+  double a1 = 0.0, a2 = 0.0;
+#pragma unroll
+  for (int d = BW; d >= 1; d--) {   // the newest value (d = 1) last: it ends the row's dependency chain
+    const double x = w[(r - d + BW) % BW];
+    if (d % 3 == 1)
+      a0 = __builtin_fma(-l[d], x, a0);
+    else if (d % 3 == 2)
+      a1 = __builtin_fma(-l[d], x, a1);
+    else
+      a2 = __builtin_fma(-l[d], x, a2);
+  }
+  return (a1 + a2) + a0;
+}
+
+// the BW entries L(m, m-d) (step 1) or L(m+d, m) (step Dd+1) of a band row or
+// column, d = 1..BW, into registers
+// (l[0] = *extra when extra is given: 1/D of the row, loaded in the same
+// group because LDS loads complete in order)
+template <int BW>
+__device__ __forceinline__ void band_load(const double* p, int step, double (&l)[BW + 1],
+                                          const double* extra = nullptr) {
+  if (extra) l[0] = *extra;
+#pragma unroll
+  for (int d = 1; d <= BW; d++) l[d] = p[d * step];
+  // keep these loads ahead of the row computed after them (the scheduler
+  // would otherwise sink them to their use in the next row)
+  __builtin_amdgcn_sched_barrier(0);
+}
+
+// One forward row m (ring slot r): z_m = e_col(m) - sum_d L(m, m-d) z_{m-d}
+// from the row's entries in cur (cur[0] = 1/D_m), while the next row's
+// entries load into nxt.
+template <int BW>
+__device__ __forceinline__ void forward_row(const double* band, const double* invd, int n, int col, int m, int r,
+                                            double (&w)[BW], const double (&cur)[BW + 1], double (&nxt)[BW + 1],
+                                            double* S) {
+  band_load<BW>(band + (m + 1) * (BW + 1), 1, nxt, invd + m + 1);
+  const double z = band_dot<BW>(cur, w, r, m == col ? 1.0 : 0.0);
+  w[r] = z;
+  if (m >= col) S[(size_t)m * n + col] = z * cur[0];
+}
+
+// One backward row m (ring slot r): x_m = t - sum_d L(m+d, m) x_{m+d}, while
+// the entries of row m - 1 load into nxt.
+template <int BW>
+__device__ __forceinline__ void backward_row(const double* band, int n, int col, int m, int r, double t,
+                                             double (&w)[BW], const double (&cur)[BW + 1], double (&nxt)[BW + 1],
+                                             double* S) {
+  if (m >= 1) band_load<BW>(band + (m - 1) * (BW + 1), BW + 2, nxt);
+  const double x = band_dot<BW>(cur, w, r, t);
+  w[r] = x;
+  if (col < n) S[(size_t)m * n + col] = x;
+}
+
+template <int BW>
+__device__ void inverse_column(const double* band, const double* invd, int n, int col, int m0, double* S) {
+  static_assert(BW % 2 == 0, "the row entries alternate between two register sets");
   constexpr int Dd = BW + 1;
-  double win[BW];
+  double w[BW];
 #pragma unroll
-  for (int d = 0; d < BW; d++) win[d] = 0.0;
-  // rows above col give z = 0 (L is unit lower triangular): start at col
-  for (int m = col; m < n; m++) {
-    double z = m == col ? 1.0 : 0.0;
-    const double* Lr = band + m * Dd;
+  for (int d = 0; d < BW; d++) w[d] = 0.0;
+  // forward, row m in w[(m - m0) % BW]; rows above the wave's first column
+  // are zero in every lane.  Row entries (and 1/D) are loaded one row ahead,
+  // alternately into la and lb (rows up to n + BW - 1 and n + 1 entries of
+  // 1/D exist); invd has n + 1 entries, the last one unused.
+  double la[BW + 1], lb[BW + 1];
+  band_load<BW>(band + m0 * Dd, 1, la, invd + m0);
+  int mb = m0;
+  for (; mb + BW <= n; mb += BW) {
 #pragma unroll
-    for (int d = 1; d <= BW; d++) z -= Lr[d] * win[d - 1];
-#pragma unroll
-    for (int d = BW - 1; d >= 1; d--) win[d] = win[d - 1];
-    win[0] = z;
-    if (m >= col) S[(size_t)m * n + col] = z * invd[m];
+    for (int r = 0; r < BW; r += 2) {
+      forward_row<BW>(band, invd, n, col, mb + r, r, w, la, lb, S);
+      forward_row<BW>(band, invd, n, col, mb + r + 1, r + 1, w, lb, la, S);
+    }
   }
 #pragma unroll
-  for (int d = 0; d < BW; d++) win[d] = 0.0;
-  constexpr int CH = 16;
-  for (int m0 = n - 1; m0 >= 0; m0 -= CH) {
-    double tv[CH];
+  for (int r = 0; r < BW; r += 2) {
+    if (mb + r < n) forward_row<BW>(band, invd, n, col, mb + r, r, w, la, lb, S);
+    if (mb + r + 1 < n) forward_row<BW>(band, invd, n, col, mb + r + 1, r + 1, w, lb, la, S);
+  }
+  // backward from m = n-1 down, row m = n-1-q in w[q % BW];
+  // L(m+d, m) = band[(m+d)*Dd + d], step Dd+1 in d
 #pragma unroll
-    for (int u = 0; u < CH; u++) {
-      const int m = m0 - u;
-      tv[u] = (m >= 0 && m >= col) ? S[(size_t)m * n + col] : 0.0;
+  for (int d = 0; d < BW; d++) w[d] = 0.0;
+  double tv[BW];
+#pragma unroll
+  for (int r = 0; r < BW; r++) {
+    const int m = n - 1 - r;
+    tv[r] = (m >= 0 && m >= col) ? S[(size_t)m * n + col] : 0.0;
+  }
+  band_load<BW>(band + (n - 1) * Dd, Dd + 1, la);
+  // the next block's t is loaded while a block computes (up to BW = 16; the
+  // widest band has no registers to spare for it)
+  constexpr bool AHEAD = BW <= 16;
+  for (int qb = 0; qb < n; qb += BW) {
+    double tn[BW];
+#pragma unroll
+    for (int r = 0; r < BW; r++) {
+      const int m = n - 1 - ((AHEAD ? qb + BW : qb) + r);
+      tn[r] = (m >= 0 && m >= col) ? S[(size_t)m * n + col] : 0.0;
+    }
+    if (!AHEAD) {
+#pragma unroll
+      for (int r = 0; r < BW; r++) tv[r] = tn[r];
     }
 #pragma unroll
-    for (int u = 0; u < CH; u++) {
-      const int m = m0 - u;
-      if (m < 0) break;
-      double x = tv[u];
-      const double* Lc = band + m * Dd;   // L(m + d, m) = Lc[d * (Dd + 1)]
+    for (int r = 0; r < BW; r += 2) {
+      const int m = n - 1 - (qb + r);
+      if (m >= 0) backward_row<BW>(band, n, col, m, r, tv[r], w, la, lb, S);
+      if (m - 1 >= 0) backward_row<BW>(band, n, col, m - 1, r + 1, tv[r + 1], w, lb, la, S);
+    }
+    if (AHEAD) {
 #pragma unroll
-      for (int d = 1; d <= BW; d++) x -= Lc[d * (Dd + 1)] * win[d - 1];
-#pragma unroll
-      for (int d = BW - 1; d >= 1; d--) win[d] = win[d - 1];
-      win[0] = x;
-      S[(size_t)m * n + col] = x;
+      for (int r = 0; r < BW; r++) tv[r] = tn[r];
     }
   }
 }
@@ -515,15 +603,17 @@ __device__ __forceinline__ void geo_solve_body(const IbmParams& p, const GeoBuff
   double* S = G.sinv + ((size_t)slot * p.W + w) * (size_t)n * n;
   for (int col0 = 0; col0 < n; col0 += GEO_THREADS) {
     const int col = col0 + t;
-    if (col < n) {
+    const int m0 = col0 + (t & ~63);   // the wave's first column
+    if (m0 < n) {   // lanes past the last column run along (exact zeros) and store nothing
+      const int c = col < n ? col : n + 64;
       if (BWc == 8)
-        inverse_column<8>(band, invd, n, col, S);
+        inverse_column<8>(band, invd, n, c, m0, S);
       else if (BWc == 12)
-        inverse_column<12>(band, invd, n, col, S);
+        inverse_column<12>(band, invd, n, c, m0, S);
       else if (BWc == 16)
-        inverse_column<16>(band, invd, n, col, S);
+        inverse_column<16>(band, invd, n, c, m0, S);
       else
-        inverse_column<BMAX>(band, invd, n, col, S);
+        inverse_column<BMAX>(band, invd, n, c, m0, S);
     }
   }
 }
@@ -532,9 +622,10 @@ __global__ __launch_bounds__(GEO_THREADS) void k_geo_solve(IbmParams p, GeoBuffe
   geo_solve_body(p, G, st, slot0, blockIdx.x, blockIdx.y);
 }
 
-// band of (SPW + BMAX) rows of stride BMAX + 1 at most, and 1/D
+// band of (SPW + BMAX) rows of stride BMAX + 1 at most, and 1/D (SPW + 1
+// entries: the inverse loads one row ahead)
 size_t geo_solve_lds_bytes(const IbmParams& p) {
-  return ((size_t)(p.SPW + BMAX) * (BMAX + 1) + p.SPW) * sizeof(double);
+  return ((size_t)(p.SPW + BMAX) * (BMAX + 1) + p.SPW + 1) * sizeof(double);
 }
 
 // ------------------------------------------------------------ coupled wings
