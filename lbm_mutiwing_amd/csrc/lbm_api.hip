@@ -445,7 +445,7 @@ int alloc_ibm(lbm_handle* h, const std::vector<double>& hx, const std::vector<do
   p.map_cap = h->W > 0 ? side * side : 1;
   const size_t per_slot = sizeof(GeoStep) + 2 * npts * 8 + 4 * nseg * 8 + 4 * nseg * 4 + 2 * BOXW * nseg * 8 +
                           W * (spw + 1) * 4 + 3 * cap * 4 + W * spw * spw * 8 + W * 2 * (size_t)p.map_cap * 4 +
-                          (size_t)CPL_KT * CPL_KT * 8;
+                          (size_t)CPL_KT * CPL_KT * 8 + W * band_cap(p) * 8;
   h->nslot = h->W > 0 ? (int)std::max<size_t>(1, std::min<size_t>(16, (256u << 20) / per_slot)) : 1;
   const size_t ns = 2 * (size_t)h->nslot;
   int rc = 0;
@@ -460,6 +460,7 @@ int alloc_ibm(lbm_handle* h, const std::vector<double>& hx, const std::vector<do
       (rc = ialloc(h, &G.phy, ns * nseg * BOXW)) || (rc = ialloc(h, &G.seg_off, ns * W * (spw + 1))) ||
       (rc = ialloc(h, &G.adj_ij, ns * cap * 2)) || (rc = ialloc(h, &G.adj_seg, ns * cap)) ||
       (rc = ialloc(h, &G.sinv, h->W > 0 ? ns * W * spw * spw : 1)) ||
+      (rc = ialloc(h, &G.bandg, h->W > 0 ? ns * W * band_cap(p) : 1)) ||
       (rc = ialloc(h, &G.map, h->W > 0 ? ns * W * 2 * (size_t)p.map_cap : 1)) ||
       (rc = ialloc(h, &G.cpl, h->W > 1 ? ns * CPL_KT * CPL_KT : 1)))
     return rc;
@@ -1648,6 +1649,7 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
   std::vector<int2> items;
   BatchItems it{};
   size_t lds_wing = 0, lds_solve = 0;
+  int inv_chunks = 1;
   for (int e = 0; e < count; e++) {
     lbm_handle* h = hs[e];
     BCHECK(b, hipStreamSynchronize(h->stream_geo));
@@ -1680,6 +1682,7 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
     if (h->W > 0) {
       lds_wing = std::max(lds_wing, geo_wing_lds_bytes(h->SPW));
       lds_solve = std::max(lds_solve, geo_solve_lds_bytes(h->ip));
+      inv_chunks = std::max(inv_chunks, inverse_chunks(h->SPW));
     }
   }
   auto add = [&](int e, int nblk) {
@@ -1806,7 +1809,7 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
     BCHECK(b, hipStreamWaitEvent(b->stream_geo, bb < seg_bb + 2 ? b->ev_start : b->ev_rel[es], 0));
     const int q = k0 + ns >= steps ? 6 : 5;  // the last (shorter) batch has its own list
     launch_batch_geometry(b->stream_geo, b->d_desc, b->d_kbase, di + off[q], (int)(off[q + 1] - off[q]), k0 - kb0,
-                          lds_wing, lds_solve);
+                          lds_wing, lds_solve, inv_chunks);
     BCHECK(b, hipEventRecord(b->ev_geo[es], b->stream_geo));
     return 0;
   };

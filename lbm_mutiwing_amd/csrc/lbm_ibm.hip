@@ -419,18 +419,8 @@ __device__ void band_ldlt(double* band, double* invd, int n, int bw, int Dd, int
   wave_sync();
 }
 
-// Column col of S^-1 = L^-T D^-1 L^-1 e_col (col >= n: a lane with no column,
-// which runs along and stores nothing): forward substitution, t = D^-1 z
-// written to the output column, then backward substitution over the stored t.
-// Every lane of a wave walks the same rows at the same time (a lane whose
-// column starts later carries exact zeros until it does), so the band loads
-// are broadcasts and the lanes' stores of a row are contiguous.  The rows are
-// taken in blocks of BW with the last BW values in a register ring indexed at
-// compile time (no window shifts), and each row's BW products are summed by
-// fused multiply-adds into three partial sums.  The band is stored with the
-// compile-time row stride BW + 1 (BW >= bw), its entries beyond bw and BW rows
-// past the last one zero, so the loads of a row have constant offsets and no
-// guards (a term with a zero L entry adds an exact zero).
+// One row of a band substitution: a0 minus the BW products of the row's
+// entries l[1..BW] with the ring of the last BW values (slot r - d for d).
 template <int BW>
 __device__ __forceinline__ double band_dot(const double (&l)[BW + 1], const double (&w)[BW], int r, double a0) {
   double a1 = 0.0, a2 = 0.0;
@@ -448,9 +438,9 @@ __device__ __forceinline__ double band_dot(const double (&l)[BW + 1], const doub
 }
 
 // the BW entries L(m, m-d) (step 1) or L(m+d, m) (step Dd+1) of a band row or
-// column, d = 1..BW, into registers
-// (l[0] = *extra when extra is given: 1/D of the row, loaded in the same
-// group because LDS loads complete in order)
+// column in LDS, d = 1..BW, into l[1..BW] (and l[0] = *extra when extra is
+// given: 1/D of the row, loaded in the same group because LDS loads complete
+// in order)
 template <int BW>
 __device__ __forceinline__ void band_load(const double* p, int step, double (&l)[BW + 1],
                                           const double* extra = nullptr) {
@@ -487,8 +477,21 @@ __device__ __forceinline__ void backward_row(const double* band, int n, int col,
   if (col < n) S[(size_t)m * n + col] = x;
 }
 
+// Column col of S^-1 = L^-T D^-1 L^-1 e_col (col >= n: a lane with no column,
+// which runs along and stores nothing) from the factors in LDS: forward
+// substitution, t = D^-1 z written to the output column, then backward
+// substitution over the stored t.  Every lane of the wave walks the same rows
+// at the same time (a lane whose column starts later carries exact zeros until
+// it does), so the band loads are broadcasts and the lanes' stores of a row
+// are contiguous.  The rows are taken in blocks of BW with the last BW values
+// in a register ring indexed at compile time, each row's BW products summed
+// by fused multiply-adds into three partial sums, and each row's entries
+// loaded while the row before it computes.  The band is stored with the
+// compile-time row stride BW + 1 (BW >= bw), its entries beyond bw and BW rows
+// past the last one zero, so the loads of a row have constant offsets and no
+// guards (a term with a zero L entry adds an exact zero).
 template <int BW>
-__device__ void inverse_column(const double* band, const double* invd, int n, int col, int m0, double* S) {
+__device__ __forceinline__ void inverse_column(const double* band, const double* invd, int n, int col, int m0, double* S) {
   static_assert(BW % 2 == 0, "the row entries alternate between two register sets");
   constexpr int Dd = BW + 1;
   double w[BW];
@@ -600,23 +603,57 @@ __device__ __forceinline__ void geo_solve_body(const IbmParams& p, const GeoBuff
   __syncthreads();
   if (wave == 0) band_ldlt(band, invd, n, bw, Dd, lane, st, w * 65536);
   __syncthreads();
-  double* S = G.sinv + ((size_t)slot * p.W + w) * (size_t)n * n;
-  for (int col0 = 0; col0 < n; col0 += GEO_THREADS) {
-    const int col = col0 + t;
-    const int m0 = col0 + (t & ~63);   // the wave's first column
-    if (m0 < n) {   // lanes past the last column run along (exact zeros) and store nothing
-      const int c = col < n ? col : n + 64;
-      if (BWc == 8)
-        inverse_column<8>(band, invd, n, c, m0, S);
-      else if (BWc == 12)
-        inverse_column<12>(band, invd, n, c, m0, S);
-      else if (BWc == 16)
-        inverse_column<16>(band, invd, n, c, m0, S);
-      else
-        inverse_column<BMAX>(band, invd, n, c, m0, S);
-    }
-  }
+  // the factors (band, then 1/D, contiguous) to global memory for the inverse
+  double* bg = G.bandg + ((size_t)slot * p.W + w) * band_cap(p);
+  const int tot = (n + BWc) * Dd + n + 1;
+  for (int e = t; e < tot; e += GEO_THREADS) bg[e] = band[e];
 }
+
+// The explicit inverse S^-1 of one (step, wing), INV_THREADS columns per
+// workgroup (one wave, so that the LDS broadcasts of the band serve one wave
+// per workgroup), from the factors the solve left in global memory, first
+// copied to LDS.
+constexpr int INV_THREADS = 64;
+
+__device__ __forceinline__ void geo_inverse_body(const IbmParams& p, const GeoBuffers& G, DevStep* st, int slot0,
+                                                 int b, int w, int chunk) {
+  extern __shared__ double lds_i[];
+  const int slot = slot0 + b, n = p.SPW;
+  const int col0 = chunk * INV_THREADS;
+  if (col0 >= n) return;
+  const GeoStep* gs = G.step + slot;
+  // the solve of this (step, wing) ran only without an error before it and
+  // with adjoint nodes and an admissible band
+  if (load_agent(&st->error) != 0 || gs->cnt[w] == 0) return;
+  const int bw = gs->bw[w];
+  if (bw > BMAX) return;
+  const int BWc = bw <= 8 ? 8 : (bw <= 12 ? 12 : (bw <= 16 ? 16 : BMAX));
+  const double* bg = G.bandg + ((size_t)slot * p.W + w) * band_cap(p);
+  const int tot = (n + BWc) * (BWc + 1) + n + 1;
+  for (int e = threadIdx.x; e < tot; e += INV_THREADS) lds_i[e] = bg[e];
+  __syncthreads();
+  const double* band = lds_i;
+  const double* invd = band + (size_t)(n + BWc) * (BWc + 1);
+  double* S = G.sinv + ((size_t)slot * p.W + w) * (size_t)n * n;
+  const int col = col0 + (int)threadIdx.x;
+  const int c = col < n ? col : n + INV_THREADS;   // lanes past the last column run along and store nothing
+  if (BWc == 8)
+    inverse_column<8>(band, invd, n, c, col0, S);
+  else if (BWc == 12)
+    inverse_column<12>(band, invd, n, c, col0, S);
+  else if (BWc == 16)
+    inverse_column<16>(band, invd, n, c, col0, S);
+  else
+    inverse_column<BMAX>(band, invd, n, c, col0, S);
+}
+
+__global__ __launch_bounds__(INV_THREADS) void k_geo_inverse(IbmParams p, GeoBuffers G, DevStep* st, int slot0,
+                                                             int chunks) {
+  const int b = blockIdx.x / chunks, chunk = blockIdx.x - b * chunks;
+  geo_inverse_body(p, G, st, slot0, b, blockIdx.y, chunk);
+}
+
+int inverse_chunks(int spw) { return (spw + INV_THREADS - 1) / INV_THREADS; }
 
 __global__ __launch_bounds__(GEO_THREADS) void k_geo_solve(IbmParams p, GeoBuffers G, DevStep* st, int slot0) {
   geo_solve_body(p, G, st, slot0, blockIdx.x, blockIdx.y);
@@ -983,6 +1020,9 @@ void launch_geometry(hipStream_t s, const IbmParams& p, const GeoBuffers& G, Dev
   hipLaunchKernelGGL(k_geo_wing, grid, dim3(GEO_THREADS), geo_wing_lds_bytes(p.SPW), s, p, G, st, slot0, n0, fidx0);
   if (with_inverse) {
     hipLaunchKernelGGL(k_geo_solve, grid, dim3(GEO_THREADS), geo_solve_lds_bytes(p), s, p, G, st, slot0);
+    const int ch = inverse_chunks(p.SPW);
+    hipLaunchKernelGGL(k_geo_inverse, dim3(count * ch, p.W), dim3(INV_THREADS), geo_solve_lds_bytes(p), s, p, G, st,
+                       slot0, ch);
     if (p.W >= 2)
       hipLaunchKernelGGL(k_geo_couple, dim3(count), dim3(CPL_THREADS), geo_couple_lds_bytes(), s, p, G, st, slot0);
   }
@@ -1182,6 +1222,18 @@ __global__ __launch_bounds__(GEO_THREADS) void kb_geo_solve(const EpDesc* __rest
   geo_solve_body(D.p, D.G, D.st, ep_slot(D, k0), b, w);
 }
 
+// inverse columns chunk*64 .. of each (step, wing) item (chunks past the
+// episode's last column return)
+__global__ __launch_bounds__(INV_THREADS) void kb_geo_inverse(const EpDesc* __restrict__ eps,
+                                                              const int2* __restrict__ items,
+                                                              const int* __restrict__ kb, int kk0) {
+  const int2 it = items[blockIdx.x];
+  const EpDesc& D = eps[it.x];
+  const int k0 = kb[0] + kk0;
+  const int b = it.y / D.p.W, w = it.y - b * D.p.W;
+  geo_inverse_body(D.p, D.G, D.st, ep_slot(D, k0), b, w, blockIdx.y);
+}
+
 // coupling pass of step b of each episode with two or more wings: the
 // workgroup of its (b, wing 0) item
 __global__ __launch_bounds__(CPL_THREADS) void kb_geo_couple(const EpDesc* __restrict__ eps,
@@ -1195,10 +1247,11 @@ __global__ __launch_bounds__(CPL_THREADS) void kb_geo_couple(const EpDesc* __res
 }
 
 void launch_batch_geometry(hipStream_t s, const EpDesc* eps, const int* kb, const int2* items, int n_items,
-                           int k0, size_t lds_wing, size_t lds_solve) {
+                           int k0, size_t lds_wing, size_t lds_solve, int inv_chunks) {
   if (n_items <= 0) return;
   hipLaunchKernelGGL(kb_geo_wing, dim3(n_items), dim3(GEO_THREADS), lds_wing, s, eps, items, kb, k0);
   hipLaunchKernelGGL(kb_geo_solve, dim3(n_items), dim3(GEO_THREADS), lds_solve, s, eps, items, kb, k0);
+  hipLaunchKernelGGL(kb_geo_inverse, dim3(n_items, inv_chunks), dim3(INV_THREADS), lds_solve, s, eps, items, kb, k0);
   hipLaunchKernelGGL(kb_geo_couple, dim3(n_items), dim3(CPL_THREADS), geo_couple_lds_bytes(), s, eps, items, kb, k0);
 }
 

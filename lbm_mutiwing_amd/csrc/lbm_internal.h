@@ -114,12 +114,19 @@ struct GeoBuffers {
   int* adj_ij;                  // [slots][cap][2] (i, j) 0-based
   int* adj_seg;                 // [slots][cap] claiming segment (wing-local)
   double* sinv;                 // [slots][W][SPW][SPW] inverse of S = D D^T, zig-zag order
+  double* bandg;                // [slots][W][band_cap] the factors L D L^T of S (band rows, then 1/D)
   double* cpl;                  // [slots][CPL_KT][CPL_KT] coupling matrix M (cpl_n x cpl_n)
   int* map;                     // [slots][W][2][map_cap] claim maps (owner, lowest position)
   const Frame* frame;           // per step of the current call
   const double* step_table;     // file mode: step_count * ncols
   const double* stream_rows;    // stream mode: start row, target row (2*LBM_MAX_MOTION_COLS)
 };
+
+// doubles per (slot, wing) of GeoBuffers::bandg: (SPW + BMAX) band rows of
+// stride BMAX + 1 at most, then SPW + 1 entries of 1/D
+__host__ __device__ __forceinline__ size_t band_cap(const IbmParams& p) {
+  return (size_t)(p.SPW + BMAX) * (BMAX + 1) + p.SPW + 1;
+}
 
 // The adjoint nodes of one step, which the full-lattice kernel leaves to
 // k_adj_collide (their only writer): the wings' support rectangles and claim
@@ -286,7 +293,9 @@ void launch_step_base(hipStream_t s, int* p, int v, bool add);
 // Geometry of substeps *kb + k0 .. *kb + k0 + count - 1 of every episode with
 // wings; items are (episode, b*W + w) for b < count.
 void launch_batch_geometry(hipStream_t s, const EpDesc* eps, const int* kb, const int2* items, int n_items,
-                           int k0, size_t lds_wing, size_t lds_solve);
+                           int k0, size_t lds_wing, size_t lds_solve, int inv_chunks);
+// workgroups of the inverse per (step, wing) of a wing of spw segments
+int inverse_chunks(int spw);
 size_t geo_wing_lds_bytes(int spw);
 int bc_ring_blocks(const Grid& g);
 int spread_blocks(const IbmParams& p);

@@ -56,6 +56,7 @@ def device_bytes(cfg: SolverConfig) -> int:
         side = math.ceil(2.0 * r / cfg.scale) + 12
         per_slot = W * spw * spw * 8 + W * 2 * side * side * 4 + 36 * W * spw * 12 + W * spw * 160
         per_slot += 64 * 64 * 8 if W > 1 else 0         # coupling matrix M
+        per_slot += W * ((spw + 24) * 25 + spw + 1) * 8  # factors of S
         nslot = max(1, min(16, (256 << 20) // per_slot))
         total += 2 * nslot * per_slot
     return int(total)
