@@ -113,6 +113,10 @@ struct lbm_handle {
   hipEvent_t ev_geo[2] = {nullptr, nullptr};  // geometry of a slot set is ready
   hipEvent_t ev_rel[2] = {nullptr, nullptr};  // last step using a slot set has finished
   int device = 0;
+  // band substitution in the flow instead of the explicit inverse: 0 never,
+  // 1 on steps with rigid-body coupling (their geometry is on the critical
+  // path), 2 on every step
+  int direct = 1;
   double feq[9];
   double w[9];
   // populations
@@ -395,6 +399,32 @@ int env_waves() {
   return -1;
 }
 
+// A bound on the half-width of S = D D^T in band positions at any pose.  Two
+// segments couple only when their support boxes (the nodes within 2.5
+// spacings of the midpoints) share a node, so when the midpoints are at most
+// 5 spacings apart in x and in y, hence at most 5 sqrt(2) spacings apart in
+// any frame.  The distance between two midpoints of a wing is fixed under its
+// rigid motion, so the largest band distance of the pairs within that radius
+// bounds every step's band.
+int band_bound(const double* xb, const double* yb, int W, int P, double scale) {
+  const int n = P - 1;
+  const double r = 5.0 * std::sqrt(2.0) * scale * (1.0 + 1e-9);
+  int bw = 0;
+  std::vector<double> mx(n), my(n);
+  for (int w = 0; w < W; w++) {
+    for (int k = 0; k < n; k++) {
+      mx[k] = 0.5 * (xb[w * P + k] + xb[w * P + k + 1]);
+      my[k] = 0.5 * (yb[w * P + k] + yb[w * P + k + 1]);
+    }
+    for (int k = 0; k < n; k++)
+      for (int l = k + 1; l < n; l++) {
+        const double dx = mx[k] - mx[l], dy = my[k] - my[l];
+        if (dx * dx + dy * dy <= r * r) bw = std::max(bw, std::abs(pos_of_seg(k, n) - pos_of_seg(l, n)));
+      }
+  }
+  return bw;
+}
+
 int lds_per_cu(int device) {
   int v = 0;
   if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device) != hipSuccess) v = 0;
@@ -450,6 +480,9 @@ int alloc_ibm(lbm_handle* h, const std::vector<double>& hx, const std::vector<do
   const size_t ns = 2 * (size_t)h->nslot;
   int rc = 0;
   double* m0;
+  // the band substitution loads ahead past both ends of a band
+  const size_t band_pad = (size_t)BAND_PAD_ROWS * (BMAX + 1);
+  double* bandg = nullptr;
   if ((rc = ialloc(h, &m0, 2 * npts))) return rc;
   G.xb0 = m0;
   G.yb0 = m0 + npts;
@@ -460,10 +493,12 @@ int alloc_ibm(lbm_handle* h, const std::vector<double>& hx, const std::vector<do
       (rc = ialloc(h, &G.phy, ns * nseg * BOXW)) || (rc = ialloc(h, &G.seg_off, ns * W * (spw + 1))) ||
       (rc = ialloc(h, &G.adj_ij, ns * cap * 2)) || (rc = ialloc(h, &G.adj_seg, ns * cap)) ||
       (rc = ialloc(h, &G.sinv, h->W > 0 ? ns * W * spw * spw : 1)) ||
-      (rc = ialloc(h, &G.bandg, h->W > 0 ? ns * W * band_cap(p) : 1)) ||
+      (rc = ialloc(h, &bandg, (h->W > 0 ? ns * W * band_cap(p) : 1) + 2 * band_pad)) ||
       (rc = ialloc(h, &G.map, h->W > 0 ? ns * W * 2 * (size_t)p.map_cap : 1)) ||
       (rc = ialloc(h, &G.cpl, h->W > 1 ? ns * CPL_KT * CPL_KT : 1)))
     return rc;
+  G.bandg = bandg + band_pad;
+  HIPCHECK(h, hipMemset(bandg, 0, sizeof(double) * ((h->W > 0 ? ns * W * band_cap(p) : 1) + 2 * band_pad)));
   const int bpw = adj_collide_blocks_per_wing(p);
   if ((rc = ialloc(h, &F.cvec, 2 * nseg)) || (rc = ialloc(h, &F.yvec, 2 * nseg)) ||
       (rc = ialloc(h, &F.adj_g, 2 * cap)) || (rc = ialloc(h, &F.partial, 2 * 3 * MAXW * (size_t)bpw)))
@@ -571,6 +606,7 @@ int lbm_create(const lbm_config* cfg, lbm_handle** out) {
   ip.motion_source = cfg->motion_source;
   ip.step_count = 0; ip.ncols = 1 + LBM_MOTION_COLS_PER_WING * h->W;
   ip.motion_dt = ip.dt;
+  ip.band_cls = band_class(band_bound(cfg->xb0, cfg->yb0, h->W, h->P, cfg->scale));
   int lds_max = 0;
   (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device);
   if (h->W > 0 && geo_solve_lds_bytes(ip) > (size_t)lds_max) {
@@ -583,6 +619,8 @@ int lbm_create(const lbm_config* cfg, lbm_handle** out) {
   {
     const char* e = std::getenv("LBM_GEO_CARRY");
     h->carry_on = !(e && e[0] == '0');
+    const char* d = std::getenv("LBM_DIRECT_SOLVE");
+    h->direct = d ? std::max(0, std::min(2, atoi(d))) : 1;
   }
 
   int rc = 0;
@@ -869,7 +907,7 @@ int advance_dynamic(lbm_handle* h, int32_t n_last, double* force_rows, double* m
       HIPCHECK(h, hipMemcpy(h->d_st->rel, h->rel, sizeof(h->rel), hipMemcpyHostToDevice));
       upload = false;
     }
-    launch_geometry(h->stream_geo, h->ip, h->G, h->d_st, slot, m, 0, 1, true);
+    launch_geometry(h->stream_geo, h->ip, h->G, h->d_st, slot, m, 0, 1, true, h->direct >= 1);
     HIPCHECK(h, hipEventRecord(h->ev_geo[set], h->stream_geo));
     h->batch_count++;
     FieldSlot* fs = &h->slot[s];
@@ -887,7 +925,7 @@ int advance_dynamic(lbm_handle* h, int32_t n_last, double* force_rows, double* m
     launch_bc_ring(h->stream_bc, h->g, h->mrt, h->R, src, dst, fs, h->f32);
     HIPCHECK(h, hipEventRecord(h->ev_bc, h->stream_bc));
     HIPCHECK(h, hipStreamWaitEvent(h->stream_ibm, h->ev_geo[set], 0));
-    launch_ibm_flow(h->stream_ibm, h->ip, h->mrt, h->G, slot, h->F, h->g, src, h->f32);
+    launch_ibm_flow(h->stream_ibm, h->ip, h->mrt, h->G, slot, h->F, h->g, src, h->f32, h->direct >= 1);
     HIPCHECK(h, hipStreamWaitEvent(h->stream_ibm, h->ev_step, 0));   // the lattice kernel's writes first
     launch_adj_collide(h->stream_ibm, h->g, h->mrt, h->ip, h->G, slot, h->F, m, src, dst, fs, h->f32);
     launch_force_row(h->stream_ibm, h->ip, h->G, slot, F, h->d_st, m, k, h->row_len);
@@ -1010,13 +1048,13 @@ int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motio
     const int gs = set_of(b), k0 = b * ns - r;   // first step of the batch, call-relative (b >= 1 when r > 0)
     const int own = std::min(ns, nsteps - k0);
     HIPCHECK(h, hipStreamWaitEvent(h->stream_geo, h->ev_rel[gs], 0));
-    launch_geometry(h->stream_geo, h->ip, h->G, h->d_st, gs * ns, n0 + k0, k0, own, true);
+    launch_geometry(h->stream_geo, h->ip, h->G, h->d_st, gs * ns, n0 + k0, k0, own, true, h->direct == 2);
     HIPCHECK(h, hipEventRecord(h->ev_geo[gs], h->stream_geo));
     const int rest = ns - own;
     if (b == nbatch - 1 && rest > 0 && h->carry_on) {
       HIPCHECK(h, hipMemsetAsync(&h->d_st_carry->error, 0, sizeof(int), h->stream_geo));
       launch_geometry(h->stream_geo, h->ip, h->G, h->d_st_carry, gs * ns + own, n0 + k0 + own, k0 + own, rest,
-                      true);
+                      true, h->direct == 2);
       HIPCHECK(h, hipMemcpyAsync(&h->h_out->carry_err, &h->d_st_carry->error, sizeof(int), hipMemcpyDeviceToHost,
                                  h->stream_geo));
       lbm_handle::GeoCarry& c = h->carry;
@@ -1085,7 +1123,7 @@ int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motio
       launch_force_row(h->stream_ibm, h->ip, h->G, prev_slot, h->F, h->d_st, m - 1, k - 1, h->row_len);
       if (batch_start) HIPCHECK(h, hipEventRecord(h->ev_rel[prev_set], h->stream_ibm));
     }
-    launch_ibm_flow(h->stream_ibm, h->ip, h->mrt, h->G, slot, h->F, h->g, src, h->f32);
+    launch_ibm_flow(h->stream_ibm, h->ip, h->mrt, h->G, slot, h->F, h->g, src, h->f32, h->direct == 2);
     launch_adj_collide(h->stream_ibm, h->g, h->mrt, h->ip, h->G, slot, h->F, m, src, dst, fs, h->f32);
     HIPCHECK(h, hipStreamWaitEvent(h->stream_ibm, h->ev_bc, 0));
     HIPCHECK(h, hipEventRecord(h->ev_ibm, h->stream_ibm));

@@ -26,6 +26,7 @@
 #include <algorithm>
 #include <climits>
 
+#include "lbm_band.h"
 #include "lbm_internal.h"
 
 namespace lbm {
@@ -557,7 +558,7 @@ __device__ __forceinline__ void inverse_column(const double* band, const double*
 // One workgroup per (step, wing): adjoint total, band assembly of S,
 // L D L^T and the explicit inverse (one column per thread).
 __device__ __forceinline__ void geo_solve_body(const IbmParams& p, const GeoBuffers& G, DevStep* st, int slot0, int b,
-                                               int w) {
+                                               int w, int direct) {
   extern __shared__ double lds_s[];
   __shared__ int s_skip;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -577,17 +578,28 @@ __device__ __forceinline__ void geo_solve_body(const IbmParams& p, const GeoBuff
       gs->adj_n = tot;
       gs->band[0] = b0;
       gs->band[1] = b1;
+      // the coupling of wings (k_geo_couple) reads the inverse; it needs
+      // wings that share a node, so support rectangles that meet
+      int meet = 0;
+      for (int q = 0; q < p.W; q++)
+        for (int r = q + 1; r < p.W; r++) {
+          const int* a = gs->rect[q];
+          const int* c = gs->rect[r];
+          meet |= a[1] >= a[0] && c[1] >= c[0] && a[0] <= c[1] && c[0] <= a[1] && a[2] <= c[3] && c[2] <= a[3];
+        }
+      gs->inv_on = !direct || meet;
     }
   }
   __syncthreads();
   const int bw = gs->bw[w];
   if (s_skip || gs->cnt[w] == 0) return;
-  if (bw > BMAX) {
+  // the LDS of the launch holds bands up to the class of the instance's bound
+  if (bw > BMAX || band_class(bw) > p.band_cls) {
     if (t == 0) raise_error(st, DERR_BAND, bw);
     return;
   }
   // row stride: the compile-time band bound of the inverse below
-  const int BWc = bw <= 8 ? 8 : (bw <= 12 ? 12 : (bw <= 16 ? 16 : BMAX));
+  const int BWc = band_class(bw);
   const int Dd = BWc + 1;
   double* band = lds_s;                          // (n + BWc) rows of Dd
   double* invd = band + (size_t)(n + BWc) * Dd;
@@ -624,10 +636,10 @@ __device__ __forceinline__ void geo_inverse_body(const IbmParams& p, const GeoBu
   const GeoStep* gs = G.step + slot;
   // the solve of this (step, wing) ran only without an error before it and
   // with adjoint nodes and an admissible band
-  if (load_agent(&st->error) != 0 || gs->cnt[w] == 0) return;
+  if (load_agent(&st->error) != 0 || gs->cnt[w] == 0 || !gs->inv_on) return;
   const int bw = gs->bw[w];
-  if (bw > BMAX) return;
-  const int BWc = bw <= 8 ? 8 : (bw <= 12 ? 12 : (bw <= 16 ? 16 : BMAX));
+  if (bw > BMAX || band_class(bw) > p.band_cls) return;
+  const int BWc = band_class(bw);
   const double* bg = G.bandg + ((size_t)slot * p.W + w) * band_cap(p);
   const int tot = (n + BWc) * (BWc + 1) + n + 1;
   for (int e = threadIdx.x; e < tot; e += INV_THREADS) lds_i[e] = bg[e];
@@ -655,14 +667,16 @@ __global__ __launch_bounds__(INV_THREADS) void k_geo_inverse(IbmParams p, GeoBuf
 
 int inverse_chunks(int spw) { return (spw + INV_THREADS - 1) / INV_THREADS; }
 
-__global__ __launch_bounds__(GEO_THREADS) void k_geo_solve(IbmParams p, GeoBuffers G, DevStep* st, int slot0) {
-  geo_solve_body(p, G, st, slot0, blockIdx.x, blockIdx.y);
+__global__ __launch_bounds__(GEO_THREADS) void k_geo_solve(IbmParams p, GeoBuffers G, DevStep* st, int slot0,
+                                                           int direct) {
+  geo_solve_body(p, G, st, slot0, blockIdx.x, blockIdx.y, direct);
 }
 
-// band of (SPW + BMAX) rows of stride BMAX + 1 at most, and 1/D (SPW + 1
-// entries: the inverse loads one row ahead)
+// band of (SPW + c) rows of stride c + 1, c = the band class of the
+// instance's bound, and 1/D (SPW + 1 entries: the inverse loads one row ahead)
 size_t geo_solve_lds_bytes(const IbmParams& p) {
-  return ((size_t)(p.SPW + BMAX) * (BMAX + 1) + p.SPW + 1) * sizeof(double);
+  const size_t c = (size_t)p.band_cls;
+  return ((p.SPW + c) * (c + 1) + p.SPW + 1) * sizeof(double);
 }
 
 // ------------------------------------------------------------ coupled wings
@@ -1014,12 +1028,13 @@ __global__ __launch_bounds__(CPL_THREADS) void k_ibm_couple(IbmParams p, GeoBuff
 }
 
 void launch_geometry(hipStream_t s, const IbmParams& p, const GeoBuffers& G, DevStep* st, int slot0, int n0,
-                     int fidx0, int count, bool with_inverse) {
+                     int fidx0, int count, bool with_inverse, bool direct) {
   if (p.W <= 0 || count <= 0) return;
   dim3 grid(count, p.W);
   hipLaunchKernelGGL(k_geo_wing, grid, dim3(GEO_THREADS), geo_wing_lds_bytes(p.SPW), s, p, G, st, slot0, n0, fidx0);
   if (with_inverse) {
-    hipLaunchKernelGGL(k_geo_solve, grid, dim3(GEO_THREADS), geo_solve_lds_bytes(p), s, p, G, st, slot0);
+    hipLaunchKernelGGL(k_geo_solve, grid, dim3(GEO_THREADS), geo_solve_lds_bytes(p), s, p, G, st, slot0,
+                       direct ? 1 : 0);
     const int ch = inverse_chunks(p.SPW);
     hipLaunchKernelGGL(k_geo_inverse, dim3(count * ch, p.W), dim3(INV_THREADS), geo_solve_lds_bytes(p), s, p, G, st,
                        slot0, ch);
@@ -1098,8 +1113,40 @@ __device__ __forceinline__ void ibm_matvec_body(const IbmParams& p, const GeoBuf
   }
 }
 
+// y = S^-1 c, one wave per row of the inverse, on steps that formed it.
 __global__ __launch_bounds__(256) void k_ibm_matvec(IbmParams p, GeoBuffers G, int slot, FlowBuffers F) {
+  if (!G.step[slot].inv_on) return;
   ibm_matvec_body(p, G, slot, F, blockIdx.x);
+}
+
+// y = S^-1 c by band substitution (lbm_band.h) on steps without the inverse:
+// workgroup w solves wing w, wave c component c.
+constexpr int BSOLVE_THREADS = 128;
+
+__global__ __launch_bounds__(BSOLVE_THREADS) void k_ibm_bsolve(IbmParams p, GeoBuffers G, int slot,
+                                                               FlowBuffers F) {
+  const GeoStep* gs = G.step + slot;
+  const int w = blockIdx.x, comp = threadIdx.x >> 6;
+  if (gs->inv_on) return;
+  const int n = p.SPW, bw = gs->bw[w];
+  const double* c = F.cvec + 2 * (size_t)w * n + comp;
+  double* y = F.yvec + 2 * (size_t)w * n + comp;
+  if (gs->cnt[w] == 0 || bw > BMAX || band_class(bw) > p.band_cls) {
+    // no factors (the wing claims no node, or the solve raised an error)
+    for (int m = threadIdx.x & 63; m < n; m += 64) y[2 * m] = 0.0;
+    return;
+  }
+  const int BWc = band_class(bw);
+  const double* band = G.bandg + ((size_t)slot * p.W + w) * band_cap(p);
+  const double* invd = band + (size_t)(n + BWc) * (BWc + 1);
+  if (BWc == 8)
+    band_solve_wave<8>(band, invd, n, c, y);
+  else if (BWc == 12)
+    band_solve_wave<12>(band, invd, n, c, y);
+  else if (BWc == 16)
+    band_solve_wave<16>(band, invd, n, c, y);
+  else
+    band_solve_wave<BMAX>(band, invd, n, c, y);
 }
 
 // g = D^T y at the adjoint nodes (boundary_condition.f90:1639-1651).  A
@@ -1185,7 +1232,7 @@ __global__ __launch_bounds__(SPREAD_THREADS) void k_ibm_spread(IbmParams p, GeoB
 }
 
 void launch_ibm_flow(hipStream_t s, const IbmParams& p, const MrtParams& P, const GeoBuffers& G, int slot,
-                     const FlowBuffers& F, const Grid& g, const void* src, bool f32) {
+                     const FlowBuffers& F, const Grid& g, const void* src, bool f32, bool direct) {
   if (p.W <= 0) return;
   const int blk = (p.nseg + 3) / 4;
   if (f32)
@@ -1193,6 +1240,7 @@ void launch_ibm_flow(hipStream_t s, const IbmParams& p, const MrtParams& P, cons
   else
     hipLaunchKernelGGL(k_ibm_rhs<double>, dim3(blk), dim3(256), 0, s, p, P, G, slot, F, g, (const double*)src);
   hipLaunchKernelGGL(k_ibm_matvec, dim3(blk), dim3(256), 0, s, p, G, slot, F);
+  if (direct) hipLaunchKernelGGL(k_ibm_bsolve, dim3(p.W), dim3(BSOLVE_THREADS), 0, s, p, G, slot, F);
   if (p.W >= 2) hipLaunchKernelGGL(k_ibm_couple, dim3(1), dim3(CPL_THREADS), 0, s, p, G, slot, F);
   const int nb = (p.SPW + SPREAD_SEG - 1) / SPREAD_SEG;
   hipLaunchKernelGGL(k_ibm_spread, dim3(p.W * nb), dim3(SPREAD_THREADS), 0, s, p, G, slot, F);
@@ -1219,7 +1267,7 @@ __global__ __launch_bounds__(GEO_THREADS) void kb_geo_solve(const EpDesc* __rest
   const EpDesc& D = eps[it.x];
   const int k0 = kb[0] + kk0;
   const int b = it.y / D.p.W, w = it.y - b * D.p.W;
-  geo_solve_body(D.p, D.G, D.st, ep_slot(D, k0), b, w);
+  geo_solve_body(D.p, D.G, D.st, ep_slot(D, k0), b, w, 0);
 }
 
 // inverse columns chunk*64 .. of each (step, wing) item (chunks past the

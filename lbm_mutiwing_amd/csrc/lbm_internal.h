@@ -6,6 +6,7 @@
 #include <stdint.h>
 
 #include "../../include/lbm_hip.h"
+#include "lbm_band.h"
 #include "lbm_device.h"
 
 namespace lbm {
@@ -76,6 +77,10 @@ struct GeoStep {
   int cpl_on, cpl_n;
   int cpl_off[MAXW + 1];
   int cpl_k[CPL_KT], cpl_w[CPL_KT];
+  // 1: the step's system is solved with the explicit inverse S^-1 (dense
+  // mat-vec); 0: by substitution with the band factors (launches asked for
+  // it and no two wings' support rectangles meet)
+  int inv_on;
   Frame frame;
 };
 
@@ -90,7 +95,7 @@ struct IbmParams {
   int motion_source;       // LBM_MOTION_FILE / LBM_MOTION_STREAM
   int step_count;          // rows of the file-mode step table
   int ncols;
-  int pad;
+  int band_cls;            // band_class() of a bound on every step's band half-width
   double motion_dt;
   // rigid-body coupling of wing 1 (boundary_condition.f90:1694-1719)
   int dyn_on, dyn_rk4;
@@ -215,6 +220,12 @@ struct BatchItems {
 // Band position <-> segment in a wing of n segments: positions 0,1,2,3,...
 // hold segments 0, n-1, 1, n-2, ... so that segments facing each other on
 // the upper and lower surface are neighbours and S is a narrow band.
+// Row stride class of a band of half-width bw (the band kernels are compiled
+// for these strides; BMAX is the widest band accepted).
+__host__ __device__ __forceinline__ int band_class(int bw) {
+  return bw <= 8 ? 8 : (bw <= 12 ? 12 : (bw <= 16 ? 16 : BMAX));
+}
+
 __host__ __device__ __forceinline__ int seg_of_pos(int q, int n) { return (q & 1) ? n - 1 - (q >> 1) : (q >> 1); }
 __host__ __device__ __forceinline__ int pos_of_seg(int k, int n) {
   return k < (n + 1) / 2 ? 2 * k : 2 * (n - 1 - k) + 1;
@@ -273,8 +284,10 @@ int adj_collide_blocks_per_wing(const IbmParams& p);
 
 // IBM geometry kernels (lbm_ibm.hip), one launch covers `count` consecutive
 // steps n0.. into slots slot0..; frames are indexed from fidx0.
+// direct: the flow solves by band substitution when the step allows it (no
+// explicit inverse); otherwise the inverse is formed
 void launch_geometry(hipStream_t s, const IbmParams& p, const GeoBuffers& G, DevStep* st, int slot0, int n0,
-                     int fidx0, int count, bool with_inverse);
+                     int fidx0, int count, bool with_inverse, bool direct = false);
 size_t geo_solve_lds_bytes(const IbmParams& p);
 // Batched launches over the episodes of eps (device array).  The substep of a
 // launch is *it.kbase + k (the base lives on the device, so that a captured
@@ -299,8 +312,9 @@ int inverse_chunks(int spw);
 size_t geo_wing_lds_bytes(int spw);
 int bc_ring_blocks(const Grid& g);
 int spread_blocks(const IbmParams& p);
-// IBM flow kernels of one step.
+// IBM flow kernels of one step (direct: the step's geometry was launched
+// with direct, so that its system may be solved by band substitution)
 void launch_ibm_flow(hipStream_t s, const IbmParams& p, const MrtParams& P, const GeoBuffers& G, int slot,
-                     const FlowBuffers& F, const Grid& g, const void* src, bool f32);
+                     const FlowBuffers& F, const Grid& g, const void* src, bool f32, bool direct = false);
 
 }  // namespace lbm
