@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <exception>
 #include <functional>
 #include <vector>
 
@@ -211,6 +212,16 @@ int fail(lbm_handle* h, int code, const char* fmt, ...) {
   return code;
 }
 
+// Host exceptions (a failed host allocation) never leave the C ABI: the
+// entry point returns LBM_E_INVALID with the exception's text.
+#define LBM_CATCH(h, name)                                                                 \
+  catch (const std::exception& e_) {                                                         \
+    return fail((h), LBM_E_INVALID, "%s: host exception: %s", name, e_.what());             \
+  }                                                                                          \
+  catch (...) {                                                                              \
+    return fail((h), LBM_E_INVALID, "%s: unknown host exception", name);                    \
+  }
+
 #define HIPCHECK(h, expr)                                                                         \
   do {                                                                                            \
     hipError_t e_ = (expr);                                                                       \
@@ -407,6 +418,7 @@ int env_waves() {
 // rigid motion, so the largest band distance of the pairs within that radius
 // bounds every step's band.
 int band_bound(const double* xb, const double* yb, int W, int P, double scale) {
+  if (W <= 0 || P < 2) return 0;   // no wings: no band
   const int n = P - 1;
   const double r = 5.0 * std::sqrt(2.0) * scale * (1.0 + 1e-9);
   int bw = 0;
@@ -528,7 +540,7 @@ int lbm_last_error(const lbm_handle* h, char* buf, size_t n) {
   return 0;
 }
 
-int lbm_create(const lbm_config* cfg, lbm_handle** out) {
+int lbm_create(const lbm_config* cfg, lbm_handle** out) try {
   if (!cfg || !out) return fail(nullptr, LBM_E_INVALID, "null argument");
   *out = nullptr;
   if (cfg->nx < 8 || cfg->ny < 8) return fail(nullptr, LBM_E_INVALID, "lattice must be at least 8x8");
@@ -701,7 +713,7 @@ int lbm_create(const lbm_config* cfg, lbm_handle** out) {
   }
   *out = h;
   return 0;
-}
+} LBM_CATCH(nullptr, "lbm_create")
 
 int lbm_destroy(lbm_handle* h) {
   if (!h) return 0;
@@ -712,7 +724,7 @@ int lbm_destroy(lbm_handle* h) {
   return 0;
 }
 
-int lbm_reset(lbm_handle* h) {
+int lbm_reset(lbm_handle* h) try {
   if (!h) return LBM_E_INVALID;
   HIPCHECK(h, hipStreamSynchronize(h->stream_geo));
   HIPCHECK(h, hipStreamSynchronize(h->stream_ibm));
@@ -738,10 +750,10 @@ int lbm_reset(lbm_handle* h) {
   h->geo_epoch++;
   update_body_extra(h);
   return 0;
-}
+} LBM_CATCH(h, "lbm_reset")
 
 // load_motion_table checks + build_motion_step_table (mode.f90:104-178, 327-412).
-int lbm_set_motion_table(lbm_handle* h, const double* rows, int32_t nrows, int32_t ncols, double* end_time) {
+int lbm_set_motion_table(lbm_handle* h, const double* rows, int32_t nrows, int32_t ncols, double* end_time) try {
   if (!h || !rows) return fail(h, LBM_E_INVALID, "null argument");
   const int nc = h->ip.ncols;
   if (ncols != nc) return fail(h, LBM_E_INVALID, "motion table must have %d columns, got %d", nc, ncols);
@@ -795,9 +807,9 @@ int lbm_set_motion_table(lbm_handle* h, const double* rows, int32_t nrows, int32
   h->motion_ready = true;
   if (end_time) *end_time = end;
   return 0;
-}
+} LBM_CATCH(h, "lbm_set_motion_table")
 
-int lbm_set_stream_segment(lbm_handle* h, const double* start_row, const double* target_row, int32_t ncols) {
+int lbm_set_stream_segment(lbm_handle* h, const double* start_row, const double* target_row, int32_t ncols) try {
   if (!h || !start_row || !target_row) return fail(h, LBM_E_INVALID, "null argument");
   if (ncols != h->ip.ncols) return fail(h, LBM_E_INVALID, "stream rows must have %d columns", h->ip.ncols);
   std::vector<double> z(2 * LBM_MAX_MOTION_COLS, 0.0);
@@ -810,7 +822,7 @@ int lbm_set_stream_segment(lbm_handle* h, const double* start_row, const double*
   h->motion_ready = true;
   h->geo_epoch++;
   return 0;
-}
+} LBM_CATCH(h, "lbm_set_stream_segment")
 
 int lbm_build_stream_target_row(int32_t W, const double* start, double target_time, const double* delta,
                                 double* target) {
@@ -965,7 +977,7 @@ int advance_dynamic(lbm_handle* h, int32_t n_last, double* force_rows, double* m
 extern "C" {
 
 int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motion_rows, double* row_sum,
-                double* last_row, double* err) {
+                double* last_row, double* err) try {
   if (!h) return LBM_E_INVALID;
   if (h->cfg.motion_source == LBM_MOTION_FILE && !h->motion_ready && h->W > 0)
     return fail(h, LBM_E_STATE, "motion table not loaded");
@@ -1154,7 +1166,7 @@ int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motio
   rc = finish_advance(h, nsteps, n_last, h->timing, force_rows, motion_rows, row_sum, last_row, err);
   if (rc == 0 && carry_pending) h->carry.valid = true;
   return rc;
-}
+} LBM_CATCH(h, "lbm_advance")
 
 }  // extern "C"
 
@@ -1241,7 +1253,7 @@ int lbm_get_state(lbm_handle* h, int32_t* n_next, int32_t* movenum, int32_t* fla
   return 0;
 }
 
-int lbm_get_fields(lbm_handle* h, double* rho, double* u, double* v, double* pre) {
+int lbm_get_fields(lbm_handle* h, double* rho, double* u, double* v, double* pre) try {
   if (!h) return LBM_E_INVALID;
   const int m = h->n - 1;
   const int s = ((m % 3) + 3) % 3;
@@ -1257,7 +1269,7 @@ int lbm_get_fields(lbm_handle* h, double* rho, double* u, double* v, double* pre
     for (size_t q = 0; q < np; q++) pre[q] = r[q] / 3.0;
   }
   return 0;
-}
+} LBM_CATCH(h, "lbm_get_fields")
 
 int lbm_get_mass(lbm_handle* h, double* mass) {
   if (!h || !mass) return LBM_E_INVALID;
@@ -1302,7 +1314,7 @@ int lbm_get_mass_noalloc(const void* f, int32_t nx, int32_t ny, int32_t precisio
   return 0;
 }
 
-int lbm_get_populations(lbm_handle* h, double* f, double* f0) {
+int lbm_get_populations(lbm_handle* h, double* f, double* f0) try {
   if (!h) return LBM_E_INVALID;
   const size_t np = (size_t)h->g.nx * h->g.ny;
   HIPCHECK(h, hipStreamSynchronize(h->stream));
@@ -1330,9 +1342,9 @@ int lbm_get_populations(lbm_handle* h, double* f, double* f0) {
       }
   }
   return 0;
-}
+} LBM_CATCH(h, "lbm_get_populations")
 
-int lbm_set_populations(lbm_handle* h, const double* f, const double* f0) {
+int lbm_set_populations(lbm_handle* h, const double* f, const double* f0) try {
   if (!h) return LBM_E_INVALID;
   const size_t np = (size_t)h->g.nx * h->g.ny;
   HIPCHECK(h, hipStreamSynchronize(h->stream));
@@ -1365,7 +1377,7 @@ int lbm_set_populations(lbm_handle* h, const double* f, const double* f0) {
   }
   HIPCHECK(h, hipStreamSynchronize(h->stream));
   return 0;
-}
+} LBM_CATCH(h, "lbm_set_populations")
 
 int lbm_get_markers(lbm_handle* h, double* xb1, double* yb1, double* xb2, double* yb2) {
   if (!h) return LBM_E_INVALID;
@@ -1388,7 +1400,7 @@ int lbm_get_markers(lbm_handle* h, double* xb1, double* yb1, double* xb2, double
 }
 
 int lbm_get_ibm(lbm_handle* h, int32_t capacity, int32_t* ij, int32_t* wing_id, double* wing_force,
-                double* wing_moment) {
+                double* wing_moment) try {
   if (!h) return LBM_E_INVALID;
   HIPCHECK(h, hipStreamSynchronize(h->stream));
   DevStep st;
@@ -1420,7 +1432,7 @@ int lbm_get_ibm(lbm_handle* h, int32_t capacity, int32_t* ij, int32_t* wing_id, 
     if (wing_moment) wing_moment[w] = w < h->W ? st.moment[w] : 0.0;
   }
   return 0;
-}
+} LBM_CATCH(h, "lbm_get_ibm")
 
 int lbm_set_timing(lbm_handle* h, int32_t enable) {
   if (!h) return LBM_E_INVALID;
@@ -1554,12 +1566,21 @@ struct lbm_batch {
 namespace {
 
 int bfail(lbm_batch* b, int code, const char* fmt, ...) {
+  if (!b) return code;
   va_list ap;
   va_start(ap, fmt);
   vsnprintf(b->msg, sizeof(b->msg), fmt, ap);
   va_end(ap);
   return code;
 }
+
+#define LBM_BATCH_CATCH(b, name)                                                           \
+  catch (const std::exception& e_) {                                                         \
+    return bfail((b), LBM_E_INVALID, "%s: host exception: %s", name, e_.what());            \
+  }                                                                                          \
+  catch (...) {                                                                              \
+    return bfail((b), LBM_E_INVALID, "%s: unknown host exception", name);                   \
+  }
 
 #define BCHECK(b, expr)                                                                            \
   do {                                                                                             \
@@ -1647,7 +1668,7 @@ int lbm_batch_last_error(const lbm_batch* b, char* buf, size_t n) {
 }
 
 int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_t steps, double* row_sums,
-                      double* last_rows, double* errs, int32_t* status) {
+                      double* last_rows, double* errs, int32_t* status) try {
   if (!b) return LBM_E_INVALID;
   if (!hs || count < 1 || steps < 0) return bfail(b, LBM_E_INVALID, "need at least one instance and steps >= 0");
   const bool f32 = hs[0] && hs[0]->f32;
@@ -2050,6 +2071,6 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
   }
   if (first_bad >= 0) return bfail(b, first_code, "instance %d: %s", first_bad, hs[first_bad]->msg);
   return 0;
-}
+} LBM_BATCH_CATCH(b, "lbm_batch_advance")
 
 }  // extern "C"
