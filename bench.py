@@ -215,6 +215,7 @@ def run_batch(args):
     b.advance(solvers, K)
     t_batch = time.perf_counter() - t0
     tile, rows, waves = b.tuning_full()
+    cpol = b.cache_policy()
     nodes = sum(c.nx * c.ny for c in cfgs)
     for s in solvers:
         s.close()
@@ -250,7 +251,9 @@ def run_batch(args):
                                f"chord 32-128, Re 100-500", "episodes": n_ep, "seed": args.seed,
                    "total_nodes": nodes},
         "aggregate_mlups": round(nodes * K / t_batch / 1e6, 1),
-        "lattice_tile": tile, "lattice_tiles_per_workgroup": rows, "lattice_waves_per_cu_cap": waves, "timed": "second STEP of the same episodes",
+        "lattice_tile": tile, "lattice_tiles_per_workgroup": rows, "lattice_waves_per_cu_cap": waves,
+        "lattice_cache_policy": ["plain", "non-temporal stores", "non-temporal loads and stores"][cpol],
+        "timed": "second STEP of the same episodes",
         "one_at_a_time": {"fp64_env_steps_per_s": round(base64, 3), "fp32_env_steps_per_s": round(base32, 3),
                           "sample": f"episodes 0..{len(sample) - 1}, {ksample} steps each, scaled to {K}"},
         "speedup_vs_fp64_one_at_a_time": round(value / base64, 3),
@@ -657,6 +660,7 @@ def main():
     s.advance(args.warmup + 2 * args.steps)
     lat_ms, launches = s.timing()
     tile, waves = s.tuning_full()
+    cpol = s.cache_policy()
     st = s.state()
     # the lattice kernel's memory pattern alone (pull of nine planes, no
     # arithmetic), timed on the same device: the ceiling of that pattern
@@ -700,13 +704,15 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_lattice (fused pull-stream + moments + MRT collide)",
                      "kernel_avg_us": round(lat_avg_ms * 1e3, 2), "tile": tile, "waves_per_cu_cap": waves,
+                     "cache_policy": ["plain", "non-temporal stores", "non-temporal loads and stores"][cpol],
                      "algorithmic_bytes_per_launch": bpn * interior},
         "pull_copy_ceiling": {"avg_us": round(ceil_us.value, 2),
                               "achieved": round(bpn * interior / (ceil_us.value * 1e-6) / 1e9, 1), "unit": "GB/s",
                               "frac": round(bpn * interior / (ceil_us.value * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
                               "kernel_over_ceiling": round(lat_avg_ms * 1e3 / ceil_us.value, 4),
                               "what": "the lattice kernel's loads and stores alone (nine pulled planes in, nine out, "
-                                      "no arithmetic), same lattice and storage, 20 launches"},
+                                      "no arithmetic), same lattice and storage, fastest of its cache policies and "
+                                      "wave caps, 20 launches each"},
         "last_force_row": [round(x, 8) for x in r.last_row[:10]],
         "mass_check": {"before": mass0, "after": mass1, "drift_rel": (mass1 - mass0) / mass0,
                        "what": "FP64 sum of every population of the lattice before and after the timed steps "

@@ -143,7 +143,9 @@ int lbm_get_fields(lbm_handle *h, double *rho, double *u, double *v,
 
 /* Populations f (9*Nx*Ny, FP64 values; FP32-shifted instances return
  * w_k + stored deviation) and the persistent f0 boundary ring of the
- * streaming buffer (same layout; interior values are unspecified). */
+ * streaming buffer (same layout; columns 1 and Nx and rows 1 and Ny, as
+ * move_mesh leaves them, move_mesh.f90:30-34; interior values are
+ * unspecified). */
 int lbm_get_populations(lbm_handle *h, double *f, double *f0);
 /* Overwrite f (interior and ring) and the f0 ring; u/v of the previous step
  * become the moments of f (test and restart hook, solution.f90:1721-1851). */
@@ -245,8 +247,9 @@ int lbm_get_timing(lbm_handle *h, double *lattice_ms, int32_t *launches);
  * interior node of an nx x ny lattice pulls its nine populations from its
  * neighbours and writes them, in the storage of `precision`, with no
  * arithmetic), on the current device, in its own buffers, which it
- * allocates and frees.  The ceiling the lattice kernel's time is compared
- * with. */
+ * allocates and frees: the fastest over the cache policies and wave caps the
+ * lattice kernel's autotuner tries.  The ceiling the lattice kernel's time
+ * is compared with. */
 int lbm_measure_pull_copy(int32_t nx, int32_t ny, int32_t precision, int32_t launches, double *avg_us);
 
 /* Batched advance (no counterpart in the reference, which runs one episode
@@ -299,6 +302,16 @@ int lbm_get_tuning(const lbm_handle *h, int32_t *lattice_tile,
                    int32_t *waves_per_cu);
 int lbm_batch_get_tuning(const lbm_batch *b, int32_t *lattice_tile,
                          int32_t *rows_per_item, int32_t *waves_per_cu);
+
+/* Cache policy of the lattice kernel's population loads and stores, chosen
+ * by the same timing (no counterpart in the reference): 0 ordinary, 1
+ * non-temporal stores, 2 non-temporal loads and stores (a batch times 0 and
+ * 2 with its wave cap).  LBM_LATTICE_CP=<0|1|2> in the environment fixes it. */
+#define LBM_CACHE_PLAIN 0
+#define LBM_CACHE_NT_STORE 1
+#define LBM_CACHE_NT 2
+int lbm_get_lattice_policy(const lbm_handle *h, int32_t *cache_policy);
+int lbm_batch_get_lattice_policy(const lbm_batch *b, int32_t *cache_policy);
 
 /* Message of the last failed call on h (or of the last failed lbm_create when
  * h is NULL).  Always NUL-terminates when n > 0. */

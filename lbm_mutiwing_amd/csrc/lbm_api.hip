@@ -189,7 +189,9 @@ struct lbm_handle {
   std::vector<Frame> frames;         // per step of the current call
   int lat_bx = 0;                    // lattice tile width (0: not tuned yet)
   int lat_waves = 0;                 // lattice waves per CU cap (0: none)
+  int lat_cp = 0;                    // cache policy of the lattice kernel's population accesses
   size_t lat_lds = 0;                // dynamic LDS per lattice workgroup that sets the cap
+  size_t dyn_lds = 0;                // the same for the cap beside a coupled step's geometry
   // lattice-kernel timing (lbm_set_timing)
   bool timing = false;
   std::vector<hipEvent_t> ev;        // start/stop pairs of the current call
@@ -294,6 +296,8 @@ void free_all(lbm_handle* h) {
   dfree(h->R.left);
   dfree(h->R.right);
   dfree(h->R.mom);
+  dfree(h->R.bottom);
+  dfree(h->R.top);
   for (void* p : h->ibm_allocs) dfree(p);
   h->ibm_allocs.clear();
   dfree(h->d_frame);
@@ -443,32 +447,62 @@ int lds_per_cu(int device) {
   return v > 0 ? v : 65536;
 }
 
+int lds_per_block(int device) {
+  int v = 0;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess) v = 0;
+  return v > 0 ? v : 65536;
+}
+
+// Lattice waves per CU beside the geometry of a step with rigid-body
+// coupling (one 256-wide workgroup per CU): the geometry workgroups on the
+// step's critical path need LDS and wave slots.
+constexpr int DYN_LATTICE_WAVES = 4;
+
 // Caps on the lattice waves per CU the autotuner tries (0: none): fewer
 // resident lattice waves leave room for the IBM and ring kernels beside it.
 constexpr int kLatticeWaves[5] = {0, 24, 16, 12, 8};
 
-// Tile width and wave cap of one instance, timed on its own buffers: the
-// launches write interior nodes of the idle population buffer, which the
-// next step overwrites entirely.  Candidates are bx * 64 + waves.
+// LBM_LATTICE_CP=<0|1|2> in the environment fixes the cache policy of the
+// lattice kernel (CP_PLAIN, CP_NT_STORE, CP_NT), -1 otherwise.
+int env_cp() {
+  if (const char* e = getenv("LBM_LATTICE_CP")) {
+    const int v = atoi(e);
+    if (v >= CP_PLAIN && v <= CP_NT) return v;
+  }
+  return -1;
+}
+
+// Tile width, wave cap and cache policy of one instance, timed on its own
+// buffers: the launches write interior nodes of the idle population buffer,
+// which the next step overwrites entirely.  Candidates are
+// cp << 16 | bx << 6 | waves.  Also the LDS of the tighter cap beside the
+// geometry of a coupled step.
 void tune_instance(lbm_handle* h) {
   if (h->lat_bx) return;
   const void* src = h->fbuf[h->cur];
   void* dst = h->fbuf[1 - h->cur];
   const AdjMask none{nullptr, nullptr, 0, 0};
-  const int fixed = env_tile(), fw = env_waves(), lds_cu = lds_per_cu(h->device);
+  const int fixed = env_tile(), fw = env_waves(), fcp = env_cp();
+  const int lds_cu = lds_per_cu(h->device), lds_blk = lds_per_block(h->device);
   std::vector<int> cands;
-  for (int bx : {256, 128, 512})
-    for (int wv : kLatticeWaves)
-      if ((!fixed || bx == fixed) && (fw < 0 || wv == fw) && lattice_lds_for_waves(bx, wv, lds_cu) >= 0)
-        cands.push_back(bx * 64 + wv);
-  if (cands.empty()) cands.push_back((fixed ? fixed : 256) * 64);
+  for (int cp : {CP_PLAIN, CP_NT_STORE, CP_NT})
+    for (int bx : {256, 128, 512})
+      for (int wv : kLatticeWaves)
+        if ((fcp < 0 || cp == fcp) && (!fixed || bx == fixed) && (fw < 0 || wv == fw) &&
+            lattice_lds_for_waves(bx, wv, lds_cu, lds_blk) >= 0)
+          cands.push_back(cp << 16 | bx << 6 | wv);
+  if (cands.empty()) cands.push_back((fcp < 0 ? 0 : fcp) << 16 | (fixed ? fixed : 256) << 6);
   const int best = autotune(h->stream, cands, [&](int c) {
-    launch_lattice(h->stream, h->g, h->mrt, src, dst, nullptr, h->f32, c / 64, none, nullptr, nullptr,
-                   (size_t)lattice_lds_for_waves(c / 64, c % 64, lds_cu));
+    const int bx = (c >> 6) & 1023, wv = c & 63;
+    launch_lattice(h->stream, h->g, h->mrt, src, dst, nullptr, h->f32, bx, c >> 16, none, nullptr, nullptr,
+                   (size_t)lattice_lds_for_waves(bx, wv, lds_cu, lds_blk));
   });
-  h->lat_bx = best / 64;
-  h->lat_waves = best % 64;
-  h->lat_lds = (size_t)lattice_lds_for_waves(h->lat_bx, h->lat_waves, lds_cu);
+  h->lat_bx = (best >> 6) & 1023;
+  h->lat_waves = best & 63;
+  h->lat_cp = best >> 16;
+  h->lat_lds = (size_t)lattice_lds_for_waves(h->lat_bx, h->lat_waves, lds_cu, lds_blk);
+  const long long dyn = lattice_lds_for_waves(h->lat_bx, DYN_LATTICE_WAVES, lds_cu, lds_blk);
+  h->dyn_lds = dyn >= 0 ? (size_t)dyn : h->lat_lds;
 }
 
 // Per-slot geometry arrays and flow buffers of the IBM.  The number of steps
@@ -587,19 +621,23 @@ int lbm_create(const lbm_config* cfg, lbm_handle** out) try {
   }
   // constants of the FP32-shifted storage: weights, rounded rates and inverse,
   // and the equilibrium moments of the rest state M w
-  for (int k = 0; k < 9; k++) {
-    h->mrt.w[k] = h->w[k];
-    h->mrt.s32[k] = (float)h->mrt.s[k];
-  }
-  for (int q = 0; q < 81; q++) h->mrt.minv32[q] = (float)h->mrt.minv[q];
   {
-    double mw[9];
+    static const double kC[9] = {1.0 / 9, 1.0 / 36, 1.0 / 36, 1.0 / 6, 1.0 / 12, 1.0 / 6, 1.0 / 12, 0.25, 0.25};
+    double mw[9], ws = 0.0;
+    for (int k = 0; k < 9; k++) {
+      h->mrt.w[k] = h->w[k];
+      ws += h->w[k];
+      h->mrt.ra32[k] = (float)(kC[k] * (1.0 - h->mrt.s[k]));
+      h->mrt.rb32[k] = (float)(kC[k] * h->mrt.s[k]);
+    }
     for (int r = 0; r < 9; r++) {
       mw[r] = 0.0;
       for (int c = 0; c < 9; c++) mw[r] += kMatM[r][c] * h->w[c];
     }
-    h->mrt.eq1 = (float)(-2.0 - mw[1]);
-    h->mrt.eq2 = (float)(1.0 - mw[2]);
+    h->mrt.wsum = ws;
+    h->mrt.rk0 = (float)(-kC[0] * h->mrt.s[0] * ws);
+    h->mrt.eq1 = (float)(-2.0 * ws - mw[1]);
+    h->mrt.eq2 = (float)(ws - mw[2]);
   }
   h->g.nx = cfg->nx;
   h->g.ny = cfg->ny;
@@ -624,6 +662,12 @@ int lbm_create(const lbm_config* cfg, lbm_handle** out) try {
   if (h->W > 0 && geo_solve_lds_bytes(ip) > (size_t)lds_max) {
     int rc = fail(nullptr, LBM_E_INVALID, "%d segments per wing need %zu B of LDS, device allows %d", h->SPW,
                   geo_solve_lds_bytes(ip), lds_max);
+    delete h;
+    return rc;
+  }
+  if (h->W > 1 && geo_couple_lds_total() > (size_t)lds_max) {
+    int rc = fail(nullptr, LBM_E_INVALID, "the coupling of wings needs %zu B of LDS per workgroup, device allows %d",
+                  geo_couple_lds_total(), lds_max);
     delete h;
     return rc;
   }
@@ -685,6 +729,8 @@ int lbm_create(const lbm_config* cfg, lbm_handle** out) try {
   ALLOC(h->R.left, 9 * (size_t)cfg->ny);
   ALLOC(h->R.right, 9 * (size_t)cfg->ny);
   ALLOC(h->R.mom, 6 * (size_t)cfg->ny);
+  ALLOC(h->R.bottom, 9 * (size_t)cfg->nx);
+  ALLOC(h->R.top, 9 * (size_t)cfg->nx);
   ALLOC(h->d_stream, 2 * LBM_MAX_MOTION_COLS);
   ALLOC(h->d_st, 1);
   ALLOC(h->d_st_carry, 1);
@@ -856,6 +902,10 @@ int apply_move(lbm_handle* h, int dr, int s, int m) {
   h->cur = 1 - h->cur;
   launch_copy_ring_column(h->stream, h->g, h->mrt, h->fbuf[h->cur], dr == 1 ? 0 : h->g.nx - 1,
                           dr == 1 ? h->R.left : h->R.right, h->f32);
+  // f0 = the shifted f in every column but the kept one, the wall rows included
+  const int keep = dr == 1 ? h->g.nx - 1 : 0;
+  launch_copy_ring_row(h->stream, h->g, h->mrt, h->fbuf[h->cur], 0, keep, h->R.bottom, h->f32);
+  launch_copy_ring_row(h->stream, h->g, h->mrt, h->fbuf[h->cur], h->g.ny - 1, keep, h->R.top, h->f32);
   launch_ring_moments(h->stream, h->g, h->R);
   FieldSlot& cs = h->slot[s];
   launch_shift_x(h->stream, h->g, dr, cs.rho, h->spare.rho, 1, false);
@@ -878,8 +928,6 @@ int step_error(lbm_handle* h, const DevStep& st) {
 
 int finish_advance(lbm_handle* h, int nsteps, int n_last, bool timed, double* force_rows, double* motion_rows,
                    double* row_sum, double* last_row, double* err);
-
-constexpr int DYN_LATTICE_WAVES = 4;
 
 // Advance of an instance with rigid-body coupling, one step at a time: the
 // geometry of step m needs the body state left by the force of step m-1,
@@ -929,9 +977,8 @@ int advance_dynamic(lbm_handle* h, int32_t n_last, double* force_rows, double* m
     every_node.W = 0;
     // a tighter wave cap than the tuned one: the step's geometry beside the
     // lattice kernel is the critical path here
-    const int dyn_lds = lattice_lds_for_waves(h->lat_bx, DYN_LATTICE_WAVES, lds_per_cu(h->device));
-    launch_lattice(h->stream, h->g, h->mrt, src, dst, fs, h->f32, h->lat_bx, every_node, nullptr, nullptr,
-                   dyn_lds >= 0 ? (size_t)dyn_lds : h->lat_lds);
+    launch_lattice(h->stream, h->g, h->mrt, src, dst, fs, h->f32, h->lat_bx, h->lat_cp, every_node, nullptr, nullptr,
+                   h->dyn_lds);
     HIPCHECK(h, hipGetLastError());
     HIPCHECK(h, hipEventRecord(h->ev_step, h->stream));
     launch_bc_ring(h->stream_bc, h->g, h->mrt, h->R, src, dst, fs, h->f32);
@@ -1125,7 +1172,7 @@ int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motio
     HIPCHECK(h, hipStreamWaitEvent(h->stream_ibm, step_ev, 0));
     HIPCHECK(h, hipStreamWaitEvent(h->stream_bc, step_ev, 0));
     step_ev = h->timing ? h->ev[2 * k + 1] : h->ev_step;
-    launch_lattice(h->stream, h->g, h->mrt, src, dst, fs, h->f32, h->lat_bx, adj_mask(h->ip, h->G, slot),
+    launch_lattice(h->stream, h->g, h->mrt, src, dst, fs, h->f32, h->lat_bx, h->lat_cp, adj_mask(h->ip, h->G, slot),
                    h->timing ? h->ev[2 * k] : nullptr, step_ev, h->lat_lds);
     HIPCHECK(h, hipGetLastError());
     launch_bc_ring(h->stream_bc, h->g, h->mrt, h->R, src, dst, fs, h->f32);
@@ -1332,14 +1379,22 @@ int lbm_get_populations(lbm_handle* h, double* f, double* f0) try {
   }
   if (f0) {
     std::memset(f0, 0, 9 * np * sizeof(double));
-    std::vector<double> L(9 * (size_t)h->g.ny), Rr(9 * (size_t)h->g.ny);
+    const size_t nx = h->g.nx, ny = h->g.ny;
+    std::vector<double> L(9 * ny), Rr(9 * ny), B(9 * nx), T(9 * nx);
     HIPCHECK(h, hipMemcpy(L.data(), h->R.left, L.size() * sizeof(double), hipMemcpyDeviceToHost));
     HIPCHECK(h, hipMemcpy(Rr.data(), h->R.right, Rr.size() * sizeof(double), hipMemcpyDeviceToHost));
-    for (int k = 0; k < 9; k++)
-      for (int j = 0; j < h->g.ny; j++) {
-        f0[k * np + (size_t)j * h->g.nx] = L[(size_t)k * h->g.ny + j];
-        f0[k * np + (size_t)j * h->g.nx + h->g.nx - 1] = Rr[(size_t)k * h->g.ny + j];
+    HIPCHECK(h, hipMemcpy(B.data(), h->R.bottom, B.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHECK(h, hipMemcpy(T.data(), h->R.top, T.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int k = 0; k < 9; k++) {
+      for (size_t i = 0; i < nx; i++) {
+        f0[k * np + i] = B[k * nx + i];
+        f0[k * np + (ny - 1) * nx + i] = T[k * nx + i];
       }
+      for (size_t j = 0; j < ny; j++) {
+        f0[k * np + j * nx] = L[k * ny + j];
+        f0[k * np + j * nx + nx - 1] = Rr[k * ny + j];
+      }
+    }
   }
   return 0;
 } LBM_CATCH(h, "lbm_get_populations")
@@ -1365,14 +1420,22 @@ int lbm_set_populations(lbm_handle* h, const double* f, const double* f0) try {
     h->slot_step[s] = m;
   }
   if (f0) {
-    std::vector<double> L(9 * (size_t)h->g.ny), Rr(9 * (size_t)h->g.ny);
-    for (int k = 0; k < 9; k++)
-      for (int j = 0; j < h->g.ny; j++) {
-        L[(size_t)k * h->g.ny + j] = f0[k * np + (size_t)j * h->g.nx];
-        Rr[(size_t)k * h->g.ny + j] = f0[k * np + (size_t)j * h->g.nx + h->g.nx - 1];
+    const size_t nx = h->g.nx, ny = h->g.ny;
+    std::vector<double> L(9 * ny), Rr(9 * ny), B(9 * nx), T(9 * nx);
+    for (int k = 0; k < 9; k++) {
+      for (size_t j = 0; j < ny; j++) {
+        L[k * ny + j] = f0[k * np + j * nx];
+        Rr[k * ny + j] = f0[k * np + j * nx + nx - 1];
       }
+      for (size_t i = 0; i < nx; i++) {
+        B[k * nx + i] = f0[k * np + i];
+        T[k * nx + i] = f0[k * np + (ny - 1) * nx + i];
+      }
+    }
     HIPCHECK(h, hipMemcpy(h->R.left, L.data(), L.size() * sizeof(double), hipMemcpyHostToDevice));
     HIPCHECK(h, hipMemcpy(h->R.right, Rr.data(), Rr.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHECK(h, hipMemcpy(h->R.bottom, B.data(), B.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHECK(h, hipMemcpy(h->R.top, T.data(), T.size() * sizeof(double), hipMemcpyHostToDevice));
     launch_ring_moments(h->stream, h->g, h->R);
   }
   HIPCHECK(h, hipStreamSynchronize(h->stream));
@@ -1460,6 +1523,12 @@ int lbm_get_tuning(const lbm_handle* h, int32_t* lattice_tile, int32_t* waves_pe
   if (!h) return LBM_E_INVALID;
   if (lattice_tile) *lattice_tile = h->lat_bx;
   if (waves_per_cu) *waves_per_cu = h->lat_waves;
+  return 0;
+}
+
+int lbm_get_lattice_policy(const lbm_handle* h, int32_t* cache_policy) {
+  if (!h || !cache_policy) return LBM_E_INVALID;
+  *cache_policy = h->lat_cp;
   return 0;
 }
 
@@ -1559,6 +1628,7 @@ struct lbm_batch {
   int lat_bx = 0;                    // lattice tile width (tuned on the first call)
   int lat_rows = 1;                  // lattice tiles per work item (tuned with the width)
   int lat_waves = 0;                 // lattice waves per CU cap (0: none), tuned last
+  int lat_cp = 0;                    // cache policy of the lattice kernel, tuned with the cap
   size_t lat_lds = 0;
   char msg[512] = "";
 };
@@ -1658,6 +1728,12 @@ int lbm_batch_get_tuning(const lbm_batch* b, int32_t* lattice_tile, int32_t* row
   if (lattice_tile) *lattice_tile = b->lat_bx;
   if (rows_per_item) *rows_per_item = b->lat_rows;
   if (waves_per_cu) *waves_per_cu = b->lat_waves;
+  return 0;
+}
+
+int lbm_batch_get_lattice_policy(const lbm_batch* b, int32_t* cache_policy) {
+  if (!b || !cache_policy) return LBM_E_INVALID;
+  *cache_policy = b->lat_cp;
   return 0;
 }
 
@@ -1789,32 +1865,36 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
       lt.kbase = b->d_kbase;
       lt.lattice = b->d_items;
       lt.n_lattice = (int)items.size();
-      launch_batch_lattice(b->stream, b->d_desc, lt, 0, f32, false, bx, rows, false);
+      launch_batch_lattice(b->stream, b->d_desc, lt, 0, f32, false, bx, rows, CP_PLAIN, false);
     });
     if (rc) return rc;
     b->lat_bx = best / 8;
     b->lat_rows = best % 8;
-    // then the wave cap for that shape
-    const int fw = env_waves(), lds_cu = lds_per_cu(b->device);
+    // then the wave cap and cache policy for that shape (cp * 64 + waves)
+    const int fw = env_waves(), fcp = env_cp(), lds_cu = lds_per_cu(b->device), lds_blk = lds_per_block(b->device);
     std::vector<int> wc;
-    for (int wv : kLatticeWaves)
-      if ((fw < 0 || wv == fw) && lattice_lds_for_waves(b->lat_bx, wv, lds_cu) >= 0) wc.push_back(wv);
-    if (wc.empty()) wc.push_back(0);
+    for (int cp : {CP_PLAIN, CP_NT})
+      for (int wv : kLatticeWaves)
+        if ((fcp < 0 || cp == fcp) && (fw < 0 || wv == fw) && lattice_lds_for_waves(b->lat_bx, wv, lds_cu, lds_blk) >= 0)
+          wc.push_back(cp * 64 + wv);
+    if (wc.empty()) wc.push_back(fcp == CP_NT ? CP_NT * 64 : 0);
     items.clear();
     for (int e = 0; e < count; e++) {
       desc[e].tiles_x = lattice_tiles_x(desc[e].g, b->lat_bx);
       add(e, (desc[e].tiles_x * (desc[e].g.ny - 2) + b->lat_rows - 1) / b->lat_rows);
     }
     if ((rc = upload())) return rc;
-    b->lat_waves = autotune(b->stream, wc, [&](int wv) {
+    const int bw = autotune(b->stream, wc, [&](int c) {
       BatchItems lt{};
       lt.kbase = b->d_kbase;
       lt.lattice = b->d_items;
       lt.n_lattice = (int)items.size();
-      launch_batch_lattice(b->stream, b->d_desc, lt, 0, f32, false, b->lat_bx, b->lat_rows, false, nullptr,
-                           (size_t)lattice_lds_for_waves(b->lat_bx, wv, lds_cu));
+      launch_batch_lattice(b->stream, b->d_desc, lt, 0, f32, false, b->lat_bx, b->lat_rows, c / 64, false, nullptr,
+                           (size_t)lattice_lds_for_waves(b->lat_bx, c % 64, lds_cu, lds_blk));
     });
-    b->lat_lds = (size_t)lattice_lds_for_waves(b->lat_bx, b->lat_waves, lds_cu);
+    b->lat_waves = bw % 64;
+    b->lat_cp = bw / 64;
+    b->lat_lds = (size_t)lattice_lds_for_waves(b->lat_bx, b->lat_waves, lds_cu, lds_blk);
     items.clear();
   }
   for (int e = 0; e < count; e++) desc[e].tiles_x = lattice_tiles_x(desc[e].g, b->lat_bx);
@@ -1910,7 +1990,7 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
       launch_batch_adj_collide(b->stream_ibm, b->d_desc, it, kk, f32, fields);
       BCHECK(b, hipStreamWaitEvent(b->stream_ibm, b->ev_bc, 0));
       BCHECK(b, hipEventRecord(b->ev_ibm, b->stream_ibm));
-      launch_batch_lattice(b->stream, b->d_desc, it, kk, f32, fields, b->lat_bx, b->lat_rows, true,
+      launch_batch_lattice(b->stream, b->d_desc, it, kk, f32, fields, b->lat_bx, b->lat_rows, b->lat_cp, true,
                            bind ? b->ev_step : nullptr, b->lat_lds);
       BCHECK(b, hipGetLastError());
       BCHECK(b, hipStreamWaitEvent(b->stream, b->ev_ibm, 0));
@@ -1935,8 +2015,9 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
   // The key: the composition, the device lists and, for a whole advance,
   // its length and the last geometry list (a period graph uses the full
   // list only).
-  std::vector<long long> key = {count, ns, f32, wings, b->lat_bx, b->lat_rows, b->lat_waves, (long long)(size_t)b->d_desc,
-                                (long long)(size_t)b->d_items, (long long)lds_wing, (long long)lds_solve};
+  std::vector<long long> key = {count, ns, f32, wings, b->lat_bx, b->lat_rows, b->lat_waves, b->lat_cp,
+                                (long long)(size_t)b->d_desc, (long long)(size_t)b->d_items, (long long)lds_wing,
+                                (long long)lds_solve};
   for (int q = 0; q < 7; q++) key.push_back((long long)off[q]);
   for (int e = 0; e < count; e++) key.push_back((long long)(size_t)hs[e]);
   auto find_graph = [&](long long tag) -> BatchGraph* {

@@ -738,6 +738,15 @@ __device__ int block_scan(int* s, int v) {
 // Dynamic LDS of the coupling pass: G, then [I + O G | O] as two K x K blocks.
 size_t geo_couple_lds_bytes() { return (size_t)3 * CPL_KT * CPL_KT * sizeof(double); }
 
+__global__ __launch_bounds__(CPL_THREADS) void k_geo_couple(IbmParams p, GeoBuffers G, DevStep* st, int slot0);
+
+// Static plus dynamic LDS of a coupling workgroup (0 if the runtime cannot say).
+size_t geo_couple_lds_total() {
+  hipFuncAttributes a{};
+  if (hipFuncGetAttributes(&a, reinterpret_cast<const void*>(&k_geo_couple)) != hipSuccess) return 0;
+  return a.sharedSizeBytes + geo_couple_lds_bytes();
+}
+
 // One workgroup per step, after the per-wing solves of the step.
 __device__ __forceinline__ void geo_couple_body(const IbmParams& p, const GeoBuffers& G, DevStep* st, int slot) {
   extern __shared__ double lds_c[];

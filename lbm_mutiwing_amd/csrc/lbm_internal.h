@@ -176,6 +176,11 @@ struct RingBuffers {
   double* left;     // f0 ring column i=0, 9 * ny
   double* right;    // f0 ring column i=nx-1, 9 * ny
   double* mom;      // rho,u,v of left column then right column: 6 * ny
+  // f0 ring rows j=0 and j=ny-1, 9 * nx each: no step reads them (the wall
+  // rows' moments are overwritten by the wall BC), but move_mesh shifts them
+  // (move_mesh.f90:30-34) and the checkpoint saves them (solution.f90:1673-1684)
+  double* bottom;
+  double* top;
 };
 
 struct FieldSlot {
@@ -240,16 +245,18 @@ constexpr int kLatticeTiles[3] = {128, 256, 512};
 // start and at its end (no separate event packets in the stream); lds:
 // dynamic LDS per workgroup, which the kernel does not use: it caps the
 // workgroups per CU (lattice_lds_for_waves)
+// cp: cache policy of the population accesses (CP_PLAIN, CP_NT_STORE, CP_NT)
 void launch_lattice(hipStream_t s, const Grid& g, const MrtParams& P, const void* src, void* dst, FieldSlot* fields,
-                    bool f32, int bx, const AdjMask& mask, hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr,
+                    bool f32, int bx, int cp, const AdjMask& mask, hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr,
                     size_t lds = 0);
 // Dynamic LDS that lets at most `waves` waves of bx-thread workgroups share a
 // CU of lds_cu bytes of LDS: 0 for waves = 0 (no cap), -1 when no size does.
-inline long long lattice_lds_for_waves(int bx, int waves, int lds_cu) {
+// No size does when it would exceed lds_blk, the device's LDS per workgroup.
+inline long long lattice_lds_for_waves(int bx, int waves, int lds_cu, int lds_blk) {
   if (waves <= 0) return 0;
   const long long wg = waves / (bx / 64) > 0 ? waves / (bx / 64) : 1;
   const long long bytes = ((lds_cu / (wg + 1) + 1) + 511) / 512 * 512;
-  return wg * bytes <= lds_cu ? bytes : -1;
+  return wg * bytes <= lds_cu && bytes <= lds_blk ? bytes : -1;
 }
 int lattice_tiles_x(const Grid& g, int bx);
 void launch_adj_collide(hipStream_t s, const Grid& g, const MrtParams& P, const IbmParams& p, const GeoBuffers& G,
@@ -280,6 +287,9 @@ int measure_pull_copy(int nx, int ny, bool f32, int launches, double* avg_us);
 void launch_shift_x(hipStream_t s, const Grid& g, int dr, const void* f, void* fshift, int planes, bool f32);
 void launch_copy_ring_column(hipStream_t s, const Grid& g, const MrtParams& P, const void* f, int col, double* ring,
                              bool f32);
+// f0 ring row `row` (0 or ny-1) from the populations f, every column but `keep`
+void launch_copy_ring_row(hipStream_t s, const Grid& g, const MrtParams& P, const void* f, int row, int keep,
+                          double* ring, bool f32);
 int adj_collide_blocks_per_wing(const IbmParams& p);
 
 // IBM geometry kernels (lbm_ibm.hip), one launch covers `count` consecutive
@@ -289,6 +299,7 @@ int adj_collide_blocks_per_wing(const IbmParams& p);
 void launch_geometry(hipStream_t s, const IbmParams& p, const GeoBuffers& G, DevStep* st, int slot0, int n0,
                      int fidx0, int count, bool with_inverse, bool direct = false);
 size_t geo_solve_lds_bytes(const IbmParams& p);
+size_t geo_couple_lds_total();   // static + dynamic LDS of the coupling workgroup
 // Batched launches over the episodes of eps (device array).  The substep of a
 // launch is *it.kbase + k (the base lives on the device, so that a captured
 // graph of one geometry batch replays for every batch of a long advance).
@@ -296,7 +307,7 @@ size_t geo_solve_lds_bytes(const IbmParams& p);
 // rows: tiles per work item (1, 2 or 4; the items were built for it);
 // t1 (optional): an event bound to the launch, completing at its end
 void launch_batch_lattice(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields,
-                          int bx, int rows, bool masked, hipEvent_t t1 = nullptr, size_t lds = 0);
+                          int bx, int rows, int cp, bool masked, hipEvent_t t1 = nullptr, size_t lds = 0);
 void launch_batch_bc_ring(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields);
 void launch_batch_adj_collide(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields);
 void launch_batch_ibm_flow(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32);

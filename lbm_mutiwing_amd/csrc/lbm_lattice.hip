@@ -22,13 +22,29 @@
 
 namespace lbm {
 
+// Cache policy of the population loads and stores (CP): each population of
+// a step is read once and written once, so the accesses may be marked
+// non-temporal.  CP_PLAIN: ordinary accesses; CP_NT_STORE: non-temporal
+// stores; CP_NT: non-temporal loads and stores.  Which is fastest depends on
+// the lattice and the wave cap, so the autotuner times them.
+template <int CP, typename T>
+__device__ __forceinline__ T ld_pop(const T* __restrict__ p) {
+  if constexpr (CP == CP_NT) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+template <int CP, typename T>
+__device__ __forceinline__ void st_pop(T* __restrict__ p, T v) {
+  if constexpr (CP == CP_NT_STORE || CP == CP_NT) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+
 // One thread per node of a row tile of BX nodes; tiles start at i = 0 so
 // that every wavefront stores whole, aligned spans of each population plane
 // (the ring columns i = 0 and i = nx-1 are skipped and left to k_bc_ring).
 // Workgroups follow the hardware's round-robin placement over the XCDs, which
 // measured faster than remapping tiles to XCD-contiguous runs.  The tile
 // width is chosen per instance (or per batch) by timing (lattice_autotune).
-template <bool FIELDS, int BX>
+template <bool FIELDS, int BX, int CP>
 __device__ __forceinline__ void lattice_body(const Grid& g, const MrtParams& P, const double* __restrict__ src,
                                              double* __restrict__ dst, const FieldSlot& fs, int tiles_x, int blk,
                                              const AdjMask& am) {
@@ -37,14 +53,15 @@ __device__ __forceinline__ void lattice_body(const Grid& g, const MrtParams& P, 
   const int j = 1 + ty;
   if (i < 1 || i > g.nx - 2 || is_adjoint(am, i, j)) return;
   double f[9];
-  pull_node(src, g, i, j, f);
+#pragma unroll
+  for (int k = 0; k < 9; k++) f[k] = ld_pop<CP>(src + (long long)k * g.ps + (long long)(j - EY(k)) * g.nx + (i - EX(k)));
   double rho, u, v;
   node_moments(f, rho, u, v);
   double out[9];
   collide_node(P, f, rho, u, v, 0.0, 0.0, out);
   const long long o = (long long)j * g.nx + i;
 #pragma unroll
-  for (int k = 0; k < 9; k++) dst[(long long)k * g.ps + o] = out[k];
+  for (int k = 0; k < 9; k++) st_pop<CP>(dst + (long long)k * g.ps + o, out[k]);
   if (FIELDS) {
     if (fs.rho) fs.rho[o] = rho;   // null: velocities only (erranalysis)
     fs.u[o] = u;
@@ -52,17 +69,24 @@ __device__ __forceinline__ void lattice_body(const Grid& g, const MrtParams& P, 
   }
 }
 
-template <bool FIELDS, int BX>
+template <bool FIELDS, int BX, int CP>
 __global__ __launch_bounds__(BX) void k_lattice(Grid g, MrtParams P, const double* __restrict__ src,
                                                 double* __restrict__ dst, FieldSlot fs, int tiles_x, AdjMask am) {
-  lattice_body<FIELDS, BX>(g, P, src, dst, fs, tiles_x, blockIdx.x, am);
+  lattice_body<FIELDS, BX, CP>(g, P, src, dst, fs, tiles_x, blockIdx.x, am);
 }
 
-// FP32 bulk step on deviations d = f - w: rho = 1 + sum d, j = sum e d (the
-// weights carry no momentum), m' = M d, and the equilibrium moments shifted
-// by M w.  Relaxation and the inverse transform are linear, so d* = M^-1 m*'
-// is the deviation of the post-collision populations.
-template <bool FIELDS, int BX>
+// FP32 bulk step on deviations d = f - w from the rest equilibrium, in FP32.
+// The moments m' = M d come from shared partial sums of the nine loads
+// (M of solution.f90:17-25); the weights carry no momentum, so j = sum e d,
+// and with the incompressible equilibrium (solution.f90:613-624) rho u = j,
+// so the equilibrium moments need no division: e_eq = -2 rho + 3 j^2,
+// eps_eq = rho - 3 j^2, q_eq = -j, p_eq = (jx^2 - jy^2, jx jy), each shifted
+// by (M w) (rho = sum w + sum d; MrtParams eq1, eq2).  Relaxation and the
+// inverse transform are linear: with M^-1 = M^T diag(c), c = (1/9, 1/36,
+// 1/36, 1/6, 1/12, 1/6, 1/12, 1/4, 1/4), the scaled relaxed moments are
+// tt_r = c_r (1 - s_r) m_r + c_r s_r m_eq,r (MrtParams ra32, rb32, rk0) and
+// d* = M^T tt, again from shared sums.  About 90 FP32 operations per node.
+template <bool FIELDS, int BX, int CP>
 __device__ __forceinline__ void lattice_f32_body(const Grid& g, const MrtParams& P, const float* __restrict__ src,
                                                  float* __restrict__ dst, const FieldSlot& fs, int tiles_x, int blk,
                                                  const AdjMask& am) {
@@ -72,75 +96,71 @@ __device__ __forceinline__ void lattice_f32_body(const Grid& g, const MrtParams&
   if (i < 1 || i > g.nx - 2 || is_adjoint(am, i, j)) return;
   float d[9];
 #pragma unroll
-  for (int k = 0; k < 9; k++) d[k] = src[(long long)k * g.ps + (long long)(j - EY(k)) * g.nx + (i - EX(k))];
-  const float rp = d[0] + d[1] + d[2] + d[3] + d[4] + d[5] + d[6] + d[7] + d[8];
-  const float jx = d[1] - d[3] + d[5] - d[6] - d[7] + d[8];
-  const float jy = d[2] - d[4] + d[5] + d[6] - d[7] - d[8];
-  const float rho = 1.0f + rp;
-  const float u = jx / rho, v = jy / rho;
-  float m[9], t[9];
-  m[0] = rp;
-  m[1] = -4.0f * d[0] - d[1] - d[2] - d[3] - d[4] + 2.0f * (d[5] + d[6] + d[7] + d[8]);
-  m[2] = 4.0f * d[0] - 2.0f * (d[1] + d[2] + d[3] + d[4]) + d[5] + d[6] + d[7] + d[8];
-  m[3] = jx;
-  m[4] = -2.0f * d[1] + 2.0f * d[3] + d[5] - d[6] - d[7] + d[8];
-  m[5] = jy;
-  m[6] = -2.0f * d[2] + 2.0f * d[4] + d[5] + d[6] - d[7] - d[8];
-  m[7] = d[1] - d[2] + d[3] - d[4];
-  m[8] = d[5] - d[6] + d[7] - d[8];
-  const float ru = rho * u, rv = rho * v;
-  const float e2 = ru * ru + rv * rv;
-  t[0] = m[0];
-  t[1] = -2.0f * rp + 3.0f * e2 + P.eq1;
-  t[2] = rp - 3.0f * e2 + P.eq2;
-  t[3] = m[3];
-  t[4] = -ru;
-  t[5] = m[5];
-  t[6] = -rv;
-  t[7] = ru * ru - rv * rv;
-  t[8] = ru * rv;
-#pragma unroll
-  for (int k = 0; k < 9; k++) t[k] = m[k] + (t[k] - m[k]) * P.s32[k];
-  const float* mi = P.minv32;
+  for (int k = 0; k < 9; k++) d[k] = ld_pop<CP>(src + (long long)k * g.ps + (long long)(j - EY(k)) * g.nx + (i - EX(k)));
+  // moments of d
+  const float h13 = d[1] + d[3], h24 = d[2] + d[4];     // axis pairs
+  const float h57 = d[5] + d[7], h68 = d[6] + d[8];     // diagonal pairs
+  const float s14 = h13 + h24, s58 = h57 + h68;
+  const float rp = d[0] + s14 + s58;                    // m0 = rho - sum w
+  const float a13 = d[1] - d[3], a24 = d[2] - d[4];
+  const float x58 = (d[5] + d[8]) - (d[6] + d[7]);
+  const float y58 = (d[5] + d[6]) - (d[7] + d[8]);
+  const float jx = a13 + x58, jy = a24 + y58;
+  const float m1 = (s58 + s58) - s14 - 4.0f * d[0];
+  const float m2 = 4.0f * d[0] - (s14 + s14) + s58;
+  const float m4 = x58 - (a13 + a13);
+  const float m6 = y58 - (a24 + a24);
+  const float m7 = h13 - h24;
+  const float m8 = h57 - h68;
+  // relaxed moments, scaled by c
+  const float xx = jx * jx, yy = jy * jy, jj = xx + yy;
+  const float* A = P.ra32;
+  const float* B = P.rb32;
+  const float t0 = fmaf(A[0], rp, P.rk0);
+  const float t1 = fmaf(A[1], m1, B[1] * (3.0f * jj - (rp + rp) + P.eq1));
+  const float t2 = fmaf(A[2], m2, B[2] * (rp - 3.0f * jj + P.eq2));
+  const float t3 = A[3] * jx;
+  const float t4 = fmaf(A[4], m4, -B[4] * jx);
+  const float t5 = A[5] * jy;
+  const float t6 = fmaf(A[6], m6, -B[6] * jy);
+  const float t7 = fmaf(A[7], m7, B[7] * (xx - yy));
+  const float t8 = fmaf(A[8], m8, B[8] * (jx * jy));
+  // d* = M^T tt
+  const float pa = t0 - t1 - (t2 + t2);                 // axis directions
+  const float qd = t0 + (t1 + t1) + t2;                 // diagonals
+  const float ex = t3 - (t4 + t4), ey = t5 - (t6 + t6);
+  const float dx = t3 + t4, dy = t5 + t6;
   float out[9];
-  out[0] = mi[0] * t[0] + mi[1] * t[1] + mi[2] * t[2];
-#pragma unroll
-  for (int r = 1; r <= 3; r += 2) {
-    const float* a = mi + 9 * r;
-    out[r] = a[0] * t[0] + a[1] * t[1] + a[2] * t[2] + a[3] * t[3] + a[4] * t[4] + a[7] * t[7];
-  }
-#pragma unroll
-  for (int r = 2; r <= 4; r += 2) {
-    const float* a = mi + 9 * r;
-    out[r] = a[0] * t[0] + a[1] * t[1] + a[2] * t[2] + a[5] * t[5] + a[6] * t[6] + a[7] * t[7];
-  }
-#pragma unroll
-  for (int r = 5; r <= 8; r++) {
-    const float* a = mi + 9 * r;
-    out[r] = a[0] * t[0] + a[1] * t[1] + a[2] * t[2] + a[3] * t[3] + a[4] * t[4] + a[5] * t[5] + a[6] * t[6] +
-             a[8] * t[8];
-  }
+  out[0] = t0 + 4.0f * (t2 - t1);
+  out[1] = pa + ex + t7;
+  out[3] = pa - ex + t7;
+  out[2] = pa + ey - t7;
+  out[4] = pa - ey - t7;
+  out[5] = qd + (dx + dy) + t8;
+  out[7] = qd - (dx + dy) + t8;
+  out[6] = qd + (dy - dx) - t8;
+  out[8] = qd - (dy - dx) - t8;
   const long long o = (long long)j * g.nx + i;
 #pragma unroll
-  for (int k = 0; k < 9; k++) dst[(long long)k * g.ps + o] = out[k];
+  for (int k = 0; k < 9; k++) st_pop<CP>(dst + (long long)k * g.ps + o, out[k]);
   if (FIELDS) {
-    const double rd = 1.0 + (double)rp;
+    const double rd = P.wsum + (double)rp;
     if (fs.rho) fs.rho[o] = rd;
     fs.u[o] = (double)jx / rd;
     fs.v[o] = (double)jy / rd;
   }
 }
 
-template <bool FIELDS, int BX>
+template <bool FIELDS, int BX, int CP>
 __global__ __launch_bounds__(BX) void k_lattice_f32(Grid g, MrtParams P, const float* __restrict__ src,
                                                     float* __restrict__ dst, FieldSlot fs, int tiles_x, AdjMask am) {
-  lattice_f32_body<FIELDS, BX>(g, P, src, dst, fs, tiles_x, blockIdx.x, am);
+  lattice_f32_body<FIELDS, BX, CP>(g, P, src, dst, fs, tiles_x, blockIdx.x, am);
 }
 
 int lattice_tiles_x(const Grid& g, int bx) { return (g.nx + bx - 1) / bx; }
 
-template <int BX>
-void launch_lattice_bx(hipStream_t s, const Grid& g, const MrtParams& P, const void* src, void* dst,
+template <int BX, int CP>
+void launch_lattice_cp(hipStream_t s, const Grid& g, const MrtParams& P, const void* src, void* dst,
                        const FieldSlot* fields, bool f32, const AdjMask& am, hipEvent_t t0, hipEvent_t t1,
                        size_t lds) {
   const int tiles_x = lattice_tiles_x(g, BX);
@@ -150,27 +170,40 @@ void launch_lattice_bx(hipStream_t s, const Grid& g, const MrtParams& P, const v
     const float* a = (const float*)src;
     float* b = (float*)dst;
     if (fields)
-      hipExtLaunchKernelGGL((k_lattice_f32<true, BX>), grid, block, lds, s, t0, t1, 0, g, P, a, b, fs, tiles_x, am);
+      hipExtLaunchKernelGGL((k_lattice_f32<true, BX, CP>), grid, block, lds, s, t0, t1, 0, g, P, a, b, fs, tiles_x, am);
     else
-      hipExtLaunchKernelGGL((k_lattice_f32<false, BX>), grid, block, lds, s, t0, t1, 0, g, P, a, b, fs, tiles_x, am);
+      hipExtLaunchKernelGGL((k_lattice_f32<false, BX, CP>), grid, block, lds, s, t0, t1, 0, g, P, a, b, fs, tiles_x,
+                            am);
   } else {
     const double* a = (const double*)src;
     double* b = (double*)dst;
     if (fields)
-      hipExtLaunchKernelGGL((k_lattice<true, BX>), grid, block, lds, s, t0, t1, 0, g, P, a, b, fs, tiles_x, am);
+      hipExtLaunchKernelGGL((k_lattice<true, BX, CP>), grid, block, lds, s, t0, t1, 0, g, P, a, b, fs, tiles_x, am);
     else
-      hipExtLaunchKernelGGL((k_lattice<false, BX>), grid, block, lds, s, t0, t1, 0, g, P, a, b, fs, tiles_x, am);
+      hipExtLaunchKernelGGL((k_lattice<false, BX, CP>), grid, block, lds, s, t0, t1, 0, g, P, a, b, fs, tiles_x, am);
   }
 }
 
-void launch_lattice(hipStream_t s, const Grid& g, const MrtParams& P, const void* src, void* dst, FieldSlot* fields,
-                    bool f32, int bx, const AdjMask& mask, hipEvent_t t0, hipEvent_t t1, size_t lds) {
-  if (bx == 128)
-    launch_lattice_bx<128>(s, g, P, src, dst, fields, f32, mask, t0, t1, lds);
-  else if (bx == 512)
-    launch_lattice_bx<512>(s, g, P, src, dst, fields, f32, mask, t0, t1, lds);
+template <int BX>
+void launch_lattice_bx(hipStream_t s, const Grid& g, const MrtParams& P, const void* src, void* dst,
+                       const FieldSlot* fields, bool f32, int cp, const AdjMask& am, hipEvent_t t0, hipEvent_t t1,
+                       size_t lds) {
+  if (cp == CP_NT)
+    launch_lattice_cp<BX, CP_NT>(s, g, P, src, dst, fields, f32, am, t0, t1, lds);
+  else if (cp == CP_NT_STORE)
+    launch_lattice_cp<BX, CP_NT_STORE>(s, g, P, src, dst, fields, f32, am, t0, t1, lds);
   else
-    launch_lattice_bx<256>(s, g, P, src, dst, fields, f32, mask, t0, t1, lds);
+    launch_lattice_cp<BX, CP_PLAIN>(s, g, P, src, dst, fields, f32, am, t0, t1, lds);
+}
+
+void launch_lattice(hipStream_t s, const Grid& g, const MrtParams& P, const void* src, void* dst, FieldSlot* fields,
+                    bool f32, int bx, int cp, const AdjMask& mask, hipEvent_t t0, hipEvent_t t1, size_t lds) {
+  if (bx == 128)
+    launch_lattice_bx<128>(s, g, P, src, dst, fields, f32, cp, mask, t0, t1, lds);
+  else if (bx == 512)
+    launch_lattice_bx<512>(s, g, P, src, dst, fields, f32, cp, mask, t0, t1, lds);
+  else
+    launch_lattice_bx<256>(s, g, P, src, dst, fields, f32, cp, mask, t0, t1, lds);
 }
 
 // Re-collide the adjoint nodes with the IBM-corrected velocity u = u* + g and
@@ -635,6 +668,13 @@ __global__ void k_init_fields(Grid g, MrtParams P, double uu, Feq9 feq, S* f, Ri
         R.right[(long long)k * g.ny + q] = feq.v[k];
       }
     }
+    if (q < g.nx) {
+#pragma unroll
+      for (int k = 0; k < 9; k++) {
+        R.bottom[(long long)k * g.nx + q] = feq.v[k];
+        R.top[(long long)k * g.nx + q] = feq.v[k];
+      }
+    }
   }
 }
 
@@ -876,6 +916,24 @@ void launch_copy_ring_column(hipStream_t s, const Grid& g, const MrtParams& P, c
     hipLaunchKernelGGL(k_copy_ring_column<double>, grid, block, 0, s, g, P, (const double*)f, col, ring);
 }
 
+template <typename S>
+__global__ void k_copy_ring_row(Grid g, MrtParams P, const S* __restrict__ f, int row, int keep, double* ring) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 9 * g.nx) return;
+  const int k = t / g.nx, i = t - k * g.nx;
+  if (i == keep) return;
+  ring[(long long)k * g.nx + i] = Store<S>::ld(f, (long long)k * g.ps + (long long)row * g.nx + i, P.w[k]);
+}
+
+void launch_copy_ring_row(hipStream_t s, const Grid& g, const MrtParams& P, const void* f, int row, int keep,
+                          double* ring, bool f32) {
+  dim3 grid((9 * g.nx + 255) / 256), block(256);
+  if (f32)
+    hipLaunchKernelGGL(k_copy_ring_row<float>, grid, block, 0, s, g, P, (const float*)f, row, keep, ring);
+  else
+    hipLaunchKernelGGL(k_copy_ring_row<double>, grid, block, 0, s, g, P, (const double*)f, row, keep, ring);
+}
+
 // ------------------------------------------------------- batched launches
 
 int bc_ring_blocks(const Grid& g) { return (2 * (g.ny - 2) + 2 * g.nx + 255) / 256; }
@@ -885,7 +943,7 @@ int bc_ring_blocks(const Grid& g) { return (2 * (g.ny - 2) + 2 * g.nx + 255) / 2
 // row-major tile order), so that the chain of dependent loads a workgroup
 // needs before its first population load (item, then descriptor, then
 // buffer pointers) is paid once per ROWS tiles.
-template <bool F32, bool FIELDS, int BX, int ROWS>
+template <bool F32, bool FIELDS, int BX, int ROWS, int CP>
 __global__ __launch_bounds__(BX) void kb_lattice(const EpDesc* __restrict__ eps, const int2* __restrict__ items,
                                                  const int* __restrict__ kb, int kk, int masked) {
   const int2 it = items[blockIdx.x];
@@ -902,10 +960,10 @@ __global__ __launch_bounds__(BX) void kb_lattice(const EpDesc* __restrict__ eps,
     const int blk = it.y * ROWS + r;
     if (blk >= ntiles) break;
     if (F32)
-      lattice_f32_body<FIELDS, BX>(g, D.P, (const float*)D.fbuf[cur], (float*)D.fbuf[cur ^ 1], fs, D.tiles_x, blk,
+      lattice_f32_body<FIELDS, BX, CP>(g, D.P, (const float*)D.fbuf[cur], (float*)D.fbuf[cur ^ 1], fs, D.tiles_x, blk,
                                    am);
     else
-      lattice_body<FIELDS, BX>(g, D.P, (const double*)D.fbuf[cur], (double*)D.fbuf[cur ^ 1], fs, D.tiles_x, blk,
+      lattice_body<FIELDS, BX, CP>(g, D.P, (const double*)D.fbuf[cur], (double*)D.fbuf[cur ^ 1], fs, D.tiles_x, blk,
                                am);
   }
 }
@@ -952,47 +1010,57 @@ void launch_step_base(hipStream_t s, int* p, int v, bool add) {
   hipLaunchKernelGGL(k_step_base, dim3(1), dim3(64), 0, s, p, v, add ? 1 : 0);
 }
 
-template <int BX, int ROWS>
-void launch_batch_lattice_bx(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields,
+template <int BX, int ROWS, int CP>
+void launch_batch_lattice_cp(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields,
                              int masked, hipEvent_t t1, size_t lds) {
   dim3 grid(it.n_lattice), block(BX);
   if (f32) {
     if (fields)
-      hipExtLaunchKernelGGL((kb_lattice<true, true, BX, ROWS>), grid, block, lds, s, nullptr, t1, 0, eps, it.lattice,
-                            it.kbase, k, masked);
+      hipExtLaunchKernelGGL((kb_lattice<true, true, BX, ROWS, CP>), grid, block, lds, s, nullptr, t1, 0, eps,
+                            it.lattice, it.kbase, k, masked);
     else
-      hipExtLaunchKernelGGL((kb_lattice<true, false, BX, ROWS>), grid, block, lds, s, nullptr, t1, 0, eps, it.lattice,
-                            it.kbase, k, masked);
+      hipExtLaunchKernelGGL((kb_lattice<true, false, BX, ROWS, CP>), grid, block, lds, s, nullptr, t1, 0, eps,
+                            it.lattice, it.kbase, k, masked);
   } else {
     if (fields)
-      hipExtLaunchKernelGGL((kb_lattice<false, true, BX, ROWS>), grid, block, lds, s, nullptr, t1, 0, eps, it.lattice,
-                            it.kbase, k, masked);
+      hipExtLaunchKernelGGL((kb_lattice<false, true, BX, ROWS, CP>), grid, block, lds, s, nullptr, t1, 0, eps,
+                            it.lattice, it.kbase, k, masked);
     else
-      hipExtLaunchKernelGGL((kb_lattice<false, false, BX, ROWS>), grid, block, lds, s, nullptr, t1, 0, eps, it.lattice,
-                            it.kbase, k, masked);
+      hipExtLaunchKernelGGL((kb_lattice<false, false, BX, ROWS, CP>), grid, block, lds, s, nullptr, t1, 0, eps,
+                            it.lattice, it.kbase, k, masked);
   }
+}
+
+// The batch times CP_PLAIN and CP_NT (non-temporal loads and stores).
+template <int BX, int ROWS>
+void launch_batch_lattice_bx(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields,
+                             int cp, int masked, hipEvent_t t1, size_t lds) {
+  if (cp == CP_NT)
+    launch_batch_lattice_cp<BX, ROWS, CP_NT>(s, eps, it, k, f32, fields, masked, t1, lds);
+  else
+    launch_batch_lattice_cp<BX, ROWS, CP_PLAIN>(s, eps, it, k, f32, fields, masked, t1, lds);
 }
 
 template <int BX>
 void launch_batch_lattice_rows(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields,
-                               int rows, int masked, hipEvent_t t1, size_t lds) {
+                               int rows, int cp, int masked, hipEvent_t t1, size_t lds) {
   if (rows == 4)
-    launch_batch_lattice_bx<BX, 4>(s, eps, it, k, f32, fields, masked, t1, lds);
+    launch_batch_lattice_bx<BX, 4>(s, eps, it, k, f32, fields, cp, masked, t1, lds);
   else if (rows == 2)
-    launch_batch_lattice_bx<BX, 2>(s, eps, it, k, f32, fields, masked, t1, lds);
+    launch_batch_lattice_bx<BX, 2>(s, eps, it, k, f32, fields, cp, masked, t1, lds);
   else
-    launch_batch_lattice_bx<BX, 1>(s, eps, it, k, f32, fields, masked, t1, lds);
+    launch_batch_lattice_bx<BX, 1>(s, eps, it, k, f32, fields, cp, masked, t1, lds);
 }
 
 void launch_batch_lattice(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields,
-                          int bx, int rows, bool masked, hipEvent_t t1, size_t lds) {
+                          int bx, int rows, int cp, bool masked, hipEvent_t t1, size_t lds) {
   if (it.n_lattice <= 0) return;
   if (bx == 128)
-    launch_batch_lattice_rows<128>(s, eps, it, k, f32, fields, rows, masked ? 1 : 0, t1, lds);
+    launch_batch_lattice_rows<128>(s, eps, it, k, f32, fields, rows, cp, masked, t1, lds);
   else if (bx == 512)
-    launch_batch_lattice_rows<512>(s, eps, it, k, f32, fields, rows, masked ? 1 : 0, t1, lds);
+    launch_batch_lattice_rows<512>(s, eps, it, k, f32, fields, rows, cp, masked, t1, lds);
   else
-    launch_batch_lattice_rows<256>(s, eps, it, k, f32, fields, rows, masked ? 1 : 0, t1, lds);
+    launch_batch_lattice_rows<256>(s, eps, it, k, f32, fields, rows, cp, masked, t1, lds);
 }
 
 void launch_batch_bc_ring(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields) {
@@ -1021,46 +1089,73 @@ void launch_batch_adj_collide(hipStream_t s, const EpDesc* eps, const BatchItems
 
 // The lattice kernel's memory pattern alone: every interior node pulls its
 // nine populations from the neighbours of the source planes and writes them
-// to the destination planes, with no moment or collision arithmetic.  Its
-// time is the ceiling of that access pattern on the device.
-template <typename T, int BX>
+// to the destination planes, with no moment or collision arithmetic, under
+// the lattice kernel's launch options (cache policy, wave cap).  Its time is
+// the ceiling of that access pattern on the device.
+template <typename T, int BX, int CP>
 __global__ __launch_bounds__(BX) void k_pull_copy(const T* __restrict__ src, T* __restrict__ dst, int nx, int ny) {
   const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y;
   if (i < 1 || i > nx - 2 || j < 1 || j > ny - 2) return;
   const size_t ps = (size_t)nx * ny;
   T f[9];
 #pragma unroll
-  for (int k = 0; k < 9; k++) f[k] = src[k * ps + (size_t)(j - EY(k)) * nx + (i - EX(k))];
+  for (int k = 0; k < 9; k++) f[k] = ld_pop<CP>(src + k * ps + (size_t)(j - EY(k)) * nx + (i - EX(k)));
 #pragma unroll
-  for (int k = 0; k < 9; k++) dst[k * ps + (size_t)j * nx + i] = f[k];
+  for (int k = 0; k < 9; k++) st_pop<CP>(dst + k * ps + (size_t)j * nx + i, f[k]);
 }
 
+template <int CP>
+void launch_pull_copy(bool f32, const void* s, void* d, int nx, int ny, size_t lds) {
+  const dim3 grid((nx + 255) / 256, ny), block(256);
+  if (f32)
+    hipLaunchKernelGGL((k_pull_copy<float, 256, CP>), grid, block, lds, 0, (const float*)s, (float*)d, nx, ny);
+  else
+    hipLaunchKernelGGL((k_pull_copy<double, 256, CP>), grid, block, lds, 0, (const double*)s, (double*)d, nx, ny);
+}
+
+// The fastest average over the cache policies and the wave caps the lattice
+// kernel's autotuner tries (each variant timed over `launches` launches, the
+// best of three rounds).
 int measure_pull_copy(int nx, int ny, bool f32, int launches, double* avg_us) {
   const size_t ps = (size_t)nx * ny, bytes = 9 * ps * (f32 ? 4 : 8);
   void *a = nullptr, *b = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  int rc = 0;
-  float ms = 0.f;
-  const dim3 grid((nx + 255) / 256, ny), block(256);
-  auto launch = [&](const void* s, void* d) {
-    if (f32)
-      hipLaunchKernelGGL((k_pull_copy<float, 256>), grid, block, 0, 0, (const float*)s, (float*)d, nx, ny);
+  int rc = 0, dev = 0, lds_cu = 0, lds_blk = 0;
+  (void)hipGetDevice(&dev);
+  if (hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess ||
+      hipDeviceGetAttribute(&lds_blk, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+    lds_cu = lds_blk = 0;
+  auto launch = [&](int cp, size_t lds, const void* s, void* d) {
+    if (cp == CP_NT)
+      launch_pull_copy<CP_NT>(f32, s, d, nx, ny, lds);
+    else if (cp == CP_NT_STORE)
+      launch_pull_copy<CP_NT_STORE>(f32, s, d, nx, ny, lds);
     else
-      hipLaunchKernelGGL((k_pull_copy<double, 256>), grid, block, 0, 0, (const double*)s, (double*)d, nx, ny);
+      launch_pull_copy<CP_PLAIN>(f32, s, d, nx, ny, lds);
   };
+  double best = 1e300;
   if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess || hipMemset(a, 0, bytes) != hipSuccess ||
       hipMemset(b, 0, bytes) != hipSuccess || hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
     rc = LBM_E_DEVICE;
   } else {
-    for (int r = 0; r < 3; r++) launch(r & 1 ? b : a, r & 1 ? a : b);
-    (void)hipEventRecord(e0, 0);
-    for (int r = 0; r < launches; r++) launch(r & 1 ? b : a, r & 1 ? a : b);
-    (void)hipEventRecord(e1, 0);
-    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess ||
-        hipGetLastError() != hipSuccess)
-      rc = LBM_E_DEVICE;
-    else
-      *avg_us = 1000.0 * ms / launches;
+    for (int round = 0; round < 3 && rc == 0; round++)
+      for (int cp : {CP_PLAIN, CP_NT_STORE, CP_NT})
+        for (int wv : {0, 24, 16, 12, 8}) {
+          const long long lds = lattice_lds_for_waves(256, wv, lds_cu, lds_blk);
+          if (lds < 0 || (wv > 0 && lds_cu <= 0)) continue;
+          for (int r = 0; r < 2; r++) launch(cp, (size_t)lds, r & 1 ? b : a, r & 1 ? a : b);
+          float ms = 0.f;
+          (void)hipEventRecord(e0, 0);
+          for (int r = 0; r < launches; r++) launch(cp, (size_t)lds, r & 1 ? b : a, r & 1 ? a : b);
+          (void)hipEventRecord(e1, 0);
+          if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess ||
+              hipGetLastError() != hipSuccess) {
+            rc = LBM_E_DEVICE;
+            break;
+          }
+          best = std::min(best, 1000.0 * ms / launches);
+        }
+    if (rc == 0) *avg_us = best;
   }
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);

@@ -37,8 +37,10 @@ def save_checkpoint(solver: Solver, out_dir: str, n: int, ring: dict) -> None:
     st = solver.state()
     _, u, v, _ = solver.fields()
     f, f0 = solver.populations()
-    ring["f0_left"] = f0[:, :, 0].copy()      # the f0 ring columns as the library holds them
+    ring["f0_left"] = f0[:, :, 0].copy()      # the f0 ring as the library holds it (move_mesh shifts it)
     ring["f0_right"] = f0[:, :, -1].copy()
+    ring["f0_bottom"] = f0[:, 0, :].copy()
+    ring["f0_top"] = f0[:, -1, :].copy()
     W = solver.cfg.wing_count
     rel = solver.body_rel() if W else np.zeros((11, 0))
     ck = formats.Checkpoint(cycle=n, movenum=st["movenum"], body_rel=rel, u=u, v=v, f=f, ring=ring)

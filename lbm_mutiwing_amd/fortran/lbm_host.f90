@@ -247,9 +247,11 @@ contains
         call check(lbm_get_fields(h, rho, uf, vf, pre), 'lbm_get_fields')
         call check(lbm_get_populations(h, f, f0), 'lbm_get_populations')
         if (wing_count > 0) call check(lbm_get_body_rel(h, rel), 'lbm_get_body_rel')
-        ! the f0 ring columns as the library holds them (move_mesh shifts them)
+        ! the f0 ring as the library holds it (move_mesh shifts it, rows included)
         ring%f0_left = f0(1, :, :)
         ring%f0_right = f0(nx, :, :)
+        ring%f0_bottom = f0(:, 1, :)
+        ring%f0_top = f0(:, ny, :)
         do q = 1, 2
             call write_checkpoint(trim(output_dir) // '/' // trim(names(q)) // '.dat', n, wing_count, rel, &
                                   int(movenum), nx, ny, uf, vf, f, ring, ok)

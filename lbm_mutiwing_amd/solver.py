@@ -280,6 +280,16 @@ class Solver:
         self._check(fn(self._h, ctypes.byref(v), ctypes.byref(w)))
         return v.value, w.value
 
+    def cache_policy(self) -> int:
+        """Cache policy of the lattice kernel's population accesses chosen by
+        the autotuner: 0 ordinary, 1 non-temporal stores, 2 non-temporal
+        loads and stores (HIP library only)."""
+        fn = self.lib.lbm_get_lattice_policy
+        fn.restype, fn.argtypes = ctypes.c_int, [ctypes.c_void_p, _ip]
+        c = ctypes.c_int32()
+        self._check(fn(self._h, ctypes.byref(c)))
+        return c.value
+
 
 class TimedSolver(Solver):
     """Solver with the lattice-kernel event timing of the HIP library."""
@@ -340,6 +350,15 @@ class Batch:
         v, r, w = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
         fn(self._b, ctypes.byref(v), ctypes.byref(r), ctypes.byref(w))
         return v.value, r.value, w.value
+
+    def cache_policy(self) -> int:
+        """Cache policy of the batched lattice kernel (0 ordinary, 2
+        non-temporal loads and stores)."""
+        fn = self.lib.lbm_batch_get_lattice_policy
+        fn.restype, fn.argtypes = ctypes.c_int, [ctypes.c_void_p, _ip]
+        c = ctypes.c_int32()
+        fn(self._b, ctypes.byref(c))
+        return c.value
 
     def __del__(self):
         try:

@@ -138,6 +138,21 @@ def test_move_mesh_shift(lib, oracle_mod):
     assert np.allclose(f0g[:, :, 0], f0o[:, :, 0], rtol=0, atol=1e-15)   # shifted f0 ring
     assert rel(fg, fo) < FIELD_TOL
 
+    def ring_matches():
+        """f0 ring columns and rows (the rows are shifted too,
+        move_mesh.f90:30-34, and saved by the checkpoint)."""
+        f0g, f0o = g.populations()[1], o.populations()[1]
+        for sl in ((slice(None), slice(None), 0), (slice(None), slice(None), -1),
+                   (slice(None), 0, slice(None)), (slice(None), -1, slice(None))):
+            assert rel(f0g[sl], f0o[sl]) < FIELD_TOL, sl
+    ring_matches()
+    for s_ in (g, o):   # and a move back (dr = -1, move_mesh.f90:171-214)
+        s_.set_body_rel_pos(1, -1.0)
+    g.advance(81)
+    o.advance(81)
+    assert g.state()["movenum"] == o.state()["movenum"] == 0
+    ring_matches()
+
 
 def test_run_to_run_bitwise_deterministic(lib):
     cfg = preset("small")
