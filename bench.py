@@ -679,9 +679,9 @@ def main():
     achieved = bpn * interior / (lat_avg_ms * 1e-3) / 1e9
     traffic = None
     tp = os.path.join(ROOT, "profiles", f"traffic_{args.config}_{args.precision}.json")
-    if os.path.exists(tp):  # the lattice kernel's measured HBM bytes at the tile width in use
+    if os.path.exists(tp):  # the lattice kernel's measured HBM bytes at the tile width and cache policy in use
         with open(tp) as fh:
-            traffic = json.load(fh).get("by_tile", {}).get(str(tile), {}).get("hbm_bytes_per_launch")
+            traffic = json.load(fh).get("by_tile_and_policy", {}).get(f"{tile}/{cpol}", {}).get("hbm_bytes_per_launch")
     out = {
         "metric": f"MLUPS (whole time step incl. IBM), {args.precision.upper()} single instance",
         "value": round(mlups, 1),
