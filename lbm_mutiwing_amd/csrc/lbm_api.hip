@@ -1811,7 +1811,7 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
     D.n0 = h->n;
     D.ns = ns;
     D.bc0 = (int)(h->batch_count & 1u);
-    D.tiles_x = 0;  // set once the tile width is known
+    D.tiles = 0;  // set once the tile width is known
     D.bpw = adj_collide_blocks_per_wing(h->ip);
     D.pad0 = D.pad1 = 0;
     if (h->W > 0) {
@@ -1857,8 +1857,8 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
       const int bx = c / 8, rows = c % 8;
       items.clear();
       for (int e = 0; e < count; e++) {
-        desc[e].tiles_x = lattice_tiles_x(desc[e].g, bx);
-        add(e, (desc[e].tiles_x * (desc[e].g.ny - 2) + rows - 1) / rows);
+        desc[e].tiles = lattice_tiles_flat(desc[e].g, bx);
+        add(e, (desc[e].tiles + rows - 1) / rows);
       }
       if ((rc = upload())) return;
       BatchItems lt{};
@@ -1880,8 +1880,8 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
     if (wc.empty()) wc.push_back(fcp == CP_NT ? CP_NT * 64 : 0);
     items.clear();
     for (int e = 0; e < count; e++) {
-      desc[e].tiles_x = lattice_tiles_x(desc[e].g, b->lat_bx);
-      add(e, (desc[e].tiles_x * (desc[e].g.ny - 2) + b->lat_rows - 1) / b->lat_rows);
+      desc[e].tiles = lattice_tiles_flat(desc[e].g, b->lat_bx);
+      add(e, (desc[e].tiles + b->lat_rows - 1) / b->lat_rows);
     }
     if ((rc = upload())) return rc;
     const int bw = autotune(b->stream, wc, [&](int c) {
@@ -1897,12 +1897,12 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
     b->lat_lds = (size_t)lattice_lds_for_waves(b->lat_bx, b->lat_waves, lds_cu, lds_blk);
     items.clear();
   }
-  for (int e = 0; e < count; e++) desc[e].tiles_x = lattice_tiles_x(desc[e].g, b->lat_bx);
+  for (int e = 0; e < count; e++) desc[e].tiles = lattice_tiles_flat(desc[e].g, b->lat_bx);
   // work lists: lattice, ring, adjoint, segment, spread, then the geometry
   // lists of a full and of the last (shorter) geometry batch
   size_t off[8];
   off[0] = items.size();
-  for (int e = 0; e < count; e++) add(e, (desc[e].tiles_x * (desc[e].g.ny - 2) + b->lat_rows - 1) / b->lat_rows);
+  for (int e = 0; e < count; e++) add(e, (desc[e].tiles + b->lat_rows - 1) / b->lat_rows);
   off[1] = items.size();
   for (int e = 0; e < count; e++) add(e, bc_ring_blocks(desc[e].g));
   off[2] = items.size();

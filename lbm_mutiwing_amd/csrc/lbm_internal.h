@@ -202,7 +202,7 @@ struct EpDesc {
   void* fbuf[2];
   FieldSlot fslot[3];      // field slot of step n is fslot[n % 3]
   int cur0, n0, ns, bc0;
-  int tiles_x;             // lattice row tiles of the batch's tile width
+  int tiles;               // lattice tiles of the batch's tile width over the interior rows taken as one run
   int bpw;                 // adjoint-collide blocks per wing
   int pad0, pad1;
 };
@@ -258,7 +258,8 @@ inline long long lattice_lds_for_waves(int bx, int waves, int lds_cu, int lds_bl
   const long long bytes = ((lds_cu / (wg + 1) + 1) + 511) / 512 * 512;
   return wg * bytes <= lds_cu && bytes <= lds_blk ? bytes : -1;
 }
-int lattice_tiles_x(const Grid& g, int bx);
+int lattice_tiles_x(const Grid& g, int bx);      // row tiles of one row (single instance)
+int lattice_tiles_flat(const Grid& g, int bx);   // tiles of the interior rows taken as one run (batch)
 void launch_adj_collide(hipStream_t s, const Grid& g, const MrtParams& P, const IbmParams& p, const GeoBuffers& G,
                         int slot, const FlowBuffers& F, int n, const void* src, void* dst, FieldSlot* fields,
                         bool f32);

@@ -38,19 +38,12 @@ __device__ __forceinline__ void st_pop(T* __restrict__ p, T v) {
   else *p = v;
 }
 
-// One thread per node of a row tile of BX nodes; tiles start at i = 0 so
-// that every wavefront stores whole, aligned spans of each population plane
-// (the ring columns i = 0 and i = nx-1 are skipped and left to k_bc_ring).
-// Workgroups follow the hardware's round-robin placement over the XCDs, which
-// measured faster than remapping tiles to XCD-contiguous runs.  The tile
-// width is chosen per instance (or per batch) by timing (lattice_autotune).
-template <bool FIELDS, int BX, int CP>
-__device__ __forceinline__ void lattice_body(const Grid& g, const MrtParams& P, const double* __restrict__ src,
-                                             double* __restrict__ dst, const FieldSlot& fs, int tiles_x, int blk,
+// The FP64 step of interior node (i, j) (the ring columns i = 0 and nx-1 and
+// the step's adjoint nodes excepted: k_bc_ring and k_adj_collide write those).
+template <bool FIELDS, int CP>
+__device__ __forceinline__ void lattice_node(const Grid& g, const MrtParams& P, const double* __restrict__ src,
+                                             double* __restrict__ dst, const FieldSlot& fs, int i, int j,
                                              const AdjMask& am) {
-  const int ty = blk / tiles_x, tx = blk - ty * tiles_x;
-  const int i = tx * BX + (int)threadIdx.x;
-  const int j = 1 + ty;
   if (i < 1 || i > g.nx - 2 || is_adjoint(am, i, j)) return;
   double f[9];
 #pragma unroll
@@ -69,10 +62,35 @@ __device__ __forceinline__ void lattice_body(const Grid& g, const MrtParams& P, 
   }
 }
 
+// One thread per node of a row tile of BX nodes; tiles start at i = 0 so
+// that every wavefront stores whole, aligned spans of each population plane
+// (the ring columns i = 0 and i = nx-1 are skipped and left to k_bc_ring).
+// Workgroups follow the hardware's round-robin placement over the XCDs, which
+// measured faster than remapping tiles to XCD-contiguous runs.  The tile
+// width is chosen per instance by timing (tune_instance).
 template <bool FIELDS, int BX, int CP>
 __global__ __launch_bounds__(BX) void k_lattice(Grid g, MrtParams P, const double* __restrict__ src,
                                                 double* __restrict__ dst, FieldSlot fs, int tiles_x, AdjMask am) {
-  lattice_body<FIELDS, BX, CP>(g, P, src, dst, fs, tiles_x, blockIdx.x, am);
+  const int ty = blockIdx.x / tiles_x, tx = blockIdx.x - ty * tiles_x;
+  lattice_node<FIELDS, CP>(g, P, src, dst, fs, tx * BX + (int)threadIdx.x, 1 + ty, am);
+}
+
+// Node of lane t in tile `blk` of BX consecutive nodes of the rows j = 1..ny-2
+// taken as one run (row-major, rows back to back): the batch covers each
+// lattice's interior rows with whole tiles and no partial tile at each row
+// end.  A tile wraps into the next row at most once when BX <= nx (every
+// lattice of the batched workloads), more often on narrower lattices; a lane
+// past the last interior row gets j > ny - 2.
+template <int BX>
+__device__ __forceinline__ void flat_node(const Grid& g, int blk, int t, int& i, int& j) {
+  const int q0 = blk * BX;                 // offset of the tile in the run
+  const int j0 = q0 / g.nx;                // uniform over the tile
+  i = q0 - j0 * g.nx + t;
+  j = 1 + j0;
+  while (i >= g.nx) {
+    i -= g.nx;
+    j += 1;
+  }
 }
 
 // FP32 bulk step on deviations d = f - w from the rest equilibrium, in FP32.
@@ -86,13 +104,10 @@ __global__ __launch_bounds__(BX) void k_lattice(Grid g, MrtParams P, const doubl
 // 1/36, 1/6, 1/12, 1/6, 1/12, 1/4, 1/4), the scaled relaxed moments are
 // tt_r = c_r (1 - s_r) m_r + c_r s_r m_eq,r (MrtParams ra32, rb32, rk0) and
 // d* = M^T tt, again from shared sums.  About 90 FP32 operations per node.
-template <bool FIELDS, int BX, int CP>
-__device__ __forceinline__ void lattice_f32_body(const Grid& g, const MrtParams& P, const float* __restrict__ src,
-                                                 float* __restrict__ dst, const FieldSlot& fs, int tiles_x, int blk,
+template <bool FIELDS, int CP>
+__device__ __forceinline__ void lattice_f32_node(const Grid& g, const MrtParams& P, const float* __restrict__ src,
+                                                 float* __restrict__ dst, const FieldSlot& fs, int i, int j,
                                                  const AdjMask& am) {
-  const int ty = blk / tiles_x, tx = blk - ty * tiles_x;
-  const int i = tx * BX + (int)threadIdx.x;
-  const int j = 1 + ty;
   if (i < 1 || i > g.nx - 2 || is_adjoint(am, i, j)) return;
   float d[9];
 #pragma unroll
@@ -154,10 +169,12 @@ __device__ __forceinline__ void lattice_f32_body(const Grid& g, const MrtParams&
 template <bool FIELDS, int BX, int CP>
 __global__ __launch_bounds__(BX) void k_lattice_f32(Grid g, MrtParams P, const float* __restrict__ src,
                                                     float* __restrict__ dst, FieldSlot fs, int tiles_x, AdjMask am) {
-  lattice_f32_body<FIELDS, BX, CP>(g, P, src, dst, fs, tiles_x, blockIdx.x, am);
+  const int ty = blockIdx.x / tiles_x, tx = blockIdx.x - ty * tiles_x;
+  lattice_f32_node<FIELDS, CP>(g, P, src, dst, fs, tx * BX + (int)threadIdx.x, 1 + ty, am);
 }
 
 int lattice_tiles_x(const Grid& g, int bx) { return (g.nx + bx - 1) / bx; }
+int lattice_tiles_flat(const Grid& g, int bx) { return (int)(((long long)(g.ny - 2) * g.nx + bx - 1) / bx); }
 
 template <int BX, int CP>
 void launch_lattice_cp(hipStream_t s, const Grid& g, const MrtParams& P, const void* src, void* dst,
@@ -952,19 +969,19 @@ __global__ __launch_bounds__(BX) void kb_lattice(const EpDesc* __restrict__ eps,
   const int n = D.n0 + k, cur = (D.cur0 + k) & 1;
   AdjMask am = adj_mask(D.p, D.G, ep_slot(D, k));
   if (!masked) am.W = 0;
-  const int ntiles = D.tiles_x * (D.g.ny - 2);
   const Grid g = D.g;
   const FieldSlot fs = D.fslot[n % 3];
 #pragma unroll 1
   for (int r = 0; r < ROWS; r++) {
     const int blk = it.y * ROWS + r;
-    if (blk >= ntiles) break;
+    if (blk >= D.tiles) break;
+    int i, j;
+    flat_node<BX>(g, blk, (int)threadIdx.x, i, j);
+    if (j > g.ny - 2) break;
     if (F32)
-      lattice_f32_body<FIELDS, BX, CP>(g, D.P, (const float*)D.fbuf[cur], (float*)D.fbuf[cur ^ 1], fs, D.tiles_x, blk,
-                                   am);
+      lattice_f32_node<FIELDS, CP>(g, D.P, (const float*)D.fbuf[cur], (float*)D.fbuf[cur ^ 1], fs, i, j, am);
     else
-      lattice_body<FIELDS, BX, CP>(g, D.P, (const double*)D.fbuf[cur], (double*)D.fbuf[cur ^ 1], fs, D.tiles_x, blk,
-                               am);
+      lattice_node<FIELDS, CP>(g, D.P, (const double*)D.fbuf[cur], (double*)D.fbuf[cur ^ 1], fs, i, j, am);
   }
 }
 
