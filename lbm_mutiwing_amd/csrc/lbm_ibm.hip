@@ -380,16 +380,25 @@ __device__ __forceinline__ double s_entry(const int* box, const double* phx, con
 // band[(k+i)*Dd + i] = L(k+i, k), invd[k] = 1/D_k.  Step k updates
 // S(k+i, k+j) -= (S(k+i, k) / D_k) S(k+j, k) for 1 <= j <= i <= bw with the
 // column k not yet scaled: the bw(bw+1)/2 updates write distinct entries and
-// read only column k, so each lane takes up to five (i, j) pairs of its own
-// and a step costs one round of LDS updates and one wave barrier.  Column k
-// is scaled by 1/D_k after the last step (it is not read again).
+// read only column k, so each lane takes up to NPL (i, j) pairs of its own
+// and a step costs one round of LDS loads and stores.  The loads of a step do
+// not wait for its pivot: each lane loads its pairs' operands, and every lane
+// the entries of the next pivot, as the step starts, and forms the next
+// pivot D_{k+1} itself with the arithmetic of the (1, 1) update, so that the
+// division of step k+1 overlaps the stores of step k.  LDS operations of one
+// wave complete in order, so a step's loads see the previous step's stores.
+// Column k is scaled by 1/D_k after the last step (it is not read again).
 constexpr int LDLT_PAIRS_PER_LANE = (BMAX * (BMAX + 1) / 2 + 63) / 64;
 
-__device__ void band_ldlt(double* band, double* invd, int n, int bw, int Dd, int lane, DevStep* st, int info) {
+template <int NPL>
+__device__ void band_ldlt_t(double* band, double* invd, int n, int bw, int Dd, int lane, DevStep* st, int info) {
   const int npairs = bw * (bw + 1) / 2;
-  int pi[LDLT_PAIRS_PER_LANE], pj[LDLT_PAIRS_PER_LANE];  // pair q = i(i-1)/2 + j-1; i = 0: none
+  // offsets from row k of a pair's target S(k+i, k+j), and of its operands
+  // S(k+i, k) and S(k+j, k); pairs past the last row or past npairs write
+  // nothing (their loads read row k)
+  int ot[NPL], oa[NPL], oc[NPL], pi[NPL];
 #pragma unroll
-  for (int u = 0; u < LDLT_PAIRS_PER_LANE; u++) {
+  for (int u = 0; u < NPL; u++) {
     const int q = lane + 64 * u;
     int i = 0, j = 0;
     if (q < npairs) {
@@ -398,26 +407,50 @@ __device__ void band_ldlt(double* band, double* invd, int n, int bw, int Dd, int
       j = q - i * (i - 1) / 2 + 1;
     }
     pi[u] = i;
-    pj[u] = j;
+    ot[u] = i > 0 ? i * Dd + (i - j) : 0;
+    oa[u] = i > 0 ? i * Dd + i : 0;
+    oc[u] = i > 0 ? j * Dd + j : 0;
   }
+  double d = band[0];
+  int bad = -1;
   for (int k = 0; k < n; k++) {
-    const double d = band[k * Dd];
-    if (lane == 0 && !(d > 0.0)) raise_error(st, DERR_NOT_SPD, info + k);
+    double* bk = band + (size_t)k * Dd;
+    double vt[NPL], va[NPL], vc[NPL];
+#pragma unroll
+    for (int u = 0; u < NPL; u++) {
+      vt[u] = bk[ot[u]];
+      va[u] = bk[oa[u]];
+      vc[u] = bk[oc[u]];
+    }
+    const double dn = bk[Dd], ln = bk[Dd + 1];   // S(k+1, k+1), S(k+1, k): row k+1 exists (padding rows)
+    if (!(d > 0.0) && bad < 0) bad = k;
     const double rd = 1.0 / (d > 0.0 ? d : 1.0);
 #pragma unroll
-    for (int u = 0; u < LDLT_PAIRS_PER_LANE; u++) {
-      const int i = pi[u], j = pj[u];
-      if (i > 0 && k + i < n) band[(k + i) * Dd + (i - j)] -= band[(k + i) * Dd + i] * rd * band[(k + j) * Dd + j];
-    }
+    for (int u = 0; u < NPL; u++)
+      if (pi[u] > 0 && k + pi[u] < n) bk[ot[u]] = vt[u] - va[u] * rd * vc[u];
     if (lane == 0) invd[k] = rd;
+    d = dn - ln * rd * ln;   // the (1, 1) update: D_{k+1} (entries beyond bw are zero)
     wave_sync();
   }
+  if (lane == 0 && bad >= 0) raise_error(st, DERR_NOT_SPD, info + bad);
   // L(k + i, k) = S(k + i, k) / D_k
   for (int e = lane; e < n * bw; e += 64) {
     const int k = e / bw, i = 1 + e - (e / bw) * bw;
     if (k + i < n) band[(k + i) * Dd + i] = band[(k + i) * Dd + i] * invd[k];
   }
   wave_sync();
+}
+
+__device__ void band_ldlt(double* band, double* invd, int n, int bw, int Dd, int lane, DevStep* st, int info) {
+  const int npl = (bw * (bw + 1) / 2 + 63) / 64;
+  if (npl <= 1)
+    band_ldlt_t<1>(band, invd, n, bw, Dd, lane, st, info);
+  else if (npl == 2)
+    band_ldlt_t<2>(band, invd, n, bw, Dd, lane, st, info);
+  else if (npl == 3)
+    band_ldlt_t<3>(band, invd, n, bw, Dd, lane, st, info);
+  else
+    band_ldlt_t<LDLT_PAIRS_PER_LANE>(band, invd, n, bw, Dd, lane, st, info);
 }
 
 // One row of a band substitution: a0 minus the BW products of the row's
