@@ -520,7 +520,7 @@ int alloc_ibm(lbm_handle* h, const std::vector<double>& hx, const std::vector<do
   const int side = (int)std::ceil(2.0 * std::sqrt(r2) / h->cfg.scale) + 12;
   p.map_cap = h->W > 0 ? side * side : 1;
   const size_t per_slot = sizeof(GeoStep) + 2 * npts * 8 + 4 * nseg * 8 + 4 * nseg * 4 + 2 * BOXW * nseg * 8 +
-                          W * (spw + 1) * 4 + 3 * cap * 4 + W * spw * spw * 8 + W * 2 * (size_t)p.map_cap * 4 +
+                          W * (spw + 1) * 4 + 3 * cap * 4 + W * spw * spw * 8 + W * (size_t)p.map_cap * 4 +
                           (size_t)CPL_KT * CPL_KT * 8 + W * band_cap(p) * 8;
   h->nslot = h->W > 0 ? (int)std::max<size_t>(1, std::min<size_t>(16, (256u << 20) / per_slot)) : 1;
   const size_t ns = 2 * (size_t)h->nslot;
@@ -540,7 +540,7 @@ int alloc_ibm(lbm_handle* h, const std::vector<double>& hx, const std::vector<do
       (rc = ialloc(h, &G.adj_ij, ns * cap * 2)) || (rc = ialloc(h, &G.adj_seg, ns * cap)) ||
       (rc = ialloc(h, &G.sinv, h->W > 0 ? ns * W * spw * spw : 1)) ||
       (rc = ialloc(h, &bandg, (h->W > 0 ? ns * W * band_cap(p) : 1) + 2 * band_pad)) ||
-      (rc = ialloc(h, &G.map, h->W > 0 ? ns * W * 2 * (size_t)p.map_cap : 1)) ||
+      (rc = ialloc(h, &G.map, h->W > 0 ? ns * W * (size_t)p.map_cap : 1)) ||
       (rc = ialloc(h, &G.cpl, h->W > 1 ? ns * CPL_KT * CPL_KT : 1)))
     return rc;
   G.bandg = bandg + band_pad;
@@ -1630,6 +1630,7 @@ struct lbm_batch {
   int lat_waves = 0;                 // lattice waves per CU cap (0: none), tuned last
   int lat_cp = 0;                    // cache policy of the lattice kernel, tuned with the cap
   size_t lat_lds = 0;
+  bool direct = false;               // some episode of the call solves by band substitution
   char msg[512] = "";
 };
 
@@ -1784,6 +1785,7 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
   std::vector<int2> items;
   BatchItems it{};
   size_t lds_wing = 0, lds_solve = 0;
+  b->direct = false;
   int inv_chunks = 1;
   for (int e = 0; e < count; e++) {
     lbm_handle* h = hs[e];
@@ -1813,7 +1815,9 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
     D.bc0 = (int)(h->batch_count & 1u);
     D.tiles = 0;  // set once the tile width is known
     D.bpw = adj_collide_blocks_per_wing(h->ip);
-    D.pad0 = D.pad1 = 0;
+    D.direct = h->direct == 2 ? 1 : 0;   // as the instance's own advance solves its prescribed steps
+    D.pad1 = 0;
+    if (D.direct && h->W > 0) b->direct = true;
     if (h->W > 0) {
       lds_wing = std::max(lds_wing, geo_wing_lds_bytes(h->SPW));
       lds_solve = std::max(lds_solve, geo_solve_lds_bytes(h->ip));
@@ -1900,7 +1904,7 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
   for (int e = 0; e < count; e++) desc[e].tiles = lattice_tiles_flat(desc[e].g, b->lat_bx);
   // work lists: lattice, ring, adjoint, segment, spread, then the geometry
   // lists of a full and of the last (shorter) geometry batch
-  size_t off[8];
+  size_t off[9];
   off[0] = items.size();
   for (int e = 0; e < count; e++) add(e, (desc[e].tiles + b->lat_rows - 1) / b->lat_rows);
   off[1] = items.size();
@@ -1921,6 +1925,9 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
   for (int e = 0; e < count; e++)
     if (desc[e].p.W > 0) add(e, tail * desc[e].p.W);
   off[7] = items.size();
+  for (int e = 0; e < count; e++)
+    if (desc[e].p.W > 0) add(e, desc[e].p.W);
+  off[8] = items.size();
   {
     const int rc = upload();
     if (rc) return rc;
@@ -1931,6 +1938,7 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
   it.adj = di + off[2]; it.n_adj = (int)(off[3] - off[2]);
   it.seg = di + off[3]; it.n_seg = (int)(off[4] - off[3]);
   it.spread = di + off[4]; it.n_spread = (int)(off[5] - off[4]);
+  it.wing = di + off[7]; it.n_wing = (int)(off[8] - off[7]);
   it.kbase = b->d_kbase;
   // The launch sequence of the call, in segments of whole geometry batches
   // (pl steps; 16 without wings), each ending with the force row of its last
@@ -1986,7 +1994,7 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
         const int rc = issue_geometry(k / ns + 1, seg_bb, kb0);
         if (rc) return rc;
       }
-      launch_batch_ibm_flow(b->stream_ibm, b->d_desc, it, kk, f32);
+      launch_batch_ibm_flow(b->stream_ibm, b->d_desc, it, kk, f32, b->direct);
       launch_batch_adj_collide(b->stream_ibm, b->d_desc, it, kk, f32, fields);
       BCHECK(b, hipStreamWaitEvent(b->stream_ibm, b->ev_bc, 0));
       BCHECK(b, hipEventRecord(b->ev_ibm, b->stream_ibm));
@@ -2015,10 +2023,11 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
   // The key: the composition, the device lists and, for a whole advance,
   // its length and the last geometry list (a period graph uses the full
   // list only).
-  std::vector<long long> key = {count, ns, f32, wings, b->lat_bx, b->lat_rows, b->lat_waves, b->lat_cp,
+  std::vector<long long> key = {count, ns, f32, wings, b->lat_bx, b->lat_rows, b->lat_waves, b->lat_cp, b->direct,
                                 (long long)(size_t)b->d_desc, (long long)(size_t)b->d_items, (long long)lds_wing,
                                 (long long)lds_solve};
   for (int q = 0; q < 7; q++) key.push_back((long long)off[q]);
+  key.push_back((long long)off[8]);
   for (int e = 0; e < count; e++) key.push_back((long long)(size_t)hs[e]);
   auto find_graph = [&](long long tag) -> BatchGraph* {
     std::vector<long long> kq = key;

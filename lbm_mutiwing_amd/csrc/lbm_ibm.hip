@@ -137,15 +137,14 @@ size_t geo_wing_lds_bytes(int spw) {
 // support boxes (airfoil.f90:288-321), delta weights of every box column and
 // row (boundary_condition.f90:1558-1600), then the adjoint list: every box
 // cell claims its node in the wing's claim map with its segment index
-// (atomicMin: lowest segment wins, which is the reference's first claimant)
-// and its band position (atomicMin: the band half-width of S is the largest
-// position spread of any node's segments).
+// (atomicMin: lowest segment wins, which is the reference's first claimant).
+// The band half-width of S follows in the solve, from the boxes.
 __device__ __forceinline__ void geo_wing_body(const IbmParams& p, const GeoBuffers& G, DevStep* st, int slot0,
                                               int n0, int fidx0, int b, int w) {
   extern __shared__ unsigned long long lds_geo[];
   __shared__ WingMotionState s_m;
   __shared__ int s_rect[4];
-  __shared__ int s_bw, s_total;
+  __shared__ int s_total;
   __shared__ int s_part[GEO_THREADS];
   const int t = threadIdx.x;
   const int slot = slot0 + b, n = n0 + b, SPW = p.SPW;
@@ -160,7 +159,6 @@ __device__ __forceinline__ void geo_wing_body(const IbmParams& p, const GeoBuffe
     s_m = wing_motion(p, G, w, time, &ok);
     if (!ok) raise_error(st, DERR_MOTION, w);
     s_rect[0] = INT_MAX; s_rect[1] = INT_MIN; s_rect[2] = INT_MAX; s_rect[3] = INT_MIN;
-    s_bw = 0;
   }
   __syncthreads();
   const WingMotionState m = s_m;
@@ -251,12 +249,8 @@ __device__ __forceinline__ void geo_wing_body(const IbmParams& p, const GeoBuffe
     }
     return;
   }
-  int* M = G.map + ((size_t)slot * p.W + w) * 2 * (size_t)p.map_cap;  // owner segment per cell
-  int* Q = M + p.map_cap;                                              // lowest band position per cell
-  for (long long c = t; c < cells; c += GEO_THREADS) {
-    M[c] = INT_MAX;
-    Q[c] = INT_MAX;
-  }
+  int* M = G.map + ((size_t)slot * p.W + w) * (size_t)p.map_cap;  // owner segment per cell
+  for (long long c = t; c < cells; c += GEO_THREADS) M[c] = INT_MAX;
   __threadfence();
   __syncthreads();
   for (int e = t; e < SPW * BOXC; e += GEO_THREADS) {
@@ -264,29 +258,21 @@ __device__ __forceinline__ void geo_wing_body(const IbmParams& p, const GeoBuffe
     const int* bx = sbox + 4 * k;
     const int i = bx[0] + c / BOXW, j = bx[2] + c % BOXW;
     if (i > bx[1] || j > bx[3]) continue;
-    const int idx = (i - i0) + wr * (j - j0);
-    atomicMin(&M[idx], k);
-    atomicMin(&Q[idx], pos_of_seg(k, SPW));
+    atomicMin(&M[(i - i0) + wr * (j - j0)], k);
   }
   __threadfence();
   __syncthreads();
-  int bwl = 0;
   for (int e0 = t; e0 < SPW * BOXC; e0 += CLAIM_UNROLL * GEO_THREADS) {
-    int own[CLAIM_UNROLL], qm[CLAIM_UNROLL];
+    int own[CLAIM_UNROLL];
 #pragma unroll
     for (int u = 0; u < CLAIM_UNROLL; u++) {
       const int e = e0 + u * GEO_THREADS;
       own[u] = -1;
-      qm[u] = INT_MAX;
       if (e < SPW * BOXC) {
         const int k = e / BOXC, c = e - k * BOXC;
         const int* bx = sbox + 4 * k;
         const int i = bx[0] + c / BOXW, j = bx[2] + c % BOXW;
-        if (i <= bx[1] && j <= bx[3]) {
-          const int idx = (i - i0) + wr * (j - j0);
-          own[u] = load_agent(&M[idx]);
-          qm[u] = load_agent(&Q[idx]);
-        }
+        if (i <= bx[1] && j <= bx[3]) own[u] = load_agent(&M[(i - i0) + wr * (j - j0)]);
       }
     }
 #pragma unroll
@@ -295,11 +281,8 @@ __device__ __forceinline__ void geo_wing_body(const IbmParams& p, const GeoBuffe
       if (own[u] < 0) continue;
       const int k = e / BOXC, c = e - k * BOXC;
       if (own[u] == k) atomicOr(&smask[k], 1ull << c);
-      bwl = max(bwl, pos_of_seg(k, SPW) - qm[u]);
     }
   }
-  for (int off = 32; off > 0; off >>= 1) bwl = max(bwl, __shfl_xor(bwl, off));
-  if ((t & 63) == 0) atomicMax(&s_bw, bwl);
   __syncthreads();
   // per-segment claim counts, exclusive scan, list in (segment, i, j) order
   for (int k = t; k < SPW; k += GEO_THREADS) scnt[k] = __popcll(smask[k]);
@@ -346,7 +329,6 @@ __device__ __forceinline__ void geo_wing_body(const IbmParams& p, const GeoBuffe
   if (t == 0) {
     gs->wing[w] = m;
     gs->cnt[w] = s_total;
-    gs->bw[w] = s_bw;
     gs->rect[w][0] = i0; gs->rect[w][1] = i0 + wr - 1; gs->rect[w][2] = j0; gs->rect[w][3] = j0 + hr - 1;
     if (w == 0) { gs->time = time; gs->n = n; gs->frame = fr; }
   }
@@ -593,7 +575,7 @@ __device__ __forceinline__ void inverse_column(const double* band, const double*
 __device__ __forceinline__ void geo_solve_body(const IbmParams& p, const GeoBuffers& G, DevStep* st, int slot0, int b,
                                                int w, int direct) {
   extern __shared__ double lds_s[];
-  __shared__ int s_skip;
+  __shared__ int s_skip, s_bw;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int slot = slot0 + b, n = p.SPW;
   GeoStep* gs = G.step + slot;
@@ -624,8 +606,35 @@ __device__ __forceinline__ void geo_solve_body(const IbmParams& p, const GeoBuff
     }
   }
   __syncthreads();
-  const int bw = gs->bw[w];
-  if (s_skip || gs->cnt[w] == 0) return;
+  if (s_skip || gs->cnt[w] == 0) {
+    if (t == 0) gs->bw[w] = 0;
+    return;
+  }
+  const size_t base = (size_t)slot * p.nseg + (size_t)w * n;
+  const int* box = G.box + base * 4;
+  // band half-width: the largest spread in band positions of two segments
+  // whose support boxes share a node (checked one past BMAX, so that a wider
+  // band is reported)
+  {
+    int* sb = reinterpret_cast<int*>(lds_s);
+    for (int e = t; e < 4 * n; e += GEO_THREADS) sb[e] = box[e];
+    if (t == 0) s_bw = 0;
+    __syncthreads();
+    constexpr int DMAX = BMAX + 1;
+    int bwl = 0;
+    for (int e = t; e < n * DMAX; e += GEO_THREADS) {
+      const int pp = e / DMAX, d = 1 + e - pp * DMAX;
+      if (pp - d < 0) continue;
+      const int* bk = sb + 4 * seg_of_pos(pp, n);
+      const int* bl = sb + 4 * seg_of_pos(pp - d, n);
+      if (max(bk[0], bl[0]) <= min(bk[1], bl[1]) && max(bk[2], bl[2]) <= min(bk[3], bl[3])) bwl = max(bwl, d);
+    }
+    for (int off = 32; off > 0; off >>= 1) bwl = max(bwl, __shfl_xor(bwl, off));
+    if (lane == 0) atomicMax(&s_bw, bwl);
+    __syncthreads();
+  }
+  const int bw = s_bw;
+  if (t == 0) gs->bw[w] = bw;
   // the LDS of the launch holds bands up to the class of the instance's bound
   if (bw > BMAX || band_class(bw) > p.band_cls) {
     if (t == 0) raise_error(st, DERR_BAND, bw);
@@ -636,8 +645,6 @@ __device__ __forceinline__ void geo_solve_body(const IbmParams& p, const GeoBuff
   const int Dd = BWc + 1;
   double* band = lds_s;                          // (n + BWc) rows of Dd
   double* invd = band + (size_t)(n + BWc) * Dd;
-  const size_t base = (size_t)slot * p.nseg + (size_t)w * n;
-  const int* box = G.box + base * 4;
   const double* phx = G.phx + base * BOXW;
   const double* phy = G.phy + base * BOXW;
   for (int e = t; e < (n + BWc) * Dd; e += GEO_THREADS) {
@@ -814,8 +821,8 @@ __device__ __forceinline__ void geo_couple_body(const IbmParams& p, const GeoBuf
       const int ia = max(r1[0], r2[0]), ib = min(r1[1], r2[1]);
       const int ja = max(r1[2], r2[2]), jb = min(r1[3], r2[3]);
       if (ia > ib || ja > jb) continue;
-      const int* M1 = G.map + ((size_t)slot * W + w) * 2 * (size_t)p.map_cap;
-      const int* M2 = G.map + ((size_t)slot * W + w2) * 2 * (size_t)p.map_cap;
+      const int* M1 = G.map + ((size_t)slot * W + w) * (size_t)p.map_cap;
+      const int* M2 = G.map + ((size_t)slot * W + w2) * (size_t)p.map_cap;
       const int w1w = r1[1] - r1[0] + 1, w2w = r2[1] - r2[0] + 1;
       const int ow = ib - ia + 1, oc = ow * (jb - ja + 1);
       for (int c = t; c < oc; c += CPL_THREADS) {
@@ -975,7 +982,7 @@ __device__ __forceinline__ void geo_couple_body(const IbmParams& p, const GeoBuf
         for (int a = 0; a < b && !owned; a++) {
           if (!s_flag[a * MAXW + b]) continue;
           const int* ra = gs->rect[a];
-          const int* MA = G.map + ((size_t)slot * W + a) * 2 * (size_t)p.map_cap;
+          const int* MA = G.map + ((size_t)slot * W + a) * (size_t)p.map_cap;
           owned = i >= ra[0] && i <= ra[1] && j >= ra[2] && j <= ra[3] &&
                   MA[(i - ra[0]) + (ra[1] - ra[0] + 1) * (j - ra[2])] != INT_MAX;
         }
@@ -1136,7 +1143,7 @@ __global__ __launch_bounds__(256) void k_ibm_rhs(IbmParams p, MrtParams P, GeoBu
 __device__ __forceinline__ void ibm_matvec_body(const IbmParams& p, const GeoBuffers& G, int slot,
                                                 const FlowBuffers& F, int blk) {
   const int gp = blk * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-  if (gp >= p.nseg) return;
+  if (gp >= p.nseg || !G.step[slot].inv_on) return;   // no inverse: the band substitution solves
   const int n = p.SPW, w = gp / n, r = gp - w * n;
   const double* row = G.sinv + (((size_t)slot * p.W + w) * n + r) * (size_t)n;
   const double* c = F.cvec + (size_t)w * n * 2;
@@ -1165,10 +1172,10 @@ __global__ __launch_bounds__(256) void k_ibm_matvec(IbmParams p, GeoBuffers G, i
 // workgroup w solves wing w, wave c component c.
 constexpr int BSOLVE_THREADS = 128;
 
-__global__ __launch_bounds__(BSOLVE_THREADS) void k_ibm_bsolve(IbmParams p, GeoBuffers G, int slot,
-                                                               FlowBuffers F) {
+__device__ __forceinline__ void ibm_bsolve_body(const IbmParams& p, const GeoBuffers& G, int slot,
+                                                const FlowBuffers& F, int w) {
   const GeoStep* gs = G.step + slot;
-  const int w = blockIdx.x, comp = threadIdx.x >> 6;
+  const int comp = threadIdx.x >> 6;
   if (gs->inv_on) return;
   const int n = p.SPW, bw = gs->bw[w];
   const double* c = F.cvec + 2 * (size_t)w * n + comp;
@@ -1189,6 +1196,11 @@ __global__ __launch_bounds__(BSOLVE_THREADS) void k_ibm_bsolve(IbmParams p, GeoB
     band_solve_wave<16>(band, invd, n, c, y);
   else
     band_solve_wave<BMAX>(band, invd, n, c, y);
+}
+
+__global__ __launch_bounds__(BSOLVE_THREADS) void k_ibm_bsolve(IbmParams p, GeoBuffers G, int slot,
+                                                               FlowBuffers F) {
+  ibm_bsolve_body(p, G, slot, F, blockIdx.x);
 }
 
 // g = D^T y at the adjoint nodes (boundary_condition.f90:1639-1651).  A
@@ -1309,7 +1321,7 @@ __global__ __launch_bounds__(GEO_THREADS) void kb_geo_solve(const EpDesc* __rest
   const EpDesc& D = eps[it.x];
   const int k0 = kb[0] + kk0;
   const int b = it.y / D.p.W, w = it.y - b * D.p.W;
-  geo_solve_body(D.p, D.G, D.st, ep_slot(D, k0), b, w, 0);
+  geo_solve_body(D.p, D.G, D.st, ep_slot(D, k0), b, w, D.direct);
 }
 
 // inverse columns chunk*64 .. of each (step, wing) item (chunks past the
@@ -1384,13 +1396,26 @@ __global__ __launch_bounds__(CPL_THREADS) void kb_ibm_couple(const EpDesc* __res
   ibm_couple_body(D.p, D.G, ep_slot(D, k), D.F);
 }
 
-void launch_batch_ibm_flow(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32) {
+// y = S^-1 c of every (episode, wing) of the wing list by band substitution,
+// on the steps whose geometry formed no inverse
+__global__ __launch_bounds__(BSOLVE_THREADS) void kb_ibm_bsolve(const EpDesc* __restrict__ eps,
+                                                                const int2* __restrict__ items,
+                                                                const int* __restrict__ kb, int kk) {
+  const int2 it = items[blockIdx.x];
+  const EpDesc& D = eps[it.x];
+  const int k = kb[0] + kk;
+  ibm_bsolve_body(D.p, D.G, ep_slot(D, k), D.F, it.y);
+}
+
+void launch_batch_ibm_flow(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool direct) {
   if (it.n_seg <= 0) return;
   if (f32)
     hipLaunchKernelGGL(kb_ibm_rhs<float>, dim3(it.n_seg), dim3(256), 0, s, eps, it.seg, it.kbase, k);
   else
     hipLaunchKernelGGL(kb_ibm_rhs<double>, dim3(it.n_seg), dim3(256), 0, s, eps, it.seg, it.kbase, k);
   hipLaunchKernelGGL(kb_ibm_matvec, dim3(it.n_seg), dim3(256), 0, s, eps, it.seg, it.kbase, k);
+  if (direct && it.n_wing > 0)
+    hipLaunchKernelGGL(kb_ibm_bsolve, dim3(it.n_wing), dim3(BSOLVE_THREADS), 0, s, eps, it.wing, it.kbase, k);
   hipLaunchKernelGGL(kb_ibm_couple, dim3(it.n_seg), dim3(CPL_THREADS), 0, s, eps, it.seg, it.kbase, k);
   hipLaunchKernelGGL(kb_ibm_spread, dim3(it.n_spread), dim3(SPREAD_THREADS), 0, s, eps, it.spread, it.kbase, k);
 }

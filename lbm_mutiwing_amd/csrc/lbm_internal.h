@@ -121,7 +121,7 @@ struct GeoBuffers {
   double* sinv;                 // [slots][W][SPW][SPW] inverse of S = D D^T, zig-zag order
   double* bandg;                // [slots][W][band_cap] the factors L D L^T of S (band rows, then 1/D)
   double* cpl;                  // [slots][CPL_KT][CPL_KT] coupling matrix M (cpl_n x cpl_n)
-  int* map;                     // [slots][W][2][map_cap] claim maps (owner, lowest position)
+  int* map;                     // [slots][W][map_cap] claim maps (owner segment per node)
   const Frame* frame;           // per step of the current call
   const double* step_table;     // file mode: step_count * ncols
   const double* stream_rows;    // stream mode: start row, target row (2*LBM_MAX_MOTION_COLS)
@@ -138,12 +138,12 @@ __host__ __device__ __forceinline__ size_t band_cap(const IbmParams& p) {
 // maps in the step's geometry slot.  W = 0 selects no node.
 struct AdjMask {
   const GeoStep* gs;
-  const int* map;          // claim maps of the slot, [W][2][map_cap]; owner INT_MAX = free
+  const int* map;          // claim maps of the slot, [W][map_cap]; owner INT_MAX = free
   int W, map_cap;
 };
 
 __host__ __device__ __forceinline__ AdjMask adj_mask(const IbmParams& p, const GeoBuffers& G, int slot) {
-  return AdjMask{G.step + slot, G.map + (size_t)slot * p.W * 2 * (size_t)p.map_cap, p.W, p.map_cap};
+  return AdjMask{G.step + slot, G.map + (size_t)slot * p.W * (size_t)p.map_cap, p.W, p.map_cap};
 }
 
 // Is (i, j) (0-based) an adjoint node of the mask's step?  Rows outside the
@@ -154,7 +154,7 @@ __device__ __forceinline__ bool is_adjoint(const AdjMask& am, int i, int j) {
     const int* r = am.gs->rect[w];
     const int i0 = r[0], i1 = r[1], j0 = r[2], j1 = r[3];
     if (j < j0 || j > j1 || i < i0 || i > i1) continue;
-    const int* M = am.map + (size_t)w * 2 * am.map_cap;
+    const int* M = am.map + (size_t)w * am.map_cap;
     if (M[(i - i0) + (i1 - i0 + 1) * (j - j0)] != INT_MAX) return true;
   }
   return false;
@@ -204,7 +204,8 @@ struct EpDesc {
   int cur0, n0, ns, bc0;
   int tiles;               // lattice tiles of the batch's tile width over the interior rows taken as one run
   int bpw;                 // adjoint-collide blocks per wing
-  int pad0, pad1;
+  int direct;              // 1: band substitution instead of the inverse on steps whose wings do not meet
+  int pad1;
 };
 
 __host__ __device__ __forceinline__ int ep_slot(const EpDesc& D, int k) {
@@ -220,6 +221,7 @@ struct BatchItems {
   const int2* adj; int n_adj;
   const int2* seg; int n_seg;        // k_ibm_rhs / k_ibm_matvec blocks
   const int2* spread; int n_spread;
+  const int2* wing; int n_wing;      // (episode, wing): band substitution of a wing's system
 };
 
 // Band position <-> segment in a wing of n segments: positions 0,1,2,3,...
@@ -311,7 +313,8 @@ void launch_batch_lattice(hipStream_t s, const EpDesc* eps, const BatchItems& it
                           int bx, int rows, int cp, bool masked, hipEvent_t t1 = nullptr, size_t lds = 0);
 void launch_batch_bc_ring(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields);
 void launch_batch_adj_collide(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields);
-void launch_batch_ibm_flow(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32);
+// direct: some episode solves by band substitution (EpDesc::direct)
+void launch_batch_ibm_flow(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool direct);
 void launch_batch_force_row(hipStream_t s, const EpDesc* eps, const int* kb, int count, int k);
 // *p = v (add = false) or *p += v (add = true), one thread, in stream order
 void launch_step_base(hipStream_t s, int* p, int v, bool add);
