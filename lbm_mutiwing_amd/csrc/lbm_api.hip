@@ -111,6 +111,7 @@ struct lbm_handle {
   hipStream_t stream_geo = nullptr;  // IBM geometry of the coming steps
   hipStream_t stream_bc = nullptr;   // boundary ring, concurrent with the IBM flow chain
   hipEvent_t ev_ibm = nullptr, ev_step = nullptr, ev_bc = nullptr;
+  hipEvent_t ev_fr = nullptr, ev_state = nullptr;  // coupled steps: force row done, body state on the host
   hipEvent_t ev_geo[2] = {nullptr, nullptr};  // geometry of a slot set is ready
   hipEvent_t ev_rel[2] = {nullptr, nullptr};  // last step using a slot set has finished
   int device = 0;
@@ -323,6 +324,8 @@ void free_all(lbm_handle* h) {
   if (h->stream_geo) (void)hipStreamDestroy(h->stream_geo);
   if (h->stream_bc) (void)hipStreamDestroy(h->stream_bc);
   if (h->ev_bc) (void)hipEventDestroy(h->ev_bc);
+  if (h->ev_fr) (void)hipEventDestroy(h->ev_fr);
+  if (h->ev_state) (void)hipEventDestroy(h->ev_state);
   for (auto e : h->ev) (void)hipEventDestroy(e);
   h->ev.clear();
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -501,7 +504,9 @@ void tune_instance(lbm_handle* h) {
   h->lat_waves = best & 63;
   h->lat_cp = best >> 16;
   h->lat_lds = (size_t)lattice_lds_for_waves(h->lat_bx, h->lat_waves, lds_cu, lds_blk);
-  const long long dyn = lattice_lds_for_waves(h->lat_bx, DYN_LATTICE_WAVES, lds_cu, lds_blk);
+  int dyn_waves = DYN_LATTICE_WAVES;
+  if (const char* e = getenv("LBM_DYN_WAVES")) dyn_waves = std::max(0, atoi(e));   // 0: no cap
+  const long long dyn = lattice_lds_for_waves(h->lat_bx, dyn_waves, lds_cu, lds_blk);
   h->dyn_lds = dyn >= 0 ? (size_t)dyn : h->lat_lds;
 }
 
@@ -665,6 +670,7 @@ int lbm_create(const lbm_config* cfg, lbm_handle** out) try {
     delete h;
     return rc;
   }
+  ip.geo_stage = h->W > 0 && geo_solve_lds_bytes_staged(ip) <= (size_t)lds_max ? 1 : 0;
   if (h->W > 1 && geo_couple_lds_total() > (size_t)lds_max) {
     int rc = fail(nullptr, LBM_E_INVALID, "the coupling of wings needs %zu B of LDS per workgroup, device allows %d",
                   geo_couple_lds_total(), lds_max);
@@ -687,6 +693,8 @@ int lbm_create(const lbm_config* cfg, lbm_handle** out) try {
       hipEventCreateWithFlags(&h->ev_bc, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&h->ev_ibm, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&h->ev_step, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&h->ev_fr, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&h->ev_state, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&h->ev_geo[0], hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&h->ev_geo[1], hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&h->ev_rel[0], hipEventDisableTiming) != hipSuccess ||
@@ -929,15 +937,22 @@ int step_error(lbm_handle* h, const DevStep& st) {
 int finish_advance(lbm_handle* h, int nsteps, int n_last, bool timed, double* force_rows, double* motion_rows,
                    double* row_sum, double* last_row, double* err);
 
-// Advance of an instance with rigid-body coupling, one step at a time: the
-// geometry of step m needs the body state left by the force of step m-1,
-// and the move_mesh check after step m reads that state on the host.  Each
-// step runs the same kernels as lbm_advance (fields written every step, so
-// that a move can shift them), with its force row, and the body update in
-// it, right after the adjoint collision on the IBM stream.  The lattice
-// kernel does not wait for the step's geometry: it writes every interior
-// node, beside the geometry solve, and the adjoint collision overwrites the
-// adjoint nodes (populations and fields) after it.
+// Advance of an instance with rigid-body coupling: the geometry of step m
+// needs the body state left by the force of step m-1, and the move_mesh check
+// after step m reads that state on the host.  Each step runs the same kernels
+// as lbm_advance (fields written every step, so that a move can shift them),
+// with its force row, and the body update in it, right after the adjoint
+// collision on the IBM stream.  The lattice kernel does not wait for the
+// step's geometry: it writes every interior node, beside the geometry solve,
+// and the adjoint collision overwrites the adjoint nodes (populations and
+// fields) after it.  The steps are pipelined: everything of step m+1 but its
+// force row (the geometry from the frame the force row of step m wrote on the
+// device, the lattice kernel, the ring, the IBM flow and the adjoint
+// collision) is issued before the host reads the body state of step m; the
+// force row of m+1 is issued once that state shows no move.  A move
+// (|body_rel_pos(1)| >= 1) waits for the work of step m+1 in flight, shifts
+// the lattice, uploads the new frame and issues step m+1 again: its kernels
+// overwrite everything the first issue wrote.
 int advance_dynamic(lbm_handle* h, int32_t n_last, double* force_rows, double* motion_rows, double* row_sum,
                     double* last_row, double* err) {
   const int n0 = h->n;
@@ -956,25 +971,34 @@ int advance_dynamic(lbm_handle* h, int32_t n_last, double* force_rows, double* m
   const int ns = h->nslot;
   // the force row of each step writes the next step's frame and body state
   // on the device; the host uploads them at the start and after a move
-  bool upload = true;
   FlowBuffers F = h->F;
   F.frame_out = h->d_frame;
-  for (int m = n0; m <= n_last; m++) {
-    const int k = m - n0;
-    const int set = (int)(h->batch_count & 1u), slot = set * ns, s = m % 3;
-    if (upload) {
-      HIPCHECK(h, hipMemcpy(h->d_frame, &h->fr, sizeof(Frame), hipMemcpyHostToDevice));
-      HIPCHECK(h, hipMemcpy(h->d_st->rel, h->rel, sizeof(h->rel), hipMemcpyHostToDevice));
-      upload = false;
-    }
+  auto upload = [&]() -> int {
+    HIPCHECK(h, hipMemcpy(h->d_frame, &h->fr, sizeof(Frame), hipMemcpyHostToDevice));
+    HIPCHECK(h, hipMemcpy(h->d_st->rel, h->rel, sizeof(h->rel), hipMemcpyHostToDevice));
+    return 0;
+  };
+  auto sync_all = [&]() -> int {
+    for (hipStream_t q : {h->stream_geo, h->stream_bc, h->stream_ibm, h->stream}) HIPCHECK(h, hipStreamSynchronize(q));
+    return 0;
+  };
+  // Everything of step m but its force row, from buffer `cur` into the other
+  // one with geometry slot set `set`.  first: no earlier step is in flight
+  // (the host has synchronised), so nothing is waited for.
+  auto issue_body = [&](int m, int cur, int set, bool first) -> int {
+    const int slot = set * ns, s = m % 3;
+    if (!first) HIPCHECK(h, hipStreamWaitEvent(h->stream_geo, h->ev_fr, 0));   // the frame of step m
     launch_geometry(h->stream_geo, h->ip, h->G, h->d_st, slot, m, 0, 1, true, h->direct >= 1);
     HIPCHECK(h, hipEventRecord(h->ev_geo[set], h->stream_geo));
-    h->batch_count++;
     FieldSlot* fs = &h->slot[s];
-    const void* src = h->fbuf[h->cur];
-    void* dst = h->fbuf[1 - h->cur];
+    const void* src = h->fbuf[cur];
+    void* dst = h->fbuf[1 - cur];
     AdjMask every_node = adj_mask(h->ip, h->G, slot);
     every_node.W = 0;
+    if (!first) {   // the populations of step m-1 are complete
+      HIPCHECK(h, hipStreamWaitEvent(h->stream, h->ev_ibm, 0));
+      HIPCHECK(h, hipStreamWaitEvent(h->stream_bc, h->ev_ibm, 0));
+    }
     // a tighter wave cap than the tuned one: the step's geometry beside the
     // lattice kernel is the critical path here
     launch_lattice(h->stream, h->g, h->mrt, src, dst, fs, h->f32, h->lat_bx, h->lat_cp, every_node, nullptr, nullptr,
@@ -987,33 +1011,68 @@ int advance_dynamic(lbm_handle* h, int32_t n_last, double* force_rows, double* m
     launch_ibm_flow(h->stream_ibm, h->ip, h->mrt, h->G, slot, h->F, h->g, src, h->f32, h->direct >= 1);
     HIPCHECK(h, hipStreamWaitEvent(h->stream_ibm, h->ev_step, 0));   // the lattice kernel's writes first
     launch_adj_collide(h->stream_ibm, h->g, h->mrt, h->ip, h->G, slot, h->F, m, src, dst, fs, h->f32);
-    launch_force_row(h->stream_ibm, h->ip, h->G, slot, F, h->d_st, m, k, h->row_len);
     HIPCHECK(h, hipStreamWaitEvent(h->stream_ibm, h->ev_bc, 0));
-    HIPCHECK(h, hipEventRecord(h->ev_ibm, h->stream_ibm));
-    HIPCHECK(h, hipStreamWaitEvent(h->stream, h->ev_ibm, 0));
-    h->slot_step[s] = m;
-    h->cur = 1 - h->cur;
-    h->last_slot = slot;
-    HIPCHECK(h, hipMemcpyAsync(h->h_st, h->d_st, sizeof(DevStep), hipMemcpyDeviceToHost, h->stream));
-    HIPCHECK(h, hipStreamSynchronize(h->stream));
-    const DevStep& st = *h->h_st;
-    if (st.error != 0) {
-      h->n = m + 1;
-      return step_error(h, st);
+    HIPCHECK(h, hipEventRecord(h->ev_ibm, h->stream_ibm));   // step m's populations are complete
+    return 0;
+  };
+  // the force row of step m (body update, next frame), then its body state to the host
+  auto issue_force = [&](int m, int set) -> int {
+    launch_force_row(h->stream_ibm, h->ip, h->G, set * ns, F, h->d_st, m, m - n0, h->row_len);
+    HIPCHECK(h, hipEventRecord(h->ev_fr, h->stream_ibm));
+    HIPCHECK(h, hipMemcpyAsync(h->h_st, h->d_st, sizeof(DevStep), hipMemcpyDeviceToHost, h->stream_ibm));
+    HIPCHECK(h, hipEventRecord(h->ev_state, h->stream_ibm));
+    return 0;
+  };
+  if (nsteps > 0) {
+    if ((rc = upload())) return rc;
+    const int set0 = (int)(h->batch_count & 1u);
+    h->batch_count++;
+    if ((rc = issue_body(n0, h->cur, set0, true))) return rc;
+    if ((rc = issue_force(n0, set0))) return rc;
+    int set = set0;
+    for (int m = n0; m <= n_last; m++) {
+      const int s = m % 3;
+      // step m+1 on the device while the host waits for the state of step m
+      int set_next = -1;
+      const int cur_next = 1 - h->cur;
+      if (m < n_last) {
+        set_next = (int)(h->batch_count & 1u);
+        h->batch_count++;
+        if ((rc = issue_body(m + 1, cur_next, set_next, false))) return rc;
+      }
+      HIPCHECK(h, hipEventSynchronize(h->ev_state));
+      h->slot_step[s] = m;
+      h->cur = cur_next;
+      h->last_slot = set * ns;
+      const DevStep& st = *h->h_st;
+      if (st.error != 0) {
+        if ((rc = sync_all())) return rc;
+        h->n = m + 1;
+        return step_error(h, st);
+      }
+      for (int q = 0; q < REL_FIELDS; q++)
+        for (int w = 0; w < MAXW; w++) h->rel[q][w] = st.rel[q][w];
+      update_body_extra(h);
+      // the move_mesh check of the run loop (main.f90:393-412)
+      const double brp = h->rel[REL_POS][0];
+      const int dr = brp >= 1.0 ? 1 : (brp <= -1.0 ? -1 : 0);
+      if (dr != 0) {
+        if ((rc = sync_all())) return rc;             // step m+1 as issued is discarded
+        rc = apply_move(h, dr, s, m);
+        if (rc) return rc;
+        HIPCHECK(h, hipStreamSynchronize(h->stream));  // the next step's ring and IBM streams read the shifted lattice
+        if ((rc = upload())) return rc;                // pos and movenum changed on the host
+        // an error the discarded issue raised (its geometry used the frame
+        // before the move) is not the step's
+        HIPCHECK(h, hipMemset(&h->d_st->error, 0, sizeof(int)));
+        if (m < n_last && (rc = issue_body(m + 1, h->cur, set_next, true))) return rc;
+      }
+      if (h->flag2 > 0) h->flag2 = h->flag2 - 1;
+      if (m < n_last) {
+        if ((rc = issue_force(m + 1, set_next))) return rc;
+        set = set_next;
+      }
     }
-    for (int q = 0; q < REL_FIELDS; q++)
-      for (int w = 0; w < MAXW; w++) h->rel[q][w] = st.rel[q][w];
-    update_body_extra(h);
-    // the move_mesh check of the run loop (main.f90:393-412)
-    const double brp = h->rel[REL_POS][0];
-    const int dr = brp >= 1.0 ? 1 : (brp <= -1.0 ? -1 : 0);
-    if (dr != 0) {
-      rc = apply_move(h, dr, s, m);
-      if (rc) return rc;
-      HIPCHECK(h, hipStreamSynchronize(h->stream));  // the next step's ring and IBM streams read the shifted lattice
-      upload = true;                                 // pos and movenum changed on the host
-    }
-    if (h->flag2 > 0) h->flag2 = h->flag2 - 1;
   }
   h->n = std::max(n0, n_last + 1);
   return finish_advance(h, nsteps, n_last, false, force_rows, motion_rows, row_sum, last_row, err);

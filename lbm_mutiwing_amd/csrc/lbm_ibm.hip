@@ -395,6 +395,7 @@ __device__ void band_ldlt_t(double* band, double* invd, int n, int bw, int Dd, i
   }
   double d = band[0];
   int bad = -1;
+  double* sink = invd + n;   // an unused entry: the store of a pair with nothing to update this step
   for (int k = 0; k < n; k++) {
     double* bk = band + (size_t)k * Dd;
     double vt[NPL], va[NPL], vc[NPL];
@@ -407,13 +408,21 @@ __device__ void band_ldlt_t(double* band, double* invd, int n, int bw, int Dd, i
     const double dn = bk[Dd], ln = bk[Dd + 1];   // S(k+1, k+1), S(k+1, k): row k+1 exists (padding rows)
     if (!(d > 0.0) && bad < 0) bad = k;
     const double rd = 1.0 / (d > 0.0 ? d : 1.0);
-#pragma unroll
-    for (int u = 0; u < NPL; u++)
-      if (pi[u] > 0 && k + pi[u] < n) bk[ot[u]] = vt[u] - va[u] * rd * vc[u];
-    if (lane == 0) invd[k] = rd;
     d = dn - ln * rd * ln;   // the (1, 1) update: D_{k+1} (entries beyond bw are zero)
-    wave_sync();
+    // every lane stores (to the sink when its pair has no update), so that
+    // the loads above stay ahead of the division instead of sinking into a
+    // conditional store; 1/D_k is the same in every lane
+#pragma unroll
+    for (int u = 0; u < NPL; u++) {
+      double* dst = pi[u] > 0 && k + pi[u] < n ? bk + ot[u] : sink;
+      *dst = vt[u] - va[u] * rd * vc[u];
+    }
+    invd[k] = rd;
+    // no wait for these stores to complete: the next step's loads may alias
+    // them, so the compiler keeps them after the stores, and the LDS executes
+    // a wave's operations in program order
   }
+  wave_sync();
   if (lane == 0 && bad >= 0) raise_error(st, DERR_NOT_SPD, info + bad);
   // L(k + i, k) = S(k + i, k) / D_k
   for (int e = lane; e < n * bw; e += 64) {
@@ -573,7 +582,7 @@ __device__ __forceinline__ void inverse_column(const double* band, const double*
 // One workgroup per (step, wing): adjoint total, band assembly of S,
 // L D L^T and the explicit inverse (one column per thread).
 __device__ __forceinline__ void geo_solve_body(const IbmParams& p, const GeoBuffers& G, DevStep* st, int slot0, int b,
-                                               int w, int direct) {
+                                               int w, int direct, bool stage) {
   extern __shared__ double lds_s[];
   __shared__ int s_skip, s_bw;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -647,9 +656,27 @@ __device__ __forceinline__ void geo_solve_body(const IbmParams& p, const GeoBuff
   double* invd = band + (size_t)(n + BWc) * Dd;
   const double* phx = G.phx + base * BOXW;
   const double* phy = G.phy + base * BOXW;
+  const int* bxs = box;
+  if (stage) {
+    // the wing's boxes and 1-D weights into LDS past the band (coalesced
+    // loads), so that the entries below read LDS instead of chains of
+    // dependent global loads
+    double* sphx = invd + n + 1;
+    double* sphy = sphx + (size_t)n * BOXW;
+    int* sbx = reinterpret_cast<int*>(sphy + (size_t)n * BOXW);
+    for (int e = t; e < n * BOXW; e += GEO_THREADS) {
+      sphx[e] = phx[e];
+      sphy[e] = phy[e];
+    }
+    for (int e = t; e < 4 * n; e += GEO_THREADS) sbx[e] = box[e];
+    __syncthreads();
+    phx = sphx;
+    phy = sphy;
+    bxs = sbx;
+  }
   for (int e = t; e < (n + BWc) * Dd; e += GEO_THREADS) {
     const int pp = e / Dd, d = e - pp * Dd;
-    band[e] = pp < n && d <= bw && pp - d >= 0 ? s_entry(box, phx, phy, seg_of_pos(pp, n), seg_of_pos(pp - d, n))
+    band[e] = pp < n && d <= bw && pp - d >= 0 ? s_entry(bxs, phx, phy, seg_of_pos(pp, n), seg_of_pos(pp - d, n))
                                                : 0.0;
   }
   __syncthreads();
@@ -708,8 +735,8 @@ __global__ __launch_bounds__(INV_THREADS) void k_geo_inverse(IbmParams p, GeoBuf
 int inverse_chunks(int spw) { return (spw + INV_THREADS - 1) / INV_THREADS; }
 
 __global__ __launch_bounds__(GEO_THREADS) void k_geo_solve(IbmParams p, GeoBuffers G, DevStep* st, int slot0,
-                                                           int direct) {
-  geo_solve_body(p, G, st, slot0, blockIdx.x, blockIdx.y, direct);
+                                                           int direct, int stage) {
+  geo_solve_body(p, G, st, slot0, blockIdx.x, blockIdx.y, direct, stage != 0);
 }
 
 // band of (SPW + c) rows of stride c + 1, c = the band class of the
@@ -717,6 +744,11 @@ __global__ __launch_bounds__(GEO_THREADS) void k_geo_solve(IbmParams p, GeoBuffe
 size_t geo_solve_lds_bytes(const IbmParams& p) {
   const size_t c = (size_t)p.band_cls;
   return ((p.SPW + c) * (c + 1) + p.SPW + 1) * sizeof(double);
+}
+
+// the same with the wing's boxes and 1-D weights staged after it
+size_t geo_solve_lds_bytes_staged(const IbmParams& p) {
+  return geo_solve_lds_bytes(p) + (size_t)p.SPW * (2 * BOXW * sizeof(double) + 4 * sizeof(int));
 }
 
 // ------------------------------------------------------------ coupled wings
@@ -1082,8 +1114,10 @@ void launch_geometry(hipStream_t s, const IbmParams& p, const GeoBuffers& G, Dev
   dim3 grid(count, p.W);
   hipLaunchKernelGGL(k_geo_wing, grid, dim3(GEO_THREADS), geo_wing_lds_bytes(p.SPW), s, p, G, st, slot0, n0, fidx0);
   if (with_inverse) {
-    hipLaunchKernelGGL(k_geo_solve, grid, dim3(GEO_THREADS), geo_solve_lds_bytes(p), s, p, G, st, slot0,
-                       direct ? 1 : 0);
+    // the boxes and weights staged in LDS when the device's LDS per workgroup allows (p.geo_stage)
+    hipLaunchKernelGGL(k_geo_solve, grid, dim3(GEO_THREADS),
+                       p.geo_stage ? geo_solve_lds_bytes_staged(p) : geo_solve_lds_bytes(p), s, p, G, st, slot0,
+                       direct ? 1 : 0, p.geo_stage);
     const int ch = inverse_chunks(p.SPW);
     hipLaunchKernelGGL(k_geo_inverse, dim3(count * ch, p.W), dim3(INV_THREADS), geo_solve_lds_bytes(p), s, p, G, st,
                        slot0, ch);
@@ -1321,7 +1355,7 @@ __global__ __launch_bounds__(GEO_THREADS) void kb_geo_solve(const EpDesc* __rest
   const EpDesc& D = eps[it.x];
   const int k0 = kb[0] + kk0;
   const int b = it.y / D.p.W, w = it.y - b * D.p.W;
-  geo_solve_body(D.p, D.G, D.st, ep_slot(D, k0), b, w, D.direct);
+  geo_solve_body(D.p, D.G, D.st, ep_slot(D, k0), b, w, D.direct, false);
 }
 
 // inverse columns chunk*64 .. of each (step, wing) item (chunks past the
