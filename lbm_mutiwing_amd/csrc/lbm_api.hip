@@ -670,7 +670,6 @@ int lbm_create(const lbm_config* cfg, lbm_handle** out) try {
     delete h;
     return rc;
   }
-  ip.geo_stage = h->W > 0 && geo_solve_lds_bytes_staged(ip) <= (size_t)lds_max ? 1 : 0;
   if (h->W > 1 && geo_couple_lds_total() > (size_t)lds_max) {
     int rc = fail(nullptr, LBM_E_INVALID, "the coupling of wings needs %zu B of LDS per workgroup, device allows %d",
                   geo_couple_lds_total(), lds_max);

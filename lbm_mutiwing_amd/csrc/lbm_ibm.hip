@@ -582,7 +582,7 @@ __device__ __forceinline__ void inverse_column(const double* band, const double*
 // One workgroup per (step, wing): adjoint total, band assembly of S,
 // L D L^T and the explicit inverse (one column per thread).
 __device__ __forceinline__ void geo_solve_body(const IbmParams& p, const GeoBuffers& G, DevStep* st, int slot0, int b,
-                                               int w, int direct, bool stage) {
+                                               int w, int direct) {
   extern __shared__ double lds_s[];
   __shared__ int s_skip, s_bw;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -656,27 +656,9 @@ __device__ __forceinline__ void geo_solve_body(const IbmParams& p, const GeoBuff
   double* invd = band + (size_t)(n + BWc) * Dd;
   const double* phx = G.phx + base * BOXW;
   const double* phy = G.phy + base * BOXW;
-  const int* bxs = box;
-  if (stage) {
-    // the wing's boxes and 1-D weights into LDS past the band (coalesced
-    // loads), so that the entries below read LDS instead of chains of
-    // dependent global loads
-    double* sphx = invd + n + 1;
-    double* sphy = sphx + (size_t)n * BOXW;
-    int* sbx = reinterpret_cast<int*>(sphy + (size_t)n * BOXW);
-    for (int e = t; e < n * BOXW; e += GEO_THREADS) {
-      sphx[e] = phx[e];
-      sphy[e] = phy[e];
-    }
-    for (int e = t; e < 4 * n; e += GEO_THREADS) sbx[e] = box[e];
-    __syncthreads();
-    phx = sphx;
-    phy = sphy;
-    bxs = sbx;
-  }
   for (int e = t; e < (n + BWc) * Dd; e += GEO_THREADS) {
     const int pp = e / Dd, d = e - pp * Dd;
-    band[e] = pp < n && d <= bw && pp - d >= 0 ? s_entry(bxs, phx, phy, seg_of_pos(pp, n), seg_of_pos(pp - d, n))
+    band[e] = pp < n && d <= bw && pp - d >= 0 ? s_entry(box, phx, phy, seg_of_pos(pp, n), seg_of_pos(pp - d, n))
                                                : 0.0;
   }
   __syncthreads();
@@ -735,8 +717,8 @@ __global__ __launch_bounds__(INV_THREADS) void k_geo_inverse(IbmParams p, GeoBuf
 int inverse_chunks(int spw) { return (spw + INV_THREADS - 1) / INV_THREADS; }
 
 __global__ __launch_bounds__(GEO_THREADS) void k_geo_solve(IbmParams p, GeoBuffers G, DevStep* st, int slot0,
-                                                           int direct, int stage) {
-  geo_solve_body(p, G, st, slot0, blockIdx.x, blockIdx.y, direct, stage != 0);
+                                                           int direct) {
+  geo_solve_body(p, G, st, slot0, blockIdx.x, blockIdx.y, direct);
 }
 
 // band of (SPW + c) rows of stride c + 1, c = the band class of the
@@ -744,11 +726,6 @@ __global__ __launch_bounds__(GEO_THREADS) void k_geo_solve(IbmParams p, GeoBuffe
 size_t geo_solve_lds_bytes(const IbmParams& p) {
   const size_t c = (size_t)p.band_cls;
   return ((p.SPW + c) * (c + 1) + p.SPW + 1) * sizeof(double);
-}
-
-// the same with the wing's boxes and 1-D weights staged after it
-size_t geo_solve_lds_bytes_staged(const IbmParams& p) {
-  return geo_solve_lds_bytes(p) + (size_t)p.SPW * (2 * BOXW * sizeof(double) + 4 * sizeof(int));
 }
 
 // ------------------------------------------------------------ coupled wings
@@ -1114,10 +1091,8 @@ void launch_geometry(hipStream_t s, const IbmParams& p, const GeoBuffers& G, Dev
   dim3 grid(count, p.W);
   hipLaunchKernelGGL(k_geo_wing, grid, dim3(GEO_THREADS), geo_wing_lds_bytes(p.SPW), s, p, G, st, slot0, n0, fidx0);
   if (with_inverse) {
-    // the boxes and weights staged in LDS when the device's LDS per workgroup allows (p.geo_stage)
-    hipLaunchKernelGGL(k_geo_solve, grid, dim3(GEO_THREADS),
-                       p.geo_stage ? geo_solve_lds_bytes_staged(p) : geo_solve_lds_bytes(p), s, p, G, st, slot0,
-                       direct ? 1 : 0, p.geo_stage);
+    hipLaunchKernelGGL(k_geo_solve, grid, dim3(GEO_THREADS), geo_solve_lds_bytes(p), s, p, G, st, slot0,
+                       direct ? 1 : 0);
     const int ch = inverse_chunks(p.SPW);
     hipLaunchKernelGGL(k_geo_inverse, dim3(count * ch, p.W), dim3(INV_THREADS), geo_solve_lds_bytes(p), s, p, G, st,
                        slot0, ch);
@@ -1355,7 +1330,7 @@ __global__ __launch_bounds__(GEO_THREADS) void kb_geo_solve(const EpDesc* __rest
   const EpDesc& D = eps[it.x];
   const int k0 = kb[0] + kk0;
   const int b = it.y / D.p.W, w = it.y - b * D.p.W;
-  geo_solve_body(D.p, D.G, D.st, ep_slot(D, k0), b, w, D.direct, false);
+  geo_solve_body(D.p, D.G, D.st, ep_slot(D, k0), b, w, D.direct);
 }
 
 // inverse columns chunk*64 .. of each (step, wing) item (chunks past the

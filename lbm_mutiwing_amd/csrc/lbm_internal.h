@@ -96,7 +96,6 @@ struct IbmParams {
   int step_count;          // rows of the file-mode step table
   int ncols;
   int band_cls;            // band_class() of a bound on every step's band half-width
-  int geo_stage;           // 1: the single-instance solve stages boxes and weights in LDS (it fits)
   double motion_dt;
   // rigid-body coupling of wing 1 (boundary_condition.f90:1694-1719)
   int dyn_on, dyn_rk4;
@@ -303,7 +302,6 @@ int adj_collide_blocks_per_wing(const IbmParams& p);
 void launch_geometry(hipStream_t s, const IbmParams& p, const GeoBuffers& G, DevStep* st, int slot0, int n0,
                      int fidx0, int count, bool with_inverse, bool direct = false);
 size_t geo_solve_lds_bytes(const IbmParams& p);
-size_t geo_solve_lds_bytes_staged(const IbmParams& p);   // with the wing's boxes and weights
 size_t geo_couple_lds_total();   // static + dynamic LDS of the coupling workgroup
 // Batched launches over the episodes of eps (device array).  The substep of a
 // launch is *it.kbase + k (the base lives on the device, so that a captured
