@@ -146,7 +146,6 @@ __device__ __forceinline__ void lattice_f32_node(const Grid& g, const MrtParams&
   const float ex = t3 - (t4 + t4), ey = t5 - (t6 + t6);
   const float dx = t3 + t4, dy = t5 + t6;
   float out[9];
-  out[0] = t0 + 4.0f * (t2 - t1);
   out[1] = pa + ex + t7;
   out[3] = pa - ex + t7;
   out[2] = pa + ey - t7;
@@ -155,6 +154,14 @@ __device__ __forceinline__ void lattice_f32_node(const Grid& g, const MrtParams&
   out[7] = qd - (dx + dy) + t8;
   out[6] = qd + (dy - dx) - t8;
   out[8] = qd - (dy - dx) - t8;
+  // with s_0 = 0 (rk0 = 0: the collision conserves the density moment, the
+  // reference's rates) the rest population is the remainder of the mass:
+  // t0 + 4 (t2 - t1) in exact arithmetic, and the nine deviations then sum
+  // to rp up to one rounding, with no systematic FP32 mass drift from the
+  // rounded 1/9 in t0
+  out[0] = P.rk0 == 0.0f
+               ? rp - (((out[1] + out[3]) + (out[2] + out[4])) + ((out[5] + out[7]) + (out[6] + out[8])))
+               : t0 + 4.0f * (t2 - t1);
   const long long o = (long long)j * g.nx + i;
 #pragma unroll
   for (int k = 0; k < 9; k++) st_pop<CP>(dst + (long long)k * g.ps + o, out[k]);
