@@ -111,7 +111,7 @@ struct lbm_handle {
   hipStream_t stream_geo = nullptr;  // IBM geometry of the coming steps
   hipStream_t stream_bc = nullptr;   // boundary ring, concurrent with the IBM flow chain
   hipEvent_t ev_ibm = nullptr, ev_step = nullptr, ev_bc = nullptr;
-  hipEvent_t ev_fr = nullptr, ev_state = nullptr;  // coupled steps: force row done, body state on the host
+  hipEvent_t ev_state = nullptr;   // coupled steps: body state on the host
   hipEvent_t ev_geo[2] = {nullptr, nullptr};  // geometry of a slot set is ready
   hipEvent_t ev_rel[2] = {nullptr, nullptr};  // last step using a slot set has finished
   int device = 0;
@@ -271,6 +271,7 @@ const char* dev_error_text(int code) {
     case DERR_BOX_EDGE: return "IBM support box of a segment reaches within 3 nodes of the lattice edge";
     case DERR_MAP: return "support rectangle of a wing exceeds its claim map";
     case DERR_BAND: return "IBM band half-width exceeds the solver limit";
+    case DERR_WINGS_MET: return "wings' support rectangles met on a step launched as apart";
     case DERR_WINGS_COUPLED:
       return "more than 32 segments of one wing, or 64 in all, couple wings whose supports share lattice nodes";
     case DERR_NOT_SPD: return "IBM matrix lost positive definiteness";
@@ -324,7 +325,6 @@ void free_all(lbm_handle* h) {
   if (h->stream_geo) (void)hipStreamDestroy(h->stream_geo);
   if (h->stream_bc) (void)hipStreamDestroy(h->stream_bc);
   if (h->ev_bc) (void)hipEventDestroy(h->ev_bc);
-  if (h->ev_fr) (void)hipEventDestroy(h->ev_fr);
   if (h->ev_state) (void)hipEventDestroy(h->ev_state);
   for (auto e : h->ev) (void)hipEventDestroy(e);
   h->ev.clear();
@@ -457,9 +457,11 @@ int lds_per_block(int device) {
 }
 
 // Lattice waves per CU beside the geometry of a step with rigid-body
-// coupling (one 256-wide workgroup per CU): the geometry workgroups on the
-// step's critical path need LDS and wave slots.
-constexpr int DYN_LATTICE_WAVES = 4;
+// coupling (two 256-wide workgroups per CU): the geometry workgroups on the
+// step's critical path need LDS and wave slots.  Measured at 4096x2048 with
+// two wings: caps 4, 6, 8 and 12 give 0.386, 0.385, 0.381 and 0.446 ms per
+// coupled step.
+constexpr int DYN_LATTICE_WAVES = 8;
 
 // Caps on the lattice waves per CU the autotuner tries (0: none): fewer
 // resident lattice waves leave room for the IBM and ring kernels beside it.
@@ -692,7 +694,6 @@ int lbm_create(const lbm_config* cfg, lbm_handle** out) try {
       hipEventCreateWithFlags(&h->ev_bc, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&h->ev_ibm, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&h->ev_step, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&h->ev_fr, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&h->ev_state, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&h->ev_geo[0], hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&h->ev_geo[1], hipEventDisableTiming) != hipSuccess ||
@@ -941,17 +942,18 @@ int finish_advance(lbm_handle* h, int nsteps, int n_last, bool timed, double* fo
 // after step m reads that state on the host.  Each step runs the same kernels
 // as lbm_advance (fields written every step, so that a move can shift them),
 // with its force row, and the body update in it, right after the adjoint
-// collision on the IBM stream.  The lattice kernel does not wait for the
-// step's geometry: it writes every interior node, beside the geometry solve,
-// and the adjoint collision overwrites the adjoint nodes (populations and
-// fields) after it.  The steps are pipelined: everything of step m+1 but its
-// force row (the geometry from the frame the force row of step m wrote on the
-// device, the lattice kernel, the ring, the IBM flow and the adjoint
-// collision) is issued before the host reads the body state of step m; the
-// force row of m+1 is issued once that state shows no move.  A move
-// (|body_rel_pos(1)| >= 1) waits for the work of step m+1 in flight, shifts
-// the lattice, uploads the new frame and issues step m+1 again: its kernels
-// overwrite everything the first issue wrote.
+// collision.  The step's chain (geometry, IBM flow, adjoint collision, force
+// row) runs in order on one stream, with no event between its kernels; the
+// lattice kernel does not wait for the step's geometry: it writes every
+// interior node, beside the chain, and the adjoint collision overwrites the
+// adjoint nodes (populations and fields) after it.  The steps are pipelined:
+// everything of step m+1 but its force row is issued before the host reads
+// the body state of step m; the force row of m+1 is issued once that state
+// shows no move.  A move (|body_rel_pos(1)| >= 1) waits for the work of step
+// m+1 in flight, shifts the lattice, uploads the new frame and issues step
+// m+1 again: its kernels overwrite everything the first issue wrote.  While
+// the wings' support rectangles stay more than APART_GAP nodes apart, a step
+// launches neither the inverse nor the coupling pass.
 int advance_dynamic(lbm_handle* h, int32_t n_last, double* force_rows, double* motion_rows, double* row_sum,
                     double* last_row, double* err) {
   const int n0 = h->n;
@@ -968,6 +970,7 @@ int advance_dynamic(lbm_handle* h, int32_t n_last, double* force_rows, double* m
   if (!h->h_st) HIPCHECK(h, hipHostMalloc((void**)&h->h_st, sizeof(DevStep), hipHostMallocDefault));
   h->carry.valid = false;   // these steps reuse the slot sets
   const int ns = h->nslot;
+  hipStream_t chain = h->stream_geo;   // geometry, IBM flow, adjoint collision, force row
   // the force row of each step writes the next step's frame and body state
   // on the device; the host uploads them at the start and after a move
   FlowBuffers F = h->F;
@@ -981,24 +984,28 @@ int advance_dynamic(lbm_handle* h, int32_t n_last, double* force_rows, double* m
     for (hipStream_t q : {h->stream_geo, h->stream_bc, h->stream_ibm, h->stream}) HIPCHECK(h, hipStreamSynchronize(q));
     return 0;
   };
+  // the smallest gap between the wings' support rectangles last read on the
+  // host (two steps before the one being issued, or one after a move)
+  int gap = h->W < 2 ? INT_MAX : INT_MIN;
   // Everything of step m but its force row, from buffer `cur` into the other
   // one with geometry slot set `set`.  first: no earlier step is in flight
-  // (the host has synchronised), so nothing is waited for.
+  // (the host has synchronised), so the lattice and ring wait for nothing.
   auto issue_body = [&](int m, int cur, int set, bool first) -> int {
     const int slot = set * ns, s = m % 3;
-    if (!first) HIPCHECK(h, hipStreamWaitEvent(h->stream_geo, h->ev_fr, 0));   // the frame of step m
-    launch_geometry(h->stream_geo, h->ip, h->G, h->d_st, slot, m, 0, 1, true, h->direct >= 1);
-    HIPCHECK(h, hipEventRecord(h->ev_geo[set], h->stream_geo));
+    const bool apart = h->W < 2 || gap > APART_GAP;
+    launch_geometry(chain, h->ip, h->G, h->d_st, slot, m, 0, 1, true, h->direct >= 1, apart);
     FieldSlot* fs = &h->slot[s];
     const void* src = h->fbuf[cur];
     void* dst = h->fbuf[1 - cur];
     AdjMask every_node = adj_mask(h->ip, h->G, slot);
     every_node.W = 0;
-    if (!first) {   // the populations of step m-1 are complete
-      HIPCHECK(h, hipStreamWaitEvent(h->stream, h->ev_ibm, 0));
-      HIPCHECK(h, hipStreamWaitEvent(h->stream_bc, h->ev_ibm, 0));
+    if (!first) {   // the populations of step m-1 are complete: adjoint nodes and ring
+      for (hipStream_t q : {h->stream, h->stream_bc}) {
+        HIPCHECK(h, hipStreamWaitEvent(q, h->ev_ibm, 0));
+        HIPCHECK(h, hipStreamWaitEvent(q, h->ev_bc, 0));
+      }
     }
-    // a tighter wave cap than the tuned one: the step's geometry beside the
+    // a tighter wave cap than the tuned one: the step's chain beside the
     // lattice kernel is the critical path here
     launch_lattice(h->stream, h->g, h->mrt, src, dst, fs, h->f32, h->lat_bx, h->lat_cp, every_node, nullptr, nullptr,
                    h->dyn_lds);
@@ -1006,18 +1013,19 @@ int advance_dynamic(lbm_handle* h, int32_t n_last, double* force_rows, double* m
     HIPCHECK(h, hipEventRecord(h->ev_step, h->stream));
     launch_bc_ring(h->stream_bc, h->g, h->mrt, h->R, src, dst, fs, h->f32);
     HIPCHECK(h, hipEventRecord(h->ev_bc, h->stream_bc));
-    HIPCHECK(h, hipStreamWaitEvent(h->stream_ibm, h->ev_geo[set], 0));
-    launch_ibm_flow(h->stream_ibm, h->ip, h->mrt, h->G, slot, h->F, h->g, src, h->f32, h->direct >= 1);
-    HIPCHECK(h, hipStreamWaitEvent(h->stream_ibm, h->ev_step, 0));   // the lattice kernel's writes first
-    launch_adj_collide(h->stream_ibm, h->g, h->mrt, h->ip, h->G, slot, h->F, m, src, dst, fs, h->f32);
-    HIPCHECK(h, hipStreamWaitEvent(h->stream_ibm, h->ev_bc, 0));
-    HIPCHECK(h, hipEventRecord(h->ev_ibm, h->stream_ibm));   // step m's populations are complete
+    launch_ibm_flow(chain, h->ip, h->mrt, h->G, slot, h->F, h->g, src, h->f32, h->direct >= 1, apart);
+    HIPCHECK(h, hipStreamWaitEvent(chain, h->ev_step, 0));   // the lattice kernel's writes first
+    launch_adj_collide(chain, h->g, h->mrt, h->ip, h->G, slot, h->F, m, src, dst, fs, h->f32);
+    HIPCHECK(h, hipEventRecord(h->ev_ibm, chain));   // step m's adjoint nodes are complete (the ring: ev_bc)
     return 0;
   };
-  // the force row of step m (body update, next frame), then its body state to the host
+  // the force row of step m (body update, next frame), then its body state
+  // to the host from another stream, so that the chain goes on to the next
+  // step's geometry
   auto issue_force = [&](int m, int set) -> int {
-    launch_force_row(h->stream_ibm, h->ip, h->G, set * ns, F, h->d_st, m, m - n0, h->row_len);
-    HIPCHECK(h, hipEventRecord(h->ev_fr, h->stream_ibm));
+    launch_force_row(chain, h->ip, h->G, set * ns, F, h->d_st, m, m - n0, h->row_len);
+    HIPCHECK(h, hipEventRecord(h->ev_state, chain));
+    HIPCHECK(h, hipStreamWaitEvent(h->stream_ibm, h->ev_state, 0));
     HIPCHECK(h, hipMemcpyAsync(h->h_st, h->d_st, sizeof(DevStep), hipMemcpyDeviceToHost, h->stream_ibm));
     HIPCHECK(h, hipEventRecord(h->ev_state, h->stream_ibm));
     return 0;
@@ -1049,6 +1057,7 @@ int advance_dynamic(lbm_handle* h, int32_t n_last, double* force_rows, double* m
         h->n = m + 1;
         return step_error(h, st);
       }
+      if (h->W >= 2) gap = st.rect_gap;
       for (int q = 0; q < REL_FIELDS; q++)
         for (int w = 0; w < MAXW; w++) h->rel[q][w] = st.rel[q][w];
       update_body_extra(h);

@@ -582,7 +582,7 @@ __device__ __forceinline__ void inverse_column(const double* band, const double*
 // One workgroup per (step, wing): adjoint total, band assembly of S,
 // L D L^T and the explicit inverse (one column per thread).
 __device__ __forceinline__ void geo_solve_body(const IbmParams& p, const GeoBuffers& G, DevStep* st, int slot0, int b,
-                                               int w, int direct) {
+                                               int w, int direct, int apart) {
   extern __shared__ double lds_s[];
   __shared__ int s_skip, s_bw;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -604,14 +604,18 @@ __device__ __forceinline__ void geo_solve_body(const IbmParams& p, const GeoBuff
       gs->band[1] = b1;
       // the coupling of wings (k_geo_couple) reads the inverse; it needs
       // wings that share a node, so support rectangles that meet
-      int meet = 0;
+      int meet = 0, gap = INT_MAX;
       for (int q = 0; q < p.W; q++)
         for (int r = q + 1; r < p.W; r++) {
           const int* a = gs->rect[q];
           const int* c = gs->rect[r];
-          meet |= a[1] >= a[0] && c[1] >= c[0] && a[0] <= c[1] && c[0] <= a[1] && a[2] <= c[3] && c[2] <= a[3];
+          if (a[1] < a[0] || c[1] < c[0]) continue;   // a wing with no nodes
+          meet |= a[0] <= c[1] && c[0] <= a[1] && a[2] <= c[3] && c[2] <= a[3];
+          gap = min(gap, max(max(a[0] - c[1], c[0] - a[1]), max(a[2] - c[3], c[2] - a[3])));
         }
       gs->inv_on = !direct || meet;
+      st->rect_gap = gap;
+      if (apart && meet) raise_error(st, DERR_WINGS_MET, gs->n);
     }
   }
   __syncthreads();
@@ -717,8 +721,8 @@ __global__ __launch_bounds__(INV_THREADS) void k_geo_inverse(IbmParams p, GeoBuf
 int inverse_chunks(int spw) { return (spw + INV_THREADS - 1) / INV_THREADS; }
 
 __global__ __launch_bounds__(GEO_THREADS) void k_geo_solve(IbmParams p, GeoBuffers G, DevStep* st, int slot0,
-                                                           int direct) {
-  geo_solve_body(p, G, st, slot0, blockIdx.x, blockIdx.y, direct);
+                                                           int direct, int apart) {
+  geo_solve_body(p, G, st, slot0, blockIdx.x, blockIdx.y, direct, apart);
 }
 
 // band of (SPW + c) rows of stride c + 1, c = the band class of the
@@ -1086,13 +1090,14 @@ __global__ __launch_bounds__(CPL_THREADS) void k_ibm_couple(IbmParams p, GeoBuff
 }
 
 void launch_geometry(hipStream_t s, const IbmParams& p, const GeoBuffers& G, DevStep* st, int slot0, int n0,
-                     int fidx0, int count, bool with_inverse, bool direct) {
+                     int fidx0, int count, bool with_inverse, bool direct, bool apart) {
   if (p.W <= 0 || count <= 0) return;
   dim3 grid(count, p.W);
   hipLaunchKernelGGL(k_geo_wing, grid, dim3(GEO_THREADS), geo_wing_lds_bytes(p.SPW), s, p, G, st, slot0, n0, fidx0);
   if (with_inverse) {
     hipLaunchKernelGGL(k_geo_solve, grid, dim3(GEO_THREADS), geo_solve_lds_bytes(p), s, p, G, st, slot0,
-                       direct ? 1 : 0);
+                       direct ? 1 : 0, direct && apart ? 1 : 0);
+    if (direct && apart) return;   // no step of the launch forms the inverse or couples wings
     const int ch = inverse_chunks(p.SPW);
     hipLaunchKernelGGL(k_geo_inverse, dim3(count * ch, p.W), dim3(INV_THREADS), geo_solve_lds_bytes(p), s, p, G, st,
                        slot0, ch);
@@ -1295,16 +1300,17 @@ __global__ __launch_bounds__(SPREAD_THREADS) void k_ibm_spread(IbmParams p, GeoB
 }
 
 void launch_ibm_flow(hipStream_t s, const IbmParams& p, const MrtParams& P, const GeoBuffers& G, int slot,
-                     const FlowBuffers& F, const Grid& g, const void* src, bool f32, bool direct) {
+                     const FlowBuffers& F, const Grid& g, const void* src, bool f32, bool direct, bool apart) {
   if (p.W <= 0) return;
   const int blk = (p.nseg + 3) / 4;
+  const bool sub_only = direct && apart;   // the geometry formed no inverse and no wings couple
   if (f32)
     hipLaunchKernelGGL(k_ibm_rhs<float>, dim3(blk), dim3(256), 0, s, p, P, G, slot, F, g, (const float*)src);
   else
     hipLaunchKernelGGL(k_ibm_rhs<double>, dim3(blk), dim3(256), 0, s, p, P, G, slot, F, g, (const double*)src);
-  hipLaunchKernelGGL(k_ibm_matvec, dim3(blk), dim3(256), 0, s, p, G, slot, F);
+  if (!sub_only) hipLaunchKernelGGL(k_ibm_matvec, dim3(blk), dim3(256), 0, s, p, G, slot, F);
   if (direct) hipLaunchKernelGGL(k_ibm_bsolve, dim3(p.W), dim3(BSOLVE_THREADS), 0, s, p, G, slot, F);
-  if (p.W >= 2) hipLaunchKernelGGL(k_ibm_couple, dim3(1), dim3(CPL_THREADS), 0, s, p, G, slot, F);
+  if (p.W >= 2 && !sub_only) hipLaunchKernelGGL(k_ibm_couple, dim3(1), dim3(CPL_THREADS), 0, s, p, G, slot, F);
   const int nb = (p.SPW + SPREAD_SEG - 1) / SPREAD_SEG;
   hipLaunchKernelGGL(k_ibm_spread, dim3(p.W * nb), dim3(SPREAD_THREADS), 0, s, p, G, slot, F);
 }
@@ -1330,7 +1336,7 @@ __global__ __launch_bounds__(GEO_THREADS) void kb_geo_solve(const EpDesc* __rest
   const EpDesc& D = eps[it.x];
   const int k0 = kb[0] + kk0;
   const int b = it.y / D.p.W, w = it.y - b * D.p.W;
-  geo_solve_body(D.p, D.G, D.st, ep_slot(D, k0), b, w, D.direct);
+  geo_solve_body(D.p, D.G, D.st, ep_slot(D, k0), b, w, D.direct, 0);
 }
 
 // inverse columns chunk*64 .. of each (step, wing) item (chunks past the

@@ -27,6 +27,7 @@ enum DevError : int {
   DERR_WINGS_COUPLED = 4,  // more coupling segments than CPL_K of a wing / CPL_KT in all
   DERR_NOT_SPD = 5,        // non-positive pivot in the band factorisation
   DERR_MOTION = 6,         // motion source not ready
+  DERR_WINGS_MET = 7,      // wings' support rectangles met on a step launched without the coupling pass
 };
 
 struct WingMotionState {
@@ -45,6 +46,8 @@ enum BodyRel : int {
 struct DevStep {
   int error;
   int error_info;
+  int rect_gap;            // smallest gap in nodes between two wings' support rectangles of the last solved step
+  int pad;
   double force[MAXW][2];   // normalised per-wing force of the last step
   double moment[MAXW];
   double rel[REL_FIELDS][MAXW];
@@ -299,8 +302,13 @@ int adj_collide_blocks_per_wing(const IbmParams& p);
 // steps n0.. into slots slot0..; frames are indexed from fidx0.
 // direct: the flow solves by band substitution when the step allows it (no
 // explicit inverse); otherwise the inverse is formed
+// apart: the caller knows that no two wings' support rectangles meet at these
+// steps (their gap was more than APART_GAP nodes a step or two before, and
+// wings move less than a node per step): the inverse and the coupling pass
+// are not launched, and a step whose rectangles meet raises DERR_WINGS_MET
 void launch_geometry(hipStream_t s, const IbmParams& p, const GeoBuffers& G, DevStep* st, int slot0, int n0,
-                     int fidx0, int count, bool with_inverse, bool direct = false);
+                     int fidx0, int count, bool with_inverse, bool direct = false, bool apart = false);
+constexpr int APART_GAP = 4;
 size_t geo_solve_lds_bytes(const IbmParams& p);
 size_t geo_couple_lds_total();   // static + dynamic LDS of the coupling workgroup
 // Batched launches over the episodes of eps (device array).  The substep of a
@@ -330,6 +338,7 @@ int spread_blocks(const IbmParams& p);
 // IBM flow kernels of one step (direct: the step's geometry was launched
 // with direct, so that its system may be solved by band substitution)
 void launch_ibm_flow(hipStream_t s, const IbmParams& p, const MrtParams& P, const GeoBuffers& G, int slot,
-                     const FlowBuffers& F, const Grid& g, const void* src, bool f32, bool direct = false);
+                     const FlowBuffers& F, const Grid& g, const void* src, bool f32, bool direct = false,
+                     bool apart = false);
 
 }  // namespace lbm
