@@ -626,25 +626,25 @@ int lbm_create(const lbm_config* cfg, lbm_handle** out) try {
     const double utwo = u1 * u1 + v1 * v1;
     h->feq[k] = h->w[k] * 1.0 * (1.0 + 3.0 * eu + 4.5 * eu * eu - 1.5 * utwo);
   }
-  // constants of the FP32-shifted storage: weights, rounded rates and inverse,
-  // and the equilibrium moments of the rest state M w
+  // constants of the FP32-shifted storage: weights, scaled rates, and the
+  // equilibrium moments of the rest state M w
   {
     static const double kC[9] = {1.0 / 9, 1.0 / 36, 1.0 / 36, 1.0 / 6, 1.0 / 12, 1.0 / 6, 1.0 / 12, 0.25, 0.25};
     double mw[9], ws = 0.0;
     for (int k = 0; k < 9; k++) {
       h->mrt.w[k] = h->w[k];
       ws += h->w[k];
-      h->mrt.ra32[k] = (float)(kC[k] * (1.0 - h->mrt.s[k]));
-      h->mrt.rb32[k] = (float)(kC[k] * h->mrt.s[k]);
+      h->mrt.ra[k] = kC[k] * (1.0 - h->mrt.s[k]);
+      h->mrt.rb[k] = kC[k] * h->mrt.s[k];
     }
     for (int r = 0; r < 9; r++) {
       mw[r] = 0.0;
       for (int c = 0; c < 9; c++) mw[r] += kMatM[r][c] * h->w[c];
     }
     h->mrt.wsum = ws;
-    h->mrt.rk0 = (float)(-kC[0] * h->mrt.s[0] * ws);
-    h->mrt.eq1 = (float)(-2.0 * ws - mw[1]);
-    h->mrt.eq2 = (float)(ws - mw[2]);
+    h->mrt.rk0 = -kC[0] * h->mrt.s[0] * ws;
+    h->mrt.eq1 = -2.0 * ws - mw[1];
+    h->mrt.eq2 = ws - mw[2];
   }
   h->g.nx = cfg->nx;
   h->g.ny = cfg->ny;

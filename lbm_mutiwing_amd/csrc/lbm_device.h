@@ -26,12 +26,12 @@ struct MrtParams {
   double cs;        // sqrt(3.0)/3 with a single-precision sqrt (boundary_condition.f90:304)
   double w[9];      // lattice weights (single-precision literals, D2Q9model.f90:21-29)
   // FP32-shifted storage: populations are kept as d = f - w in FP32 and the
-  // bulk collision runs on d in FP32 with these rounded constants.  With
+  // bulk collision runs on d in FP64 with these constants.  With
   // c = (1/9, 1/36, 1/36, 1/6, 1/12, 1/6, 1/12, 1/4, 1/4) (M^-1 = M^T diag c):
-  float ra32[9];    // c_r (1 - s_r)
-  float rb32[9];    // c_r s_r
-  float rk0;        // -c_0 s_0 sum w: the density's equilibrium moment is 0 (solution.f90:613)
-  float eq1, eq2;   // -2 sum w - (M w)_1 and sum w - (M w)_2: equilibrium moments minus those of w
+  double ra[9];     // c_r (1 - s_r)
+  double rb[9];     // c_r s_r
+  double rk0;       // -c_0 s_0 sum w: the density's equilibrium moment is 0 (solution.f90:613)
+  double eq1, eq2;  // -2 sum w - (M w)_1 and sum w - (M w)_2: equilibrium moments minus those of w
   double wsum;      // sum of the weights (1 + 7.45e-9 with single-precision literals)
 };
 

@@ -10,9 +10,10 @@
 // pass waits for the other.
 //
 // Every kernel that touches populations is a template on the storage type S:
-// double (FP64 values) or float (FP32 deviations d = f - w_k).  Only the bulk
-// collision of the FP32 storage computes in FP32; the boundary ring, the
-// adjoint nodes and all reductions compute in FP64 on loaded f = w_k + d.
+// double (FP64 values) or float (FP32 deviations d = f - w_k).  Every
+// kernel computes in FP64: the bulk collision of the FP32 storage on the
+// loaded deviations, the boundary ring, the adjoint nodes and all reductions
+// on loaded f = w_k + d.
 #include <algorithm>
 #include <mutex>
 
@@ -93,83 +94,89 @@ __device__ __forceinline__ void flat_node(const Grid& g, int blk, int t, int& i,
   }
 }
 
-// FP32 bulk step on deviations d = f - w from the rest equilibrium, in FP32.
-// The moments m' = M d come from shared partial sums of the nine loads
-// (M of solution.f90:17-25); the weights carry no momentum, so j = sum e d,
-// and with the incompressible equilibrium (solution.f90:613-624) rho u = j,
-// so the equilibrium moments need no division: e_eq = -2 rho + 3 j^2,
-// eps_eq = rho - 3 j^2, q_eq = -j, p_eq = (jx^2 - jy^2, jx jy), each shifted
-// by (M w) (rho = sum w + sum d; MrtParams eq1, eq2).  Relaxation and the
-// inverse transform are linear: with M^-1 = M^T diag(c), c = (1/9, 1/36,
-// 1/36, 1/6, 1/12, 1/6, 1/12, 1/4, 1/4), the scaled relaxed moments are
-// tt_r = c_r (1 - s_r) m_r + c_r s_r m_eq,r (MrtParams ra32, rb32, rk0) and
-// d* = M^T tt, again from shared sums.  About 90 FP32 operations per node.
+// Bulk step of the FP32-shifted storage: deviations d = f - w from the rest
+// equilibrium, loaded as FP32 and collided in FP64, so that the only rounding
+// a step adds beyond FP64's is that of the nine FP32 stores.  (Colliding in
+// FP32 as well was 7 % faster but its rounding is biased: at 4001x2001, Re
+// 1000, the window-averaged forces after 4000 steps moved by up to 8.5e-3 of
+// a value, against 9e-5 with the FP64 collision.)  The moments m' = M d come
+// from shared partial sums of the nine loads (M of solution.f90:17-25); the
+// weights carry no momentum, so j = sum e d, and with the incompressible
+// equilibrium (solution.f90:613-624) rho u = j, so the equilibrium moments
+// need no division: e_eq = -2 rho + 3 j^2, eps_eq = rho - 3 j^2, q_eq = -j,
+// p_eq = (jx^2 - jy^2, jx jy), each shifted by (M w) (rho = sum w + sum d;
+// MrtParams eq1, eq2).  Relaxation and the inverse transform are linear: with
+// M^-1 = M^T diag(c), c = (1/9, 1/36, 1/36, 1/6, 1/12, 1/6, 1/12, 1/4, 1/4),
+// the scaled relaxed moments are tt_r = c_r (1 - s_r) m_r + c_r s_r m_eq,r
+// (MrtParams ra, rb, rk0) and d* = M^T tt, again from shared sums.  About 90
+// FP64 operations per node.
 template <bool FIELDS, int CP>
 __device__ __forceinline__ void lattice_f32_node(const Grid& g, const MrtParams& P, const float* __restrict__ src,
                                                  float* __restrict__ dst, const FieldSlot& fs, int i, int j,
                                                  const AdjMask& am) {
   if (i < 1 || i > g.nx - 2 || is_adjoint(am, i, j)) return;
-  float d[9];
+  double d[9];
 #pragma unroll
-  for (int k = 0; k < 9; k++) d[k] = ld_pop<CP>(src + (long long)k * g.ps + (long long)(j - EY(k)) * g.nx + (i - EX(k)));
+  for (int k = 0; k < 9; k++)
+    d[k] = (double)ld_pop<CP>(src + (long long)k * g.ps + (long long)(j - EY(k)) * g.nx + (i - EX(k)));
   // moments of d
-  const float h13 = d[1] + d[3], h24 = d[2] + d[4];     // axis pairs
-  const float h57 = d[5] + d[7], h68 = d[6] + d[8];     // diagonal pairs
-  const float s14 = h13 + h24, s58 = h57 + h68;
-  const float rp = d[0] + s14 + s58;                    // m0 = rho - sum w
-  const float a13 = d[1] - d[3], a24 = d[2] - d[4];
-  const float x58 = (d[5] + d[8]) - (d[6] + d[7]);
-  const float y58 = (d[5] + d[6]) - (d[7] + d[8]);
-  const float jx = a13 + x58, jy = a24 + y58;
-  const float m1 = (s58 + s58) - s14 - 4.0f * d[0];
-  const float m2 = 4.0f * d[0] - (s14 + s14) + s58;
-  const float m4 = x58 - (a13 + a13);
-  const float m6 = y58 - (a24 + a24);
-  const float m7 = h13 - h24;
-  const float m8 = h57 - h68;
+  const double h13 = d[1] + d[3], h24 = d[2] + d[4];     // axis pairs
+  const double h57 = d[5] + d[7], h68 = d[6] + d[8];     // diagonal pairs
+  const double s14 = h13 + h24, s58 = h57 + h68;
+  const double rp = d[0] + s14 + s58;                    // m0 = rho - sum w
+  const double a13 = d[1] - d[3], a24 = d[2] - d[4];
+  const double x58 = (d[5] + d[8]) - (d[6] + d[7]);
+  const double y58 = (d[5] + d[6]) - (d[7] + d[8]);
+  const double jx = a13 + x58, jy = a24 + y58;
+  const double m1 = (s58 + s58) - s14 - 4.0 * d[0];
+  const double m2 = 4.0 * d[0] - (s14 + s14) + s58;
+  const double m4 = x58 - (a13 + a13);
+  const double m6 = y58 - (a24 + a24);
+  const double m7 = h13 - h24;
+  const double m8 = h57 - h68;
   // relaxed moments, scaled by c
-  const float xx = jx * jx, yy = jy * jy, jj = xx + yy;
-  const float* A = P.ra32;
-  const float* B = P.rb32;
-  const float t0 = fmaf(A[0], rp, P.rk0);
-  const float t1 = fmaf(A[1], m1, B[1] * (3.0f * jj - (rp + rp) + P.eq1));
-  const float t2 = fmaf(A[2], m2, B[2] * (rp - 3.0f * jj + P.eq2));
-  const float t3 = A[3] * jx;
-  const float t4 = fmaf(A[4], m4, -B[4] * jx);
-  const float t5 = A[5] * jy;
-  const float t6 = fmaf(A[6], m6, -B[6] * jy);
-  const float t7 = fmaf(A[7], m7, B[7] * (xx - yy));
-  const float t8 = fmaf(A[8], m8, B[8] * (jx * jy));
+  const double xx = jx * jx, yy = jy * jy, jj = xx + yy;
+  const double* A = P.ra;
+  const double* B = P.rb;
+  const double t0 = fma(A[0], rp, P.rk0);
+  const double t1 = fma(A[1], m1, B[1] * (3.0 * jj - (rp + rp) + P.eq1));
+  const double t2 = fma(A[2], m2, B[2] * (rp - 3.0 * jj + P.eq2));
+  const double t3 = A[3] * jx;
+  const double t4 = fma(A[4], m4, -B[4] * jx);
+  const double t5 = A[5] * jy;
+  const double t6 = fma(A[6], m6, -B[6] * jy);
+  const double t7 = fma(A[7], m7, B[7] * (xx - yy));
+  const double t8 = fma(A[8], m8, B[8] * (jx * jy));
   // d* = M^T tt
-  const float pa = t0 - t1 - (t2 + t2);                 // axis directions
-  const float qd = t0 + (t1 + t1) + t2;                 // diagonals
-  const float ex = t3 - (t4 + t4), ey = t5 - (t6 + t6);
-  const float dx = t3 + t4, dy = t5 + t6;
-  float out[9];
-  out[1] = pa + ex + t7;
-  out[3] = pa - ex + t7;
-  out[2] = pa + ey - t7;
-  out[4] = pa - ey - t7;
-  out[5] = qd + (dx + dy) + t8;
-  out[7] = qd - (dx + dy) + t8;
-  out[6] = qd + (dy - dx) - t8;
-  out[8] = qd - (dy - dx) - t8;
+  const double pa = t0 - t1 - (t2 + t2);                 // axis directions
+  const double qd = t0 + (t1 + t1) + t2;                 // diagonals
+  const double ex = t3 - (t4 + t4), ey = t5 - (t6 + t6);
+  const double dx = t3 + t4, dy = t5 + t6;
+  float out[9];   // the stored deviations
+  out[1] = (float)(pa + ex + t7);
+  out[3] = (float)(pa - ex + t7);
+  out[2] = (float)(pa + ey - t7);
+  out[4] = (float)(pa - ey - t7);
+  out[5] = (float)(qd + (dx + dy) + t8);
+  out[7] = (float)(qd - (dx + dy) + t8);
+  out[6] = (float)(qd + (dy - dx) - t8);
+  out[8] = (float)(qd - (dy - dx) - t8);
   // with s_0 = 0 (rk0 = 0: the collision conserves the density moment, the
-  // reference's rates) the rest population is the remainder of the mass:
-  // t0 + 4 (t2 - t1) in exact arithmetic, and the nine deviations then sum
-  // to rp up to one rounding, with no systematic FP32 mass drift from the
-  // rounded 1/9 in t0
-  out[0] = P.rk0 == 0.0f
-               ? rp - (((out[1] + out[3]) + (out[2] + out[4])) + ((out[5] + out[7]) + (out[6] + out[8])))
-               : t0 + 4.0f * (t2 - t1);
+  // reference's rates) the rest population is the remainder of the mass,
+  // t0 + 4 (t2 - t1) in exact arithmetic, taken against the eight rounded
+  // deviations: the nine stored values then sum to rp up to the one rounding
+  // of out[0], so the FP32 stores add no systematic mass drift
+  const double o13 = (double)out[1] + (double)out[3], o24 = (double)out[2] + (double)out[4];
+  const double o57 = (double)out[5] + (double)out[7], o68 = (double)out[6] + (double)out[8];
+  out[0] = (float)(P.rk0 == 0.0 ? rp - ((o13 + o24) + (o57 + o68)) : t0 + 4.0 * (t2 - t1));
   const long long o = (long long)j * g.nx + i;
 #pragma unroll
   for (int k = 0; k < 9; k++) st_pop<CP>(dst + (long long)k * g.ps + o, out[k]);
   if (FIELDS) {
-    const double rd = P.wsum + (double)rp;
+    const double rd = P.wsum + rp;
     if (fs.rho) fs.rho[o] = rd;
-    fs.u[o] = (double)jx / rd;
-    fs.v[o] = (double)jy / rd;
+    fs.u[o] = jx / rd;
+    fs.v[o] = jy / rd;
   }
 }
 

@@ -1,9 +1,9 @@
 """FP32-shifted storage (LBM_PRECISION_FP32_SHIFTED) against the FP64 CPU
 oracle on the same seeded inputs.
 
-Populations are stored and bulk-collided in FP32 as deviations f - w_k; the
-boundary ring, the adjoint nodes, the IBM solve and the force reductions stay
-FP64.  Tolerances (FP64 oracle as the reference):
+Populations are stored in FP32 as deviations f - w_k and collided in FP64
+(the bulk kernel on the loaded deviations; the boundary ring, the adjoint
+nodes, the IBM solve and the force reductions on f = w_k + d).  Tolerances (FP64 oracle as the reference):
   * window-averaged per-wing forces and moments: 1e-3 relative (the bar set
     for the FP32 path),
   * rho, u, v after the run: 1e-4 relative L2 with wings, 1e-5 without.
