@@ -291,7 +291,8 @@ int lbm_batch_last_error(const lbm_batch *b, char *buf, size_t n);
 
 /* Autotuner result (no counterpart in the reference): the tile width of the
  * fused lattice kernel (128, 256 or 512 nodes per workgroup) and its cap on
- * resident waves per CU (0 none, 24 or 16; set through unused dynamic LDS),
+ * resident waves per CU (0 none, 24, 16, 12 or 8; set through unused
+ * dynamic LDS),
  * chosen by timing every pair on the instance's own lattice at its first
  * lbm_advance (0 before that; LBM_LATTICE_TILE=<width> and
  * LBM_LATTICE_WAVES=<n> in the environment fix them).  lbm_batch_get_tuning
