@@ -193,6 +193,7 @@ struct lbm_handle {
   int lat_cp = 0;                    // cache policy of the lattice kernel's population accesses
   size_t lat_lds = 0;                // dynamic LDS per lattice workgroup that sets the cap
   size_t dyn_lds = 0;                // the same for the cap beside a coupled step's geometry
+  bool dyn_narrow = false;           // that cap held by the register-capped kernel instead
   // lattice-kernel timing (lbm_set_timing)
   bool timing = false;
   std::vector<hipEvent_t> ev;        // start/stop pairs of the current call
@@ -510,6 +511,10 @@ void tune_instance(lbm_handle* h) {
   if (const char* e = getenv("LBM_DYN_WAVES")) dyn_waves = std::max(0, atoi(e));   // 0: no cap
   const long long dyn = lattice_lds_for_waves(h->lat_bx, dyn_waves, lds_cu, lds_blk);
   h->dyn_lds = dyn >= 0 ? (size_t)dyn : h->lat_lds;
+  // the default cap of 8 waves per CU by register count, with no LDS taken
+  // (LBM_DYN_NARROW=0: by unused LDS, as for the other caps)
+  const char* nw = getenv("LBM_DYN_NARROW");
+  h->dyn_narrow = dyn_waves == 8 && !(nw && nw[0] == '0');
 }
 
 // Per-slot geometry arrays and flow buffers of the IBM.  The number of steps
@@ -666,9 +671,10 @@ int lbm_create(const lbm_config* cfg, lbm_handle** out) try {
   ip.band_cls = band_class(band_bound(cfg->xb0, cfg->yb0, h->W, h->P, cfg->scale));
   int lds_max = 0;
   (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device);
-  if (h->W > 0 && geo_solve_lds_bytes(ip) > (size_t)lds_max) {
+  const size_t lds_need = std::max(geo_solve_lds_bytes(ip, true), bsolve_lds_bytes(ip));
+  if (h->W > 0 && lds_need > (size_t)lds_max) {
     int rc = fail(nullptr, LBM_E_INVALID, "%d segments per wing need %zu B of LDS, device allows %d", h->SPW,
-                  geo_solve_lds_bytes(ip), lds_max);
+                  lds_need, lds_max);
     delete h;
     return rc;
   }
@@ -1007,8 +1013,11 @@ int advance_dynamic(lbm_handle* h, int32_t n_last, double* force_rows, double* m
     }
     // a tighter wave cap than the tuned one: the step's chain beside the
     // lattice kernel is the critical path here
-    launch_lattice(h->stream, h->g, h->mrt, src, dst, fs, h->f32, h->lat_bx, h->lat_cp, every_node, nullptr, nullptr,
-                   h->dyn_lds);
+    if (h->dyn_narrow)
+      launch_lattice_narrow(h->stream, h->g, h->mrt, src, dst, *fs, h->f32, h->lat_bx, h->lat_cp, every_node);
+    else
+      launch_lattice(h->stream, h->g, h->mrt, src, dst, fs, h->f32, h->lat_bx, h->lat_cp, every_node, nullptr,
+                     nullptr, h->dyn_lds);
     HIPCHECK(h, hipGetLastError());
     HIPCHECK(h, hipEventRecord(h->ev_step, h->stream));
     launch_bc_ring(h->stream_bc, h->g, h->mrt, h->R, src, dst, fs, h->f32);
@@ -1698,6 +1707,7 @@ struct lbm_batch {
   int lat_cp = 0;                    // cache policy of the lattice kernel, tuned with the cap
   size_t lat_lds = 0;
   bool direct = false;               // some episode of the call solves by band substitution
+  size_t bsolve_lds = 0;             // LDS of its band substitution (the largest episode's)
   char msg[512] = "";
 };
 
@@ -1851,8 +1861,9 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
   std::vector<EpDesc> desc(count);
   std::vector<int2> items;
   BatchItems it{};
-  size_t lds_wing = 0, lds_solve = 0;
+  size_t lds_wing = 0, lds_solve = 0, lds_inv = 0;
   b->direct = false;
+  b->bsolve_lds = 0;
   int inv_chunks = 1;
   for (int e = 0; e < count; e++) {
     lbm_handle* h = hs[e];
@@ -1884,10 +1895,14 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
     D.bpw = adj_collide_blocks_per_wing(h->ip);
     D.direct = h->direct == 2 ? 1 : 0;   // as the instance's own advance solves its prescribed steps
     D.pad1 = 0;
-    if (D.direct && h->W > 0) b->direct = true;
+    if (D.direct && h->W > 0) {
+      b->direct = true;
+      b->bsolve_lds = std::max(b->bsolve_lds, bsolve_lds_bytes(h->ip));
+    }
     if (h->W > 0) {
       lds_wing = std::max(lds_wing, geo_wing_lds_bytes(h->SPW));
-      lds_solve = std::max(lds_solve, geo_solve_lds_bytes(h->ip));
+      lds_solve = std::max(lds_solve, geo_solve_lds_bytes(h->ip, h->direct == 2));
+      lds_inv = std::max(lds_inv, geo_factor_lds_bytes(h->ip));
       inv_chunks = std::max(inv_chunks, inverse_chunks(h->SPW));
     }
   }
@@ -2023,7 +2038,7 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
     BCHECK(b, hipStreamWaitEvent(b->stream_geo, bb < seg_bb + 2 ? b->ev_start : b->ev_rel[es], 0));
     const int q = k0 + ns >= steps ? 6 : 5;  // the last (shorter) batch has its own list
     launch_batch_geometry(b->stream_geo, b->d_desc, b->d_kbase, di + off[q], (int)(off[q + 1] - off[q]), k0 - kb0,
-                          lds_wing, lds_solve, inv_chunks);
+                          lds_wing, lds_solve, lds_inv, inv_chunks);
     BCHECK(b, hipEventRecord(b->ev_geo[es], b->stream_geo));
     return 0;
   };
@@ -2061,7 +2076,7 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
         const int rc = issue_geometry(k / ns + 1, seg_bb, kb0);
         if (rc) return rc;
       }
-      launch_batch_ibm_flow(b->stream_ibm, b->d_desc, it, kk, f32, b->direct);
+      launch_batch_ibm_flow(b->stream_ibm, b->d_desc, it, kk, f32, b->direct ? b->bsolve_lds : 0);
       launch_batch_adj_collide(b->stream_ibm, b->d_desc, it, kk, f32, fields);
       BCHECK(b, hipStreamWaitEvent(b->stream_ibm, b->ev_bc, 0));
       BCHECK(b, hipEventRecord(b->ev_ibm, b->stream_ibm));

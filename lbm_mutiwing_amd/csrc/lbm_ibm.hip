@@ -130,6 +130,25 @@ __device__ WingMotionState wing_motion(const IbmParams& p, const GeoBuffers& G, 
 // wide; a batch of episodes issues hundreds, which fill the GPU better as
 // narrower workgroups (measured on 64 episodes: 74.3 -> 79.5 env-steps/s at
 // 256; a single 4096x2048 instance with two wings loses 10 % at 256).
+// dst[e] = src[e] for e < count by NT threads, LOAD_BATCH loads per thread in
+// flight at a time: a plain copy loop waits for each load before the next
+// (the store needs its value), one memory latency per element per thread
+constexpr int LOAD_BATCH = 8;
+template <int NT, typename T>
+__device__ __forceinline__ void copy_batched(T* __restrict__ dst, const T* __restrict__ src, int count) {
+  for (int e0 = (int)threadIdx.x; e0 < count; e0 += LOAD_BATCH * NT) {
+    T v[LOAD_BATCH];
+#pragma unroll
+    for (int u = 0; u < LOAD_BATCH; u++) {
+      const int e = e0 + u * NT;
+      v[u] = src[e < count ? e : 0];
+    }
+#pragma unroll
+    for (int u = 0; u < LOAD_BATCH; u++)
+      if (e0 + u * NT < count) dst[e0 + u * NT] = v[u];
+  }
+}
+
 constexpr int GEO_THREADS = 512;
 constexpr int GEO_THREADS_BATCH = 256;
 constexpr int CLAIM_UNROLL = 12;
@@ -364,6 +383,58 @@ __device__ __forceinline__ double s_entry(const int* box, const double* phx, con
   return sx * sy;
 }
 
+// s_entry with every load unconditional (box from LDS, the six weights of
+// each segment at clamped offsets) and the same sums in the same order, so
+// that the loads of several entries can be issued together
+__device__ __forceinline__ double s_entry_flat(const int* box, const double* phx, const double* phy, int k, int l) {
+  const int* bk = box + 4 * k;
+  const int* bl = box + 4 * l;
+  const int ia = max(bk[0], bl[0]), ib = min(bk[1], bl[1]);
+  const int ja = max(bk[2], bl[2]), jb = min(bk[3], bl[3]);
+  const double* pxk = phx + BOXW * k;
+  const double* pyk = phy + BOXW * k;
+  const double* pxl = phx + BOXW * l;
+  const double* pyl = phy + BOXW * l;
+  double sx = 0.0, sy = 0.0;
+#pragma unroll
+  for (int c = 0; c < BOXW; c++) {   // i = bk[0] + c covers ia..ib (boxes are at most BOXW wide)
+    const int i = bk[0] + c;
+    const bool in = i >= ia && i <= ib;
+    const double a = pxk[c], b = pxl[in ? i - bl[0] : 0];
+    if (in) sx += a * b;
+  }
+#pragma unroll
+  for (int c = 0; c < BOXW; c++) {
+    const int j = bk[2] + c;
+    const bool in = j >= ja && j <= jb;
+    const double a = pyk[c], b = pyl[in ? j - bl[2] : 0];
+    if (in) sy += a * b;
+  }
+  return ia > ib || ja > jb ? 0.0 : sx * sy;
+}
+
+// dst[e] = S(k, l) for e < count, where entry(e, k, l) names the segments of
+// entry e (false: the entry is zero), ASSEMBLE_BATCH entries per thread at a
+// time with all their loads issued before their sums
+constexpr int ASSEMBLE_BATCH = 4;
+template <int NT, typename Entry>
+__device__ __forceinline__ void assemble_s(double* dst, int count, const int* sbox, const double* phx,
+                                           const double* phy, Entry entry) {
+  for (int e0 = (int)threadIdx.x; e0 < count; e0 += ASSEMBLE_BATCH * NT) {
+    double v[ASSEMBLE_BATCH];
+#pragma unroll
+    for (int u = 0; u < ASSEMBLE_BATCH; u++) {
+      int k = 0, l = 0;
+      const bool ok = entry(e0 + u * NT, k, l);
+      const double x = s_entry_flat(sbox, phx, phy, ok ? k : 0, ok ? l : 0);
+      v[u] = ok ? x : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < ASSEMBLE_BATCH; u++)
+      if (e0 + u * NT < count) dst[e0 + u * NT] = v[u];
+  }
+}
+
 // In-place S = L D L^T of the lower band (band[r*Dd + d] = S(r, r-d), row
 // stride Dd > bw) by one wave: afterwards band[k*Dd] = D_k,
 // band[(k+i)*Dd + i] = L(k+i, k), invd[k] = 1/D_k.  Step k updates
@@ -380,7 +451,8 @@ __device__ __forceinline__ double s_entry(const int* box, const double* phx, con
 constexpr int LDLT_PAIRS_PER_LANE = (BMAX * (BMAX + 1) / 2 + 63) / 64;
 
 template <int NPL>
-__device__ void band_ldlt_t(double* band, double* invd, int n, int bw, int Dd, int lane, DevStep* st, int info) {
+__device__ void band_ldlt_t(double* band, double* invd, int n, int npiv, int bw, int Dd, int lane, DevStep* st,
+                            int info) {
   const int npairs = bw * (bw + 1) / 2;
   // offsets from row k of a pair's target S(k+i, k+j), and of its operands
   // S(k+i, k) and S(k+j, k); pairs past the last row or past npairs write
@@ -403,7 +475,7 @@ __device__ void band_ldlt_t(double* band, double* invd, int n, int bw, int Dd, i
   double d = band[0];
   int bad = -1;
   double* sink = invd + n;   // an unused entry: the store of a pair with nothing to update this step
-  for (int k = 0; k < n; k++) {
+  for (int k = 0; k < npiv; k++) {
     double* bk = band + (size_t)k * Dd;
     double vt[NPL], va[NPL], vc[NPL];
 #pragma unroll
@@ -432,23 +504,27 @@ __device__ void band_ldlt_t(double* band, double* invd, int n, int bw, int Dd, i
   wave_sync();
   if (lane == 0 && bad >= 0) raise_error(st, DERR_NOT_SPD, info + bad);
   // L(k + i, k) = S(k + i, k) / D_k
-  for (int e = lane; e < n * bw; e += 64) {
+  for (int e = lane; e < npiv * bw; e += 64) {
     const int k = e / bw, i = 1 + e - (e / bw) * bw;
     if (k + i < n) band[(k + i) * Dd + i] = band[(k + i) * Dd + i] * invd[k];
   }
   wave_sync();
 }
 
-__device__ void band_ldlt(double* band, double* invd, int n, int bw, int Dd, int lane, DevStep* st, int info) {
+// L D L^T of the first npiv pivots of an n-row band (npiv = n: the whole
+// factorisation; npiv < n leaves rows npiv.. holding the Schur complement of
+// the eliminated block, their entries in columns < npiv scaled to L)
+__device__ void band_ldlt(double* band, double* invd, int n, int npiv, int bw, int Dd, int lane, DevStep* st,
+                          int info) {
   const int npl = (bw * (bw + 1) / 2 + 63) / 64;
   if (npl <= 1)
-    band_ldlt_t<1>(band, invd, n, bw, Dd, lane, st, info);
+    band_ldlt_t<1>(band, invd, n, npiv, bw, Dd, lane, st, info);
   else if (npl == 2)
-    band_ldlt_t<2>(band, invd, n, bw, Dd, lane, st, info);
+    band_ldlt_t<2>(band, invd, n, npiv, bw, Dd, lane, st, info);
   else if (npl == 3)
-    band_ldlt_t<3>(band, invd, n, bw, Dd, lane, st, info);
+    band_ldlt_t<3>(band, invd, n, npiv, bw, Dd, lane, st, info);
   else
-    band_ldlt_t<LDLT_PAIRS_PER_LANE>(band, invd, n, bw, Dd, lane, st, info);
+    band_ldlt_t<LDLT_PAIRS_PER_LANE>(band, invd, n, npiv, bw, Dd, lane, st, info);
 }
 
 // One row of a band substitution: a0 minus the BW products of the row's
@@ -586,18 +662,116 @@ __device__ __forceinline__ void inverse_column(const double* band, const double*
   }
 }
 
+// The twisted factors of one (step, wing) (Twist): T and R assembled from
+// the boxes in LDS, the top and bottom ends eliminated by waves 0 and 1 at
+// once, then the middle block (T's and R's Schur-complemented copies less S
+// counted twice) factored densely by wave 0; the middle rows' entries among
+// themselves are then cleared and their 1/D set to 1, so that the band
+// substitutions leave them as the middle solve sets them.  All of it, in the
+// layout of twist_doubles, to bg.
+template <int NT>
+__device__ void geo_twisted_factor(double* lds, const int* sbox, const double* phx, const double* phy, int n, int bw,
+                                   int BWc, DevStep* st, int w, double* bg) {
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int Dd = BWc + 1;
+  const Twist tw = twist_split(n, bw);
+  const int m = tw.m;
+  double* T = lds;                               // nT + BWc rows
+  double* R = T + (size_t)(tw.nT + BWc) * Dd;    // nR + BWc rows
+  double* iT = R + (size_t)(tw.nR + BWc) * Dd;   // nT + 1
+  double* iR = iT + tw.nT + 1;                   // nR + 1
+  double* mid = iR + tw.nR + 1;                  // m x m: the middle block, then its L
+  double* midd = mid + m * m;                    // m: its 1/D
+  double* smid = midd + m;                       // m x m: S of the middle block
+  assemble_s<NT>(T, (tw.nT + BWc) * Dd, sbox, phx, phy, [&](int e, int& k, int& l) {
+    const int r = e / Dd, d = e - r * Dd;
+    k = seg_of_pos(r, n);
+    l = seg_of_pos(r - d, n);
+    return r < tw.nT && d <= bw && r - d >= 0;
+  });
+  // R(r, r-d) = S(n-1-r+d, n-1-r): the higher band position first, as in T
+  assemble_s<NT>(R, (tw.nR + BWc) * Dd, sbox, phx, phy, [&](int e, int& k, int& l) {
+    const int r = e / Dd, d = e - r * Dd;
+    k = seg_of_pos(n - 1 - r + d, n);
+    l = seg_of_pos(n - 1 - r, n);
+    return r < tw.nR && d <= bw && r - d >= 0;
+  });
+  __syncthreads();
+  for (int e = t; e < m * m; e += NT) {
+    const int i = e / m, j = e - i * m;
+    smid[e] = j <= i ? T[(size_t)(tw.p + i) * Dd + (i - j)] : 0.0;
+  }
+  __syncthreads();
+  if (wave == 0) band_ldlt(T, iT, tw.nT, tw.p, bw, Dd, lane, st, w * 65536);
+  else if (wave == 1) band_ldlt(R, iR, tw.nR, tw.q, bw, Dd, lane, st, w * 65536 + 32768);
+  __syncthreads();
+  if (wave == 0) {
+    // S(p+i, p+j), j <= i: T's copy holds it less the top end's complement,
+    // R's (row q+m-1-j, offset i-j) less the bottom end's
+    for (int e = lane; e < m * m; e += 64) {
+      const int i = e / m, j = e - i * m;
+      mid[e] = j <= i ? T[(size_t)(tw.p + i) * Dd + (i - j)] + R[(size_t)(tw.q + m - 1 - j) * Dd + (i - j)] - smid[e]
+                      : 0.0;
+    }
+    wave_sync();
+    int bad = -1;
+    for (int k = 0; k < m; k++) {
+      const double dk = mid[k * m + k];
+      if (!(dk > 0.0) && bad < 0) bad = k;
+      const double rd = 1.0 / (dk > 0.0 ? dk : 1.0);
+      for (int e = lane; e < m * m; e += 64) {
+        const int i = e / m, j = e - i * m;
+        if (j > k && i >= j) mid[e] = mid[e] - mid[i * m + k] * rd * mid[j * m + k];
+      }
+      if (lane == 0) midd[k] = rd;
+      wave_sync();
+    }
+    for (int e = lane; e < m * m; e += 64) {   // L(i, k) = M(i, k) / D_k below the diagonal
+      const int i = e / m, k = e - i * m;
+      if (k < i) mid[e] = mid[e] * midd[k];
+    }
+    if (lane == 0 && bad >= 0) raise_error(st, DERR_NOT_SPD, w * 65536 + tw.p + bad);
+    for (int e = lane; e < m * m; e += 64) {
+      const int i = e / m, j = e - i * m;
+      if (j <= i) {
+        T[(size_t)(tw.p + i) * Dd + (i - j)] = 0.0;
+        R[(size_t)(tw.q + i) * Dd + (i - j)] = 0.0;
+      }
+    }
+    for (int i = lane; i < m; i += 64) {
+      iT[tw.p + i] = 1.0;
+      iR[tw.q + i] = 1.0;
+    }
+  }
+  __syncthreads();
+  const int tot = (int)twist_doubles(n, bw, BWc);
+  for (int e = t; e < tot; e += NT) bg[e] = T[e];
+}
+
 // One workgroup per (step, wing): adjoint total, band assembly of S,
 // L D L^T and the explicit inverse (one column per thread).
 template <int NT>
 __device__ __forceinline__ void geo_solve_body(const IbmParams& p, const GeoBuffers& G, DevStep* st, int slot0, int b,
                                                int w, int direct, int apart) {
   extern __shared__ double lds_s[];
-  __shared__ int s_skip, s_bw;
+  __shared__ int s_skip, s_bw, s_meet;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int slot = slot0 + b, n = p.SPW;
   GeoStep* gs = G.step + slot;
   if (t == 0) {
     s_skip = load_agent(&st->error) != 0;
+    // the coupling of wings (k_geo_couple) reads the inverse; it needs wings
+    // that share a node, so support rectangles that meet
+    int meet = 0, gap = INT_MAX;
+    for (int q = 0; q < p.W; q++)
+      for (int r = q + 1; r < p.W; r++) {
+        const int* a = gs->rect[q];
+        const int* c = gs->rect[r];
+        if (a[1] < a[0] || c[1] < c[0]) continue;   // a wing with no nodes
+        meet |= a[0] <= c[1] && c[0] <= a[1] && a[2] <= c[3] && c[2] <= a[3];
+        gap = min(gap, max(max(a[0] - c[1], c[0] - a[1]), max(a[2] - c[3], c[2] - a[3])));
+      }
+    s_meet = meet;
     if (w == 0) {
       int tot = 0, b0 = INT_MAX, b1 = INT_MIN;
       for (int q = 0; q < p.W; q++) {
@@ -610,17 +784,6 @@ __device__ __forceinline__ void geo_solve_body(const IbmParams& p, const GeoBuff
       gs->adj_n = tot;
       gs->band[0] = b0;
       gs->band[1] = b1;
-      // the coupling of wings (k_geo_couple) reads the inverse; it needs
-      // wings that share a node, so support rectangles that meet
-      int meet = 0, gap = INT_MAX;
-      for (int q = 0; q < p.W; q++)
-        for (int r = q + 1; r < p.W; r++) {
-          const int* a = gs->rect[q];
-          const int* c = gs->rect[r];
-          if (a[1] < a[0] || c[1] < c[0]) continue;   // a wing with no nodes
-          meet |= a[0] <= c[1] && c[0] <= a[1] && a[2] <= c[3] && c[2] <= a[3];
-          gap = min(gap, max(max(a[0] - c[1], c[0] - a[1]), max(a[2] - c[3], c[2] - a[3])));
-        }
       gs->inv_on = !direct || meet;
       st->rect_gap = gap;
       if (apart && meet) raise_error(st, DERR_WINGS_MET, gs->n);
@@ -638,7 +801,7 @@ __device__ __forceinline__ void geo_solve_body(const IbmParams& p, const GeoBuff
   // band is reported)
   {
     int* sb = reinterpret_cast<int*>(lds_s);
-    for (int e = t; e < 4 * n; e += NT) sb[e] = box[e];
+    copy_batched<NT>(sb, box, 4 * n);
     if (t == 0) s_bw = 0;
     __syncthreads();
     constexpr int DMAX = BMAX + 1;
@@ -664,20 +827,27 @@ __device__ __forceinline__ void geo_solve_body(const IbmParams& p, const GeoBuff
   // row stride: the compile-time band bound of the inverse below
   const int BWc = band_class(bw);
   const int Dd = BWc + 1;
-  double* band = lds_s;                          // (n + BWc) rows of Dd
-  double* invd = band + (size_t)(n + BWc) * Dd;
   const double* phx = G.phx + base * BOXW;
   const double* phy = G.phy + base * BOXW;
-  for (int e = t; e < (n + BWc) * Dd; e += NT) {
-    const int pp = e / Dd, d = e - pp * Dd;
-    band[e] = pp < n && d <= bw && pp - d >= 0 ? s_entry(box, phx, phy, seg_of_pos(pp, n), seg_of_pos(pp - d, n))
-                                               : 0.0;
+  double* bg = G.bandg + ((size_t)slot * p.W + w) * band_cap(p);
+  const int* sbox = reinterpret_cast<const int*>(lds_s);   // the boxes, staged above
+  double* fac = lds_s + 2 * (size_t)n;                      // the factors after them
+  if (direct && !s_meet && twist_ok(n, bw)) {   // factors for the band substitution only: twisted
+    geo_twisted_factor<NT>(fac, sbox, phx, phy, n, bw, BWc, st, w, bg);
+    return;
   }
+  double* band = fac;                            // (n + BWc) rows of Dd
+  double* invd = band + (size_t)(n + BWc) * Dd;
+  assemble_s<NT>(band, (n + BWc) * Dd, sbox, phx, phy, [&](int e, int& k, int& l) {
+    const int pp = e / Dd, d = e - pp * Dd;
+    k = seg_of_pos(pp, n);
+    l = seg_of_pos(pp - d, n);
+    return pp < n && d <= bw && pp - d >= 0;
+  });
   __syncthreads();
-  if (wave == 0) band_ldlt(band, invd, n, bw, Dd, lane, st, w * 65536);
+  if (wave == 0) band_ldlt(band, invd, n, n, bw, Dd, lane, st, w * 65536);
   __syncthreads();
   // the factors (band, then 1/D, contiguous) to global memory for the inverse
-  double* bg = G.bandg + ((size_t)slot * p.W + w) * band_cap(p);
   const int tot = (n + BWc) * Dd + n + 1;
   for (int e = t; e < tot; e += NT) bg[e] = band[e];
 }
@@ -703,7 +873,7 @@ __device__ __forceinline__ void geo_inverse_body(const IbmParams& p, const GeoBu
   const int BWc = band_class(bw);
   const double* bg = G.bandg + ((size_t)slot * p.W + w) * band_cap(p);
   const int tot = (n + BWc) * (BWc + 1) + n + 1;
-  for (int e = threadIdx.x; e < tot; e += INV_THREADS) lds_i[e] = bg[e];
+  copy_batched<INV_THREADS>(lds_i, bg, tot);
   __syncthreads();
   const double* band = lds_i;
   const double* invd = band + (size_t)(n + BWc) * (BWc + 1);
@@ -735,9 +905,27 @@ __global__ __launch_bounds__(GEO_THREADS) void k_geo_solve(IbmParams p, GeoBuffe
 
 // band of (SPW + c) rows of stride c + 1, c = the band class of the
 // instance's bound, and 1/D (SPW + 1 entries: the inverse loads one row ahead)
-size_t geo_solve_lds_bytes(const IbmParams& p) {
+size_t geo_factor_lds_bytes(const IbmParams& p) {
   const size_t c = (size_t)p.band_cls;
   return ((p.SPW + c) * (c + 1) + p.SPW + 1) * sizeof(double);
+}
+
+// the segments' boxes (4 ints each), then the plain factors or, for solves
+// that may form twisted ones (direct), the larger of those and the twisted
+// factors (rows of T and R, their 1/D, the middle block twice and its 1/D)
+size_t geo_solve_lds_bytes(const IbmParams& p, bool direct) {
+  const size_t c = (size_t)p.band_cls;
+  const size_t twisted = ((p.SPW + 3 * c) * (c + 1) + p.SPW + c + 2 + 2 * c * c + c) * sizeof(double);
+  return 2 * (size_t)p.SPW * sizeof(double) + std::max(geo_factor_lds_bytes(p), direct ? twisted : 0);
+}
+
+// the band substitution's LDS: a wing's factors (plain or twisted, with
+// BAND_RING rows before and after), its right-hand sides and its solution
+size_t bsolve_lds_bytes(const IbmParams& p) {
+  const size_t c = (size_t)p.band_cls;
+  const size_t plain = (p.SPW + c) * (c + 1) + p.SPW + 1;
+  const size_t twisted = (p.SPW + 3 * c) * (c + 1) + p.SPW + 2 * c + 2 + c * c + c;
+  return (std::max(plain, twisted) + 2 * BAND_RING * (c + 1) + 4 * (size_t)p.SPW) * sizeof(double);
 }
 
 // ------------------------------------------------------------ coupled wings
@@ -1103,12 +1291,12 @@ void launch_geometry(hipStream_t s, const IbmParams& p, const GeoBuffers& G, Dev
   dim3 grid(count, p.W);
   hipLaunchKernelGGL(k_geo_wing, grid, dim3(GEO_THREADS), geo_wing_lds_bytes(p.SPW), s, p, G, st, slot0, n0, fidx0);
   if (with_inverse) {
-    hipLaunchKernelGGL(k_geo_solve, grid, dim3(GEO_THREADS), geo_solve_lds_bytes(p), s, p, G, st, slot0,
+    hipLaunchKernelGGL(k_geo_solve, grid, dim3(GEO_THREADS), geo_solve_lds_bytes(p, direct), s, p, G, st, slot0,
                        direct ? 1 : 0, direct && apart ? 1 : 0);
     if (direct && apart) return;   // no step of the launch forms the inverse or couples wings
     const int ch = inverse_chunks(p.SPW);
-    hipLaunchKernelGGL(k_geo_inverse, dim3(count * ch, p.W), dim3(INV_THREADS), geo_solve_lds_bytes(p), s, p, G, st,
-                       slot0, ch);
+    hipLaunchKernelGGL(k_geo_inverse, dim3(count * ch, p.W), dim3(INV_THREADS), geo_factor_lds_bytes(p), s, p, G,
+                       st, slot0, ch);
     if (p.W >= 2)
       hipLaunchKernelGGL(k_geo_couple, dim3(count), dim3(CPL_THREADS), geo_couple_lds_bytes(), s, p, G, st, slot0);
   }
@@ -1191,33 +1379,113 @@ __global__ __launch_bounds__(256) void k_ibm_matvec(IbmParams p, GeoBuffers G, i
 }
 
 // y = S^-1 c by band substitution (lbm_band.h) on steps without the inverse:
-// workgroup w solves wing w, wave c component c.
-constexpr int BSOLVE_THREADS = 128;
+// workgroup w solves wing w; wave c component c (two more waves for the
+// bottom ends of twisted factors).
+constexpr int BSOLVE_THREADS = 256;   // two components x two ends of a twisted solve
 
+// Twisted substitutions of one wing (four waves: wave & 1 the component,
+// wave >> 1 the end): forward from both ends to the middle block, the middle
+// block's right-hand sides (each end's products removed, S's own c once),
+// its dense solve by the top end's waves, then backward from the middle out.
+// band, c and y in LDS (c and y interleaved by component, c and y pointing
+// at the wave's component).
+template <int BW>
+__device__ void twisted_solve(const double* band, int n, int bw, const double* c, double* y) {
+  __shared__ double s_cm[2][2][BMAX];   // [end][component][middle row]: c less that end's products
+  const int wave = threadIdx.x >> 6, comp = wave & 1, end = wave >> 1, lane = threadIdx.x & 63;
+  constexpr int Dd = BW + 1;
+  const Twist tw = twist_split(n, bw);
+  const int m = tw.m;
+  const double* T = band;
+  const double* R = T + (size_t)(tw.nT + BW) * Dd;
+  const double* iT = R + (size_t)(tw.nR + BW) * Dd;
+  const double* iR = iT + tw.nT + 1;
+  const double* mid = iR + tw.nR + 1;
+  const double* midd = mid + m * m;
+  const double* crev = c + 2L * (n - 1);   // the rows of R are those of S in reverse order
+  double* yrev = y + 2L * (n - 1);
+  // after the forward pass lane l holds middle row l - 1 of its end's order
+  const double f = end == 0 ? band_forward_wave<BW, 2>(T, tw.nT, tw.p, c, y)
+                            : band_forward_wave<BW, -2>(R, tw.nR, tw.q, crev, yrev);
+  if (lane >= 1 && lane <= m) s_cm[end][comp][lane - 1] = f;
+  __syncthreads();
+  if (end == 0) {
+    // lane i: middle row p + i (R holds it as its row q + m - 1 - i)
+    double v = lane < m ? s_cm[0][comp][lane] + s_cm[1][comp][m - 1 - lane] - c[2L * (tw.p + lane)] : 0.0;
+    for (int j = 0; j < m; j++) {   // L z = v
+      const double zj = lane_value(v, j);
+      if (lane > j && lane < m) v = v - mid[lane * m + j] * zj;
+    }
+    if (lane < m) v = v * midd[lane];
+    for (int j = m - 1; j >= 0; j--) {   // L^T x = D^-1 z
+      const double xj = lane_value(v, j);
+      if (lane < j) v = v - mid[j * m + lane] * xj;
+    }
+    if (lane < m) y[2L * (tw.p + lane)] = v;
+  }
+  __syncthreads();   // the middle values in y, for both ends
+  if (end == 0)
+    band_backward_wave<BW, 2>(T, iT, tw.nT, y);
+  else
+    band_backward_wave<BW, -2>(R, iR, tw.nR, yrev);
+}
+
+// doubles of a wing's factors in bandg (twisted or plain, by the band)
+__device__ __forceinline__ size_t factor_doubles(int n, int bw, int BWc) {
+  return twist_ok(n, bw) ? twist_doubles(n, bw, BWc) : (size_t)(n + BWc) * (BWc + 1) + n + 1;
+}
+
+// The band substitutions run from LDS: a wing's factors (with BAND_RING rows
+// of whatever precedes and follows them, which the passes' look-ahead loads
+// read), its right-hand sides and its solution are staged there first, so
+// that each step of the dependent chain waits for an LDS load rather than
+// for global memory (a global load per step, even issued BAND_RING steps
+// ahead, left the chain waiting on memory latency).
 __device__ __forceinline__ void ibm_bsolve_body(const IbmParams& p, const GeoBuffers& G, int slot,
                                                 const FlowBuffers& F, int w) {
+  extern __shared__ double lds_b[];
   const GeoStep* gs = G.step + slot;
-  const int comp = threadIdx.x >> 6;
+  const int t = threadIdx.x, wave = t >> 6, comp = wave & 1;
   if (gs->inv_on) return;
   const int n = p.SPW, bw = gs->bw[w];
-  const double* c = F.cvec + 2 * (size_t)w * n + comp;
-  double* y = F.yvec + 2 * (size_t)w * n + comp;
+  double* yg = F.yvec + 2 * (size_t)w * n;
   if (gs->cnt[w] == 0 || bw > BMAX || band_class(bw) > p.band_cls) {
     // no factors (the wing claims no node, or the solve raised an error)
-    for (int m = threadIdx.x & 63; m < n; m += 64) y[2 * m] = 0.0;
+    for (int e = t; e < 2 * n; e += BSOLVE_THREADS) yg[e] = 0.0;
     return;
   }
   const int BWc = band_class(bw);
-  const double* band = G.bandg + ((size_t)slot * p.W + w) * band_cap(p);
-  const double* invd = band + (size_t)(n + BWc) * (BWc + 1);
-  if (BWc == 8)
-    band_solve_wave<8>(band, invd, n, c, y);
-  else if (BWc == 12)
-    band_solve_wave<12>(band, invd, n, c, y);
-  else if (BWc == 16)
-    band_solve_wave<16>(band, invd, n, c, y);
-  else
-    band_solve_wave<BMAX>(band, invd, n, c, y);
+  const double* gband = G.bandg + ((size_t)slot * p.W + w) * band_cap(p);
+  const int pad = BAND_RING * (BWc + 1);
+  const int nb = (int)factor_doubles(n, bw, BWc) + 2 * pad;
+  double* band = lds_b + pad;
+  double* c = lds_b + nb;
+  double* y = c + 2 * n;
+  copy_batched<BSOLVE_THREADS>(lds_b, gband - pad, nb);
+  copy_batched<BSOLVE_THREADS>(c, F.cvec + 2 * (size_t)w * n, 2 * n);
+  __syncthreads();
+  if (twist_ok(n, bw)) {   // the geometry solve left twisted factors (a step that forms no inverse)
+    if (BWc == 8)
+      twisted_solve<8>(band, n, bw, c + comp, y + comp);
+    else if (BWc == 12)
+      twisted_solve<12>(band, n, bw, c + comp, y + comp);
+    else if (BWc == 16)
+      twisted_solve<16>(band, n, bw, c + comp, y + comp);
+    else
+      twisted_solve<BMAX>(band, n, bw, c + comp, y + comp);
+  } else if (wave < 2) {   // one wave per component
+    const double* invd = band + (size_t)(n + BWc) * (BWc + 1);
+    if (BWc == 8)
+      band_solve_wave<8>(band, invd, n, c + comp, y + comp);
+    else if (BWc == 12)
+      band_solve_wave<12>(band, invd, n, c + comp, y + comp);
+    else if (BWc == 16)
+      band_solve_wave<16>(band, invd, n, c + comp, y + comp);
+    else
+      band_solve_wave<BMAX>(band, invd, n, c + comp, y + comp);
+  }
+  __syncthreads();
+  for (int e = t; e < 2 * n; e += BSOLVE_THREADS) yg[e] = y[e];
 }
 
 __global__ __launch_bounds__(BSOLVE_THREADS) void k_ibm_bsolve(IbmParams p, GeoBuffers G, int slot,
@@ -1317,7 +1585,7 @@ void launch_ibm_flow(hipStream_t s, const IbmParams& p, const MrtParams& P, cons
   else
     hipLaunchKernelGGL(k_ibm_rhs<double>, dim3(blk), dim3(256), 0, s, p, P, G, slot, F, g, (const double*)src);
   if (!sub_only) hipLaunchKernelGGL(k_ibm_matvec, dim3(blk), dim3(256), 0, s, p, G, slot, F);
-  if (direct) hipLaunchKernelGGL(k_ibm_bsolve, dim3(p.W), dim3(BSOLVE_THREADS), 0, s, p, G, slot, F);
+  if (direct) hipLaunchKernelGGL(k_ibm_bsolve, dim3(p.W), dim3(BSOLVE_THREADS), bsolve_lds_bytes(p), s, p, G, slot, F);
   if (p.W >= 2 && !sub_only) hipLaunchKernelGGL(k_ibm_couple, dim3(1), dim3(CPL_THREADS), 0, s, p, G, slot, F);
   const int nb = (p.SPW + SPREAD_SEG - 1) / SPREAD_SEG;
   hipLaunchKernelGGL(k_ibm_spread, dim3(p.W * nb), dim3(SPREAD_THREADS), 0, s, p, G, slot, F);
@@ -1372,11 +1640,11 @@ __global__ __launch_bounds__(CPL_THREADS) void kb_geo_couple(const EpDesc* __res
 }
 
 void launch_batch_geometry(hipStream_t s, const EpDesc* eps, const int* kb, const int2* items, int n_items,
-                           int k0, size_t lds_wing, size_t lds_solve, int inv_chunks) {
+                           int k0, size_t lds_wing, size_t lds_solve, size_t lds_inv, int inv_chunks) {
   if (n_items <= 0) return;
   hipLaunchKernelGGL(kb_geo_wing, dim3(n_items), dim3(GEO_THREADS_BATCH), lds_wing, s, eps, items, kb, k0);
   hipLaunchKernelGGL(kb_geo_solve, dim3(n_items), dim3(GEO_THREADS_BATCH), lds_solve, s, eps, items, kb, k0);
-  hipLaunchKernelGGL(kb_geo_inverse, dim3(n_items, inv_chunks), dim3(INV_THREADS), lds_solve, s, eps, items, kb, k0);
+  hipLaunchKernelGGL(kb_geo_inverse, dim3(n_items, inv_chunks), dim3(INV_THREADS), lds_inv, s, eps, items, kb, k0);
   hipLaunchKernelGGL(kb_geo_couple, dim3(n_items), dim3(CPL_THREADS), geo_couple_lds_bytes(), s, eps, items, kb, k0);
 }
 
@@ -1430,15 +1698,17 @@ __global__ __launch_bounds__(BSOLVE_THREADS) void kb_ibm_bsolve(const EpDesc* __
   ibm_bsolve_body(D.p, D.G, ep_slot(D, k), D.F, it.y);
 }
 
-void launch_batch_ibm_flow(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool direct) {
+void launch_batch_ibm_flow(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32,
+                           size_t bsolve_lds) {
   if (it.n_seg <= 0) return;
   if (f32)
     hipLaunchKernelGGL(kb_ibm_rhs<float>, dim3(it.n_seg), dim3(256), 0, s, eps, it.seg, it.kbase, k);
   else
     hipLaunchKernelGGL(kb_ibm_rhs<double>, dim3(it.n_seg), dim3(256), 0, s, eps, it.seg, it.kbase, k);
   hipLaunchKernelGGL(kb_ibm_matvec, dim3(it.n_seg), dim3(256), 0, s, eps, it.seg, it.kbase, k);
-  if (direct && it.n_wing > 0)
-    hipLaunchKernelGGL(kb_ibm_bsolve, dim3(it.n_wing), dim3(BSOLVE_THREADS), 0, s, eps, it.wing, it.kbase, k);
+  if (bsolve_lds > 0 && it.n_wing > 0)
+    hipLaunchKernelGGL(kb_ibm_bsolve, dim3(it.n_wing), dim3(BSOLVE_THREADS), bsolve_lds, s, eps, it.wing, it.kbase,
+                       k);
   hipLaunchKernelGGL(kb_ibm_couple, dim3(it.n_seg), dim3(CPL_THREADS), 0, s, eps, it.seg, it.kbase, k);
   hipLaunchKernelGGL(kb_ibm_spread, dim3(it.n_spread), dim3(SPREAD_THREADS), 0, s, eps, it.spread, it.kbase, k);
 }

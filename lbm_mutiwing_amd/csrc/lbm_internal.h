@@ -130,10 +130,44 @@ struct GeoBuffers {
   const double* stream_rows;    // stream mode: start row, target row (2*LBM_MAX_MOTION_COLS)
 };
 
-// doubles per (slot, wing) of GeoBuffers::bandg: (SPW + BMAX) band rows of
-// stride BMAX + 1 at most, then SPW + 1 entries of 1/D
+// Twisted L D L^T of a band S of n rows and half-width bw, for the band
+// substitution at flow time: the first p pivots are eliminated from the top
+// (band T: rows 0 .. p+m-1 of S) and the last q from the bottom (band R: the
+// rows of S in reverse order, rows 0 .. q+m-1 of J S J), by two waves at
+// once; the m = bw rows between them form a dense middle block, S's own
+// entries there plus the Schur complements of both ends (the ends share no
+// entry: they are more than bw rows apart).  Its substitutions likewise run
+// from both ends towards the middle, solve the middle block, and run back
+// out.  twist_ok: the bands long enough for it (both ends at least bw + 16
+// pivots).
+struct Twist {
+  int p, q, m;   // pivots of the top end, of the bottom end, middle rows
+  int nT, nR;    // rows of T (p + m) and of R (q + m)
+};
+__host__ __device__ __forceinline__ bool twist_ok(int n, int bw) { return n >= 4 * bw + 32; }
+__host__ __device__ __forceinline__ Twist twist_split(int n, int bw) {
+  Twist t;
+  t.m = bw;
+  t.p = (n - bw + 1) / 2;
+  t.q = n - t.p - bw;
+  t.nT = t.p + bw;
+  t.nR = t.q + bw;
+  return t;
+}
+// doubles of the twisted factors (row stride BWc + 1): T with BWc zero rows
+// after it, R likewise, 1/D of T (nT + 1 entries), of R (nR + 1), the middle
+// block's unit lower L (m x m, row-major) and its 1/D (m)
+__host__ __device__ __forceinline__ size_t twist_doubles(int n, int bw, int BWc) {
+  const Twist t = twist_split(n, bw);
+  return (size_t)(t.nT + BWc + t.nR + BWc) * (BWc + 1) + t.nT + 1 + t.nR + 1 + (size_t)bw * bw + bw;
+}
+
+// doubles per (slot, wing) of GeoBuffers::bandg: the factors L D L^T, either
+// (SPW + BMAX) band rows of stride BMAX + 1 at most then SPW + 1 entries of
+// 1/D, or the twisted factors (at most SPW + 3 BMAX rows, SPW + 2 BMAX + 2
+// entries of 1/D, BMAX^2 + BMAX for the middle block)
 __host__ __device__ __forceinline__ size_t band_cap(const IbmParams& p) {
-  return (size_t)(p.SPW + BMAX) * (BMAX + 1) + p.SPW + 1;
+  return (size_t)(p.SPW + 3 * BMAX) * (BMAX + 1) + p.SPW + 2 * BMAX + 2 + BMAX * BMAX + BMAX;
 }
 
 // The adjoint nodes of one step, which the full-lattice kernel leaves to
@@ -254,6 +288,11 @@ constexpr int kLatticeTiles[3] = {128, 256, 512};
 void launch_lattice(hipStream_t s, const Grid& g, const MrtParams& P, const void* src, void* dst, FieldSlot* fields,
                     bool f32, int bx, int cp, const AdjMask& mask, hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr,
                     size_t lds = 0);
+// The lattice step of a coupled step (fields written, every interior node):
+// at most 8 waves per CU, held there by the kernel's register count, so that
+// no LDS is taken from the step's chain
+void launch_lattice_narrow(hipStream_t s, const Grid& g, const MrtParams& P, const void* src, void* dst,
+                           const FieldSlot& fields, bool f32, int bx, int cp, const AdjMask& mask);
 // Dynamic LDS that lets at most `waves` waves of bx-thread workgroups share a
 // CU of lds_cu bytes of LDS: 0 for waves = 0 (no cap), -1 when no size does.
 // No size does when it would exceed lds_blk, the device's LDS per workgroup.
@@ -309,7 +348,9 @@ int adj_collide_blocks_per_wing(const IbmParams& p);
 void launch_geometry(hipStream_t s, const IbmParams& p, const GeoBuffers& G, DevStep* st, int slot0, int n0,
                      int fidx0, int count, bool with_inverse, bool direct = false, bool apart = false);
 constexpr int APART_GAP = 4;
-size_t geo_solve_lds_bytes(const IbmParams& p);
+size_t geo_solve_lds_bytes(const IbmParams& p, bool direct);   // geometry solve (direct: twisted factors possible)
+size_t geo_factor_lds_bytes(const IbmParams& p);   // the plain factors alone (the inverse)
+size_t bsolve_lds_bytes(const IbmParams& p);       // band substitution of one wing
 size_t geo_couple_lds_total();   // static + dynamic LDS of the coupling workgroup
 // Batched launches over the episodes of eps (device array).  The substep of a
 // launch is *it.kbase + k (the base lives on the device, so that a captured
@@ -321,15 +362,17 @@ void launch_batch_lattice(hipStream_t s, const EpDesc* eps, const BatchItems& it
                           int bx, int rows, int cp, bool masked, hipEvent_t t1 = nullptr, size_t lds = 0);
 void launch_batch_bc_ring(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields);
 void launch_batch_adj_collide(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields);
-// direct: some episode solves by band substitution (EpDesc::direct)
-void launch_batch_ibm_flow(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool direct);
+// bsolve_lds: LDS of the band substitution (bsolve_lds_bytes, the largest of
+// the episodes that solve by it: EpDesc::direct), 0 when none does
+void launch_batch_ibm_flow(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32,
+                           size_t bsolve_lds);
 void launch_batch_force_row(hipStream_t s, const EpDesc* eps, const int* kb, int count, int k);
 // *p = v (add = false) or *p += v (add = true), one thread, in stream order
 void launch_step_base(hipStream_t s, int* p, int v, bool add);
 // Geometry of substeps *kb + k0 .. *kb + k0 + count - 1 of every episode with
 // wings; items are (episode, b*W + w) for b < count.
 void launch_batch_geometry(hipStream_t s, const EpDesc* eps, const int* kb, const int2* items, int n_items,
-                           int k0, size_t lds_wing, size_t lds_solve, int inv_chunks);
+                           int k0, size_t lds_wing, size_t lds_solve, size_t lds_inv, int inv_chunks);
 // workgroups of the inverse per (step, wing) of a wing of spw segments
 int inverse_chunks(int spw);
 size_t geo_wing_lds_bytes(int spw);

@@ -187,6 +187,33 @@ __global__ __launch_bounds__(BX) void k_lattice_f32(Grid g, MrtParams P, const f
   lattice_f32_node<FIELDS, CP>(g, P, src, dst, fs, tx * BX + (int)threadIdx.x, 1 + ty, am);
 }
 
+// The kernels of a coupled step (fields written, every interior node), held
+// to two waves per SIMD (8 per CU) by their register count instead of by
+// unused LDS: an empty asm statement that clobbers v255 makes the kernel
+// declare 256 VGPRs, so no third wave of it fits a SIMD, and the CU's LDS
+// stays free for the chain beside it, whose geometry solve and band
+// substitution stage 50-100 KB each.  (A grid of 8 waves' worth of
+// workgroups per CU striding over the tiles held the same cap with nothing
+// reserved, but its long-lived waves starved the chain: the marker pass took
+// 277 instead of 85 us, the coupled step 0.50 instead of 0.34-0.36 ms.)
+template <int BX, int CP>
+__global__ __launch_bounds__(BX) void k_lattice_narrow(Grid g, MrtParams P, const double* __restrict__ src,
+                                                       double* __restrict__ dst, FieldSlot fs, int tiles_x,
+                                                       AdjMask am) {
+  asm volatile("" ::: "v255");
+  const int ty = blockIdx.x / tiles_x, tx = blockIdx.x - ty * tiles_x;
+  lattice_node<true, CP>(g, P, src, dst, fs, tx * BX + (int)threadIdx.x, 1 + ty, am);
+}
+
+template <int BX, int CP>
+__global__ __launch_bounds__(BX) void k_lattice_f32_narrow(Grid g, MrtParams P, const float* __restrict__ src,
+                                                           float* __restrict__ dst, FieldSlot fs, int tiles_x,
+                                                           AdjMask am) {
+  asm volatile("" ::: "v255");
+  const int ty = blockIdx.x / tiles_x, tx = blockIdx.x - ty * tiles_x;
+  lattice_f32_node<true, CP>(g, P, src, dst, fs, tx * BX + (int)threadIdx.x, 1 + ty, am);
+}
+
 int lattice_tiles_x(const Grid& g, int bx) { return (g.nx + bx - 1) / bx; }
 int lattice_tiles_flat(const Grid& g, int bx) { return (int)(((long long)(g.ny - 2) * g.nx + bx - 1) / bx); }
 
@@ -213,6 +240,40 @@ void launch_lattice_cp(hipStream_t s, const Grid& g, const MrtParams& P, const v
     else
       hipExtLaunchKernelGGL((k_lattice<false, BX, CP>), grid, block, lds, s, t0, t1, 0, g, P, a, b, fs, tiles_x, am);
   }
+}
+
+template <int BX, int CP>
+void launch_narrow_cp(hipStream_t s, const Grid& g, const MrtParams& P, const void* src, void* dst,
+                      const FieldSlot& fs, bool f32, const AdjMask& am) {
+  const int tiles_x = lattice_tiles_x(g, BX);
+  dim3 grid(tiles_x * (g.ny - 2)), block(BX);
+  if (f32)
+    hipLaunchKernelGGL((k_lattice_f32_narrow<BX, CP>), grid, block, 0, s, g, P, (const float*)src, (float*)dst, fs,
+                       tiles_x, am);
+  else
+    hipLaunchKernelGGL((k_lattice_narrow<BX, CP>), grid, block, 0, s, g, P, (const double*)src, (double*)dst, fs,
+                       tiles_x, am);
+}
+
+template <int BX>
+void launch_narrow_bx(hipStream_t s, const Grid& g, const MrtParams& P, const void* src, void* dst,
+                      const FieldSlot& fs, bool f32, int cp, const AdjMask& am) {
+  if (cp == CP_NT)
+    launch_narrow_cp<BX, CP_NT>(s, g, P, src, dst, fs, f32, am);
+  else if (cp == CP_NT_STORE)
+    launch_narrow_cp<BX, CP_NT_STORE>(s, g, P, src, dst, fs, f32, am);
+  else
+    launch_narrow_cp<BX, CP_PLAIN>(s, g, P, src, dst, fs, f32, am);
+}
+
+void launch_lattice_narrow(hipStream_t s, const Grid& g, const MrtParams& P, const void* src, void* dst,
+                           const FieldSlot& fields, bool f32, int bx, int cp, const AdjMask& mask) {
+  if (bx == 128)
+    launch_narrow_bx<128>(s, g, P, src, dst, fields, f32, cp, mask);
+  else if (bx == 512)
+    launch_narrow_bx<512>(s, g, P, src, dst, fields, f32, cp, mask);
+  else
+    launch_narrow_bx<256>(s, g, P, src, dst, fields, f32, cp, mask);
 }
 
 template <int BX>
@@ -755,7 +816,7 @@ __global__ __launch_bounds__(ERR_THREADS) void k_err_partial(Grid g, FieldSlot c
   __shared__ double s1[ERR_THREADS], s2[ERR_THREADS];
   double a1 = 0.0, a2 = 0.0;
   const int j = 1 + blockIdx.x;
-  constexpr int U = 4;   // nodes per thread whose loads are issued together
+  constexpr int U = 16;  // nodes per thread whose loads are issued together (a whole row up to nx = 4098)
   for (int i0 = 1 + threadIdx.x; i0 <= g.nx - 2; i0 += U * ERR_THREADS) {
     double cu[U], cv[U], pu[U], pv[U];
 #pragma unroll
