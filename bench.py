@@ -186,6 +186,7 @@ def run_batch(args):
     import numpy as np
     from lbm_mutiwing_amd import (PRECISION_FP32_SHIFTED, PRECISION_FP64, Batch, Solver, random_episode,
                                   seeded_motions, sinusoidal_motion_table)
+    from lbm_mutiwing_amd.solver import IBM_SOLVE_SUBSTITUTION
     n_ep = args.episodes or 64
     K = args.substeps
     rng = np.random.default_rng(args.seed)
@@ -194,17 +195,19 @@ def run_batch(args):
     tables = [sinusoidal_motion_table(c, seeded_motions(c, args.seed + i), warm + 3 * K + 2)
               for i, c in enumerate(cfgs)]
 
-    def build(precision, idx):
+    def build(precision, idx, batched=False):
         out = []
         for i in idx:
             c = cfgs[i]
             c.precision = precision
             s = Solver(c)
+            if batched:   # no inverse formed: the geometry of 64 episodes weighs less beside the lattice kernel
+                s.set_ibm_solve(IBM_SOLVE_SUBSTITUTION)
             s.set_motion_table(tables[i])
             out.append(s)
         return out
 
-    solvers = build(PRECISION_FP32_SHIFTED, range(n_ep))
+    solvers = build(PRECISION_FP32_SHIFTED, range(n_ep), batched=True)
     b = Batch()
     note(f"batch: {n_ep} episodes built, warm-up of {warm} steps")
     b.advance(solvers, warm)
@@ -274,11 +277,13 @@ def _batch_oracle_error(cfgs, tables, idx, steps):
     and mass compared with the oracle's."""
     import dataclasses
     from lbm_mutiwing_amd import PRECISION_FP32_SHIFTED, PRECISION_FP64, Batch, Solver
+    from lbm_mutiwing_amd.solver import IBM_SOLVE_SUBSTITUTION
     from oracle import oracle
     oracle.set_threads(min(16, os.cpu_count() or 1))
     ss = []
     for i in idx:
         g = Solver(dataclasses.replace(cfgs[i], precision=PRECISION_FP32_SHIFTED))
+        g.set_ibm_solve(IBM_SOLVE_SUBSTITUTION)   # as the batched episodes are solved
         g.set_motion_table(tables[i])
         ss.append(g)
     b = Batch()
@@ -469,7 +474,7 @@ def _mix_sensitivity(args, episodes, cmds, r64, n_cpl):
     import numpy as np
     from lbm_mutiwing_amd import MOTION_STREAM, PRECISION_FP64, preset
     from lbm_mutiwing_amd.protocol import Session, run_job_alone
-    from lbm_mutiwing_amd.solver import Dynamics, Solver
+    from lbm_mutiwing_amd.solver import IBM_SOLVE_SUBSTITUTION, Dynamics, Solver
     w = np.array([4.0 / 9.0] + [1.0 / 9.0] * 4 + [1.0 / 36.0] * 4, dtype=np.float32).astype(np.float64)
     rng = np.random.default_rng(args.seed + 7)
     out = {"episodes": [], "replay_equals_served": True, "perturbed_max": 0.0, "perturbed_max_by_step": None}
@@ -479,6 +484,7 @@ def _mix_sensitivity(args, episodes, cmds, r64, n_cpl):
             cfg = preset("c5")
             cfg.precision, cfg.motion_source = PRECISION_FP64, MOTION_STREAM
             s = Solver(cfg)
+            s.set_ibm_solve(IBM_SOLVE_SUBSTITUTION)   # as the server solves its episodes
             if i < n_cpl:
                 s.set_dynamics(Dynamics())
             sess = Session(s)

@@ -235,6 +235,26 @@ typedef struct lbm_dynamics {
 } lbm_dynamics;
 int lbm_set_dynamics(lbm_handle *h, const lbm_dynamics *d);
 
+/* How the IBM system S y = c of a step is solved at flow time (no
+ * counterpart in the reference, which solves its dense system by LU,
+ * boundary_condition.f90:1618-1640; every mode gives its solution to
+ * rounding): from the explicit inverse S^-1, formed by the geometry ahead of
+ * the flow (LBM_IBM_SOLVE_INVERSE); by band substitution with the factors of
+ * S in steps with rigid-body coupling, whose geometry is on the step's
+ * critical path, and from the inverse otherwise (LBM_IBM_SOLVE_COUPLED, the
+ * default); or by band substitution in every step (LBM_IBM_SOLVE_SUBSTITUTION:
+ * no inverse is formed, which suits instances advanced together by
+ * lbm_batch_advance, where the geometry of all episodes competes with the
+ * lattice kernel; a single instance with prescribed motion is faster with
+ * the inverse).  Steps whose wings' supports meet use the inverse in every
+ * mode.  The batched advance solves each instance in its own mode, so an
+ * instance gives the same bits alone and in a batch.  LBM_DIRECT_SOLVE=<mode>
+ * in the environment sets the default at lbm_create. */
+#define LBM_IBM_SOLVE_INVERSE 0
+#define LBM_IBM_SOLVE_COUPLED 1
+#define LBM_IBM_SOLVE_SUBSTITUTION 2
+int lbm_set_ibm_solve(lbm_handle *h, int32_t mode);
+
 /* Measurement hook (no counterpart in the reference): when enabled, every
  * launch of the fused lattice kernel is bracketed by HIP events on the
  * instance's stream; lbm_get_timing returns the summed kernel time (ms) and

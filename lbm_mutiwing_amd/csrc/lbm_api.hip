@@ -689,7 +689,7 @@ int lbm_create(const lbm_config* cfg, lbm_handle** out) try {
     const char* e = std::getenv("LBM_GEO_CARRY");
     h->carry_on = !(e && e[0] == '0');
     const char* d = std::getenv("LBM_DIRECT_SOLVE");
-    h->direct = d ? std::max(0, std::min(2, atoi(d))) : 1;
+    h->direct = d ? std::max(0, std::min(2, atoi(d))) : LBM_IBM_SOLVE_COUPLED;
   }
 
   int rc = 0;
@@ -1664,6 +1664,15 @@ int lbm_set_dynamics(lbm_handle* h, const lbm_dynamics* d) {
   p.dyn_p1 = d->par1 * uu * uu * sc * sc;
   p.dyn_p2 = d->par2 * (sc * sc * sc * sc);
   p.dyn_alphad = d->alphad;
+  return 0;
+}
+
+int lbm_set_ibm_solve(lbm_handle* h, int32_t mode) {
+  if (!h) return LBM_E_INVALID;
+  if (mode < LBM_IBM_SOLVE_INVERSE || mode > LBM_IBM_SOLVE_SUBSTITUTION)
+    return fail(h, LBM_E_INVALID, "IBM solve mode must be 0, 1 or 2");
+  h->direct = mode;
+  h->geo_epoch++;   // geometry computed ahead for the other mode is not carried over
   return 0;
 }
 

@@ -174,6 +174,13 @@ module lbm_hip
             type(lbm_dynamics), intent(in) :: d
         end function
 
+        ! mode: 0 explicit inverse, 1 band substitution in coupled steps (default), 2 in every step
+        integer(c_int) function lbm_set_ibm_solve(h, mode) bind(C, name='lbm_set_ibm_solve')
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: h
+            integer(c_int32_t), value :: mode
+        end function
+
         integer(c_int) function lbm_last_error(h, buf, n) bind(C, name='lbm_last_error')
             import :: c_int, c_ptr, c_char, c_size_t
             type(c_ptr), value :: h

@@ -35,7 +35,7 @@ from typing import Callable, Dict, List, Optional, Tuple
 
 from .config import PRECISION_FP32_SHIFTED, SolverConfig
 from .protocol import CMD_RESET, CMD_STOP, Session, parse_command
-from .solver import Batch, LbmError, Solver
+from .solver import IBM_SOLVE_SUBSTITUTION, Batch, LbmError, Solver
 
 
 # lbm_batch_advance captures advances of at most this many steps as HIP
@@ -154,6 +154,10 @@ class EpisodeServer:
                 return
             self.waiting.popleft()
             solver = Solver(cfg, self.lib)
+            # batched episodes solve the IBM system by band substitution:
+            # no inverse is formed, so the geometry of all episodes weighs
+            # less beside the batched lattice kernel
+            solver.set_ibm_solve(IBM_SOLVE_SUBSTITUTION)
             if self.dynamics.get(client) is not None:
                 solver.set_dynamics(self.dynamics[client])
             sess = Session(solver, self.save_hook)

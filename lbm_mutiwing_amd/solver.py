@@ -47,6 +47,8 @@ _PROTOS = {
 }
 
 
+IBM_SOLVE_INVERSE, IBM_SOLVE_COUPLED, IBM_SOLVE_SUBSTITUTION = 0, 1, 2   # lbm_set_ibm_solve
+
 BODY_REL_FIELDS = ("a", "a_prev", "v", "v_prev", "s", "s_prev", "pos", "theta_prev", "theta", "temp_theta_prev",
                    "temp_theta")
 
@@ -266,6 +268,17 @@ class Solver:
         (None)."""
         d = dyn if dyn is not None else Dynamics(enable=False)
         self._check(self._fn("set_dynamics")(self._h, ctypes.byref(d)))
+
+    def set_ibm_solve(self, mode: int) -> None:
+        """How the IBM system is solved at flow time (lbm_set_ibm_solve):
+        IBM_SOLVE_INVERSE, IBM_SOLVE_COUPLED (the default) or
+        IBM_SOLVE_SUBSTITUTION.  A library without the entry point (the CPU
+        oracle, which solves its dense system directly) has no modes."""
+        fn = getattr(self.lib, "lbm_set_ibm_solve", None)
+        if fn is None:
+            return
+        fn.restype, fn.argtypes = ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]
+        self._check(fn(self._h, int(mode)))
 
     def tuning(self) -> int:
         """Lattice tile width chosen by the autotuner (HIP library only)."""

@@ -18,6 +18,7 @@ from lbm_mutiwing_amd import MOTION_STREAM, Solver, formats, preset
 from lbm_mutiwing_amd.build import zmq_library
 from lbm_mutiwing_amd.case import case_config, read_case
 from lbm_mutiwing_amd.protocol import Session, run_job_alone
+from lbm_mutiwing_amd.solver import IBM_SOLVE_SUBSTITUTION
 
 pytestmark = pytest.mark.gpu
 
@@ -91,7 +92,9 @@ def test_three_clients_match_alone(tmp_path, monkeypatch):
     assert rc == 0, (tmp_path / "server.log").read_text()
 
     for k in range(3):
-        sess = Session(Solver(base), save_hook=formats.tecplot_save_hook(str(tmp_path / f"alone_{k}")))
+        s = Solver(base)
+        s.set_ibm_solve(IBM_SOLVE_SUBSTITUTION)   # as the server solves its episodes
+        sess = Session(s, save_hook=formats.tecplot_save_hook(str(tmp_path / f"alone_{k}")))
         want = []
         for msg in cmds[k]:
             reply, job = sess.receive(msg)
