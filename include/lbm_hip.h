@@ -326,8 +326,8 @@ int lbm_batch_get_tuning(const lbm_batch *b, int32_t *lattice_tile,
 
 /* Cache policy of the lattice kernel's population loads and stores, chosen
  * by the same timing (no counterpart in the reference): 0 ordinary, 1
- * non-temporal stores, 2 non-temporal loads and stores (a batch times 0 and
- * 2 with its wave cap).  LBM_LATTICE_CP=<0|1|2> in the environment fixes it. */
+ * non-temporal stores, 2 non-temporal loads and stores (a batch times all
+ * three with its wave cap).  LBM_LATTICE_CP=<0|1|2> in the environment fixes it. */
 #define LBM_CACHE_PLAIN 0
 #define LBM_CACHE_NT_STORE 1
 #define LBM_CACHE_NT 2

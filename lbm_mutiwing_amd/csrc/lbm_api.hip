@@ -1968,11 +1968,11 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
     // then the wave cap and cache policy for that shape (cp * 64 + waves)
     const int fw = env_waves(), fcp = env_cp(), lds_cu = lds_per_cu(b->device), lds_blk = lds_per_block(b->device);
     std::vector<int> wc;
-    for (int cp : {CP_PLAIN, CP_NT})
+    for (int cp : {CP_PLAIN, CP_NT_STORE, CP_NT})
       for (int wv : kLatticeWaves)
         if ((fcp < 0 || cp == fcp) && (fw < 0 || wv == fw) && lattice_lds_for_waves(b->lat_bx, wv, lds_cu, lds_blk) >= 0)
           wc.push_back(cp * 64 + wv);
-    if (wc.empty()) wc.push_back(fcp == CP_NT ? CP_NT * 64 : 0);
+    if (wc.empty()) wc.push_back(fcp > 0 ? fcp * 64 : 0);
     items.clear();
     for (int e = 0; e < count; e++) {
       desc[e].tiles = lattice_tiles_flat(desc[e].g, b->lat_bx);

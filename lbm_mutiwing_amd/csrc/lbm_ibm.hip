@@ -125,11 +125,6 @@ __device__ WingMotionState wing_motion(const IbmParams& p, const GeoBuffers& G, 
 
 // ------------------------------------------------ geometry: markers, list
 
-// Threads of a geometry workgroup (marker pass and band factor).  A single
-// instance issues few (step, wing) workgroups per geometry batch, so each is
-// wide; a batch of episodes issues hundreds, which fill the GPU better as
-// narrower workgroups (measured on 64 episodes: 74.3 -> 79.5 env-steps/s at
-// 256; a single 4096x2048 instance with two wings loses 10 % at 256).
 // dst[e] = src[e] for e < count by NT threads, LOAD_BATCH loads per thread in
 // flight at a time: a plain copy loop waits for each load before the next
 // (the store needs its value), one memory latency per element per thread
@@ -149,8 +144,14 @@ __device__ __forceinline__ void copy_batched(T* __restrict__ dst, const T* __res
   }
 }
 
+// Threads of a geometry workgroup (marker pass and band factor).  A single
+// instance issues few (step, wing) workgroups per geometry batch, so each is
+// wide; a batch of episodes issues thousands, which fill the GPU better as
+// narrower workgroups (measured on 64 episodes: 512 -> 256 -> 128 threads
+// 74.3 -> 79.5 -> 80.9 env-steps/s; a single 4096x2048 instance with two
+// wings loses 10 % at 256).
 constexpr int GEO_THREADS = 512;
-constexpr int GEO_THREADS_BATCH = 256;
+constexpr int GEO_THREADS_BATCH = 128;
 constexpr int CLAIM_UNROLL = 12;
 
 size_t geo_wing_lds_bytes(int spw) {
@@ -414,23 +415,25 @@ __device__ __forceinline__ double s_entry_flat(const int* box, const double* phx
 }
 
 // dst[e] = S(k, l) for e < count, where entry(e, k, l) names the segments of
-// entry e (false: the entry is zero), ASSEMBLE_BATCH entries per thread at a
-// time with all their loads issued before their sums
-constexpr int ASSEMBLE_BATCH = 4;
+// entry e (false: the entry is zero), AB entries per thread at a time with
+// all their loads issued before their sums.  AB = 4 for the wide workgroups
+// of a single instance, whose few solves wait on memory latency; 1 for the
+// batch, whose many solves share the CUs (AB = 4 takes 231 VGPRs a lane).
 template <int NT, typename Entry>
 __device__ __forceinline__ void assemble_s(double* dst, int count, const int* sbox, const double* phx,
                                            const double* phy, Entry entry) {
-  for (int e0 = (int)threadIdx.x; e0 < count; e0 += ASSEMBLE_BATCH * NT) {
-    double v[ASSEMBLE_BATCH];
+  constexpr int AB = NT >= GEO_THREADS ? 4 : 1;
+  for (int e0 = (int)threadIdx.x; e0 < count; e0 += AB * NT) {
+    double v[AB];
 #pragma unroll
-    for (int u = 0; u < ASSEMBLE_BATCH; u++) {
+    for (int u = 0; u < AB; u++) {
       int k = 0, l = 0;
       const bool ok = entry(e0 + u * NT, k, l);
       const double x = s_entry_flat(sbox, phx, phy, ok ? k : 0, ok ? l : 0);
       v[u] = ok ? x : 0.0;
     }
 #pragma unroll
-    for (int u = 0; u < ASSEMBLE_BATCH; u++)
+    for (int u = 0; u < AB; u++)
       if (e0 + u * NT < count) dst[e0 + u * NT] = v[u];
   }
 }
@@ -832,7 +835,8 @@ __device__ __forceinline__ void geo_solve_body(const IbmParams& p, const GeoBuff
   double* bg = G.bandg + ((size_t)slot * p.W + w) * band_cap(p);
   const int* sbox = reinterpret_cast<const int*>(lds_s);   // the boxes, staged above
   double* fac = lds_s + 2 * (size_t)n;                      // the factors after them
-  if (direct && !s_meet && twist_ok(n, bw)) {   // factors for the band substitution only: twisted
+  // factors for a coupled step's band substitution (on its critical path): twisted
+  if (direct && !s_meet && uses_twist(p, bw)) {
     geo_twisted_factor<NT>(fac, sbox, phx, phy, n, bw, BWc, st, w, bg);
     return;
   }
@@ -916,7 +920,7 @@ size_t geo_factor_lds_bytes(const IbmParams& p) {
 size_t geo_solve_lds_bytes(const IbmParams& p, bool direct) {
   const size_t c = (size_t)p.band_cls;
   const size_t twisted = ((p.SPW + 3 * c) * (c + 1) + p.SPW + c + 2 + 2 * c * c + c) * sizeof(double);
-  return 2 * (size_t)p.SPW * sizeof(double) + std::max(geo_factor_lds_bytes(p), direct ? twisted : 0);
+  return 2 * (size_t)p.SPW * sizeof(double) + std::max(geo_factor_lds_bytes(p), direct && p.dyn_on ? twisted : 0);
 }
 
 // the band substitution's LDS: a wing's factors (plain or twisted, with
@@ -924,7 +928,7 @@ size_t geo_solve_lds_bytes(const IbmParams& p, bool direct) {
 size_t bsolve_lds_bytes(const IbmParams& p) {
   const size_t c = (size_t)p.band_cls;
   const size_t plain = (p.SPW + c) * (c + 1) + p.SPW + 1;
-  const size_t twisted = (p.SPW + 3 * c) * (c + 1) + p.SPW + 2 * c + 2 + c * c + c;
+  const size_t twisted = p.dyn_on ? (p.SPW + 3 * c) * (c + 1) + p.SPW + 2 * c + 2 + c * c + c : 0;
   return (std::max(plain, twisted) + 2 * BAND_RING * (c + 1) + 4 * (size_t)p.SPW) * sizeof(double);
 }
 
@@ -1431,8 +1435,8 @@ __device__ void twisted_solve(const double* band, int n, int bw, const double* c
 }
 
 // doubles of a wing's factors in bandg (twisted or plain, by the band)
-__device__ __forceinline__ size_t factor_doubles(int n, int bw, int BWc) {
-  return twist_ok(n, bw) ? twist_doubles(n, bw, BWc) : (size_t)(n + BWc) * (BWc + 1) + n + 1;
+__device__ __forceinline__ size_t factor_doubles(int n, int bw, int BWc, bool twisted) {
+  return twisted ? twist_doubles(n, bw, BWc) : (size_t)(n + BWc) * (BWc + 1) + n + 1;
 }
 
 // The band substitutions run from LDS: a wing's factors (with BAND_RING rows
@@ -1457,14 +1461,15 @@ __device__ __forceinline__ void ibm_bsolve_body(const IbmParams& p, const GeoBuf
   const int BWc = band_class(bw);
   const double* gband = G.bandg + ((size_t)slot * p.W + w) * band_cap(p);
   const int pad = BAND_RING * (BWc + 1);
-  const int nb = (int)factor_doubles(n, bw, BWc) + 2 * pad;
+  const bool twisted = uses_twist(p, bw);
+  const int nb = (int)factor_doubles(n, bw, BWc, twisted) + 2 * pad;
   double* band = lds_b + pad;
   double* c = lds_b + nb;
   double* y = c + 2 * n;
   copy_batched<BSOLVE_THREADS>(lds_b, gband - pad, nb);
   copy_batched<BSOLVE_THREADS>(c, F.cvec + 2 * (size_t)w * n, 2 * n);
   __syncthreads();
-  if (twist_ok(n, bw)) {   // the geometry solve left twisted factors (a step that forms no inverse)
+  if (twisted) {   // the geometry solve left twisted factors (a coupled step that forms no inverse)
     if (BWc == 8)
       twisted_solve<8>(band, n, bw, c + comp, y + comp);
     else if (BWc == 12)

@@ -154,6 +154,13 @@ __host__ __device__ __forceinline__ Twist twist_split(int n, int bw) {
   t.nR = t.q + bw;
   return t;
 }
+// The twisted factors serve the steps with rigid-body coupling (p.dyn_on),
+// whose solve is on the step's critical path; with prescribed motion the
+// solve runs beside the lattice kernel, and the plain factors take half the
+// LDS (which the batched geometry of many episodes shares).
+__host__ __device__ __forceinline__ bool uses_twist(const IbmParams& p, int bw) {
+  return p.dyn_on && twist_ok(p.SPW, bw);
+}
 // doubles of the twisted factors (row stride BWc + 1): T with BWc zero rows
 // after it, R likewise, 1/D of T (nT + 1 entries), of R (nR + 1), the middle
 // block's unit lower L (m x m, row-major) and its 1/D (m)

@@ -1123,12 +1123,13 @@ void launch_batch_lattice_cp(hipStream_t s, const EpDesc* eps, const BatchItems&
   }
 }
 
-// The batch times CP_PLAIN and CP_NT (non-temporal loads and stores).
 template <int BX, int ROWS>
 void launch_batch_lattice_bx(hipStream_t s, const EpDesc* eps, const BatchItems& it, int k, bool f32, bool fields,
                              int cp, int masked, hipEvent_t t1, size_t lds) {
   if (cp == CP_NT)
     launch_batch_lattice_cp<BX, ROWS, CP_NT>(s, eps, it, k, f32, fields, masked, t1, lds);
+  else if (cp == CP_NT_STORE)
+    launch_batch_lattice_cp<BX, ROWS, CP_NT_STORE>(s, eps, it, k, f32, fields, masked, t1, lds);
   else
     launch_batch_lattice_cp<BX, ROWS, CP_PLAIN>(s, eps, it, k, f32, fields, masked, t1, lds);
 }
