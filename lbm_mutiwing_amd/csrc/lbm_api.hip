@@ -1978,12 +1978,18 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
     if (rc) return rc;
     b->lat_bx = best / 8;
     b->lat_rows = best % 8;
-    // then the wave cap and cache policy for that shape (cp * 64 + waves)
+    // then the cache policy for that shape (cp * 64 + waves).  No wave cap
+    // unless LBM_LATTICE_WAVES asks for one: timed alone, a cap can win by a
+    // few per cent, but in the batch's steps, beside the IBM kernels and the
+    // geometry of many episodes, the uncapped kernel was faster (64 uneven
+    // episodes: tile 512, one tile per item, cap none 83.9, cap 24 83.6,
+    // cap 16 81.2 env-steps/s)
     const int fw = env_waves(), fcp = env_cp(), lds_cu = lds_per_cu(b->device), lds_blk = lds_per_block(b->device);
     std::vector<int> wc;
     for (int cp : {CP_PLAIN, CP_NT_STORE, CP_NT})
       for (int wv : kLatticeWaves)
-        if ((fcp < 0 || cp == fcp) && (fw < 0 || wv == fw) && lattice_lds_for_waves(b->lat_bx, wv, lds_cu, lds_blk) >= 0)
+        if ((fcp < 0 || cp == fcp) && (fw < 0 ? wv == 0 : wv == fw) &&
+            lattice_lds_for_waves(b->lat_bx, wv, lds_cu, lds_blk) >= 0)
           wc.push_back(cp * 64 + wv);
     if (wc.empty()) wc.push_back(fcp > 0 ? fcp * 64 : 0);
     items.clear();
