@@ -101,3 +101,47 @@ def test_substitution_matches_inverse_with_coupling(lib, monkeypatch):
         assert rel(a, b) < 1e-12
     assert rel(rd.force_rows, ri.force_rows) < 1e-11
     assert np.allclose(bd, bi, rtol=1e-11, atol=1e-16)
+
+
+def _run_mode(cfg, table, mode, dyn=None, rel0=None, steps=N):
+    """As _run, with the solve mode set through lbm_set_ibm_solve."""
+    s = Solver(cfg)
+    s.set_ibm_solve(mode)
+    s.set_motion_table(table)
+    if dyn is not None:
+        s.set_dynamics(dyn)
+    if rel0 is not None:
+        s.set_body_rel(rel0)
+    r = s.advance(steps, want_rows=True)
+    out = (r, s.fields()[:3], s.body_rel(), s.state()["movenum"])
+    s.close()
+    return out
+
+
+def test_solve_mode_entry_point(lib):
+    """lbm_set_ibm_solve: the twisted substitutions of coupled steps at C1
+    (126 segments per wing) and the plain ones of prescribed steps against
+    the inverse, to rounding; a mode outside 0..2 is an error."""
+    from lbm_mutiwing_amd.solver import (IBM_SOLVE_COUPLED, IBM_SOLVE_INVERSE, IBM_SOLVE_SUBSTITUTION,
+                                         LbmError)
+    cfg = preset("c1")
+    table = sinusoidal_motion_table(cfg, seeded_motions(cfg, 7), N + 2)
+    print("c1 coupled: twisted substitution vs inverse", flush=True)
+    dyn = Dynamics(rk4=True, solid_den=50.0, par1=-2.0, par2=50.0)
+    ri, fi, bi, mi = _run_mode(cfg, table, IBM_SOLVE_INVERSE, dyn)
+    rt, ft, bt, mt = _run_mode(cfg, table, IBM_SOLVE_COUPLED, dyn)
+    assert mt == mi
+    for a, b in zip(ft, fi):
+        assert rel(a, b) < 1e-12
+    assert rel(rt.force_rows, ri.force_rows) < 1e-11
+    assert np.allclose(bt, bi, rtol=1e-11, atol=1e-16)
+    print("c1 prescribed: plain substitution vs inverse", flush=True)
+    ri, fi, _, _ = _run_mode(cfg, table, IBM_SOLVE_INVERSE)
+    rp, fp, _, _ = _run_mode(cfg, table, IBM_SOLVE_SUBSTITUTION)
+    for a, b in zip(fp, fi):
+        assert rel(a, b) < 1e-12
+    assert rel(rp.force_rows, ri.force_rows) < 1e-11
+    s = Solver(cfg)
+    with pytest.raises(LbmError):
+        s.set_ibm_solve(3)
+    s.close()
