@@ -941,8 +941,7 @@ int step_error(lbm_handle* h, const DevStep& st) {
 }
 
 int finish_advance(lbm_handle* h, int nsteps, int n_last, bool timed, double* force_rows, double* motion_rows,
-                   double* row_sum, double* last_row, double* err, bool err_issued = false);
-bool issue_err_fields(lbm_handle* h, int ml, hipStream_t s);
+                   double* row_sum, double* last_row, double* err);
 
 // Advance of an instance with rigid-body coupling: the geometry of step m
 // needs the body state left by the force of step m-1, and the move_mesh check
@@ -1213,7 +1212,6 @@ int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motio
   // each lattice launch (no event packet between two lattice kernels), or a
   // recorded event after a move_mesh
   hipEvent_t step_ev = h->ev_step;
-  bool err_issued = false;
   for (int m = n0; m <= n_last; m++) {
     const int k = m - n0, kv = k + r;
     const bool batch_start = h->W > 0 && (k == 0 || kv % ns == 0);
@@ -1249,10 +1247,6 @@ int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motio
     // next batch last, so that the step's longest kernel is queued first.
     HIPCHECK(h, hipStreamWaitEvent(h->stream_ibm, step_ev, 0));
     HIPCHECK(h, hipStreamWaitEvent(h->stream_bc, step_ev, 0));
-    // the erranalysis of the call (u of steps n_last-1 and n_last-2, both
-    // complete once the IBM stream has passed step_ev) runs on the IBM
-    // stream beside the last step's lattice kernel instead of after it
-    if (m == n_last && nsteps >= 1 && issue_err_fields(h, n_last, h->stream_ibm)) err_issued = true;
     step_ev = h->timing ? h->ev[2 * k + 1] : h->ev_step;
     launch_lattice(h->stream, h->g, h->mrt, src, dst, fs, h->f32, h->lat_bx, h->lat_cp, adj_mask(h->ip, h->G, slot),
                    h->timing ? h->ev[2 * k] : nullptr, step_ev, h->lat_lds);
@@ -1292,7 +1286,7 @@ int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motio
   h->batch_count = bc0 + (unsigned)nbatch;
   if (nsteps > 0 && h->W > 0) h->last_slot = slot;
   h->n = std::max(n0, n_last + 1);
-  rc = finish_advance(h, nsteps, n_last, h->timing, force_rows, motion_rows, row_sum, last_row, err, err_issued);
+  rc = finish_advance(h, nsteps, n_last, h->timing, force_rows, motion_rows, row_sum, last_row, err);
   if (rc == 0 && carry_pending) h->carry.valid = true;
   return rc;
 } LBM_CATCH(h, "lbm_advance")
@@ -1301,33 +1295,26 @@ int lbm_advance(lbm_handle* h, int32_t n_last, double* force_rows, double* motio
 
 namespace {
 
-// The erranalysis kernels of step ml (solution.f90:77-130: u of step ml-1
-// against step ml-2, both kept in field slots) on stream s, when those slots
-// hold them; false when they do not (or ml < 2, or a move right before).
-bool issue_err_fields(lbm_handle* h, int ml, hipStream_t s) {
-  if (ml < 2 || h->last_move_step == ml - 1) return false;
-  const int s1 = (ml - 1) % 3, s2 = (ml - 2) % 3;
-  if (h->slot_step[s1] != ml - 1 || h->slot_step[s2] != ml - 2) return false;
-  const int nb = err_blocks(h->g);
-  launch_err_partial(s, h->g, h->slot[s1], h->slot[s2], h->err_partial, nb);
-  launch_err_final(s, h->err_partial, nb, h->err_out);
-  return true;
-}
-
 // Common end of an advance over steps n_last-nsteps+1 .. n_last: the
-// erranalysis of the last step (unless err_issued: the step loop has issued
-// it on the IBM stream, which the lattice stream has joined), the device
-// error check and the outputs.
+// erranalysis of the last step, the device error check and the outputs.
 int finish_advance(lbm_handle* h, int nsteps, int n_last, bool timed, double* force_rows, double* motion_rows,
-                   double* row_sum, double* last_row, double* err, bool err_issued) {
+                   double* row_sum, double* last_row, double* err) {
+  // erranalysis of the last step (solution.f90:77-130): u of step n-1 against step n-2
   double errv = 0.0;
   const int ml = n_last;
-  bool have_err = err_issued;
-  if (!err_issued && nsteps > 0 && ml >= 2) {
-    if (h->last_move_step == ml - 1)
+  bool have_err = false;
+  if (nsteps > 0 && ml >= 2) {
+    if (h->last_move_step == ml - 1) {
       have_err = true;  // move_mesh copies u into u0: the interior difference is zero
-    else
-      have_err = issue_err_fields(h, ml, h->stream);
+    } else {
+      const int s1 = (ml - 1) % 3, s2 = (ml - 2) % 3;
+      if (h->slot_step[s1] == ml - 1 && h->slot_step[s2] == ml - 2) {
+        const int nb = err_blocks(h->g);
+        launch_err_partial(h->stream, h->g, h->slot[s1], h->slot[s2], h->err_partial, nb);
+        launch_err_final(h->stream, h->err_partial, nb, h->err_out);
+        have_err = true;
+      }
+    }
   }
   HIPCHECK(h, hipGetLastError());
   // the outputs into pinned memory behind the step, then one synchronisation
