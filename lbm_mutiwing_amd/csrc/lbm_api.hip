@@ -654,7 +654,10 @@ int lbm_create(const lbm_config* cfg, lbm_handle** out) try {
   h->g.nx = cfg->nx;
   h->g.ny = cfg->ny;
   const long long np = (long long)cfg->nx * cfg->ny;
-  h->g.ps = np;
+  // Population planes are np nodes apart rounded up to a multiple of 64, so
+  // that every plane starts on a 256-byte boundary and a 128-byte line of one
+  // plane is one of every plane; the gaps are never read or written.
+  h->g.ps = (np + 63) & ~63LL;
   // IBM params
   IbmParams& ip = h->ip;
   ip.W = h->W; ip.P = h->P; ip.SPW = h->SPW; ip.nseg = h->nseg; ip.npts = h->npts;
@@ -718,15 +721,15 @@ int lbm_create(const lbm_config* cfg, lbm_handle** out) try {
   }
   if (h->f32) {
     float *a = nullptr, *b = nullptr;
-    ALLOC(a, 9 * np);
+    ALLOC(a, 9 * (size_t)h->g.ps);
     h->fbuf[0] = a;
-    ALLOC(b, 9 * np);
+    ALLOC(b, 9 * (size_t)h->g.ps);
     h->fbuf[1] = b;
   } else {
     double *a = nullptr, *b = nullptr;
-    ALLOC(a, 9 * np);
+    ALLOC(a, 9 * (size_t)h->g.ps);
     h->fbuf[0] = a;
-    ALLOC(b, 9 * np);
+    ALLOC(b, 9 * (size_t)h->g.ps);
     h->fbuf[1] = b;
   }
   for (auto& s : h->slot) {

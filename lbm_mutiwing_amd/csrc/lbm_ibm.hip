@@ -276,9 +276,16 @@ __device__ __forceinline__ void geo_wing_body(const IbmParams& p, const GeoBuffe
     }
     return;
   }
+  // The claim map of a (step, wing) is written and read by this workgroup
+  // alone within the launch, so the barriers' workgroup-scope ordering is
+  // enough (the claims are read past the CU's cache, by load_agent).  An
+  // agent-scope fence here compiles to a write-back and an invalidation of the
+  // XCD's whole L2 (buffer_wbl2 / buffer_inv), twice per workgroup: beside
+  // the thousands of workgroups of a batched geometry batch (64 episodes) the
+  // lattice pass took 1.41 ms with it and 1.00 ms without (0.67 ms in the
+  // steps without geometry), and the marker pass itself 0.68 against 0.27 ms.
   int* M = G.map + ((size_t)slot * p.W + w) * (size_t)p.map_cap;  // owner segment per cell
   for (long long c = t; c < cells; c += NT) M[c] = INT_MAX;
-  __threadfence();
   __syncthreads();
   for (int e = t; e < SPW * BOXC; e += NT) {
     const int k = e / BOXC, c = e - k * BOXC;
@@ -287,7 +294,6 @@ __device__ __forceinline__ void geo_wing_body(const IbmParams& p, const GeoBuffe
     if (i > bx[1] || j > bx[3]) continue;
     atomicMin(&M[(i - i0) + wr * (j - j0)], k);
   }
-  __threadfence();
   __syncthreads();
   for (int e0 = t; e0 < SPW * BOXC; e0 += CLAIM_UNROLL * NT) {
     int own[CLAIM_UNROLL];

@@ -45,10 +45,12 @@ template <bool FIELDS, int CP>
 __device__ __forceinline__ void lattice_node(const Grid& g, const MrtParams& P, const double* __restrict__ src,
                                              double* __restrict__ dst, const FieldSlot& fs, int i, int j,
                                              const AdjMask& am) {
-  if (i < 1 || i > g.nx - 2 || is_adjoint(am, i, j)) return;
+  if (i < 1 || i > g.nx - 2) return;
   double f[9];
 #pragma unroll
   for (int k = 0; k < 9; k++) f[k] = ld_pop<CP>(src + (long long)k * g.ps + (long long)(j - EY(k)) * g.nx + (i - EX(k)));
+  // after the loads, so that they do not wait for the mask's own loads
+  if (is_adjoint(am, i, j)) return;
   double rho, u, v;
   node_moments(f, rho, u, v);
   double out[9];
@@ -79,15 +81,22 @@ __global__ __launch_bounds__(BX) void k_lattice(Grid g, MrtParams P, const doubl
 // Node of lane t in tile `blk` of BX consecutive nodes of the rows j = 1..ny-2
 // taken as one run (row-major, rows back to back): the batch covers each
 // lattice's interior rows with whole tiles and no partial tile at each row
-// end.  A tile wraps into the next row at most once when BX <= nx (every
-// lattice of the batched workloads), more often on narrower lattices; a lane
-// past the last interior row gets j > ny - 2.
+// end.  The run starts at node offset flat_base, the start of row 1 rounded
+// down to a multiple of 32 nodes, so that every tile and wavefront (BX a
+// multiple of 64) begins on a 128-byte line of the unshifted loads and of
+// every store whatever the lattice's width (the plane stride is a multiple of
+// 64 nodes); the lanes before row 1 get j = 0.  (Started at row 1
+// itself, 20 wingless 2040x1020 episodes ran their step at 0.51 of 8 TB/s,
+// 2016x1008 ones at 0.61.)  A tile wraps into the next row at most once when BX <= nx
+// (every lattice of the batched workloads), more often on narrower lattices;
+// a lane past the last interior row gets j > ny - 2.
+__host__ __device__ __forceinline__ int flat_base(const Grid& g) { return g.nx & ~31; }
+
 template <int BX>
 __device__ __forceinline__ void flat_node(const Grid& g, int blk, int t, int& i, int& j) {
-  const int q0 = blk * BX;                 // offset of the tile in the run
-  const int j0 = q0 / g.nx;                // uniform over the tile
-  i = q0 - j0 * g.nx + t;
-  j = 1 + j0;
+  const int q0 = flat_base(g) + blk * BX;  // node offset of the tile's first lane
+  j = q0 / g.nx;                           // uniform over the tile
+  i = q0 - j * g.nx + t;
   while (i >= g.nx) {
     i -= g.nx;
     j += 1;
@@ -114,11 +123,12 @@ template <bool FIELDS, int CP>
 __device__ __forceinline__ void lattice_f32_node(const Grid& g, const MrtParams& P, const float* __restrict__ src,
                                                  float* __restrict__ dst, const FieldSlot& fs, int i, int j,
                                                  const AdjMask& am) {
-  if (i < 1 || i > g.nx - 2 || is_adjoint(am, i, j)) return;
+  if (i < 1 || i > g.nx - 2) return;
   double d[9];
 #pragma unroll
   for (int k = 0; k < 9; k++)
     d[k] = (double)ld_pop<CP>(src + (long long)k * g.ps + (long long)(j - EY(k)) * g.nx + (i - EX(k)));
+  if (is_adjoint(am, i, j)) return;
   // moments of d
   const double h13 = d[1] + d[3], h24 = d[2] + d[4];     // axis pairs
   const double h57 = d[5] + d[7], h68 = d[6] + d[8];     // diagonal pairs
@@ -215,7 +225,9 @@ __global__ __launch_bounds__(BX) void k_lattice_f32_narrow(Grid g, MrtParams P, 
 }
 
 int lattice_tiles_x(const Grid& g, int bx) { return (g.nx + bx - 1) / bx; }
-int lattice_tiles_flat(const Grid& g, int bx) { return (int)(((long long)(g.ny - 2) * g.nx + bx - 1) / bx); }
+int lattice_tiles_flat(const Grid& g, int bx) {
+  return (int)(((long long)(g.ny - 1) * g.nx - flat_base(g) + bx - 1) / bx);
+}
 
 template <int BX, int CP>
 void launch_lattice_cp(hipStream_t s, const Grid& g, const MrtParams& P, const void* src, void* dst,
@@ -1035,10 +1047,21 @@ int bc_ring_blocks(const Grid& g) { return (2 * (g.ny - 2) + 2 * g.nx + 255) / 2
 // row-major tile order), so that the chain of dependent loads a workgroup
 // needs before its first population load (item, then descriptor, then
 // buffer pointers) is paid once per ROWS tiles.
+//
+// Workgroups are placed on the 8 XCDs round-robin (workgroup b on XCD b % 8),
+// and each XCD has its own L2.  Workgroup b takes item run(b % 8) + b / 8,
+// where XCD x's run is the x-th eighth of the list in order, so that the
+// rows a tile pulls from (j - 1 and j + 1) were loaded by tiles of the same
+// XCD shortly before, whatever the lattice's width.  (In list order a row's
+// neighbour rows fall on the same XCD only when a row holds a multiple of 8
+// tiles; on 64 uneven episodes the mapping took the kernel from 705 to
+// 611 us at the median.)
 template <bool F32, bool FIELDS, int BX, int ROWS, int CP>
 __global__ __launch_bounds__(BX) void kb_lattice(const EpDesc* __restrict__ eps, const int2* __restrict__ items,
                                                  const int* __restrict__ kb, int kk, int masked) {
-  const int2 it = items[blockIdx.x];
+  const int nb = (int)gridDim.x, per = nb >> 3, extra = nb & 7;
+  const int x = (int)blockIdx.x & 7;
+  const int2 it = items[x * per + min(x, extra) + ((int)blockIdx.x >> 3)];
   const EpDesc& D = eps[it.x];
   const int k = kb[0] + kk;
   const int n = D.n0 + k, cur = (D.cur0 + k) & 1;
@@ -1053,6 +1076,7 @@ __global__ __launch_bounds__(BX) void kb_lattice(const EpDesc* __restrict__ eps,
     int i, j;
     flat_node<BX>(g, blk, (int)threadIdx.x, i, j);
     if (j > g.ny - 2) break;
+    if (j < 1) continue;
     if (F32)
       lattice_f32_node<FIELDS, CP>(g, D.P, (const float*)D.fbuf[cur], (float*)D.fbuf[cur ^ 1], fs, i, j, am);
     else
