@@ -2049,14 +2049,25 @@ int lbm_batch_advance(lbm_batch* b, lbm_handle* const* hs, int32_t count, int32_
   // is issued one batch ahead of the flow (at the first step of batch bb-1).
   // Launches take the substep relative to the device step base kb0.
   const int pl = wings ? ns : 16;
+  // A geometry batch issued one batch ahead of the flow has ns steps before
+  // the flow needs it: its marker pass and band factor run from a grid of
+  // kGeoAheadGrid workgroups (about one per CU) that loop over the items,
+  // instead of one workgroup per item (thousands, 8 per CU), which halves
+  // what they take from the lattice pass running beside them.  On 64 uneven
+  // episodes the lattice pass's mean exceeded its median by 16 us per step
+  // with one workgroup per item and by 8 us with 256 (the geometry itself
+  // taking 0.75 and 2.3 ms of the batch's 10 ms).  The first batch of a
+  // call, which the flow waits for at once, keeps one workgroup per item.
+  constexpr int kGeoAheadGrid = 256, kGeoAheadMinSteps = 8;
   auto issue_geometry = [&](int bb, int seg_bb, int kb0) -> int {
     const int k0 = bb * ns;
     if (!wings || k0 >= steps) return 0;
     const int es = bb & 1;
     BCHECK(b, hipStreamWaitEvent(b->stream_geo, bb < seg_bb + 2 ? b->ev_start : b->ev_rel[es], 0));
     const int q = k0 + ns >= steps ? 6 : 5;  // the last (shorter) batch has its own list
+    const int grid = bb > 0 && ns >= kGeoAheadMinSteps ? kGeoAheadGrid : 0;
     launch_batch_geometry(b->stream_geo, b->d_desc, b->d_kbase, di + off[q], (int)(off[q + 1] - off[q]), k0 - kb0,
-                          lds_wing, lds_solve, lds_inv, inv_chunks);
+                          lds_wing, lds_solve, lds_inv, inv_chunks, grid);
     BCHECK(b, hipEventRecord(b->ev_geo[es], b->stream_geo));
     return 0;
   };

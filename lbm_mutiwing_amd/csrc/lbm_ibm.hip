@@ -1606,24 +1606,35 @@ void launch_ibm_flow(hipStream_t s, const IbmParams& p, const MrtParams& P, cons
 
 int spread_blocks(const IbmParams& p) { return p.W * ((p.SPW + SPREAD_SEG - 1) / SPREAD_SEG); }
 
+// The marker pass and the band factor of a batch loop over their (episode,
+// step, wing) items from a grid of one workgroup per item or fewer (the
+// barrier at the top of each item keeps the previous item's readers of the
+// LDS and of the shared flags behind its writers; every return in the
+// bodies is taken by the whole workgroup).
 __global__ __launch_bounds__(GEO_THREADS_BATCH) void kb_geo_wing(const EpDesc* __restrict__ eps,
                                                            const int2* __restrict__ items,
-                                                           const int* __restrict__ kb, int kk0) {
-  const int2 it = items[blockIdx.x];
-  const EpDesc& D = eps[it.x];
-  const int k0 = kb[0] + kk0;
-  const int b = it.y / D.p.W, w = it.y - b * D.p.W;
-  geo_wing_body<GEO_THREADS_BATCH>(D.p, D.G, D.st, ep_slot(D, k0), D.n0 + k0, k0, b, w);
+                                                           const int* __restrict__ kb, int kk0, int n_items) {
+  for (int q = blockIdx.x; q < n_items; q += gridDim.x) {
+    __syncthreads();
+    const int2 it = items[q];
+    const EpDesc& D = eps[it.x];
+    const int k0 = kb[0] + kk0;
+    const int b = it.y / D.p.W, w = it.y - b * D.p.W;
+    geo_wing_body<GEO_THREADS_BATCH>(D.p, D.G, D.st, ep_slot(D, k0), D.n0 + k0, k0, b, w);
+  }
 }
 
 __global__ __launch_bounds__(GEO_THREADS_BATCH) void kb_geo_solve(const EpDesc* __restrict__ eps,
                                                             const int2* __restrict__ items,
-                                                            const int* __restrict__ kb, int kk0) {
-  const int2 it = items[blockIdx.x];
-  const EpDesc& D = eps[it.x];
-  const int k0 = kb[0] + kk0;
-  const int b = it.y / D.p.W, w = it.y - b * D.p.W;
-  geo_solve_body<GEO_THREADS_BATCH>(D.p, D.G, D.st, ep_slot(D, k0), b, w, D.direct, 0);
+                                                            const int* __restrict__ kb, int kk0, int n_items) {
+  for (int q = blockIdx.x; q < n_items; q += gridDim.x) {
+    __syncthreads();
+    const int2 it = items[q];
+    const EpDesc& D = eps[it.x];
+    const int k0 = kb[0] + kk0;
+    const int b = it.y / D.p.W, w = it.y - b * D.p.W;
+    geo_solve_body<GEO_THREADS_BATCH>(D.p, D.G, D.st, ep_slot(D, k0), b, w, D.direct, 0);
+  }
 }
 
 // inverse columns chunk*64 .. of each (step, wing) item (chunks past the
@@ -1651,10 +1662,11 @@ __global__ __launch_bounds__(CPL_THREADS) void kb_geo_couple(const EpDesc* __res
 }
 
 void launch_batch_geometry(hipStream_t s, const EpDesc* eps, const int* kb, const int2* items, int n_items,
-                           int k0, size_t lds_wing, size_t lds_solve, size_t lds_inv, int inv_chunks) {
+                           int k0, size_t lds_wing, size_t lds_solve, size_t lds_inv, int inv_chunks, int grid) {
   if (n_items <= 0) return;
-  hipLaunchKernelGGL(kb_geo_wing, dim3(n_items), dim3(GEO_THREADS_BATCH), lds_wing, s, eps, items, kb, k0);
-  hipLaunchKernelGGL(kb_geo_solve, dim3(n_items), dim3(GEO_THREADS_BATCH), lds_solve, s, eps, items, kb, k0);
+  const int g = grid > 0 && grid < n_items ? grid : n_items;
+  hipLaunchKernelGGL(kb_geo_wing, dim3(g), dim3(GEO_THREADS_BATCH), lds_wing, s, eps, items, kb, k0, n_items);
+  hipLaunchKernelGGL(kb_geo_solve, dim3(g), dim3(GEO_THREADS_BATCH), lds_solve, s, eps, items, kb, k0, n_items);
   hipLaunchKernelGGL(kb_geo_inverse, dim3(n_items, inv_chunks), dim3(INV_THREADS), lds_inv, s, eps, items, kb, k0);
   hipLaunchKernelGGL(kb_geo_couple, dim3(n_items), dim3(CPL_THREADS), geo_couple_lds_bytes(), s, eps, items, kb, k0);
 }

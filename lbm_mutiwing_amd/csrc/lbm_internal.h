@@ -378,8 +378,10 @@ void launch_batch_force_row(hipStream_t s, const EpDesc* eps, const int* kb, int
 void launch_step_base(hipStream_t s, int* p, int v, bool add);
 // Geometry of substeps *kb + k0 .. *kb + k0 + count - 1 of every episode with
 // wings; items are (episode, b*W + w) for b < count.
+// grid: workgroups of the marker pass and the band factor, each looping over
+// the items (0: one per item)
 void launch_batch_geometry(hipStream_t s, const EpDesc* eps, const int* kb, const int2* items, int n_items,
-                           int k0, size_t lds_wing, size_t lds_solve, size_t lds_inv, int inv_chunks);
+                           int k0, size_t lds_wing, size_t lds_solve, size_t lds_inv, int inv_chunks, int grid);
 // workgroups of the inverse per (step, wing) of a wing of spw segments
 int inverse_chunks(int spw);
 size_t geo_wing_lds_bytes(int spw);
