@@ -163,8 +163,9 @@ int lbm_get_mass(lbm_handle *h, double *mass);
 /* Allocation-free variant of lbm_get_mass, with no counterpart in the
  * reference: for callers that cannot allocate device memory (code under HIP
  * graph capture, tight serving loops).  f points to caller-owned device
- * populations in the layout of this library (9 planes of nx*ny values,
- * population k of node (i, j) at i + nx*j + nx*ny*k): FP64 values for
+ * populations in compact SoA planes (9 planes of nx*ny values, population
+ * k of node (i, j) at i + nx*j + nx*ny*k; the library's own buffers space
+ * their planes nx*ny rounded up to a multiple of 64 apart): FP64 values for
  * LBM_PRECISION_FP64, FP32 deviations f - w_k for
  * LBM_PRECISION_FP32_SHIFTED.  The kernels are those of lbm_get_mass (same
  * value, bit for bit), enqueued on `stream` (a hipStream_t; NULL is the null
