@@ -86,3 +86,26 @@ def test_n_steps_for_matches_file_loop():
     cfg.uu = 0.01
     # default reference run: motion_end_time 22.0, dt 1e-4 -> 220001 steps
     assert cfg.n_steps_for(22.0) == 220001
+
+
+def test_kernels_hold_no_device_wide_fence():
+    """gfx950 compiles an agent-scope (device-wide) fence to a write-back and
+    an invalidation of the XCD's whole L2 (buffer_wbl2 / buffer_inv).  In a
+    kernel that runs beside the lattice pass it throws away the rows the pass
+    has cached: a marker pass with two such fences per workgroup took the
+    lattice pass of a batched geometry step (64 episodes) from 1.00 to
+    1.41 ms.  The kernels order their data at workgroup scope (barriers) or
+    at launch boundaries instead, and no device- or system-wide fence may
+    come back into their sources."""
+    csrc = os.path.join(ROOT, "lbm_mutiwing_amd", "csrc")
+    bad = re.compile(r'__threadfence\s*\(\s*\)|__threadfence_system|__atomic_thread_fence'
+                     r'|__builtin_amdgcn_fence\s*\([^)]*"(agent|)"\s*\)')
+    hits = []
+    for name in sorted(os.listdir(csrc)):
+        if not name.endswith((".hip", ".h")):
+            continue
+        with open(os.path.join(csrc, name)) as fh:
+            for n, line in enumerate(fh, 1):
+                if bad.search(line.split("//")[0]):
+                    hits.append(f"{name}:{n}: {line.strip()}")
+    assert not hits, "\n".join(hits)
